@@ -1,0 +1,1373 @@
+// Oracle: CPU restatement of PhysiK's soft-body implicit-Euler step.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker for the GPU path. Only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load the
+// library built from it. Nothing under physik_amd/ links, loads or calls it.
+//
+// The restatement is templated on the scalar type. Its float instantiation is
+// meant to reproduce the reference's FP32 arithmetic operation for operation
+// (same association order, no FMA contraction: built with -ffp-contract=off),
+// and is checked bit-for-bit against fixtures produced by the compiled
+// reference. Its double instantiation is the FP64 oracle the GPU path is held
+// to (relative L2 <= 1e-9 per step).
+//
+// Each routine cites the reference file:line it restates (paths relative to
+// the reference repository root).
+//
+// Pinning (what checks each function; see tests/test_oracle_*.py):
+//   Mat3/Vec3 algebra ........ reference-test known answers + fixtures (bitwise FP32)
+//   rest data ................ unit-tet closed form (V=1/6, gradients), degenerate -> zeros
+//   element forces/stiffness . fixtures from the compiled reference (bitwise FP32),
+//                              closed form K_ab, rigid null space, sum of forces = 0
+//   polar rotation ........... R^T R = I, det R = +1, R(QS) = Q, fixtures incl. inverted tet
+//   pattern / scatter ........ fixtures from the compiled reference (bit-exact)
+//   SpMV ..................... reference-test closed-form vectors + dense brute force
+//   PCG ...................... reference-test known solutions + scipy/dense solve
+//   full step ................ 100-step config-1 FP32 trajectory fixture (bitwise)
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <unordered_map>
+#include <unordered_set>
+#include <utility>
+#include <vector>
+
+namespace oracle
+{
+
+// ---------------------------------------------------------------------------
+// 3-vector / 3x3 algebra. Follows include/PhysiK/Math/Vec3.h:7-95 and
+// include/PhysiK/Math/Mat3.h:9-84: column-major storage, matrix*vector as the
+// weighted sum of columns, inverse from cross products with a 1e-6 floor that
+// returns Identity.
+// ---------------------------------------------------------------------------
+template <class T>
+struct V3
+{
+    T x = T(0), y = T(0), z = T(0);
+};
+
+template <class T> inline V3<T> add(const V3<T>& a, const V3<T>& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> inline V3<T> sub(const V3<T>& a, const V3<T>& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> inline V3<T> neg(const V3<T>& a) { return {-a.x, -a.y, -a.z}; }
+template <class T> inline V3<T> scl(const V3<T>& a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+// Vec3.h:68-71 and :84-87: left-to-right sums of products.
+template <class T> inline T dot(const V3<T>& a, const V3<T>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> inline T len2(const V3<T>& a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+template <class T> inline V3<T> cross(const V3<T>& a, const V3<T>& b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T> inline bool finite(T v) { return std::isfinite(v); }
+template <class T> inline bool finite(const V3<T>& v) { return finite(v.x) && finite(v.y) && finite(v.z); }
+
+template <class T>
+struct M3
+{
+    V3<T> c[3] = {{T(1), T(0), T(0)}, {T(0), T(1), T(0)}, {T(0), T(0), T(1)}};  // Mat3.h:11-14
+};
+
+template <class T> inline M3<T> cols(const V3<T>& a, const V3<T>& b, const V3<T>& c)
+{
+    M3<T> m;
+    m.c[0] = a;
+    m.c[1] = b;
+    m.c[2] = c;
+    return m;
+}
+template <class T> inline M3<T> zero3() { return cols(V3<T>{}, V3<T>{}, V3<T>{}); }
+template <class T> inline M3<T> diag3(T s)  // SolverData.cpp:36-42
+{
+    return cols(V3<T>{s, T(0), T(0)}, V3<T>{T(0), s, T(0)}, V3<T>{T(0), T(0), s});
+}
+// Mat3.h:24-27: (c0*x + c1*y) + c2*z
+template <class T> inline V3<T> mul(const M3<T>& m, const V3<T>& v)
+{
+    return add(add(scl(m.c[0], v.x), scl(m.c[1], v.y)), scl(m.c[2], v.z));
+}
+template <class T> inline M3<T> mul(const M3<T>& a, const M3<T>& b)  // Mat3.h:39-42
+{
+    return cols(mul(a, b.c[0]), mul(a, b.c[1]), mul(a, b.c[2]));
+}
+template <class T> inline M3<T> tr(const M3<T>& m)  // Mat3.h:44-50
+{
+    return cols(V3<T>{m.c[0].x, m.c[1].x, m.c[2].x}, V3<T>{m.c[0].y, m.c[1].y, m.c[2].y},
+                V3<T>{m.c[0].z, m.c[1].z, m.c[2].z});
+}
+template <class T> inline M3<T> madd(const M3<T>& a, const M3<T>& b)  // FEMModel.cpp:74-80
+{
+    return cols(add(a.c[0], b.c[0]), add(a.c[1], b.c[1]), add(a.c[2], b.c[2]));
+}
+template <class T> inline M3<T> mscl(const M3<T>& a, T s)  // FEMModel.cpp:82-88
+{
+    return cols(scl(a.c[0], s), scl(a.c[1], s), scl(a.c[2], s));
+}
+template <class T> inline T det(const M3<T>& m) { return dot(m.c[0], cross(m.c[1], m.c[2])); }  // Mat3.h:52-55
+template <class T> inline bool finite(const M3<T>& m) { return finite(m.c[0]) && finite(m.c[1]) && finite(m.c[2]); }
+
+// Mat3.h:57-74. The floor is the float literal 1e-6f in every precision.
+template <class T> inline M3<T> inv(const M3<T>& m)
+{
+    const V3<T> r0 = cross(m.c[1], m.c[2]);
+    const V3<T> r1 = cross(m.c[2], m.c[0]);
+    const V3<T> r2 = cross(m.c[0], m.c[1]);
+    const T d = dot(m.c[0], r0);
+    if (std::abs(d) <= T(0.000001f))
+        return M3<T>{};
+    const T id = T(1) / d;
+    return tr(cols(scl(r0, id), scl(r1, id), scl(r2, id)));
+}
+template <class T> inline T at(const M3<T>& m, int row, int col)
+{
+    const V3<T>& c = m.c[col];
+    return row == 0 ? c.x : (row == 1 ? c.y : c.z);
+}
+template <class T> inline void set(M3<T>& m, int row, int col, T v)
+{
+    V3<T>& c = m.c[col];
+    if (row == 0) c.x = v; else if (row == 1) c.y = v; else c.z = v;
+}
+template <class T> inline V3<T> rowof(const M3<T>& m, int row)  // FEMModel.cpp:26-39
+{
+    return {at(m, row, 0), at(m, row, 1), at(m, row, 2)};
+}
+
+// ---------------------------------------------------------------------------
+// Per-tet data and the FEM element (src/Core/Physics/FEM/FEMModel.cpp).
+// ---------------------------------------------------------------------------
+template <class T>
+struct Tet
+{
+    int n[4] = {-1, -1, -1, -1};
+    bool active = true;
+    T vol = T(0);
+    M3<T> dmInv;
+    V3<T> restPos[4];
+    V3<T> grad[4];
+    T young = T(25), poisson = T(0.3f), damping = T(0.25f), density = T(1);
+};
+
+template <class T>
+struct FemCache
+{
+    T B[6][12];
+    T D[6][6];
+    T Ke[12][12];
+};
+
+template <class T>
+struct Element
+{
+    V3<T> f[4];
+    M3<T> k[4][4];
+    M3<T> R;
+};
+
+// FEMModel.cpp:504-548 (MinTetVolume = 1e-8f at :14).
+template <class T>
+void init_rest(Tet<T>& t, const std::vector<V3<T>>& rest)
+{
+    const int n = static_cast<int>(rest.size());
+    for (int a = 0; a < 4; ++a)
+        if (t.n[a] < 0 || t.n[a] >= n)
+            return;
+    const V3<T>& p0 = rest[t.n[0]];
+    const V3<T>& p1 = rest[t.n[1]];
+    const V3<T>& p2 = rest[t.n[2]];
+    const V3<T>& p3 = rest[t.n[3]];
+    const M3<T> dm = cols(sub(p1, p0), sub(p2, p0), sub(p3, p0));
+    const T d = det(dm);
+    t.restPos[0] = p0;
+    t.restPos[1] = p1;
+    t.restPos[2] = p2;
+    t.restPos[3] = p3;
+    const T vol = std::abs(d) / T(6);
+    if (vol <= T(1.0e-8f))
+    {
+        t.vol = T(0);
+        t.dmInv = zero3<T>();
+        for (int a = 0; a < 4; ++a)
+            t.grad[a] = V3<T>{};
+        return;
+    }
+    t.vol = vol;
+    t.dmInv = inv(dm);
+    t.grad[1] = rowof(t.dmInv, 0);
+    t.grad[2] = rowof(t.dmInv, 1);
+    t.grad[3] = rowof(t.dmInv, 2);
+    t.grad[0] = neg(add(add(t.grad[1], t.grad[2]), t.grad[3]));
+}
+
+// FEMModel.cpp:174-200: E clamped at 0 (zero D if E <= 0), nu clamped to [-0.99, 0.49].
+template <class T>
+void build_D(T D[6][6], T young, T poisson)
+{
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j)
+            D[i][j] = T(0);
+    const T e = std::max(T(0), young);
+    if (e <= T(0))
+        return;
+    const T nu = std::max(T(-0.99f), std::min(T(0.49f), poisson));
+    const T lambda = e * nu / ((T(1) + nu) * (T(1) - T(2) * nu));
+    const T mu = e / (T(2) * (T(1) + nu));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            D[i][j] = lambda;
+    D[0][0] = D[1][1] = D[2][2] = lambda + T(2) * mu;
+    D[3][3] = D[4][4] = D[5][5] = mu;
+}
+
+// FEMModel.cpp:202-225: engineering strain [exx, eyy, ezz, gxy, gyz, gxz].
+template <class T>
+void build_B(T B[6][12], const Tet<T>& t)
+{
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 12; ++j)
+            B[i][j] = T(0);
+    for (int a = 0; a < 4; ++a)
+    {
+        const V3<T>& g = t.grad[a];
+        const int c = 3 * a;
+        B[0][c + 0] = g.x;
+        B[1][c + 1] = g.y;
+        B[2][c + 2] = g.z;
+        B[3][c + 0] = g.y;
+        B[3][c + 1] = g.x;
+        B[4][c + 1] = g.z;
+        B[4][c + 2] = g.y;
+        B[5][c + 0] = g.z;
+        B[5][c + 2] = g.x;
+    }
+}
+
+// FEMModel.cpp:312-334 (loop order row, column, i, j; volume applied last)
+// and :550-557.
+template <class T>
+void build_cache(FemCache<T>& c, const Tet<T>& t)
+{
+    build_B(c.B, t);
+    build_D(c.D, t.young, t.poisson);
+    for (int row = 0; row < 12; ++row)
+        for (int col = 0; col < 12; ++col)
+        {
+            T v = T(0);
+            for (int i = 0; i < 6; ++i)
+                for (int j = 0; j < 6; ++j)
+                    v += c.B[i][row] * c.D[i][j] * c.B[j][col];
+            c.Ke[row][col] = t.vol * v;
+        }
+}
+
+// FEMModel.cpp:104-113
+template <class T>
+T frob_diff2(const M3<T>& a, const M3<T>& b)
+{
+    T total = T(0);
+    for (int k = 0; k < 3; ++k)
+        total += len2(sub(a.c[k], b.c[k]));
+    return total;
+}
+
+// FEMModel.cpp:115-163: Newton polar iteration R <- (R + R^-T)/2, <= 12
+// iterations, stop on ||dR||_F^2 <= 1e-10f, Identity fallbacks, column-2 flip
+// when det R < 0. MinPolarDeterminant = 1e-8f (:15).
+template <class T>
+M3<T> polar(const M3<T>& F, int* iterations = nullptr)
+{
+    if (iterations) *iterations = 0;
+    if (!finite(F) || std::abs(det(F)) <= T(1.0e-8f))
+        return M3<T>{};
+    M3<T> R = F;
+    for (int it = 0; it < 12; ++it)
+    {
+        if (iterations) *iterations = it + 1;
+        const M3<T> ti = inv(tr(R));
+        if (!finite(ti) || std::abs(det(ti)) <= T(1.0e-8f))
+            return M3<T>{};
+        const M3<T> next = mscl(madd(R, ti), T(0.5f));
+        if (!finite(next))
+            return M3<T>{};
+        if (frob_diff2(next, R) <= T(1.0e-10f))
+        {
+            R = next;
+            break;
+        }
+        R = next;
+    }
+    if (!finite(R))
+        return M3<T>{};
+    if (det(R) < T(0))
+        R.c[2] = scl(R.c[2], T(-1.0f));
+    return R;
+}
+
+// FEMModel.cpp:247-271: u_a = R^T (x_a - x_0) + X_0 - X_a
+template <class T>
+void corotated_u(T u[12], const Tet<T>& t, const std::vector<V3<T>>& x, const M3<T>& R)
+{
+    const V3<T>& x0 = x[t.n[0]];
+    const M3<T> Rt = tr(R);
+    for (int a = 0; a < 4; ++a)
+    {
+        const V3<T> local = add(mul(Rt, sub(x[t.n[a]], x0)), t.restPos[0]);
+        const V3<T> d = sub(local, t.restPos[a]);
+        u[3 * a + 0] = d.x;
+        u[3 * a + 1] = d.y;
+        u[3 * a + 2] = d.z;
+    }
+}
+
+// FEMModel.cpp:377-424 (forces, per-tet damping -c*v) and :336-375 (blocks
+// R * K_ab * R^T, built from Ke entries).
+template <class T>
+void element(Element<T>& out, const Tet<T>& t, const FemCache<T>& c, const std::vector<V3<T>>& v, const T u[12],
+             const M3<T>& R)
+{
+    T strain[6], stress[6], fint[12];
+    for (int r = 0; r < 6; ++r)  // :273-284
+    {
+        strain[r] = T(0);
+        for (int k = 0; k < 12; ++k)
+            strain[r] += c.B[r][k] * u[k];
+    }
+    for (int r = 0; r < 6; ++r)  // :286-297
+    {
+        stress[r] = T(0);
+        for (int k = 0; k < 6; ++k)
+            stress[r] += c.D[r][k] * strain[k];
+    }
+    for (int k = 0; k < 12; ++k)  // :299-310
+    {
+        fint[k] = T(0);
+        for (int r = 0; r < 6; ++r)
+            fint[k] += c.B[r][k] * stress[r];
+    }
+    for (int k = 0; k < 12; ++k)
+        fint[k] *= t.vol;
+    for (int a = 0; a < 4; ++a)
+    {
+        V3<T> fe{-fint[3 * a + 0], -fint[3 * a + 1], -fint[3 * a + 2]};
+        fe = mul(R, fe);
+        const V3<T> damp = scl(v[t.n[a]], -t.damping);
+        out.f[a] = add(fe, damp);
+    }
+    const M3<T> Rt = tr(R);
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+        {
+            M3<T> blk = zero3<T>();
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    set(blk, i, j, c.Ke[3 * a + i][3 * b + j]);
+            out.k[a][b] = mul(mul(R, blk), Rt);
+        }
+    out.R = R;
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric-upper block-sparse matrix (src/Math/SparseBlockMatrix.cpp).
+// ---------------------------------------------------------------------------
+template <class T>
+struct Bsr
+{
+    int n = 0;
+    std::vector<int> rowStart, col;
+    std::vector<M3<T>> val;
+    std::unordered_map<uint64_t, int> lookup;
+
+    static uint64_t key(int r, int c)  // SparseBlockMatrix.cpp:284-288
+    {
+        return (static_cast<uint64_t>(static_cast<uint32_t>(r)) << 32u) | static_cast<uint32_t>(c);
+    }
+    void clear()
+    {
+        n = 0;
+        rowStart.clear();
+        col.clear();
+        val.clear();
+        lookup.clear();
+    }
+    // SparseBlockMatrix.cpp:83-129: canonical (min,max), per-row sort + unique.
+    void build(int count, const std::vector<std::pair<int, int>>& coords)
+    {
+        clear();
+        if (count <= 0)
+            return;
+        n = count;
+        std::vector<std::vector<int>> rows(static_cast<size_t>(n));
+        for (const auto& rc : coords)
+        {
+            int r = rc.first, c = rc.second;
+            if (r < 0 || r >= n || c < 0 || c >= n)
+                continue;
+            if (r > c)
+                std::swap(r, c);
+            rows[static_cast<size_t>(r)].push_back(c);
+        }
+        rowStart.assign(static_cast<size_t>(n + 1), 0);
+        for (int r = 0; r < n; ++r)
+        {
+            std::vector<int>& cs = rows[static_cast<size_t>(r)];
+            std::sort(cs.begin(), cs.end());
+            cs.erase(std::unique(cs.begin(), cs.end()), cs.end());
+            rowStart[static_cast<size_t>(r + 1)] = rowStart[static_cast<size_t>(r)] + static_cast<int>(cs.size());
+            for (int c : cs)
+            {
+                lookup[key(r, c)] = static_cast<int>(col.size());
+                col.push_back(c);
+                val.push_back(zero3<T>());
+            }
+        }
+    }
+    int find(int r, int c) const  // :270-282
+    {
+        if (r > c)
+            std::swap(r, c);
+        const auto it = lookup.find(key(r, c));
+        return it == lookup.end() ? -1 : it->second;
+    }
+    bool add_block(int r, int c, const M3<T>& b)  // :131-144
+    {
+        const int s = find(r, c);
+        if (s < 0)
+            return false;
+        val[static_cast<size_t>(s)] = madd(val[static_cast<size_t>(s)], r <= c ? b : tr(b));
+        return true;
+    }
+    void clear_values() { std::fill(val.begin(), val.end(), zero3<T>()); }
+    // :209-268: y = A x with the stored upper block B at (r,c) applied as
+    // y_r += B x_c and, off the diagonal, y_c += B^T x_r.
+    void multiply(const std::vector<V3<T>>& x, std::vector<V3<T>>& y) const
+    {
+        if (x.size() < static_cast<size_t>(std::max(0, n)) || rowStart.size() != static_cast<size_t>(n + 1))
+        {
+            y.assign(static_cast<size_t>(std::max(0, n)), V3<T>{});
+            return;
+        }
+        y.assign(static_cast<size_t>(n), V3<T>{});
+        for (int r = 0; r < n; ++r)
+            for (int k = rowStart[r]; k < rowStart[r + 1]; ++k)
+            {
+                const int c = col[k];
+                const M3<T>& B = val[k];
+                const V3<T>& xc = x[c];
+                y[r].x = y[r].x + (B.c[0].x * xc.x + B.c[1].x * xc.y + B.c[2].x * xc.z);
+                y[r].y = y[r].y + (B.c[0].y * xc.x + B.c[1].y * xc.y + B.c[2].y * xc.z);
+                y[r].z = y[r].z + (B.c[0].z * xc.x + B.c[1].z * xc.y + B.c[2].z * xc.z);
+                if (r == c)
+                    continue;
+                const V3<T>& xr = x[r];
+                y[c].x = y[c].x + (B.c[0].x * xr.x + B.c[0].y * xr.y + B.c[0].z * xr.z);
+                y[c].y = y[c].y + (B.c[1].x * xr.x + B.c[1].y * xr.y + B.c[1].z * xr.z);
+                y[c].z = y[c].z + (B.c[2].x * xr.x + B.c[2].y * xr.y + B.c[2].z * xr.z);
+            }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Jacobi-preconditioned CG (src/Core/Solvers/Linear/ConjugateGradientSolver.cpp).
+// ---------------------------------------------------------------------------
+template <class T>
+struct CgResult
+{
+    int iterations = 0;
+    T residual = T(0);
+    bool converged = false;
+};
+
+template <class T> T vdot(const std::vector<V3<T>>& a, const std::vector<V3<T>>& b)  // :41-52
+{
+    T s = T(0);
+    const size_t n = std::min(a.size(), b.size());
+    for (size_t i = 0; i < n; ++i)
+        s += dot(a[i], b[i]);
+    return s;
+}
+template <class T> bool vfinite(const std::vector<V3<T>>& a)
+{
+    for (const auto& v : a)
+        if (!finite(v)) return false;
+    return true;
+}
+template <class T> T resnorm(const std::vector<V3<T>>& r)  // :102-112
+{
+    const T s = vdot(r, r);
+    if (!finite(s) || s < T(0))
+        return std::numeric_limits<T>::infinity();
+    return std::sqrt(s);
+}
+template <class T> bool precond(const std::vector<M3<T>>& minv, const std::vector<V3<T>>& in, std::vector<V3<T>>& out)
+{  // :67-100
+    if (minv.size() != in.size())
+    {
+        out.clear();
+        return false;
+    }
+    out.resize(in.size());
+    for (size_t i = 0; i < in.size(); ++i)
+    {
+        if (!finite(minv[i]))
+        {
+            out.clear();
+            return false;
+        }
+        out[i] = mul(minv[i], in[i]);
+        if (!finite(out[i]))
+        {
+            out.clear();
+            return false;
+        }
+    }
+    return true;
+}
+
+// :114-241
+template <class T>
+CgResult<T> pcg(std::vector<V3<T>>& x, const Bsr<T>& A, const std::vector<V3<T>>& b, int maxIt, T tol,
+                const std::vector<M3<T>>& minv)
+{
+    CgResult<T> res;
+    std::vector<V3<T>> r, d, q;
+    if (b.empty())
+    {
+        x.clear();
+        res.converged = true;
+        return res;
+    }
+    if (A.n != static_cast<int>(b.size()) || A.rowStart.size() != static_cast<size_t>(A.n + 1) || maxIt <= 0 ||
+        !finite(tol) || tol <= T(0) || !vfinite(b))
+    {
+        x.clear();
+        return res;
+    }
+    x.assign(b.size(), V3<T>{});
+    r = b;
+    if (!precond(minv, r, d))
+        return res;
+    T delta = vdot(r, d);
+    if (!finite(delta) || delta < T(0))
+        return res;
+    const T target = tol * tol * delta;
+    if (!finite(target))
+        return res;
+    if (delta <= target)
+    {
+        res.converged = true;
+        res.residual = resnorm(r);
+        return res;
+    }
+    for (int it = 0; it < maxIt; ++it)
+    {
+        if (delta <= target)
+        {
+            res.converged = true;
+            break;
+        }
+        A.multiply(d, q);
+        if (q.size() != b.size() || !vfinite(q))
+            break;
+        const T dq = vdot(d, q);
+        if (!finite(dq) || dq <= T(0))
+            break;
+        const T alpha = delta / dq;
+        if (!finite(alpha))
+            break;
+        for (size_t i = 0; i < x.size(); ++i)  // :54-65
+            x[i] = add(x[i], scl(d[i], alpha));
+        for (size_t i = 0; i < r.size(); ++i)
+            r[i] = add(r[i], scl(q[i], -alpha));
+        if (!precond(minv, r, q))
+            break;
+        const T old = delta;
+        delta = vdot(r, q);
+        if (!finite(delta) || !finite(old) || old <= T(0))
+            break;
+        const T beta = delta / old;
+        if (!finite(beta))
+            break;
+        for (size_t i = 0; i < d.size(); ++i)
+            d[i] = add(q[i], scl(d[i], beta));
+        res.iterations = it + 1;
+    }
+    if (delta <= target)
+        res.converged = true;
+    res.residual = resnorm(r);
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// World: node state, the single global implicit system and the step loop
+// (src/Core/World/World.cpp, src/Core/Solvers/SolverData.cpp,
+// src/Components/TetMeshPhysicsComponent.cpp). One tet list over world node
+// ids stands for the components in insertion order: components own disjoint,
+// increasing node ranges, so pushing their tets in order reproduces every
+// push and emission order of the reference.
+// ---------------------------------------------------------------------------
+template <class T>
+struct World
+{
+    int N = 0;
+    std::vector<V3<T>> x, v;
+    std::vector<uint8_t> fixed, nodeActive;
+    std::vector<Tet<T>> tets;
+    std::vector<FemCache<T>> cache;
+    V3<T> gravity;
+    int substeps = 1;
+    bool implicit = true;
+    int maxIt = 128;
+    T tol = T(1.0e-4f);
+    bool jacobi = true;
+    CgResult<T> last;
+
+    // FEM-level pattern over world nodes (TetMeshPhysicsComponent.cpp:36-60,
+    // 221-232 -> SparseBlockMatrix::BuildPattern), built from every tet.
+    Bsr<T> fem;
+    bool femDirty = true;
+
+    // Last implicit assembly (SolverData.cpp:126-167, 239-465).
+    std::vector<int> dyn;
+    int nd = 0;
+    std::vector<T> mass;
+    std::vector<V3<T>> force;
+    Bsr<T> A;
+    std::vector<V3<T>> rhs, dv;
+    std::vector<M3<T>> minv;
+    std::vector<Element<T>> elems;  // per tet, last computed
+    std::vector<uint8_t> elemValid;
+
+    struct Block
+    {
+        int a, b;
+        M3<T> k;
+    };
+
+    void rebuild_rest(const std::vector<V3<T>>& rest)
+    {
+        cache.resize(tets.size());
+        for (size_t i = 0; i < tets.size(); ++i)
+        {
+            init_rest(tets[i], rest);
+            build_cache(cache[i], tets[i]);
+        }
+        femDirty = true;
+    }
+    void rebuild_cache()  // TetMeshPhysicsComponent.cpp:211-219
+    {
+        for (size_t i = 0; i < tets.size(); ++i)
+            build_cache(cache[i], tets[i]);
+    }
+    void ensure_fem_pattern()
+    {
+        if (!femDirty && fem.n == N)
+            return;
+        std::vector<std::pair<int, int>> coords;
+        coords.reserve(tets.size() * 16);
+        for (const auto& t : tets)  // FEMModel.cpp:715-734
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
+                    coords.push_back({t.n[a], t.n[b]});
+        fem.build(N, coords);
+        femDirty = false;
+    }
+
+    // World.cpp:625-630 -> 551-558 -> SolverData.cpp:126-167.
+    // Returns 0 on success, 1 when a free node has no mass (World.cpp:560-580
+    // throws), 2 when the implicit assembly rejects non-finite input.
+    int precompute(T dt)
+    {
+        // TetMeshPhysicsComponent::UpdateSystem (:293-354)
+        std::vector<std::pair<int, T>> massPush;
+        std::vector<std::pair<int, V3<T>>> forcePush;
+        // FEMModel.cpp:681-713 + TMPC.cpp:127-152
+        {
+            for (const auto& t : tets)
+            {
+                const T density = std::max(T(0), t.density);
+                if (!finite(density))
+                    continue;
+                if (!t.active)
+                    continue;
+                if (!finite(t.vol) || t.vol <= T(0))
+                    continue;
+                const T nm = density * t.vol * T(0.25f);
+                if (!finite(nm) || nm <= T(0))
+                    continue;
+                for (int a = 0; a < 4; ++a)
+                {
+                    const int g = t.n[a];
+                    if (g < 0 || g >= N || fixed[g])
+                        continue;
+                    massPush.push_back({g, nm});
+                }
+            }
+        }
+        ensure_fem_pattern();
+        // FEMModel.cpp:635-679 + TMPC.cpp:331-339 (contributions in tet order)
+        fem.clear_values();
+        elems.assign(tets.size(), Element<T>{});
+        elemValid.assign(tets.size(), 0);
+        for (size_t ti = 0; ti < tets.size(); ++ti)
+        {
+            const Tet<T>& t = tets[ti];
+            if (!t.active)
+                continue;
+            bool valid = true;
+            for (int a = 0; a < 4; ++a)
+                if (t.n[a] < 0 || t.n[a] >= N)
+                    valid = false;
+            if (!valid || t.vol <= T(0))
+                continue;
+            const M3<T> ds = cols(sub(x[t.n[1]], x[t.n[0]]), sub(x[t.n[2]], x[t.n[0]]), sub(x[t.n[3]], x[t.n[0]]));
+            const M3<T> F = mul(ds, t.dmInv);
+            if (!finite(F))
+                continue;
+            const M3<T> R = polar(F);
+            T u[12];
+            corotated_u(u, t, x, R);
+            Element<T>& e = elems[ti];
+            element(e, t, cache[ti], v, u, R);
+            elemValid[ti] = 1;
+            for (int a = 0; a < 4; ++a)  // TMPC.cpp:86-96
+                forcePush.push_back({t.n[a], e.f[a]});
+            for (int a = 0; a < 4; ++a)  // TMPC.cpp:98-124
+                for (int b = 0; b < 4; ++b)
+                    if (t.n[a] <= t.n[b])
+                        fem.add_block(t.n[a], t.n[b], e.k[a][b]);
+        }
+        // TMPC.cpp:341-353 + :62-76: emit upper blocks and their mirrors.
+        std::vector<Block> blocks;
+        blocks.reserve(fem.col.size() * 2);
+        for (int r = 0; r < fem.n; ++r)
+            for (int k = fem.rowStart[r]; k < fem.rowStart[r + 1]; ++k)
+            {
+                const int c = fem.col[k];
+                blocks.push_back({r, c, fem.val[k]});
+                if (r != c)
+                    blocks.push_back({c, r, tr(fem.val[k])});
+            }
+        // SolverData::AssembleMasses (:87-101)
+        mass.assign(static_cast<size_t>(N), T(0));
+        for (const auto& m : massPush)
+        {
+            if (m.first < 0 || m.first >= N || !finite(m.second) || m.second <= T(0))
+                continue;
+            mass[m.first] += m.second;
+        }
+        // World::ValidateNodeMasses (World.cpp:560-580)
+        for (int i = 0; i < N; ++i)
+        {
+            if (!nodeActive[i] || fixed[i])
+                continue;
+            if (finite(mass[i]) && mass[i] > T(0))
+                continue;
+            return 1;
+        }
+        // World::AddGravityForces (World.cpp:582-600)
+        for (int i = 0; i < N; ++i)
+        {
+            if (!nodeActive[i] || fixed[i])
+                continue;
+            if (!finite(mass[i]) || mass[i] <= T(0))
+                continue;
+            forcePush.push_back({i, scl(gravity, mass[i])});
+        }
+        // SolverData::PrecomputeImplicitSolve (:126-167)
+        force.assign(static_cast<size_t>(N), V3<T>{});
+        for (const auto& f : forcePush)
+        {
+            if (f.first < 0 || f.first >= N || !finite(f.second))
+                continue;
+            force[f.first] = add(force[f.first], f.second);
+        }
+        if (dt <= T(0))
+            return 2;
+        // BuildDynamicNodeMapping (:239-258)
+        dyn.assign(static_cast<size_t>(N), -1);
+        nd = 0;
+        for (int i = 0; i < N; ++i)
+            if (nodeActive[i] && !fixed[i] && finite(mass[i]) && mass[i] > T(0))
+                dyn[i] = nd++;
+        if (nd <= 0)
+            return 2;
+        // AssembleImplicitMatrixAndRhs (:308-465)
+        rhs.assign(static_cast<size_t>(nd), V3<T>{});
+        std::vector<std::pair<int, int>> coords;
+        coords.reserve(static_cast<size_t>(nd) + blocks.size());
+        std::unordered_set<uint64_t> dynPairs;
+        for (int i = 0; i < N; ++i)
+            if (dyn[i] >= 0)
+                coords.push_back({dyn[i], dyn[i]});
+        const T s = dt * dt;
+        for (const Block& bl : blocks)
+        {
+            if (bl.a < 0 || bl.a >= N || bl.b < 0 || bl.b >= N)
+                continue;
+            const int rb = dyn[bl.a];
+            if (rb < 0)
+                continue;
+            if (!finite(bl.k))
+                return 2;
+            const V3<T>& vb = v[bl.b];
+            if (!finite(vb))
+                return 2;
+            const V3<T> kv = mul(bl.k, vb);
+            if (!finite(kv))
+                return 2;
+            rhs[rb] = sub(rhs[rb], scl(kv, s));
+            const int cb = dyn[bl.b];
+            if (cb < 0)
+                continue;
+            dynPairs.insert(Bsr<T>::key(rb, cb));
+            coords.push_back({rb, cb});
+        }
+        A.build(nd, coords);
+        for (int i = 0; i < N; ++i)
+            if (dyn[i] >= 0)
+                A.add_block(dyn[i], dyn[i], diag3(mass[i]));
+        for (const Block& bl : blocks)
+        {
+            if (bl.a < 0 || bl.a >= N || bl.b < 0 || bl.b >= N || !finite(bl.k))
+                continue;
+            const int rb = dyn[bl.a], cb = dyn[bl.b];
+            if (rb < 0 || cb < 0)
+                continue;
+            if (rb > cb && dynPairs.count(Bsr<T>::key(cb, rb)))
+                continue;
+            A.add_block(rb, cb, mscl(bl.k, s));
+        }
+        for (int i = 0; i < N; ++i)
+            if (dyn[i] >= 0)
+                rhs[dyn[i]] = add(rhs[dyn[i]], scl(force[i], dt));
+        return 0;
+    }
+
+    // SolverData.cpp:260-306 (DeterminantTolerance = 1e-8f)
+    bool build_minv()
+    {
+        minv.assign(static_cast<size_t>(std::max(0, nd)), M3<T>{});
+        if (!jacobi)
+            return true;
+        for (int i = 0; i < nd; ++i)
+        {
+            const int s = A.find(i, i);
+            if (s < 0)
+                return false;
+            const M3<T>& db = A.val[s];
+            if (!finite(db))
+                return false;
+            const T d = det(db);
+            if (!finite(d) || std::abs(d) <= T(1.0e-8f))
+                return false;
+            const M3<T> iv = inv(db);
+            if (!finite(iv))
+                return false;
+            minv[i] = iv;
+        }
+        return true;
+    }
+
+    // SolverData.cpp:169-212
+    bool solve()
+    {
+        last = CgResult<T>{};
+        if (nd <= 0)
+            return false;
+        if (!build_minv())
+            return false;
+        last = pcg(dv, A, rhs, maxIt, tol, minv);
+        if (dv.size() != static_cast<size_t>(nd))
+            return false;
+        return vfinite(dv);
+    }
+
+    // World.cpp:644-695: all-or-nothing update of dynamic nodes.
+    bool integrate_implicit(T dt)
+    {
+        std::vector<V3<T>> nv(static_cast<size_t>(N)), nx(static_cast<size_t>(N));
+        for (int i = 0; i < N; ++i)
+        {
+            const int d = dyn[i];
+            if (d < 0)
+                continue;
+            if (d >= static_cast<int>(dv.size()))
+                return false;
+            if (!finite(x[i]) || !finite(v[i]) || !finite(dv[d]))
+                return false;
+            const V3<T> uv = add(v[i], dv[d]);
+            const V3<T> ux = add(x[i], scl(uv, dt));
+            if (!finite(uv) || !finite(ux))
+                return false;
+            nv[i] = uv;
+            nx[i] = ux;
+        }
+        for (int i = 0; i < N; ++i)
+            if (dyn[i] >= 0)
+            {
+                v[i] = nv[i];
+                x[i] = nx[i];
+            }
+        return true;
+    }
+
+    // World.cpp:697-725
+    void integrate_explicit(T dt)
+    {
+        for (int i = 0; i < N; ++i)
+        {
+            if (!nodeActive[i] || fixed[i])
+                continue;
+            const T m = mass[i];
+            if (!finite(m) || m <= T(0))
+                continue;
+            const V3<T> acc{force[i].x / m, force[i].y / m, force[i].z / m};
+            v[i] = add(v[i], scl(acc, dt));
+            x[i] = add(x[i], scl(v[i], dt));
+        }
+    }
+
+    // World.cpp:32-73. Returns 0, or 1 if a free node had no mass (the
+    // reference throws std::runtime_error from here).
+    int step(T frameDt)
+    {
+        if (frameDt < T(0) || frameDt == T(0))
+            return 0;
+        const int steps = std::max(1, substeps);
+        const T dt = frameDt / static_cast<T>(steps);
+        for (int i = 0; i < steps; ++i)
+        {
+            const int rc = precompute(dt);
+            if (rc == 1)
+                return 1;
+            if (implicit)
+            {
+                if (rc != 0)
+                {
+                    last = CgResult<T>{};
+                    continue;
+                }
+                if (solve())
+                    integrate_implicit(dt);
+            }
+            else
+            {
+                integrate_explicit(dt);
+            }
+        }
+        return 0;
+    }
+};
+
+struct Handle
+{
+    int precision;
+    std::unique_ptr<World<float>> w32;
+    std::unique_ptr<World<double>> w64;
+};
+
+template <class T>
+void fill_world(World<T>& w, int n, const double* rest, int t, const int* tets, const double* young,
+                const double* poisson, const double* damping, const double* density)
+{
+    w.N = n;
+    std::vector<V3<T>> r(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i)
+        r[i] = V3<T>{static_cast<T>(rest[3 * i]), static_cast<T>(rest[3 * i + 1]), static_cast<T>(rest[3 * i + 2])};
+    w.x = r;
+    w.v.assign(static_cast<size_t>(n), V3<T>{});
+    w.fixed.assign(static_cast<size_t>(n), 0);
+    w.nodeActive.assign(static_cast<size_t>(n), 1);
+    w.tets.resize(static_cast<size_t>(t));
+    for (int i = 0; i < t; ++i)
+    {
+        Tet<T>& tt = w.tets[i];
+        for (int a = 0; a < 4; ++a)
+            tt.n[a] = tets[4 * i + a];
+        tt.young = static_cast<T>(young[i]);
+        tt.poisson = static_cast<T>(poisson[i]);
+        tt.damping = static_cast<T>(damping[i]);
+        tt.density = static_cast<T>(density[i]);
+    }
+    w.rebuild_rest(r);
+}
+
+template <class T> void put3(double* out, const V3<T>& v)
+{
+    out[0] = static_cast<double>(v.x);
+    out[1] = static_cast<double>(v.y);
+    out[2] = static_cast<double>(v.z);
+}
+template <class T> void putm(double* out, const M3<T>& m)  // row-major 3x3
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            out[3 * i + j] = static_cast<double>(at(m, i, j));
+}
+template <class T> M3<T> getm(const double* in)
+{
+    M3<T> m;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            set(m, i, j, static_cast<T>(in[3 * i + j]));
+    return m;
+}
+
+}  // namespace oracle
+
+namespace oracle
+{
+
+template <class T>
+void build_bsr(oracle::Bsr<T>& A, int n, int nnz, const int* row_start, const int* col_index, const double* vals)
+{
+    std::vector<std::pair<int, int>> coords;
+    for (int r = 0; r < n; ++r)
+        for (int k = row_start[r]; k < row_start[r + 1]; ++k)
+            coords.push_back({r, col_index[k]});
+    A.build(n, coords);
+    for (int r = 0; r < n; ++r)
+        for (int k = row_start[r]; k < row_start[r + 1]; ++k)
+            A.add_block(r, col_index[k], oracle::getm<T>(vals + 9 * k));
+    (void)nnz;
+}
+
+template <class T>
+void spmv_t(int n, int nnz, const int* rs, const int* ci, const double* vals, const double* x, double* y)
+{
+    oracle::Bsr<T> A;
+    build_bsr(A, n, nnz, rs, ci, vals);
+    std::vector<oracle::V3<T>> xv(static_cast<size_t>(n)), yv;
+    for (int i = 0; i < n; ++i)
+        xv[i] = {static_cast<T>(x[3 * i]), static_cast<T>(x[3 * i + 1]), static_cast<T>(x[3 * i + 2])};
+    A.multiply(xv, yv);
+    for (int i = 0; i < n; ++i)
+        oracle::put3(y + 3 * i, yv[i]);
+}
+
+template <class T>
+int pcg_t(int n, int nnz, const int* rs, const int* ci, const double* vals, const double* b, const double* minv,
+                 int max_it, double tol, double* x, double* residual, int* converged)
+{
+    oracle::Bsr<T> A;
+    build_bsr(A, n, nnz, rs, ci, vals);
+    std::vector<oracle::V3<T>> bv(static_cast<size_t>(n)), xv;
+    std::vector<oracle::M3<T>> mv(static_cast<size_t>(minv ? n : 0));
+    for (int i = 0; i < n; ++i)
+    {
+        bv[i] = {static_cast<T>(b[3 * i]), static_cast<T>(b[3 * i + 1]), static_cast<T>(b[3 * i + 2])};
+        if (minv)
+            mv[i] = oracle::getm<T>(minv + 9 * i);
+    }
+    const auto res = oracle::pcg(xv, A, bv, max_it, static_cast<T>(tol), mv);
+    for (size_t i = 0; i < xv.size() && i < static_cast<size_t>(n); ++i)
+        oracle::put3(x + 3 * i, xv[i]);
+    *residual = static_cast<double>(res.residual);
+    *converged = res.converged ? 1 : 0;
+    return static_cast<int>(xv.size()) == n ? res.iterations : -1 - res.iterations;
+}
+}  // namespace oracle
+
+using oracle::Handle;
+
+#define ORC_DISPATCH(h, ...)            \
+    do                                   \
+    {                                    \
+        if ((h)->precision == 32)        \
+        {                                \
+            auto& w = *(h)->w32;         \
+            using T = float;             \
+            (void)sizeof(T);             \
+            __VA_ARGS__;                 \
+        }                                \
+        else                             \
+        {                                \
+            auto& w = *(h)->w64;         \
+            using T = double;            \
+            (void)sizeof(T);             \
+            __VA_ARGS__;                 \
+        }                                \
+    } while (0)
+
+extern "C"
+{
+// Creates an oracle world holding one tet mesh over world node ids.
+// precision: 32 or 64. Per-tet arrays are length tet_count.
+void* orc_create(int precision, int node_count, const double* rest, int tet_count, const int* tets,
+                 const double* young, const double* poisson, const double* damping, const double* density)
+{
+    auto* h = new Handle();
+    h->precision = precision == 32 ? 32 : 64;
+    if (h->precision == 32)
+    {
+        h->w32.reset(new oracle::World<float>());
+        oracle::fill_world(*h->w32, node_count, rest, tet_count, tets, young, poisson, damping, density);
+    }
+    else
+    {
+        h->w64.reset(new oracle::World<double>());
+        oracle::fill_world(*h->w64, node_count, rest, tet_count, tets, young, poisson, damping, density);
+    }
+    return h;
+}
+
+void orc_destroy(void* p) { delete static_cast<Handle*>(p); }
+
+void orc_set_params(void* p, double gx, double gy, double gz, int substeps, double tol, int max_iter, int jacobi,
+                    int implicit)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        w.gravity = oracle::V3<T>{static_cast<T>(gx), static_cast<T>(gy), static_cast<T>(gz)};
+        w.substeps = std::max(1, substeps);
+        w.tol = static_cast<T>(tol);
+        w.maxIt = max_iter;
+        w.jacobi = jacobi != 0;
+        w.implicit = implicit != 0;
+    });
+}
+
+// World.cpp:403-417: fixing a node zeroes its velocity.
+void orc_set_fixed(void* p, const unsigned char* fixed)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        for (int i = 0; i < w.N; ++i)
+        {
+            w.fixed[i] = fixed[i] ? 1 : 0;
+            if (w.fixed[i])
+                w.v[i] = oracle::V3<T>{};
+        }
+    });
+}
+
+// Tet activation through the C ABI (PhysiKAPI.cpp:1324-1346): a changed tet's
+// corner nodes stay active only while some active tet references them
+// (ReconcileTetNodeActiveStates, :206-229); deactivated nodes lose velocity.
+void orc_set_tet_active(void* p, const unsigned char* active)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        std::vector<uint8_t> changed(w.tets.size(), 0);
+        for (size_t i = 0; i < w.tets.size(); ++i)
+        {
+            const bool a = active[i] != 0;
+            changed[i] = w.tets[i].active != a;
+            w.tets[i].active = a;
+        }
+        std::vector<uint8_t> referenced(static_cast<size_t>(w.N), 0);
+        for (const auto& t : w.tets)
+            if (t.active)
+                for (int a = 0; a < 4; ++a)
+                    if (t.n[a] >= 0 && t.n[a] < w.N)
+                        referenced[t.n[a]] = 1;
+        for (size_t i = 0; i < w.tets.size(); ++i)
+            if (changed[i])
+                for (int a = 0; a < 4; ++a)
+                {
+                    const int g = w.tets[i].n[a];
+                    if (g < 0 || g >= w.N)
+                        continue;
+                    w.nodeActive[g] = referenced[g];
+                    if (!referenced[g])
+                        w.v[g] = oracle::V3<T>{};
+                }
+    });
+}
+
+void orc_set_node_active(void* p, const unsigned char* active)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        for (int i = 0; i < w.N; ++i)
+            w.nodeActive[i] = active[i] ? 1 : 0;
+    });
+}
+
+// Per-tet Young's modulus; rebuilds the element cache like
+// TetMeshPhysicsComponent::RebuildTetFemCache (:211-219).
+void orc_set_young(void* p, const double* young)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        for (size_t i = 0; i < w.tets.size(); ++i)
+            w.tets[i].young = static_cast<T>(young[i]);
+        w.rebuild_cache();
+    });
+}
+
+void orc_set_state(void* p, const double* x, const double* v)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        for (int i = 0; i < w.N; ++i)
+        {
+            if (x)
+                w.x[i] = oracle::V3<T>{static_cast<T>(x[3 * i]), static_cast<T>(x[3 * i + 1]), static_cast<T>(x[3 * i + 2])};
+            if (v)
+                w.v[i] = oracle::V3<T>{static_cast<T>(v[3 * i]), static_cast<T>(v[3 * i + 1]), static_cast<T>(v[3 * i + 2])};
+        }
+    });
+}
+
+void orc_get_state(void* p, double* x, double* v)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        for (int i = 0; i < w.N; ++i)
+        {
+            if (x) oracle::put3(x + 3 * i, w.x[i]);
+            if (v) oracle::put3(v + 3 * i, w.v[i]);
+        }
+    });
+}
+
+// Advances one frame. Returns 0, or 1 when a free node has no mass.
+int orc_step(void* p, double frame_dt)
+{
+    auto* h = static_cast<Handle*>(p);
+    int rc = 0;
+    ORC_DISPATCH(h, { rc = w.step(static_cast<T>(frame_dt)); });
+    return rc;
+}
+
+void orc_get_cg(void* p, int* iterations, double* residual, int* converged)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        *iterations = w.last.iterations;
+        *residual = static_cast<double>(w.last.residual);
+        *converged = w.last.converged ? 1 : 0;
+    });
+}
+
+// Runs the assembly of one substep of length dt (no solve, no state change).
+// Returns the precompute status and the sizes of the dynamic system.
+int orc_assemble(void* p, double dt, int* nd, int* nnzb_upper)
+{
+    auto* h = static_cast<Handle*>(p);
+    int rc = 0;
+    ORC_DISPATCH(h, {
+        rc = w.precompute(static_cast<T>(dt));
+        *nd = w.nd;
+        *nnzb_upper = static_cast<int>(w.A.col.size());
+    });
+    return rc;
+}
+
+// Copies the last assembled system: dynamic map (N), upper BSR pattern,
+// block values (row-major 3x3 each), rhs (nd x 3), node masses (N).
+void orc_get_system(void* p, int* dyn, int* row_start, int* col_index, double* vals, double* rhs, double* mass)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        for (int i = 0; i < w.N; ++i)
+        {
+            if (dyn) dyn[i] = w.dyn[i];
+            if (mass) mass[i] = static_cast<double>(w.mass[i]);
+        }
+        for (size_t i = 0; i < w.A.rowStart.size(); ++i)
+            if (row_start) row_start[i] = w.A.rowStart[i];
+        for (size_t k = 0; k < w.A.col.size(); ++k)
+        {
+            if (col_index) col_index[k] = w.A.col[k];
+            if (vals) oracle::putm(vals + 9 * k, w.A.val[k]);
+        }
+        for (int i = 0; i < w.nd; ++i)
+            if (rhs) oracle::put3(rhs + 3 * i, w.rhs[i]);
+    });
+}
+
+// Element -> slot scatter map of the last assembled system: for every tet
+// and every local pair (a, b), the index of the upper block holding the
+// canonical (dyn(n_a), dyn(n_b)) coupling, or -1 when either node is not
+// dynamic. Output is tet_count x 16.
+void orc_get_scatter(void* p, int* slots)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        for (size_t t = 0; t < w.tets.size(); ++t)
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
+                {
+                    const int ra = w.dyn[w.tets[t].n[a]], rb = w.dyn[w.tets[t].n[b]];
+                    slots[16 * t + 4 * a + b] = (ra < 0 || rb < 0) ? -1 : w.A.find(ra, rb);
+                }
+    });
+}
+
+// Per-tet outputs of the last assembly: valid flag, rotation (row-major),
+// 4 forces and 16 blocks R K_ab R^T (row-major each).
+void orc_get_elements(void* p, unsigned char* valid, double* R, double* f, double* K)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        for (size_t t = 0; t < w.tets.size(); ++t)
+        {
+            if (valid) valid[t] = w.elemValid[t];
+            const auto& e = w.elems[t];
+            if (R) oracle::putm(R + 9 * t, e.R);
+            for (int a = 0; a < 4; ++a)
+            {
+                if (f) oracle::put3(f + 12 * t + 3 * a, e.f[a]);
+                for (int b = 0; b < 4; ++b)
+                    if (K) oracle::putm(K + 144 * t + 9 * (4 * a + b), e.k[a][b]);
+            }
+        }
+    });
+}
+
+// Rest data of every tet: volume, Dm^-1 (row-major), gradients (4 x 3).
+void orc_get_rest(void* p, double* vol, double* dminv, double* grad)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        for (size_t t = 0; t < w.tets.size(); ++t)
+        {
+            const auto& tt = w.tets[t];
+            if (vol) vol[t] = static_cast<double>(tt.vol);
+            if (dminv) oracle::putm(dminv + 9 * t, tt.dmInv);
+            for (int a = 0; a < 4; ++a)
+                if (grad) oracle::put3(grad + 12 * t + 3 * a, tt.grad[a]);
+        }
+    });
+}
+
+// Newton polar rotation of one 3x3 (row-major in and out). Returns the
+// iteration count.
+int orc_polar(int precision, const double* F, double* R)
+{
+    int iters = 0;
+    if (precision == 32)
+        oracle::putm(R, oracle::polar(oracle::getm<float>(F), &iters));
+    else
+        oracle::putm(R, oracle::polar(oracle::getm<double>(F), &iters));
+    return iters;
+}
+
+
+
+// y = A x for a symmetric-upper BSR given as (row_start, col_index, vals).
+void orc_spmv(int precision, int n, int nnz, const int* row_start, const int* col_index, const double* vals,
+              const double* x, double* y)
+{
+    if (precision == 32)
+        oracle::spmv_t<float>(n, nnz, row_start, col_index, vals, x, y);
+    else
+        oracle::spmv_t<double>(n, nnz, row_start, col_index, vals, x, y);
+}
+
+
+// Jacobi PCG on a symmetric-upper BSR. Returns the iteration count, or
+// -1 - iterations when the solution vector was cleared (rejected input).
+int orc_pcg(int precision, int n, int nnz, const int* row_start, const int* col_index, const double* vals,
+            const double* b, const double* minv, int max_it, double tol, double* x, double* residual, int* converged)
+{
+    if (precision == 32)
+        return oracle::pcg_t<float>(n, nnz, row_start, col_index, vals, b, minv, max_it, tol, x, residual, converged);
+    return oracle::pcg_t<double>(n, nnz, row_start, col_index, vals, b, minv, max_it, tol, x, residual, converged);
+}
+}

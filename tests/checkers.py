@@ -1,0 +1,325 @@
+"""ctypes bindings for the checker libraries (test infrastructure).
+
+* ``Oracle``   -> oracle/liboracle.so, the CPU restatement (float or double).
+* ``Reference``-> oracle/_ref/libphysik_ref.so, the reference's own sources
+  compiled in place plus oracle/ref_driver.cpp. It exists only where the
+  reference tree was available at build time; tests that need it skip
+  otherwise.
+
+Both take float64 numpy arrays at the boundary (the reference binding converts
+to float32, exactly as its FP32 ABI does).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libphysik_ref.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_bp = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+_oracle = None
+_ref = None
+
+
+def oracle_lib():
+    global _oracle
+    if _oracle is None:
+        lib = C.CDLL(ORACLE_SO)
+        lib.orc_create.restype = C.c_void_p
+        lib.orc_create.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _ip, _dp, _dp, _dp, _dp]
+        lib.orc_destroy.argtypes = [C.c_void_p]
+        lib.orc_set_params.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int,
+                                       C.c_int, C.c_int]
+        lib.orc_set_fixed.argtypes = [C.c_void_p, _bp]
+        lib.orc_set_tet_active.argtypes = [C.c_void_p, _bp]
+        lib.orc_set_node_active.argtypes = [C.c_void_p, _bp]
+        lib.orc_set_young.argtypes = [C.c_void_p, _dp]
+        lib.orc_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_get_state.argtypes = [C.c_void_p, _dp, _dp]
+        lib.orc_step.argtypes = [C.c_void_p, C.c_double]
+        lib.orc_step.restype = C.c_int
+        lib.orc_get_cg.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        lib.orc_assemble.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.orc_assemble.restype = C.c_int
+        lib.orc_get_system.argtypes = [C.c_void_p, _ip, _ip, _ip, _dp, _dp, _dp]
+        lib.orc_get_scatter.argtypes = [C.c_void_p, _ip]
+        lib.orc_get_elements.argtypes = [C.c_void_p, _bp, _dp, _dp, _dp]
+        lib.orc_get_rest.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.orc_polar.argtypes = [C.c_int, _dp, _dp]
+        lib.orc_polar.restype = C.c_int
+        lib.orc_spmv.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp]
+        lib.orc_pcg.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, C.c_void_p, C.c_int, C.c_double, _dp,
+                                C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        lib.orc_pcg.restype = C.c_int
+        _oracle = lib
+    return _oracle
+
+
+def reference_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        lib = C.CDLL(REF_SO)
+        lib.ref_create.restype = C.c_void_p
+        lib.ref_create.argtypes = [_fp, C.c_int, _ip, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]
+        lib.ref_destroy.argtypes = [C.c_void_p]
+        lib.ref_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.ref_counts.restype = C.c_int
+        lib.ref_set_params.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_int, C.c_float, C.c_int,
+                                       C.c_int]
+        lib.ref_set_fixed.argtypes = [C.c_void_p, _bp]
+        lib.ref_set_tet_active.argtypes = [C.c_void_p, _bp]
+        lib.ref_set_young.argtypes = [C.c_void_p, _fp]
+        lib.ref_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ref_get_state.argtypes = [C.c_void_p, _fp, _fp]
+        lib.ref_step.argtypes = [C.c_void_p, C.c_float]
+        lib.ref_step.restype = C.c_int
+        lib.ref_get_cg.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_int)]
+        lib.ref_assemble.argtypes = [C.c_void_p, C.c_float, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.ref_assemble.restype = C.c_int
+        lib.ref_get_system.argtypes = [C.c_void_p, _ip, _ip, _ip, _fp, _fp, _fp]
+        lib.ref_get_scatter.argtypes = [C.c_void_p, _ip]
+        lib.ref_get_elements.argtypes = [C.c_void_p, _bp, _fp, _fp]
+        lib.ref_get_rest.argtypes = [C.c_void_p, _fp, _fp, _fp]
+        lib.ref_spmv.argtypes = [C.c_int, _ip, _ip, _fp, _fp, _fp]
+        _ref = lib
+    return _ref
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class Oracle:
+    """One oracle world over a single tet mesh (world node ids)."""
+
+    def __init__(self, precision, rest, tets, young, poisson, damping, density):
+        self.lib = oracle_lib()
+        self.n = int(rest.shape[0])
+        self.t = int(tets.shape[0])
+        t = self.t
+
+        def per_tet(v):
+            a = np.asarray(v, dtype=np.float64)
+            return _c(np.broadcast_to(a, (t,)), np.float64)
+
+        self.h = self.lib.orc_create(precision, self.n, _c(rest, np.float64).reshape(-1), t,
+                                     _c(tets, np.int32).reshape(-1), per_tet(young), per_tet(poisson),
+                                     per_tet(damping), per_tet(density))
+
+    def close(self):
+        if self.h:
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, gravity=(0.0, 0.0, 0.0), substeps=1, tol=1e-4, max_iter=128, jacobi=True, implicit=True):
+        self.lib.orc_set_params(self.h, gravity[0], gravity[1], gravity[2], substeps, tol, max_iter, int(jacobi),
+                                int(implicit))
+
+    def set_fixed(self, fixed):
+        self.lib.orc_set_fixed(self.h, _c(fixed, np.uint8))
+
+    def set_tet_active(self, active):
+        self.lib.orc_set_tet_active(self.h, _c(active, np.uint8))
+
+    def set_node_active(self, active):
+        self.lib.orc_set_node_active(self.h, _c(active, np.uint8))
+
+    def set_young(self, young):
+        self.lib.orc_set_young(self.h, _c(young, np.float64))
+
+    def set_state(self, x=None, v=None):
+        xa = None if x is None else _c(x, np.float64)
+        va = None if v is None else _c(v, np.float64)
+        self.lib.orc_set_state(self.h, None if xa is None else xa.ctypes.data,
+                               None if va is None else va.ctypes.data)
+
+    def state(self):
+        x = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        self.lib.orc_get_state(self.h, x, v)
+        return x, v
+
+    def step(self, dt):
+        return self.lib.orc_step(self.h, dt)
+
+    def cg(self):
+        it, res, conv = C.c_int(), C.c_double(), C.c_int()
+        self.lib.orc_get_cg(self.h, C.byref(it), C.byref(res), C.byref(conv))
+        return it.value, res.value, bool(conv.value)
+
+    def assemble(self, dt):
+        nd, nnz = C.c_int(), C.c_int()
+        rc = self.lib.orc_assemble(self.h, dt, C.byref(nd), C.byref(nnz))
+        nd, nnz = nd.value, nnz.value
+        dyn = np.zeros(self.n, np.int32)
+        rs = np.zeros(max(nd, 0) + 1, np.int32)
+        ci = np.zeros(max(nnz, 1), np.int32)
+        vals = np.zeros(max(nnz, 1) * 9)
+        rhs = np.zeros(max(nd, 1) * 3)
+        mass = np.zeros(self.n)
+        if nd > 0:
+            self.lib.orc_get_system(self.h, dyn, rs, ci, vals, rhs, mass)
+        return dict(rc=rc, nd=nd, nnz=nnz, dyn=dyn, row_start=rs, col=ci[:nnz], vals=vals[:9 * nnz].reshape(-1, 3, 3),
+                    rhs=rhs[:3 * nd].reshape(-1, 3), mass=mass)
+
+    def scatter(self):
+        s = np.zeros(self.t * 16, np.int32)
+        self.lib.orc_get_scatter(self.h, s)
+        return s.reshape(self.t, 4, 4)
+
+    def elements(self):
+        valid = np.zeros(self.t, np.uint8)
+        R = np.zeros(self.t * 9)
+        f = np.zeros(self.t * 12)
+        K = np.zeros(self.t * 144)
+        self.lib.orc_get_elements(self.h, valid, R, f, K)
+        return valid.astype(bool), R.reshape(-1, 3, 3), f.reshape(-1, 4, 3), K.reshape(-1, 4, 4, 3, 3)
+
+    def rest(self):
+        vol = np.zeros(self.t)
+        dminv = np.zeros(self.t * 9)
+        grad = np.zeros(self.t * 12)
+        self.lib.orc_get_rest(self.h, vol, dminv, grad)
+        return vol, dminv.reshape(-1, 3, 3), grad.reshape(-1, 4, 3)
+
+
+def oracle_polar(F, precision=64):
+    R = np.zeros(9)
+    it = oracle_lib().orc_polar(precision, _c(F, np.float64).reshape(-1), R)
+    return R.reshape(3, 3), it
+
+
+def oracle_spmv(row_start, col, vals, x, precision=64):
+    n = len(row_start) - 1
+    y = np.zeros(n * 3)
+    oracle_lib().orc_spmv(precision, n, len(col), _c(row_start, np.int32), _c(col, np.int32),
+                          _c(vals, np.float64).reshape(-1), _c(x, np.float64).reshape(-1), y)
+    return y.reshape(-1, 3)
+
+
+def oracle_pcg(row_start, col, vals, b, minv, max_it, tol, precision=64):
+    n = len(row_start) - 1
+    x = np.zeros(n * 3)
+    res = C.c_double()
+    conv = C.c_int()
+    mv = None if minv is None else _c(minv, np.float64).reshape(-1)
+    it = oracle_lib().orc_pcg(precision, n, len(col), _c(row_start, np.int32), _c(col, np.int32),
+                              _c(vals, np.float64).reshape(-1), _c(b, np.float64).reshape(-1),
+                              None if mv is None else mv.ctypes.data, max_it, tol, x, C.byref(res), C.byref(conv))
+    return x.reshape(-1, 3), it, res.value, bool(conv.value)
+
+
+class Reference:
+    """The compiled reference driven through its own C ABI (FP32)."""
+
+    def __init__(self, rest, tets, density, young, poisson, damping):
+        self.lib = ref_lib()
+        self.h = self.lib.ref_create(_c(rest, np.float32).reshape(-1), int(rest.shape[0]),
+                                     _c(tets, np.int32).reshape(-1), int(tets.shape[0]), density, young, poisson,
+                                     damping)
+        n, t = C.c_int(), C.c_int()
+        self.lib.ref_counts(self.h, C.byref(n), C.byref(t))
+        self.n, self.t = n.value, t.value
+
+    def close(self):
+        if self.h:
+            self.lib.ref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, gravity=(0.0, 0.0, 0.0), substeps=1, tol=1e-4, max_iter=128, implicit=True):
+        self.lib.ref_set_params(self.h, gravity[0], gravity[1], gravity[2], substeps, tol, max_iter, int(implicit))
+
+    def set_fixed(self, fixed):
+        self.lib.ref_set_fixed(self.h, _c(fixed, np.uint8))
+
+    def set_tet_active(self, active):
+        self.lib.ref_set_tet_active(self.h, _c(active, np.uint8))
+
+    def set_young(self, young):
+        self.lib.ref_set_young(self.h, _c(young, np.float32))
+
+    def set_state(self, x=None, v=None):
+        xa = None if x is None else _c(x, np.float32)
+        va = None if v is None else _c(v, np.float32)
+        self.lib.ref_set_state(self.h, None if xa is None else xa.ctypes.data,
+                               None if va is None else va.ctypes.data)
+
+    def state(self):
+        x = np.zeros((self.n, 3), np.float32)
+        v = np.zeros((self.n, 3), np.float32)
+        self.lib.ref_get_state(self.h, x, v)
+        return x, v
+
+    def step(self, dt):
+        return self.lib.ref_step(self.h, dt)
+
+    def cg(self):
+        it, res, conv = C.c_int(), C.c_float(), C.c_int()
+        self.lib.ref_get_cg(self.h, C.byref(it), C.byref(res), C.byref(conv))
+        return it.value, res.value, bool(conv.value)
+
+    def assemble(self, dt):
+        nd, nnz = C.c_int(), C.c_int()
+        rc = self.lib.ref_assemble(self.h, dt, C.byref(nd), C.byref(nnz))
+        nd, nnz = nd.value, nnz.value
+        dyn = np.zeros(self.n, np.int32)
+        rs = np.zeros(nd + 1, np.int32)
+        ci = np.zeros(max(nnz, 1), np.int32)
+        vals = np.zeros(max(nnz, 1) * 9, np.float32)
+        rhs = np.zeros(max(nd, 1) * 3, np.float32)
+        mass = np.zeros(self.n, np.float32)
+        self.lib.ref_get_system(self.h, dyn, rs, ci, vals, rhs, mass)
+        return dict(rc=rc, nd=nd, nnz=nnz, dyn=dyn, row_start=rs, col=ci[:nnz], vals=vals[:9 * nnz].reshape(-1, 3, 3),
+                    rhs=rhs[:3 * nd].reshape(-1, 3), mass=mass)
+
+    def scatter(self):
+        s = np.zeros(self.t * 16, np.int32)
+        self.lib.ref_get_scatter(self.h, s)
+        return s.reshape(self.t, 4, 4)
+
+    def elements(self):
+        valid = np.zeros(self.t, np.uint8)
+        f = np.zeros(self.t * 12, np.float32)
+        K = np.zeros(self.t * 144, np.float32)
+        self.lib.ref_get_elements(self.h, valid, f, K)
+        return valid.astype(bool), f.reshape(-1, 4, 3), K.reshape(-1, 4, 4, 3, 3)
+
+    def rest(self):
+        vol = np.zeros(self.t, np.float32)
+        dminv = np.zeros(self.t * 9, np.float32)
+        grad = np.zeros(self.t * 12, np.float32)
+        self.lib.ref_get_rest(self.h, vol, dminv, grad)
+        return vol, dminv.reshape(-1, 3, 3), grad.reshape(-1, 4, 3)
+
+
+def ref_spmv(row_start, col, vals, x):
+    n = len(row_start) - 1
+    y = np.zeros(n * 3, np.float32)
+    ref_lib().ref_spmv(n, _c(row_start, np.int32), _c(col, np.int32), _c(vals, np.float32).reshape(-1),
+                       _c(x, np.float32).reshape(-1), y)
+    return y.reshape(-1, 3)

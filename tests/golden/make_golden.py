@@ -1,0 +1,206 @@
+"""Writes the golden fixtures in tests/golden/ from the compiled reference.
+
+Run in a container that holds the reference tree, after ``make -C oracle ref``:
+
+    python tests/golden/make_golden.py
+
+Every fixture is data produced by the reference's own code
+(oracle/_ref/libphysik_ref.so: the reference sources compiled in place, driven
+through their C ABI by oracle/ref_driver.cpp). Inputs are synthetic meshes from
+physik_amd.meshgen; all values are float32 because the reference is FP32-only.
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from physik_amd import meshgen as mg  # noqa: E402
+from checkers import Reference  # noqa: E402
+
+G = (0.0, 0.0, -9.81)
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float32)
+
+
+def config1():
+    """Config-1 beam: 10^3 cells of 0.1 m, 5-tet split, x=0 face fixed."""
+    rest, tets = mg.box_mesh(10, 10, 10, 0.1, "five")
+    fixed = np.zeros(len(rest), np.uint8)
+    fixed[mg.face_nodes(10, 10, 10, 0)] = 1
+    dt = np.float32(1.0 / 60.0)
+    ref = Reference(rest, tets, 1000.0, 1e5, 0.3, 0.0)
+    ref.set_params(G, 1, 1e-6, 200)
+    ref.set_fixed(fixed)
+
+    sysm = ref.assemble(float(dt))
+    scatter = ref.scatter()
+    valid, fe, ke = ref.elements()
+    snaps = {}
+    iters, res, conv = [], [], []
+    for s in range(1, 101):
+        assert ref.step(float(dt)) == 0
+        it, r, c = ref.cg()
+        iters.append(it)
+        res.append(r)
+        conv.append(c)
+        if s in (1, 2, 10, 100):
+            x, v = ref.state()
+            snaps[f"x{s}"] = x
+            snaps[f"v{s}"] = v
+    ref.close()
+    np.savez_compressed(
+        os.path.join(HERE, "config1_fp32.npz"),
+        rest=f32(rest), tets=tets, fixed=fixed, dt=dt,
+        cg_iterations=np.array(iters, np.int32), cg_residual=f32(res), cg_converged=np.array(conv, np.uint8),
+        **snaps)
+    np.savez_compressed(
+        os.path.join(HERE, "config1_system_step1.npz"),
+        dyn=sysm["dyn"], row_start=sysm["row_start"], col=sysm["col"], vals=sysm["vals"], rhs=sysm["rhs"],
+        mass=sysm["mass"], scatter=scatter)
+    print("config1:", len(rest), "nodes", len(tets), "tets, nd", sysm["nd"], "nnzb_upper", sysm["nnz"],
+          "cg iterations avg", np.mean(iters), "max", max(iters))
+
+
+def elements():
+    """Single tets in assorted states: element forces and R K R^T blocks."""
+    rng = mg.Pcg32(11)
+    unit = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float64)
+    cases = []
+
+    def rot(axis, ang):
+        axis = np.asarray(axis, float) / np.linalg.norm(axis)
+        K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+        return np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+
+    for k in range(12):
+        rest = unit * (0.1 + 0.9 * rng.uniform(1)[0]) + (rng.uniform(12).reshape(4, 3) - 0.5) * 0.05
+        defo = np.eye(3) + (rng.uniform(9).reshape(3, 3) - 0.5) * 0.3
+        Q = rot(rng.uniform(3) - 0.5, 3.0 * rng.uniform(1)[0])
+        x = (rest - rest[0]) @ (Q @ defo).T + rest[0] + (rng.uniform(3) - 0.5)
+        v = (rng.uniform(12).reshape(4, 3) - 0.5) * 2.0
+        cases.append((rest, x, v, 10 ** (2 + 4 * rng.uniform(1)[0]), 0.2 + 0.28 * rng.uniform(1)[0],
+                      rng.uniform(1)[0]))
+    # large twist
+    cases.append((unit * 0.2, (unit * 0.2) @ rot([1, 0, 0], 1.4).T + np.array([0, 0, 0.03]) * unit[:, :1],
+                  np.zeros((4, 3)), 1e6, 0.45, 0.0))
+    # inverted (node 3 pushed through the base)
+    inv = unit.copy()
+    inv[3] = [0.2, 0.2, -0.4]
+    cases.append((unit, inv, np.zeros((4, 3)), 1e4, 0.3, 0.0))
+    # nearly flat (det F small but above the floor)
+    flat = unit.copy()
+    flat[3] = [0.1, 0.1, 2e-3]
+    cases.append((unit, flat, np.zeros((4, 3)), 1e4, 0.3, 0.1))
+    # rigid rotation (corotational force must vanish)
+    cases.append((unit, unit @ rot([0, 0, 1], np.pi / 2).T, np.zeros((4, 3)), 1000.0, 0.25, 0.0))
+    # Poisson clamp and negative E (sanitised in BuildElasticityMatrix)
+    cases.append((unit, unit + np.array([[0, 0, 0], [0.1, 0, 0], [0, 0, 0], [0, 0, 0]]), np.zeros((4, 3)),
+                  100.0, 0.99, 0.0))
+    cases.append((unit, unit + np.array([[0, 0, 0], [0.1, 0, 0], [0, 0, 0], [0, 0, 0]]), np.zeros((4, 3)),
+                  -10.0, 0.25, 0.0))
+
+    out = {k: [] for k in ("rest", "x", "v", "young", "poisson", "damping", "valid", "f", "K", "vol", "dminv")}
+    tets = np.array([[0, 1, 2, 3]], np.int32)
+    for rest, x, v, E, nu, c in cases:
+        ref = Reference(rest, tets, 1.0, E, nu, c)
+        ref.set_state(x, v)
+        valid, fe, ke = ref.elements()
+        vol, dminv, _ = ref.rest()
+        ref.close()
+        out["rest"].append(f32(rest))
+        out["x"].append(f32(x))
+        out["v"].append(f32(v))
+        out["young"].append(np.float32(E))
+        out["poisson"].append(np.float32(nu))
+        out["damping"].append(np.float32(c))
+        out["valid"].append(valid[0])
+        out["f"].append(fe[0])
+        out["K"].append(ke[0])
+        out["vol"].append(vol[0])
+        out["dminv"].append(dminv[0])
+    np.savez_compressed(os.path.join(HERE, "elements_fp32.npz"), **{k: np.array(v) for k, v in out.items()})
+    print("elements:", len(cases), "cases")
+
+
+def small_cases():
+    """Tiny meshes stepped a few frames under assorted settings."""
+    cases = {}
+    unit = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float64)
+    two = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, -1]], np.float64)
+    # name: (rest, tets, fixed, E, nu, damping, density, gravity, substeps, tol, maxit, dt, steps, x0, v0, active)
+    specs = {
+        "single_tet_gravity": (unit, [[0, 1, 2, 3]], [0, 0, 0, 0], 25.0, 0.3, 0.0, 24.0, (1, 0, 0), 1, 1e-4, 16,
+                               0.1, 4, None, None, None),
+        "single_tet_stretched_damped": (unit, [[0, 1, 2, 3]], [1, 1, 1, 0], 100.0, 0.25, 0.3, 1.0, (0, 0, 0), 1,
+                                        1e-6, 64, 0.05, 6, unit + [[0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0.2]],
+                                        None, None),
+        "two_tets_substeps": (two, [[0, 1, 2, 3], [0, 1, 2, 4]], [1, 0, 0, 0, 0], 50.0, 0.3, 0.1, 2.0, (0, -9.81, 0),
+                              3, 1e-5, 100, 0.1, 5, None, [[0, 0, 0], [0.1, 0, 0], [0, 0.2, 0], [0, 0, 0.3],
+                                                            [0, 0, 0]], None),
+        "two_tets_maxit1": (two, [[0, 1, 2, 3], [0, 1, 2, 4]], [0, 0, 0, 0, 0], 1e3, 0.3, 0.0, 1.0, (0, 0, -1), 1,
+                            1e-8, 1, 0.02, 3, None, None, None),
+        "two_tets_one_inactive": (two, [[0, 1, 2, 3], [0, 1, 2, 4]], [1, 0, 0, 0, 1], 80.0, 0.3, 0.0, 3.0,
+                                  (0, 0, -9.81), 1, 1e-6, 50, 0.05, 4, None, None, [1, 0]),
+        "two_tets_deactivate_free_node": (two, [[0, 1, 2, 3], [0, 1, 2, 4]], [1, 0, 0, 0, 0], 80.0, 0.3, 0.0, 3.0,
+                                          (0, 0, -9.81), 1, 1e-6, 50, 0.05, 3, None,
+                                          [[0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0.5, 0, 0]], [1, 0]),
+    }
+    # a 2x2x2 Kuhn block with jitter and a twist, the config-2 shape in miniature
+    rest, tets = mg.box_mesh(2, 2, 2, 0.25, "kuhn")
+    rng = mg.Pcg32(5)
+    rest = mg.jitter(rest, 0.25, 0.1, rng)
+    x0 = mg.twist_about_x(rest, 2, 2, 2, 0.25, np.pi / 2)
+    fixed = np.zeros(len(rest), np.uint8)
+    fixed[mg.face_nodes(2, 2, 2, 0)] = 1
+    specs["kuhn2_twist"] = (rest, tets, fixed, 1e6, 0.45, 0.0, 1000.0, (0, 0, -9.81), 1, 1e-6, 1024, 1.0 / 120.0,
+                            5, x0, None, None)
+    for name, (rest, tets, fixed, E, nu, c, rho, g, sub, tol, maxit, dt, steps, x0, v0, active) in specs.items():
+        rest = np.asarray(rest, np.float64)
+        tets = np.asarray(tets, np.int32)
+        ref = Reference(rest, tets, rho, E, nu, c)
+        ref.set_params(g, sub, tol, maxit)
+        fixed = np.asarray(fixed, np.uint8)
+        ref.set_fixed(fixed)
+        if x0 is not None or v0 is not None:
+            ref.set_state(None if x0 is None else np.asarray(x0, np.float64),
+                          None if v0 is None else np.asarray(v0, np.float64))
+        if active is not None:
+            ref.set_tet_active(np.asarray(active, np.uint8))
+        x_init, v_init = ref.state()
+        xs, vs, its, rs, cs = [], [], [], [], []
+        for _ in range(steps):
+            assert ref.step(np.float32(dt)) == 0
+            x, v = ref.state()
+            it, r, cv = ref.cg()
+            xs.append(x)
+            vs.append(v)
+            its.append(it)
+            rs.append(r)
+            cs.append(cv)
+        ref.close()
+        cases[name] = dict(rest=f32(rest), tets=tets, fixed=fixed, young=np.float32(E), poisson=np.float32(nu),
+                           damping=np.float32(c), density=np.float32(rho), gravity=f32(g), substeps=np.int32(sub),
+                           tol=np.float32(tol), max_iter=np.int32(maxit), dt=np.float32(dt),
+                           x_init=x_init, v_init=v_init,
+                           active=np.ones(len(tets), np.uint8) if active is None else np.asarray(active, np.uint8),
+                           x=np.array(xs), v=np.array(vs), cg_iterations=np.array(its, np.int32), cg_residual=f32(rs),
+                           cg_converged=np.array(cs, np.uint8))
+    flat = {f"{n}__{k}": v for n, d in cases.items() for k, v in d.items()}
+    np.savez_compressed(os.path.join(HERE, "small_cases_fp32.npz"), names=np.array(list(cases)), **flat)
+    print("small cases:", list(cases))
+
+
+if __name__ == "__main__":
+    config1()
+    elements()
+    small_cases()

@@ -1,0 +1,97 @@
+"""The oracle's FP32 instantiation against fixtures from the compiled reference.
+
+Every comparison here is bit-exact: the restatement follows the reference's
+operation order and both are built without FMA contraction, so any deviation
+(a dropped term, a swapped index, a different summation order) shows up as a
+nonzero difference.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from checkers import Oracle
+
+
+def load(golden_dir, name):
+    return np.load(os.path.join(golden_dir, name), allow_pickle=False)
+
+
+def make_config1(g, precision=32):
+    o = Oracle(precision, g["rest"].astype(np.float64), g["tets"], 1e5, 0.3, 0.0, 1000.0)
+    o.set_params((0.0, 0.0, -9.81), 1, 1e-6, 200)
+    o.set_fixed(g["fixed"])
+    return o
+
+
+def test_config1_trajectory_bitwise(golden_dir):
+    g = load(golden_dir, "config1_fp32.npz")
+    o = make_config1(g)
+    dt = float(g["dt"])
+    for s in range(1, 101):
+        assert o.step(dt) == 0
+        it, res, conv = o.cg()
+        assert it == g["cg_iterations"][s - 1]
+        assert np.float32(res) == g["cg_residual"][s - 1]
+        assert conv == bool(g["cg_converged"][s - 1])
+        if f"x{s}" in g:
+            x, v = o.state()
+            assert np.array_equal(x.astype(np.float32), g[f"x{s}"]), f"positions differ at step {s}"
+            assert np.array_equal(v.astype(np.float32), g[f"v{s}"]), f"velocities differ at step {s}"
+
+
+def test_config1_system_bitwise(golden_dir):
+    g = load(golden_dir, "config1_fp32.npz")
+    s = load(golden_dir, "config1_system_step1.npz")
+    o = make_config1(g)
+    sysm = o.assemble(float(g["dt"]))
+    assert sysm["rc"] == 0
+    assert np.array_equal(sysm["dyn"], s["dyn"])
+    assert np.array_equal(sysm["row_start"], s["row_start"])
+    assert np.array_equal(sysm["col"], s["col"])
+    assert np.array_equal(o.scatter(), s["scatter"])
+    assert np.array_equal(sysm["vals"].astype(np.float32), s["vals"])
+    assert np.array_equal(sysm["rhs"].astype(np.float32), s["rhs"])
+    assert np.array_equal(sysm["mass"].astype(np.float32), s["mass"])
+
+
+def test_elements_bitwise(golden_dir):
+    e = load(golden_dir, "elements_fp32.npz")
+    tets = np.array([[0, 1, 2, 3]], np.int32)
+    for i in range(len(e["young"])):
+        o = Oracle(32, e["rest"][i].astype(np.float64), tets, float(e["young"][i]), float(e["poisson"][i]),
+                   float(e["damping"][i]), 1.0)
+        o.set_state(e["x"][i].astype(np.float64), e["v"][i].astype(np.float64))
+        o.assemble(0.01)
+        valid, R, f, K = o.elements()
+        vol, dminv, _ = o.rest()
+        assert np.float32(vol[0]) == e["vol"][i]
+        assert np.array_equal(dminv[0].astype(np.float32), e["dminv"][i])
+        assert valid[0] == bool(e["valid"][i]), f"case {i}"
+        if valid[0]:
+            assert np.array_equal(f[0].astype(np.float32), e["f"][i]), f"forces differ in case {i}"
+            assert np.array_equal(K[0].astype(np.float32), e["K"][i]), f"blocks differ in case {i}"
+
+
+@pytest.mark.parametrize("name", ["single_tet_gravity", "single_tet_stretched_damped", "two_tets_substeps",
+                                  "two_tets_maxit1", "two_tets_one_inactive",
+                                  "two_tets_deactivate_free_node", "kuhn2_twist"])
+def test_small_cases_bitwise(golden_dir, name):
+    z = load(golden_dir, "small_cases_fp32.npz")
+    c = {k.split("__", 1)[1]: z[k] for k in z.files if k.startswith(name + "__")}
+    o = Oracle(32, c["rest"].astype(np.float64), c["tets"], float(c["young"]), float(c["poisson"]),
+               float(c["damping"]), float(c["density"]))
+    o.set_params(tuple(float(a) for a in c["gravity"]), int(c["substeps"]), float(c["tol"]), int(c["max_iter"]))
+    o.set_fixed(c["fixed"])
+    o.set_state(c["x_init"].astype(np.float64), c["v_init"].astype(np.float64))
+    o.set_tet_active(c["active"])
+    for s in range(len(c["x"])):
+        assert o.step(float(c["dt"])) == 0
+        x, v = o.state()
+        it, res, conv = o.cg()
+        assert it == c["cg_iterations"][s]
+        assert np.float32(res) == c["cg_residual"][s]
+        assert conv == bool(c["cg_converged"][s])
+        assert np.array_equal(x.astype(np.float32), c["x"][s]), f"{name}: positions differ at step {s}"
+        assert np.array_equal(v.astype(np.float32), c["v"][s]), f"{name}: velocities differ at step {s}"
