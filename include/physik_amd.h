@@ -1,0 +1,175 @@
+/*
+ * physik_amd.h - the device-level C ABI of the MI355X implicit-Euler step.
+ *
+ * This layer has no single counterpart in the reference: it is the thin
+ * C ABI between the PHYSIK_* host API (PhysiKAPI.h in this directory) and the
+ * HIP kernels. It replaces, as one GPU-resident object, the per-frame work the
+ * reference does on the host in
+ *   World::Step substep loop ............ src/Core/World/World.cpp:32-73
+ *   TetMeshPhysicsComponent::UpdateSystem  src/Components/TetMeshPhysicsComponent.cpp:293-354
+ *   FEMModel corotational element ....... src/Core/Physics/FEM/FEMModel.cpp:614-679
+ *   SolverData implicit assembly ........ src/Core/Solvers/SolverData.cpp:126-465
+ *   block-Jacobi PCG .................... src/Core/Solvers/Linear/ConjugateGradientSolver.cpp:114-241
+ *   IntegrateImplicitEuler .............. src/Core/World/World.cpp:644-695
+ *
+ * Conventions: opaque handles, int status (PK_OK = 0), no exceptions across
+ * the ABI, caller-owned host buffers, system-owned device buffers. Vectors are
+ * node-major xyz triples; 3x3 blocks are row-major. All entry points are
+ * synchronous with respect to the host unless stated otherwise.
+ *
+ * A system may hold several independent bodies ("groups"): each group gets
+ * its own CG solve with its own convergence test, all advanced by the same
+ * kernel launches. One group reproduces the reference's single global solve
+ * over every component of a World.
+ */
+#ifndef PHYSIK_AMD_H
+#define PHYSIK_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#pragma GCC visibility push(default)
+
+typedef struct pk_system pk_system;
+typedef struct pk_pattern pk_pattern;
+
+enum pk_status {
+    PK_OK = 0,
+    PK_ERR_ARG = 1,       /* invalid argument or size */
+    PK_ERR_HIP = 2,       /* a HIP runtime call failed (message in pk_last_error) */
+    PK_ERR_MASS = 3,      /* a free active node has no mass: the reference throws
+                             std::runtime_error here (World.cpp:560-580) */
+    PK_ERR_NO_DEVICE = 4  /* no GPU visible */
+};
+
+enum pk_precision {
+    PK_FP64 = 0,          /* everything in FP64 */
+    PK_FP32_STORAGE = 1   /* A, M^-1 and the SpMV vectors stored in FP32; dots,
+                             residual update, solution and state in FP64 */
+};
+
+/* Per-group step outcome (pk_system_group_results). */
+enum pk_group_status {
+    PK_GROUP_INTEGRATED = 0,     /* solve usable, state advanced */
+    PK_GROUP_SOLVE_FAILED = 1,   /* preconditioner or inputs rejected: state kept
+                                    (SolverData.cpp:169-212, World.cpp:56-62) */
+    PK_GROUP_NONFINITE = 2,      /* integration produced a non-finite value: state
+                                    kept (World.cpp:667-677) */
+    PK_GROUP_NO_DYNAMIC = 3      /* no dynamic node in the group */
+};
+
+/* Kernel ids for pk_system_kernel_time. */
+enum pk_kernel {
+    PK_K_ELEMENT = 0,     /* per-tet corotation, forces, R Ke R^T */
+    PK_K_ASSEMBLE = 1,    /* atomic-free gather into BSR slots + block-Jacobi inverse */
+    PK_K_RHS = 2,         /* right-hand side gather + CG start */
+    PK_K_SPMV = 3,        /* q = A d fused with d.q */
+    PK_K_UPDATE = 4,      /* x, r update, z = M^-1 r, r.z, r.r */
+    PK_K_DIRECTION = 5,   /* d = z + beta d */
+    PK_K_INTEGRATE = 6,   /* v += dv, x += dt v (check + commit) */
+    PK_K_COUNT = 7
+};
+
+typedef struct pk_mesh_desc {
+    int node_count;
+    const double* rest;              /* node_count x 3 rest positions */
+    int tet_count;
+    const int* tets;                 /* tet_count x 4 node ids */
+    const double* young;             /* tet_count */
+    const double* poisson;           /* tet_count */
+    const double* damping;           /* tet_count */
+    const double* density;           /* tet_count (the owning component's density) */
+    const unsigned char* tet_active; /* tet_count, or NULL for all active */
+    const unsigned char* node_fixed; /* node_count, or NULL for none fixed */
+    const unsigned char* node_active;/* node_count, or NULL for all active */
+    const int* node_group;           /* node_count non-decreasing group ids starting at 0,
+                                        or NULL for a single group */
+} pk_mesh_desc;
+
+/* Message of the last failed call on this thread. */
+const char* pk_last_error(void);
+
+/* Number of visible GPUs (0 when none; never fails). */
+int pk_device_count(void);
+
+/* Uploads a mesh and builds everything that only depends on topology and
+ * material: rest data, lumped masses, dynamic-node numbering, the BSR pattern
+ * and the element->slot gather lists. Positions start at rest, velocities at
+ * zero. Runs on the current HIP device. */
+int pk_system_create(const pk_mesh_desc* desc, int precision, pk_system** out);
+void pk_system_destroy(pk_system* sys);
+
+int pk_system_sizes(const pk_system* sys, int* nodes, int* tets, int* groups, int* dynamic_rows,
+                    long long* nnzb_upper, long long* nnzb_full);
+
+/* Node state, node_count x 3 each; either pointer may be NULL. */
+int pk_system_set_state(pk_system* sys, const double* x, const double* v);
+int pk_system_get_state(pk_system* sys, double* x, double* v);
+
+int pk_system_set_gravity(pk_system* sys, double gx, double gy, double gz);
+/* CG settings, applied to every group (ConjugateGradientSettings,
+ * ConjugateGradientSolver.h:12-17). No clamping happens here. */
+int pk_system_set_solver(pk_system* sys, double tolerance, int max_iterations, int use_jacobi);
+/* 1 = implicit Euler (default), 0 = explicit Euler (World.cpp:697-725). */
+int pk_system_set_mode(pk_system* sys, int implicit);
+/* Per-tet Young's modulus (tet_count values); topology is unchanged. */
+int pk_system_set_young(pk_system* sys, const double* young);
+
+/* Advances frames x substeps substeps of length dt / substeps each, entirely
+ * on the GPU (World::Step, World.cpp:32-73). dt is the frame length. Returns
+ * PK_OK, or PK_ERR_MASS before any work when a free node has no mass. */
+int pk_system_step(pk_system* sys, double dt, int substeps, int frames);
+
+/* Outcome of the last substep, one entry per group (any pointer may be NULL). */
+int pk_system_group_results(pk_system* sys, int* iterations, double* residual_norm, int* converged,
+                            int* status);
+
+/* Kernel timing with HIP events on the system's stream: enable, reset, read.
+ * Only launches that did work are counted (CG launches after a group has
+ * converged are excluded). */
+int pk_system_profile(pk_system* sys, int enable);
+int pk_system_kernel_time(pk_system* sys, int kernel, long long* launches, double* total_ms);
+
+/* ---- inspection hooks used by the parity tests ---- */
+
+/* Runs the element and assembly kernels of one substep of length dt (no
+ * solve, no state change). */
+int pk_system_assemble(pk_system* sys, double dt);
+/* Assembled system after pk_system_assemble or a step: full BSR values
+ * (nnzb_full x 9, row-major blocks, in pattern order), rhs (dynamic_rows x 3)
+ * and block-Jacobi inverses (dynamic_rows x 9). Any pointer may be NULL. */
+int pk_system_get_matrix(pk_system* sys, double* full_vals, double* rhs, double* minv);
+/* Per-tet element record of the last assembly: the 10 blocks R K_ab R^T for
+ * a <= b in order (0,0),(0,1),(0,2),(0,3),(1,1),(1,2),(1,3),(2,2),(2,3),(3,3)
+ * (tet_count x 90), the element forces (tet_count x 12) and K v per corner
+ * (tet_count x 12). Any pointer may be NULL. */
+int pk_system_get_elements(pk_system* sys, double* blocks, double* forces, double* kv);
+/* y = A x with the assembled full BSR (dynamic_rows x 3). */
+int pk_system_spmv(pk_system* sys, const double* x, double* y);
+/* Solution dv of the last solve (dynamic_rows x 3). */
+int pk_system_get_solution(pk_system* sys, double* dv);
+
+/* ---- host-only topology (no GPU needed) ---- */
+
+/* Builds the dynamic numbering and the BSR patterns exactly as pk_system_create
+ * does, on the host. */
+int pk_pattern_create(const pk_mesh_desc* desc, pk_pattern** out);
+void pk_pattern_destroy(pk_pattern* p);
+int pk_pattern_sizes(const pk_pattern* p, int* dynamic_rows, long long* nnzb_upper, long long* nnzb_full,
+                     long long* contributions);
+/* dyn: node_count (dynamic row or -1); upper_start/upper_col: symmetric-upper
+ * pattern (the reference's SolverData matrix, SparseBlockMatrix.cpp:83-129);
+ * full_start/full_col: both triangles; scatter: tet_count x 16 upper slot of
+ * each local pair (a, b) or -1 (bit-identical to FindBlockIndex,
+ * SparseBlockMatrix.cpp:270-282). Any pointer may be NULL. */
+int pk_pattern_copy(const pk_pattern* p, int* dyn, int* upper_start, int* upper_col, int* full_start,
+                    int* full_col, int* scatter);
+
+#pragma GCC visibility pop
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHYSIK_AMD_H */

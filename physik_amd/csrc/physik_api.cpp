@@ -1,0 +1,858 @@
+// PHYSIK_* drop-in host API over the GPU system (physik_amd.h).
+//
+// World bookkeeping (nodes, components, handles, settings) lives on the host,
+// as in the reference (src/Core/World/World.cpp, src/API/PhysiKAPI.cpp); the
+// implicit step itself runs on the GPU. The device copy of the system is
+// rebuilt when topology, flags or materials change, and node state moves
+// between host and device only when one side has changed it.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "PhysiKAPI.h"
+#include "physik_amd.h"
+
+namespace physik_amd
+{
+
+thread_local std::string g_apiError;
+
+// Material defaults of include/PhysiK/PhysicsData/Material.h:5-11.
+struct Material
+{
+    float density = 1.0f;
+    float youngModulus = 25.0f;
+    float poissonRatio = 0.3f;
+    float damping = 0.25f;
+};
+
+// TetMeshGenerator output (src/Geometry/TetMeshGenerator.cpp:153-282).
+struct GeneratedMesh
+{
+    std::vector<double> positions;  // welded nodes, xyz
+    std::vector<int> tets;          // 4 per tet, local ids
+};
+
+std::unordered_set<GeneratedMesh*> g_meshes;  // PhysiKAPI.cpp:24
+
+// Node welding within 1e-5 on a hash grid of that cell size, then removal of
+// tets with invalid, duplicate or nearly coplanar nodes ((6V)^2 <= 1e-24).
+// S is the arithmetic type of the checks: float for the reference ABI,
+// double for the FP64 variant.
+template <class S>
+GeneratedMesh* generate(const S* pos, int nodeCount, const int* tets, int tetCount)
+{
+    auto* m = new GeneratedMesh();
+    if (pos == nullptr || nodeCount <= 0)
+        return m;
+    const S tol = static_cast<S>(1.0e-5f);
+    const S tol2 = tol * tol;
+    struct Key
+    {
+        int x, y, z;
+        bool operator==(const Key& o) const { return x == o.x && y == o.y && z == o.z; }
+    };
+    struct KeyHash
+    {
+        size_t operator()(const Key& k) const
+        {
+            return std::hash<int>()(k.x) ^ (std::hash<int>()(k.y) << 1u) ^ (std::hash<int>()(k.z) << 2u);
+        }
+    };
+    std::unordered_map<Key, std::vector<int>, KeyHash> grid;
+    std::vector<S> kept;
+    std::vector<int> remap(static_cast<size_t>(nodeCount), -1);
+    auto cell = [&](const S* p) {
+        return Key{static_cast<int>(std::floor(p[0] / tol)), static_cast<int>(std::floor(p[1] / tol)),
+                   static_cast<int>(std::floor(p[2] / tol))};
+    };
+    for (int i = 0; i < nodeCount; ++i)
+    {
+        const S* p = pos + 3 * i;
+        const Key c = cell(p);
+        int found = -1;
+        for (int dz = -1; dz <= 1 && found < 0; ++dz)
+            for (int dy = -1; dy <= 1 && found < 0; ++dy)
+                for (int dx = -1; dx <= 1 && found < 0; ++dx)
+                {
+                    auto it = grid.find(Key{c.x + dx, c.y + dy, c.z + dz});
+                    if (it == grid.end())
+                        continue;
+                    for (int nn : it->second)
+                    {
+                        const S ex = kept[3 * nn] - p[0], ey = kept[3 * nn + 1] - p[1], ez = kept[3 * nn + 2] - p[2];
+                        if (ex * ex + ey * ey + ez * ez <= tol2)
+                        {
+                            found = nn;
+                            break;
+                        }
+                    }
+                }
+        if (found >= 0)
+        {
+            remap[i] = found;
+            continue;
+        }
+        const int nn = static_cast<int>(kept.size() / 3);
+        kept.push_back(p[0]);
+        kept.push_back(p[1]);
+        kept.push_back(p[2]);
+        remap[i] = nn;
+        grid[c].push_back(nn);
+    }
+    m->positions.assign(kept.begin(), kept.end());
+    if (tets && tetCount > 0)
+        for (int t = 0; t < tetCount; ++t)
+        {
+            const int* r = tets + 4 * t;
+            bool bad = false;
+            for (int a = 0; a < 4; ++a)
+                bad = bad || r[a] < 0 || r[a] >= nodeCount;
+            if (bad)
+                continue;
+            const int a = remap[r[0]], b = remap[r[1]], c = remap[r[2]], d = remap[r[3]];
+            if (a == b || a == c || a == d || b == c || b == d || c == d)
+                continue;
+            const S* pa = kept.data() + 3 * a;
+            const S* pb = kept.data() + 3 * b;
+            const S* pc = kept.data() + 3 * c;
+            const S* pd = kept.data() + 3 * d;
+            const S ab[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+            const S ac[3] = {pc[0] - pa[0], pc[1] - pa[1], pc[2] - pa[2]};
+            const S ad[3] = {pd[0] - pa[0], pd[1] - pa[1], pd[2] - pa[2]};
+            const S cx = ab[1] * ac[2] - ab[2] * ac[1], cy = ab[2] * ac[0] - ab[0] * ac[2],
+                    cz = ab[0] * ac[1] - ab[1] * ac[0];
+            const S six = cx * ad[0] + cy * ad[1] + cz * ad[2];
+            if (six * six <= static_cast<S>(1.0e-24f))
+                continue;
+            m->tets.push_back(a);
+            m->tets.push_back(b);
+            m->tets.push_back(c);
+            m->tets.push_back(d);
+        }
+    return m;
+}
+
+struct Component
+{
+    bool physics = false;
+    bool active = true;
+    int globalBegin = -1;
+    int globalCount = 0;
+    std::vector<double> rest;   // local rest positions
+    std::vector<int> tets;      // local ids
+    std::vector<uint8_t> tetActive;
+    std::vector<double> young;  // per tet
+    Material material;
+};
+
+struct World
+{
+    std::vector<double> x, v;
+    std::vector<uint8_t> fixed, nodeActive;
+    std::vector<std::unique_ptr<Component>> components;
+    std::vector<uint32_t> generations;
+    std::vector<uint32_t> freeSlots;
+
+    float gravity[3] = {0.f, 0.f, 0.f};
+    int substeps = 1;
+    bool implicit = false;  // SolverMode::Explicit by default (World.h:133)
+    float tolerance = 1.0e-4f;
+    int maxIterations = 128;
+    int precision = PK_FP64;
+    int lastIterations = 0;
+    float lastResidual = 0.f;
+    bool lastConverged = false;
+    int lastStatus = PK_GROUP_NO_DYNAMIC;
+
+    pk_system* sys = nullptr;
+    bool topoDirty = true;     // rebuild the device system before the next step
+    bool hostNewer = false;    // host state changed since the last upload
+    bool deviceNewer = false;  // device state changed since the last download
+
+    ~World()
+    {
+        if (sys)
+            pk_system_destroy(sys);
+    }
+    int nodeCount() const { return static_cast<int>(fixed.size()); }
+
+    void pull()
+    {
+        if (!deviceNewer || !sys)
+            return;
+        if (pk_system_get_state(sys, x.data(), v.data()) == PK_OK)
+            deviceNewer = false;
+    }
+    void touch_host()
+    {
+        pull();
+        hostNewer = true;
+    }
+    int add_node(double px, double py, double pz)
+    {
+        touch_host();
+        x.push_back(px);
+        x.push_back(py);
+        x.push_back(pz);
+        v.push_back(0.0);
+        v.push_back(0.0);
+        v.push_back(0.0);
+        fixed.push_back(0);
+        nodeActive.push_back(1);
+        topoDirty = true;
+        return nodeCount() - 1;
+    }
+    Component* get(PhysiK::ComponentHandle h)
+    {
+        if (!h.IsValid() || h.index >= components.size())
+            return nullptr;
+        if (generations[h.index] != h.generation || !components[h.index])
+            return nullptr;
+        return components[h.index].get();
+    }
+    PhysiK::ComponentHandle add(std::unique_ptr<Component> c)  // World.cpp:448-481
+    {
+        if (!freeSlots.empty())
+        {
+            const uint32_t slot = freeSlots.back();
+            freeSlots.pop_back();
+            components[slot] = std::move(c);
+            return PhysiK::ComponentHandle{slot, generations[slot]};
+        }
+        components.push_back(std::move(c));
+        generations.push_back(1u);
+        return PhysiK::ComponentHandle{static_cast<uint32_t>(components.size() - 1), 1u};
+    }
+    void destroy(PhysiK::ComponentHandle h)  // World.cpp:105-124
+    {
+        if (!get(h))
+            return;
+        components[h.index].reset();
+        ++generations[h.index];
+        if (generations[h.index] == 0u)
+            generations[h.index] = 1u;
+        freeSlots.push_back(h.index);
+        topoDirty = true;
+    }
+
+    // Rebuilds the device system from every active physics component.
+    int rebuild()
+    {
+        pull();
+        const int N = nodeCount();
+        std::vector<int> tets;
+        std::vector<double> young, poisson, damping, density;
+        std::vector<uint8_t> tact;
+        for (const auto& cp : components)
+        {
+            if (!cp || !cp->physics || !cp->active)
+                continue;
+            const Component& c = *cp;
+            for (size_t t = 0; t < c.tetActive.size(); ++t)
+            {
+                for (int a = 0; a < 4; ++a)
+                    tets.push_back(c.globalBegin + c.tets[4 * t + a]);
+                young.push_back(c.young[t]);
+                poisson.push_back(c.material.poissonRatio);
+                damping.push_back(c.material.damping);
+                density.push_back(c.material.density);
+                tact.push_back(c.tetActive[t]);
+            }
+        }
+        pk_mesh_desc d{};
+        d.node_count = N;
+        d.rest = x.data();  // rest data is per tet; the system's x is replaced below
+        d.tet_count = static_cast<int>(young.size());
+        d.tets = tets.data();
+        d.young = young.data();
+        d.poisson = poisson.data();
+        d.damping = damping.data();
+        d.density = density.data();
+        d.tet_active = tact.data();
+        d.node_fixed = fixed.data();
+        d.node_active = nodeActive.data();
+        // Rest geometry comes from each component's own rest positions.
+        std::vector<double> rest(x);
+        for (const auto& cp : components)
+            if (cp && cp->physics && cp->active)
+                std::copy(cp->rest.begin(), cp->rest.end(), rest.begin() + 3 * cp->globalBegin);
+        d.rest = rest.data();
+        if (sys)
+        {
+            pk_system_destroy(sys);
+            sys = nullptr;
+        }
+        int rc = pk_system_create(&d, precision, &sys);
+        if (rc != PK_OK)
+            return rc;
+        rc = pk_system_set_state(sys, x.data(), v.data());
+        if (rc != PK_OK)
+            return rc;
+        topoDirty = false;
+        hostNewer = false;
+        deviceNewer = false;
+        return PK_OK;
+    }
+
+    int step(float frameDt)  // World.cpp:32-73
+    {
+        if (frameDt < 0.0f || frameDt == 0.0f)
+            return PK_OK;
+        if (topoDirty || !sys)
+        {
+            const int rc = rebuild();
+            if (rc != PK_OK)
+                return rc;
+        }
+        if (hostNewer)
+        {
+            const int rc = pk_system_set_state(sys, x.data(), v.data());
+            if (rc != PK_OK)
+                return rc;
+            hostNewer = false;
+        }
+        const int steps = std::max(1, substeps);
+        pk_system_set_gravity(sys, gravity[0], gravity[1], gravity[2]);
+        pk_system_set_mode(sys, implicit ? 1 : 0);
+        int rc = pk_system_set_solver(sys, tolerance, maxIterations, 1);
+        if (rc != PK_OK)
+            return rc;
+        rc = pk_system_step(sys, static_cast<double>(frameDt), steps, 1);
+        if (rc != PK_OK)
+            return rc;
+        deviceNewer = true;
+        if (implicit)
+        {
+            int it = 0, conv = 0, status = 0;
+            double res = 0.0;
+            pk_system_group_results(sys, &it, &res, &conv, &status);
+            lastIterations = it;
+            lastResidual = static_cast<float>(res);
+            lastConverged = conv != 0;
+            lastStatus = status;
+        }
+        return PK_OK;
+    }
+};
+
+World* W(PhysiK::WorldHandle h) { return static_cast<World*>(h); }
+
+GeneratedMesh* M(PhysiK::GeneratedTetMeshHandle h)
+{
+    auto* m = static_cast<GeneratedMesh*>(h);
+    return (m && g_meshes.count(m)) ? m : nullptr;
+}
+
+bool valid_node(World* w, int i) { return w && i >= 0 && i < w->nodeCount(); }
+
+Component* tet_mesh(World* w, PhysiK::ComponentHandle h) { return w ? w->get(h) : nullptr; }
+
+Component* physics(World* w, PhysiK::ComponentHandle h)
+{
+    Component* c = tet_mesh(w, h);
+    return (c && c->physics) ? c : nullptr;
+}
+
+int global_of(const Component& c, int local)  // TetMeshPhysicsComponent.cpp:264-274
+{
+    if (!c.physics || c.globalBegin < 0 || local < 0 || local >= c.globalCount)
+        return -1;
+    return c.globalBegin + local;
+}
+
+// PhysiKAPI.cpp:159-229: a node stays active while an active tet of an active
+// tet-mesh component references it; a deactivated node loses its velocity.
+void reconcile(World* w, const Component& c, int tet)
+{
+    for (int a = 0; a < 4; ++a)
+    {
+        const int g = global_of(c, c.tets[4 * tet + a]);
+        if (!valid_node(w, g))
+            continue;
+        bool referenced = false;
+        for (const auto& cp : w->components)
+        {
+            if (!cp || !cp->active || referenced)
+                continue;
+            for (size_t t = 0; t < cp->tetActive.size() && !referenced; ++t)
+            {
+                if (!cp->tetActive[t])
+                    continue;
+                for (int b = 0; b < 4; ++b)
+                    if (global_of(*cp, cp->tets[4 * t + b]) == g)
+                        referenced = true;
+            }
+        }
+        w->nodeActive[g] = referenced ? 1 : 0;
+        if (!referenced)
+            for (int i = 0; i < 3; ++i)
+                w->v[3 * g + i] = 0.0;
+    }
+}
+
+std::unique_ptr<Component> component_from(const GeneratedMesh& m)  // TetMeshComponent.cpp:31-67
+{
+    auto c = std::make_unique<Component>();
+    c->rest = m.positions;
+    const int n = static_cast<int>(m.positions.size() / 3);
+    for (size_t t = 0; t + 3 < m.tets.size(); t += 4)
+    {
+        bool ok = true;
+        for (int a = 0; a < 4; ++a)
+            ok = ok && m.tets[t + a] >= 0 && m.tets[t + a] < n;
+        if (!ok)
+            continue;
+        c->tets.insert(c->tets.end(), m.tets.begin() + t, m.tets.begin() + t + 4);
+    }
+    c->tetActive.assign(c->tets.size() / 4, 1);
+    return c;
+}
+
+void apply_material(Component& c, const Material& m)
+{
+    c.material = m;
+    c.young.assign(c.tetActive.size(), m.youngModulus);
+}
+
+Material to_material(const PhysikMaterialDesc& d)  // PhysiKAPI.cpp:50-58
+{
+    Material m;
+    m.density = d.density;
+    m.youngModulus = d.youngModulus;
+    m.poissonRatio = d.poissonRatio;
+    m.damping = d.damping;
+    return m;
+}
+
+}  // namespace physik_amd
+
+using namespace physik_amd;
+
+extern "C"
+{
+
+PhysiK::WorldHandle PHYSIK_CreateWorld() { return new World(); }
+
+void PHYSIK_DestroyWorld(PhysiK::WorldHandle world) { delete W(world); }
+
+void PHYSIK_Step(PhysiK::WorldHandle world, float dt)
+{
+    World* w = W(world);
+    if (!w)
+        return;
+    const int rc = w->step(dt);
+    if (rc == PK_ERR_MASS)
+        throw std::runtime_error(pk_last_error());
+    if (rc != PK_OK)
+        throw std::runtime_error(std::string("PHYSIK_Step: ") + pk_last_error());
+}
+
+int PHYSIK_AMD_TryStep(PhysiK::WorldHandle world, float dt)
+{
+    World* w = W(world);
+    if (!w)
+        return 1;
+    const int rc = w->step(dt);
+    if (rc == PK_OK)
+        return 0;
+    g_apiError = pk_last_error();
+    return rc == PK_ERR_MASS ? 3 : 2;
+}
+
+void PHYSIK_SetSubstepCount(PhysiK::WorldHandle world, int substepCount)
+{
+    if (World* w = W(world))
+        w->substeps = std::max(1, substepCount);
+}
+
+void PHYSIK_SetSolverMode(PhysiK::WorldHandle world, int mode)
+{
+    if (World* w = W(world))
+        w->implicit = mode == 1;
+}
+
+void PHYSIK_SetConjugateGradientTolerance(PhysiK::WorldHandle world, float tolerance)
+{
+    World* w = W(world);
+    if (!w || !std::isfinite(tolerance))
+        return;
+    w->tolerance = std::clamp(tolerance, 1.0e-8f, 1.0e-1f);  // World.cpp:177-188
+}
+
+float PHYSIK_GetConjugateGradientTolerance(PhysiK::WorldHandle world)
+{
+    World* w = W(world);
+    return w ? w->tolerance : 0.0f;
+}
+
+void PHYSIK_SetConjugateGradientMaxIterations(PhysiK::WorldHandle world, int maxIterations)
+{
+    if (World* w = W(world))
+        w->maxIterations = std::clamp(maxIterations, 1, 1024);  // World.cpp:195-201
+}
+
+int PHYSIK_GetConjugateGradientMaxIterations(PhysiK::WorldHandle world)
+{
+    World* w = W(world);
+    return w ? w->maxIterations : 0;
+}
+
+int PHYSIK_GetLastConjugateGradientIterations(PhysiK::WorldHandle world)
+{
+    World* w = W(world);
+    return w ? w->lastIterations : 0;
+}
+
+float PHYSIK_GetLastConjugateGradientResidualNorm(PhysiK::WorldHandle world)
+{
+    World* w = W(world);
+    return w ? w->lastResidual : 0.0f;
+}
+
+int PHYSIK_DidLastConjugateGradientSolveConverge(PhysiK::WorldHandle world)
+{
+    World* w = W(world);
+    return (w && w->lastConverged) ? 1 : 0;
+}
+
+void PHYSIK_SetGravity(PhysiK::WorldHandle world, float x, float y, float z)
+{
+    if (World* w = W(world))
+    {
+        w->gravity[0] = x;
+        w->gravity[1] = y;
+        w->gravity[2] = z;
+    }
+}
+
+int PHYSIK_AddNode(PhysiK::WorldHandle world, float x, float y, float z)
+{
+    World* w = W(world);
+    return w ? w->add_node(x, y, z) : -1;
+}
+
+void PHYSIK_SetNodeFixed(PhysiK::WorldHandle world, int nodeIndex, int fixed)  // World.cpp:403-417
+{
+    World* w = W(world);
+    if (!valid_node(w, nodeIndex))
+        return;
+    w->touch_host();
+    const uint8_t f = fixed != 0 ? 1 : 0;
+    if (f)
+        for (int i = 0; i < 3; ++i)
+            w->v[3 * nodeIndex + i] = 0.0;
+    if (w->fixed[nodeIndex] != f)
+        w->topoDirty = true;
+    w->fixed[nodeIndex] = f;
+}
+
+int PHYSIK_IsNodeFixed(PhysiK::WorldHandle world, int nodeIndex)
+{
+    World* w = W(world);
+    return (valid_node(w, nodeIndex) && w->fixed[nodeIndex]) ? 1 : 0;
+}
+
+PhysiK::GeneratedTetMeshHandle PHYSIK_GenerateTetMesh(const PhysiK::Vec3* positions, int nodeCount,
+                                                      const int* tetLocalNodeIndices, int tetCount)
+{
+    GeneratedMesh* m = generate<float>(reinterpret_cast<const float*>(positions), nodeCount, tetLocalNodeIndices,
+                                       tetCount);
+    g_meshes.insert(m);
+    return m;
+}
+
+PhysiK::GeneratedTetMeshHandle PHYSIK_AMD_GenerateTetMeshF64(const double* positions, int nodeCount,
+                                                             const int* tetLocalNodeIndices, int tetCount)
+{
+    GeneratedMesh* m = generate<double>(positions, nodeCount, tetLocalNodeIndices, tetCount);
+    g_meshes.insert(m);
+    return m;
+}
+
+int PHYSIK_IsGeneratedTetMeshHandleValid(PhysiK::GeneratedTetMeshHandle h) { return M(h) ? 1 : 0; }
+
+void PHYSIK_DestroyGeneratedTetMesh(PhysiK::GeneratedTetMeshHandle h)
+{
+    GeneratedMesh* m = M(h);
+    if (!m)
+        return;
+    g_meshes.erase(m);
+    delete m;
+}
+
+int PHYSIK_GetGeneratedTetMeshVertexCount(PhysiK::GeneratedTetMeshHandle h)
+{
+    GeneratedMesh* m = M(h);
+    return m ? static_cast<int>(m->positions.size() / 3) : 0;
+}
+
+int PHYSIK_GetGeneratedTetMeshTetCount(PhysiK::GeneratedTetMeshHandle h)
+{
+    GeneratedMesh* m = M(h);
+    return m ? static_cast<int>(m->tets.size() / 4) : 0;
+}
+
+int PHYSIK_GetGeneratedTetMeshTetIndexCount(PhysiK::GeneratedTetMeshHandle h)
+{
+    GeneratedMesh* m = M(h);
+    return m ? static_cast<int>(m->tets.size()) : 0;
+}
+
+int PHYSIK_GetGeneratedTetMeshVertex(PhysiK::GeneratedTetMeshHandle h, int i, float* outX, float* outY, float* outZ)
+{
+    GeneratedMesh* m = M(h);
+    if (!m || !outX || !outY || !outZ || i < 0 || i >= static_cast<int>(m->positions.size() / 3))
+        return 0;
+    *outX = static_cast<float>(m->positions[3 * i]);
+    *outY = static_cast<float>(m->positions[3 * i + 1]);
+    *outZ = static_cast<float>(m->positions[3 * i + 2]);
+    return 1;
+}
+
+int PHYSIK_GetGeneratedTetMeshTetNodeIndex(PhysiK::GeneratedTetMeshHandle h, int i, int* outNodeIndex)
+{
+    GeneratedMesh* m = M(h);
+    if (!m || !outNodeIndex || i < 0 || i >= static_cast<int>(m->tets.size()))
+        return 0;
+    *outNodeIndex = m->tets[i];
+    return 1;
+}
+
+PhysiK::ComponentHandle PHYSIK_CreateTetMeshComponent(PhysiK::WorldHandle world, PhysiK::GeneratedTetMeshHandle h)
+{
+    World* w = W(world);
+    GeneratedMesh* m = M(h);
+    if (!w || !m || m->positions.empty() || m->tets.empty())
+        return PhysiK::ComponentHandle{};
+    return w->add(component_from(*m));
+}
+
+// TetMeshPhysicsComponent::CreateFromGeneratedTetMesh (TetMeshPhysicsComponent.cpp:163-187)
+PhysiK::ComponentHandle PHYSIK_CreateTetMeshPhysicsComponent(PhysiK::WorldHandle world,
+                                                             PhysiK::GeneratedTetMeshHandle h,
+                                                             const PhysikMaterialDesc* material)
+{
+    World* w = W(world);
+    GeneratedMesh* m = M(h);
+    if (!w || !m || !material || m->positions.empty() || m->tets.empty())
+        return PhysiK::ComponentHandle{};
+    auto c = component_from(*m);
+    c->physics = true;
+    apply_material(*c, to_material(*material));
+    c->globalBegin = w->nodeCount();
+    c->globalCount = static_cast<int>(c->rest.size() / 3);
+    for (int i = 0; i < c->globalCount; ++i)
+        w->add_node(c->rest[3 * i], c->rest[3 * i + 1], c->rest[3 * i + 2]);
+    w->topoDirty = true;
+    return w->add(std::move(c));
+}
+
+int PHYSIK_GetTetMeshGlobalNodeBeginIndex(PhysiK::WorldHandle world, PhysiK::ComponentHandle h)
+{
+    Component* c = physics(W(world), h);
+    return c ? c->globalBegin : -1;
+}
+
+int PHYSIK_GetTetMeshGlobalNodeCount(PhysiK::WorldHandle world, PhysiK::ComponentHandle h)
+{
+    Component* c = physics(W(world), h);
+    return c ? c->globalCount : -1;
+}
+
+int PHYSIK_GetTetMeshGlobalNodeIndex(PhysiK::WorldHandle world, PhysiK::ComponentHandle h, int local)
+{
+    Component* c = physics(W(world), h);
+    return c ? global_of(*c, local) : -1;
+}
+
+void PHYSIK_SetTetMeshMaterial(PhysiK::WorldHandle world, PhysiK::ComponentHandle h,
+                               const PhysikMaterialDesc* material)
+{
+    World* w = W(world);
+    if (!w || !material)
+        return;
+    Component* c = physics(w, h);
+    if (!c)
+        return;
+    apply_material(*c, to_material(*material));
+    w->topoDirty = true;
+}
+
+int PHYSIK_AMD_SetTetMeshYoungModuli(PhysiK::WorldHandle world, PhysiK::ComponentHandle h, const double* young,
+                                     int count)
+{
+    World* w = W(world);
+    Component* c = physics(w, h);
+    if (!c || !young || count != static_cast<int>(c->young.size()))
+        return 0;
+    c->young.assign(young, young + count);
+    w->topoDirty = true;
+    return 1;
+}
+
+void PHYSIK_DestroyComponent(PhysiK::WorldHandle world, PhysiK::ComponentHandle h)
+{
+    if (World* w = W(world))
+        w->destroy(h);
+}
+
+int PHYSIK_IsComponentHandleValid(PhysiK::WorldHandle world, PhysiK::ComponentHandle h)
+{
+    World* w = W(world);
+    return (w && w->get(h)) ? 1 : 0;
+}
+
+int PHYSIK_GetTetMeshTetCount(PhysiK::WorldHandle world, PhysiK::ComponentHandle h)
+{
+    Component* c = tet_mesh(W(world), h);
+    return c ? static_cast<int>(c->tetActive.size()) : 0;
+}
+
+int PHYSIK_GetTetMeshNodeCount(PhysiK::WorldHandle world, PhysiK::ComponentHandle h)
+{
+    Component* c = tet_mesh(W(world), h);
+    return c ? static_cast<int>(c->rest.size() / 3) : 0;
+}
+
+int PHYSIK_IsTetActive(PhysiK::WorldHandle world, PhysiK::ComponentHandle h, int tet)
+{
+    Component* c = tet_mesh(W(world), h);
+    if (!c || tet < 0 || tet >= static_cast<int>(c->tetActive.size()))
+        return 0;
+    return c->tetActive[tet] ? 1 : 0;
+}
+
+void PHYSIK_SetTetActive(PhysiK::WorldHandle world, PhysiK::ComponentHandle h, int tet, int active)
+{
+    World* w = W(world);
+    Component* c = tet_mesh(w, h);
+    if (!c || tet < 0 || tet >= static_cast<int>(c->tetActive.size()))
+        return;
+    const uint8_t a = active != 0 ? 1 : 0;
+    if (c->tetActive[tet] == a)
+        return;
+    w->touch_host();
+    c->tetActive[tet] = a;
+    reconcile(w, *c, tet);
+    w->topoDirty = true;
+}
+
+void PHYSIK_DeactivateTet(PhysiK::WorldHandle world, PhysiK::ComponentHandle h, int tet)
+{
+    PHYSIK_SetTetActive(world, h, tet, 0);
+}
+
+int PHYSIK_GetActiveTetCount(PhysiK::WorldHandle world, PhysiK::ComponentHandle h)
+{
+    Component* c = tet_mesh(W(world), h);
+    if (!c)
+        return 0;
+    return static_cast<int>(std::count(c->tetActive.begin(), c->tetActive.end(), 1));
+}
+
+void PHYSIK_GetNodePosition(PhysiK::WorldHandle world, int i, float* outX, float* outY, float* outZ)
+{
+    World* w = W(world);
+    if (!valid_node(w, i))
+        return;
+    w->pull();
+    if (outX) *outX = static_cast<float>(w->x[3 * i]);
+    if (outY) *outY = static_cast<float>(w->x[3 * i + 1]);
+    if (outZ) *outZ = static_cast<float>(w->x[3 * i + 2]);
+}
+
+void PHYSIK_GetNodeVelocity(PhysiK::WorldHandle world, int i, float* outX, float* outY, float* outZ)
+{
+    World* w = W(world);
+    if (!valid_node(w, i))
+        return;
+    w->pull();
+    if (outX) *outX = static_cast<float>(w->v[3 * i]);
+    if (outY) *outY = static_cast<float>(w->v[3 * i + 1]);
+    if (outZ) *outZ = static_cast<float>(w->v[3 * i + 2]);
+}
+
+void PHYSIK_SetNodePosition(PhysiK::WorldHandle world, int i, float x, float y, float z)  // World.cpp:394-401
+{
+    World* w = W(world);
+    if (!valid_node(w, i))
+        return;
+    w->touch_host();
+    w->x[3 * i] = x;
+    w->x[3 * i + 1] = y;
+    w->x[3 * i + 2] = z;
+    for (int k = 0; k < 3; ++k)
+        w->v[3 * i + k] = 0.0;
+}
+
+void PHYSIK_SetNodeVelocity(PhysiK::WorldHandle world, int i, float x, float y, float z)
+{
+    World* w = W(world);
+    if (!valid_node(w, i))
+        return;
+    w->touch_host();
+    w->v[3 * i] = x;
+    w->v[3 * i + 1] = y;
+    w->v[3 * i + 2] = z;
+}
+
+int PHYSIK_AMD_SetPrecision(PhysiK::WorldHandle world, int precision)
+{
+    World* w = W(world);
+    if (!w || (precision != PK_FP64 && precision != PK_FP32_STORAGE))
+        return 0;
+    if (w->precision != precision)
+        w->topoDirty = true;
+    w->precision = precision;
+    return 1;
+}
+
+int PHYSIK_AMD_GetLastStepStatus(PhysiK::WorldHandle world)
+{
+    World* w = W(world);
+    return w ? w->lastStatus : PK_GROUP_NO_DYNAMIC;
+}
+
+int PHYSIK_AMD_CopyNodeStateF64(PhysiK::WorldHandle world, double* outPositions, double* outVelocities, int count)
+{
+    World* w = W(world);
+    if (!w)
+        return 0;
+    w->pull();
+    const int n = std::min(count, w->nodeCount());
+    if (n <= 0)
+        return 0;
+    if (outPositions)
+        std::memcpy(outPositions, w->x.data(), sizeof(double) * 3 * n);
+    if (outVelocities)
+        std::memcpy(outVelocities, w->v.data(), sizeof(double) * 3 * n);
+    return n;
+}
+
+int PHYSIK_AMD_SetNodeStateF64(PhysiK::WorldHandle world, const double* positions, const double* velocities,
+                               int count)
+{
+    World* w = W(world);
+    if (!w)
+        return 0;
+    w->touch_host();
+    const int n = std::min(count, w->nodeCount());
+    if (n <= 0)
+        return 0;
+    if (positions)
+        std::memcpy(w->x.data(), positions, sizeof(double) * 3 * n);
+    if (velocities)
+        std::memcpy(w->v.data(), velocities, sizeof(double) * 3 * n);
+    return n;
+}
+
+const char* PHYSIK_AMD_GetLastError() { return g_apiError.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+"""Python handle on the GPU system (include/physik_amd.h).
+
+``System`` owns a pk_system: one mesh (or a batch of bodies, one CG group per
+body) resident on the GPU, stepped with pk_system_step. ``Pattern`` builds the
+same dynamic numbering and BSR patterns on the host only (no GPU needed).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native as N
+
+
+class _Desc:
+    """Keeps the numpy arrays behind a MeshDesc alive."""
+
+    def __init__(self, rest, tets, young, poisson, damping, density, tet_active=None, node_fixed=None,
+                 node_active=None, node_group=None):
+        self.rest = N.as_f64(rest).reshape(-1, 3)
+        self.tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+        t = self.tets.shape[0]
+
+        def per_tet(v):
+            return np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (t,)))
+
+        self.young, self.poisson = per_tet(young), per_tet(poisson)
+        self.damping, self.density = per_tet(damping), per_tet(density)
+        u8 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.uint8)  # noqa: E731
+        self.tet_active, self.node_fixed, self.node_active = u8(tet_active), u8(node_fixed), u8(node_active)
+        self.node_group = None if node_group is None else np.ascontiguousarray(node_group, dtype=np.int32)
+        self.c = N.MeshDesc(self.rest.shape[0], N.ptr(self.rest), t, N.ptr(self.tets), N.ptr(self.young),
+                            N.ptr(self.poisson), N.ptr(self.damping), N.ptr(self.density), N.ptr(self.tet_active),
+                            N.ptr(self.node_fixed), N.ptr(self.node_active), N.ptr(self.node_group))
+
+
+class Pattern:
+    def __init__(self, rest, tets, young=1.0, poisson=0.3, damping=0.0, density=1.0, **flags):
+        lib = N.lib()
+        d = _Desc(rest, tets, young, poisson, damping, density, **flags)
+        h = C.c_void_p()
+        N.check(lib.pk_pattern_create(C.byref(d.c), C.byref(h)), "pk_pattern_create")
+        try:
+            nd, nu, nf, nc = C.c_int(), C.c_longlong(), C.c_longlong(), C.c_longlong()
+            lib.pk_pattern_sizes(h, C.byref(nd), C.byref(nu), C.byref(nf), C.byref(nc))
+            self.nd, self.nnzb_upper, self.nnzb_full, self.contributions = nd.value, nu.value, nf.value, nc.value
+            n, t = d.rest.shape[0], d.tets.shape[0]
+            self.dyn = np.zeros(n, np.int32)
+            self.upper_start = np.zeros(self.nd + 1, np.int32)
+            self.upper_col = np.zeros(max(self.nnzb_upper, 1), np.int32)
+            self.full_start = np.zeros(self.nd + 1, np.int32)
+            self.full_col = np.zeros(max(self.nnzb_full, 1), np.int32)
+            self.scatter = np.zeros(max(t * 16, 1), np.int32)
+            lib.pk_pattern_copy(h, N.ptr(self.dyn), N.ptr(self.upper_start), N.ptr(self.upper_col),
+                                N.ptr(self.full_start), N.ptr(self.full_col), N.ptr(self.scatter))
+            self.upper_col = self.upper_col[:self.nnzb_upper]
+            self.full_col = self.full_col[:self.nnzb_full]
+            self.scatter = self.scatter[:t * 16].reshape(t, 4, 4)
+        finally:
+            lib.pk_pattern_destroy(h)
+
+
+class System:
+    """A mesh (or a batch of bodies) resident on the GPU."""
+
+    def __init__(self, rest, tets, young, poisson, damping, density, precision=N.PK_FP64, **flags):
+        self.lib = N.lib()
+        self._d = _Desc(rest, tets, young, poisson, damping, density, **flags)
+        h = C.c_void_p()
+        N.check(self.lib.pk_system_create(C.byref(self._d.c), precision, C.byref(h)), "pk_system_create")
+        self.h = h
+        self.precision = precision
+        n, t, g, nd = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        nu, nf = C.c_longlong(), C.c_longlong()
+        self.lib.pk_system_sizes(h, C.byref(n), C.byref(t), C.byref(g), C.byref(nd), C.byref(nu), C.byref(nf))
+        self.n, self.t, self.groups, self.nd = n.value, t.value, g.value, nd.value
+        self.nnzb_upper, self.nnzb_full = nu.value, nf.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pk_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, x=None, v=None):
+        xa = None if x is None else N.as_f64(x).reshape(-1)
+        va = None if v is None else N.as_f64(v).reshape(-1)
+        N.check(self.lib.pk_system_set_state(self.h, N.ptr(xa), N.ptr(va)), "pk_system_set_state")
+
+    def state(self):
+        x = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        N.check(self.lib.pk_system_get_state(self.h, N.ptr(x), N.ptr(v)), "pk_system_get_state")
+        return x, v
+
+    def set_gravity(self, g):
+        N.check(self.lib.pk_system_set_gravity(self.h, *[float(a) for a in g]), "pk_system_set_gravity")
+
+    def set_solver(self, tol, max_iter, jacobi=True):
+        N.check(self.lib.pk_system_set_solver(self.h, tol, max_iter, int(jacobi)), "pk_system_set_solver")
+
+    def set_mode(self, implicit=True):
+        N.check(self.lib.pk_system_set_mode(self.h, int(implicit)), "pk_system_set_mode")
+
+    def set_young(self, young):
+        y = np.ascontiguousarray(np.broadcast_to(np.asarray(young, np.float64), (self.t,)))
+        N.check(self.lib.pk_system_set_young(self.h, y), "pk_system_set_young")
+
+    def step(self, dt, substeps=1, frames=1):
+        rc = self.lib.pk_system_step(self.h, dt, substeps, frames)
+        if rc == N.PK_ERR_MASS:
+            raise RuntimeError(N.last_error())
+        N.check(rc, "pk_system_step")
+
+    def results(self):
+        g = self.groups
+        it = np.zeros(g, np.int32)
+        res = np.zeros(g)
+        conv = np.zeros(g, np.int32)
+        st = np.zeros(g, np.int32)
+        N.check(self.lib.pk_system_group_results(self.h, N.ptr(it), N.ptr(res), N.ptr(conv), N.ptr(st)),
+                "pk_system_group_results")
+        return it, res, conv.astype(bool), st
+
+    def profile(self, enable=True):
+        N.check(self.lib.pk_system_profile(self.h, int(enable)), "pk_system_profile")
+
+    def kernel_time(self, kernel):
+        n, ms = C.c_longlong(), C.c_double()
+        N.check(self.lib.pk_system_kernel_time(self.h, kernel, C.byref(n), C.byref(ms)), "pk_system_kernel_time")
+        return n.value, ms.value
+
+    def assemble(self, dt):
+        N.check(self.lib.pk_system_assemble(self.h, dt), "pk_system_assemble")
+
+    def matrix(self):
+        vals = np.zeros((self.nnzb_full, 3, 3))
+        rhs = np.zeros((self.nd, 3))
+        minv = np.zeros((self.nd, 3, 3))
+        N.check(self.lib.pk_system_get_matrix(self.h, N.ptr(vals), N.ptr(rhs), N.ptr(minv)), "pk_system_get_matrix")
+        return vals, rhs, minv
+
+    def elements(self):
+        blocks = np.zeros((self.t, 10, 3, 3))
+        forces = np.zeros((self.t, 4, 3))
+        kv = np.zeros((self.t, 4, 3))
+        N.check(self.lib.pk_system_get_elements(self.h, N.ptr(blocks), N.ptr(forces), N.ptr(kv)),
+                "pk_system_get_elements")
+        return blocks, forces, kv
+
+    def spmv(self, x):
+        xa = N.as_f64(x).reshape(-1)
+        y = np.zeros(self.nd * 3)
+        N.check(self.lib.pk_system_spmv(self.h, xa, y), "pk_system_spmv")
+        return y.reshape(-1, 3)
+
+    def solution(self):
+        dv = np.zeros(self.nd * 3)
+        N.check(self.lib.pk_system_get_solution(self.h, dv), "pk_system_get_solution")
+        return dv.reshape(-1, 3)
+
+
+def device_count() -> int:
+    return N.lib().pk_device_count()

@@ -1,0 +1,180 @@
+"""Python mirror of the PHYSIK_* C ABI (include/PhysiKAPI.h).
+
+Method names follow the reference entry points (include/PhysiK/API/PhysiKAPI.h
+in the reference tree) without the PHYSIK_ prefix; arguments and return values
+keep their meaning, so tests read like the reference's own
+(tests/PointConnectionTests.cpp drives the same calls).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native as N
+
+
+def material(density, young, poisson, damping):
+    return N.PhysikMaterialDesc(density, young, poisson, damping)
+
+
+class World:
+    def __init__(self):
+        self.lib = N.lib()
+        self.h = self.lib.PHYSIK_CreateWorld()
+
+    def close(self):
+        if self.h:
+            self.lib.PHYSIK_DestroyWorld(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # stepping and settings
+    def Step(self, dt):
+        """PHYSIK_Step; a missing node mass raises RuntimeError (the reference throws)."""
+        rc = self.lib.PHYSIK_AMD_TryStep(self.h, dt)
+        if rc == 3:
+            raise RuntimeError(self.lib.PHYSIK_AMD_GetLastError().decode())
+        if rc != 0:
+            raise RuntimeError(f"PHYSIK_Step failed ({rc}): {self.lib.PHYSIK_AMD_GetLastError().decode()}")
+
+    def SetSubstepCount(self, n):
+        self.lib.PHYSIK_SetSubstepCount(self.h, n)
+
+    def SetSolverMode(self, mode):
+        self.lib.PHYSIK_SetSolverMode(self.h, mode)
+
+    def SetConjugateGradientTolerance(self, tol):
+        self.lib.PHYSIK_SetConjugateGradientTolerance(self.h, tol)
+
+    def GetConjugateGradientTolerance(self):
+        return self.lib.PHYSIK_GetConjugateGradientTolerance(self.h)
+
+    def SetConjugateGradientMaxIterations(self, n):
+        self.lib.PHYSIK_SetConjugateGradientMaxIterations(self.h, n)
+
+    def GetConjugateGradientMaxIterations(self):
+        return self.lib.PHYSIK_GetConjugateGradientMaxIterations(self.h)
+
+    def GetLastConjugateGradientIterations(self):
+        return self.lib.PHYSIK_GetLastConjugateGradientIterations(self.h)
+
+    def GetLastConjugateGradientResidualNorm(self):
+        return self.lib.PHYSIK_GetLastConjugateGradientResidualNorm(self.h)
+
+    def DidLastConjugateGradientSolveConverge(self):
+        return self.lib.PHYSIK_DidLastConjugateGradientSolveConverge(self.h)
+
+    def SetGravity(self, x, y, z):
+        self.lib.PHYSIK_SetGravity(self.h, x, y, z)
+
+    # nodes
+    def AddNode(self, x, y, z):
+        return self.lib.PHYSIK_AddNode(self.h, x, y, z)
+
+    def SetNodeFixed(self, i, fixed):
+        self.lib.PHYSIK_SetNodeFixed(self.h, i, int(fixed))
+
+    def IsNodeFixed(self, i):
+        return self.lib.PHYSIK_IsNodeFixed(self.h, i)
+
+    def GetNodePosition(self, i):
+        x, y, z = C.c_float(), C.c_float(), C.c_float()
+        self.lib.PHYSIK_GetNodePosition(self.h, i, C.byref(x), C.byref(y), C.byref(z))
+        return np.array([x.value, y.value, z.value], np.float32)
+
+    def GetNodeVelocity(self, i):
+        x, y, z = C.c_float(), C.c_float(), C.c_float()
+        self.lib.PHYSIK_GetNodeVelocity(self.h, i, C.byref(x), C.byref(y), C.byref(z))
+        return np.array([x.value, y.value, z.value], np.float32)
+
+    def SetNodePosition(self, i, x, y, z):
+        self.lib.PHYSIK_SetNodePosition(self.h, i, x, y, z)
+
+    def SetNodeVelocity(self, i, x, y, z):
+        self.lib.PHYSIK_SetNodeVelocity(self.h, i, x, y, z)
+
+    # meshes and components
+    def GenerateTetMesh(self, positions, tets):
+        p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+        return self.lib.PHYSIK_GenerateTetMesh(N.ptr(p), p.shape[0], N.ptr(t), t.shape[0])
+
+    def GenerateTetMeshF64(self, positions, tets):
+        p = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+        return self.lib.PHYSIK_AMD_GenerateTetMeshF64(N.ptr(p), p.shape[0], N.ptr(t), t.shape[0])
+
+    def DestroyGeneratedTetMesh(self, mesh):
+        self.lib.PHYSIK_DestroyGeneratedTetMesh(mesh)
+
+    def CreateTetMeshPhysicsComponent(self, mesh, mat):
+        return self.lib.PHYSIK_CreateTetMeshPhysicsComponent(self.h, mesh, C.byref(mat))
+
+    def CreateTetMeshComponent(self, mesh):
+        return self.lib.PHYSIK_CreateTetMeshComponent(self.h, mesh)
+
+    def GetTetMeshGlobalNodeBeginIndex(self, comp):
+        return self.lib.PHYSIK_GetTetMeshGlobalNodeBeginIndex(self.h, comp)
+
+    def GetTetMeshGlobalNodeCount(self, comp):
+        return self.lib.PHYSIK_GetTetMeshGlobalNodeCount(self.h, comp)
+
+    def GetTetMeshGlobalNodeIndex(self, comp, local):
+        return self.lib.PHYSIK_GetTetMeshGlobalNodeIndex(self.h, comp, local)
+
+    def SetTetMeshMaterial(self, comp, mat):
+        self.lib.PHYSIK_SetTetMeshMaterial(self.h, comp, C.byref(mat))
+
+    def DestroyComponent(self, comp):
+        self.lib.PHYSIK_DestroyComponent(self.h, comp)
+
+    def IsComponentHandleValid(self, comp):
+        return self.lib.PHYSIK_IsComponentHandleValid(self.h, comp)
+
+    def GetTetMeshTetCount(self, comp):
+        return self.lib.PHYSIK_GetTetMeshTetCount(self.h, comp)
+
+    def GetTetMeshNodeCount(self, comp):
+        return self.lib.PHYSIK_GetTetMeshNodeCount(self.h, comp)
+
+    def IsTetActive(self, comp, t):
+        return self.lib.PHYSIK_IsTetActive(self.h, comp, t)
+
+    def SetTetActive(self, comp, t, active):
+        self.lib.PHYSIK_SetTetActive(self.h, comp, t, int(active))
+
+    def DeactivateTet(self, comp, t):
+        self.lib.PHYSIK_DeactivateTet(self.h, comp, t)
+
+    def GetActiveTetCount(self, comp):
+        return self.lib.PHYSIK_GetActiveTetCount(self.h, comp)
+
+    # additive
+    def SetPrecision(self, precision):
+        return self.lib.PHYSIK_AMD_SetPrecision(self.h, precision)
+
+    def GetLastStepStatus(self):
+        return self.lib.PHYSIK_AMD_GetLastStepStatus(self.h)
+
+    def SetTetMeshYoungModuli(self, comp, young):
+        y = np.ascontiguousarray(young, dtype=np.float64)
+        return self.lib.PHYSIK_AMD_SetTetMeshYoungModuli(self.h, comp, N.ptr(y), y.size)
+
+    def node_state(self, count):
+        x = np.zeros((count, 3))
+        v = np.zeros((count, 3))
+        self.lib.PHYSIK_AMD_CopyNodeStateF64(self.h, N.ptr(x), N.ptr(v), count)
+        return x, v
+
+    def set_node_state(self, x=None, v=None):
+        xa = None if x is None else N.as_f64(x).reshape(-1)
+        va = None if v is None else N.as_f64(v).reshape(-1)
+        n = (xa if xa is not None else va).size // 3
+        return self.lib.PHYSIK_AMD_SetNodeStateF64(self.h, N.ptr(xa), N.ptr(va), n)
