@@ -103,6 +103,12 @@ void pk_system_destroy(pk_system* sys);
 int pk_system_sizes(const pk_system* sys, int* nodes, int* tets, int* groups, int* dynamic_rows,
                     long long* nnzb_upper, long long* nnzb_full);
 
+/* Launch shape of the CG kernels: lanes per BSR row in the SpMV (4, 8, 16 or
+ * 32), rows per SpMV workgroup and rows per vector-kernel workgroup. Results
+ * of a body do not depend on the other bodies of the system, only on these
+ * three numbers. Chosen from the row lengths at creation; tuning hook. */
+int pk_system_tune(pk_system* sys, int lanes_per_row, int spmv_rows, int vector_rows);
+
 /* Node state, node_count x 3 each; either pointer may be NULL. */
 int pk_system_set_state(pk_system* sys, const double* x, const double* v);
 int pk_system_get_state(pk_system* sys, double* x, double* v);
@@ -136,8 +142,9 @@ int pk_system_kernel_time(pk_system* sys, int kernel, long long* launches, doubl
 /* Runs the element and assembly kernels of one substep of length dt (no
  * solve, no state change). */
 int pk_system_assemble(pk_system* sys, double dt);
-/* Assembled system after pk_system_assemble or a step: full BSR values
- * (nnzb_full x 9, row-major blocks, in pattern order), rhs (dynamic_rows x 3)
+/* Assembled system: full BSR values (nnzb_full x 9, row-major blocks, in
+ * pattern order), the CG residual vector (dynamic_rows x 3: the right-hand
+ * side b right after pk_system_assemble, the final residual after a step)
  * and block-Jacobi inverses (dynamic_rows x 9). Any pointer may be NULL. */
 int pk_system_get_matrix(pk_system* sys, double* full_vals, double* rhs, double* minv);
 /* Per-tet element record of the last assembly: the 10 blocks R K_ab R^T for

@@ -1,0 +1,225 @@
+// Device-side types and helpers shared by the HIP kernels, and the launch
+// interface between the host runtime and the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace pk
+{
+
+constexpr int NT = 256;  // threads per workgroup for every kernel
+
+// Per-group CG state. No workgroup reads a field that another workgroup of
+// the same launch may write: delta is double-buffered by iteration parity,
+// and stop decisions are taken identically by every workgroup of the group
+// (each sums the same partials in the same order) before one designated
+// workgroup records them for later launches and the host.
+struct GroupState
+{
+    double delta[2];  // r.z entering iteration k, at index k & 1
+    double target;    // tol^2 * delta_0
+    double residual;  // reported residual norm
+    double tol;
+    int maxIt;
+    int iters;
+    int stopAt;       // launch index that ended the loop (-1 while running)
+    int stopInUpdate; // 1 when the update launch of stopAt ended it, 0 for the SpMV launch
+    int converged;
+    int failed;       // the solve is unusable: do not integrate
+    int nonfinite;    // integration produced a non-finite value
+    int hasRows;
+    int jacobi;
+    int pad;
+};
+
+// One workgroup of a row-tiled kernel: rows [rowBegin, rowEnd) of group `group`.
+struct RowTile
+{
+    int group, rowBegin, rowEnd, pad;
+};
+
+// Per-workgroup partial sums of a reduction.
+struct Partial
+{
+    double a, b;
+    int flag, pad;
+};
+
+// A tiled launch: tiles of consecutive rows, grouped by body. Workgroup b runs
+// tile (b % 8) * per + b / 8 so that each of the 8 XCDs (workgroups are
+// dispatched round-robin over them) walks one contiguous eighth of the rows
+// and keeps its share of the gathered vectors in its own L2.
+struct TileView
+{
+    const RowTile* tiles;
+    const int* groupBegin;  // first tile of each group
+    const int* groupCount;  // tiles of each group
+    int count;              // tiles
+    int per;                // ceil(count / 8)
+};
+
+__device__ inline int tile_of(const TileView& tv)
+{
+    const int b = static_cast<int>(blockIdx.x);
+    const int t = (b & 7) * tv.per + (b >> 3);
+    return t < tv.count ? t : -1;
+}
+
+__device__ inline bool fin(double v) { return __builtin_isfinite(v); }
+
+__device__ inline double norm_of(double rr) { return (fin(rr) && rr >= 0.0) ? sqrt(rr) : INFINITY; }
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Workgroup sums of three values, returned to every thread (fixed order).
+__device__ inline void block_sum3(double& a, double& b, double& c)
+{
+    __shared__ double sh[12];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+    {
+        sh[w] = a;
+        sh[4 + w] = b;
+        sh[8 + w] = c;
+    }
+    __syncthreads();
+    a = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    b = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+    c = (sh[8] + sh[9]) + (sh[10] + sh[11]);
+    __syncthreads();
+}
+
+// Sums partials [begin, begin + count) in a fixed order; every workgroup that
+// calls this on the same range gets bit-identical totals.
+__device__ inline void sum_partials(const Partial* p, int begin, int count, double& a, double& b, int& flag)
+{
+    double x = 0.0, y = 0.0, z = 0.0;
+    int f = 0;
+    for (int i = threadIdx.x; i < count; i += NT)
+    {
+        const Partial q = p[begin + i];
+        x += q.a;
+        y += q.b;
+        f |= q.flag;
+    }
+    block_sum3(x, y, z);
+    a = x;
+    b = y;
+    flag = __syncthreads_or(f);
+}
+
+__device__ inline double sum_doubles(const double* p, int begin, int count)
+{
+    double x = 0.0, y = 0.0, z = 0.0;
+    for (int i = threadIdx.x; i < count; i += NT)
+        x += p[begin + i];
+    block_sum3(x, y, z);
+    return x;
+}
+
+// 3x3 row-major helpers. Column j of m is (m[j], m[3+j], m[6+j]).
+__device__ inline double det3(const double* m)
+{
+    // c0 . (c1 x c2), Mat3.h:52-55
+    const double rx = m[4] * m[8] - m[7] * m[5];
+    const double ry = m[7] * m[2] - m[1] * m[8];
+    const double rz = m[1] * m[5] - m[4] * m[2];
+    return m[0] * rx + m[3] * ry + m[6] * rz;
+}
+
+// Mat3::Inverse (Mat3.h:57-74): rows of the inverse are the cross products of
+// the columns divided by the determinant; |det| <= 1e-6f gives Identity.
+__device__ inline void inv3(const double* m, double* o)
+{
+    const double c0x = m[0], c0y = m[3], c0z = m[6];
+    const double c1x = m[1], c1y = m[4], c1z = m[7];
+    const double c2x = m[2], c2y = m[5], c2z = m[8];
+    const double r0x = c1y * c2z - c1z * c2y, r0y = c1z * c2x - c1x * c2z, r0z = c1x * c2y - c1y * c2x;
+    const double r1x = c2y * c0z - c2z * c0y, r1y = c2z * c0x - c2x * c0z, r1z = c2x * c0y - c2y * c0x;
+    const double r2x = c0y * c1z - c0z * c1y, r2y = c0z * c1x - c0x * c1z, r2z = c0x * c1y - c0y * c1x;
+    const double d = c0x * r0x + c0y * r0y + c0z * r0z;
+    if (fabs(d) <= static_cast<double>(0.000001f))
+    {
+        o[0] = 1.0; o[1] = 0.0; o[2] = 0.0;
+        o[3] = 0.0; o[4] = 1.0; o[5] = 0.0;
+        o[6] = 0.0; o[7] = 0.0; o[8] = 1.0;
+        return;
+    }
+    const double id = 1.0 / d;
+    o[0] = r0x * id; o[1] = r0y * id; o[2] = r0z * id;
+    o[3] = r1x * id; o[4] = r1y * id; o[5] = r1z * id;
+    o[6] = r2x * id; o[7] = r2y * id; o[8] = r2z * id;
+}
+
+__device__ inline bool fin9(const double* m)
+{
+    bool ok = true;
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+        ok = ok && fin(m[e]);
+    return ok;
+}
+
+// Device pointers and sizes of one system, passed to the launchers.
+struct DeviceView
+{
+    hipStream_t stream;
+    int T, nd, G;
+    long long nnzu, nnzf;
+    // topology
+    const int4* tets;
+    const unsigned char* live;
+    const double* prm;
+    const int* cstart;
+    const int* clist;
+    const int* uFull;
+    const int* uMirror;
+    const int* slotRow;
+    const int* rowGroup;
+    const int* fstart;
+    const int* fcol;
+    const int* nstart;
+    const int* nlist;
+    const int* rowNode;
+    const double* mrow;
+    // state and work arrays
+    double* x;
+    double* v;
+    double* elem;
+    double* minv;
+    double* xs;   // CG solution dv
+    double* r;
+    void* vals;   // M
+    void* z;      // M
+    void* d[2];   // M, by iteration parity
+    void* q;      // M
+    GroupState* gs;
+    int* active;
+    // reductions
+    TileView vec, spmv;
+    double* pa;        // per SpMV tile: d.q
+    Partial* pb[2];    // per vector tile: (r.z, r.r, flags), by iteration parity
+};
+
+// Launchers (pk_assembly.hip, pk_cg.hip). M is the storage type of A and of
+// the vectors the SpMV reads (double, or float for FP32 storage).
+void launch_reset(const DeviceView& v);
+void launch_element(const DeviceView& v);
+template <class M> void launch_assemble(const DeviceView& v, double dt);
+template <class M> void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt);
+void launch_integrate(const DeviceView& v, double dt, int commit);
+void launch_explicit(const DeviceView& v, double gx, double gy, double gz, double dt);
+template <class M> void launch_spmv(const DeviceView& v, int k, int lanesPerRow);
+template <class M> void launch_update(const DeviceView& v, int k);
+template <class M> void launch_spmv_plain(const DeviceView& v, const double* x, double* y);
+
+}  // namespace pk

@@ -1,0 +1,825 @@
+// pk_system runtime: device buffers, the substep driver and the pk_* C ABI.
+//
+// One substep (World.cpp:48-70) is issued on the system's own stream:
+//   reset -> element -> assemble -> rhs -> CG launches -> integrate (check,
+//   commit), or the explicit integrator in explicit mode.
+// The CG loop is launched in chunks; after each chunk the host queues a copy
+// of the count of bodies still iterating and checks the previous chunk's copy,
+// so the GPU never waits on the host and at most two chunks run past the end.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "physik_amd.h"
+#include "pk_device.h"
+#include "pk_topology.h"
+
+namespace pk
+{
+
+thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+
+#define PK_HIP(expr)                                                                                   \
+    do                                                                                                 \
+    {                                                                                                  \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+        {                                                                                              \
+            set_error(std::string(#expr) + " failed: " + hipGetErrorString(e_));                      \
+            return PK_ERR_HIP;                                                                         \
+        }                                                                                              \
+    } while (0)
+
+template <class T>
+struct DevBuf
+{
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf()
+    {
+        if (p)
+            (void)hipFree(p);
+    }
+    int alloc(size_t count)
+    {
+        if (p)
+        {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        n = count;
+        if (count == 0)
+            return PK_OK;
+        PK_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        return PK_OK;
+    }
+    int upload(const std::vector<T>& h, hipStream_t st)
+    {
+        const int rc = alloc(h.size());
+        if (rc != PK_OK)
+            return rc;
+        if (!h.empty())
+            PK_HIP(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        return PK_OK;
+    }
+};
+
+struct Tiles
+{
+    DevBuf<RowTile> tiles;
+    DevBuf<int> groupBegin, groupCount;
+    int count = 0;
+    int per = 0;
+    std::vector<int> hostBegin;  // first tile of each group
+
+    TileView view() const { return TileView{tiles.p, groupBegin.p, groupCount.p, count, per}; }
+};
+
+struct Profile
+{
+    bool on = false;
+    long long launches[PK_K_COUNT] = {};
+    double ms[PK_K_COUNT] = {};
+    std::vector<hipEvent_t> pool;
+    int used = 0;
+    struct Rec
+    {
+        int kernel, iter;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+};
+
+}  // namespace pk
+
+struct pk_system
+{
+    pk::Topology topo;
+    int precision = PK_FP64;
+    int lanesPerRow = 16;
+    int spmvRows = 256;
+    int vecRows = 1024;
+    hipStream_t stream = nullptr;
+
+    pk::DevBuf<int4> tets;
+    pk::DevBuf<unsigned char> live;
+    pk::DevBuf<double> prm, x, v, elem, mrow, minv, xs, r, pa;
+    pk::DevBuf<int> cstart, clist, uFull, uMirror, slotRow, rowGroup, fstart, fcol, nstart, nlist, rowNode, active;
+    pk::DevBuf<char> vals, z, d0, d1, q;  // element type set by the precision
+    pk::DevBuf<pk::Partial> pb0, pb1;
+    pk::DevBuf<pk::GroupState> gs;
+    pk::Tiles vecTiles, spmvTiles;
+    int* pinned = nullptr;
+    hipEvent_t pollEv[2] = {nullptr, nullptr};
+
+    double gravity[3] = {0, 0, 0};
+    double tol = 1e-4;
+    int maxIt = 128;
+    int jacobi = 1;
+    int implicit = 1;
+    int lastIters = 0;
+    std::vector<pk::GroupState> hostGs;
+    pk::Profile prof;
+
+    ~pk_system()
+    {
+        if (stream)
+            (void)hipStreamSynchronize(stream);
+        for (auto e : prof.pool)
+            (void)hipEventDestroy(e);
+        for (auto e : pollEv)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (pinned)
+            (void)hipHostFree(pinned);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+
+    pk::DeviceView view() const
+    {
+        pk::DeviceView dv{};
+        dv.stream = stream;
+        dv.T = topo.T;
+        dv.nd = topo.nd;
+        dv.G = topo.G;
+        dv.nnzu = topo.nnzu();
+        dv.nnzf = topo.nnzf();
+        dv.tets = tets.p;
+        dv.live = live.p;
+        dv.prm = prm.p;
+        dv.cstart = cstart.p;
+        dv.clist = clist.p;
+        dv.uFull = uFull.p;
+        dv.uMirror = uMirror.p;
+        dv.slotRow = slotRow.p;
+        dv.rowGroup = rowGroup.p;
+        dv.fstart = fstart.p;
+        dv.fcol = fcol.p;
+        dv.nstart = nstart.p;
+        dv.nlist = nlist.p;
+        dv.rowNode = rowNode.p;
+        dv.mrow = mrow.p;
+        dv.x = x.p;
+        dv.v = v.p;
+        dv.elem = elem.p;
+        dv.minv = minv.p;
+        dv.xs = xs.p;
+        dv.r = r.p;
+        dv.vals = vals.p;
+        dv.z = z.p;
+        dv.d[0] = d0.p;
+        dv.d[1] = d1.p;
+        dv.q = q.p;
+        dv.gs = gs.p;
+        dv.active = active.p;
+        dv.vec = vecTiles.view();
+        dv.spmv = spmvTiles.view();
+        dv.pa = pa.p;
+        dv.pb[0] = pb0.p;
+        dv.pb[1] = pb1.p;
+        return dv;
+    }
+};
+
+namespace pk
+{
+
+static int build_tiles(pk_system* s, Tiles& tt, int rowsPerTile)
+{
+    const Topology& t = s->topo;
+    std::vector<RowTile> tiles;
+    std::vector<int> gb(static_cast<size_t>(t.G), 0), gc(static_cast<size_t>(t.G), 0);
+    for (int g = 0; g < t.G; ++g)
+    {
+        gb[g] = static_cast<int>(tiles.size());
+        for (int r = t.groupRowBegin[g]; r < t.groupRowEnd[g]; r += rowsPerTile)
+            tiles.push_back(RowTile{g, r, std::min(r + rowsPerTile, t.groupRowEnd[g]), 0});
+        gc[g] = static_cast<int>(tiles.size()) - gb[g];
+    }
+    tt.count = static_cast<int>(tiles.size());
+    tt.per = (tt.count + 7) / 8;
+    tt.hostBegin = gb;
+    int rc;
+    if ((rc = tt.tiles.upload(tiles, s->stream)) != PK_OK) return rc;
+    if ((rc = tt.groupBegin.upload(gb, s->stream)) != PK_OK) return rc;
+    if ((rc = tt.groupCount.upload(gc, s->stream)) != PK_OK) return rc;
+    return PK_OK;
+}
+
+static int upload_groups(pk_system* s)
+{
+    const Topology& t = s->topo;
+    s->hostGs.assign(static_cast<size_t>(t.G), GroupState{});
+    for (int g = 0; g < t.G; ++g)
+    {
+        GroupState& st = s->hostGs[g];
+        std::memset(&st, 0, sizeof(st));
+        st.hasRows = t.groupRowEnd[g] > t.groupRowBegin[g] ? 1 : 0;
+        st.tol = s->tol;
+        st.maxIt = s->maxIt;
+        st.jacobi = s->jacobi;
+        st.stopAt = 0;
+    }
+    const int rc = s->gs.upload(s->hostGs, s->stream);
+    if (rc != PK_OK)
+        return rc;
+    PK_HIP(hipStreamSynchronize(s->stream));
+    return PK_OK;
+}
+
+// Event-bracketed launch when profiling.
+struct Bracket
+{
+    pk_system* s;
+    int kernel, iter;
+    hipEvent_t a = nullptr;
+    Bracket(pk_system* sys, int k, int it = -1) : s(sys), kernel(k), iter(it)
+    {
+        if (!s->prof.on)
+            return;
+        a = take();
+        (void)hipEventRecord(a, s->stream);
+    }
+    hipEvent_t take()
+    {
+        Profile& p = s->prof;
+        if (p.used == static_cast<int>(p.pool.size()))
+        {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            p.pool.push_back(e);
+        }
+        return p.pool[p.used++];
+    }
+    ~Bracket()
+    {
+        if (!s->prof.on)
+            return;
+        hipEvent_t b = take();
+        (void)hipEventRecord(b, s->stream);
+        s->prof.recs.push_back(Profile::Rec{kernel, iter, a, b});
+    }
+};
+
+// Accumulates the recorded launches; CG launches issued after every body had
+// stopped did no work and are left out.
+static void harvest_profile(pk_system* s)
+{
+    Profile& p = s->prof;
+    if (!p.on)
+        return;
+    (void)hipStreamSynchronize(s->stream);
+    int spmvWork = 0, updWork = 0;
+    for (const auto& g : s->hostGs)
+        if (g.hasRows)
+        {
+            spmvWork = std::max(spmvWork, g.stopAt + g.stopInUpdate);
+            updWork = std::max(updWork, g.stopAt);
+        }
+    for (const auto& rec : p.recs)
+    {
+        if (rec.kernel == PK_K_SPMV && rec.iter >= spmvWork)
+            continue;
+        if (rec.kernel == PK_K_UPDATE && rec.iter >= updWork)
+            continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, rec.a, rec.b) == hipSuccess)
+        {
+            p.ms[rec.kernel] += ms;
+            p.launches[rec.kernel] += 1;
+        }
+    }
+    p.recs.clear();
+    p.used = 0;
+}
+
+template <class M>
+static int run_assembly(pk_system* s, double dt)
+{
+    const DeviceView dv = s->view();
+    launch_reset(dv);
+    {
+        Bracket br(s, PK_K_ELEMENT);
+        launch_element(dv);
+    }
+    if (s->topo.nd == 0)
+        return PK_OK;
+    {
+        Bracket br(s, PK_K_ASSEMBLE);
+        launch_assemble<M>(dv, dt);
+    }
+    {
+        Bracket br(s, PK_K_RHS);
+        launch_rhs<M>(dv, s->gravity[0], s->gravity[1], s->gravity[2], dt);
+    }
+    PK_HIP(hipGetLastError());
+    return PK_OK;
+}
+
+template <class M>
+static int run_cg(pk_system* s)
+{
+    const DeviceView dv = s->view();
+    const int maxIt = std::max(0, s->maxIt);
+    // launch k runs the SpMV of iteration k (and decides the end of k-1);
+    // k = maxIt only takes that decision
+    int k = 0, poll = 0;
+    int chunk = std::max(4, s->lastIters - 4);
+    while (k <= maxIt)
+    {
+        const int end = std::min(maxIt + 1, k + chunk);
+        for (; k < end; ++k)
+        {
+            {
+                Bracket br(s, PK_K_SPMV, k);
+                launch_spmv<M>(dv, k, s->lanesPerRow);
+            }
+            if (k < maxIt)
+            {
+                Bracket br(s, PK_K_UPDATE, k);
+                launch_update<M>(dv, k);
+            }
+        }
+        chunk = 4;
+        const int slot = poll & 1;
+        PK_HIP(hipMemcpyAsync(s->pinned + slot, s->active.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        PK_HIP(hipEventRecord(s->pollEv[slot], s->stream));
+        if (poll > 0)
+        {
+            PK_HIP(hipEventSynchronize(s->pollEv[slot ^ 1]));
+            if (s->pinned[slot ^ 1] == 0)
+                break;
+        }
+        ++poll;
+    }
+    PK_HIP(hipGetLastError());
+    return PK_OK;
+}
+
+template <class M>
+static int run_substep(pk_system* s, double dt)
+{
+    int rc = run_assembly<M>(s, dt);
+    if (rc != PK_OK || s->topo.nd == 0)
+        return rc;
+    const DeviceView dv = s->view();
+    if (s->implicit)
+    {
+        if ((rc = run_cg<M>(s)) != PK_OK)
+            return rc;
+        Bracket br(s, PK_K_INTEGRATE);
+        launch_integrate(dv, dt, 0);
+        launch_integrate(dv, dt, 1);
+    }
+    else
+    {
+        Bracket br(s, PK_K_INTEGRATE);
+        launch_explicit(dv, s->gravity[0], s->gravity[1], s->gravity[2], dt);
+    }
+    PK_HIP(hipGetLastError());
+    return PK_OK;
+}
+
+static int read_groups(pk_system* s)
+{
+    if (s->hostGs.empty())
+        return PK_OK;
+    PK_HIP(hipMemcpyAsync(s->hostGs.data(), s->gs.p, s->hostGs.size() * sizeof(GroupState), hipMemcpyDeviceToHost,
+                          s->stream));
+    PK_HIP(hipStreamSynchronize(s->stream));
+    int mx = 0;
+    for (const auto& g : s->hostGs)
+        mx = std::max(mx, g.iters);
+    s->lastIters = mx;
+    return PK_OK;
+}
+
+static int choose_lanes(const Topology& t)
+{
+    if (t.nd == 0)
+        return 16;
+    const double avg = static_cast<double>(t.nnzf()) / t.nd;
+    if (avg <= 6.0)
+        return 8;
+    if (avg <= 18.0)
+        return 16;
+    return 32;
+}
+
+static int alloc_work(pk_system* s)
+{
+    const Topology& t = s->topo;
+    const size_t es = s->precision == PK_FP32_STORAGE ? sizeof(float) : sizeof(double);
+    const size_t nd3 = static_cast<size_t>(t.nd) * 3;
+    int rc;
+    if ((rc = s->elem.alloc(static_cast<size_t>(t.T) * E_STRIDE)) != PK_OK) return rc;
+    if ((rc = s->vals.alloc(static_cast<size_t>(t.nnzf()) * 9 * es)) != PK_OK) return rc;
+    if ((rc = s->z.alloc(nd3 * es)) != PK_OK) return rc;
+    if ((rc = s->d0.alloc(nd3 * es)) != PK_OK) return rc;
+    if ((rc = s->d1.alloc(nd3 * es)) != PK_OK) return rc;
+    if ((rc = s->q.alloc(nd3 * es)) != PK_OK) return rc;
+    if ((rc = s->minv.alloc(static_cast<size_t>(t.nd) * 9)) != PK_OK) return rc;
+    if ((rc = s->xs.alloc(nd3)) != PK_OK) return rc;
+    if ((rc = s->r.alloc(nd3)) != PK_OK) return rc;
+    if ((rc = s->active.alloc(1)) != PK_OK) return rc;
+    return PK_OK;
+}
+
+static int setup_tiles(pk_system* s)
+{
+    int rc;
+    if ((rc = build_tiles(s, s->vecTiles, s->vecRows)) != PK_OK) return rc;
+    if ((rc = build_tiles(s, s->spmvTiles, s->spmvRows)) != PK_OK) return rc;
+    if ((rc = s->pa.alloc(std::max(1, s->spmvTiles.count))) != PK_OK) return rc;
+    if ((rc = s->pb0.alloc(std::max(1, s->vecTiles.count))) != PK_OK) return rc;
+    if ((rc = s->pb1.alloc(std::max(1, s->vecTiles.count))) != PK_OK) return rc;
+    PK_HIP(hipStreamSynchronize(s->stream));
+    return PK_OK;
+}
+
+static int setup_device(pk_system* s)
+{
+    const Topology& t = s->topo;
+    hipStream_t st = s->stream;
+    int rc;
+    std::vector<int4> tv(static_cast<size_t>(t.T));
+    for (int i = 0; i < t.T; ++i)
+        tv[i] = make_int4(t.tets[4 * i], t.tets[4 * i + 1], t.tets[4 * i + 2], t.tets[4 * i + 3]);
+    if ((rc = s->tets.upload(tv, st)) != PK_OK) return rc;
+    std::vector<unsigned char> live(t.live.begin(), t.live.end());
+    if ((rc = s->live.upload(live, st)) != PK_OK) return rc;
+    if ((rc = s->prm.upload(t.prm, st)) != PK_OK) return rc;
+    std::vector<double> mrow(static_cast<size_t>(t.nd));
+    for (int r = 0; r < t.nd; ++r)
+        mrow[r] = t.mass[t.rowNode[r]];
+    if ((rc = s->mrow.upload(mrow, st)) != PK_OK) return rc;
+    if ((rc = s->cstart.upload(t.cstart, st)) != PK_OK) return rc;
+    if ((rc = s->clist.upload(t.clist, st)) != PK_OK) return rc;
+    if ((rc = s->uFull.upload(t.uFull, st)) != PK_OK) return rc;
+    if ((rc = s->uMirror.upload(t.uMirror, st)) != PK_OK) return rc;
+    std::vector<int> slotRow(static_cast<size_t>(t.nnzu())), rowGroup(static_cast<size_t>(t.nd));
+    for (int r = 0; r < t.nd; ++r)
+        for (int k = t.ustart[r]; k < t.ustart[r + 1]; ++k)
+            slotRow[k] = r;
+    for (int g = 0; g < t.G; ++g)
+        for (int r = t.groupRowBegin[g]; r < t.groupRowEnd[g]; ++r)
+            rowGroup[r] = g;
+    if ((rc = s->slotRow.upload(slotRow, st)) != PK_OK) return rc;
+    if ((rc = s->rowGroup.upload(rowGroup, st)) != PK_OK) return rc;
+    if ((rc = s->fstart.upload(t.fstart, st)) != PK_OK) return rc;
+    if ((rc = s->fcol.upload(t.fcol, st)) != PK_OK) return rc;
+    if ((rc = s->nstart.upload(t.nstart, st)) != PK_OK) return rc;
+    if ((rc = s->nlist.upload(t.nlist, st)) != PK_OK) return rc;
+    if ((rc = s->rowNode.upload(t.rowNode, st)) != PK_OK) return rc;
+    if ((rc = alloc_work(s)) != PK_OK) return rc;
+    std::vector<double> zero(static_cast<size_t>(t.N) * 3, 0.0);
+    if ((rc = s->v.upload(zero, st)) != PK_OK) return rc;
+    if ((rc = s->x.alloc(static_cast<size_t>(t.N) * 3)) != PK_OK) return rc;
+    s->lanesPerRow = choose_lanes(t);
+    if ((rc = setup_tiles(s)) != PK_OK) return rc;
+    return upload_groups(s);
+}
+
+}  // namespace pk
+
+using namespace pk;
+
+extern "C" {
+
+const char* pk_last_error(void) { return g_error.c_str(); }
+
+int pk_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+        return 0;
+    return n;
+}
+
+int pk_system_create(const pk_mesh_desc* desc, int precision, pk_system** out)
+{
+    if (!desc || !out || (precision != PK_FP64 && precision != PK_FP32_STORAGE))
+    {
+        set_error("pk_system_create: bad argument");
+        return PK_ERR_ARG;
+    }
+    *out = nullptr;
+    if (pk_device_count() <= 0)
+    {
+        set_error("pk_system_create: no GPU visible");
+        return PK_ERR_NO_DEVICE;
+    }
+    std::unique_ptr<pk_system> s(new pk_system());
+    s->precision = precision;
+    std::string err;
+    int rc = build_topology(*desc, s->topo, err, false);
+    if (rc != PK_OK)
+    {
+        set_error(err);
+        return rc;
+    }
+    PK_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    PK_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->pinned), 2 * sizeof(int), hipHostMallocDefault));
+    PK_HIP(hipEventCreateWithFlags(&s->pollEv[0], hipEventDisableTiming));
+    PK_HIP(hipEventCreateWithFlags(&s->pollEv[1], hipEventDisableTiming));
+    if ((rc = setup_device(s.get())) != PK_OK)
+        return rc;
+    if (desc->node_count > 0)
+        PK_HIP(hipMemcpyAsync(s->x.p, desc->rest, 3 * static_cast<size_t>(desc->node_count) * sizeof(double),
+                              hipMemcpyHostToDevice, s->stream));
+    PK_HIP(hipStreamSynchronize(s->stream));
+    *out = s.release();
+    return PK_OK;
+}
+
+void pk_system_destroy(pk_system* sys) { delete sys; }
+
+int pk_system_sizes(const pk_system* s, int* nodes, int* tets, int* groups, int* rows, long long* nnzu,
+                    long long* nnzf)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    if (nodes) *nodes = s->topo.N;
+    if (tets) *tets = s->topo.T;
+    if (groups) *groups = s->topo.G;
+    if (rows) *rows = s->topo.nd;
+    if (nnzu) *nnzu = s->topo.nnzu();
+    if (nnzf) *nnzf = s->topo.nnzf();
+    return PK_OK;
+}
+
+int pk_system_tune(pk_system* s, int lanes_per_row, int spmv_rows, int vector_rows)
+{
+    if (!s || (lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16 && lanes_per_row != 32) ||
+        spmv_rows < 1 || vector_rows < 1)
+    {
+        set_error("pk_system_tune: lanes_per_row must be 4, 8, 16 or 32 and row counts positive");
+        return PK_ERR_ARG;
+    }
+    s->lanesPerRow = lanes_per_row;
+    s->spmvRows = spmv_rows;
+    s->vecRows = vector_rows;
+    return setup_tiles(s);
+}
+
+int pk_system_set_state(pk_system* s, const double* x, const double* v)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    const size_t bytes = static_cast<size_t>(s->topo.N) * 3 * sizeof(double);
+    if (x && bytes)
+        PK_HIP(hipMemcpyAsync(s->x.p, x, bytes, hipMemcpyHostToDevice, s->stream));
+    if (v && bytes)
+        PK_HIP(hipMemcpyAsync(s->v.p, v, bytes, hipMemcpyHostToDevice, s->stream));
+    PK_HIP(hipStreamSynchronize(s->stream));
+    return PK_OK;
+}
+
+int pk_system_get_state(pk_system* s, double* x, double* v)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    const size_t bytes = static_cast<size_t>(s->topo.N) * 3 * sizeof(double);
+    if (x && bytes)
+        PK_HIP(hipMemcpyAsync(x, s->x.p, bytes, hipMemcpyDeviceToHost, s->stream));
+    if (v && bytes)
+        PK_HIP(hipMemcpyAsync(v, s->v.p, bytes, hipMemcpyDeviceToHost, s->stream));
+    PK_HIP(hipStreamSynchronize(s->stream));
+    return PK_OK;
+}
+
+int pk_system_set_gravity(pk_system* s, double gx, double gy, double gz)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    s->gravity[0] = gx;
+    s->gravity[1] = gy;
+    s->gravity[2] = gz;
+    return PK_OK;
+}
+
+int pk_system_set_solver(pk_system* s, double tolerance, int max_iterations, int use_jacobi)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    s->tol = tolerance;
+    s->maxIt = max_iterations;
+    s->jacobi = use_jacobi ? 1 : 0;
+    return upload_groups(s);
+}
+
+int pk_system_set_mode(pk_system* s, int implicit)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    s->implicit = implicit ? 1 : 0;
+    return PK_OK;
+}
+
+int pk_system_set_young(pk_system* s, const double* young)
+{
+    if (!s || !young)
+        return PK_ERR_ARG;
+    Topology& t = s->topo;
+    if (t.T == 0)
+        return PK_OK;
+    const size_t T = static_cast<size_t>(t.T);
+    for (size_t i = 0; i < T; ++i)
+        lame(young[i], t.poisson[i], &t.prm[P_LAMBDA * T + i], &t.prm[P_MU * T + i]);
+    // lambda and mu are adjacent planes of the parameter block
+    PK_HIP(hipMemcpyAsync(s->prm.p + P_LAMBDA * T, t.prm.data() + P_LAMBDA * T, 2 * T * sizeof(double),
+                          hipMemcpyHostToDevice, s->stream));
+    PK_HIP(hipStreamSynchronize(s->stream));
+    return PK_OK;
+}
+
+int pk_system_step(pk_system* s, double dt, int substeps, int frames)
+{
+    if (!s || substeps < 1 || frames < 0)
+    {
+        set_error("pk_system_step: bad argument");
+        return PK_ERR_ARG;
+    }
+    if (s->topo.massErrorNode >= 0)
+    {
+        set_error("Non-fixed node has no physical mass contribution. nodeIndex=" +
+                  std::to_string(s->topo.massErrorNode));
+        return PK_ERR_MASS;
+    }
+    if (!(dt > 0.0))
+        return PK_OK;  // World::Step does nothing for dt <= 0
+    const double h = dt / substeps;
+    for (int f = 0; f < frames; ++f)
+        for (int k = 0; k < substeps; ++k)
+        {
+            const int rc = s->precision == PK_FP32_STORAGE ? run_substep<float>(s, h) : run_substep<double>(s, h);
+            if (rc != PK_OK)
+                return rc;
+            if (s->implicit && s->topo.nd > 0)
+            {
+                // the outcome sizes the next solve's first chunk
+                const int rc2 = read_groups(s);
+                if (rc2 != PK_OK)
+                    return rc2;
+                harvest_profile(s);
+            }
+        }
+    PK_HIP(hipStreamSynchronize(s->stream));
+    return PK_OK;
+}
+
+int pk_system_group_results(pk_system* s, int* iterations, double* residual, int* converged, int* status)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    for (int g = 0; g < s->topo.G; ++g)
+    {
+        const GroupState& st = s->hostGs[g];
+        if (iterations) iterations[g] = st.iters;
+        if (residual) residual[g] = st.residual;
+        if (converged) converged[g] = st.converged;
+        if (status)
+            status[g] = !st.hasRows ? PK_GROUP_NO_DYNAMIC
+                        : st.failed ? PK_GROUP_SOLVE_FAILED
+                        : st.nonfinite ? PK_GROUP_NONFINITE
+                                       : PK_GROUP_INTEGRATED;
+    }
+    return PK_OK;
+}
+
+int pk_system_profile(pk_system* s, int enable)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    s->prof.on = enable != 0;
+    for (int k = 0; k < PK_K_COUNT; ++k)
+    {
+        s->prof.launches[k] = 0;
+        s->prof.ms[k] = 0.0;
+    }
+    s->prof.recs.clear();
+    s->prof.used = 0;
+    return PK_OK;
+}
+
+int pk_system_kernel_time(pk_system* s, int kernel, long long* launches, double* total_ms)
+{
+    if (!s || kernel < 0 || kernel >= PK_K_COUNT)
+        return PK_ERR_ARG;
+    if (launches) *launches = s->prof.launches[kernel];
+    if (total_ms) *total_ms = s->prof.ms[kernel];
+    return PK_OK;
+}
+
+int pk_system_assemble(pk_system* s, double dt)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    const int rc = s->precision == PK_FP32_STORAGE ? run_assembly<float>(s, dt) : run_assembly<double>(s, dt);
+    if (rc != PK_OK)
+        return rc;
+    PK_HIP(hipStreamSynchronize(s->stream));
+    const int rc2 = read_groups(s);
+    s->prof.recs.clear();
+    s->prof.used = 0;
+    return rc2;
+}
+
+int pk_system_get_matrix(pk_system* s, double* full_vals, double* rhs, double* minv)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    const Topology& t = s->topo;
+    const size_t F = static_cast<size_t>(t.nnzf());
+    PK_HIP(hipStreamSynchronize(s->stream));
+    if (full_vals && F)
+    {
+        const size_t es = s->precision == PK_FP32_STORAGE ? 4 : 8;
+        std::vector<char> raw(F * 9 * es);
+        PK_HIP(hipMemcpy(raw.data(), s->vals.p, raw.size(), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < F; ++k)
+            for (int e = 0; e < 9; ++e)
+                full_vals[9 * k + e] = es == 4 ? reinterpret_cast<const float*>(raw.data())[e * F + k]
+                                               : reinterpret_cast<const double*>(raw.data())[e * F + k];
+    }
+    if (rhs && t.nd)
+        PK_HIP(hipMemcpy(rhs, s->r.p, static_cast<size_t>(t.nd) * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (minv && t.nd)
+    {
+        std::vector<double> raw(static_cast<size_t>(t.nd) * 9);
+        PK_HIP(hipMemcpy(raw.data(), s->minv.p, raw.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int r = 0; r < t.nd; ++r)
+            for (int e = 0; e < 9; ++e)
+                minv[9 * r + e] = raw[static_cast<size_t>(e) * t.nd + r];
+    }
+    return PK_OK;
+}
+
+int pk_system_get_elements(pk_system* s, double* blocks, double* forces, double* kv)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    const Topology& t = s->topo;
+    PK_HIP(hipStreamSynchronize(s->stream));
+    std::vector<double> raw(static_cast<size_t>(t.T) * E_STRIDE);
+    if (!raw.empty())
+        PK_HIP(hipMemcpy(raw.data(), s->elem.p, raw.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < t.T; ++i)
+    {
+        const double* e = raw.data() + static_cast<size_t>(i) * E_STRIDE;
+        const bool lv = t.live[i] != 0;
+        for (int k = 0; k < 90; ++k)
+            if (blocks) blocks[90 * static_cast<size_t>(i) + k] = lv ? e[E_BLOCKS + k] : 0.0;
+        for (int k = 0; k < 12; ++k)
+        {
+            if (forces) forces[12 * static_cast<size_t>(i) + k] = lv ? e[E_FORCE + k] : 0.0;
+            if (kv) kv[12 * static_cast<size_t>(i) + k] = lv ? e[E_KV + k] : 0.0;
+        }
+    }
+    return PK_OK;
+}
+
+int pk_system_spmv(pk_system* s, const double* x, double* y)
+{
+    if (!s || !x || !y)
+        return PK_ERR_ARG;
+    const Topology& t = s->topo;
+    if (t.nd == 0)
+        return PK_OK;
+    DevBuf<double> dx, dy;
+    int rc;
+    if ((rc = dx.alloc(static_cast<size_t>(t.nd) * 3)) != PK_OK) return rc;
+    if ((rc = dy.alloc(static_cast<size_t>(t.nd) * 3)) != PK_OK) return rc;
+    PK_HIP(hipMemcpyAsync(dx.p, x, static_cast<size_t>(t.nd) * 3 * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    const DeviceView dv = s->view();
+    if (s->precision == PK_FP32_STORAGE)
+        launch_spmv_plain<float>(dv, dx.p, dy.p);
+    else
+        launch_spmv_plain<double>(dv, dx.p, dy.p);
+    PK_HIP(hipGetLastError());
+    PK_HIP(hipMemcpyAsync(y, dy.p, static_cast<size_t>(t.nd) * 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    PK_HIP(hipStreamSynchronize(s->stream));
+    return PK_OK;
+}
+
+int pk_system_get_solution(pk_system* s, double* dv)
+{
+    if (!s || !dv)
+        return PK_ERR_ARG;
+    PK_HIP(hipStreamSynchronize(s->stream));
+    if (s->topo.nd)
+        PK_HIP(hipMemcpy(dv, s->xs.p, static_cast<size_t>(s->topo.nd) * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    return PK_OK;
+}
+
+}  // extern "C"

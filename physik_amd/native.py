@@ -54,6 +54,7 @@ PK_SIGNATURES = {
     "pk_system_destroy": (None, [_vp]),
     "pk_system_sizes": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "pk_system_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "pk_system_set_state": (C.c_int, [_vp, _vp, _vp]),
     "pk_system_get_state": (C.c_int, [_vp, _vp, _vp]),
     "pk_system_set_gravity": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),

@@ -89,6 +89,9 @@ class System:
         except Exception:
             pass
 
+    def tune(self, lanes_per_row=16, spmv_rows=256, vector_rows=1024):
+        N.check(self.lib.pk_system_tune(self.h, lanes_per_row, spmv_rows, vector_rows), "pk_system_tune")
+
     def set_state(self, x=None, v=None):
         xa = None if x is None else N.as_f64(x).reshape(-1)
         va = None if v is None else N.as_f64(v).reshape(-1)
