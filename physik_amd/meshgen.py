@@ -172,19 +172,20 @@ def jitter(nodes: np.ndarray, h: float, frac: float, rng: Pcg32) -> np.ndarray:
 
 
 def twist_about_x(rest: np.ndarray, nx: int, ny: int, nz: int, h: float, angle: float) -> np.ndarray:
-    """Rotates every grid plane i about the x axis through the box centre line
-    by angle * i / nx (so the x = 0 face stays put and the far face turns by
-    ``angle``)."""
+    """Rotates every node about the x axis through the box centre line by
+    angle * x / (nx h), with x the node's own rest coordinate, so the map is a
+    smooth twist (volume preserving up to discretisation) whose far face turns
+    by ``angle``. Nodes of the x = 0 grid face keep their rest position."""
     g = grid_index(nx, ny, nz)
-    i_of_node = np.empty(rest.shape[0], dtype=np.float64)
-    i_of_node[g.ravel()] = np.broadcast_to(np.arange(nx + 1), g.shape).ravel()
-    theta = angle * i_of_node / nx
+    theta = angle * rest[:, 0] / (nx * h)
+    theta[g[:, :, 0].ravel()] = 0.0
     cy = 0.5 * ny * h
     cz = 0.5 * nz * h
     y = rest[:, 1] - cy
     z = rest[:, 2] - cz
     c, s = np.cos(theta), np.sin(theta)
     out = rest.copy()
-    out[:, 1] = cy + c * y - s * z
-    out[:, 2] = cz + s * y + c * z
+    # displacement form: nodes with theta = 0 keep their exact coordinates
+    out[:, 1] = rest[:, 1] + ((c - 1.0) * y - s * z)
+    out[:, 2] = rest[:, 2] + (s * y + (c - 1.0) * z)
     return out

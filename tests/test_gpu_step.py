@@ -1,0 +1,240 @@
+"""GPU implicit-Euler step against the FP64 oracle and the reference fixtures.
+
+Tolerances (per step, relative L2 over all nodes):
+  FP64 system vs FP64 oracle ............. <= 1e-9, CG iterations within +-1
+  FP32-storage system vs FP64 oracle ...... <= 1e-4
+  either vs the FP32 reference fixture .... <= 1e-4
+Element, assembly and SpMV outputs are compared entrywise against the oracle
+at 1e-12 (relative to the largest entry).
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from checkers import Oracle, oracle_spmv
+from physik_amd import meshgen as mg
+from physik_amd import native as N
+from physik_amd import workloads as W
+from physik_amd.system import System
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def oracle_for(w, precision=64):
+    o = Oracle(precision, w.rest, w.tets, w.young, w.poisson, w.damping, w.density)
+    o.set_params(w.gravity, 1, w.tol, w.max_iter)
+    o.set_fixed(w.fixed)
+    o.set_state(w.x0, w.v0)
+    return o
+
+
+def run_pair(w, precision, steps, tol, iters_exact=True):
+    print(f"case {w.name} precision {precision} steps {steps}", flush=True)
+    s = W.make_system(w, precision)
+    o = oracle_for(w)
+    for k in range(steps):
+        s.step(w.dt)
+        assert o.step(w.dt) == 0
+        xg, vg = s.state()
+        xo, vo = o.state()
+        it, res, conv, st = s.results()
+        io, ro, co = o.cg()
+        assert st[0] == N.GROUP_INTEGRATED
+        assert rel(xg, xo) <= tol and rel(vg, vo) <= tol, f"step {k}: {rel(xg, xo):.2e} {rel(vg, vo):.2e}"
+        if iters_exact:
+            assert abs(int(it[0]) - io) <= 1 and bool(conv[0]) == co, f"step {k}: iterations {it[0]} vs {io}"
+    return s, o
+
+
+def test_beam10_fp64_100_steps_matches_oracle():
+    run_pair(W.beam10(), N.PK_FP64, 100, 1e-9)
+
+
+def test_beam10_fp32_storage_matches_oracle():
+    run_pair(W.beam10(), N.PK_FP32_STORAGE, 30, 1e-4, iters_exact=False)
+
+
+@pytest.mark.parametrize("precision", [N.PK_FP64, N.PK_FP32_STORAGE])
+def test_beam10_matches_reference_fixture(golden_dir, precision):
+    g = np.load(os.path.join(golden_dir, "config1_fp32.npz"))
+    w = W.beam10()
+    w.rest = g["rest"].astype(np.float64)
+    w.x0 = w.rest.copy()
+    w.dt = float(g["dt"])
+    s = W.make_system(w, precision)
+    for k in range(1, 101):
+        s.step(w.dt)
+        if f"x{k}" in g:
+            x, v = s.state()
+            assert rel(x, g[f"x{k}"]) <= 1e-4 and rel(v, g[f"v{k}"]) <= 1e-4, f"step {k}"
+    it = s.results()[0][0]
+    assert abs(int(it) - int(g["cg_iterations"][-1])) <= 2
+
+
+def test_twisted_jittered_kuhn_matches_oracle():
+    run_pair(W.twist64(n=12, h=1.0 / 6), N.PK_FP64, 4, 1e-9)
+
+
+def deformed_beam():
+    w = W.beam10(n=4)
+    rng = mg.Pcg32(21)
+    w.x0 = w.rest + (rng.uniform(w.rest.size).reshape(-1, 3) - 0.5) * 0.04
+    w.v0 = rng.uniform(w.rest.size).reshape(-1, 3) - 0.5
+    w.x0[w.fixed == 1] = w.rest[w.fixed == 1]
+    w.v0[w.fixed == 1] = 0
+    w.damping = np.full(w.n_tets, 0.3)
+    return w
+
+
+PAIRS = [(a, b) for a in range(4) for b in range(a, 4)]
+
+
+def test_element_kernel_matches_oracle():
+    w = deformed_beam()
+    s = W.make_system(w)
+    s.assemble(w.dt)
+    blocks, forces, kv = s.elements()
+    o = oracle_for(w)
+    o.assemble(w.dt)
+    valid, R, f, K = o.elements()
+    assert valid.all()
+    Kp = np.stack([K[:, a, b] for a, b in PAIRS], axis=1)
+    assert np.abs(blocks - Kp).max() <= 1e-12 * np.abs(Kp).max()
+    assert np.abs(forces - f).max() <= 1e-12 * np.abs(f).max()
+    # K v per corner from the oracle's blocks
+    kv_ref = np.einsum("tabij,tbj->tai", K, w.v0[w.tets])
+    assert np.abs(kv - kv_ref).max() <= 1e-12 * np.abs(kv_ref).max()
+
+
+def test_assembly_and_spmv_match_oracle():
+    w = deformed_beam()
+    s = W.make_system(w)
+    s.assemble(w.dt)
+    vals, rhs, minv = s.matrix()
+    o = oracle_for(w)
+    ref = o.assemble(w.dt)
+    from physik_amd.system import Pattern
+    p = Pattern(w.rest, w.tets, node_fixed=w.fixed)
+    # full BSR -> every upper block and its mirror
+    for r in range(p.nd):
+        for k in range(p.full_start[r], p.full_start[r + 1]):
+            c = p.full_col[k]
+            lo, hi = min(r, c), max(r, c)
+            j = ref["row_start"][lo] + np.searchsorted(ref["col"][ref["row_start"][lo]:ref["row_start"][lo + 1]], hi)
+            expect = ref["vals"][j] if r <= c else ref["vals"][j].T
+            assert np.abs(vals[k] - expect).max() <= 1e-12 * np.abs(ref["vals"]).max()
+    assert np.abs(rhs - ref["rhs"]).max() <= 1e-11 * np.abs(ref["rhs"]).max()
+    diag = np.array([ref["vals"][ref["row_start"][r]] for r in range(p.nd)])
+    assert np.abs(minv - np.linalg.inv(diag)).max() <= 1e-12 * np.abs(minv).max()
+    x = np.sin(np.arange(p.nd * 3.0)).reshape(-1, 3)
+    y = s.spmv(x)
+    y_ref = oracle_spmv(ref["row_start"], ref["col"], ref["vals"], x)
+    assert np.abs(y - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
+
+
+def test_two_runs_are_bit_identical():
+    w = W.twist64(n=10, h=0.1)
+    out = []
+    for _ in range(2):
+        s = W.make_system(w)
+        s.step(w.dt, 1, 3)
+        out.append(s.state())
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_batched_bodies_equal_single_body_runs():
+    w = W.batch256(bodies=6, lo=2, hi=6, steps=3)
+    sb = W.make_system(w)
+    sb.tune(16, 64, 256)
+    sb.step(w.dt, 1, w.steps)
+    xb, vb = sb.state()
+    itb, resb, convb, stb = sb.results()
+    for b in range(w.n_bodies):
+        wb = w.body(b)
+        s1 = W.make_system(wb)
+        s1.tune(16, 64, 256)
+        s1.step(wb.dt, 1, wb.steps)
+        x1, v1 = s1.state()
+        n0, n1 = w.body_nodes[b]
+        assert np.array_equal(xb[n0:n1], x1) and np.array_equal(vb[n0:n1], v1), f"body {b}"
+        assert s1.results()[0][0] == itb[b]
+        o = oracle_for(wb)
+        for _ in range(wb.steps):
+            o.step(wb.dt)
+        xo, vo = o.state()
+        assert rel(x1, xo) <= 1e-9 and rel(v1, vo) <= 1e-9
+        assert o.cg()[0] == itb[b]
+
+
+def test_explicit_mode_matches_oracle():
+    w = W.beam10(n=3)
+    w.young = np.full(w.n_tets, 50.0)
+    s = W.make_system(w)
+    s.set_mode(False)
+    o = oracle_for(w)
+    o.set_params(w.gravity, 1, w.tol, w.max_iter, True, False)
+    for _ in range(5):
+        s.step(1e-3)
+        o.step(1e-3)
+    x, v = s.state()
+    xo, vo = o.state()
+    assert rel(x, xo) <= 1e-12 and rel(v, vo) <= 1e-12
+
+
+def test_iteration_cap_accepts_unconverged_solution():
+    # SolverData accepts a finite unconverged CG result (PointConnectionTests.cpp:2480-2510)
+    w = W.beam10(n=3)
+    w.max_iter = 1
+    s, o = run_pair(w, N.PK_FP64, 2, 1e-9)
+    it, res, conv, st = s.results()
+    assert it[0] == 1 and not conv[0] and st[0] == N.GROUP_INTEGRATED
+
+
+def test_nonfinite_velocity_fails_the_solve_and_keeps_state():
+    w = W.beam10(n=3)
+    s = W.make_system(w)
+    v = np.zeros_like(w.rest)
+    v[41] = [np.nan, 0, 0]  # a free node (i = 1)
+    s.set_state(w.rest, v)
+    s.step(w.dt)
+    it, res, conv, st = s.results()
+    assert st[0] == N.GROUP_SOLVE_FAILED and it[0] == 0 and not conv[0]
+    x, vv = s.state()
+    assert np.array_equal(x, w.rest)
+
+
+def test_free_node_without_mass_raises():
+    rest = np.vstack([np.eye(4, 3), [[5.0, 5.0, 5.0]]])
+    s = System(rest, [[0, 1, 2, 3]], 100.0, 0.3, 0.0, 1.0)
+    with pytest.raises(RuntimeError, match="no physical mass"):
+        s.step(0.01)
+
+
+def test_twist64_full_size_properties():
+    """The benchmark mesh at full size: fixed face stays put, the solve meets
+    its stopping rule on the true residual, and a second run is identical."""
+    w = W.twist64()
+    s = W.make_system(w)
+    s.step(w.dt)
+    it, res, conv, st = s.results()
+    assert conv[0] and st[0] == N.GROUP_INTEGRATED and 0 < it[0] < w.max_iter
+    x, v = s.state()
+    assert np.array_equal(x[w.fixed == 1], w.x0[w.fixed == 1])
+    assert not v[w.fixed == 1].any()
+    # A dv = b up to the CG tolerance, with A and b from the same assembly
+    s2 = W.make_system(w)
+    s2.assemble(w.dt)
+    vals, b, minv = s2.matrix()
+    dv = (v - w.v0)[w.fixed == 0]
+    r = b - s2.spmv(dv)
+    z = np.einsum("nij,nj->ni", minv, r)
+    z0 = np.einsum("nij,nj->ni", minv, b)
+    assert np.sqrt(np.sum(r * z) / np.sum(b * z0)) <= 2 * w.tol
