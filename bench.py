@@ -1,13 +1,13 @@
 """Benchmark of the implicit-Euler soft-body step on one MI355X.
 
     python bench.py [--steps K] [--warmup W] [--workload twist64|beam10|batch256]
-                    [--precision fp64|fp32] [--no-cpu-baseline]
+                    [--precision fp64|fp32] [--no-cpu-baseline] [--no-autotune | --shape L,R,V,U]
 
 A step is one implicit-Euler frame (element corotation, assembly, block-Jacobi
 PCG to rtol 1e-6, integration) over the whole workload, with every input
 resident in GPU memory. Prints one JSON line: tet-steps/s over exactly K timed
 steps after W untimed ones, the roofline of the dominant kernel (the CG SpMV)
-from HIP events on the system's stream over the timed region, and a CPU
+from the kernel's own wall-clock span over the timed region, and a CPU
 baseline: the reference's own FP32 step (its sources compiled in place) timed
 on one host core over a bounded sample of the same workload.
 """
@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default CG launch shape")
+    ap.add_argument("--shape", default=None, help="CG launch shape lanes,spmv_rows,vector_rows,unroll (no autotune)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the batched and single-body "
@@ -135,6 +137,12 @@ def main():
     s = W.make_system(w, prec)
     for _ in range(args.warmup):
         s.step(w.dt)
+    s.set_state(w.x0, w.v0)
+    if args.shape:
+        s.tune(*[int(a) for a in args.shape.split(",")])
+        shape = s.get_tune()
+    else:
+        shape = s.get_tune() if args.no_autotune else s.autotune(w.dt)
     s.set_state(w.x0, w.v0)  # the timed run starts from the workload's initial state
     s.profile(True)
     t0 = time.perf_counter()
@@ -169,6 +177,7 @@ def main():
             "tets": w.n_tets, "nodes": w.n_nodes, "bodies": w.n_bodies, "dynamic_rows": s.nd,
             "nnzb_full": s.nnzb_full, "nnzb_upper": s.nnzb_upper,
             "cg_iterations_last": int(its.max()), "precision": args.precision,
+            "cg_launch_shape": dict(zip(("lanes_per_row", "spmv_rows", "vector_rows", "unroll"), shape)),
             "parallelism": "single GPU",
         },
         "roofline": {

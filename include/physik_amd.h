@@ -104,10 +104,13 @@ int pk_system_sizes(const pk_system* sys, int* nodes, int* tets, int* groups, in
                     long long* nnzb_upper, long long* nnzb_full);
 
 /* Launch shape of the CG kernels: lanes per BSR row in the SpMV (4, 8, 16 or
- * 32), rows per SpMV workgroup and rows per vector-kernel workgroup. Results
- * of a body do not depend on the other bodies of the system, only on these
- * three numbers. Chosen from the row lengths at creation; tuning hook. */
-int pk_system_tune(pk_system* sys, int lanes_per_row, int spmv_rows, int vector_rows);
+ * 32), rows per SpMV workgroup, rows per vector-kernel workgroup, and rows
+ * each SpMV lane group loads before reducing (1, 2, or 4 up to 16 lanes).
+ * Results of a body do not depend on the other bodies of the system, only on
+ * these numbers. A default is chosen from the row lengths at creation; the
+ * autotuner (physik_amd/autotune.py) searches them. */
+int pk_system_tune(pk_system* sys, int lanes_per_row, int spmv_rows, int vector_rows, int unroll);
+int pk_system_get_tune(const pk_system* sys, int* lanes_per_row, int* spmv_rows, int* vector_rows, int* unroll);
 
 /* Node state, node_count x 3 each; either pointer may be NULL. */
 int pk_system_set_state(pk_system* sys, const double* x, const double* v);
@@ -131,9 +134,11 @@ int pk_system_step(pk_system* sys, double dt, int substeps, int frames);
 int pk_system_group_results(pk_system* sys, int* iterations, double* residual_norm, int* converged,
                             int* status);
 
-/* Kernel timing with HIP events on the system's stream: enable, reset, read.
- * Only launches that did work are counted (CG launches after a group has
- * converged are excluded). */
+/* Kernel timing on the system's stream: enable, reset, read. The CG kernels
+ * (SpMV, update) are timed from the wall clock read inside the kernel (first
+ * workgroup start to last workgroup end, the span a kernel trace reports);
+ * the others with HIP events around the launch. Only CG launches that did
+ * work are counted (those after every body stopped are excluded). */
 int pk_system_profile(pk_system* sys, int enable);
 int pk_system_kernel_time(pk_system* sys, int kernel, long long* launches, double* total_ms);
 
@@ -156,6 +161,13 @@ int pk_system_get_elements(pk_system* sys, double* blocks, double* forces, doubl
 int pk_system_spmv(pk_system* sys, const double* x, double* y);
 /* Solution dv of the last solve (dynamic_rows x 3). */
 int pk_system_get_solution(pk_system* sys, double* dv);
+
+/* ---- diagnostics ---- */
+
+/* Copies `bytes` (a multiple of 32) between two fresh device buffers `reps`
+ * times with a plain streaming kernel and returns the average time per copy.
+ * Used to calibrate the HBM byte counters of the profiler. */
+int pk_diag_copy(long long bytes, int reps, double* avg_ms);
 
 /* ---- host-only topology (no GPU needed) ---- */
 

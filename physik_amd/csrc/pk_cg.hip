@@ -16,6 +16,11 @@
 // needed inside a launch. One designated workgroup per body records the
 // decisions (delta, stop, iterations, residual) for later launches and the
 // host; the host polls the count of bodies still iterating.
+//
+// SpMV layout: block values are stored as 9 planes (A_ij of every block in
+// one contiguous plane), so LPR lanes of a row read consecutive blocks with
+// fully coalesced 8-byte (or 4-byte) loads; U rows per lane group are loaded
+// before any is reduced, to keep more bytes in flight per wave.
 #include "pk_device.h"
 
 namespace pk
@@ -46,37 +51,36 @@ __device__ inline int read_uniform(const int* p)
     return v;
 }
 
+// Wall-clock ticks at the start and end of each workgroup, one slot pair per
+// workgroup (plain stores, no contention), when profiling.
+__device__ inline void stamp_begin(unsigned long long* stamp)
+{
+    if (stamp && threadIdx.x == 0)
+        stamp[2 * blockIdx.x] = static_cast<unsigned long long>(wall_clock64());
+}
+__device__ inline void stamp_end(unsigned long long* stamp)
+{
+    if (stamp && threadIdx.x == 0)
+        stamp[2 * blockIdx.x + 1] = static_cast<unsigned long long>(wall_clock64());
+}
+
 template <class M>
 __device__ inline M direction(M z, M dprev, double beta)
 {
     return static_cast<M>(__builtin_fma(beta, static_cast<double>(dprev), static_cast<double>(z)));
 }
 
-template <class M, int LPR>
-__global__ __launch_bounds__(NT) void k_spmv(int k, TileView st, TileView vt, const int* __restrict__ fstart,
-                                             const int* __restrict__ fcol, const M* __restrict__ vals,
-                                             long long nnzf, const M* __restrict__ dprev, M* __restrict__ dcur,
-                                             const M* __restrict__ z, M* __restrict__ q, double* __restrict__ pa,
-                                             const Partial* __restrict__ pbPrev, GroupState* gs, int* active)
+// The reference's stop rules for the end of iteration k-1 (k = 0: the CG
+// start, ConjugateGradientSolver.cpp:151-176; k > 0: :206-231 and the loop
+// test at :180). Identical in every workgroup of the body. Returns true when
+// the loop is over; otherwise beta is set for k > 0.
+__device__ inline bool stop_rules(int k, GroupState& s, double rz, double rr, int flags, bool designated, int* active,
+                                  double& beta)
 {
-    const int t = tile_of(st);
-    if (t < 0)
-        return;
-    const RowTile tl = st.tiles[t];
-    const int g = tl.group;
-    GroupState& s = gs[g];
-    if (read_uniform(&s.stopAt) >= 0)
-        return;
-    double rz, rr;
-    int flags;
-    sum_partials(pbPrev, vt.groupBegin[g], vt.groupCount[g], rz, rr, flags);
-    const bool designated = t == st.groupBegin[g] && threadIdx.x == 0;
-
-    // Stop decisions for the end of iteration k-1 (k = 0: the CG start).
-    double beta = 0.0;
     bool stop = false, conv = false, failed = false;
     double resid = 0.0;
     int iters = k;
+    beta = 0.0;
     if (k == 0)
     {
         double target = 0.0;
@@ -128,81 +132,129 @@ __global__ __launch_bounds__(NT) void k_spmv(int k, TileView st, TileView vt, co
         if (designated && !stop)
             s.delta[k & 1] = rz;
     }
-    if (stop)
-    {
-        if (designated)
-            record_stop(s, active, k, 0, iters, conv, resid, failed);
+    if (stop && designated)
+        record_stop(s, active, k, 0, iters, conv, resid, failed);
+    return stop;
+}
+
+template <class M, int LPR, int U>
+__device__ inline void spmv_body(int k, const TileView& st, const TileView& vt, const int* __restrict__ fstart,
+                                 const int* __restrict__ fcol, const M* __restrict__ vals, long long nnzf,
+                                 const M* __restrict__ dprev, M* __restrict__ dcur, const M* __restrict__ z,
+                                 M* __restrict__ q, double* __restrict__ pa, const Partial* __restrict__ pbPrev,
+                                 GroupState* gs, int* active)
+{
+    const int t = tile_of(st);
+    if (t < 0)
         return;
-    }
+    const RowTile tl = st.tiles[t];
+    const int g = tl.group;
+    GroupState& s = gs[g];
+    if (read_uniform(&s.stopAt) >= 0)
+        return;
+    double rz, rr;
+    int flags;
+    sum_partials(pbPrev, vt.groupBegin[g], vt.groupCount[g], rz, rr, flags);
+    double beta;
+    if (stop_rules(k, s, rz, rr, flags, t == st.groupBegin[g] && threadIdx.x == 0, active, beta))
+        return;
 
     const int lane = threadIdx.x % LPR;
     const int sub = threadIdx.x / LPR;
     constexpr int RPP = NT / LPR;
     const size_t F = static_cast<size_t>(nnzf);
+    // d_k of column c: stored at k = 0, formed from z and d_{k-1} after
+    auto dir = [&](int c, double* o) {
+        if (k == 0)
+        {
+            o[0] = dcur[3 * c];
+            o[1] = dcur[3 * c + 1];
+            o[2] = dcur[3 * c + 2];
+        }
+        else
+        {
+            o[0] = direction(z[3 * c], dprev[3 * c], beta);
+            o[1] = direction(z[3 * c + 1], dprev[3 * c + 1], beta);
+            o[2] = direction(z[3 * c + 2], dprev[3 * c + 2], beta);
+        }
+    };
     double pd = 0.0;
-    for (int row = tl.rowBegin + sub; row < tl.rowEnd; row += RPP)
+    for (int row0 = tl.rowBegin + sub; row0 < tl.rowEnd; row0 += RPP * U)
     {
-        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-        const int k1 = fstart[row + 1];
-        for (int kk = fstart[row] + lane; kk < k1; kk += LPR)
-        {
-            const int c = fcol[kk];
-            M m0, m1, m2;
-            if (k == 0)
-            {
-                m0 = dcur[3 * c];
-                m1 = dcur[3 * c + 1];
-                m2 = dcur[3 * c + 2];
-            }
-            else
-            {
-                m0 = direction(z[3 * c], dprev[3 * c], beta);
-                m1 = direction(z[3 * c + 1], dprev[3 * c + 1], beta);
-                m2 = direction(z[3 * c + 2], dprev[3 * c + 2], beta);
-            }
-            const double d0 = m0, d1 = m1, d2 = m2;
-            const M* a = vals + kk;
-            y0 += static_cast<double>(__builtin_nontemporal_load(a)) * d0 +
-                  static_cast<double>(__builtin_nontemporal_load(a + F)) * d1 +
-                  static_cast<double>(__builtin_nontemporal_load(a + 2 * F)) * d2;
-            y1 += static_cast<double>(__builtin_nontemporal_load(a + 3 * F)) * d0 +
-                  static_cast<double>(__builtin_nontemporal_load(a + 4 * F)) * d1 +
-                  static_cast<double>(__builtin_nontemporal_load(a + 5 * F)) * d2;
-            y2 += static_cast<double>(__builtin_nontemporal_load(a + 6 * F)) * d0 +
-                  static_cast<double>(__builtin_nontemporal_load(a + 7 * F)) * d1 +
-                  static_cast<double>(__builtin_nontemporal_load(a + 8 * F)) * d2;
-        }
+        int kb[U], ke[U];
 #pragma unroll
-        for (int o = LPR / 2; o > 0; o >>= 1)
+        for (int u = 0; u < U; ++u)
         {
-            y0 += __shfl_xor(y0, o, LPR);
-            y1 += __shfl_xor(y1, o, LPR);
-            y2 += __shfl_xor(y2, o, LPR);
+            const int row = row0 + u * RPP;
+            const bool live = row < tl.rowEnd;
+            kb[u] = live ? fstart[row] + lane : 0;
+            ke[u] = live ? fstart[row + 1] : 0;
         }
-        if (lane == 0)
+        // first block of every row: all loads issued before any use
+        int col[U];
+        M a[U][9];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
         {
-            M r0, r1, r2;
-            if (k == 0)
+            const int idx = kb[u] < ke[u] ? kb[u] : 0;
+            col[u] = fcol[idx];
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                a[u][e] = __builtin_nontemporal_load(vals + e * F + idx);
+        }
+        double y[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            double dc[3];
+            dir(col[u], dc);
+            const bool has = kb[u] < ke[u];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                y[u][i] = has ? static_cast<double>(a[u][3 * i]) * dc[0] + static_cast<double>(a[u][3 * i + 1]) * dc[1] +
+                                    static_cast<double>(a[u][3 * i + 2]) * dc[2]
+                              : 0.0;
+        }
+        // rows longer than LPR blocks
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            for (int kk = kb[u] + LPR; kk < ke[u]; kk += LPR)
             {
-                r0 = dcur[3 * row];
-                r1 = dcur[3 * row + 1];
-                r2 = dcur[3 * row + 2];
+                double dc[3];
+                dir(fcol[kk], dc);
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    y[u][i] += static_cast<double>(__builtin_nontemporal_load(vals + (3 * i) * F + kk)) * dc[0] +
+                               static_cast<double>(__builtin_nontemporal_load(vals + (3 * i + 1) * F + kk)) * dc[1] +
+                               static_cast<double>(__builtin_nontemporal_load(vals + (3 * i + 2) * F + kk)) * dc[2];
             }
-            else
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+#pragma unroll
+            for (int o = LPR / 2; o > 0; o >>= 1)
             {
-                r0 = direction(z[3 * row], dprev[3 * row], beta);
-                r1 = direction(z[3 * row + 1], dprev[3 * row + 1], beta);
-                r2 = direction(z[3 * row + 2], dprev[3 * row + 2], beta);
-                dcur[3 * row] = r0;
-                dcur[3 * row + 1] = r1;
-                dcur[3 * row + 2] = r2;
+                y[u][0] += __shfl_xor(y[u][0], o, LPR);
+                y[u][1] += __shfl_xor(y[u][1], o, LPR);
+                y[u][2] += __shfl_xor(y[u][2], o, LPR);
             }
-            const M q0 = static_cast<M>(y0), q1 = static_cast<M>(y1), q2 = static_cast<M>(y2);
-            q[3 * row] = q0;
-            q[3 * row + 1] = q1;
-            q[3 * row + 2] = q2;
-            pd += static_cast<double>(r0) * static_cast<double>(q0) + static_cast<double>(r1) * static_cast<double>(q1) +
-                  static_cast<double>(r2) * static_cast<double>(q2);
+            const int row = row0 + u * RPP;
+            if (lane == 0 && row < tl.rowEnd)
+            {
+                double dr[3];
+                dir(row, dr);
+                if (k > 0)
+                {
+                    dcur[3 * row] = static_cast<M>(dr[0]);
+                    dcur[3 * row + 1] = static_cast<M>(dr[1]);
+                    dcur[3 * row + 2] = static_cast<M>(dr[2]);
+                }
+                const M q0 = static_cast<M>(y[u][0]), q1 = static_cast<M>(y[u][1]), q2 = static_cast<M>(y[u][2]);
+                q[3 * row] = q0;
+                q[3 * row + 1] = q1;
+                q[3 * row + 2] = q2;
+                pd += dr[0] * static_cast<double>(q0) + dr[1] * static_cast<double>(q1) + dr[2] * static_cast<double>(q2);
+            }
         }
     }
     double u1 = 0.0, u2 = 0.0;
@@ -211,12 +263,25 @@ __global__ __launch_bounds__(NT) void k_spmv(int k, TileView st, TileView vt, co
         pa[t] = pd;
 }
 
+template <class M, int LPR, int U>
+__global__ __launch_bounds__(NT) void k_spmv(int k, TileView st, TileView vt, const int* __restrict__ fstart,
+                                             const int* __restrict__ fcol, const M* __restrict__ vals,
+                                             long long nnzf, const M* __restrict__ dprev, M* __restrict__ dcur,
+                                             const M* __restrict__ z, M* __restrict__ q, double* __restrict__ pa,
+                                             const Partial* __restrict__ pbPrev, GroupState* gs, int* active,
+                                             unsigned long long* stamp)
+{
+    stamp_begin(stamp);
+    spmv_body<M, LPR, U>(k, st, vt, fstart, fcol, vals, nnzf, dprev, dcur, z, q, pa, pbPrev, gs, active);
+    stamp_end(stamp);
+}
+
 template <class M>
-__global__ __launch_bounds__(NT) void k_update(int k, TileView vt, TileView st, const double* __restrict__ minv,
-                                               int nd, const M* __restrict__ dcur, const M* __restrict__ q,
-                                               double* __restrict__ xs, double* __restrict__ rv, M* __restrict__ zv,
-                                               const double* __restrict__ pa, Partial* __restrict__ pbCur,
-                                               const Partial* __restrict__ pbPrev, GroupState* gs, int* active)
+__device__ inline void update_body(int k, const TileView& vt, const TileView& st, const double* __restrict__ minv,
+                                   int nd, const M* __restrict__ dcur, const M* __restrict__ q,
+                                   double* __restrict__ xs, double* __restrict__ rv, M* __restrict__ zv,
+                                   const double* __restrict__ pa, Partial* __restrict__ pbCur,
+                                   const Partial* __restrict__ pbPrev, GroupState* gs, int* active)
 {
     const int t = tile_of(vt);
     if (t < 0)
@@ -277,6 +342,51 @@ __global__ __launch_bounds__(NT) void k_update(int k, TileView vt, TileView st, 
 }
 
 template <class M>
+__global__ __launch_bounds__(NT) void k_update(int k, TileView vt, TileView st, const double* __restrict__ minv,
+                                               int nd, const M* __restrict__ dcur, const M* __restrict__ q,
+                                               double* __restrict__ xs, double* __restrict__ rv, M* __restrict__ zv,
+                                               const double* __restrict__ pa, Partial* __restrict__ pbCur,
+                                               const Partial* __restrict__ pbPrev, GroupState* gs, int* active,
+                                               unsigned long long* stamp)
+{
+    stamp_begin(stamp);
+    update_body<M>(k, vt, st, minv, nd, dcur, q, xs, rv, zv, pa, pbCur, pbPrev, gs, active);
+    stamp_end(stamp);
+}
+
+// Span of each launch: earliest workgroup start to latest workgroup end.
+// Workgroup l reduces launch l (grid slot pairs at raw + 2 * grid * l).
+__global__ __launch_bounds__(NT) void k_stamp_reduce(const unsigned long long* __restrict__ raw, int grid,
+                                                     unsigned long long* __restrict__ span)
+{
+    const unsigned long long* p = raw + 2 * static_cast<size_t>(grid) * blockIdx.x;
+    unsigned long long lo = ~0ull, hi = 0ull;
+    for (int i = threadIdx.x; i < grid; i += NT)
+    {
+        lo = min(lo, p[2 * i]);
+        hi = max(hi, p[2 * i + 1]);
+    }
+    __shared__ unsigned long long sl[NT], sh[NT];
+    sl[threadIdx.x] = lo;
+    sh[threadIdx.x] = hi;
+    __syncthreads();
+    for (int o = NT / 2; o > 0; o >>= 1)
+    {
+        if (static_cast<int>(threadIdx.x) < o)
+        {
+            sl[threadIdx.x] = min(sl[threadIdx.x], sl[threadIdx.x + o]);
+            sh[threadIdx.x] = max(sh[threadIdx.x], sh[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        span[2 * blockIdx.x] = sl[0];
+        span[2 * blockIdx.x + 1] = sh[0];
+    }
+}
+
+template <class M>
 __global__ __launch_bounds__(NT) void k_spmv_plain(int nd, const int* __restrict__ fstart,
                                                    const int* __restrict__ fcol, const M* __restrict__ vals,
                                                    long long nnzf, const double* __restrict__ xin,
@@ -301,39 +411,54 @@ __global__ __launch_bounds__(NT) void k_spmv_plain(int nd, const int* __restrict
         y[3 * row + i] = acc[i];
 }
 
-template <class M>
-void launch_spmv(const DeviceView& v, int k, int lanesPerRow)
+template <class M, int LPR, int U>
+static void launch_spmv_shape(const DeviceView& v, int k, unsigned long long* stamp)
 {
-    if (v.spmv.count == 0)
-        return;
-    const dim3 grid(8u * static_cast<unsigned>(v.spmv.per)), block(NT);
-    const M* dprev = static_cast<const M*>(v.d[(k + 1) & 1]);
-    M* dcur = static_cast<M*>(v.d[k & 1]);
-    const M* z = static_cast<const M*>(v.z);
-    M* q = static_cast<M*>(v.q);
-    const M* vals = static_cast<const M*>(v.vals);
-    const Partial* pbPrev = v.pb[(k + 1) & 1];
-#define PK_SPMV(L)                                                                                              \
-    hipLaunchKernelGGL((k_spmv<M, L>), grid, block, 0, v.stream, k, v.spmv, v.vec, v.fstart, v.fcol, vals,     \
-                       v.nnzf, dprev, dcur, z, q, v.pa, pbPrev, v.gs, v.active)
-    switch (lanesPerRow)
-    {
-    case 4: PK_SPMV(4); break;
-    case 8: PK_SPMV(8); break;
-    case 32: PK_SPMV(32); break;
-    default: PK_SPMV(16); break;
-    }
-#undef PK_SPMV
+    hipLaunchKernelGGL((k_spmv<M, LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), 0, v.stream, k,
+                       v.spmv, v.vec, v.fstart, v.fcol, static_cast<const M*>(v.vals), v.nnzf,
+                       static_cast<const M*>(v.d[(k + 1) & 1]), static_cast<M*>(v.d[k & 1]),
+                       static_cast<const M*>(v.z), static_cast<M*>(v.q), v.pa, v.pb[(k + 1) & 1], v.gs, v.active,
+                       stamp);
 }
 
 template <class M>
-void launch_update(const DeviceView& v, int k)
+void launch_spmv(const DeviceView& v, int k, int lanesPerRow, int unroll, unsigned long long* stamp)
+{
+    if (v.spmv.count == 0)
+        return;
+#define PK_SHAPE(L, U)                        \
+    if (lanesPerRow == L && unroll == U)      \
+    {                                         \
+        launch_spmv_shape<M, L, U>(v, k, stamp); \
+        return;                               \
+    }
+    PK_SHAPE(4, 1) PK_SHAPE(4, 2) PK_SHAPE(4, 4)
+    PK_SHAPE(8, 1) PK_SHAPE(8, 2) PK_SHAPE(8, 4)
+    PK_SHAPE(16, 1) PK_SHAPE(16, 2) PK_SHAPE(16, 4)
+    PK_SHAPE(32, 1) PK_SHAPE(32, 2)
+#undef PK_SHAPE
+    launch_spmv_shape<M, 16, 1>(v, k, stamp);
+}
+
+template <class M>
+void launch_update(const DeviceView& v, int k, unsigned long long* stamp)
 {
     if (v.vec.count == 0)
         return;
     hipLaunchKernelGGL((k_update<M>), dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, k, v.vec,
                        v.spmv, v.minv, v.nd, static_cast<const M*>(v.d[k & 1]), static_cast<const M*>(v.q), v.xs,
-                       v.r, static_cast<M*>(v.z), v.pa, v.pb[k & 1], v.pb[(k + 1) & 1], v.gs, v.active);
+                       v.r, static_cast<M*>(v.z), v.pa, v.pb[k & 1], v.pb[(k + 1) & 1], v.gs, v.active, stamp);
+}
+
+int spmv_grid(const DeviceView& v) { return 8 * v.spmv.per; }
+int update_grid(const DeviceView& v) { return 8 * v.vec.per; }
+
+void launch_stamp_reduce(hipStream_t stream, const unsigned long long* raw, int grid, int launches,
+                         unsigned long long* span)
+{
+    if (launches > 0 && grid > 0)
+        hipLaunchKernelGGL(k_stamp_reduce, dim3(static_cast<unsigned>(launches)), dim3(NT), 0, stream, raw, grid,
+                           span);
 }
 
 template <class M>
@@ -344,10 +469,10 @@ void launch_spmv_plain(const DeviceView& v, const double* x, double* y)
                            v.fcol, static_cast<const M*>(v.vals), v.nnzf, x, y);
 }
 
-template void launch_spmv<double>(const DeviceView&, int, int);
-template void launch_spmv<float>(const DeviceView&, int, int);
-template void launch_update<double>(const DeviceView&, int);
-template void launch_update<float>(const DeviceView&, int);
+template void launch_spmv<double>(const DeviceView&, int, int, int, unsigned long long*);
+template void launch_spmv<float>(const DeviceView&, int, int, int, unsigned long long*);
+template void launch_update<double>(const DeviceView&, int, unsigned long long*);
+template void launch_update<float>(const DeviceView&, int, unsigned long long*);
 template void launch_spmv_plain<double>(const DeviceView&, const double*, double*);
 template void launch_spmv_plain<float>(const DeviceView&, const double*, double*);
 

@@ -218,8 +218,16 @@ template <class M> void launch_assemble(const DeviceView& v, double dt);
 template <class M> void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt);
 void launch_integrate(const DeviceView& v, double dt, int commit);
 void launch_explicit(const DeviceView& v, double gx, double gy, double gz, double dt);
-template <class M> void launch_spmv(const DeviceView& v, int k, int lanesPerRow);
-template <class M> void launch_update(const DeviceView& v, int k);
+template <class M>
+void launch_spmv(const DeviceView& v, int k, int lanesPerRow, int unroll, unsigned long long* stamp);
+template <class M> void launch_update(const DeviceView& v, int k, unsigned long long* stamp);
+// Profiling stamps: a CG launch given a stamp pointer writes (start, end)
+// wall-clock ticks of each of its grid workgroups there; the reduce turns
+// `launches` consecutive such blocks into one (start, end) pair each.
+int spmv_grid(const DeviceView& v);
+int update_grid(const DeviceView& v);
+void launch_stamp_reduce(hipStream_t stream, const unsigned long long* raw, int grid, int launches,
+                         unsigned long long* span);
 template <class M> void launch_spmv_plain(const DeviceView& v, const double* x, double* y);
 
 }  // namespace pk

@@ -106,8 +106,9 @@ struct pk_system
     pk::Topology topo;
     int precision = PK_FP64;
     int lanesPerRow = 16;
-    int spmvRows = 256;
+    int spmvRows = 128;
     int vecRows = 1024;
+    int unroll = 2;
     hipStream_t stream = nullptr;
 
     pk::DevBuf<int4> tets;
@@ -117,6 +118,10 @@ struct pk_system
     pk::DevBuf<char> vals, z, d0, d1, q;  // element type set by the precision
     pk::DevBuf<pk::Partial> pb0, pb1;
     pk::DevBuf<pk::GroupState> gs;
+    // profiling: per-workgroup (start, end) ticks of every CG launch, and
+    // the per-launch spans reduced from them
+    pk::DevBuf<unsigned long long> rawSpmv, rawUpd, stampSpmv, stampUpd;
+    double wallClockKHz = 0.0;
     pk::Tiles vecTiles, spmvTiles;
     int* pinned = nullptr;
     hipEvent_t pollEv[2] = {nullptr, nullptr};
@@ -286,12 +291,36 @@ static void harvest_profile(pk_system* s)
             spmvWork = std::max(spmvWork, g.stopAt + g.stopInUpdate);
             updWork = std::max(updWork, g.stopAt);
         }
+    // CG kernels: span between the first workgroup's start and the last one's
+    // end, from the wall clock read inside the kernels
+    if (s->stampSpmv.p && s->wallClockKHz > 0.0)
+    {
+        const int n = std::max(spmvWork, updWork);
+        std::vector<unsigned long long> a(2 * static_cast<size_t>(n)), b(2 * static_cast<size_t>(n));
+        if (n > 0 &&
+            hipMemcpy(a.data(), s->stampSpmv.p, a.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) ==
+                hipSuccess &&
+            hipMemcpy(b.data(), s->stampUpd.p, b.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) ==
+                hipSuccess)
+        {
+            for (int k = 0; k < n; ++k)
+            {
+                // launches k < spmvWork (updWork) all ran, so their spans are set
+                if (k < spmvWork && a[2 * k + 1] > a[2 * k])
+                {
+                    p.ms[PK_K_SPMV] += static_cast<double>(a[2 * k + 1] - a[2 * k]) / s->wallClockKHz;
+                    p.launches[PK_K_SPMV] += 1;
+                }
+                if (k < updWork && b[2 * k + 1] > b[2 * k])
+                {
+                    p.ms[PK_K_UPDATE] += static_cast<double>(b[2 * k + 1] - b[2 * k]) / s->wallClockKHz;
+                    p.launches[PK_K_UPDATE] += 1;
+                }
+            }
+        }
+    }
     for (const auto& rec : p.recs)
     {
-        if (rec.kernel == PK_K_SPMV && rec.iter >= spmvWork)
-            continue;
-        if (rec.kernel == PK_K_UPDATE && rec.iter >= updWork)
-            continue;
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, rec.a, rec.b) == hipSuccess)
         {
@@ -331,6 +360,23 @@ static int run_cg(pk_system* s)
 {
     const DeviceView dv = s->view();
     const int maxIt = std::max(0, s->maxIt);
+    const int gS = spmv_grid(dv), gU = update_grid(dv);
+    unsigned long long* stS = nullptr;
+    unsigned long long* stU = nullptr;
+    if (s->prof.on)
+    {
+        const size_t launches = static_cast<size_t>(maxIt + 1);
+        if (s->rawSpmv.n < launches * 2 * gS || s->rawUpd.n < launches * 2 * gU || s->stampSpmv.n < 2 * launches)
+        {
+            int rc;
+            if ((rc = s->rawSpmv.alloc(launches * 2 * gS)) != PK_OK) return rc;
+            if ((rc = s->rawUpd.alloc(launches * 2 * gU)) != PK_OK) return rc;
+            if ((rc = s->stampSpmv.alloc(2 * launches)) != PK_OK) return rc;
+            if ((rc = s->stampUpd.alloc(2 * launches)) != PK_OK) return rc;
+        }
+        stS = s->rawSpmv.p;
+        stU = s->rawUpd.p;
+    }
     // launch k runs the SpMV of iteration k (and decides the end of k-1);
     // k = maxIt only takes that decision
     int k = 0, poll = 0;
@@ -340,15 +386,9 @@ static int run_cg(pk_system* s)
         const int end = std::min(maxIt + 1, k + chunk);
         for (; k < end; ++k)
         {
-            {
-                Bracket br(s, PK_K_SPMV, k);
-                launch_spmv<M>(dv, k, s->lanesPerRow);
-            }
+            launch_spmv<M>(dv, k, s->lanesPerRow, s->unroll, stS ? stS + 2 * static_cast<size_t>(gS) * k : nullptr);
             if (k < maxIt)
-            {
-                Bracket br(s, PK_K_UPDATE, k);
-                launch_update<M>(dv, k);
-            }
+                launch_update<M>(dv, k, stU ? stU + 2 * static_cast<size_t>(gU) * k : nullptr);
         }
         chunk = 4;
         const int slot = poll & 1;
@@ -361,6 +401,11 @@ static int run_cg(pk_system* s)
                 break;
         }
         ++poll;
+    }
+    if (stS)
+    {
+        launch_stamp_reduce(s->stream, stS, gS, k, s->stampSpmv.p);
+        launch_stamp_reduce(s->stream, stU, gU, std::min(k, maxIt), s->stampUpd.p);
     }
     PK_HIP(hipGetLastError());
     return PK_OK;
@@ -529,6 +574,12 @@ int pk_system_create(const pk_mesh_desc* desc, int precision, pk_system** out)
         return rc;
     }
     PK_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    {
+        int dev = 0, khz = 0;
+        PK_HIP(hipGetDevice(&dev));
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess)
+            s->wallClockKHz = khz;
+    }
     PK_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->pinned), 2 * sizeof(int), hipHostMallocDefault));
     PK_HIP(hipEventCreateWithFlags(&s->pollEv[0], hipEventDisableTiming));
     PK_HIP(hipEventCreateWithFlags(&s->pollEv[1], hipEventDisableTiming));
@@ -558,18 +609,32 @@ int pk_system_sizes(const pk_system* s, int* nodes, int* tets, int* groups, int*
     return PK_OK;
 }
 
-int pk_system_tune(pk_system* s, int lanes_per_row, int spmv_rows, int vector_rows)
+int pk_system_tune(pk_system* s, int lanes_per_row, int spmv_rows, int vector_rows, int unroll)
 {
-    if (!s || (lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16 && lanes_per_row != 32) ||
-        spmv_rows < 1 || vector_rows < 1)
+    const bool lanesOk = lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16 || lanes_per_row == 32;
+    const bool unrollOk = unroll == 1 || unroll == 2 || (unroll == 4 && lanes_per_row <= 16);
+    if (!s || !lanesOk || !unrollOk || spmv_rows < 1 || vector_rows < 1)
     {
-        set_error("pk_system_tune: lanes_per_row must be 4, 8, 16 or 32 and row counts positive");
+        set_error("pk_system_tune: lanes_per_row in {4, 8, 16, 32}, unroll in {1, 2, 4} (4 only up to 16 lanes), "
+                  "positive row counts");
         return PK_ERR_ARG;
     }
     s->lanesPerRow = lanes_per_row;
     s->spmvRows = spmv_rows;
     s->vecRows = vector_rows;
+    s->unroll = unroll;
     return setup_tiles(s);
+}
+
+int pk_system_get_tune(const pk_system* s, int* lanes_per_row, int* spmv_rows, int* vector_rows, int* unroll)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    if (lanes_per_row) *lanes_per_row = s->lanesPerRow;
+    if (spmv_rows) *spmv_rows = s->spmvRows;
+    if (vector_rows) *vector_rows = s->vecRows;
+    if (unroll) *unroll = s->unroll;
+    return PK_OK;
 }
 
 int pk_system_set_state(pk_system* s, const double* x, const double* v)

@@ -1,0 +1,47 @@
+"""Profiling harness: a calibration copy of known size, then CG work.
+
+    python -m physik_amd.diag [--copy-mb 512] [--workload twist64] [--steps 1]
+                              [--precision fp64|fp32] [--shape L,R,V,U]
+
+Run under ``rocprofv3 --pmc FETCH_SIZE`` (and again with WRITE_SIZE): the copy
+kernel moves exactly copy-mb MiB each way, which calibrates the counters; the
+implicit steps that follow give the per-dispatch counts of every step kernel.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+
+from . import native as N
+from . import workloads as W
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-mb", type=int, default=512)
+    ap.add_argument("--workload", default="twist64", choices=["twist64", "beam10", "batch256"])
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--shape", default=None,
+                    help="CG launch shape lanes,spmv_rows,vector_rows,unroll (default: the system's own)")
+    args = ap.parse_args()
+    lib = N.lib()
+    ms = C.c_double()
+    nbytes = args.copy_mb * (1 << 20)
+    N.check(lib.pk_diag_copy(nbytes, 3, C.byref(ms)), "pk_diag_copy")
+    w = getattr(W, args.workload)()
+    s = W.make_system(w, N.PK_FP64 if args.precision == "fp64" else N.PK_FP32_STORAGE)
+    if args.shape:
+        s.tune(*[int(a) for a in args.shape.split(",")])
+    s.step(w.dt, 1, args.steps)
+    it = s.results()[0]
+    print(json.dumps({"copy_bytes": nbytes, "copy_ms": ms.value, "copy_GBps": 2 * nbytes / ms.value / 1e6,
+                      "workload": args.workload, "steps": args.steps, "cg_iterations": int(it.max()),
+                      "nnzb_full": s.nnzb_full, "dynamic_rows": s.nd, "tets": s.t,
+                      "shape": s.get_tune()}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

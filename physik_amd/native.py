@@ -54,7 +54,9 @@ PK_SIGNATURES = {
     "pk_system_destroy": (None, [_vp]),
     "pk_system_sizes": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
-    "pk_system_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "pk_system_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "pk_system_get_tune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]),
     "pk_system_set_state": (C.c_int, [_vp, _vp, _vp]),
     "pk_system_get_state": (C.c_int, [_vp, _vp, _vp]),
     "pk_system_set_gravity": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),
@@ -70,6 +72,7 @@ PK_SIGNATURES = {
     "pk_system_get_elements": (C.c_int, [_vp, _vp, _vp, _vp]),
     "pk_system_spmv": (C.c_int, [_vp, _dp, _dp]),
     "pk_system_get_solution": (C.c_int, [_vp, _dp]),
+    "pk_diag_copy": (C.c_int, [C.c_longlong, C.c_int, C.POINTER(C.c_double)]),
     "pk_pattern_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(_vp)]),
     "pk_pattern_destroy": (None, [_vp]),
     "pk_pattern_sizes": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),

@@ -89,8 +89,18 @@ class System:
         except Exception:
             pass
 
-    def tune(self, lanes_per_row=16, spmv_rows=256, vector_rows=1024):
-        N.check(self.lib.pk_system_tune(self.h, lanes_per_row, spmv_rows, vector_rows), "pk_system_tune")
+    def tune(self, lanes_per_row=16, spmv_rows=128, vector_rows=1024, unroll=2):
+        N.check(self.lib.pk_system_tune(self.h, lanes_per_row, spmv_rows, vector_rows, unroll), "pk_system_tune")
+
+    def autotune(self, dt, cache=None, verbose=False):
+        """Measure and apply the fastest CG launch shape (physik_amd.autotune)."""
+        from .autotune import autotune
+        return autotune(self, dt, cache=cache, verbose=verbose)
+
+    def get_tune(self):
+        v = [C.c_int() for _ in range(4)]
+        N.check(self.lib.pk_system_get_tune(self.h, *[C.byref(x) for x in v]), "pk_system_get_tune")
+        return tuple(x.value for x in v)
 
     def set_state(self, x=None, v=None):
         xa = None if x is None else N.as_f64(x).reshape(-1)

@@ -238,3 +238,65 @@ def test_twist64_full_size_properties():
     z = np.einsum("nij,nj->ni", minv, r)
     z0 = np.einsum("nij,nj->ni", minv, b)
     assert np.sqrt(np.sum(r * z) / np.sum(b * z0)) <= 2 * w.tol
+
+
+SHAPES = [(l, u) for l in (4, 8, 16, 32) for u in (1, 2, 4) if u < 4 or l <= 16]
+
+
+@pytest.mark.parametrize("lanes,unroll", SHAPES)
+def test_every_spmv_launch_shape_matches_oracle(lanes, unroll):
+    w = W.twist64(n=8, h=0.1, steps=3)
+    print(f"shape lanes {lanes} unroll {unroll}", flush=True)
+    s = W.make_system(w)
+    s.tune(lanes, 32, 64, unroll)
+    assert s.get_tune() == (lanes, 32, 64, unroll)
+    o = oracle_for(w)
+    for k in range(w.steps):
+        s.step(w.dt)
+        o.step(w.dt)
+        xg, vg = s.state()
+        xo, vo = o.state()
+        assert rel(xg, xo) <= 1e-9 and rel(vg, vo) <= 1e-9, f"step {k}"
+        assert abs(int(s.results()[0][0]) - o.cg()[0]) <= 1
+
+
+def test_bad_launch_shape_is_rejected():
+    s = W.make_system(W.beam10(n=2))
+    with pytest.raises(RuntimeError):
+        s.tune(32, 64, 64, 4)
+    with pytest.raises(RuntimeError):
+        s.tune(12, 64, 64, 1)
+
+
+def test_kernel_spans_count_working_launches():
+    w = W.beam10()
+    s = W.make_system(w)
+    s.profile(True)
+    s.step(w.dt, 1, 2)
+    n_s, ms_s = s.kernel_time(N.K_SPMV)
+    n_u, ms_u = s.kernel_time(N.K_UPDATE)
+    s.profile(False)
+    it = int(s.results()[0][0])
+    assert ms_s > 0 and ms_u > 0
+    # two solves: every iteration's SpMV and update did work
+    assert n_s >= 2 * it - 2 and n_u >= 2 * it - 2 and n_s <= 2 * (it + 3)
+
+
+def test_autotune_keeps_state_and_caches(tmp_path):
+    w = W.beam10()
+    s = W.make_system(w)
+    s.step(w.dt)
+    x0, v0 = s.state()
+    cache = str(tmp_path / "shapes.json")
+    shape = s.autotune(w.dt, cache=cache)
+    assert s.get_tune() == tuple(shape)
+    x1, v1 = s.state()
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
+    s2 = W.make_system(w)
+    assert s2.autotune(w.dt, cache=cache) == tuple(shape)
+    # the tuned system still matches the oracle
+    o = oracle_for(w)
+    o.set_state(x0, v0)
+    s.step(w.dt)
+    o.step(w.dt)
+    assert rel(s.state()[0], o.state()[0]) <= 1e-9
