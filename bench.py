@@ -64,12 +64,15 @@ def spmv_bytes_plain(nnzf, nd, s):
 
 
 def measured_traffic(workload_name, precision):
+    """HBM bytes per SpMV launch from the committed counter passes
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size), or
+    None when no pass covers this workload and precision."""
     path = os.path.join(ROOT, "profiles", "spmv_traffic.json")
     try:
         with open(path) as f:
-            data = json.load(f)
-        return data.get(f"{workload_name}/{precision}")
-    except (OSError, ValueError):
+            entry = json.load(f).get(f"{workload_name}/{precision}")
+        return None if entry is None else float(entry["bytes"])
+    except (OSError, ValueError, KeyError, TypeError):
         return None
 
 

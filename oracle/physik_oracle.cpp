@@ -276,10 +276,13 @@ T frob_diff2(const M3<T>& a, const M3<T>& b)
 // FEMModel.cpp:115-163: Newton polar iteration R <- (R + R^-T)/2, <= 12
 // iterations, stop on ||dR||_F^2 <= 1e-10f, Identity fallbacks, column-2 flip
 // when det R < 0. MinPolarDeterminant = 1e-8f (:15).
+// `kind` (optional, for the adjoint) reports how R was obtained: 0 Newton
+// limit, 1 Newton limit with column 2 negated, 2 Identity fallback.
 template <class T>
-M3<T> polar(const M3<T>& F, int* iterations = nullptr)
+M3<T> polar(const M3<T>& F, int* iterations = nullptr, int* kind = nullptr)
 {
     if (iterations) *iterations = 0;
+    if (kind) *kind = 2;
     if (!finite(F) || std::abs(det(F)) <= T(1.0e-8f))
         return M3<T>{};
     M3<T> R = F;
@@ -301,8 +304,12 @@ M3<T> polar(const M3<T>& F, int* iterations = nullptr)
     }
     if (!finite(R))
         return M3<T>{};
+    if (kind) *kind = 0;
     if (det(R) < T(0))
+    {
         R.c[2] = scl(R.c[2], T(-1.0f));
+        if (kind) *kind = 1;
+    }
     return R;
 }
 
@@ -639,6 +646,7 @@ struct World
     std::vector<M3<T>> minv;
     std::vector<Element<T>> elems;  // per tet, last computed
     std::vector<uint8_t> elemValid;
+    std::vector<V3<T>> adjX, adjV;  // initial-state adjoints of the last adjoint_run
 
     struct Block
     {
@@ -930,6 +938,24 @@ struct World
         }
     }
 
+    // One implicit substep as step() runs it. Returns 1 on a mass error, else
+    // 0; `integrated` tells whether the state was updated.
+    int substep(T dt, bool& integrated)
+    {
+        integrated = false;
+        const int rc = precompute(dt);
+        if (rc == 1)
+            return 1;
+        if (rc != 0)
+        {
+            last = CgResult<T>{};
+            return 0;
+        }
+        if (solve())
+            integrated = integrate_implicit(dt);
+        return 0;
+    }
+
     // World.cpp:32-73. Returns 0, or 1 if a free node had no mass (the
     // reference throws std::runtime_error from here).
     int step(T frameDt)
@@ -1069,6 +1095,236 @@ int pcg_t(int n, int nnz, const int* rs, const int* ci, const double* vals, cons
     *converged = res.converged ? 1 : 0;
     return static_cast<int>(xv.size()) == n ? res.iterations : -1 - res.iterations;
 }
+}  // namespace oracle
+
+
+// ---------------------------------------------------------------------------
+// Reverse mode of the implicit-Euler substep. The reference has no adjoint;
+// this FP64 CPU version is the checker for the GPU adjoint and is itself
+// pinned by central finite differences of the forward map above.
+//
+// Substep (x, v) -> (x', v'): A(x) dv = b(x, v), v' = v + dv, x' = x + dt v'
+// on dynamic nodes (identity elsewhere, and everywhere when the substep did
+// not integrate), with
+//   A = M + dt^2 K(x),   b = dt f(x, v) - dt^2 K(x) v,
+//   f_a   = -V R sigma(R^T F - I) g_a - c v_a        per tet (FEMModel.cpp:377-424),
+//   (K_e w)_a = V R sigma(R^T W) g_a,  W = sum_b w_b g_b^T  (= sum_b R K_ab R^T w_b),
+//   sigma(H) = lambda tr(H) I + mu (H + H^T).
+// Given the adjoints (xb', vb') of (x', v'):
+//   u = vb' + dt xb' on dynamic nodes,  A lam = u  (A is symmetric),
+//   xb = xb' + dPhi/dx,  vb = (u on dynamic nodes, vb' elsewhere) + dPhi/dv,
+// where, with dv and v' held fixed, Phi = lam . (b - A dv) is the sum over
+// tets of
+//   Phi_e = -dt V <L, R sigma(R^T F - I)> - dt^2 V <L, R sigma(R^T W')> - dt c lam . v,
+//   L = sum_a lam_a g_a^T,  W' = sum_a v'_a g_a^T.
+// lambda and mu scale with E, so d Phi / d log E_e is the elastic part of
+// Phi_e. R = polar(F) is differentiated as the exact polar factor:
+// dR = R Omega with S Omega + Omega S = R^T dF - dF^T R, S = R^T F.
+// ---------------------------------------------------------------------------
+namespace oracle
+{
+
+template <class T> inline T trace3(const M3<T>& m) { return m.c[0].x + m.c[1].y + m.c[2].z; }
+
+template <class T> inline T frob(const M3<T>& a, const M3<T>& b)
+{
+    return dot(a.c[0], b.c[0]) + dot(a.c[1], b.c[1]) + dot(a.c[2], b.c[2]);
+}
+
+template <class T> inline M3<T> add_diag(M3<T> m, T s)
+{
+    m.c[0].x += s;
+    m.c[1].y += s;
+    m.c[2].z += s;
+    return m;
+}
+
+// lambda tr(H) I + mu (H + H^T)
+template <class T> M3<T> sigma(const M3<T>& H, T lam, T mu)
+{
+    return add_diag(mscl(madd(H, tr(H)), mu), lam * trace3(H));
+}
+
+// sum_a p_a g_a^T
+template <class T> M3<T> outer4(const V3<T> p[4], const V3<T> g[4])
+{
+    M3<T> m = zero3<T>();
+    for (int a = 0; a < 4; ++a)
+    {
+        m.c[0] = add(m.c[0], scl(p[a], g[a].x));
+        m.c[1] = add(m.c[1], scl(p[a], g[a].y));
+        m.c[2] = add(m.c[2], scl(p[a], g[a].z));
+    }
+    return m;
+}
+
+// Solves m z = y by Cramer's rule (no threshold).
+template <class T> V3<T> solve3(const M3<T>& m, const V3<T>& y)
+{
+    const T d = det(m);
+    return V3<T>{det(cols(y, m.c[1], m.c[2])) / d, det(cols(m.c[0], y, m.c[2])) / d,
+                 det(cols(m.c[0], m.c[1], y)) / d};
+}
+
+// Vector-Jacobian product of R = polar(F): returns dL/dF given G = dL/dR.
+template <class T> M3<T> polar_vjp(const M3<T>& F, const M3<T>& R, int kind, const M3<T>& G)
+{
+    if (kind == 2)
+        return zero3<T>();  // Identity fallback: R does not depend on F
+    M3<T> Q = R, GQ = G;
+    if (kind == 1)
+    {
+        // R = Q diag(1, 1, -1)
+        Q.c[2] = neg(Q.c[2]);
+        GQ.c[2] = neg(GQ.c[2]);
+    }
+    const M3<T> QtF = mul(tr(Q), F);
+    const M3<T> S = mscl(madd(QtF, tr(QtF)), T(0.5));
+    const M3<T> QtG = mul(tr(Q), GQ);
+    // [y]x = skew(Q^T G)
+    const V3<T> y{(at(QtG, 2, 1) - at(QtG, 1, 2)) * T(0.5), (at(QtG, 0, 2) - at(QtG, 2, 0)) * T(0.5),
+                  (at(QtG, 1, 0) - at(QtG, 0, 1)) * T(0.5)};
+    // S [z]x + [z]x S = [(tr S I - S) z]x
+    const V3<T> z = solve3(add_diag(mscl(S, T(-1)), trace3(S)), y);
+    const M3<T> Z = cols(V3<T>{T(0), z.z, -z.y}, V3<T>{-z.z, T(0), z.x}, V3<T>{z.y, -z.x, T(0)});
+    return mscl(mul(Q, Z), T(2));
+}
+
+// Lame parameters exactly as build_D derives them.
+template <class T> void lame_of(const Tet<T>& t, T& lam, T& mu)
+{
+    lam = mu = T(0);
+    const T e = std::max(T(0), t.young);
+    if (e <= T(0))
+        return;
+    const T nu = std::max(T(-0.99f), std::min(T(0.49f), t.poisson));
+    lam = e * nu / ((T(1) + nu) * (T(1) - T(2) * nu));
+    mu = e / (T(2) * (T(1) + nu));
+}
+
+template <class T>
+struct TetAdjoint
+{
+    V3<T> xb[4], vb[4];
+    T dlogE = T(0);
+};
+
+// Per-tet terms of the reverse substep. False when the forward skipped the
+// tet (then it contributes nothing).
+template <class T>
+bool tet_adjoint(TetAdjoint<T>& o, const Tet<T>& t, const std::vector<V3<T>>& x, const V3<T> lam[4],
+                 const V3<T> w[4], T dt)
+{
+    const M3<T> ds = cols(sub(x[t.n[1]], x[t.n[0]]), sub(x[t.n[2]], x[t.n[0]]), sub(x[t.n[3]], x[t.n[0]]));
+    const M3<T> F = mul(ds, t.dmInv);
+    if (!finite(F))
+        return false;
+    int kind = 2;
+    const M3<T> R = polar(F, nullptr, &kind);
+    T la, mu;
+    lame_of(t, la, mu);
+    const T V = t.vol, dt2 = dt * dt;
+    const M3<T> L = outer4(lam, t.grad);
+    const M3<T> W = outer4(w, t.grad);
+    const M3<T> Rt = tr(R);
+    const M3<T> S1 = sigma(add_diag(mul(Rt, F), T(-1)), la, mu);
+    const M3<T> S2 = sigma(mul(Rt, W), la, mu);
+    const M3<T> A = mul(Rt, L);
+    const M3<T> SA = sigma(A, la, mu);
+    o.dlogE = -dt * V * frob(A, S1) - dt2 * V * frob(A, S2);
+    const M3<T> GR = madd(mscl(madd(mul(L, S1), mul(F, SA)), -dt * V), mscl(madd(mul(L, S2), mul(W, SA)), -dt2 * V));
+    const M3<T> P = mul(R, SA);
+    const M3<T> Fb = madd(mscl(P, -dt * V), polar_vjp(F, R, kind, GR));
+    for (int a = 0; a < 4; ++a)
+    {
+        o.xb[a] = mul(Fb, t.grad[a]);
+        o.vb[a] = sub(scl(mul(P, t.grad[a]), -dt2 * V), scl(lam[a], dt * t.damping));
+    }
+    return true;
+}
+
+// Records `steps` implicit substeps of length dt from the current state (all
+// states kept), then propagates (xb, vb) of the final state back to the
+// initial one. Outputs dL/dlogE per tet, dL/dx0 and dL/dv0 per node. Returns
+// 0, 1 on a mass error, 2 when an adjoint solve fails.
+template <class T>
+int adjoint_run(World<T>& w, T dt, int steps, std::vector<V3<T>> xb, std::vector<V3<T>> vb, T adjTol, int adjMax,
+                std::vector<T>& gE, std::vector<int>& adjIters)
+{
+    const size_t N = static_cast<size_t>(w.N);
+    std::vector<std::vector<V3<T>>> xs{w.x}, vs{w.v};
+    std::vector<uint8_t> ok;
+    for (int n = 0; n < steps; ++n)
+    {
+        bool integrated = false;
+        if (w.substep(dt, integrated) != 0)
+            return 1;
+        ok.push_back(integrated ? 1 : 0);
+        xs.push_back(w.x);
+        vs.push_back(w.v);
+    }
+    gE.assign(w.tets.size(), T(0));
+    adjIters.assign(static_cast<size_t>(steps), 0);
+    const std::vector<V3<T>> xEnd = w.x, vEnd = w.v;
+    for (int n = steps - 1; n >= 0; --n)
+    {
+        if (!ok[n])
+            continue;  // the substep left the state unchanged
+        w.x = xs[n];
+        w.v = vs[n];
+        if (w.precompute(dt) != 0 || !w.build_minv())
+            return 2;
+        std::vector<V3<T>> u(static_cast<size_t>(w.nd)), lam;
+        for (size_t i = 0; i < N; ++i)
+            if (w.dyn[i] >= 0)
+                u[w.dyn[i]] = add(vb[i], scl(xb[i], dt));
+        const CgResult<T> cg = pcg(lam, w.A, u, adjMax, adjTol, w.minv);
+        adjIters[n] = cg.iterations;
+        if (lam.size() != u.size())
+            return 2;
+        std::vector<V3<T>> lamNode(N);
+        for (size_t i = 0; i < N; ++i)
+            if (w.dyn[i] >= 0)
+            {
+                lamNode[i] = lam[w.dyn[i]];
+                vb[i] = u[w.dyn[i]];
+            }
+        const std::vector<V3<T>>& vNext = vs[n + 1];
+        for (size_t ti = 0; ti < w.tets.size(); ++ti)
+        {
+            const Tet<T>& t = w.tets[ti];
+            if (!t.active || t.vol <= T(0))
+                continue;
+            bool valid = true;
+            for (int a = 0; a < 4; ++a)
+                valid = valid && t.n[a] >= 0 && t.n[a] < w.N;
+            if (!valid)
+                continue;
+            V3<T> la[4], wa[4];
+            for (int a = 0; a < 4; ++a)
+            {
+                la[a] = lamNode[t.n[a]];
+                wa[a] = vNext[t.n[a]];
+            }
+            TetAdjoint<T> o;
+            if (!tet_adjoint(o, t, xs[n], la, wa, dt))
+                continue;
+            gE[ti] += o.dlogE;
+            for (int a = 0; a < 4; ++a)
+            {
+                xb[t.n[a]] = add(xb[t.n[a]], o.xb[a]);
+                vb[t.n[a]] = add(vb[t.n[a]], o.vb[a]);
+            }
+        }
+    }
+    w.x = xEnd;
+    w.v = vEnd;
+    // hand the initial-state adjoints back through the input vectors
+    w.adjX = xb;
+    w.adjV = vb;
+    return 0;
+}
+
 }  // namespace oracle
 
 using oracle::Handle;
@@ -1369,5 +1625,55 @@ int orc_pcg(int precision, int n, int nnz, const int* row_start, const int* col_
     if (precision == 32)
         return oracle::pcg_t<float>(n, nnz, row_start, col_index, vals, b, minv, max_it, tol, x, residual, converged);
     return oracle::pcg_t<double>(n, nnz, row_start, col_index, vals, b, minv, max_it, tol, x, residual, converged);
+}
+
+// Reverse mode (FP64 worlds only): runs `steps` implicit substeps of length dt
+// from the current state, then back-propagates xbar/vbar (N x 3, adjoints of
+// the final positions and velocities). Writes dL/dlogE (tet_count), dL/dx0
+// and dL/dv0 (N x 3) and the adjoint CG iterations per substep. The world is
+// left at the final state. Returns 0, 1 on a mass error, 2 when an adjoint
+// solve fails, -1 for an FP32 world.
+int orc_adjoint(void* p, double dt, int steps, const double* xbar, const double* vbar, double adj_tol, int adj_max_it,
+                double* grad_log_young, double* grad_x0, double* grad_v0, int* adj_iters)
+{
+    auto* h = static_cast<Handle*>(p);
+    if (h->precision != 64)
+        return -1;
+    auto& w = *h->w64;
+    std::vector<oracle::V3<double>> xb(static_cast<size_t>(w.N)), vb(static_cast<size_t>(w.N));
+    for (int i = 0; i < w.N; ++i)
+    {
+        xb[i] = {xbar[3 * i], xbar[3 * i + 1], xbar[3 * i + 2]};
+        vb[i] = {vbar[3 * i], vbar[3 * i + 1], vbar[3 * i + 2]};
+    }
+    std::vector<double> gE;
+    std::vector<int> its;
+    const int rc = oracle::adjoint_run(w, dt, steps, xb, vb, adj_tol, adj_max_it, gE, its);
+    if (rc != 0)
+        return rc;
+    for (size_t t = 0; t < gE.size(); ++t)
+        grad_log_young[t] = gE[t];
+    for (int i = 0; i < w.N; ++i)
+    {
+        oracle::put3(grad_x0 + 3 * i, w.adjX[i]);
+        oracle::put3(grad_v0 + 3 * i, w.adjV[i]);
+    }
+    if (adj_iters)
+        for (int n = 0; n < steps; ++n)
+            adj_iters[n] = its[n];
+    return 0;
+}
+
+// Vector-Jacobian product of the polar rotation of one 3x3 (row-major):
+// writes R and dL/dF for G = dL/dR. Returns the polar kind (0 Newton,
+// 1 Newton with column 2 negated, 2 Identity fallback).
+int orc_polar_vjp(const double* F, const double* G, double* R, double* Fbar)
+{
+    int kind = 2;
+    const auto f = oracle::getm<double>(F);
+    const auto r = oracle::polar(f, nullptr, &kind);
+    oracle::putm(R, r);
+    oracle::putm(Fbar, oracle::polar_vjp(f, r, kind, oracle::getm<double>(G)));
+    return kind;
 }
 }

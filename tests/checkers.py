@@ -60,6 +60,11 @@ def oracle_lib():
         lib.orc_pcg.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, C.c_void_p, C.c_int, C.c_double, _dp,
                                 C.POINTER(C.c_double), C.POINTER(C.c_int)]
         lib.orc_pcg.restype = C.c_int
+        lib.orc_adjoint.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, _dp,
+                                    _ip]
+        lib.orc_adjoint.restype = C.c_int
+        lib.orc_polar_vjp.argtypes = [_dp, _dp, _dp, _dp]
+        lib.orc_polar_vjp.restype = C.c_int
         _oracle = lib
     return _oracle
 
@@ -194,6 +199,21 @@ class Oracle:
         self.lib.orc_get_elements(self.h, valid, R, f, K)
         return valid.astype(bool), R.reshape(-1, 3, 3), f.reshape(-1, 4, 3), K.reshape(-1, 4, 4, 3, 3)
 
+    def adjoint(self, dt, steps, xbar, vbar=None, tol=1e-10, max_iter=10000):
+        """Runs `steps` substeps from the current state and back-propagates the
+        final-state adjoints. Returns (dL/dlogE, dL/dx0, dL/dv0, adjoint CG
+        iterations per substep); the world is left at the final state."""
+        vbar = np.zeros((self.n, 3)) if vbar is None else vbar
+        gE = np.zeros(self.t)
+        gx = np.zeros(self.n * 3)
+        gv = np.zeros(self.n * 3)
+        its = np.zeros(max(steps, 1), np.int32)
+        rc = self.lib.orc_adjoint(self.h, dt, steps, _c(xbar, np.float64).reshape(-1),
+                                  _c(vbar, np.float64).reshape(-1), tol, max_iter, gE, gx, gv, its)
+        if rc != 0:
+            raise RuntimeError(f"orc_adjoint failed with {rc}")
+        return gE, gx.reshape(-1, 3), gv.reshape(-1, 3), its[:steps]
+
     def rest(self):
         vol = np.zeros(self.t)
         dminv = np.zeros(self.t * 9)
@@ -206,6 +226,14 @@ def oracle_polar(F, precision=64):
     R = np.zeros(9)
     it = oracle_lib().orc_polar(precision, _c(F, np.float64).reshape(-1), R)
     return R.reshape(3, 3), it
+
+
+def oracle_polar_vjp(F, G):
+    """(R, dL/dF, kind) of the oracle polar rotation for G = dL/dR."""
+    R = np.zeros(9)
+    Fb = np.zeros(9)
+    kind = oracle_lib().orc_polar_vjp(_c(F, np.float64).reshape(-1), _c(G, np.float64).reshape(-1), R, Fb)
+    return R.reshape(3, 3), Fb.reshape(3, 3), kind
 
 
 def oracle_spmv(row_start, col, vals, x, precision=64):
