@@ -162,6 +162,33 @@ int pk_system_spmv(pk_system* sys, const double* x, double* y);
 /* Solution dv of the last solve (dynamic_rows x 3). */
 int pk_system_get_solution(pk_system* sys, double* dv);
 
+/* Reverse mode of the implicit step (no counterpart in the reference, which
+ * has no adjoint; the role the reference's interface would give it is an
+ * "adjoint_step" beside the forward step).
+ *
+ * pk_system_record runs `steps` implicit substeps of length dt from the
+ * current state, exactly as pk_system_step would (the state advances), and
+ * keeps device checkpoints (x, v) every `every` substeps. Any later change of
+ * material, solver, gravity, mode or launch shape discards the recording.
+ *
+ * pk_system_backward propagates the adjoints of the recorded run's final
+ * state -- xbar = dL/dx_S and vbar = dL/dv_S (node_count x 3 each; NULL means
+ * zero) -- back to its initial state. Each checkpoint segment is replayed
+ * forward on the device, then reversed substep by substep: the adjoint solve
+ * A lam = vbar + dt xbar on the dynamic rows (same matrix and preconditioner,
+ * block-Jacobi PCG to adj_tol in the same M^-1-norm rule, at most
+ * adj_max_iter iterations), then the corotational terms including dR/dF.
+ * Outputs (NULL to skip): dL/dlog E per tet (tet_count), dL/dx_0 and dL/dv_0
+ * (node_count x 3), and the largest adjoint CG iteration count of each
+ * substep (steps). Bodies of a batched system are reversed independently; a
+ * body whose forward substep did not integrate passes its adjoints through.
+ * The system is left at the recorded final state. Memory: checkpoints
+ * (steps / every + 1) x node_count x 48 bytes, segment (every + 1) x
+ * node_count x 48 bytes. */
+int pk_system_record(pk_system* sys, double dt, int steps, int every);
+int pk_system_backward(pk_system* sys, const double* xbar, const double* vbar, double adj_tol, int adj_max_iter,
+                       double* grad_log_young, double* grad_x0, double* grad_v0, int* adj_iterations);
+
 /* ---- diagnostics ---- */
 
 /* Copies `bytes` (a multiple of 32) between two fresh device buffers `reps`

@@ -19,7 +19,8 @@ ROOT = os.path.dirname(HERE)
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libphysik_amd.so")
 SOURCES = ["csrc/pk_assembly.hip", "csrc/pk_cg.hip", "csrc/pk_system.cpp", "csrc/pk_topology.cpp",
-           "csrc/physik_api.cpp", "csrc/pk_diag.hip"]
+           "csrc/physik_api.cpp", "csrc/pk_diag.hip",
+           "csrc/pk_adjoint.hip"]
 ARCH = os.environ.get("PHYSIK_AMD_ARCH", "gfx950")
 
 

@@ -15,12 +15,13 @@
 //   k_integrate all-or-nothing v += dv, x += dt v per group (World.cpp:644-695)
 //   k_explicit  explicit Euler (World.cpp:697-725)
 #include "pk_device.h"
+#include "pk_fem.h"
 #include "pk_topology.h"
 
 namespace pk
 {
 
-__global__ void k_reset(GroupState* gs, int G, int* active)
+__global__ void k_reset(GroupState* gs, int G, int* active, double tol, int maxIt)
 {
     __shared__ int cnt;
     if (threadIdx.x == 0)
@@ -30,6 +31,8 @@ __global__ void k_reset(GroupState* gs, int G, int* active)
     {
         GroupState& s = gs[g];
         s.delta[0] = s.delta[1] = s.target = s.residual = 0.0;
+        s.tol = tol;
+        s.maxIt = maxIt;
         s.iters = 0;
         s.stopAt = s.hasRows ? -1 : 0;
         s.stopInUpdate = 0;
@@ -42,71 +45,6 @@ __global__ void k_reset(GroupState* gs, int G, int* active)
     __syncthreads();
     if (threadIdx.x == 0)
         *active = cnt;
-}
-
-// FEMModel.cpp:115-163 with the reference thresholds (float literals).
-__device__ inline void polar(const double* F, double* R)
-{
-    bool ident = !fin9(F) || fabs(det3(F)) <= static_cast<double>(1.0e-8f);
-    double c[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e)
-        c[e] = F[e];
-    if (!ident)
-    {
-        for (int it = 0; it < 12; ++it)
-        {
-            double t[9], ti[9];
-            t[0] = c[0]; t[1] = c[3]; t[2] = c[6];
-            t[3] = c[1]; t[4] = c[4]; t[5] = c[7];
-            t[6] = c[2]; t[7] = c[5]; t[8] = c[8];
-            inv3(t, ti);
-            if (!fin9(ti) || fabs(det3(ti)) <= static_cast<double>(1.0e-8f))
-            {
-                ident = true;
-                break;
-            }
-            double nx[9];
-#pragma unroll
-            for (int e = 0; e < 9; ++e)
-                nx[e] = (c[e] + ti[e]) * 0.5;
-            if (!fin9(nx))
-            {
-                ident = true;
-                break;
-            }
-            double diff = 0.0;
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-            {
-                const double dx = nx[j] - c[j], dy = nx[3 + j] - c[3 + j], dz = nx[6 + j] - c[6 + j];
-                diff += dx * dx + dy * dy + dz * dz;
-            }
-#pragma unroll
-            for (int e = 0; e < 9; ++e)
-                c[e] = nx[e];
-            if (diff <= static_cast<double>(1.0e-10f))
-                break;
-        }
-        if (!ident && !fin9(c))
-            ident = true;
-    }
-    if (ident)
-    {
-#pragma unroll
-        for (int e = 0; e < 9; ++e)
-            R[e] = (e % 4 == 0) ? 1.0 : 0.0;
-        return;
-    }
-    if (det3(c) < 0.0)
-    {
-        c[2] = -c[2];
-        c[5] = -c[5];
-        c[8] = -c[8];
-    }
-#pragma unroll
-    for (int e = 0; e < 9; ++e)
-        R[e] = c[e];
 }
 
 __global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ tets,
@@ -502,9 +440,9 @@ __global__ __launch_bounds__(NT) void k_explicit(TileView tv, const int* __restr
 
 static inline unsigned grid_of(const TileView& tv) { return 8u * static_cast<unsigned>(tv.per); }
 
-void launch_reset(const DeviceView& v)
+void launch_reset(const DeviceView& v, double tol, int maxIt)
 {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(NT), 0, v.stream, v.gs, v.G, v.active);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(NT), 0, v.stream, v.gs, v.G, v.active, tol, maxIt);
 }
 
 void launch_element(const DeviceView& v)

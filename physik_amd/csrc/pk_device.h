@@ -173,7 +173,7 @@ __device__ inline bool fin9(const double* m)
 struct DeviceView
 {
     hipStream_t stream;
-    int T, nd, G;
+    int N, T, nd, G;
     long long nnzu, nnzf;
     // topology
     const int4* tets;
@@ -190,6 +190,10 @@ struct DeviceView
     const int* nstart;
     const int* nlist;
     const int* rowNode;
+    const int* astart;     // node -> (tet, corner) over live tets
+    const int* alist;
+    const int* nodeRow;    // node -> dynamic row or -1
+    const int* nodeGroup;  // node -> body
     const double* mrow;
     // state and work arrays
     double* x;
@@ -212,7 +216,7 @@ struct DeviceView
 
 // Launchers (pk_assembly.hip, pk_cg.hip). M is the storage type of A and of
 // the vectors the SpMV reads (double, or float for FP32 storage).
-void launch_reset(const DeviceView& v);
+void launch_reset(const DeviceView& v, double tol, int maxIt);
 void launch_element(const DeviceView& v);
 template <class M> void launch_assemble(const DeviceView& v, double dt);
 template <class M> void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt);
@@ -229,5 +233,13 @@ int update_grid(const DeviceView& v);
 void launch_stamp_reduce(hipStream_t stream, const unsigned long long* raw, int grid, int launches,
                          unsigned long long* span);
 template <class M> void launch_spmv_plain(const DeviceView& v, const double* x, double* y);
+
+// Reverse mode (pk_adjoint.hip). `ok` holds one flag per body: the forward
+// substep integrated. The CG of A lam = u runs through launch_spmv/update.
+void launch_record_ok(const DeviceView& v, int* ok);
+template <class M>
+void launch_adj_rhs(const DeviceView& v, const int* ok, const double* xbar, const double* vbar, double dt);
+void launch_adj_element(const DeviceView& v, const int* ok, const double* vnext, double dt, double* gradLogE);
+void launch_adj_nodes(const DeviceView& v, const int* ok, double dt, double* xbar, double* vbar);
 
 }  // namespace pk

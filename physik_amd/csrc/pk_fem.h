@@ -1,0 +1,83 @@
+// Per-tet finite-element helpers shared by the forward and reverse kernels.
+// 3x3 matrices are row-major double[9].
+#pragma once
+
+#include "pk_device.h"
+
+namespace pk
+{
+
+// FEMModel.cpp:115-163 with the reference thresholds (float literals).
+// `kind` (optional) reports how R was obtained: 0 Newton limit, 1 Newton
+// limit with column 2 negated, 2 Identity fallback.
+__device__ inline void polar(const double* F, double* R, int* kind = nullptr)
+{
+    bool ident = !fin9(F) || fabs(det3(F)) <= static_cast<double>(1.0e-8f);
+    double c[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+        c[e] = F[e];
+    if (!ident)
+    {
+        for (int it = 0; it < 12; ++it)
+        {
+            double t[9], ti[9];
+            t[0] = c[0]; t[1] = c[3]; t[2] = c[6];
+            t[3] = c[1]; t[4] = c[4]; t[5] = c[7];
+            t[6] = c[2]; t[7] = c[5]; t[8] = c[8];
+            inv3(t, ti);
+            if (!fin9(ti) || fabs(det3(ti)) <= static_cast<double>(1.0e-8f))
+            {
+                ident = true;
+                break;
+            }
+            double nx[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                nx[e] = (c[e] + ti[e]) * 0.5;
+            if (!fin9(nx))
+            {
+                ident = true;
+                break;
+            }
+            double diff = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+            {
+                const double dx = nx[j] - c[j], dy = nx[3 + j] - c[3 + j], dz = nx[6 + j] - c[6 + j];
+                diff += dx * dx + dy * dy + dz * dz;
+            }
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                c[e] = nx[e];
+            if (diff <= static_cast<double>(1.0e-10f))
+                break;
+        }
+        if (!ident && !fin9(c))
+            ident = true;
+    }
+    if (ident)
+    {
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            R[e] = (e % 4 == 0) ? 1.0 : 0.0;
+        if (kind)
+            *kind = 2;
+        return;
+    }
+    if (kind)
+        *kind = 0;
+    if (det3(c) < 0.0)
+    {
+        c[2] = -c[2];
+        c[5] = -c[5];
+        c[8] = -c[8];
+        if (kind)
+            *kind = 1;
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+        R[e] = c[e];
+}
+
+}  // namespace pk

@@ -115,6 +115,7 @@ struct pk_system
     pk::DevBuf<unsigned char> live;
     pk::DevBuf<double> prm, x, v, elem, mrow, minv, xs, r, pa;
     pk::DevBuf<int> cstart, clist, uFull, uMirror, slotRow, rowGroup, fstart, fcol, nstart, nlist, rowNode, active;
+    pk::DevBuf<int> astart, alist, nodeRow, nodeGroup;
     pk::DevBuf<char> vals, z, d0, d1, q;  // element type set by the precision
     pk::DevBuf<pk::Partial> pb0, pb1;
     pk::DevBuf<pk::GroupState> gs;
@@ -125,6 +126,16 @@ struct pk_system
     pk::Tiles vecTiles, spmvTiles;
     int* pinned = nullptr;
     hipEvent_t pollEv[2] = {nullptr, nullptr};
+
+    // reverse mode: checkpoints (x, v) every `tapeEvery` substeps of a
+    // recorded run, the states of the segment being reversed, per-body
+    // integrate flags of that segment, the adjoints and the gradient
+    pk::DevBuf<double> tapeX, tapeV, segX, segV, xbar, vbar, gradE, xEnd, vEnd;
+    pk::DevBuf<int> segOk;
+    double tapeDt = 0.0;
+    int tapeSteps = 0, tapeEvery = 0;
+    bool tapeValid = false;
+    int lastItersAdj = 0;
 
     double gravity[3] = {0, 0, 0};
     double tol = 1e-4;
@@ -154,6 +165,7 @@ struct pk_system
     {
         pk::DeviceView dv{};
         dv.stream = stream;
+        dv.N = topo.N;
         dv.T = topo.T;
         dv.nd = topo.nd;
         dv.G = topo.G;
@@ -173,6 +185,10 @@ struct pk_system
         dv.nstart = nstart.p;
         dv.nlist = nlist.p;
         dv.rowNode = rowNode.p;
+        dv.astart = astart.p;
+        dv.alist = alist.p;
+        dv.nodeRow = nodeRow.p;
+        dv.nodeGroup = nodeGroup.p;
         dv.mrow = mrow.p;
         dv.x = x.p;
         dv.v = v.p;
@@ -332,11 +348,12 @@ static void harvest_profile(pk_system* s)
     p.used = 0;
 }
 
+// reset -> element -> assemble (-> rhs with the CG start when withRhs).
 template <class M>
-static int run_assembly(pk_system* s, double dt)
+static int run_assembly(pk_system* s, double dt, double tol, int maxIt, bool withRhs)
 {
     const DeviceView dv = s->view();
-    launch_reset(dv);
+    launch_reset(dv, tol, maxIt);
     {
         Bracket br(s, PK_K_ELEMENT);
         launch_element(dv);
@@ -347,6 +364,7 @@ static int run_assembly(pk_system* s, double dt)
         Bracket br(s, PK_K_ASSEMBLE);
         launch_assemble<M>(dv, dt);
     }
+    if (withRhs)
     {
         Bracket br(s, PK_K_RHS);
         launch_rhs<M>(dv, s->gravity[0], s->gravity[1], s->gravity[2], dt);
@@ -355,11 +373,13 @@ static int run_assembly(pk_system* s, double dt)
     return PK_OK;
 }
 
+// The CG launches up to maxIt; lastIters (the previous solve's count of the
+// same kind) sizes the first chunk.
 template <class M>
-static int run_cg(pk_system* s)
+static int run_cg(pk_system* s, int maxItArg, int lastIters)
 {
     const DeviceView dv = s->view();
-    const int maxIt = std::max(0, s->maxIt);
+    const int maxIt = std::max(0, maxItArg);
     const int gS = spmv_grid(dv), gU = update_grid(dv);
     unsigned long long* stS = nullptr;
     unsigned long long* stU = nullptr;
@@ -380,7 +400,7 @@ static int run_cg(pk_system* s)
     // launch k runs the SpMV of iteration k (and decides the end of k-1);
     // k = maxIt only takes that decision
     int k = 0, poll = 0;
-    int chunk = std::max(4, s->lastIters - 4);
+    int chunk = std::max(4, lastIters - 4);
     while (k <= maxIt)
     {
         const int end = std::min(maxIt + 1, k + chunk);
@@ -414,13 +434,13 @@ static int run_cg(pk_system* s)
 template <class M>
 static int run_substep(pk_system* s, double dt)
 {
-    int rc = run_assembly<M>(s, dt);
+    int rc = run_assembly<M>(s, dt, s->tol, s->maxIt, true);
     if (rc != PK_OK || s->topo.nd == 0)
         return rc;
     const DeviceView dv = s->view();
     if (s->implicit)
     {
-        if ((rc = run_cg<M>(s)) != PK_OK)
+        if ((rc = run_cg<M>(s, s->maxIt, s->lastIters)) != PK_OK)
             return rc;
         Bracket br(s, PK_K_INTEGRATE);
         launch_integrate(dv, dt, 0);
@@ -435,7 +455,9 @@ static int run_substep(pk_system* s, double dt)
     return PK_OK;
 }
 
-static int read_groups(pk_system* s)
+// Copies the per-body CG state to the host; `lastIters` receives the largest
+// iteration count (it sizes the next solve of the same kind).
+static int read_groups(pk_system* s, int& lastIters)
 {
     if (s->hostGs.empty())
         return PK_OK;
@@ -445,7 +467,7 @@ static int read_groups(pk_system* s)
     int mx = 0;
     for (const auto& g : s->hostGs)
         mx = std::max(mx, g.iters);
-    s->lastIters = mx;
+    lastIters = mx;
     return PK_OK;
 }
 
@@ -526,6 +548,10 @@ static int setup_device(pk_system* s)
     if ((rc = s->nstart.upload(t.nstart, st)) != PK_OK) return rc;
     if ((rc = s->nlist.upload(t.nlist, st)) != PK_OK) return rc;
     if ((rc = s->rowNode.upload(t.rowNode, st)) != PK_OK) return rc;
+    if ((rc = s->astart.upload(t.astart, st)) != PK_OK) return rc;
+    if ((rc = s->alist.upload(t.alist, st)) != PK_OK) return rc;
+    if ((rc = s->nodeRow.upload(t.dyn, st)) != PK_OK) return rc;
+    if ((rc = s->nodeGroup.upload(t.nodeGroup, st)) != PK_OK) return rc;
     if ((rc = alloc_work(s)) != PK_OK) return rc;
     std::vector<double> zero(static_cast<size_t>(t.N) * 3, 0.0);
     if ((rc = s->v.upload(zero, st)) != PK_OK) return rc;
@@ -619,6 +645,7 @@ int pk_system_tune(pk_system* s, int lanes_per_row, int spmv_rows, int vector_ro
                   "positive row counts");
         return PK_ERR_ARG;
     }
+    s->tapeValid = false;  // a recorded run no longer replays bit-identically
     s->lanesPerRow = lanes_per_row;
     s->spmvRows = spmv_rows;
     s->vecRows = vector_rows;
@@ -667,6 +694,7 @@ int pk_system_set_gravity(pk_system* s, double gx, double gy, double gz)
 {
     if (!s)
         return PK_ERR_ARG;
+    s->tapeValid = false;  // a recorded run no longer replays the same map
     s->gravity[0] = gx;
     s->gravity[1] = gy;
     s->gravity[2] = gz;
@@ -677,6 +705,7 @@ int pk_system_set_solver(pk_system* s, double tolerance, int max_iterations, int
 {
     if (!s)
         return PK_ERR_ARG;
+    s->tapeValid = false;  // a recorded run no longer replays the same map
     s->tol = tolerance;
     s->maxIt = max_iterations;
     s->jacobi = use_jacobi ? 1 : 0;
@@ -687,6 +716,7 @@ int pk_system_set_mode(pk_system* s, int implicit)
 {
     if (!s)
         return PK_ERR_ARG;
+    s->tapeValid = false;  // a recorded run no longer replays the same map
     s->implicit = implicit ? 1 : 0;
     return PK_OK;
 }
@@ -695,6 +725,7 @@ int pk_system_set_young(pk_system* s, const double* young)
 {
     if (!s || !young)
         return PK_ERR_ARG;
+    s->tapeValid = false;  // a recorded run no longer replays the same map
     Topology& t = s->topo;
     if (t.T == 0)
         return PK_OK;
@@ -733,7 +764,7 @@ int pk_system_step(pk_system* s, double dt, int substeps, int frames)
             if (s->implicit && s->topo.nd > 0)
             {
                 // the outcome sizes the next solve's first chunk
-                const int rc2 = read_groups(s);
+                const int rc2 = read_groups(s, s->lastIters);
                 if (rc2 != PK_OK)
                     return rc2;
                 harvest_profile(s);
@@ -742,6 +773,211 @@ int pk_system_step(pk_system* s, double dt, int substeps, int frames)
     PK_HIP(hipStreamSynchronize(s->stream));
     return PK_OK;
 }
+
+}  // extern "C"
+
+namespace pk
+{
+
+// ---- reverse mode ----
+
+// Device copy of n doubles on the system's stream.
+static int dcopy(pk_system* s, double* dst, const double* src, size_t n)
+{
+    if (n)
+        PK_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    return PK_OK;
+}
+
+// One forward substep of a replay: the plain substep, then the per-body
+// integrate flags into `ok`.
+template <class M>
+static int replay_substep(pk_system* s, double dt, int* ok)
+{
+    int rc = run_substep<M>(s, dt);
+    if (rc != PK_OK)
+        return rc;
+    launch_record_ok(s->view(), ok);
+    PK_HIP(hipGetLastError());
+    return read_groups(s, s->lastIters);
+}
+
+// Reverses substep j of the current segment: (segX[j], segV[j]) -> segV[j+1].
+template <class M>
+static int reverse_substep(pk_system* s, double dt, int j, double adjTol, int adjMaxIt, int* iters)
+{
+    const size_t n3 = static_cast<size_t>(s->topo.N) * 3;
+    const int* ok = s->segOk.p + static_cast<size_t>(j) * s->topo.G;
+    int rc;
+    if ((rc = dcopy(s, s->x.p, s->segX.p + j * n3, n3)) != PK_OK) return rc;
+    if ((rc = dcopy(s, s->v.p, s->segV.p + j * n3, n3)) != PK_OK) return rc;
+    if ((rc = run_assembly<M>(s, dt, adjTol, adjMaxIt, false)) != PK_OK)
+        return rc;
+    const DeviceView dv = s->view();
+    launch_adj_rhs<M>(dv, ok, s->xbar.p, s->vbar.p, dt);
+    if ((rc = run_cg<M>(s, adjMaxIt, s->lastItersAdj)) != PK_OK)
+        return rc;
+    launch_adj_element(dv, ok, s->segV.p + (j + 1) * n3, dt, s->gradE.p);
+    launch_adj_nodes(dv, ok, dt, s->xbar.p, s->vbar.p);
+    PK_HIP(hipGetLastError());
+    if ((rc = read_groups(s, s->lastItersAdj)) != PK_OK)
+        return rc;
+    if (iters)
+        *iters = s->lastItersAdj;
+    return PK_OK;
+}
+
+template <class M>
+static int record_run(pk_system* s, double dt, int steps, int every)
+{
+    const size_t n3 = static_cast<size_t>(s->topo.N) * 3;
+    const int nck = steps / every + 1;
+    int rc;
+    if ((rc = s->tapeX.alloc(nck * n3)) != PK_OK) return rc;
+    if ((rc = s->tapeV.alloc(nck * n3)) != PK_OK) return rc;
+    if ((rc = s->segOk.alloc(static_cast<size_t>(every) * std::max(1, s->topo.G))) != PK_OK) return rc;
+    s->tapeValid = false;
+    if ((rc = dcopy(s, s->tapeX.p, s->x.p, n3)) != PK_OK) return rc;
+    if ((rc = dcopy(s, s->tapeV.p, s->v.p, n3)) != PK_OK) return rc;
+    for (int n = 0; n < steps; ++n)
+    {
+        if ((rc = replay_substep<M>(s, dt, s->segOk.p)) != PK_OK)
+            return rc;
+        if ((n + 1) % every == 0)
+        {
+            const size_t c = static_cast<size_t>((n + 1) / every);
+            if ((rc = dcopy(s, s->tapeX.p + c * n3, s->x.p, n3)) != PK_OK) return rc;
+            if ((rc = dcopy(s, s->tapeV.p + c * n3, s->v.p, n3)) != PK_OK) return rc;
+        }
+    }
+    PK_HIP(hipStreamSynchronize(s->stream));
+    s->tapeDt = dt;
+    s->tapeSteps = steps;
+    s->tapeEvery = every;
+    s->tapeValid = true;
+    return PK_OK;
+}
+
+template <class M>
+static int backward_run(pk_system* s, double adjTol, int adjMaxIt, int* adjIters)
+{
+    const Topology& t = s->topo;
+    const size_t n3 = static_cast<size_t>(t.N) * 3;
+    const int steps = s->tapeSteps, every = s->tapeEvery;
+    const double dt = s->tapeDt;
+    int rc;
+    if ((rc = s->segX.alloc((every + 1) * n3)) != PK_OK) return rc;
+    if ((rc = s->segV.alloc((every + 1) * n3)) != PK_OK) return rc;
+    if ((rc = s->xEnd.alloc(n3)) != PK_OK) return rc;
+    if ((rc = s->vEnd.alloc(n3)) != PK_OK) return rc;
+    if ((rc = dcopy(s, s->xEnd.p, s->x.p, n3)) != PK_OK) return rc;
+    if ((rc = dcopy(s, s->vEnd.p, s->v.p, n3)) != PK_OK) return rc;
+    const int nseg = (steps + every - 1) / every;
+    for (int seg = nseg - 1; seg >= 0; --seg)
+    {
+        const int n0 = seg * every, len = std::min(every, steps - n0);
+        if ((rc = dcopy(s, s->x.p, s->tapeX.p + seg * n3, n3)) != PK_OK) return rc;
+        if ((rc = dcopy(s, s->v.p, s->tapeV.p + seg * n3, n3)) != PK_OK) return rc;
+        if ((rc = dcopy(s, s->segX.p, s->x.p, n3)) != PK_OK) return rc;
+        if ((rc = dcopy(s, s->segV.p, s->v.p, n3)) != PK_OK) return rc;
+        for (int j = 0; j < len; ++j)
+        {
+            if ((rc = replay_substep<M>(s, dt, s->segOk.p + static_cast<size_t>(j) * t.G)) != PK_OK)
+                return rc;
+            if ((rc = dcopy(s, s->segX.p + (j + 1) * n3, s->x.p, n3)) != PK_OK) return rc;
+            if ((rc = dcopy(s, s->segV.p + (j + 1) * n3, s->v.p, n3)) != PK_OK) return rc;
+        }
+        for (int j = len - 1; j >= 0; --j)
+            if ((rc = reverse_substep<M>(s, dt, j, adjTol, adjMaxIt, adjIters ? adjIters + n0 + j : nullptr)) !=
+                PK_OK)
+                return rc;
+    }
+    if ((rc = dcopy(s, s->x.p, s->xEnd.p, n3)) != PK_OK) return rc;
+    if ((rc = dcopy(s, s->v.p, s->vEnd.p, n3)) != PK_OK) return rc;
+    PK_HIP(hipStreamSynchronize(s->stream));
+    return PK_OK;
+}
+
+}  // namespace pk
+
+extern "C" {
+
+int pk_system_record(pk_system* s, double dt, int steps, int every)
+{
+    if (!s || !(dt > 0.0) || steps < 1 || every < 1)
+    {
+        set_error("pk_system_record: need dt > 0, steps >= 1, every >= 1");
+        return PK_ERR_ARG;
+    }
+    if (!s->implicit)
+    {
+        set_error("pk_system_record: the reverse mode covers the implicit integrator only");
+        return PK_ERR_ARG;
+    }
+    if (s->topo.massErrorNode >= 0)
+    {
+        set_error("Non-fixed node has no physical mass contribution. nodeIndex=" +
+                  std::to_string(s->topo.massErrorNode));
+        return PK_ERR_MASS;
+    }
+    every = std::min(every, steps);
+    if (s->topo.nd == 0)
+    {
+        s->tapeValid = false;
+        set_error("pk_system_record: no dynamic node");
+        return PK_ERR_ARG;
+    }
+    const bool p = s->prof.on;
+    s->prof.on = false;
+    const int rc = s->precision == PK_FP32_STORAGE ? record_run<float>(s, dt, steps, every)
+                                                   : record_run<double>(s, dt, steps, every);
+    s->prof.on = p;
+    return rc;
+}
+
+int pk_system_backward(pk_system* s, const double* xbar, const double* vbar, double adj_tol, int adj_max_iter,
+                       double* grad_log_young, double* grad_x0, double* grad_v0, int* adj_iterations)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    if (!s->tapeValid)
+    {
+        set_error("pk_system_backward: no recorded run (call pk_system_record after the last change)");
+        return PK_ERR_ARG;
+    }
+    const Topology& t = s->topo;
+    const size_t n3 = static_cast<size_t>(t.N) * 3;
+    int rc;
+    if ((rc = s->xbar.alloc(n3)) != PK_OK) return rc;
+    if ((rc = s->vbar.alloc(n3)) != PK_OK) return rc;
+    if ((rc = s->gradE.alloc(static_cast<size_t>(t.T))) != PK_OK) return rc;
+    if (xbar)
+        PK_HIP(hipMemcpyAsync(s->xbar.p, xbar, n3 * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    else
+        PK_HIP(hipMemsetAsync(s->xbar.p, 0, n3 * sizeof(double), s->stream));
+    if (vbar)
+        PK_HIP(hipMemcpyAsync(s->vbar.p, vbar, n3 * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    else
+        PK_HIP(hipMemsetAsync(s->vbar.p, 0, n3 * sizeof(double), s->stream));
+    if (t.T)
+        PK_HIP(hipMemsetAsync(s->gradE.p, 0, static_cast<size_t>(t.T) * sizeof(double), s->stream));
+    const bool p = s->prof.on;
+    s->prof.on = false;
+    rc = s->precision == PK_FP32_STORAGE ? backward_run<float>(s, adj_tol, adj_max_iter, adj_iterations)
+                                         : backward_run<double>(s, adj_tol, adj_max_iter, adj_iterations);
+    s->prof.on = p;
+    if (rc != PK_OK)
+        return rc;
+    if (grad_log_young && t.T)
+        PK_HIP(hipMemcpy(grad_log_young, s->gradE.p, static_cast<size_t>(t.T) * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    if (grad_x0 && n3)
+        PK_HIP(hipMemcpy(grad_x0, s->xbar.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (grad_v0 && n3)
+        PK_HIP(hipMemcpy(grad_v0, s->vbar.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    return PK_OK;
+}
+
 
 int pk_system_group_results(pk_system* s, int* iterations, double* residual, int* converged, int* status)
 {
@@ -790,11 +1026,13 @@ int pk_system_assemble(pk_system* s, double dt)
 {
     if (!s)
         return PK_ERR_ARG;
-    const int rc = s->precision == PK_FP32_STORAGE ? run_assembly<float>(s, dt) : run_assembly<double>(s, dt);
+    const int rc = s->precision == PK_FP32_STORAGE ? run_assembly<float>(s, dt, s->tol, s->maxIt, true)
+                                                   : run_assembly<double>(s, dt, s->tol, s->maxIt, true);
     if (rc != PK_OK)
         return rc;
     PK_HIP(hipStreamSynchronize(s->stream));
-    const int rc2 = read_groups(s);
+    int unused = 0;
+    const int rc2 = read_groups(s, unused);
     s->prof.recs.clear();
     s->prof.used = 0;
     return rc2;

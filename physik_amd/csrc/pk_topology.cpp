@@ -148,6 +148,7 @@ int build_topology(const pk_mesh_desc& d, Topology& topo, std::string& err, bool
                 }
     }
     topo.G = G;
+    topo.nodeGroup = group;
 
     compute_tet_params(d, topo.prm);
     topo.poisson.assign(d.poisson, d.poisson + T);
@@ -366,6 +367,23 @@ int build_topology(const pk_mesh_desc& d, Topology& topo, std::string& err, bool
                 if (da >= 0)
                     topo.nlist[nfill[da]++] = pack_corner(t, a);
             }
+
+    // reverse-mode gather lists: node <- (tet, corner) over live tets in order
+    topo.astart.assign(static_cast<size_t>(N) + 1, 0);
+    for (int t = 0; t < T; ++t)
+        if (topo.live[t])
+            for (int a = 0; a < 4; ++a)
+                ++topo.astart[d.tets[4 * t + a] + 1];
+    for (int n = 0; n < N; ++n)
+        topo.astart[n + 1] += topo.astart[n];
+    topo.alist.assign(static_cast<size_t>(topo.astart[N]), 0);
+    {
+        std::vector<int> afill(topo.astart.begin(), topo.astart.end() - 1);
+        for (int t = 0; t < T; ++t)
+            if (topo.live[t])
+                for (int a = 0; a < 4; ++a)
+                    topo.alist[afill[d.tets[4 * t + a]]++] = pack_corner(t, a);
+    }
 
     if (wantScatter)
     {

@@ -61,6 +61,8 @@ struct Topology
     std::vector<int> uFull, uMirror;     // upper slot -> full index of (r,c) and of (c,r) (-1 on the diagonal)
     std::vector<int> cstart, clist;      // upper slot -> packed contributions, increasing tet
     std::vector<int> nstart, nlist;      // dynamic row -> packed (tet, corner), increasing tet
+    std::vector<int> astart, alist;      // every node -> packed (tet, corner) over live tets, increasing tet
+    std::vector<int> nodeGroup;          // N -> body
     std::vector<int> scatter;            // optional T*16 upper slot of each local pair, -1 if not dynamic
 
     long long nnzu() const { return static_cast<long long>(ucol.size()); }

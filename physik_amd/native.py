@@ -72,6 +72,8 @@ PK_SIGNATURES = {
     "pk_system_get_elements": (C.c_int, [_vp, _vp, _vp, _vp]),
     "pk_system_spmv": (C.c_int, [_vp, _dp, _dp]),
     "pk_system_get_solution": (C.c_int, [_vp, _dp]),
+    "pk_system_record": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
+    "pk_system_backward": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp, _vp, _vp]),
     "pk_diag_copy": (C.c_int, [C.c_longlong, C.c_int, C.POINTER(C.c_double)]),
     "pk_pattern_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(_vp)]),
     "pk_pattern_destroy": (None, [_vp]),

@@ -179,6 +179,26 @@ class System:
         N.check(self.lib.pk_system_get_solution(self.h, dv), "pk_system_get_solution")
         return dv.reshape(-1, 3)
 
+    def record(self, dt, steps, every=None):
+        """Run `steps` implicit substeps of length dt from the current state,
+        keeping device checkpoints every `every` substeps (default: all in one
+        segment) for a later backward()."""
+        self._steps = int(steps)
+        N.check(self.lib.pk_system_record(self.h, dt, int(steps), int(every or steps)), "pk_system_record")
+
+    def backward(self, xbar=None, vbar=None, tol=1e-6, max_iter=1024):
+        """Adjoints of the recorded run's final state -> (dL/dlogE per tet,
+        dL/dx0, dL/dv0, adjoint CG iterations per substep)."""
+        xb = None if xbar is None else N.as_f64(xbar).reshape(-1)
+        vb = None if vbar is None else N.as_f64(vbar).reshape(-1)
+        gE = np.zeros(self.t)
+        gx = np.zeros((self.n, 3))
+        gv = np.zeros((self.n, 3))
+        its = np.zeros(max(getattr(self, "_steps", 0), 1), np.int32)
+        N.check(self.lib.pk_system_backward(self.h, N.ptr(xb), N.ptr(vb), tol, int(max_iter), N.ptr(gE), N.ptr(gx),
+                                            N.ptr(gv), N.ptr(its)), "pk_system_backward")
+        return gE, gx, gv, its[:getattr(self, "_steps", 0)]
+
 
 def device_count() -> int:
     return N.lib().pk_device_count()

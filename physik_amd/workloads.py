@@ -66,6 +66,11 @@ class Workload:
     gravity: tuple = G
     body_nodes: list = field(default_factory=list)  # (begin, end) node range per body
     body_tets: list = field(default_factory=list)   # (begin, end) tet range per body
+    # differentiable workloads: loss = sum over loss_nodes of |x_S - target|^2
+    loss_nodes: np.ndarray | None = None
+    target: np.ndarray | None = None
+    checkpoint: int = 0                 # substeps between checkpoints of the recorded run
+    adj_tol: float = 1e-6               # adjoint CG tolerance
 
     @property
     def n_nodes(self):
@@ -83,10 +88,16 @@ class Workload:
         """The workload restricted to body b (node ids rebased to 0)."""
         n0, n1 = self.body_nodes[b]
         t0, t1 = self.body_tets[b]
-        return Workload(f"{self.name}[{b}]", self.rest[n0:n1], self.tets[t0:t1] - n0, self.young[t0:t1],
-                        self.poisson[t0:t1], self.damping[t0:t1], self.density[t0:t1], self.fixed[n0:n1],
-                        self.x0[n0:n1], self.v0[n0:n1], None, self.dt, self.steps, self.tol, self.max_iter,
-                        self.gravity, [(0, n1 - n0)], [(0, t1 - t0)])
+        w = Workload(f"{self.name}[{b}]", self.rest[n0:n1], self.tets[t0:t1] - n0, self.young[t0:t1],
+                     self.poisson[t0:t1], self.damping[t0:t1], self.density[t0:t1], self.fixed[n0:n1],
+                     self.x0[n0:n1], self.v0[n0:n1], None, self.dt, self.steps, self.tol, self.max_iter,
+                     self.gravity, [(0, n1 - n0)], [(0, t1 - t0)])
+        if self.loss_nodes is not None:
+            sel = (self.loss_nodes >= n0) & (self.loss_nodes < n1)
+            w.loss_nodes = self.loss_nodes[sel] - n0
+            w.target = self.target[sel]
+            w.checkpoint, w.adj_tol = self.checkpoint, self.adj_tol
+        return w
 
 
 def _single(name, rest, tets, young, poisson, rho, fixed, x0, v0, dt, steps, tol, max_iter):
@@ -148,6 +159,78 @@ def batch256(bodies=256, seed=2, lo=4, hi=32, h=0.1, steps=100):
     return Workload(f"batch{bodies}", rest, tets, np.concatenate(youngs), np.concatenate(poissons), np.zeros(t),
                     np.full(t, 1000.0), np.concatenate(fixeds), rest.copy(), np.concatenate(v0s),
                     np.concatenate(groups), 1.0 / 60.0, steps, 1e-6, 1024, G, body_nodes, body_tets)
+
+
+def diff32(n=32, h=1.0 / 32.0, steps=50, seed=3):
+    """Differentiable single body: n^3-cell Kuhn beam (6 n^3 tets), per-tet
+    log E ~ N(log 1e5, 0.2) from Pcg32(seed, 0), nu 0.3, rho 1000, x = 0 face
+    fixed, starting at rest; loss on the +x face against its rest positions
+    moved 0.05 m down; 50 substeps of 1/60 s, checkpoint every 10, forward
+    and adjoint CG rtol 1e-6."""
+    rest, tets = mg.box_mesh(n, n, n, h, "kuhn")
+    t = len(tets)
+    young = np.exp(np.log(1e5) + 0.2 * mg.Pcg32(seed, 0).normal(t))
+    fixed = np.zeros(len(rest), np.uint8)
+    fixed[mg.face_nodes(n, n, n, 0)] = 1
+    end = mg.face_nodes(n, n, n, 1)
+    w = _single(f"diff{n}", rest, tets, 1e5, 0.3, 1000.0, fixed, rest.copy(), np.zeros_like(rest), 1.0 / 60.0,
+                steps, 1e-6, 1024)
+    w.young = young
+    w.loss_nodes = end.astype(np.int64)
+    w.target = rest[end] - np.array([0.0, 0.0, 0.05])
+    w.checkpoint = 10
+    w.adj_tol = 1e-6
+    return w
+
+
+def batch128(bodies=128, seed=4, lo=8, hi=24, h=1.0 / 32.0, steps=40):
+    """Batched differentiable bodies: n_x, n_y, n_z ~ U{lo..hi} per body
+    (Pcg32(seed, 0)), Kuhn split, per-tet log E ~ N(log 1e5, 0.2)
+    (Pcg32(seed, 1), body by body), nu 0.3, rho 1000, x = 0 face fixed, at
+    rest; per-body loss on the +x face against its rest positions moved by
+    a per-body offset U[-0.05, 0.05]^3 m (Pcg32(seed, 2)); 40 substeps of
+    1/60 s, checkpoint every 8, forward CG rtol 1e-6, adjoint rtol 1e-7."""
+    rng, rngE, rngT = mg.Pcg32(seed, 0), mg.Pcg32(seed, 1), mg.Pcg32(seed, 2)
+    rests, tetss, fixeds, youngs, groups, ends, targets = [], [], [], [], [], [], []
+    body_nodes, body_tets = [], []
+    n_off = t_off = 0
+    for b in range(bodies):
+        nx, ny, nz = (int(v) for v in rng.randint(lo, hi, 3))
+        rest, tets = mg.box_mesh(nx, ny, nz, h, "kuhn")
+        fixed = np.zeros(len(rest), np.uint8)
+        fixed[mg.face_nodes(nx, ny, nz, 0)] = 1
+        end = mg.face_nodes(nx, ny, nz, 1)
+        off = 0.05 * (2.0 * rngT.uniform(3) - 1.0)
+        rests.append(rest)
+        tetss.append(tets + n_off)
+        fixeds.append(fixed)
+        youngs.append(np.exp(np.log(1e5) + 0.2 * rngE.normal(len(tets))))
+        groups.append(np.full(len(rest), b, np.int32))
+        ends.append(end + n_off)
+        targets.append(rest[end] + off)
+        body_nodes.append((n_off, n_off + len(rest)))
+        body_tets.append((t_off, t_off + len(tets)))
+        n_off += len(rest)
+        t_off += len(tets)
+    rest = np.concatenate(rests)
+    tets = np.concatenate(tetss).astype(np.int32)
+    t = len(tets)
+    w = Workload(f"batch{bodies}", rest, tets, np.concatenate(youngs), np.full(t, 0.3), np.zeros(t),
+                 np.full(t, 1000.0), np.concatenate(fixeds), rest.copy(), np.zeros_like(rest),
+                 np.concatenate(groups), 1.0 / 60.0, steps, 1e-6, 1024, G, body_nodes, body_tets)
+    w.loss_nodes = np.concatenate(ends).astype(np.int64)
+    w.target = np.concatenate(targets)
+    w.checkpoint = 8
+    w.adj_tol = 1e-7
+    return w
+
+
+def loss_and_seed(w: Workload, x: np.ndarray):
+    """Loss of a final state and its gradient dL/dx_S (node_count x 3)."""
+    d = x[w.loss_nodes] - w.target
+    xbar = np.zeros_like(x)
+    xbar[w.loss_nodes] = 2.0 * d
+    return float(np.sum(d * d)), xbar
 
 
 def make_system(w: Workload, precision=0, batched=True):
