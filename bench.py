@@ -1,15 +1,24 @@
 """Benchmark of the implicit-Euler soft-body step on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload twist64|beam10|batch256]
-                    [--precision fp64|fp32] [--no-cpu-baseline] [--no-autotune | --shape L,R,V,U]
+    python bench.py [--steps K] [--warmup W]
+                    [--workload batch128|diff32|twist64|beam10|batch256]
+                    [--precision fp64|fp32] [--no-plain] [--no-cpu-baseline]
+                    [--no-autotune | --shape L,R,V,U]
 
-A step is one implicit-Euler frame (element corotation, assembly, block-Jacobi
-PCG to rtol 1e-6, integration) over the whole workload, with every input
-resident in GPU memory. Prints one JSON line: tet-steps/s over exactly K timed
-steps after W untimed ones, the roofline of the dominant kernel (the CG SpMV)
-from the kernel's own wall-clock span over the timed region, and a CPU
-baseline: the reference's own FP32 step (its sources compiled in place) timed
-on one host core over a bounded sample of the same workload.
+Default: the batched differentiable workload (128 bodies, 3.2 M tets, FP32
+storage with FP64 dots and residual), forward + adjoint. A step is one
+implicit substep forward (element corotation, assembly, block-Jacobi PCG to
+rtol 1e-6, integration) and its reverse (checkpoint-segment replay, adjoint
+PCG to rtol 1e-7, corotational adjoint with dR/dF) over every body; K steps
+are recorded with checkpoints every 8 and back-propagated from the per-body
+losses, all inputs resident in GPU memory. Beside it, on the same line: the
+plain single-body step (64^3 twisted Kuhn beam, FP64, 1.57 M tets), and the
+CPU baselines (the oracle's FP64 forward + adjoint and the reference's own
+FP32 forward, one host core, a bounded sample of the same workload).
+
+`value` is tet-steps/s of the default workload; the SpMV roofline uses the
+kernel's in-kernel wall-clock span over the timed region (the span a kernel
+trace reports) and the algorithmic bytes of one launch.
 """
 
 from __future__ import annotations
@@ -28,27 +37,31 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 METRIC = "tet·steps/s fwd+adjoint batched vs plain step, 1 GPU, rtol 1e-6; SpMV % HBM"
 
-
-def workload(name):
-    from physik_amd import workloads as W
-    if name == "twist64":
-        return W.twist64()
-    if name == "beam10":
-        return W.beam10()
-    if name == "batch256":
-        return W.batch256()
-    raise SystemExit(f"unknown workload {name}")
-
-
 DESCRIPTION = {
-    "twist64": "64^3-cell Kuhn beam, 1,572,864 tets, 274,625 nodes, rest nodes jittered +-10% (seed 1), "
-               "90-degree initial twist, x=0 face fixed, E=1e6 Pa, nu=0.45, rho=1000, g=-9.81 z, dt=1/120 s, "
-               "rtol 1e-6, max 1024 CG iterations",
-    "beam10": "10^3-cell 5-tet beam, 5,000 tets, 1,331 nodes, x=0 face fixed, E=1e5, nu=0.3, rho=1000, "
-              "g=-9.81 z, dt=1/60, rtol 1e-6, max 200 CG iterations",
+    "batch128": "128 bodies, n_x,n_y,n_z ~ U{8..24} (seed 4), Kuhn split, h 1/32 m, 3,203,808 tets, 639,897 nodes, "
+                "per-tet log E ~ N(log 1e5, 0.2), nu 0.3, rho 1000, x=0 face fixed per body, per-body loss on the +x "
+                "face vs a U[-0.05,0.05]^3 m offset target, dt 1/60 s, forward rtol 1e-6, adjoint rtol 1e-7, "
+                "checkpoint every 8",
+    "diff32": "32^3-cell Kuhn beam, 196,608 tets, 35,937 nodes, per-tet log E ~ N(log 1e5, 0.2) (seed 3), nu 0.3, "
+              "rho 1000, x=0 face fixed, loss on the +x face vs 0.05 m down, dt 1/60 s, rtol 1e-6, checkpoint every "
+              "10",
+    "twist64": "64^3-cell Kuhn beam, 1,572,864 tets, 274,625 nodes, rest nodes jittered +-10% (seed 1), 90-degree "
+               "initial twist, x=0 face fixed, E=1e6 Pa, nu=0.45, rho=1000, g=-9.81 z, dt=1/120 s, rtol 1e-6, max "
+               "1024 CG iterations",
+    "beam10": "10^3-cell 5-tet beam, 5,000 tets, 1,331 nodes, x=0 face fixed, E=1e5, nu=0.3, rho=1000, g=-9.81 z, "
+              "dt=1/60, rtol 1e-6, max 200 CG iterations",
     "batch256": "256 bodies of U{4..32}^3 cells, 5-tet split (seed 2), E log-uniform [1e4,1e6], nu U[0.3,0.45], "
                 "one random face fixed, v0 ~ N(0, 0.5 m/s), dt=1/60, rtol 1e-6, one CG per body",
 }
+DIFFERENTIABLE = ("batch128", "diff32")
+DEFAULT_PRECISION = {"batch128": "fp32"}
+DEFAULT_STEPS = {"batch128": 40, "diff32": 50, "twist64": 50, "beam10": 100, "batch256": 100}
+
+
+def workload(name):
+    from physik_amd import workloads as W
+    return {"twist64": W.twist64, "beam10": W.beam10, "batch256": W.batch256, "diff32": W.diff32,
+            "batch128": W.batch128}[name]()
 
 
 def spmv_bytes(nnzf, nd, s):
@@ -76,54 +89,219 @@ def measured_traffic(workload_name, precision):
         return None
 
 
-def cpu_baseline(w, name, budget_s=30.0):
-    """The reference step on one core: oracle/_ref (reference sources) when
-    built, else the oracle's FP32 restatement. Times whole PHYSIK_Step calls
-    after setup, as many as fit the budget (at least one)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import checkers
-    ncores = 1
-    if checkers.reference_available():
-        kind = "reference"
-        sim = checkers.Reference(w.rest, w.tets, float(w.density[0]), float(w.young[0]), float(w.poisson[0]),
-                                 float(w.damping[0]))
-        if not (w.young == w.young[0]).all():
-            sim.set_young(w.young)
-        sim.set_params(w.gravity, 1, w.tol, w.max_iter)
-    else:
-        kind = "port"
-        sim = checkers.Oracle(32, w.rest, w.tets, w.young, w.poisson, w.damping, w.density)
-        sim.set_params(w.gravity, 1, w.tol, w.max_iter)
-    sim.set_fixed(w.fixed)
-    sim.set_state(w.x0, w.v0)
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        sim.step(w.dt)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or steps >= w.steps or el * (steps + 1) / steps > budget_s * 1.5:
-            break
-    it = sim.cg()[0]
-    sim.close()
+def roofline(s, name, prec_name):
+    from physik_amd import native as N
+    n_spmv, ms_spmv = s.kernel_time(N.K_SPMV)
+    esize = 8 if s.precision == N.PK_FP64 else 4
+    per_launch = spmv_bytes(s.nnzb_full, s.nd, esize)
+    avg_s = ms_spmv / max(n_spmv, 1) * 1e-3
+    achieved = per_launch / avg_s / 1e9 if n_spmv else 0.0
     return {
+        "bound": "hbm",
+        "kernel": "k_spmv (q = A d_k with d_k = z_k + beta d_{k-1} fused, d.q partial)",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": measured_traffic(name, prec_name),
+        "bytes_per_launch": per_launch,
+        "plain_spmv_bytes_per_launch": spmv_bytes_plain(s.nnzb_full, s.nd, esize),
+        "avg_launch_us": avg_s * 1e6,
+        "launches": n_spmv,
+    }
+
+
+def kernels_ms(s):
+    from physik_amd import native as N
+    return {N.KERNEL_NAMES[k]: round(s.kernel_time(k)[1], 3) for k in range(7)}
+
+
+def prepare(name, prec_name, args):
+    from physik_amd import native as N
+    from physik_amd import workloads as W
+    w = workload(name)
+    prec = N.PK_FP64 if prec_name == "fp64" else N.PK_FP32_STORAGE
+    s = W.make_system(w, prec)
+    if args.shape:
+        s.tune(*[int(a) for a in args.shape.split(",")])
+        shape = s.get_tune()
+    else:
+        s.step(w.dt)
+        s.set_state(w.x0, w.v0)
+        shape = s.get_tune() if args.no_autotune else s.autotune(w.dt)
+    s.set_state(w.x0, w.v0)
+    return w, s, dict(zip(("lanes_per_row", "spmv_rows", "vector_rows", "unroll"), shape))
+
+
+def run_plain(name, prec_name, steps, warmup, args):
+    """K plain implicit steps from the workload's initial state."""
+    w, s, shape = prepare(name, prec_name, args)
+    for _ in range(warmup):
+        s.step(w.dt)
+    s.set_state(w.x0, w.v0)
+    s.profile(True)
+    t0 = time.perf_counter()
+    s.step(w.dt, 1, steps)  # returns after the stream has drained
+    el = time.perf_counter() - t0
+    its = s.results()[0]
+    out = {
+        "workload": f"{name}: {DESCRIPTION[name]}",
         "value": w.n_tets * steps / el,
         "unit": "tet·steps/s",
-        "cores": ncores,
+        "steps": steps,
+        "ms_per_step": el / steps * 1e3,
+        "precision": prec_name,
+        "tets": w.n_tets, "nodes": w.n_nodes, "bodies": w.n_bodies, "dynamic_rows": s.nd,
+        "nnzb_full": s.nnzb_full, "nnzb_upper": s.nnzb_upper,
+        "cg_iterations_last": int(its.max()),
+        "cg_launch_shape": shape,
+        "roofline": roofline(s, name, prec_name),
+        "kernels_ms": kernels_ms(s),
+    }
+    s.profile(False)
+    s.close()
+    return out, w
+
+
+def run_fwd_adj(name, prec_name, steps, warmup, args):
+    """Record K substeps with checkpoints, then back-propagate the loss."""
+    from physik_amd import workloads as W
+    w, s, shape = prepare(name, prec_name, args)
+    every = w.checkpoint or steps
+    for _ in range(warmup):
+        s.set_state(w.x0, w.v0)
+        s.record(w.dt, steps, every)
+        x, _ = s.state()
+        s.backward(W.loss_and_seed(w, x)[1], None, tol=w.adj_tol, max_iter=w.max_iter)
+    s.set_state(w.x0, w.v0)
+    s.profile(True)
+    t0 = time.perf_counter()
+    s.record(w.dt, steps, every)
+    t1 = time.perf_counter()
+    x, _ = s.state()
+    loss, xbar = W.loss_and_seed(w, x)
+    t2 = time.perf_counter()
+    gE, _, gv, adj_its = s.backward(xbar, None, tol=w.adj_tol, max_iter=w.max_iter)
+    t3 = time.perf_counter()
+    el = (t1 - t0) + (t3 - t2)
+    out = {
+        "workload": f"{name}: {DESCRIPTION[name]}",
+        "value": w.n_tets * steps / el,
+        "unit": "tet·steps/s",
+        "steps": steps,
+        "ms_per_step": el / steps * 1e3,
+        "forward_ms": (t1 - t0) * 1e3,
+        "backward_ms": (t3 - t2) * 1e3,
+        "precision": prec_name,
+        "tets": w.n_tets, "nodes": w.n_nodes, "bodies": w.n_bodies, "dynamic_rows": s.nd,
+        "nnzb_full": s.nnzb_full, "checkpoint_every": every,
+        "loss": loss, "grad_log_young_norm": float(np.linalg.norm(gE)), "grad_v0_norm": float(np.linalg.norm(gv)),
+        "cg_iterations_last_forward": int(s.results()[0].max()),
+        "adjoint_cg_iterations": {"mean": float(np.mean(adj_its)), "max": int(np.max(adj_its))},
+        "cg_launch_shape": shape,
+        "roofline": roofline(s, name, prec_name),
+        "kernels_ms": kernels_ms(s),
+    }
+    s.profile(False)
+    s.close()
+    return out, w
+
+
+def _oracle(checkers, wb, precision=64):
+    o = checkers.Oracle(precision, wb.rest, wb.tets, wb.young, wb.poisson, wb.damping, wb.density)
+    o.set_params(wb.gravity, 1, wb.tol, wb.max_iter)
+    o.set_fixed(wb.fixed)
+    o.set_state(wb.x0, wb.v0)
+    return o
+
+
+def cpu_fwd_adj(w, name, steps, budget_s):
+    """The oracle's FP64 forward + adjoint on one host core, body by body of
+    the same workload, as many bodies as fit the budget (at least one)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import checkers
+    from physik_amd import workloads as W
+    tets, bodies, el = 0, 0, 0.0
+    for b in range(w.n_bodies):
+        wb = w.body(b) if w.n_bodies > 1 else w
+        # the loss seed needs the final state: an untimed forward run first
+        p = _oracle(checkers, wb)
+        for _ in range(steps):
+            p.step(wb.dt)
+        xbar = W.loss_and_seed(wb, p.state()[0])[1]
+        p.close()
+        o = _oracle(checkers, wb)
+        t0 = time.perf_counter()
+        o.adjoint(wb.dt, steps, xbar, None, tol=wb.adj_tol, max_iter=wb.max_iter)
+        el += time.perf_counter() - t0
+        o.close()
+        bodies += 1
+        tets += wb.n_tets
+        if el >= budget_s:
+            break
+    return {
+        "value": tets * steps / max(el, 1e-9),
+        "unit": "tet·steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{bodies} of {w.n_bodies} bodies of {name} ({tets:,} tets), {steps} substeps forward + adjoint "
+                  f"each (record, then reverse), FP64 oracle restatement (the reference has no adjoint), one host "
+                  f"core, {el:.1f} s timed",
+    }
+
+
+def cpu_reference_forward(w, name, steps, budget_s):
+    """The reference's own FP32 forward step (its sources compiled in place,
+    else the oracle's FP32 restatement) on one host core, body by body."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import checkers
+    kind = "reference" if checkers.reference_available() else "port"
+    tet_steps, el, bodies, last_it = 0, 0.0, 0, 0
+    for b in range(w.n_bodies):
+        wb = w.body(b) if w.n_bodies > 1 else w
+        if kind == "reference":
+            sim = checkers.Reference(wb.rest, wb.tets, float(wb.density[0]), float(wb.young[0]),
+                                     float(wb.poisson[0]), float(wb.damping[0]))
+            if not (wb.young == wb.young[0]).all():
+                sim.set_young(wb.young)
+            sim.set_params(wb.gravity, 1, wb.tol, wb.max_iter)
+            sim.set_fixed(wb.fixed)
+            sim.set_state(wb.x0, wb.v0)
+        else:
+            sim = _oracle(checkers, wb, 32)
+        bodies += 1
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            sim.step(wb.dt)
+            el += time.perf_counter() - t0
+            tet_steps += wb.n_tets
+            if el >= budget_s:
+                break
+        last_it = sim.cg()[0]
+        sim.close()
+        if el >= budget_s:
+            break
+    return {
+        "value": tet_steps / max(el, 1e-9),
+        "unit": "tet·steps/s",
+        "cores": 1,
         "kind": kind,
-        "sample": f"{steps} step(s) of {name} in FP32 from its initial state on one host core "
-                  f"({el:.1f} s, last CG {it} iterations; setup excluded)",
+        "sample": f"{tet_steps:,} tet-steps of {name} ({bodies} bod{'y' if bodies == 1 else 'ies'}, forward only, "
+                  f"FP32, from the initial state) on one host core, {el:.1f} s timed, last CG {last_it} iterations; "
+                  "setup excluded",
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="twist64", choices=["twist64", "beam10", "batch256"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="batch128", choices=list(DESCRIPTION))
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32"])
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain single-body step beside the headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default CG launch shape")
     ap.add_argument("--shape", default=None, help="CG launch shape lanes,spmv_rows,vector_rows,unroll (no autotune)")
     args = ap.parse_args()
@@ -132,75 +310,41 @@ def main():
               f"paths shard across GPUs by body or by slab, which this build does not launch)", file=sys.stderr)
         sys.exit(2)
 
-    from physik_amd import native as N
-    from physik_amd import workloads as W
-
-    w = workload(args.workload)
-    prec = N.PK_FP64 if args.precision == "fp64" else N.PK_FP32_STORAGE
-    s = W.make_system(w, prec)
-    for _ in range(args.warmup):
-        s.step(w.dt)
-    s.set_state(w.x0, w.v0)
-    if args.shape:
-        s.tune(*[int(a) for a in args.shape.split(",")])
-        shape = s.get_tune()
-    else:
-        shape = s.get_tune() if args.no_autotune else s.autotune(w.dt)
-    s.set_state(w.x0, w.v0)  # the timed run starts from the workload's initial state
-    s.profile(True)
-    t0 = time.perf_counter()
-    s.step(w.dt, 1, args.steps)  # returns after the stream has drained
-    el = time.perf_counter() - t0
-    its = s.results()[0]
-    n_spmv, ms_spmv = s.kernel_time(N.K_SPMV)
-    kernels = {N.KERNEL_NAMES[k]: dict(zip(("launches", "ms"), s.kernel_time(k))) for k in range(7)}
-    s.profile(False)
-
-    esize = 8 if prec == N.PK_FP64 else 4
-    per_launch = spmv_bytes(s.nnzb_full, s.nd, esize)
-    avg_s = ms_spmv / max(n_spmv, 1) * 1e-3
-    achieved = per_launch / avg_s / 1e9 if n_spmv else 0.0
-    traffic = measured_traffic(args.workload, args.precision)
-    value = w.n_tets * args.steps / el
+    name = args.workload
+    prec_name = args.precision or DEFAULT_PRECISION.get(name, "fp64")
+    steps = args.steps or DEFAULT_STEPS[name]
+    diff = name in DIFFERENTIABLE
+    head, w = (run_fwd_adj if diff else run_plain)(name, prec_name, steps, args.warmup, args)
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "tet·steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": el / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == N.PK_FP64 else "f32 storage, f64 dots",
-        "data": "synthetic (seeded meshes, no checkpoint)",
-        "config": {
-            "workload": f"{args.workload}: {DESCRIPTION[args.workload]}",
-            "tets": w.n_tets, "nodes": w.n_nodes, "bodies": w.n_bodies, "dynamic_rows": s.nd,
-            "nnzb_full": s.nnzb_full, "nnzb_upper": s.nnzb_upper,
-            "cg_iterations_last": int(its.max()), "precision": args.precision,
-            "cg_launch_shape": dict(zip(("lanes_per_row", "spmv_rows", "vector_rows", "unroll"), shape)),
-            "parallelism": "single GPU",
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_spmv (q = A d_k with d_k = z_k + beta d_{k-1} fused, d.q partial)",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "bytes_per_launch": per_launch,
-            "plain_spmv_bytes_per_launch": spmv_bytes_plain(s.nnzb_full, s.nd, esize),
-            "avg_launch_us": avg_s * 1e6,
-            "launches": n_spmv,
-        },
-        "kernels_ms": {k: round(v["ms"], 3) for k, v in kernels.items()},
+        "dtype": "f64" if prec_name == "fp64" else "f32 storage, f64 dots",
+        "data": "synthetic (seeded meshes, no checkpoint files)",
+        "config": {"workload": head["workload"], "mode": "forward+adjoint" if diff else "forward",
+                   "parallelism": "single GPU",
+                   **{k: v for k, v in head.items() if k not in ("workload", "value", "unit", "roofline",
+                                                                 "kernels_ms", "ms_per_step", "steps")}},
+        "roofline": head["roofline"],
+        "kernels_ms": head["kernels_ms"],
     }
-    s.close()
+    if not args.no_plain and name != "twist64":
+        plain, _ = run_plain("twist64", "fp64", DEFAULT_STEPS["twist64"], args.warmup, args)
+        line["plain_step"] = plain
+        line["vs_plain_step"] = head["value"] / plain["value"]
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(w, args.workload, args.cpu_budget)
+        if diff:
+            line["cpu_baseline"] = cpu_fwd_adj(w, name, steps, args.cpu_budget)
+            line["cpu_reference_forward"] = cpu_reference_forward(w, name, steps, args.cpu_budget / 2)
+        else:
+            line["cpu_baseline"] = cpu_reference_forward(w, name, steps, args.cpu_budget)
     print(json.dumps(line), flush=True)
 
 

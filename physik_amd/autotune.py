@@ -9,7 +9,9 @@ rather than guessed:
 
   stage 1  lanes per row, with (128 rows, unroll 2);
   stage 2  rows per workgroup x unroll, with the best lane count;
-  stage 3  rows per vector-kernel workgroup (the update kernel).
+  stage 3  rows per vector-kernel workgroup. It sets the update kernel's
+           grid and also how many partial sums every SpMV workgroup reduces
+           first, so this stage minimises the SpMV and update times together.
 
 Each candidate runs one implicit step from the same state with kernel timing
 on (the SpMV's time is its in-kernel wall-clock span); the state is restored
@@ -87,7 +89,7 @@ def autotune(system, dt, cache=None, verbose=False):
     best_l = min(LANES, key=lambda lanes: trial((lanes, 128, vec0, 2))[0])
     cands = [(best_l, r, vec0, u) for r in ROWS for u in UNROLL if _valid(best_l, u)]
     best = min(cands, key=lambda s: trial(s)[0])
-    best = min(((best[0], best[1], vr, best[3]) for vr in VEC_ROWS), key=lambda s: trial(s)[1])
+    best = min(((best[0], best[1], vr, best[3]) for vr in VEC_ROWS), key=lambda s: sum(trial(s)))
     system.tune(*best)
     system.set_state(x, v)
     if cache:

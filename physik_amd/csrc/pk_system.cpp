@@ -799,7 +799,10 @@ static int replay_substep(pk_system* s, double dt, int* ok)
         return rc;
     launch_record_ok(s->view(), ok);
     PK_HIP(hipGetLastError());
-    return read_groups(s, s->lastIters);
+    if ((rc = read_groups(s, s->lastIters)) != PK_OK)
+        return rc;
+    harvest_profile(s);
+    return PK_OK;
 }
 
 // Reverses substep j of the current segment: (segX[j], segV[j]) -> segV[j+1].
@@ -822,6 +825,7 @@ static int reverse_substep(pk_system* s, double dt, int j, double adjTol, int ad
     PK_HIP(hipGetLastError());
     if ((rc = read_groups(s, s->lastItersAdj)) != PK_OK)
         return rc;
+    harvest_profile(s);
     if (iters)
         *iters = s->lastItersAdj;
     return PK_OK;
@@ -927,12 +931,8 @@ int pk_system_record(pk_system* s, double dt, int steps, int every)
         set_error("pk_system_record: no dynamic node");
         return PK_ERR_ARG;
     }
-    const bool p = s->prof.on;
-    s->prof.on = false;
-    const int rc = s->precision == PK_FP32_STORAGE ? record_run<float>(s, dt, steps, every)
-                                                   : record_run<double>(s, dt, steps, every);
-    s->prof.on = p;
-    return rc;
+    return s->precision == PK_FP32_STORAGE ? record_run<float>(s, dt, steps, every)
+                                           : record_run<double>(s, dt, steps, every);
 }
 
 int pk_system_backward(pk_system* s, const double* xbar, const double* vbar, double adj_tol, int adj_max_iter,
@@ -961,11 +961,8 @@ int pk_system_backward(pk_system* s, const double* xbar, const double* vbar, dou
         PK_HIP(hipMemsetAsync(s->vbar.p, 0, n3 * sizeof(double), s->stream));
     if (t.T)
         PK_HIP(hipMemsetAsync(s->gradE.p, 0, static_cast<size_t>(t.T) * sizeof(double), s->stream));
-    const bool p = s->prof.on;
-    s->prof.on = false;
     rc = s->precision == PK_FP32_STORAGE ? backward_run<float>(s, adj_tol, adj_max_iter, adj_iterations)
                                          : backward_run<double>(s, adj_tol, adj_max_iter, adj_iterations);
-    s->prof.on = p;
     if (rc != PK_OK)
         return rc;
     if (grad_log_young && t.T)

@@ -21,7 +21,7 @@ from . import workloads as W
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-mb", type=int, default=512)
-    ap.add_argument("--workload", default="twist64", choices=["twist64", "beam10", "batch256"])
+    ap.add_argument("--workload", default="twist64", choices=["twist64", "beam10", "batch256", "diff32", "batch128"])
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--shape", default=None,
