@@ -65,14 +65,9 @@ def workload(name):
 
 
 def spmv_bytes(nnzf, nd, s):
-    """Algorithmic bytes of one fused SpMV launch: the full BSR once (9 values
-    + int32 column per block, row pointers), the direction update's reads of z
-    and d_{k-1} and its write of d_k, and the write of q = A d_k."""
-    return nnzf * (9 * s + 4) + (nd + 1) * 4 + 4 * nd * 3 * s
-
-
-def spmv_bytes_plain(nnzf, nd, s):
-    """The same for a plain SpMV y = A x (x read once, y written once)."""
+    """Algorithmic bytes of one SpMV launch q = A d: the full BSR once (9
+    values + int32 column per block, row pointers), d read once, q written
+    once."""
     return nnzf * (9 * s + 4) + (nd + 1) * 4 + 2 * nd * 3 * s
 
 
@@ -98,14 +93,13 @@ def roofline(s, name, prec_name):
     achieved = per_launch / avg_s / 1e9 if n_spmv else 0.0
     return {
         "bound": "hbm",
-        "kernel": "k_spmv (q = A d_k with d_k = z_k + beta d_{k-1} fused, d.q partial)",
+        "kernel": "k_spmv (q = A d_k on the full BSR, d.q partial per tile)",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": measured_traffic(name, prec_name),
         "bytes_per_launch": per_launch,
-        "plain_spmv_bytes_per_launch": spmv_bytes_plain(s.nnzb_full, s.nd, esize),
         "avg_launch_us": avg_s * 1e6,
         "launches": n_spmv,
     }

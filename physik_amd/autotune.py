@@ -9,9 +9,9 @@ rather than guessed:
 
   stage 1  lanes per row, with (128 rows, unroll 2);
   stage 2  rows per workgroup x unroll, with the best lane count;
-  stage 3  rows per vector-kernel workgroup. It sets the update kernel's
-           grid and also how many partial sums every SpMV workgroup reduces
-           first, so this stage minimises the SpMV and update times together.
+  stage 3  rows per vector-kernel workgroup: the grid of the direction and
+           update kernels, whose every workgroup first re-sums the other
+           kernel's partials; this stage minimises SpMV + direction + update.
 
 Each candidate runs one implicit step from the same state with kernel timing
 on (the SpMV's time is its in-kernel wall-clock span); the state is restored
@@ -48,10 +48,11 @@ def _measure(system, shape, dt, x, v):
     system.step(dt)
     n_s, ms_s = system.kernel_time(N.K_SPMV)
     n_u, ms_u = system.kernel_time(N.K_UPDATE)
+    n_d, ms_d = system.kernel_time(N.K_DIRECTION)
     system.profile(False)
     spmv = ms_s / n_s if n_s else float("inf")
-    upd = ms_u / n_u if n_u else float("inf")
-    return spmv, upd
+    vec = (ms_u + ms_d) / n_u if n_u else float("inf")  # update + direction per iteration
+    return spmv, vec
 
 
 def autotune(system, dt, cache=None, verbose=False):
@@ -83,7 +84,7 @@ def autotune(system, dt, cache=None, verbose=False):
         spmv, upd = _measure(system, shape, dt, x, v)
         log.append((shape, spmv, upd))
         if verbose:
-            print(f"autotune {shape}: spmv {spmv * 1e3:.1f} us, update {upd * 1e3:.1f} us", flush=True)
+            print(f"autotune {shape}: spmv {spmv * 1e3:.1f} us, direction + update {upd * 1e3:.1f} us", flush=True)
         return spmv, upd
 
     best_l = min(LANES, key=lambda lanes: trial((lanes, 128, vec0, 2))[0])

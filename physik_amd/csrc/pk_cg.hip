@@ -1,19 +1,22 @@
 // Block-Jacobi PCG kernels for gfx950 (ConjugateGradientSolver.cpp:114-241).
 //
-// Iteration k is two launches:
-//   k_spmv(k)    every workgroup first sums the (r.z, r.r) partials the
-//                previous launch left for its body, takes the reference's
-//                stop decisions for the end of iteration k-1 and forms
-//                beta = delta_k / delta_{k-1}; it then applies
-//                d_k = z_k + beta d_{k-1} on the fly while computing
-//                q = A d_k on the full BSR (both triangles), stores d_k for its
-//                own rows and leaves a d.q partial.
-//   k_update(k)  every workgroup sums the body's d.q partials, alpha =
-//                delta_k / d.q (or stops on d.q <= 0), updates x += alpha d,
-//                r -= alpha q, z = M^-1 r and leaves (r.z, r.r) partials.
-// Totals are recomputed by every workgroup from the same partials in the same
-// order, so no workgroup ever waits on another and no device-scope fence is
-// needed inside a launch. One designated workgroup per body records the
+// Iteration k is three launches:
+//   k_dir(k)     vector tiles. Every workgroup sums the (r.z, r.r) partials
+//                the previous launch left for its body, takes the reference's
+//                stop decisions for the end of iteration k-1 (k = 0: the CG
+//                start) and forms beta = delta_k / delta_{k-1}; it then writes
+//                d_k = z_k + beta d_{k-1} for its rows.
+//   k_spmv(k)    BSR row tiles, no reduction prologue: q = A d_k on the full
+//                BSR (both triangles) and a d.q partial per tile.
+//   k_update(k)  vector tiles. Every workgroup sums the body's d.q partials,
+//                alpha = delta_k / d.q (or stops on d.q <= 0), updates
+//                x += alpha d, r -= alpha q, z = M^-1 r and leaves (r.z, r.r)
+//                partials.
+// Totals are recomputed by every workgroup of the two vector kernels from the
+// same partials in the same order, so no workgroup waits on another and no
+// device-scope fence is needed inside a launch; the vector kernels have few
+// workgroups, so the redundant sums are cheap, and the SpMV (thousands of
+// workgroups) carries none. One designated workgroup per body records the
 // decisions (delta, stop, iterations, residual) for later launches and the
 // host; the host polls the count of bodies still iterating.
 //
@@ -137,17 +140,16 @@ __device__ inline bool stop_rules(int k, GroupState& s, double rz, double rr, in
     return stop;
 }
 
-template <class M, int LPR, int U>
-__device__ inline void spmv_body(int k, const TileView& st, const TileView& vt, const int* __restrict__ fstart,
-                                 const int* __restrict__ fcol, const M* __restrict__ vals, long long nnzf,
-                                 const M* __restrict__ dprev, M* __restrict__ dcur, const M* __restrict__ z,
-                                 M* __restrict__ q, double* __restrict__ pa, const Partial* __restrict__ pbPrev,
-                                 GroupState* gs, int* active)
+// d_k = z_k + beta d_{k-1} for the tile's rows, after the stop decisions for
+// the end of iteration k-1. At k = 0 the right-hand side already wrote d_0.
+template <class M>
+__device__ inline void dir_body(int k, const TileView& vt, const M* __restrict__ z, const M* __restrict__ dprev,
+                                M* __restrict__ dcur, const Partial* __restrict__ pbPrev, GroupState* gs, int* active)
 {
-    const int t = tile_of(st);
+    const int t = tile_of(vt);
     if (t < 0)
         return;
-    const RowTile tl = st.tiles[t];
+    const RowTile tl = vt.tiles[t];
     const int g = tl.group;
     GroupState& s = gs[g];
     if (read_uniform(&s.stopAt) >= 0)
@@ -156,28 +158,44 @@ __device__ inline void spmv_body(int k, const TileView& st, const TileView& vt, 
     int flags;
     sum_partials(pbPrev, vt.groupBegin[g], vt.groupCount[g], rz, rr, flags);
     double beta;
-    if (stop_rules(k, s, rz, rr, flags, t == st.groupBegin[g] && threadIdx.x == 0, active, beta))
+    if (stop_rules(k, s, rz, rr, flags, t == vt.groupBegin[g] && threadIdx.x == 0, active, beta))
+        return;
+    if (k == 0)
+        return;
+    for (int i = 3 * tl.rowBegin + static_cast<int>(threadIdx.x); i < 3 * tl.rowEnd; i += NT)
+        dcur[i] = direction(z[i], dprev[i], beta);
+}
+
+template <class M>
+__global__ __launch_bounds__(NT) void k_dir(int k, TileView vt, const M* __restrict__ z, const M* __restrict__ dprev,
+                                            M* __restrict__ dcur, const Partial* __restrict__ pbPrev, GroupState* gs,
+                                            int* active, unsigned long long* stamp)
+{
+    stamp_begin(stamp);
+    dir_body<M>(k, vt, z, dprev, dcur, pbPrev, gs, active);
+    stamp_end(stamp);
+}
+
+// q = A d_k for the tile's rows and the tile's d.q partial. A body whose loop
+// ended at or before k (decided by k_dir(k) or earlier) is skipped.
+template <class M, int LPR, int U>
+__device__ inline void spmv_body(int k, const TileView& st, const int* __restrict__ fstart,
+                                 const int* __restrict__ fcol, const M* __restrict__ vals, long long nnzf,
+                                 const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
+                                 const GroupState* gs)
+{
+    const int t = tile_of(st);
+    if (t < 0)
+        return;
+    const RowTile tl = st.tiles[t];
+    const int stopAt = read_uniform(&gs[tl.group].stopAt);
+    if (stopAt >= 0 && stopAt <= k)
         return;
 
     const int lane = threadIdx.x % LPR;
     const int sub = threadIdx.x / LPR;
     constexpr int RPP = NT / LPR;
     const size_t F = static_cast<size_t>(nnzf);
-    // d_k of column c: stored at k = 0, formed from z and d_{k-1} after
-    auto dir = [&](int c, double* o) {
-        if (k == 0)
-        {
-            o[0] = dcur[3 * c];
-            o[1] = dcur[3 * c + 1];
-            o[2] = dcur[3 * c + 2];
-        }
-        else
-        {
-            o[0] = direction(z[3 * c], dprev[3 * c], beta);
-            o[1] = direction(z[3 * c + 1], dprev[3 * c + 1], beta);
-            o[2] = direction(z[3 * c + 2], dprev[3 * c + 2], beta);
-        }
-    };
     double pd = 0.0;
     for (int row0 = tl.rowBegin + sub; row0 < tl.rowEnd; row0 += RPP * U)
     {
@@ -206,13 +224,14 @@ __device__ inline void spmv_body(int k, const TileView& st, const TileView& vt, 
 #pragma unroll
         for (int u = 0; u < U; ++u)
         {
-            double dc[3];
-            dir(col[u], dc);
+            const int c = col[u];
+            const double d0 = static_cast<double>(d[3 * c]), d1 = static_cast<double>(d[3 * c + 1]),
+                         d2 = static_cast<double>(d[3 * c + 2]);
             const bool has = kb[u] < ke[u];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                y[u][i] = has ? static_cast<double>(a[u][3 * i]) * dc[0] + static_cast<double>(a[u][3 * i + 1]) * dc[1] +
-                                    static_cast<double>(a[u][3 * i + 2]) * dc[2]
+                y[u][i] = has ? static_cast<double>(a[u][3 * i]) * d0 + static_cast<double>(a[u][3 * i + 1]) * d1 +
+                                    static_cast<double>(a[u][3 * i + 2]) * d2
                               : 0.0;
         }
         // rows longer than LPR blocks
@@ -220,13 +239,14 @@ __device__ inline void spmv_body(int k, const TileView& st, const TileView& vt, 
         for (int u = 0; u < U; ++u)
             for (int kk = kb[u] + LPR; kk < ke[u]; kk += LPR)
             {
-                double dc[3];
-                dir(fcol[kk], dc);
+                const int c = fcol[kk];
+                const double d0 = static_cast<double>(d[3 * c]), d1 = static_cast<double>(d[3 * c + 1]),
+                             d2 = static_cast<double>(d[3 * c + 2]);
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
-                    y[u][i] += static_cast<double>(__builtin_nontemporal_load(vals + (3 * i) * F + kk)) * dc[0] +
-                               static_cast<double>(__builtin_nontemporal_load(vals + (3 * i + 1) * F + kk)) * dc[1] +
-                               static_cast<double>(__builtin_nontemporal_load(vals + (3 * i + 2) * F + kk)) * dc[2];
+                    y[u][i] += static_cast<double>(__builtin_nontemporal_load(vals + (3 * i) * F + kk)) * d0 +
+                               static_cast<double>(__builtin_nontemporal_load(vals + (3 * i + 1) * F + kk)) * d1 +
+                               static_cast<double>(__builtin_nontemporal_load(vals + (3 * i + 2) * F + kk)) * d2;
             }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -241,19 +261,13 @@ __device__ inline void spmv_body(int k, const TileView& st, const TileView& vt, 
             const int row = row0 + u * RPP;
             if (lane == 0 && row < tl.rowEnd)
             {
-                double dr[3];
-                dir(row, dr);
-                if (k > 0)
-                {
-                    dcur[3 * row] = static_cast<M>(dr[0]);
-                    dcur[3 * row + 1] = static_cast<M>(dr[1]);
-                    dcur[3 * row + 2] = static_cast<M>(dr[2]);
-                }
                 const M q0 = static_cast<M>(y[u][0]), q1 = static_cast<M>(y[u][1]), q2 = static_cast<M>(y[u][2]);
                 q[3 * row] = q0;
                 q[3 * row + 1] = q1;
                 q[3 * row + 2] = q2;
-                pd += dr[0] * static_cast<double>(q0) + dr[1] * static_cast<double>(q1) + dr[2] * static_cast<double>(q2);
+                pd += static_cast<double>(d[3 * row]) * static_cast<double>(q0) +
+                      static_cast<double>(d[3 * row + 1]) * static_cast<double>(q1) +
+                      static_cast<double>(d[3 * row + 2]) * static_cast<double>(q2);
             }
         }
     }
@@ -264,15 +278,14 @@ __device__ inline void spmv_body(int k, const TileView& st, const TileView& vt, 
 }
 
 template <class M, int LPR, int U>
-__global__ __launch_bounds__(NT) void k_spmv(int k, TileView st, TileView vt, const int* __restrict__ fstart,
+__global__ __launch_bounds__(NT) void k_spmv(int k, TileView st, const int* __restrict__ fstart,
                                              const int* __restrict__ fcol, const M* __restrict__ vals,
-                                             long long nnzf, const M* __restrict__ dprev, M* __restrict__ dcur,
-                                             const M* __restrict__ z, M* __restrict__ q, double* __restrict__ pa,
-                                             const Partial* __restrict__ pbPrev, GroupState* gs, int* active,
+                                             long long nnzf, const M* __restrict__ d, M* __restrict__ q,
+                                             double* __restrict__ pa, const GroupState* gs,
                                              unsigned long long* stamp)
 {
     stamp_begin(stamp);
-    spmv_body<M, LPR, U>(k, st, vt, fstart, fcol, vals, nnzf, dprev, dcur, z, q, pa, pbPrev, gs, active);
+    spmv_body<M, LPR, U>(k, st, fstart, fcol, vals, nnzf, d, q, pa, gs);
     stamp_end(stamp);
 }
 
@@ -415,10 +428,18 @@ template <class M, int LPR, int U>
 static void launch_spmv_shape(const DeviceView& v, int k, unsigned long long* stamp)
 {
     hipLaunchKernelGGL((k_spmv<M, LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), 0, v.stream, k,
-                       v.spmv, v.vec, v.fstart, v.fcol, static_cast<const M*>(v.vals), v.nnzf,
-                       static_cast<const M*>(v.d[(k + 1) & 1]), static_cast<M*>(v.d[k & 1]),
-                       static_cast<const M*>(v.z), static_cast<M*>(v.q), v.pa, v.pb[(k + 1) & 1], v.gs, v.active,
-                       stamp);
+                       v.spmv, v.fstart, v.fcol, static_cast<const M*>(v.vals), v.nnzf,
+                       static_cast<const M*>(v.d[k & 1]), static_cast<M*>(v.q), v.pa, v.gs, stamp);
+}
+
+template <class M>
+void launch_dir(const DeviceView& v, int k, unsigned long long* stamp)
+{
+    if (v.vec.count == 0)
+        return;
+    hipLaunchKernelGGL((k_dir<M>), dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, k, v.vec,
+                       static_cast<const M*>(v.z), static_cast<const M*>(v.d[(k + 1) & 1]), static_cast<M*>(v.d[k & 1]),
+                       v.pb[(k + 1) & 1], v.gs, v.active, stamp);
 }
 
 template <class M>
@@ -469,6 +490,8 @@ void launch_spmv_plain(const DeviceView& v, const double* x, double* y)
                            v.fcol, static_cast<const M*>(v.vals), v.nnzf, x, y);
 }
 
+template void launch_dir<double>(const DeviceView&, int, unsigned long long*);
+template void launch_dir<float>(const DeviceView&, int, unsigned long long*);
 template void launch_spmv<double>(const DeviceView&, int, int, int, unsigned long long*);
 template void launch_spmv<float>(const DeviceView&, int, int, int, unsigned long long*);
 template void launch_update<double>(const DeviceView&, int, unsigned long long*);

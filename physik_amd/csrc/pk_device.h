@@ -222,6 +222,7 @@ template <class M> void launch_assemble(const DeviceView& v, double dt);
 template <class M> void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt);
 void launch_integrate(const DeviceView& v, double dt, int commit);
 void launch_explicit(const DeviceView& v, double gx, double gy, double gz, double dt);
+template <class M> void launch_dir(const DeviceView& v, int k, unsigned long long* stamp);
 template <class M>
 void launch_spmv(const DeviceView& v, int k, int lanesPerRow, int unroll, unsigned long long* stamp);
 template <class M> void launch_update(const DeviceView& v, int k, unsigned long long* stamp);

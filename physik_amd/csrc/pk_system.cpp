@@ -121,7 +121,7 @@ struct pk_system
     pk::DevBuf<pk::GroupState> gs;
     // profiling: per-workgroup (start, end) ticks of every CG launch, and
     // the per-launch spans reduced from them
-    pk::DevBuf<unsigned long long> rawSpmv, rawUpd, stampSpmv, stampUpd;
+    pk::DevBuf<unsigned long long> rawSpmv, rawUpd, rawDir, stampSpmv, stampUpd, stampDir;
     double wallClockKHz = 0.0;
     pk::Tiles vecTiles, spmvTiles;
     int* pinned = nullptr;
@@ -311,27 +311,34 @@ static void harvest_profile(pk_system* s)
     // end, from the wall clock read inside the kernels
     if (s->stampSpmv.p && s->wallClockKHz > 0.0)
     {
-        const int n = std::max(spmvWork, updWork);
-        std::vector<unsigned long long> a(2 * static_cast<size_t>(n)), b(2 * static_cast<size_t>(n));
-        if (n > 0 &&
-            hipMemcpy(a.data(), s->stampSpmv.p, a.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) ==
-                hipSuccess &&
-            hipMemcpy(b.data(), s->stampUpd.p, b.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) ==
-                hipSuccess)
+        // launch k of the direction kernel ran for every k <= the last stop;
+        // SpMV and update launches k < spmvWork (updWork) did work
+        int dirWork = 0;
+        for (const auto& g : s->hostGs)
+            if (g.hasRows)
+                dirWork = std::max(dirWork, g.stopAt + 1);
+        const int n = std::max(dirWork, std::max(spmvWork, updWork));
+        const size_t bytes = 2 * static_cast<size_t>(n) * sizeof(unsigned long long);
+        std::vector<unsigned long long> a(2 * static_cast<size_t>(n)), b(a.size()), c(a.size());
+        if (n > 0 && hipMemcpy(a.data(), s->stampSpmv.p, bytes, hipMemcpyDeviceToHost) == hipSuccess &&
+            hipMemcpy(b.data(), s->stampUpd.p, bytes, hipMemcpyDeviceToHost) == hipSuccess &&
+            hipMemcpy(c.data(), s->stampDir.p, bytes, hipMemcpyDeviceToHost) == hipSuccess)
         {
+            auto add = [&](int kernel, const std::vector<unsigned long long>& v, int k) {
+                if (v[2 * k + 1] > v[2 * k])
+                {
+                    p.ms[kernel] += static_cast<double>(v[2 * k + 1] - v[2 * k]) / s->wallClockKHz;
+                    p.launches[kernel] += 1;
+                }
+            };
             for (int k = 0; k < n; ++k)
             {
-                // launches k < spmvWork (updWork) all ran, so their spans are set
-                if (k < spmvWork && a[2 * k + 1] > a[2 * k])
-                {
-                    p.ms[PK_K_SPMV] += static_cast<double>(a[2 * k + 1] - a[2 * k]) / s->wallClockKHz;
-                    p.launches[PK_K_SPMV] += 1;
-                }
-                if (k < updWork && b[2 * k + 1] > b[2 * k])
-                {
-                    p.ms[PK_K_UPDATE] += static_cast<double>(b[2 * k + 1] - b[2 * k]) / s->wallClockKHz;
-                    p.launches[PK_K_UPDATE] += 1;
-                }
+                if (k < spmvWork)
+                    add(PK_K_SPMV, a, k);
+                if (k < updWork)
+                    add(PK_K_UPDATE, b, k);
+                if (k < dirWork)
+                    add(PK_K_DIRECTION, c, k);
             }
         }
     }
@@ -383,22 +390,28 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
     const int gS = spmv_grid(dv), gU = update_grid(dv);
     unsigned long long* stS = nullptr;
     unsigned long long* stU = nullptr;
+    unsigned long long* stD = nullptr;
     if (s->prof.on)
     {
         const size_t launches = static_cast<size_t>(maxIt + 1);
-        if (s->rawSpmv.n < launches * 2 * gS || s->rawUpd.n < launches * 2 * gU || s->stampSpmv.n < 2 * launches)
+        if (s->rawSpmv.n < launches * 2 * gS || s->rawUpd.n < launches * 2 * gU || s->rawDir.n < launches * 2 * gU ||
+            s->stampSpmv.n < 2 * launches)
         {
             int rc;
             if ((rc = s->rawSpmv.alloc(launches * 2 * gS)) != PK_OK) return rc;
             if ((rc = s->rawUpd.alloc(launches * 2 * gU)) != PK_OK) return rc;
+            if ((rc = s->rawDir.alloc(launches * 2 * gU)) != PK_OK) return rc;
             if ((rc = s->stampSpmv.alloc(2 * launches)) != PK_OK) return rc;
             if ((rc = s->stampUpd.alloc(2 * launches)) != PK_OK) return rc;
+            if ((rc = s->stampDir.alloc(2 * launches)) != PK_OK) return rc;
         }
         stS = s->rawSpmv.p;
         stU = s->rawUpd.p;
+        stD = s->rawDir.p;
     }
-    // launch k runs the SpMV of iteration k (and decides the end of k-1);
-    // k = maxIt only takes that decision
+    // launch k runs the direction, SpMV and update of iteration k (the
+    // direction kernel first decides the end of k-1); k = maxIt only takes
+    // that decision
     int k = 0, poll = 0;
     int chunk = std::max(4, lastIters - 4);
     while (k <= maxIt)
@@ -406,9 +419,13 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
         const int end = std::min(maxIt + 1, k + chunk);
         for (; k < end; ++k)
         {
-            launch_spmv<M>(dv, k, s->lanesPerRow, s->unroll, stS ? stS + 2 * static_cast<size_t>(gS) * k : nullptr);
+            launch_dir<M>(dv, k, stD ? stD + 2 * static_cast<size_t>(gU) * k : nullptr);
             if (k < maxIt)
+            {
+                launch_spmv<M>(dv, k, s->lanesPerRow, s->unroll,
+                               stS ? stS + 2 * static_cast<size_t>(gS) * k : nullptr);
                 launch_update<M>(dv, k, stU ? stU + 2 * static_cast<size_t>(gU) * k : nullptr);
+            }
         }
         chunk = 4;
         const int slot = poll & 1;
@@ -424,7 +441,8 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
     }
     if (stS)
     {
-        launch_stamp_reduce(s->stream, stS, gS, k, s->stampSpmv.p);
+        launch_stamp_reduce(s->stream, stD, gU, k, s->stampDir.p);
+        launch_stamp_reduce(s->stream, stS, gS, std::min(k, maxIt), s->stampSpmv.p);
         launch_stamp_reduce(s->stream, stU, gU, std::min(k, maxIt), s->stampUpd.p);
     }
     PK_HIP(hipGetLastError());
