@@ -251,14 +251,14 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
     const int full = uFull[s];
 #pragma unroll
     for (int e = 0; e < 9; ++e)
-        vals[e * F + full] = static_cast<M>(a[e]);
+        vals[BsrStore<M>::at(F, full, e)] = static_cast<M>(a[e]);
     if (mirror >= 0)
     {
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                vals[(3 * i + j) * F + mirror] = static_cast<M>(a[3 * j + i]);
+                vals[BsrStore<M>::at(F, mirror, 3 * i + j)] = static_cast<M>(a[3 * j + i]);
     }
     const bool ok = fin9(a);
     const int g = rowGroup[r];

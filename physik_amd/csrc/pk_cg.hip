@@ -176,6 +176,26 @@ __global__ __launch_bounds__(NT) void k_dir(int k, TileView vt, const M* __restr
     stamp_end(stamp);
 }
 
+// Block k of the full BSR: its 9 values (row-major) and its column.
+__device__ inline void load_block(const double* __restrict__ vals, size_t F, const int* __restrict__ fcol, int k,
+                                  double* a, int& col)
+{
+    col = fcol[k];
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+        a[e] = __builtin_nontemporal_load(vals + e * F + k);
+}
+
+__device__ inline void load_block(const float* __restrict__ vals, size_t, const int* __restrict__, int k, float* a,
+                                  int& col)
+{
+    const float2* p = reinterpret_cast<const float2*>(vals) + 5 * static_cast<size_t>(k);
+    const float2 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3], w4 = p[4];
+    a[0] = w0.x; a[1] = w0.y; a[2] = w1.x; a[3] = w1.y; a[4] = w2.x;
+    a[5] = w2.y; a[6] = w3.x; a[7] = w3.y; a[8] = w4.x;
+    col = __float_as_int(w4.y);
+}
+
 // q = A d_k for the tile's rows and the tile's d.q partial. A body whose loop
 // ended at or before k (decided by k_dir(k) or earlier) is skipped.
 template <class M, int LPR, int U>
@@ -215,10 +235,7 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
         for (int u = 0; u < U; ++u)
         {
             const int idx = kb[u] < ke[u] ? kb[u] : 0;
-            col[u] = fcol[idx];
-#pragma unroll
-            for (int e = 0; e < 9; ++e)
-                a[u][e] = __builtin_nontemporal_load(vals + e * F + idx);
+            load_block(vals, F, fcol, idx, a[u], col[u]);
         }
         double y[U][3];
 #pragma unroll
@@ -239,14 +256,15 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
         for (int u = 0; u < U; ++u)
             for (int kk = kb[u] + LPR; kk < ke[u]; kk += LPR)
             {
-                const int c = fcol[kk];
+                M b[9];
+                int c;
+                load_block(vals, F, fcol, kk, b, c);
                 const double d0 = static_cast<double>(d[3 * c]), d1 = static_cast<double>(d[3 * c + 1]),
                              d2 = static_cast<double>(d[3 * c + 2]);
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
-                    y[u][i] += static_cast<double>(__builtin_nontemporal_load(vals + (3 * i) * F + kk)) * d0 +
-                               static_cast<double>(__builtin_nontemporal_load(vals + (3 * i + 1) * F + kk)) * d1 +
-                               static_cast<double>(__builtin_nontemporal_load(vals + (3 * i + 2) * F + kk)) * d2;
+                    y[u][i] += static_cast<double>(b[3 * i]) * d0 + static_cast<double>(b[3 * i + 1]) * d1 +
+                               static_cast<double>(b[3 * i + 2]) * d2;
             }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -412,12 +430,13 @@ __global__ __launch_bounds__(NT) void k_spmv_plain(int nd, const int* __restrict
     double acc[3] = {0, 0, 0};
     for (int k = fstart[row]; k < fstart[row + 1]; ++k)
     {
-        const int c = fcol[k];
+        M a[9];
+        int c;
+        load_block(vals, F, fcol, k, a, c);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            acc[i] += static_cast<double>(vals[(3 * i) * F + k]) * xin[3 * c] +
-                      static_cast<double>(vals[(3 * i + 1) * F + k]) * xin[3 * c + 1] +
-                      static_cast<double>(vals[(3 * i + 2) * F + k]) * xin[3 * c + 2];
+            acc[i] += static_cast<double>(a[3 * i]) * xin[3 * c] + static_cast<double>(a[3 * i + 1]) * xin[3 * c + 1] +
+                      static_cast<double>(a[3 * i + 2]) * xin[3 * c + 2];
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -496,6 +515,21 @@ template void launch_spmv<double>(const DeviceView&, int, int, int, unsigned lon
 template void launch_spmv<float>(const DeviceView&, int, int, int, unsigned long long*);
 template void launch_update<double>(const DeviceView&, int, unsigned long long*);
 template void launch_update<float>(const DeviceView&, int, unsigned long long*);
+__global__ __launch_bounds__(NT) void k_record_columns(long long F, const int* __restrict__ fcol,
+                                                       float* __restrict__ vals)
+{
+    const long long k = static_cast<long long>(blockIdx.x) * NT + threadIdx.x;
+    if (k < F)
+        vals[BsrStore<float>::at(static_cast<size_t>(F), static_cast<size_t>(k), 9)] = __int_as_float(fcol[k]);
+}
+
+void launch_record_columns(const DeviceView& v)
+{
+    if (v.nnzf > 0)
+        hipLaunchKernelGGL(k_record_columns, dim3(static_cast<unsigned>((v.nnzf + NT - 1) / NT)), dim3(NT), 0,
+                           v.stream, v.nnzf, v.fcol, static_cast<float*>(v.vals));
+}
+
 template void launch_spmv_plain<double>(const DeviceView&, const double*, double*);
 template void launch_spmv_plain<float>(const DeviceView&, const double*, double*);
 

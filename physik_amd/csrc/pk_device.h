@@ -58,6 +58,23 @@ struct TileView
     int per;                // ceil(count / 8)
 };
 
+// Value storage of the full BSR. FP64: 9 planes, value e of block k at
+// e * F + k (the lanes of a row read 8-byte words of one plane). FP32: one
+// record of 10 words per block -- the 9 values, then the block's column as
+// int bits -- so a lane reads a block with five 8-byte loads instead of ten
+// 4-byte ones.
+template <class M> struct BsrStore;
+template <> struct BsrStore<double>
+{
+    static constexpr int kWords = 9;
+    __host__ __device__ static size_t at(size_t F, size_t k, int e) { return static_cast<size_t>(e) * F + k; }
+};
+template <> struct BsrStore<float>
+{
+    static constexpr int kWords = 10;
+    __host__ __device__ static size_t at(size_t, size_t k, int e) { return 10 * k + static_cast<size_t>(e); }
+};
+
 __device__ inline int tile_of(const TileView& tv)
 {
     const int b = static_cast<int>(blockIdx.x);
@@ -234,6 +251,8 @@ int update_grid(const DeviceView& v);
 void launch_stamp_reduce(hipStream_t stream, const unsigned long long* raw, int grid, int launches,
                          unsigned long long* span);
 template <class M> void launch_spmv_plain(const DeviceView& v, const double* x, double* y);
+// FP32 storage: writes each block's column into word 9 of its record.
+void launch_record_columns(const DeviceView& v);
 
 // Reverse mode (pk_adjoint.hip). `ok` holds one flag per body: the forward
 // substep integrated. The CG of A lam = u runs through launch_spmv/update.

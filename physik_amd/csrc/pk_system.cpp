@@ -508,7 +508,8 @@ static int alloc_work(pk_system* s)
     const size_t nd3 = static_cast<size_t>(t.nd) * 3;
     int rc;
     if ((rc = s->elem.alloc(static_cast<size_t>(t.T) * E_STRIDE)) != PK_OK) return rc;
-    if ((rc = s->vals.alloc(static_cast<size_t>(t.nnzf()) * 9 * es)) != PK_OK) return rc;
+    const size_t words = s->precision == PK_FP32_STORAGE ? BsrStore<float>::kWords : BsrStore<double>::kWords;
+    if ((rc = s->vals.alloc(static_cast<size_t>(t.nnzf()) * words * es)) != PK_OK) return rc;
     if ((rc = s->z.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->d0.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->d1.alloc(nd3 * es)) != PK_OK) return rc;
@@ -571,6 +572,8 @@ static int setup_device(pk_system* s)
     if ((rc = s->nodeRow.upload(t.dyn, st)) != PK_OK) return rc;
     if ((rc = s->nodeGroup.upload(t.nodeGroup, st)) != PK_OK) return rc;
     if ((rc = alloc_work(s)) != PK_OK) return rc;
+    if (s->precision == PK_FP32_STORAGE)
+        launch_record_columns(s->view());
     std::vector<double> zero(static_cast<size_t>(t.N) * 3, 0.0);
     if ((rc = s->v.upload(zero, st)) != PK_OK) return rc;
     if ((rc = s->x.alloc(static_cast<size_t>(t.N) * 3)) != PK_OK) return rc;
@@ -1062,13 +1065,14 @@ int pk_system_get_matrix(pk_system* s, double* full_vals, double* rhs, double* m
     PK_HIP(hipStreamSynchronize(s->stream));
     if (full_vals && F)
     {
-        const size_t es = s->precision == PK_FP32_STORAGE ? 4 : 8;
-        std::vector<char> raw(F * 9 * es);
+        const bool f32 = s->precision == PK_FP32_STORAGE;
+        const size_t es = f32 ? 4 : 8;
+        std::vector<char> raw(F * (f32 ? BsrStore<float>::kWords : BsrStore<double>::kWords) * es);
         PK_HIP(hipMemcpy(raw.data(), s->vals.p, raw.size(), hipMemcpyDeviceToHost));
         for (size_t k = 0; k < F; ++k)
             for (int e = 0; e < 9; ++e)
-                full_vals[9 * k + e] = es == 4 ? reinterpret_cast<const float*>(raw.data())[e * F + k]
-                                               : reinterpret_cast<const double*>(raw.data())[e * F + k];
+                full_vals[9 * k + e] = f32 ? reinterpret_cast<const float*>(raw.data())[BsrStore<float>::at(F, k, e)]
+                                           : reinterpret_cast<const double*>(raw.data())[BsrStore<double>::at(F, k, e)];
     }
     if (rhs && t.nd)
         PK_HIP(hipMemcpy(rhs, s->r.p, static_cast<size_t>(t.nd) * 3 * sizeof(double), hipMemcpyDeviceToHost));
