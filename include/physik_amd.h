@@ -152,10 +152,12 @@ int pk_system_assemble(pk_system* sys, double dt);
  * side b right after pk_system_assemble, the final residual after a step)
  * and block-Jacobi inverses (dynamic_rows x 9). Any pointer may be NULL. */
 int pk_system_get_matrix(pk_system* sys, double* full_vals, double* rhs, double* minv);
-/* Per-tet element record of the last assembly: the 10 blocks R K_ab R^T for
- * a <= b in order (0,0),(0,1),(0,2),(0,3),(1,1),(1,2),(1,3),(2,2),(2,3),(3,3)
+/* Per-tet element outputs at the current state (what the last assembly used
+ * when the state has not changed since): the 10 blocks R K_ab R^T for a <= b
+ * in order (0,0),(0,1),(0,2),(0,3),(1,1),(1,2),(1,3),(2,2),(2,3),(3,3)
  * (tet_count x 90), the element forces (tet_count x 12) and K v per corner
- * (tet_count x 12). Any pointer may be NULL. */
+ * (tet_count x 12); zero for tets that contribute nothing. Any pointer may be
+ * NULL. */
 int pk_system_get_elements(pk_system* sys, double* blocks, double* forces, double* kv);
 /* y = A x with the assembled full BSR (dynamic_rows x 3). */
 int pk_system_spmv(pk_system* sys, const double* x, double* y);

@@ -188,8 +188,8 @@ __global__ __launch_bounds__(NT) void k_adj_rhs(TileView tv, const int* __restri
         part[t] = Partial{pa, pb, flag, 0};
 }
 
-// Corner terms go to the first 24 values of the tet's element record:
-// [0, 12) dPhi/dx_a, [12, 24) dPhi/dv_a.
+// Corner terms go to element planes EP_ADJ_X (dPhi/dx_a) and EP_ADJ_V
+// (dPhi/dv_a).
 __global__ __launch_bounds__(NT) void k_adj_element(int T, const int4* __restrict__ tets,
                                                     const unsigned char* __restrict__ live,
                                                     const double* __restrict__ prm,
@@ -203,7 +203,8 @@ __global__ __launch_bounds__(NT) void k_adj_element(int T, const int4* __restric
     if (t >= T || !live[t])
         return;
     const int4 n4 = tets[t];
-    double* out = elem + static_cast<size_t>(t) * E_STRIDE;
+    const size_t TT = static_cast<size_t>(T);
+    auto out = [&](int plane) -> double& { return elem[static_cast<size_t>(plane) * TT + t]; };
     if (!ok[nodeGroup[n4.x]])
         return;  // the body's substep was identity: its nodes are not gathered
     const int ids[4] = {n4.x, n4.y, n4.z, n4.w};
@@ -220,7 +221,6 @@ __global__ __launch_bounds__(NT) void k_adj_element(int T, const int4* __restric
             ls[a][i] = row >= 0 ? lam[3 * row + i] : 0.0;
         }
     }
-    const size_t TT = static_cast<size_t>(T);
     double Dinv[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k)
@@ -241,21 +241,14 @@ __global__ __launch_bounds__(NT) void k_adj_element(int T, const int4* __restric
     {
         // the forward skipped the tet: no terms
         for (int k = 0; k < 24; ++k)
-            out[k] = 0.0;
+            out(EP_ADJ_X + k) = 0.0;
         return;
     }
     double R[9];
     int kind = 2;
     polar(F, R, &kind);
     double g[4][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-    {
-        g[1][i] = Dinv[i];
-        g[2][i] = Dinv[3 + i];
-        g[3][i] = Dinv[6 + i];
-        g[0][i] = -((g[1][i] + g[2][i]) + g[3][i]);
-    }
+    shape_gradients(Dinv, g);
     double L[9], W[9];
     outer4(ls, g, L);
     outer4(ws, g, W);
@@ -292,14 +285,14 @@ __global__ __launch_bounds__(NT) void k_adj_element(int T, const int4* __restric
 #pragma unroll
         for (int i = 0; i < 3; ++i)
         {
-            out[3 * a + i] = Fb[3 * i] * g[a][0] + Fb[3 * i + 1] * g[a][1] + Fb[3 * i + 2] * g[a][2];
-            out[12 + 3 * a + i] =
+            out(EP_ADJ_X + 3 * a + i) = Fb[3 * i] * g[a][0] + Fb[3 * i + 1] * g[a][1] + Fb[3 * i + 2] * g[a][2];
+            out(EP_ADJ_V + 3 * a + i) =
                 -dt2 * vol * (P[3 * i] * g[a][0] + P[3 * i + 1] * g[a][1] + P[3 * i + 2] * g[a][2]) -
                 dt * damp * ls[a][i];
         }
 }
 
-__global__ __launch_bounds__(NT) void k_adj_nodes(int N, const int* __restrict__ astart,
+__global__ __launch_bounds__(NT) void k_adj_nodes(int N, int T, const int* __restrict__ astart,
                                                   const int* __restrict__ alist, const double* __restrict__ elem,
                                                   const int* __restrict__ nodeRow, const int* __restrict__ nodeGroup,
                                                   const int* __restrict__ ok, double dt, double* __restrict__ xbar,
@@ -326,12 +319,13 @@ __global__ __launch_bounds__(NT) void k_adj_nodes(int N, const int* __restrict__
     for (int k = astart[n]; k < k1; ++k)
     {
         const int c = alist[k];
-        const double* e = elem + static_cast<size_t>(c >> 2) * E_STRIDE + 3 * (c & 3);
+        const size_t TT = static_cast<size_t>(T);
+        const double* e = elem + (c >> 2) + 3 * static_cast<size_t>(c & 3) * TT;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
         {
-            xb[i] += e[i];
-            vb[i] += e[12 + i];
+            xb[i] += e[(EP_ADJ_X + i) * TT];
+            vb[i] += e[(EP_ADJ_V + i) * TT];
         }
     }
 #pragma unroll
@@ -362,14 +356,14 @@ void launch_adj_element(const DeviceView& v, const int* ok, const double* vnext,
 {
     if (v.T > 0)
         hipLaunchKernelGGL(k_adj_element, dim3((v.T + NT - 1) / NT), dim3(NT), 0, v.stream, v.T, v.tets, v.live,
-                           v.prm, v.nodeGroup, ok, v.nodeRow, v.x, vnext, v.xs, dt, v.elem, gradLogE);
+                           v.prm, v.nodeGroup, ok, v.nodeRow, v.x, vnext, v.xs, dt, v.adj, gradLogE);
 }
 
 void launch_adj_nodes(const DeviceView& v, const int* ok, double dt, double* xbar, double* vbar)
 {
     if (v.N > 0)
-        hipLaunchKernelGGL(k_adj_nodes, dim3((v.N + NT - 1) / NT), dim3(NT), 0, v.stream, v.N, v.astart, v.alist,
-                           v.elem, v.nodeRow, v.nodeGroup, ok, dt, xbar, vbar);
+        hipLaunchKernelGGL(k_adj_nodes, dim3((v.N + NT - 1) / NT), dim3(NT), 0, v.stream, v.N, v.T, v.astart, v.alist,
+                           v.adj, v.nodeRow, v.nodeGroup, ok, dt, xbar, vbar);
 }
 
 template void launch_adj_rhs<double>(const DeviceView&, const int*, const double*, const double*, double);

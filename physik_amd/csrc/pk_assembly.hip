@@ -2,12 +2,14 @@
 // side with the CG start, and the integrators.
 //
 //   k_element   one lane per tet: F = Ds Dm^-1, Newton polar R, corotated
-//               forces and the 10 blocks R K_ab R^T, K v per corner
-//               (FEMModel.cpp:614-679, :377-424, :336-375, :115-163)
-//   k_assemble  one lane per upper BSR slot: sum of the slot's element blocks
-//               in tet order through the precomputed element->slot lists (no
-//               atomics), A = M + dt^2 K written to both triangles, 3x3
-//               block-Jacobi inverse on diagonal slots
+//               forces, the 10 blocks R K_ab R^T and K v per corner
+//               (FEMModel.cpp:614-679, :377-424, :336-375, :115-163), each
+//               written where its consumer reads it: blocks into the slot
+//               gather lists (precomputed element->slot scatter map, oriented
+//               for the slot), forces and K v into the row lists
+//   k_assemble  one lane per upper BSR slot: contiguous sum of the slot's
+//               blocks in tet order (no atomics), A = M + dt^2 K written to
+//               both triangles, 3x3 block-Jacobi inverse on diagonal slots
 //               (SolverData.cpp:308-465, :260-306)
 //   k_rhs       b = dt (f + m g) - dt^2 K v per dynamic row, then the CG start
 //               x = 0, r = b, z = d = M^-1 r with the (r.z, r.r) partials
@@ -47,14 +49,20 @@ __global__ void k_reset(GroupState* gs, int G, int* active, double tol, int maxI
         *active = cnt;
 }
 
-__global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ tets,
-                                                const unsigned char* __restrict__ live,
-                                                const double* __restrict__ prm, const double* __restrict__ x,
-                                                const double* __restrict__ v, double* __restrict__ elem)
+// Everything one tet contributes to a substep, in registers.
+struct ElementOut
 {
-    const int t = blockIdx.x * NT + threadIdx.x;
-    if (t >= T || !live[t])
-        return;
+    bool valid;          // F finite: otherwise the tet contributes nothing
+    double basis[EB_COUNT];  // R g_a, then R R^T (xx, xy, xz, yy, yz, zz)
+    double K[10][9];     // R K_ab R^T for the stored pairs (row-major)
+    double f[4][3];      // element forces (elastic + damping)
+    double kv[4][3];     // sum_b (R K_ab R^T) v_b
+};
+
+__device__ inline void element_compute(int t, size_t TT, const int4* __restrict__ tets,
+                                       const double* __restrict__ prm, const double* __restrict__ x,
+                                       const double* __restrict__ v, ElementOut& o)
+{
     const int4 n4 = tets[t];
     const int ids[4] = {n4.x, n4.y, n4.z, n4.w};
     double xs[4][3], vs[4][3];
@@ -66,7 +74,6 @@ __global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ 
             xs[a][i] = x[3 * ids[a] + i];
             vs[a][i] = v[3 * ids[a] + i];
         }
-    const size_t TT = static_cast<size_t>(T);
     double Dinv[9], Erest[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k)
@@ -78,7 +85,6 @@ __global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ 
     const double lam = prm[P_LAMBDA * TT + t];
     const double mu = prm[P_MU * TT + t];
     const double damp = prm[P_DAMP * TT + t];
-    double* out = elem + static_cast<size_t>(t) * E_STRIDE;
 
     double ds[3][3];  // ds[k] = x_{k+1} - x_0
 #pragma unroll
@@ -92,26 +98,13 @@ __global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ 
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             F[3 * i + j] = ds[0][i] * Dinv[j] + ds[1][i] * Dinv[3 + j] + ds[2][i] * Dinv[6 + j];
-    if (!fin9(F))
-    {
-        // the reference skips the tet: it contributes nothing this substep
-        for (int k = 0; k < E_STRIDE; ++k)
-            out[k] = 0.0;
-        return;
-    }
+    o.valid = fin9(F);
+    if (!o.valid)
+        return;  // the reference skips the tet this substep
     double R[9];
     polar(F, R);
-
-    // shape gradients: g_a = row a-1 of Dm^-1 (a = 1..3), g_0 = -(g_1 + g_2 + g_3)
     double g[4][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-    {
-        g[1][i] = Dinv[i];
-        g[2][i] = Dinv[3 + i];
-        g[3][i] = Dinv[6 + i];
-        g[0][i] = -((g[1][i] + g[2][i]) + g[3][i]);
-    }
+    shape_gradients(Dinv, g);
     // u_a = R^T (x_a - x_0) - (X_a - X_0); u_0 = 0
     double u[4][3];
 #pragma unroll
@@ -146,9 +139,8 @@ __global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ 
         const double fz = -vol * (txz * g[a][0] + tyz * g[a][1] + szz * g[a][2]);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            out[E_FORCE + 3 * a + i] = (R[3 * i] * fx + R[3 * i + 1] * fy + R[3 * i + 2] * fz) - damp * vs[a][i];
+            o.f[a][i] = (R[3 * i] * fx + R[3 * i + 1] * fy + R[3 * i + 2] * fz) - damp * vs[a][i];
     }
-    // R K_ab R^T = V (lam (R g_a)(R g_b)^T + mu (R g_b)(R g_a)^T + mu (g_a.g_b) R R^T)
     double rg[4][3];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -161,72 +153,179 @@ __global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ 
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             RRt[3 * i + j] = R[3 * i] * R[3 * j] + R[3 * i + 1] * R[3 * j + 1] + R[3 * i + 2] * R[3 * j + 2];
-    double kv[4][3];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            kv[a][i] = 0.0;
-    int p = 0;
+            o.basis[EB_RG + 3 * a + i] = rg[a][i];
+    o.basis[EB_RRT + 0] = RRt[0];
+    o.basis[EB_RRT + 1] = RRt[1];
+    o.basis[EB_RRT + 2] = RRt[2];
+    o.basis[EB_RRT + 3] = RRt[4];
+    o.basis[EB_RRT + 4] = RRt[5];
+    o.basis[EB_RRT + 5] = RRt[8];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = a; b < 4; ++b, ++p)
+        for (int i = 0; i < 3; ++i)
+            o.kv[a][i] = 0.0;
+#pragma unroll
+    for (int p = 0; p < 10; ++p)
+    {
+        int a, b;
+        pair_of(p, a, b);
+        double* K = o.K[p];
+        rotated_block(rg[a], rg[b], RRt, vol, lam, mu, gdot(g[a], g[b]), K);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            o.kv[a][i] += K[3 * i] * vs[b][0] + K[3 * i + 1] * vs[b][1] + K[3 * i + 2] * vs[b][2];
+        if (a != b)
         {
-            const double gab = mu * (g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2]);
-            double K[9];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    K[3 * i + j] = vol * (lam * rg[a][i] * rg[b][j] + mu * rg[b][i] * rg[a][j] + gab * RRt[3 * i + j]);
-#pragma unroll
-            for (int e = 0; e < 9; ++e)
-                out[E_BLOCKS + 9 * p + e] = K[e];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                kv[a][i] += K[3 * i] * vs[b][0] + K[3 * i + 1] * vs[b][1] + K[3 * i + 2] * vs[b][2];
-            if (a != b)
-            {
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    kv[b][i] += K[i] * vs[a][0] + K[3 + i] * vs[a][1] + K[6 + i] * vs[a][2];
-            }
+                o.kv[b][i] += K[i] * vs[a][0] + K[3 + i] * vs[a][1] + K[6 + i] * vs[a][2];
         }
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            out[E_KV + 3 * a + i] = kv[a][i];
+    }
 }
 
+// Writes the tet's basis planes (R g_a, R R^T) and each dynamic corner's
+// force and K v at its place in the row lists.
+__global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ tets,
+                                                const unsigned char* __restrict__ live,
+                                                const double* __restrict__ prm, const double* __restrict__ x,
+                                                const double* __restrict__ v, const int* __restrict__ cpos,
+                                                double* __restrict__ basis, double* __restrict__ corner)
+{
+    const int t = blockIdx.x * NT + threadIdx.x;
+    if (t >= T || !live[t])
+        return;
+    const size_t TT = static_cast<size_t>(T);
+    ElementOut o;
+    element_compute(t, TT, tets, prm, x, v, o);
+#pragma unroll
+    for (int k = 0; k < EB_COUNT; ++k)
+        basis[k * TT + t] = o.valid ? o.basis[k] : 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+    {
+        const int pos = cpos[a * TT + t];
+        if (pos < 0)
+            continue;
+        double* out = corner + static_cast<size_t>(pos) * N_CORNER;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            out[i] = o.valid ? o.f[a][i] : 0.0;
+            out[N_KV + i] = o.valid ? o.kv[a][i] : 0.0;
+        }
+    }
+}
+
+// g_a . g_b of the 10 local pairs into the parameter planes (rest data, once
+// per mesh), with the arithmetic the element kernel uses.
+__global__ __launch_bounds__(NT) void k_gdot(int T, double* __restrict__ prm)
+{
+    const int t = blockIdx.x * NT + threadIdx.x;
+    if (t >= T)
+        return;
+    const size_t TT = static_cast<size_t>(T);
+    double Dinv[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        Dinv[k] = prm[(P_DMINV + k) * TT + t];
+    double g[4][3];
+    shape_gradients(Dinv, g);
+#pragma unroll
+    for (int p = 0; p < 10; ++p)
+    {
+        int a, b;
+        pair_of(p, a, b);
+        prm[(P_GDOT + p) * TT + t] = gdot(g[a], g[b]);
+    }
+}
+
+// Block R K_ab R^T of local pair p of tet t from its basis planes.
+__device__ inline void element_block(const double* __restrict__ basis, const double* __restrict__ prm, size_t TT,
+                                     int t, int p, double* K)
+{
+    int a, b;
+    pair_of(p, a, b);
+    double rga[3], rgb[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        rga[i] = basis[(EB_RG + 3 * a + i) * TT + t];
+        rgb[i] = basis[(EB_RG + 3 * b + i) * TT + t];
+    }
+    const double r00 = basis[(EB_RRT + 0) * TT + t], r01 = basis[(EB_RRT + 1) * TT + t],
+                 r02 = basis[(EB_RRT + 2) * TT + t], r11 = basis[(EB_RRT + 3) * TT + t],
+                 r12 = basis[(EB_RRT + 4) * TT + t], r22 = basis[(EB_RRT + 5) * TT + t];
+    const double RRt[9] = {r00, r01, r02, r01, r11, r12, r02, r12, r22};
+    rotated_block(rga, rgb, RRt, prm[P_VOL * TT + t], prm[P_LAMBDA * TT + t], prm[P_MU * TT + t],
+                  prm[(P_GDOT + p) * TT + t], K);
+}
+
+// The element outputs of every tet as one record (10 blocks, 4 forces, 4 K v:
+// 114 doubles), for inspection; zero for tets that contribute nothing.
+__global__ __launch_bounds__(NT) void k_element_records(int T, const int4* __restrict__ tets,
+                                                        const unsigned char* __restrict__ live,
+                                                        const double* __restrict__ prm, const double* __restrict__ x,
+                                                        const double* __restrict__ v, double* __restrict__ out)
+{
+    const int t = blockIdx.x * NT + threadIdx.x;
+    if (t >= T)
+        return;
+    ElementOut o;
+    o.valid = false;
+    if (live[t])
+        element_compute(t, static_cast<size_t>(T), tets, prm, x, v, o);
+    double* r = out + 114 * static_cast<size_t>(t);
+    for (int p = 0; p < 10; ++p)
+        for (int e = 0; e < 9; ++e)
+            r[9 * p + e] = o.valid ? o.K[p][e] : 0.0;
+    for (int a = 0; a < 4; ++a)
+        for (int i = 0; i < 3; ++i)
+        {
+            r[90 + 3 * a + i] = o.valid ? o.f[a][i] : 0.0;
+            r[102 + 3 * a + i] = o.valid ? o.kv[a][i] : 0.0;
+        }
+}
+
+// Lanes per upper slot in the assembly gather: a node's star has ~20
+// contributions and an edge ~5, so one lane per slot leaves most lanes of a
+// wave idle while its diagonal slots finish.
+constexpr int kSlotLanes = 4;
+
 template <class M>
-__global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict__ cstart,
-                                                 const int* __restrict__ clist, const int* __restrict__ uFull,
+__global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict__ cstart, const int* __restrict__ clist,
+                                                 const int* __restrict__ uFull,
                                                  const int* __restrict__ uMirror, const int* __restrict__ slotRow,
-                                                 const int* __restrict__ rowGroup, const double* __restrict__ elem,
+                                                 const int* __restrict__ rowGroup, const double* __restrict__ basis,
+                                                 const double* __restrict__ prm, int T,
                                                  const double* __restrict__ mrow, double dt2, M* __restrict__ vals,
                                                  long long nnzf, double* __restrict__ minv, int nd, GroupState* gs)
 {
-    const int s = blockIdx.x * NT + threadIdx.x;
+    const int s = (blockIdx.x * NT + threadIdx.x) / kSlotLanes;
+    const int lane = threadIdx.x % kSlotLanes;
     if (s >= nnzu)
-        return;
+        return;  // whole lane groups
     double a[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e)
         a[e] = 0.0;
     const int k1 = cstart[s + 1];
-    for (int k = cstart[s]; k < k1; ++k)
+    for (int k = cstart[s] + lane; k < k1; k += kSlotLanes)
     {
         const int c = clist[k];
-        const double* b = elem + static_cast<size_t>(c >> 5) * E_STRIDE + 9 * ((c >> 1) & 15);
+        double b[9];
+        element_block(basis, prm, static_cast<size_t>(T), c >> 5, (c >> 1) & 15, b);
         if (c & 1)
         {
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+            for (int r = 0; r < 3; ++r)
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    a[3 * i + j] += b[3 * j + i];
+                for (int q = 0; q < 3; ++q)
+                    a[3 * r + q] += b[3 * q + r];
         }
         else
         {
@@ -235,6 +334,16 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
                 a[e] += b[e];
         }
     }
+    // the group's partial sums, combined in a fixed order
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+    {
+#pragma unroll
+        for (int o = kSlotLanes / 2; o > 0; o >>= 1)
+            a[e] += __shfl_xor(a[e], o, kSlotLanes);
+    }
+    if (lane != 0)
+        return;
 #pragma unroll
     for (int e = 0; e < 9; ++e)
         a[e] *= dt2;
@@ -289,6 +398,9 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
     }
 }
 
+// Lanes per dynamic row in the right-hand side gather (a node has ~20 corners).
+constexpr int kRowLanes = 4;
+
 // z = M^-1 r with the block-Jacobi inverse stored as 9 planes of nd values.
 __device__ inline void apply_minv(const double* __restrict__ minv, size_t nd, int row, const double* r, double* z)
 {
@@ -300,7 +412,7 @@ __device__ inline void apply_minv(const double* __restrict__ minv, size_t nd, in
 
 template <class M>
 __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__ nstart,
-                                            const int* __restrict__ nlist, const double* __restrict__ elem,
+                                            const double* __restrict__ corner,
                                             const double* __restrict__ mrow, double gx, double gy, double gz,
                                             double dt, double dt2, const double* __restrict__ minv, int nd,
                                             double* __restrict__ xs, double* __restrict__ rv, M* __restrict__ zv,
@@ -312,16 +424,20 @@ __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__
     const RowTile tl = tv.tiles[t];
     double pa = 0.0, pb = 0.0;
     int flag = 0;
-    for (int row = tl.rowBegin + threadIdx.x; row < tl.rowEnd; row += NT)
+    constexpr int RL = kRowLanes, RPP = NT / kRowLanes;
+    const int lane = threadIdx.x % RL;
+    // every lane group runs the same number of passes (the shuffles below
+    // need whole groups); rows past the tile are masked
+    for (int row0 = tl.rowBegin; row0 < tl.rowEnd; row0 += RPP)
     {
+        const int row = row0 + static_cast<int>(threadIdx.x) / RL;
+        const bool live = row < tl.rowEnd;
         double f[3] = {0, 0, 0}, kv[3] = {0, 0, 0};
-        const int k1 = nstart[row + 1];
-        for (int k = nstart[row]; k < k1; ++k)
+        const int k1 = live ? nstart[row + 1] : 0;
+        for (int k = live ? nstart[row] + lane : 0; k < k1; k += RL)
         {
-            const int c = nlist[k];
-            const double* e = elem + static_cast<size_t>(c >> 2) * E_STRIDE;
-            const int a = c & 3;
-            const double f0 = e[E_FORCE + 3 * a], f1 = e[E_FORCE + 3 * a + 1], f2 = e[E_FORCE + 3 * a + 2];
+            const double* e = corner + static_cast<size_t>(k) * N_CORNER;
+            const double f0 = e[0], f1 = e[1], f2 = e[2];
             if (fin(f0) && fin(f1) && fin(f2))  // non-finite pushes are skipped (SolverData.cpp:145-154)
             {
                 f[0] += f0;
@@ -330,8 +446,18 @@ __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__
             }
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                kv[i] += e[E_KV + 3 * a + i];
+                kv[i] += e[N_KV + i];
         }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int o = RL / 2; o > 0; o >>= 1)
+            {
+                f[i] += __shfl_xor(f[i], o, RL);
+                kv[i] += __shfl_xor(kv[i], o, RL);
+            }
+        if (!live || lane != 0)
+            continue;
         const double m = mrow[row];
         const double gv[3] = {gx, gy, gz};
         double b[3], z[3];
@@ -402,8 +528,8 @@ __global__ __launch_bounds__(NT) void k_integrate(TileView tv, const int* __rest
 }
 
 __global__ __launch_bounds__(NT) void k_explicit(TileView tv, const int* __restrict__ rowNode,
-                                                 const int* __restrict__ nstart, const int* __restrict__ nlist,
-                                                 const double* __restrict__ elem, const double* __restrict__ mrow,
+                                                 const int* __restrict__ nstart, const double* __restrict__ corner,
+                                                 const double* __restrict__ mrow,
                                                  double gx, double gy, double gz, double dt, double* __restrict__ x,
                                                  double* __restrict__ v)
 {
@@ -416,13 +542,13 @@ __global__ __launch_bounds__(NT) void k_explicit(TileView tv, const int* __restr
         double f[3] = {0, 0, 0};
         for (int k = nstart[row]; k < nstart[row + 1]; ++k)
         {
-            const int c = nlist[k];
-            const double* e = elem + static_cast<size_t>(c >> 2) * E_STRIDE + E_FORCE + 3 * (c & 3);
-            if (fin(e[0]) && fin(e[1]) && fin(e[2]))
+            const double* e = corner + static_cast<size_t>(k) * N_CORNER;
+            const double e0 = e[0], e1 = e[1], e2 = e[2];
+            if (fin(e0) && fin(e1) && fin(e2))
             {
-                f[0] += e[0];
-                f[1] += e[1];
-                f[2] += e[2];
+                f[0] += e0;
+                f[1] += e1;
+                f[2] += e2;
             }
         }
         const double m = mrow[row];
@@ -449,7 +575,7 @@ void launch_element(const DeviceView& v)
 {
     if (v.T > 0)
         hipLaunchKernelGGL(k_element, dim3((v.T + NT - 1) / NT), dim3(NT), 0, v.stream, v.T, v.tets, v.live, v.prm,
-                           v.x, v.v, v.elem);
+                           v.x, v.v, v.cpos, v.basis, v.corner);
 }
 
 template <class M>
@@ -457,8 +583,9 @@ void launch_assemble(const DeviceView& v, double dt)
 {
     const int n = static_cast<int>(v.nnzu);
     if (n > 0)
-        hipLaunchKernelGGL((k_assemble<M>), dim3((n + NT - 1) / NT), dim3(NT), 0, v.stream, n, v.cstart, v.clist,
-                           v.uFull, v.uMirror, v.slotRow, v.rowGroup, v.elem, v.mrow, dt * dt,
+        hipLaunchKernelGGL((k_assemble<M>), dim3((n * kSlotLanes + NT - 1) / NT), dim3(NT), 0, v.stream, n, v.cstart,
+                           v.clist,
+                           v.uFull, v.uMirror, v.slotRow, v.rowGroup, v.basis, v.prm, v.T, v.mrow, dt * dt,
                            static_cast<M*>(v.vals), v.nnzf, v.minv, v.nd, v.gs);
 }
 
@@ -466,8 +593,8 @@ template <class M>
 void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt)
 {
     if (v.vec.count > 0)
-        hipLaunchKernelGGL((k_rhs<M>), dim3(grid_of(v.vec)), dim3(NT), 0, v.stream, v.vec, v.nstart, v.nlist,
-                           v.elem, v.mrow, gx, gy, gz, dt, dt * dt, v.minv, v.nd, v.xs, v.r, static_cast<M*>(v.z),
+        hipLaunchKernelGGL((k_rhs<M>), dim3(grid_of(v.vec)), dim3(NT), 0, v.stream, v.vec, v.nstart,
+                           v.corner, v.mrow, gx, gy, gz, dt, dt * dt, v.minv, v.nd, v.xs, v.r, static_cast<M*>(v.z),
                            static_cast<M*>(v.d[0]), v.pb[1]);
 }
 
@@ -482,7 +609,20 @@ void launch_explicit(const DeviceView& v, double gx, double gy, double gz, doubl
 {
     if (v.vec.count > 0)
         hipLaunchKernelGGL(k_explicit, dim3(grid_of(v.vec)), dim3(NT), 0, v.stream, v.vec, v.rowNode, v.nstart,
-                           v.nlist, v.elem, v.mrow, gx, gy, gz, dt, v.x, v.v);
+                           v.corner, v.mrow, gx, gy, gz, dt, v.x, v.v);
+}
+
+void launch_gdot(const DeviceView& v, double* prm)
+{
+    if (v.T > 0)
+        hipLaunchKernelGGL(k_gdot, dim3((v.T + NT - 1) / NT), dim3(NT), 0, v.stream, v.T, prm);
+}
+
+void launch_element_records(const DeviceView& v, double* out)
+{
+    if (v.T > 0)
+        hipLaunchKernelGGL(k_element_records, dim3((v.T + NT - 1) / NT), dim3(NT), 0, v.stream, v.T, v.tets, v.live,
+                           v.prm, v.x, v.v, out);
 }
 
 template void launch_assemble<double>(const DeviceView&, double);

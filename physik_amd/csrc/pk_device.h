@@ -213,9 +213,12 @@ struct DeviceView
     const int* nodeGroup;  // node -> body
     const double* mrow;
     // state and work arrays
+    const int* cpos;     // 4 T: where the element kernel writes each dynamic corner
     double* x;
     double* v;
-    double* elem;
+    double* basis;       // per-tet R g_a and R R^T planes
+    double* corner;      // forces and K v in row order
+    double* adj;         // reverse-mode corner terms, EP_ADJ_COUNT planes of T
     double* minv;
     double* xs;   // CG solution dv
     double* r;
@@ -234,6 +237,11 @@ struct DeviceView
 // Launchers (pk_assembly.hip, pk_cg.hip). M is the storage type of A and of
 // the vectors the SpMV reads (double, or float for FP32 storage).
 void launch_reset(const DeviceView& v, double tol, int maxIt);
+// g_a . g_b planes of the tet parameters (after every full upload of them).
+void launch_gdot(const DeviceView& v, double* prm);
+// Every tet's element outputs at the current state as 114-double records
+// (10 blocks, 4 forces, 4 K v), for inspection.
+void launch_element_records(const DeviceView& v, double* out);
 void launch_element(const DeviceView& v);
 template <class M> void launch_assemble(const DeviceView& v, double dt);
 template <class M> void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt);

@@ -80,4 +80,42 @@ __device__ inline void polar(const double* F, double* R, int* kind = nullptr)
         R[e] = c[e];
 }
 
+// Local pair p of the 10 stored pairs a <= b: (0,0),(0,1),(0,2),(0,3),(1,1),
+// (1,2),(1,3),(2,2),(2,3),(3,3).
+__device__ inline void pair_of(int p, int& a, int& b)
+{
+    constexpr int A[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+    constexpr int B[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+    a = A[p];
+    b = B[p];
+}
+
+// Shape gradients: g_a = row a-1 of Dm^-1 (a = 1..3), g_0 = -(g_1 + g_2 + g_3).
+__device__ inline void shape_gradients(const double* Dinv, double g[4][3])
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        g[1][i] = Dinv[i];
+        g[2][i] = Dinv[3 + i];
+        g[3][i] = Dinv[6 + i];
+        g[0][i] = -((g[1][i] + g[2][i]) + g[3][i]);
+    }
+}
+
+__device__ inline double gdot(const double* ga, const double* gb) { return ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]; }
+
+// R K_ab R^T = V (lam (R g_a)(R g_b)^T + mu (R g_b)(R g_a)^T + mu (g_a.g_b) R R^T),
+// the closed form of R Ke_ab R^T (FEMModel.cpp:312-375); RRt is full 3x3.
+__device__ inline void rotated_block(const double* rga, const double* rgb, const double* RRt, double vol, double lam,
+                                     double mu, double gab, double* K)
+{
+    const double mg = mu * gab;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            K[3 * i + j] = vol * (lam * rga[i] * rgb[j] + mu * rgb[i] * rga[j] + mg * RRt[3 * i + j]);
+}
+
 }  // namespace pk

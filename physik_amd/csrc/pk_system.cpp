@@ -113,9 +113,9 @@ struct pk_system
 
     pk::DevBuf<int4> tets;
     pk::DevBuf<unsigned char> live;
-    pk::DevBuf<double> prm, x, v, elem, mrow, minv, xs, r, pa;
+    pk::DevBuf<double> prm, x, v, basis, corner, adj, mrow, minv, xs, r, pa;
     pk::DevBuf<int> cstart, clist, uFull, uMirror, slotRow, rowGroup, fstart, fcol, nstart, nlist, rowNode, active;
-    pk::DevBuf<int> astart, alist, nodeRow, nodeGroup;
+    pk::DevBuf<int> astart, alist, nodeRow, nodeGroup, cpos;
     pk::DevBuf<char> vals, z, d0, d1, q;  // element type set by the precision
     pk::DevBuf<pk::Partial> pb0, pb1;
     pk::DevBuf<pk::GroupState> gs;
@@ -192,7 +192,10 @@ struct pk_system
         dv.mrow = mrow.p;
         dv.x = x.p;
         dv.v = v.p;
-        dv.elem = elem.p;
+        dv.basis = basis.p;
+        dv.corner = corner.p;
+        dv.adj = adj.p;
+        dv.cpos = cpos.p;
         dv.minv = minv.p;
         dv.xs = xs.p;
         dv.r = r.p;
@@ -507,7 +510,8 @@ static int alloc_work(pk_system* s)
     const size_t es = s->precision == PK_FP32_STORAGE ? sizeof(float) : sizeof(double);
     const size_t nd3 = static_cast<size_t>(t.nd) * 3;
     int rc;
-    if ((rc = s->elem.alloc(static_cast<size_t>(t.T) * E_STRIDE)) != PK_OK) return rc;
+    if ((rc = s->basis.alloc(static_cast<size_t>(t.T) * EB_COUNT)) != PK_OK) return rc;
+    if ((rc = s->corner.alloc(t.nlist.size() * N_CORNER)) != PK_OK) return rc;
     const size_t words = s->precision == PK_FP32_STORAGE ? BsrStore<float>::kWords : BsrStore<double>::kWords;
     if ((rc = s->vals.alloc(static_cast<size_t>(t.nnzf()) * words * es)) != PK_OK) return rc;
     if ((rc = s->z.alloc(nd3 * es)) != PK_OK) return rc;
@@ -571,7 +575,9 @@ static int setup_device(pk_system* s)
     if ((rc = s->alist.upload(t.alist, st)) != PK_OK) return rc;
     if ((rc = s->nodeRow.upload(t.dyn, st)) != PK_OK) return rc;
     if ((rc = s->nodeGroup.upload(t.nodeGroup, st)) != PK_OK) return rc;
+    if ((rc = s->cpos.upload(t.cpos, st)) != PK_OK) return rc;
     if ((rc = alloc_work(s)) != PK_OK) return rc;
+    launch_gdot(s->view(), s->prm.p);
     if (s->precision == PK_FP32_STORAGE)
         launch_record_columns(s->view());
     std::vector<double> zero(static_cast<size_t>(t.N) * 3, 0.0);
@@ -972,6 +978,8 @@ int pk_system_backward(pk_system* s, const double* xbar, const double* vbar, dou
     if ((rc = s->xbar.alloc(n3)) != PK_OK) return rc;
     if ((rc = s->vbar.alloc(n3)) != PK_OK) return rc;
     if ((rc = s->gradE.alloc(static_cast<size_t>(t.T))) != PK_OK) return rc;
+    if (s->adj.n < static_cast<size_t>(t.T) * EP_ADJ_COUNT)
+        if ((rc = s->adj.alloc(static_cast<size_t>(t.T) * EP_ADJ_COUNT)) != PK_OK) return rc;
     if (xbar)
         PK_HIP(hipMemcpyAsync(s->xbar.p, xbar, n3 * sizeof(double), hipMemcpyHostToDevice, s->stream));
     else
@@ -1093,19 +1101,27 @@ int pk_system_get_elements(pk_system* s, double* blocks, double* forces, double*
         return PK_ERR_ARG;
     const Topology& t = s->topo;
     PK_HIP(hipStreamSynchronize(s->stream));
-    std::vector<double> raw(static_cast<size_t>(t.T) * E_STRIDE);
-    if (!raw.empty())
-        PK_HIP(hipMemcpy(raw.data(), s->elem.p, raw.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int i = 0; i < t.T; ++i)
+    const size_t T = static_cast<size_t>(t.T);
+    if (T == 0)
+        return PK_OK;
+    DevBuf<double> tmp;
+    int rc;
+    if ((rc = tmp.alloc(T * 114)) != PK_OK)
+        return rc;
+    launch_element_records(s->view(), tmp.p);
+    PK_HIP(hipGetLastError());
+    PK_HIP(hipStreamSynchronize(s->stream));
+    std::vector<double> raw(T * 114);
+    PK_HIP(hipMemcpy(raw.data(), tmp.p, raw.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < T; ++i)
     {
-        const double* e = raw.data() + static_cast<size_t>(i) * E_STRIDE;
-        const bool lv = t.live[i] != 0;
+        const double* r = raw.data() + 114 * i;
         for (int k = 0; k < 90; ++k)
-            if (blocks) blocks[90 * static_cast<size_t>(i) + k] = lv ? e[E_BLOCKS + k] : 0.0;
+            if (blocks) blocks[90 * i + k] = r[k];
         for (int k = 0; k < 12; ++k)
         {
-            if (forces) forces[12 * static_cast<size_t>(i) + k] = lv ? e[E_FORCE + k] : 0.0;
-            if (kv) kv[12 * static_cast<size_t>(i) + k] = lv ? e[E_KV + k] : 0.0;
+            if (forces) forces[12 * i + k] = r[90 + k];
+            if (kv) kv[12 * i + k] = r[102 + k];
         }
     }
     return PK_OK;

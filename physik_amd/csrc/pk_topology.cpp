@@ -344,6 +344,7 @@ int build_topology(const pk_mesh_desc& d, Topology& topo, std::string& err, bool
             }
     }
 
+
     // rhs gather lists: dynamic row <- (tet, corner) over live tets in order.
     std::vector<int> ncount(static_cast<size_t>(nd) + 1, 0);
     for (int t = 0; t < T; ++t)
@@ -367,6 +368,9 @@ int build_topology(const pk_mesh_desc& d, Topology& topo, std::string& err, bool
                 if (da >= 0)
                     topo.nlist[nfill[da]++] = pack_corner(t, a);
             }
+    topo.cpos.assign(static_cast<size_t>(T) * 4, -1);
+    for (size_t k = 0; k < topo.nlist.size(); ++k)
+        topo.cpos[static_cast<size_t>(topo.nlist[k] & 3) * T + (topo.nlist[k] >> 2)] = static_cast<int>(k);
 
     // reverse-mode gather lists: node <- (tet, corner) over live tets in order
     topo.astart.assign(static_cast<size_t>(N) + 1, 0);

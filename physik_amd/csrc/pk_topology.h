@@ -4,6 +4,8 @@
 // gather lists are bit-exact functions of the inputs).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -22,16 +24,30 @@ enum TetParam
     P_LAMBDA = 19,
     P_MU = 20,
     P_DAMP = 21,
-    P_COUNT = 22
+    P_GDOT = 22,    // 10 values g_a . g_b for the local pairs a <= b (written on the device)
+    P_COUNT = 32
 };
 
-// Element record written by the element kernel, one per tet.
+// Element output of one substep:
+//   basis   18 planes, basis[k * T + t]: R g_a (k = 3a + i), then R R^T (xx,
+//           xy, xz, yy, yz, zz). The blocks R K_ab R^T are not stored; the
+//           slot gather rebuilds each from these with the closed form the
+//           element kernel also uses (coalesced writes, 144 B per tet).
+//   corner  6 doubles per (dynamic row, tet, corner) in the order of the row
+//           lists: the element force f_a, then sum_b (R K_ab R^T) v_b, written
+//           at cpos[a * T + t], so the right-hand side reads a contiguous run.
+// Reverse mode keeps its per-tet corner terms in planes adj[k * T + t]:
+// dPhi/dx_a (k = 3a + i), then dPhi/dv_a (k = 12 + 3a + i).
 enum ElemLayout
 {
-    E_BLOCKS = 0,     // 10 blocks R K_ab R^T (a <= b), 9 values each, row-major
-    E_FORCE = 90,     // 4 x 3 element forces (elastic + damping)
-    E_KV = 102,       // 4 x 3 sum_b (R K_ab R^T) v_b
-    E_STRIDE = 114
+    EB_RG = 0,
+    EB_RRT = 12,
+    EB_COUNT = 18,
+    N_CORNER = 6,     // doubles per corner entry
+    N_KV = 3,         // offset of K v in a corner entry
+    EP_ADJ_X = 0,
+    EP_ADJ_V = 12,
+    EP_ADJ_COUNT = 24
 };
 
 // Packed gather entry for one upper slot: tet << 5 | pair << 1 | transpose.
@@ -61,6 +77,7 @@ struct Topology
     std::vector<int> uFull, uMirror;     // upper slot -> full index of (r,c) and of (c,r) (-1 on the diagonal)
     std::vector<int> cstart, clist;      // upper slot -> packed contributions, increasing tet
     std::vector<int> nstart, nlist;      // dynamic row -> packed (tet, corner), increasing tet
+    std::vector<int> cpos;               // 4 T: row-list index of corner a of tet t, or -1
     std::vector<int> astart, alist;      // every node -> packed (tet, corner) over live tets, increasing tet
     std::vector<int> nodeGroup;          // N -> body
     std::vector<int> scatter;            // optional T*16 upper slot of each local pair, -1 if not dynamic

@@ -113,9 +113,12 @@ def test_element_kernel_matches_oracle():
     assert np.abs(kv - kv_ref).max() <= 1e-12 * np.abs(kv_ref).max()
 
 
-def test_assembly_and_spmv_match_oracle():
+@pytest.mark.parametrize("precision,tol", [(N.PK_FP64, 1e-12), (N.PK_FP32_STORAGE, 1e-6)])
+def test_assembly_and_spmv_match_oracle(precision, tol):
+    # FP32 storage keeps A in 40-byte records (9 floats + column): a rounding
+    # of each entry, nothing else
     w = deformed_beam()
-    s = W.make_system(w)
+    s = W.make_system(w, precision)
     s.assemble(w.dt)
     vals, rhs, minv = s.matrix()
     o = oracle_for(w)
@@ -129,14 +132,14 @@ def test_assembly_and_spmv_match_oracle():
             lo, hi = min(r, c), max(r, c)
             j = ref["row_start"][lo] + np.searchsorted(ref["col"][ref["row_start"][lo]:ref["row_start"][lo + 1]], hi)
             expect = ref["vals"][j] if r <= c else ref["vals"][j].T
-            assert np.abs(vals[k] - expect).max() <= 1e-12 * np.abs(ref["vals"]).max()
+            assert np.abs(vals[k] - expect).max() <= tol * np.abs(ref["vals"]).max()
     assert np.abs(rhs - ref["rhs"]).max() <= 1e-11 * np.abs(ref["rhs"]).max()
     diag = np.array([ref["vals"][ref["row_start"][r]] for r in range(p.nd)])
     assert np.abs(minv - np.linalg.inv(diag)).max() <= 1e-12 * np.abs(minv).max()
     x = np.sin(np.arange(p.nd * 3.0)).reshape(-1, 3)
     y = s.spmv(x)
     y_ref = oracle_spmv(ref["row_start"], ref["col"], ref["vals"], x)
-    assert np.abs(y - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
+    assert np.abs(y - y_ref).max() <= 10 * tol * np.abs(y_ref).max()
 
 
 def test_two_runs_are_bit_identical():
