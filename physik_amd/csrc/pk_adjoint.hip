@@ -141,7 +141,7 @@ template <class M>
 __global__ __launch_bounds__(NT) void k_adj_rhs(TileView tv, const int* __restrict__ rowNode,
                                                 const int* __restrict__ ok, const double* __restrict__ xbar,
                                                 const double* __restrict__ vbar, double dt,
-                                                const double* __restrict__ minv, int nd, double* __restrict__ xs,
+                                                const M* __restrict__ minv, int nd, double* __restrict__ xs,
                                                 double* __restrict__ rv, M* __restrict__ zv, M* __restrict__ d0,
                                                 Partial* __restrict__ part)
 {
@@ -164,8 +164,9 @@ __global__ __launch_bounds__(NT) void k_adj_rhs(TileView tv, const int* __restri
             flag |= 1;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            z[i] = minv[(3 * i) * N + row] * b[0] + minv[(3 * i + 1) * N + row] * b[1] +
-                   minv[(3 * i + 2) * N + row] * b[2];
+            z[i] = static_cast<double>(minv[(3 * i) * N + row]) * b[0] +
+                   static_cast<double>(minv[(3 * i + 1) * N + row]) * b[1] +
+                   static_cast<double>(minv[(3 * i + 2) * N + row]) * b[2];
         if (!(fin(z[0]) && fin(z[1]) && fin(z[2])))
             flag |= 2;
         M zs[3];
@@ -348,7 +349,7 @@ void launch_adj_rhs(const DeviceView& v, const int* ok, const double* xbar, cons
     if (v.vec.count == 0)
         return;
     hipLaunchKernelGGL((k_adj_rhs<M>), dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, v.vec,
-                       v.rowNode, ok, xbar, vbar, dt, v.minv, v.nd, v.xs, v.r, static_cast<M*>(v.z),
+                       v.rowNode, ok, xbar, vbar, dt, static_cast<const M*>(v.minv), v.nd, v.xs, v.r, static_cast<M*>(v.z),
                        static_cast<M*>(v.d[0]), v.pb[1]);
 }
 

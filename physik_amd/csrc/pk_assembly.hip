@@ -303,7 +303,7 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
                                                  const int* __restrict__ rowGroup, const double* __restrict__ basis,
                                                  const double* __restrict__ prm, int T,
                                                  const double* __restrict__ mrow, double dt2, M* __restrict__ vals,
-                                                 long long nnzf, double* __restrict__ minv, int nd, GroupState* gs)
+                                                 long long nnzf, M* __restrict__ minv, int nd, GroupState* gs)
 {
     const int s = (blockIdx.x * NT + threadIdx.x) / kSlotLanes;
     const int lane = threadIdx.x % kSlotLanes;
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
         }
 #pragma unroll
         for (int e = 0; e < 9; ++e)
-            minv[static_cast<size_t>(e) * nd + r] = mi[e];
+            minv[static_cast<size_t>(e) * nd + r] = static_cast<M>(mi[e]);
     }
 }
 
@@ -402,19 +402,21 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
 constexpr int kRowLanes = 4;
 
 // z = M^-1 r with the block-Jacobi inverse stored as 9 planes of nd values.
-__device__ inline void apply_minv(const double* __restrict__ minv, size_t nd, int row, const double* r, double* z)
+template <class M>
+__device__ inline void apply_minv(const M* __restrict__ minv, size_t nd, int row, const double* r, double* z)
 {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        z[i] = minv[(3 * i) * nd + row] * r[0] + minv[(3 * i + 1) * nd + row] * r[1] +
-               minv[(3 * i + 2) * nd + row] * r[2];
+        z[i] = static_cast<double>(minv[(3 * i) * nd + row]) * r[0] +
+               static_cast<double>(minv[(3 * i + 1) * nd + row]) * r[1] +
+               static_cast<double>(minv[(3 * i + 2) * nd + row]) * r[2];
 }
 
 template <class M>
 __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__ nstart,
                                             const double* __restrict__ corner,
                                             const double* __restrict__ mrow, double gx, double gy, double gz,
-                                            double dt, double dt2, const double* __restrict__ minv, int nd,
+                                            double dt, double dt2, const M* __restrict__ minv, int nd,
                                             double* __restrict__ xs, double* __restrict__ rv, M* __restrict__ zv,
                                             M* __restrict__ d0, Partial* __restrict__ part)
 {
@@ -586,7 +588,7 @@ void launch_assemble(const DeviceView& v, double dt)
         hipLaunchKernelGGL((k_assemble<M>), dim3((n * kSlotLanes + NT - 1) / NT), dim3(NT), 0, v.stream, n, v.cstart,
                            v.clist,
                            v.uFull, v.uMirror, v.slotRow, v.rowGroup, v.basis, v.prm, v.T, v.mrow, dt * dt,
-                           static_cast<M*>(v.vals), v.nnzf, v.minv, v.nd, v.gs);
+                           static_cast<M*>(v.vals), v.nnzf, static_cast<M*>(v.minv), v.nd, v.gs);
 }
 
 template <class M>
@@ -594,7 +596,7 @@ void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt)
 {
     if (v.vec.count > 0)
         hipLaunchKernelGGL((k_rhs<M>), dim3(grid_of(v.vec)), dim3(NT), 0, v.stream, v.vec, v.nstart,
-                           v.corner, v.mrow, gx, gy, gz, dt, dt * dt, v.minv, v.nd, v.xs, v.r, static_cast<M*>(v.z),
+                           v.corner, v.mrow, gx, gy, gz, dt, dt * dt, static_cast<const M*>(v.minv), v.nd, v.xs, v.r, static_cast<M*>(v.z),
                            static_cast<M*>(v.d[0]), v.pb[1]);
 }
 

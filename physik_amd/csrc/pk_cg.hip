@@ -308,7 +308,7 @@ __global__ __launch_bounds__(NT) void k_spmv(int k, TileView st, const int* __re
 }
 
 template <class M>
-__device__ inline void update_body(int k, const TileView& vt, const TileView& st, const double* __restrict__ minv,
+__device__ inline void update_body(int k, const TileView& vt, const TileView& st, const M* __restrict__ minv,
                                    int nd, const M* __restrict__ dcur, const M* __restrict__ q,
                                    double* __restrict__ xs, double* __restrict__ rv, M* __restrict__ zv,
                                    const double* __restrict__ pa, Partial* __restrict__ pbCur,
@@ -351,8 +351,9 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            z[i] = minv[(3 * i) * N + row] * r[0] + minv[(3 * i + 1) * N + row] * r[1] +
-                   minv[(3 * i + 2) * N + row] * r[2];
+            z[i] = static_cast<double>(minv[(3 * i) * N + row]) * r[0] +
+                   static_cast<double>(minv[(3 * i + 1) * N + row]) * r[1] +
+                   static_cast<double>(minv[(3 * i + 2) * N + row]) * r[2];
         if (!(fin(z[0]) && fin(z[1]) && fin(z[2])))
             flag = 1;
         M zs[3];
@@ -373,7 +374,7 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
 }
 
 template <class M>
-__global__ __launch_bounds__(NT) void k_update(int k, TileView vt, TileView st, const double* __restrict__ minv,
+__global__ __launch_bounds__(NT) void k_update(int k, TileView vt, TileView st, const M* __restrict__ minv,
                                                int nd, const M* __restrict__ dcur, const M* __restrict__ q,
                                                double* __restrict__ xs, double* __restrict__ rv, M* __restrict__ zv,
                                                const double* __restrict__ pa, Partial* __restrict__ pbCur,
@@ -486,7 +487,7 @@ void launch_update(const DeviceView& v, int k, unsigned long long* stamp)
     if (v.vec.count == 0)
         return;
     hipLaunchKernelGGL((k_update<M>), dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, k, v.vec,
-                       v.spmv, v.minv, v.nd, static_cast<const M*>(v.d[k & 1]), static_cast<const M*>(v.q), v.xs,
+                       v.spmv, static_cast<const M*>(v.minv), v.nd, static_cast<const M*>(v.d[k & 1]), static_cast<const M*>(v.q), v.xs,
                        v.r, static_cast<M*>(v.z), v.pa, v.pb[k & 1], v.pb[(k + 1) & 1], v.gs, v.active, stamp);
 }
 

@@ -219,7 +219,7 @@ struct DeviceView
     double* basis;       // per-tet R g_a and R R^T planes
     double* corner;      // forces and K v in row order
     double* adj;         // reverse-mode corner terms, EP_ADJ_COUNT planes of T
-    double* minv;
+    void* minv;   // M: block-Jacobi inverse, 9 planes of nd
     double* xs;   // CG solution dv
     double* r;
     void* vals;   // M

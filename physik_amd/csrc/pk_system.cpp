@@ -113,7 +113,8 @@ struct pk_system
 
     pk::DevBuf<int4> tets;
     pk::DevBuf<unsigned char> live;
-    pk::DevBuf<double> prm, x, v, basis, corner, adj, mrow, minv, xs, r, pa;
+    pk::DevBuf<double> prm, x, v, basis, corner, adj, mrow, xs, r, pa;
+    pk::DevBuf<char> minv;  // storage type, like the CG vectors
     pk::DevBuf<int> cstart, clist, uFull, uMirror, slotRow, rowGroup, fstart, fcol, nstart, nlist, rowNode, active;
     pk::DevBuf<int> astart, alist, nodeRow, nodeGroup, cpos;
     pk::DevBuf<char> vals, z, d0, d1, q;  // element type set by the precision
@@ -518,7 +519,7 @@ static int alloc_work(pk_system* s)
     if ((rc = s->d0.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->d1.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->q.alloc(nd3 * es)) != PK_OK) return rc;
-    if ((rc = s->minv.alloc(static_cast<size_t>(t.nd) * 9)) != PK_OK) return rc;
+    if ((rc = s->minv.alloc(static_cast<size_t>(t.nd) * 9 * es)) != PK_OK) return rc;
     if ((rc = s->xs.alloc(nd3)) != PK_OK) return rc;
     if ((rc = s->r.alloc(nd3)) != PK_OK) return rc;
     if ((rc = s->active.alloc(1)) != PK_OK) return rc;
@@ -1086,11 +1087,16 @@ int pk_system_get_matrix(pk_system* s, double* full_vals, double* rhs, double* m
         PK_HIP(hipMemcpy(rhs, s->r.p, static_cast<size_t>(t.nd) * 3 * sizeof(double), hipMemcpyDeviceToHost));
     if (minv && t.nd)
     {
-        std::vector<double> raw(static_cast<size_t>(t.nd) * 9);
-        PK_HIP(hipMemcpy(raw.data(), s->minv.p, raw.size() * sizeof(double), hipMemcpyDeviceToHost));
+        const bool f32 = s->precision == PK_FP32_STORAGE;
+        std::vector<char> raw(static_cast<size_t>(t.nd) * 9 * (f32 ? 4 : 8));
+        PK_HIP(hipMemcpy(raw.data(), s->minv.p, raw.size(), hipMemcpyDeviceToHost));
         for (int r = 0; r < t.nd; ++r)
             for (int e = 0; e < 9; ++e)
-                minv[9 * r + e] = raw[static_cast<size_t>(e) * t.nd + r];
+            {
+                const size_t i = static_cast<size_t>(e) * t.nd + r;
+                minv[9 * r + e] = f32 ? reinterpret_cast<const float*>(raw.data())[i]
+                                      : reinterpret_cast<const double*>(raw.data())[i];
+            }
     }
     return PK_OK;
 }
