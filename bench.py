@@ -16,9 +16,10 @@ plain single-body step (64^3 twisted Kuhn beam, FP64, 1.57 M tets), and the
 CPU baselines (the oracle's FP64 forward + adjoint and the reference's own
 FP32 forward, one host core, a bounded sample of the same workload).
 
-`value` is tet-steps/s of the default workload; the SpMV roofline uses the
-kernel's in-kernel wall-clock span over the timed region (the span a kernel
-trace reports) and the algorithmic bytes of one launch.
+`value` is tet-steps/s of the default workload; the SpMV roofline divides
+the algorithmic bytes of the work the timed launches did (blocks and rows of
+the bodies still iterating in each launch) by the kernel's summed in-kernel
+wall-clock spans (the span a kernel trace reports).
 """
 
 from __future__ import annotations
@@ -42,7 +43,8 @@ DESCRIPTION = {
                 "per-tet log E ~ N(log 1e5, 0.2), nu 0.3, rho 1000, x=0 face fixed per body, per-body loss on the +x "
                 "face vs a U[-0.05,0.05]^3 m offset target, dt 1/60 s, forward rtol 1e-6, adjoint rtol 1e-7, "
                 "checkpoint every 8",
-    "diff32": "32^3-cell Kuhn beam, 196,608 tets, 35,937 nodes, per-tet log E ~ N(log 1e5, 0.2) (seed 3), nu 0.3, "
+    "diff32": "32^3-cell Kuhn beam, h 1/64 m, 196,608 tets, 35,937 nodes, per-tet log E ~ N(log 1e5, 0.2) (seed 3), "
+              "nu 0.3, "
               "rho 1000, x=0 face fixed, loss on the +x face vs 0.05 m down, dt 1/60 s, rtol 1e-6, checkpoint every "
               "10",
     "twist64": "64^3-cell Kuhn beam, 1,572,864 tets, 274,625 nodes, rest nodes jittered +-10% (seed 1), 90-degree "
@@ -64,33 +66,41 @@ def workload(name):
             "batch128": W.batch128}[name]()
 
 
-def spmv_bytes(nnzf, nd, s):
-    """Algorithmic bytes of one SpMV launch q = A d: the full BSR once (9
-    values + int32 column per block, row pointers), d read once, q written
-    once."""
-    return nnzf * (9 * s + 4) + (nd + 1) * 4 + 2 * nd * 3 * s
+def spmv_bytes(blocks, rows, body_launches, s):
+    """Algorithmic bytes of SpMV work q = A d: every multiplied 3x3 block once
+    (9 values of s bytes + int32 column), per row its int32 row pointer, one
+    d entry read and one q entry written (3 values each), and the closing row
+    pointer of each body in each launch. One launch over a whole system is
+    spmv_bytes(nnzb_full, nd, bodies, s)."""
+    return blocks * (9 * s + 4) + rows * (4 + 2 * 3 * s) + 4 * body_launches
 
 
 def measured_traffic(workload_name, precision):
     """HBM bytes per SpMV launch from the committed counter passes
-    (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size), or
-    None when no pass covers this workload and precision."""
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size) and
+    the algorithmic bytes per launch of that same profiled run, or None when
+    no pass covers this workload and precision."""
     path = os.path.join(ROOT, "profiles", "spmv_traffic.json")
     try:
         with open(path) as f:
             entry = json.load(f).get(f"{workload_name}/{precision}")
-        return None if entry is None else float(entry["bytes"])
+        return None if entry is None else (float(entry["bytes"]), float(entry["algorithmic"]))
     except (OSError, ValueError, KeyError, TypeError):
         return None
 
 
 def roofline(s, name, prec_name):
+    """SpMV roofline over the timed region: algorithmic bytes of the work the
+    counted launches did (a batched launch skips bodies whose CG loop has
+    ended, so late launches move less) over their summed in-kernel spans."""
     from physik_amd import native as N
     n_spmv, ms_spmv = s.kernel_time(N.K_SPMV)
     esize = 8 if s.precision == N.PK_FP64 else 4
-    per_launch = spmv_bytes(s.nnzb_full, s.nd, esize)
-    avg_s = ms_spmv / max(n_spmv, 1) * 1e-3
-    achieved = per_launch / avg_s / 1e9 if n_spmv else 0.0
+    blocks, rows, body_launches = s.spmv_work()
+    total = spmv_bytes(blocks, rows, body_launches, esize)
+    full = spmv_bytes(s.nnzb_full, s.nd, s.groups, esize)
+    achieved = total / (ms_spmv * 1e-3) / 1e9 if n_spmv and ms_spmv > 0 else 0.0
+    measured = measured_traffic(name, prec_name)
     return {
         "bound": "hbm",
         "kernel": "k_spmv (q = A d_k on the full BSR, d.q partial per tile)",
@@ -98,9 +108,12 @@ def roofline(s, name, prec_name):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": measured_traffic(name, prec_name),
-        "bytes_per_launch": per_launch,
-        "avg_launch_us": avg_s * 1e6,
+        "traffic": None if measured is None else measured[0],
+        "traffic_vs_algorithmic": None if measured is None else measured[0] / measured[1],
+        "bytes_per_launch": total / max(n_spmv, 1),
+        "bytes_per_full_launch": full,
+        "active_fraction": total / (full * max(n_spmv, 1)),
+        "avg_launch_us": ms_spmv / max(n_spmv, 1) * 1e3,
         "launches": n_spmv,
     }
 
@@ -211,34 +224,44 @@ def _oracle(checkers, wb, precision=64):
 
 def cpu_fwd_adj(w, name, steps, budget_s):
     """The oracle's FP64 forward + adjoint on one host core, body by body of
-    the same workload, as many bodies as fit the budget (at least one)."""
+    the same workload, as many bodies as fit the budget (at least one).  A
+    body too large for the budget is sampled over its first substeps: the
+    untimed forward that produces the loss seed stops once it has used a
+    third of the budget (the recorded forward plus the reverse pass cost
+    about three forwards)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import checkers
     from physik_amd import workloads as W
-    tets, bodies, el = 0, 0, 0.0
+    tet_steps, bodies, el, sub = 0, 0, 0.0, steps
     for b in range(w.n_bodies):
         wb = w.body(b) if w.n_bodies > 1 else w
         # the loss seed needs the final state: an untimed forward run first
         p = _oracle(checkers, wb)
-        for _ in range(steps):
+        t0 = time.perf_counter()
+        sub = 0
+        while sub < steps:
             p.step(wb.dt)
+            sub += 1
+            if bodies == 0 and time.perf_counter() - t0 >= budget_s / 3:
+                break
         xbar = W.loss_and_seed(wb, p.state()[0])[1]
         p.close()
         o = _oracle(checkers, wb)
         t0 = time.perf_counter()
-        o.adjoint(wb.dt, steps, xbar, None, tol=wb.adj_tol, max_iter=wb.max_iter)
+        o.adjoint(wb.dt, sub, xbar, None, tol=wb.adj_tol, max_iter=wb.max_iter)
         el += time.perf_counter() - t0
         o.close()
         bodies += 1
-        tets += wb.n_tets
-        if el >= budget_s:
+        tet_steps += wb.n_tets * sub
+        if el >= budget_s or sub < steps:
             break
+    what = f"{steps} substeps" if sub == steps else f"the first {sub} of {steps} substeps"
     return {
-        "value": tets * steps / max(el, 1e-9),
+        "value": tet_steps / max(el, 1e-9),
         "unit": "tet·steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{bodies} of {w.n_bodies} bodies of {name} ({tets:,} tets), {steps} substeps forward + adjoint "
+        "sample": f"{bodies} of {w.n_bodies} bodies of {name} ({tet_steps // sub:,} tets), {what} forward + adjoint "
                   f"each (record, then reverse), FP64 oracle restatement (the reference has no adjoint), one host "
                   f"core, {el:.1f} s timed",
     }

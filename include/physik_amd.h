@@ -142,6 +142,12 @@ int pk_system_group_results(pk_system* sys, int* iterations, double* residual_no
 int pk_system_profile(pk_system* sys, int enable);
 int pk_system_kernel_time(pk_system* sys, int kernel, long long* launches, double* total_ms);
 
+/* Work the profiled SpMV launches did, summed over launches: full BSR blocks
+ * and dynamic rows multiplied, and (body, launch) pairs. The rows of a body
+ * whose CG loop has ended are skipped, so in a batched solve late launches
+ * multiply only the bodies still iterating. Reset by pk_system_profile. */
+int pk_system_spmv_work(pk_system* sys, long long* blocks, long long* rows, long long* body_launches);
+
 /* ---- inspection hooks used by the parity tests ---- */
 
 /* Runs the element and assembly kernels of one substep of length dt (no

@@ -89,6 +89,7 @@ struct Profile
     bool on = false;
     long long launches[PK_K_COUNT] = {};
     double ms[PK_K_COUNT] = {};
+    long long spmvBlocks = 0, spmvRows = 0, spmvBodyLaunches = 0;  // work of the counted SpMV launches
     std::vector<hipEvent_t> pool;
     int used = 0;
     struct Rec
@@ -145,6 +146,7 @@ struct pk_system
     int implicit = 1;
     int lastIters = 0;
     std::vector<pk::GroupState> hostGs;
+    std::vector<long long> groupBlocks, groupRows;  // full BSR blocks and dynamic rows per group
     pk::Profile prof;
 
     ~pk_system()
@@ -335,6 +337,16 @@ static void harvest_profile(pk_system* s)
                     p.launches[kernel] += 1;
                 }
             };
+            for (size_t g = 0; g < s->hostGs.size(); ++g)
+                if (s->hostGs[g].hasRows)
+                {
+                    // SpMV launches k < stopAt (+1 when the loop ended in the
+                    // update) multiplied this body's rows
+                    const long long c = s->hostGs[g].stopAt + s->hostGs[g].stopInUpdate;
+                    p.spmvBlocks += c * s->groupBlocks[g];
+                    p.spmvRows += c * s->groupRows[g];
+                    p.spmvBodyLaunches += c;
+                }
             for (int k = 0; k < n; ++k)
             {
                 if (k < spmvWork)
@@ -562,9 +574,16 @@ static int setup_device(pk_system* s)
     for (int r = 0; r < t.nd; ++r)
         for (int k = t.ustart[r]; k < t.ustart[r + 1]; ++k)
             slotRow[k] = r;
+    s->groupBlocks.assign(static_cast<size_t>(t.G), 0);
+    s->groupRows.assign(static_cast<size_t>(t.G), 0);
     for (int g = 0; g < t.G; ++g)
+    {
         for (int r = t.groupRowBegin[g]; r < t.groupRowEnd[g]; ++r)
             rowGroup[r] = g;
+        s->groupRows[g] = t.groupRowEnd[g] - t.groupRowBegin[g];
+        if (s->groupRows[g] > 0)
+            s->groupBlocks[g] = t.fstart[t.groupRowEnd[g]] - t.fstart[t.groupRowBegin[g]];
+    }
     if ((rc = s->slotRow.upload(slotRow, st)) != PK_OK) return rc;
     if ((rc = s->rowGroup.upload(rowGroup, st)) != PK_OK) return rc;
     if ((rc = s->fstart.upload(t.fstart, st)) != PK_OK) return rc;
@@ -1035,8 +1054,19 @@ int pk_system_profile(pk_system* s, int enable)
         s->prof.launches[k] = 0;
         s->prof.ms[k] = 0.0;
     }
+    s->prof.spmvBlocks = s->prof.spmvRows = s->prof.spmvBodyLaunches = 0;
     s->prof.recs.clear();
     s->prof.used = 0;
+    return PK_OK;
+}
+
+int pk_system_spmv_work(pk_system* s, long long* blocks, long long* rows, long long* body_launches)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    if (blocks) *blocks = s->prof.spmvBlocks;
+    if (rows) *rows = s->prof.spmvRows;
+    if (body_launches) *body_launches = s->prof.spmvBodyLaunches;
     return PK_OK;
 }
 

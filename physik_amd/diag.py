@@ -6,6 +6,8 @@
 Run under ``rocprofv3 --pmc FETCH_SIZE`` (and again with WRITE_SIZE): the copy
 kernel moves exactly copy-mb MiB each way, which calibrates the counters; the
 implicit steps that follow give the per-dispatch counts of every step kernel.
+The JSON line it prints carries the work the SpMV launches did (blocks and
+rows of the bodies still iterating), to set the counters against.
 """
 
 from __future__ import annotations
@@ -35,12 +37,15 @@ def main():
     s = W.make_system(w, N.PK_FP64 if args.precision == "fp64" else N.PK_FP32_STORAGE)
     if args.shape:
         s.tune(*[int(a) for a in args.shape.split(",")])
+    s.profile(True)
     s.step(w.dt, 1, args.steps)
     it = s.results()[0]
+    blocks, rows, body_launches = s.spmv_work()
     print(json.dumps({"copy_bytes": nbytes, "copy_ms": ms.value, "copy_GBps": 2 * nbytes / ms.value / 1e6,
                       "workload": args.workload, "steps": args.steps, "cg_iterations": int(it.max()),
-                      "nnzb_full": s.nnzb_full, "dynamic_rows": s.nd, "tets": s.t,
-                      "shape": s.get_tune()}), flush=True)
+                      "nnzb_full": s.nnzb_full, "dynamic_rows": s.nd, "tets": s.t, "bodies": s.groups,
+                      "spmv_launches": s.kernel_time(N.K_SPMV)[0], "spmv_blocks": blocks, "spmv_rows": rows,
+                      "spmv_body_launches": body_launches, "shape": s.get_tune()}), flush=True)
 
 
 if __name__ == "__main__":

@@ -150,6 +150,13 @@ class System:
         N.check(self.lib.pk_system_kernel_time(self.h, kernel, C.byref(n), C.byref(ms)), "pk_system_kernel_time")
         return n.value, ms.value
 
+    def spmv_work(self):
+        """(full BSR blocks, dynamic rows, body-launches) the profiled SpMV
+        launches multiplied; stopped bodies' rows are not counted."""
+        v = [C.c_longlong() for _ in range(3)]
+        N.check(self.lib.pk_system_spmv_work(self.h, *[C.byref(x) for x in v]), "pk_system_spmv_work")
+        return tuple(x.value for x in v)
+
     def assemble(self, dt):
         N.check(self.lib.pk_system_assemble(self.h, dt), "pk_system_assemble")
 

@@ -8,6 +8,10 @@ Bars:
   batched bodies: bit-identical to the same bodies reversed one by one
   the GPU adjoint vs central differences of the GPU forward itself (FP64,
       tight CG): relative error <= 1e-5
+  diff32 at full size (196,608 tets, 50 substeps, checkpoint 10): 16 random
+      parameters vs central differences (h = 1e-6) to 1e-4 relative above the
+      measured difference noise, and random directional derivatives to 1e-6
+  the same body at 16^3 cells vs the FP64 oracle adjoint: relative L2 <= 1e-8
 The oracle adjoint is itself pinned by finite differences on the CPU
 (tests/test_oracle_adjoint.py).
 """
@@ -52,7 +56,7 @@ def small(n=(4, 3, 3), steps=6, damping=0.3, seed=21, tol=1e-9):
 
 def gpu_grad(w, precision=N.PK_FP64, every=None, vbar_scale=0.1):
     s = W.make_system(w, precision)
-    s.record(w.dt, w.steps, every or w.steps)
+    s.record(w.dt, w.steps, every or w.checkpoint or w.steps)
     x, v = s.state()
     L, xbar = W.loss_and_seed(w, x)
     vbar = 2 * vbar_scale * v
@@ -166,3 +170,71 @@ def test_backward_without_record_is_rejected():
     s.set_young(2e4)  # a material change discards the recording
     with pytest.raises(RuntimeError, match="no recorded run"):
         s.backward(None, None)
+
+
+def test_diff32_gradients_match_finite_differences_on_16_parameters():
+    # the differentiable single-body workload at full size: 32^3-cell Kuhn beam
+    # (196,608 tets), 50 substeps, checkpoint every 10; forward and adjoint CG
+    # tightened to 1e-10 so the differences measure the map, not the solves.
+    # One log E or one v0 component moves the loss by ~1e-10 at h = 1e-6,
+    # which is also the size of the solver noise in the loss: central
+    # differences carry an absolute error of up to ~8e-10 here (measured, the
+    # same at CG 1e-12), so each parameter is held to 1e-4 relative plus a
+    # floor of 2e-6 * max|gradient| (3e-9 for log E, 1.4e-8 for v0).
+    w = W.diff32()
+    w.tol = w.adj_tol = 1e-10
+    print("case diff32 record/backward", flush=True)
+    s = W.make_system(w)
+    s.record(w.dt, w.steps, w.checkpoint)
+    x, _ = s.state()
+    L0, xbar = W.loss_and_seed(w, x)
+    gE, _, gv, its = s.backward(xbar, None, tol=w.adj_tol, max_iter=w.max_iter)
+    assert (its > 0).all() and (its < w.max_iter).all()
+
+    def loss(young=None, v0=None):
+        s.set_young(w.young if young is None else young)
+        s.set_state(w.x0, w.v0 if v0 is None else v0)
+        s.step(w.dt, 1, w.steps)
+        return W.loss_and_seed(w, s.state()[0])[0]
+
+    assert loss() == L0, "a replayed forward must reproduce the recorded run"
+    h = 1e-6
+    rng = np.random.default_rng(32)
+    floorE, floorV = 2e-6 * np.abs(gE).max(), 2e-6 * np.abs(gv).max()
+    for e in rng.choice(w.n_tets, 8, replace=False):
+        fd = (loss(young=w.young * np.exp(h * (np.arange(w.n_tets) == e)))
+              - loss(young=w.young * np.exp(-h * (np.arange(w.n_tets) == e)))) / (2 * h)
+        print(f"  log E[{e}] adjoint {gE[e]: .8e} fd {fd: .8e}", flush=True)
+        assert abs(gE[e] - fd) <= 1e-4 * abs(fd) + floorE, (e, gE[e], fd)
+    free = np.flatnonzero(w.fixed == 0)
+    for i in rng.choice(free, 8, replace=False):
+        k = int(rng.integers(3))
+        vp, vm = w.v0.copy(), w.v0.copy()
+        vp[i, k] += h
+        vm[i, k] -= h
+        fd = (loss(v0=vp) - loss(v0=vm)) / (2 * h)
+        print(f"  v0[{i},{k}] adjoint {gv[i, k]: .8e} fd {fd: .8e}", flush=True)
+        assert abs(gv[i, k] - fd) <= 1e-4 * abs(fd) + floorV, (i, k, gv[i, k], fd)
+    # every parameter at once: directional derivatives along random directions,
+    # where the signal is the whole gradient (measured agreement ~4e-8)
+    dE = rng.normal(size=w.n_tets)
+    dv = rng.normal(size=w.v0.shape) * (w.fixed == 0)[:, None]
+    h = 1e-5
+    fdE = (loss(young=w.young * np.exp(h * dE)) - loss(young=w.young * np.exp(-h * dE))) / (2 * h)
+    fdV = (loss(v0=w.v0 + h * dv) - loss(v0=w.v0 - h * dv)) / (2 * h)
+    print(f"  directional: log E {gE @ dE:.10e} vs {fdE:.10e}, v0 {np.sum(gv * dv):.10e} vs {fdV:.10e}", flush=True)
+    assert abs(gE @ dE - fdE) <= 1e-6 * abs(fdE)
+    assert abs(np.sum(gv * dv) - fdV) <= 1e-6 * abs(fdV)
+    s.close()
+
+
+def test_diff16_gradients_match_the_cpu_oracle_adjoint():
+    # the same 0.5 m body at 16^3 cells (24,576 tets), GPU against the FP64 CPU adjoint
+    w = W.diff32(n=16, h=1.0 / 32.0, steps=20)
+    w.checkpoint = 5
+    print("case diff16 vs oracle", flush=True)
+    _, gE, gx, gv, its, x = gpu_grad(w, vbar_scale=0.0)
+    oE, ox, ov, oits, xo = oracle_grad(w, vbar_scale=0.0)
+    assert rel(x, xo) <= 1e-11
+    assert rel(gE, oE) <= 1e-8 and rel(gv, ov) <= 1e-8 and rel(gx, ox) <= 1e-8, (rel(gE, oE), rel(gv, ov), rel(gx, ox))
+    assert np.abs(its - oits).max() <= 1

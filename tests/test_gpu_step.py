@@ -17,7 +17,7 @@ from checkers import Oracle, oracle_spmv
 from physik_amd import meshgen as mg
 from physik_amd import native as N
 from physik_amd import workloads as W
-from physik_amd.system import System
+from physik_amd.system import Pattern, System
 
 pytestmark = pytest.mark.gpu
 
@@ -283,6 +283,40 @@ def test_kernel_spans_count_working_launches():
     assert ms_s > 0 and ms_u > 0
     # two solves: every iteration's SpMV and update did work
     assert n_s >= 2 * it - 2 and n_u >= 2 * it - 2 and n_s <= 2 * (it + 3)
+
+
+def test_spmv_work_counts_only_bodies_still_iterating():
+    # one body: every counted launch multiplied the whole matrix
+    w = W.beam10()
+    s = W.make_system(w)
+    s.profile(True)
+    s.step(w.dt, 1, 2)
+    n_s, _ = s.kernel_time(N.K_SPMV)
+    blocks, rows, bl = s.spmv_work()
+    assert bl == n_s and blocks == n_s * s.nnzb_full and rows == n_s * s.nd
+    # bodies that converge at different iterations: a body takes part in its
+    # iterations' launches (one more when its loop ended in the update)
+    w = W.batch256(bodies=6, lo=3, hi=8, steps=1)
+    s = W.make_system(w)
+    s.profile(True)
+    s.step(w.dt)
+    n_s, _ = s.kernel_time(N.K_SPMV)
+    blocks, rows, bl = s.spmv_work()
+    it = s.results()[0]
+    pat = Pattern(w.rest, w.tets, node_fixed=w.fixed, node_group=w.node_group)
+    per_blocks, per_rows = [], []
+    for n0, n1 in w.body_nodes:
+        r = pat.dyn[n0:n1]
+        r = r[r >= 0]
+        per_rows.append(len(r))
+        per_blocks.append(int(pat.full_start[r.max() + 1] - pat.full_start[r.min()]))
+    per_blocks, per_rows = np.array(per_blocks), np.array(per_rows)
+    assert per_blocks.sum() == s.nnzb_full and per_rows.sum() == s.nd
+    assert len(set(it.tolist())) > 1, "bodies should converge at different iterations"
+    assert it.sum() <= bl <= (it + 1).sum() and it.max() <= n_s <= it.max() + 1
+    assert (per_blocks * it).sum() <= blocks <= (per_blocks * (it + 1)).sum()
+    assert (per_rows * it).sum() <= rows <= (per_rows * (it + 1)).sum()
+    assert blocks < n_s * s.nnzb_full
 
 
 def test_autotune_keeps_state_and_caches(tmp_path):
