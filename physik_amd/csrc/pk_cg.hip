@@ -146,10 +146,10 @@ template <class M>
 __device__ inline void dir_body(int k, const TileView& vt, const M* __restrict__ z, const M* __restrict__ dprev,
                                 M* __restrict__ dcur, const Partial* __restrict__ pbPrev, GroupState* gs, int* active)
 {
-    const int t = tile_of(vt);
-    if (t < 0)
+    int t;
+    RowTile tl;
+    if (!tile_for(vt, t, tl))
         return;
-    const RowTile tl = vt.tiles[t];
     const int g = tl.group;
     GroupState& s = gs[g];
     if (read_uniform(&s.stopAt) >= 0)
@@ -183,7 +183,7 @@ __device__ inline void load_block(const double* __restrict__ vals, size_t F, con
     col = fcol[k];
 #pragma unroll
     for (int e = 0; e < 9; ++e)
-        a[e] = __builtin_nontemporal_load(vals + e * F + k);
+        a[e] = vals[e * F + k];
 }
 
 __device__ inline void load_block(const float* __restrict__ vals, size_t, const int* __restrict__, int k, float* a,
@@ -204,10 +204,10 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
                                  const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
                                  const GroupState* gs)
 {
-    const int t = tile_of(st);
-    if (t < 0)
+    int t;
+    RowTile tl;
+    if (!tile_for(st, t, tl))
         return;
-    const RowTile tl = st.tiles[t];
     const int stopAt = read_uniform(&gs[tl.group].stopAt);
     if (stopAt >= 0 && stopAt <= k)
         return;
@@ -314,10 +314,10 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
                                    const double* __restrict__ pa, Partial* __restrict__ pbCur,
                                    const Partial* __restrict__ pbPrev, GroupState* gs, int* active)
 {
-    const int t = tile_of(vt);
-    if (t < 0)
+    int t;
+    RowTile tl;
+    if (!tile_for(vt, t, tl))
         return;
-    const RowTile tl = vt.tiles[t];
     const int g = tl.group;
     GroupState& s = gs[g];
     const int stopAt = read_uniform(&s.stopAt);
@@ -489,6 +489,90 @@ void launch_update(const DeviceView& v, int k, unsigned long long* stamp)
     hipLaunchKernelGGL((k_update<M>), dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, k, v.vec,
                        v.spmv, static_cast<const M*>(v.minv), v.nd, static_cast<const M*>(v.d[k & 1]), static_cast<const M*>(v.q), v.xs,
                        v.r, static_cast<M*>(v.z), v.pa, v.pb[k & 1], v.pb[(k + 1) & 1], v.gs, v.active, stamp);
+}
+
+// Builds the workgroup-to-tile maps of a compacted batched CG launch (see
+// TileView) for the vector tiling and the SpMV tiling: the tiles of the
+// bodies whose loop is still running form a list in increasing order, entry i
+// of which goes to workgroup (i % per) * 8 + i / per with per = ceil(n / 8),
+// the XCD-contiguous order of an uncompacted launch over n tiles; every other
+// workgroup of the full grid gets group -1. Every workgroup forms the
+// per-group list offsets in LDS (an exclusive scan over the groups, -1 for a
+// stopped body); thread j of workgroups [0, vecBlocks) then places vector
+// tile j and settles workgroup slot j of the vector map, the rest the same
+// for the SpMV tiling.
+__global__ __launch_bounds__(NT) void k_compact(int G, const GroupState* __restrict__ gs, TileView vt, TileView st,
+                                                int vecBlocks, RowTile* __restrict__ vecPerm,
+                                                RowTile* __restrict__ spmvPerm)
+{
+    __shared__ int offV[kCompactMaxGroups], offS[kCompactMaxGroups];
+    __shared__ int scanV[NT], scanS[NT];
+    __shared__ int carryV, carryS;
+    const int tid = static_cast<int>(threadIdx.x);
+    if (tid == 0)
+        carryV = carryS = 0;
+    __syncthreads();
+    for (int g0 = 0; g0 < G; g0 += NT)
+    {
+        const int g = g0 + tid;
+        const bool run = g < G && gs[g].stopAt < 0;
+        const int cv = run ? vt.groupCount[g] : 0;
+        const int cs = run ? st.groupCount[g] : 0;
+        scanV[tid] = cv;
+        scanS[tid] = cs;
+        __syncthreads();
+        for (int o = 1; o < NT; o <<= 1)  // inclusive Hillis-Steele scan
+        {
+            const int av = tid >= o ? scanV[tid - o] : 0;
+            const int as = tid >= o ? scanS[tid - o] : 0;
+            __syncthreads();
+            scanV[tid] += av;
+            scanS[tid] += as;
+            __syncthreads();
+        }
+        if (g < G)
+        {
+            offV[g] = run ? carryV + scanV[tid] - cv : -1;
+            offS[g] = run ? carryS + scanS[tid] - cs : -1;
+        }
+        __syncthreads();
+        if (tid == NT - 1)
+        {
+            carryV += scanV[NT - 1];
+            carryS += scanS[NT - 1];
+        }
+        __syncthreads();
+    }
+    const bool vec = static_cast<int>(blockIdx.x) < vecBlocks;
+    const TileView& tv = vec ? vt : st;
+    RowTile* perm = vec ? vecPerm : spmvPerm;
+    const int* off = vec ? offV : offS;
+    const int n = vec ? carryV : carryS;
+    const int per = (n + 7) >> 3;
+    const int j = (vec ? static_cast<int>(blockIdx.x) : static_cast<int>(blockIdx.x) - vecBlocks) * NT + tid;
+    if (j < tv.count)
+    {
+        const int g = tv.tiles[j].group;
+        const int o = off[g];
+        if (o >= 0)
+        {
+            const int i = o + j - tv.groupBegin[g];
+            RowTile tl = tv.tiles[j];
+            tl.pad = j;
+            perm[(i % per) * 8 + i / per] = tl;
+        }
+    }
+    if (j < 8 * tv.per && ((j >> 3) >= per || (j & 7) * per + (j >> 3) >= n))
+        perm[j] = RowTile{-1, 0, 0, -1};
+}
+
+void launch_compact(const DeviceView& v, RowTile* vecPerm, RowTile* spmvPerm)
+{
+    const int vb = (8 * v.vec.per + NT - 1) / NT;
+    const int sb = (8 * v.spmv.per + NT - 1) / NT;
+    if (v.G > 0 && v.G <= kCompactMaxGroups && vb + sb > 0)
+        hipLaunchKernelGGL(k_compact, dim3(static_cast<unsigned>(vb + sb)), dim3(NT), 0, v.stream, v.G, v.gs, v.vec,
+                           v.spmv, vb, vecPerm, spmvPerm);
 }
 
 int spmv_grid(const DeviceView& v) { return 8 * v.spmv.per; }

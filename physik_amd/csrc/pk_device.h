@@ -49,6 +49,14 @@ struct Partial
 // tile (b % 8) * per + b / 8 so that each of the 8 XCDs (workgroups are
 // dispatched round-robin over them) walks one contiguous eighth of the rows
 // and keeps its share of the gathered vectors in its own L2.
+//
+// A compacted view (perm != nullptr, CG launches of a batch) instead gives
+// each workgroup of the full grid its tile directly: perm[b] is the tile of
+// workgroup b (its id in `pad`) or has group -1. The compaction kernel fills
+// it with the tiles whose body is still iterating, in the same XCD-contiguous
+// order over that shorter list, so the bodies left at the end of a batched
+// solve are spread over all XCDs instead of the one or two that own their
+// rows, and the workgroups past the list return at once.
 struct TileView
 {
     const RowTile* tiles;
@@ -56,7 +64,14 @@ struct TileView
     const int* groupCount;  // tiles of each group
     int count;              // tiles
     int per;                // ceil(count / 8)
+    const RowTile* perm = nullptr;  // compacted: tile of each workgroup (8 * per entries)
 };
+
+// Largest group count the compaction handles (its per-group offsets live in
+// LDS); larger batches run uncompacted.
+constexpr int kCompactMaxGroups = 4096;
+// CG iterations between two compactions of a batched solve.
+constexpr int kCompactEvery = 4;
 
 // Value storage of the full BSR. FP64: 9 planes, value e of block k at
 // e * F + k (the lanes of a row read 8-byte words of one plane). FP32: one
@@ -80,6 +95,23 @@ __device__ inline int tile_of(const TileView& tv)
     const int b = static_cast<int>(blockIdx.x);
     const int t = (b & 7) * tv.per + (b >> 3);
     return t < tv.count ? t : -1;
+}
+
+// The tile (id t, rows tl) of this workgroup in a view that may be compacted;
+// false when the workgroup has none.
+__device__ inline bool tile_for(const TileView& tv, int& t, RowTile& tl)
+{
+    if (tv.perm != nullptr)
+    {
+        tl = tv.perm[blockIdx.x];
+        t = tl.pad;
+        return tl.group >= 0;
+    }
+    t = tile_of(tv);
+    if (t < 0)
+        return false;
+    tl = tv.tiles[t];
+    return true;
 }
 
 __device__ inline bool fin(double v) { return __builtin_isfinite(v); }
@@ -256,6 +288,9 @@ template <class M> void launch_update(const DeviceView& v, int k, unsigned long 
 // `launches` consecutive such blocks into one (start, end) pair each.
 int spmv_grid(const DeviceView& v);
 int update_grid(const DeviceView& v);
+// Writes the compacted workgroup-to-tile maps of the bodies still iterating
+// (see TileView), 8 * per entries for each of the two tilings.
+void launch_compact(const DeviceView& v, RowTile* vecPerm, RowTile* spmvPerm);
 void launch_stamp_reduce(hipStream_t stream, const unsigned long long* raw, int grid, int launches,
                          unsigned long long* span);
 template <class M> void launch_spmv_plain(const DeviceView& v, const double* x, double* y);

@@ -120,6 +120,9 @@ struct pk_system
     pk::DevBuf<int> astart, alist, nodeRow, nodeGroup, cpos;
     pk::DevBuf<char> vals, z, d0, d1, q;  // element type set by the precision
     pk::DevBuf<pk::Partial> pb0, pb1;
+    // batched solves: the tile of each CG workgroup, over the bodies still
+    // iterating (vector and SpMV tilings), rebuilt every few iterations
+    pk::DevBuf<pk::RowTile> vecPerm, spmvPerm;
     pk::DevBuf<pk::GroupState> gs;
     // profiling: per-workgroup (start, end) ticks of every CG launch, and
     // the per-launch spans reduced from them
@@ -428,6 +431,10 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
     // launch k runs the direction, SpMV and update of iteration k (the
     // direction kernel first decides the end of k-1); k = maxIt only takes
     // that decision
+    // a batch switches its CG launches to the compacted tile lists at the
+    // first compaction (every kCompactEvery iterations)
+    DeviceView cv = dv;
+    const bool compact = s->spmvPerm.p != nullptr;
     int k = 0, poll = 0;
     int chunk = std::max(4, lastIters - 4);
     while (k <= maxIt)
@@ -435,12 +442,18 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
         const int end = std::min(maxIt + 1, k + chunk);
         for (; k < end; ++k)
         {
-            launch_dir<M>(dv, k, stD ? stD + 2 * static_cast<size_t>(gU) * k : nullptr);
+            if (compact && k > 0 && k % kCompactEvery == 0)
+            {
+                launch_compact(dv, s->vecPerm.p, s->spmvPerm.p);
+                cv.vec.perm = s->vecPerm.p;
+                cv.spmv.perm = s->spmvPerm.p;
+            }
+            launch_dir<M>(cv, k, stD ? stD + 2 * static_cast<size_t>(gU) * k : nullptr);
             if (k < maxIt)
             {
-                launch_spmv<M>(dv, k, s->lanesPerRow, s->unroll,
+                launch_spmv<M>(cv, k, s->lanesPerRow, s->unroll,
                                stS ? stS + 2 * static_cast<size_t>(gS) * k : nullptr);
-                launch_update<M>(dv, k, stU ? stU + 2 * static_cast<size_t>(gU) * k : nullptr);
+                launch_update<M>(cv, k, stU ? stU + 2 * static_cast<size_t>(gU) * k : nullptr);
             }
         }
         chunk = 4;
@@ -546,6 +559,11 @@ static int setup_tiles(pk_system* s)
     if ((rc = s->pa.alloc(std::max(1, s->spmvTiles.count))) != PK_OK) return rc;
     if ((rc = s->pb0.alloc(std::max(1, s->vecTiles.count))) != PK_OK) return rc;
     if ((rc = s->pb1.alloc(std::max(1, s->vecTiles.count))) != PK_OK) return rc;
+    if (s->topo.G > 1 && s->topo.G <= kCompactMaxGroups)
+    {
+        if ((rc = s->vecPerm.alloc(std::max(8, 8 * s->vecTiles.per))) != PK_OK) return rc;
+        if ((rc = s->spmvPerm.alloc(std::max(8, 8 * s->spmvTiles.per))) != PK_OK) return rc;
+    }
     PK_HIP(hipStreamSynchronize(s->stream));
     return PK_OK;
 }
