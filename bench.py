@@ -137,6 +137,7 @@ def prepare(name, prec_name, args):
         s.set_state(w.x0, w.v0)
         shape = s.get_tune() if args.no_autotune else s.autotune(w.dt)
     s.set_state(w.x0, w.v0)
+    s.set_graphs(args.cg_launch == "graphs")
     return w, s, dict(zip(("lanes_per_row", "spmv_rows", "vector_rows", "unroll"), shape))
 
 
@@ -162,6 +163,7 @@ def run_plain(name, prec_name, steps, warmup, args):
         "nnzb_full": s.nnzb_full, "nnzb_upper": s.nnzb_upper,
         "cg_iterations_last": int(its.max()),
         "cg_launch_shape": shape,
+        "cg_launch": args.cg_launch,
         "roofline": roofline(s, name, prec_name),
         "kernels_ms": kernels_ms(s),
     }
@@ -206,6 +208,7 @@ def run_fwd_adj(name, prec_name, steps, warmup, args):
         "cg_iterations_last_forward": int(s.results()[0].max()),
         "adjoint_cg_iterations": {"mean": float(np.mean(adj_its)), "max": int(np.max(adj_its))},
         "cg_launch_shape": shape,
+        "cg_launch": args.cg_launch,
         "roofline": roofline(s, name, prec_name),
         "kernels_ms": kernels_ms(s),
     }
@@ -321,6 +324,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default CG launch shape")
     ap.add_argument("--shape", default=None, help="CG launch shape lanes,spmv_rows,vector_rows,unroll (no autotune)")
+    ap.add_argument("--cg-launch", default="kernels", choices=["kernels", "graphs"],
+                    help="issue each CG launch from the host, or replay captured 4-iteration HIP graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the batched and single-body "

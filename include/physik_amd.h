@@ -134,6 +134,13 @@ int pk_system_step(pk_system* sys, double dt, int substeps, int frames);
 int pk_system_group_results(pk_system* sys, int* iterations, double* residual_norm, int* converged,
                             int* status);
 
+/* CG launch mode (no reference counterpart). enable != 0: every solve replays
+ * its CG four iterations at a time from HIP graphs captured on first use
+ * (direction, SpMV and update launches, plus a batch's tile compaction),
+ * instead of issuing each launch from the host; results are identical either
+ * way. A capture is redone after pk_system_tune or a profiling change. */
+int pk_system_set_graphs(pk_system* sys, int enable);
+
 /* Kernel timing on the system's stream: enable, reset, read. The CG kernels
  * (SpMV, update) are timed from the wall clock read inside the kernel (first
  * workgroup start to last workgroup end, the span a kernel trace reports);

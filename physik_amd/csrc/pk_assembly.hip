@@ -23,11 +23,14 @@
 namespace pk
 {
 
-__global__ void k_reset(GroupState* gs, int G, int* active, double tol, int maxIt)
+__global__ void k_reset(GroupState* gs, int G, int* active, int* kbase, double tol, int maxIt)
 {
     __shared__ int cnt;
     if (threadIdx.x == 0)
+    {
         cnt = 0;
+        *kbase = 0;
+    }
     __syncthreads();
     for (int g = threadIdx.x; g < G; g += blockDim.x)
     {
@@ -570,7 +573,7 @@ static inline unsigned grid_of(const TileView& tv) { return 8u * static_cast<uns
 
 void launch_reset(const DeviceView& v, double tol, int maxIt)
 {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(NT), 0, v.stream, v.gs, v.G, v.active, tol, maxIt);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(NT), 0, v.stream, v.gs, v.G, v.active, v.kbase, tol, maxIt);
 }
 
 void launch_element(const DeviceView& v)

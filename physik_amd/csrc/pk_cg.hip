@@ -67,6 +67,26 @@ __device__ inline void stamp_end(unsigned long long* stamp)
         stamp[2 * blockIdx.x + 1] = static_cast<unsigned long long>(wall_clock64());
 }
 
+// Iteration index and stamp slots of a CG launch. Issued on its own
+// (kb == nullptr): k0 itself, and the host has already pointed `stamps` at
+// this launch's slots. Replayed from a captured chunk: *kb + k0 (the chunk's
+// base lives on the device and advances after every replay), and the slots
+// of launch k are found from the buffer base. The first form adds no load to
+// the start of a kernel, which matters for the short vector kernels.
+struct LaunchIndex
+{
+    int k;
+    unsigned long long* stamp;
+};
+
+__device__ inline LaunchIndex launch_index(const int* kb, int k0, unsigned long long* stamps)
+{
+    if (kb == nullptr)
+        return LaunchIndex{k0, stamps};
+    const int k = __builtin_amdgcn_readfirstlane(*kb + k0);
+    return LaunchIndex{k, stamps ? stamps + 2ull * gridDim.x * static_cast<unsigned long long>(k) : nullptr};
+}
+
 template <class M>
 __device__ inline M direction(M z, M dprev, double beta)
 {
@@ -167,10 +187,14 @@ __device__ inline void dir_body(int k, const TileView& vt, const M* __restrict__
 }
 
 template <class M>
-__global__ __launch_bounds__(NT) void k_dir(int k, TileView vt, const M* __restrict__ z, const M* __restrict__ dprev,
-                                            M* __restrict__ dcur, const Partial* __restrict__ pbPrev, GroupState* gs,
-                                            int* active, unsigned long long* stamp)
+__global__ __launch_bounds__(NT) void k_dir(const int* kb, int k0, TileView vt, const M* __restrict__ z,
+                                            const M* __restrict__ dprev, M* __restrict__ dcur,
+                                            const Partial* __restrict__ pbPrev, GroupState* gs, int* active,
+                                            unsigned long long* stamps)
 {
+    const LaunchIndex li = launch_index(kb, k0, stamps);
+    const int k = li.k;
+    unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
     dir_body<M>(k, vt, z, dprev, dcur, pbPrev, gs, active);
     stamp_end(stamp);
@@ -296,12 +320,15 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
 }
 
 template <class M, int LPR, int U>
-__global__ __launch_bounds__(NT) void k_spmv(int k, TileView st, const int* __restrict__ fstart,
+__global__ __launch_bounds__(NT) void k_spmv(const int* kb, int k0, TileView st, const int* __restrict__ fstart,
                                              const int* __restrict__ fcol, const M* __restrict__ vals,
                                              long long nnzf, const M* __restrict__ d, M* __restrict__ q,
                                              double* __restrict__ pa, const GroupState* gs,
-                                             unsigned long long* stamp)
+                                             unsigned long long* stamps)
 {
+    const LaunchIndex li = launch_index(kb, k0, stamps);
+    const int k = li.k;
+    unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
     spmv_body<M, LPR, U>(k, st, fstart, fcol, vals, nnzf, d, q, pa, gs);
     stamp_end(stamp);
@@ -374,13 +401,17 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
 }
 
 template <class M>
-__global__ __launch_bounds__(NT) void k_update(int k, TileView vt, TileView st, const M* __restrict__ minv,
-                                               int nd, const M* __restrict__ dcur, const M* __restrict__ q,
-                                               double* __restrict__ xs, double* __restrict__ rv, M* __restrict__ zv,
+__global__ __launch_bounds__(NT) void k_update(const int* kb, int k0, TileView vt, TileView st,
+                                               const M* __restrict__ minv, int nd, const M* __restrict__ dcur,
+                                               const M* __restrict__ q, double* __restrict__ xs,
+                                               double* __restrict__ rv, M* __restrict__ zv,
                                                const double* __restrict__ pa, Partial* __restrict__ pbCur,
                                                const Partial* __restrict__ pbPrev, GroupState* gs, int* active,
-                                               unsigned long long* stamp)
+                                               unsigned long long* stamps)
 {
+    const LaunchIndex li = launch_index(kb, k0, stamps);
+    const int k = li.k;
+    unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
     update_body<M>(k, vt, st, minv, nd, dcur, q, xs, rv, zv, pa, pbCur, pbPrev, gs, active);
     stamp_end(stamp);
@@ -444,33 +475,35 @@ __global__ __launch_bounds__(NT) void k_spmv_plain(int nd, const int* __restrict
         y[3 * row + i] = acc[i];
 }
 
+// The iteration of a launch is k0 (kb == nullptr) or *kb + k0 with *kb even,
+// so k0 alone picks the ping-pong buffers.
 template <class M, int LPR, int U>
-static void launch_spmv_shape(const DeviceView& v, int k, unsigned long long* stamp)
+static void launch_spmv_shape(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp)
 {
-    hipLaunchKernelGGL((k_spmv<M, LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), 0, v.stream, k,
-                       v.spmv, v.fstart, v.fcol, static_cast<const M*>(v.vals), v.nnzf,
-                       static_cast<const M*>(v.d[k & 1]), static_cast<M*>(v.q), v.pa, v.gs, stamp);
+    hipLaunchKernelGGL((k_spmv<M, LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), 0, v.stream, kb,
+                       k0, v.spmv, v.fstart, v.fcol, static_cast<const M*>(v.vals), v.nnzf,
+                       static_cast<const M*>(v.d[k0 & 1]), static_cast<M*>(v.q), v.pa, v.gs, stamp);
 }
 
 template <class M>
-void launch_dir(const DeviceView& v, int k, unsigned long long* stamp)
+void launch_dir(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp)
 {
     if (v.vec.count == 0)
         return;
-    hipLaunchKernelGGL((k_dir<M>), dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, k, v.vec,
-                       static_cast<const M*>(v.z), static_cast<const M*>(v.d[(k + 1) & 1]), static_cast<M*>(v.d[k & 1]),
-                       v.pb[(k + 1) & 1], v.gs, v.active, stamp);
+    hipLaunchKernelGGL((k_dir<M>), dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, kb, k0,
+                       v.vec, static_cast<const M*>(v.z), static_cast<const M*>(v.d[(k0 + 1) & 1]),
+                       static_cast<M*>(v.d[k0 & 1]), v.pb[(k0 + 1) & 1], v.gs, v.active, stamp);
 }
 
 template <class M>
-void launch_spmv(const DeviceView& v, int k, int lanesPerRow, int unroll, unsigned long long* stamp)
+void launch_spmv(const DeviceView& v, const int* kb, int k0, int lanesPerRow, int unroll, unsigned long long* stamp)
 {
     if (v.spmv.count == 0)
         return;
 #define PK_SHAPE(L, U)                        \
     if (lanesPerRow == L && unroll == U)      \
     {                                         \
-        launch_spmv_shape<M, L, U>(v, k, stamp); \
+        launch_spmv_shape<M, L, U>(v, kb, k0, stamp); \
         return;                               \
     }
     PK_SHAPE(4, 1) PK_SHAPE(4, 2) PK_SHAPE(4, 4)
@@ -478,17 +511,29 @@ void launch_spmv(const DeviceView& v, int k, int lanesPerRow, int unroll, unsign
     PK_SHAPE(16, 1) PK_SHAPE(16, 2) PK_SHAPE(16, 4)
     PK_SHAPE(32, 1) PK_SHAPE(32, 2)
 #undef PK_SHAPE
-    launch_spmv_shape<M, 16, 1>(v, k, stamp);
+    launch_spmv_shape<M, 16, 1>(v, kb, k0, stamp);
 }
 
 template <class M>
-void launch_update(const DeviceView& v, int k, unsigned long long* stamp)
+void launch_update(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp)
 {
     if (v.vec.count == 0)
         return;
-    hipLaunchKernelGGL((k_update<M>), dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, k, v.vec,
-                       v.spmv, static_cast<const M*>(v.minv), v.nd, static_cast<const M*>(v.d[k & 1]), static_cast<const M*>(v.q), v.xs,
-                       v.r, static_cast<M*>(v.z), v.pa, v.pb[k & 1], v.pb[(k + 1) & 1], v.gs, v.active, stamp);
+    hipLaunchKernelGGL((k_update<M>), dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, kb, k0,
+                       v.vec, v.spmv, static_cast<const M*>(v.minv), v.nd, static_cast<const M*>(v.d[k0 & 1]),
+                       static_cast<const M*>(v.q), v.xs, v.r, static_cast<M*>(v.z), v.pa, v.pb[k0 & 1],
+                       v.pb[(k0 + 1) & 1], v.gs, v.active, stamp);
+}
+
+__global__ void k_advance(int* kb, int by)
+{
+    if (threadIdx.x == 0)
+        *kb += by;
+}
+
+void launch_advance(const DeviceView& v, int* kb, int by)
+{
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, v.stream, kb, by);
 }
 
 // Builds the workgroup-to-tile maps of a compacted batched CG launch (see
@@ -594,12 +639,12 @@ void launch_spmv_plain(const DeviceView& v, const double* x, double* y)
                            v.fcol, static_cast<const M*>(v.vals), v.nnzf, x, y);
 }
 
-template void launch_dir<double>(const DeviceView&, int, unsigned long long*);
-template void launch_dir<float>(const DeviceView&, int, unsigned long long*);
-template void launch_spmv<double>(const DeviceView&, int, int, int, unsigned long long*);
-template void launch_spmv<float>(const DeviceView&, int, int, int, unsigned long long*);
-template void launch_update<double>(const DeviceView&, int, unsigned long long*);
-template void launch_update<float>(const DeviceView&, int, unsigned long long*);
+template void launch_dir<double>(const DeviceView&, const int*, int, unsigned long long*);
+template void launch_dir<float>(const DeviceView&, const int*, int, unsigned long long*);
+template void launch_spmv<double>(const DeviceView&, const int*, int, int, int, unsigned long long*);
+template void launch_spmv<float>(const DeviceView&, const int*, int, int, int, unsigned long long*);
+template void launch_update<double>(const DeviceView&, const int*, int, unsigned long long*);
+template void launch_update<float>(const DeviceView&, const int*, int, unsigned long long*);
 __global__ __launch_bounds__(NT) void k_record_columns(long long F, const int* __restrict__ fcol,
                                                        float* __restrict__ vals)
 {

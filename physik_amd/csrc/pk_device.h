@@ -260,6 +260,7 @@ struct DeviceView
     void* q;      // M
     GroupState* gs;
     int* active;
+    int* kbase;        // base iteration of a replayed CG chunk (zeroed by the reset)
     // reductions
     TileView vec, spmv;
     double* pa;        // per SpMV tile: d.q
@@ -279,13 +280,18 @@ template <class M> void launch_assemble(const DeviceView& v, double dt);
 template <class M> void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt);
 void launch_integrate(const DeviceView& v, double dt, int commit);
 void launch_explicit(const DeviceView& v, double gx, double gy, double gz, double dt);
-template <class M> void launch_dir(const DeviceView& v, int k, unsigned long long* stamp);
+template <class M> void launch_dir(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp);
 template <class M>
-void launch_spmv(const DeviceView& v, int k, int lanesPerRow, int unroll, unsigned long long* stamp);
-template <class M> void launch_update(const DeviceView& v, int k, unsigned long long* stamp);
-// Profiling stamps: a CG launch given a stamp pointer writes (start, end)
-// wall-clock ticks of each of its grid workgroups there; the reduce turns
-// `launches` consecutive such blocks into one (start, end) pair each.
+void launch_spmv(const DeviceView& v, const int* kb, int k0, int lanesPerRow, int unroll,
+                 unsigned long long* stamp);
+template <class M> void launch_update(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp);
+// *kb += by (the base of a replayed CG chunk).
+void launch_advance(const DeviceView& v, int* kb, int by);
+// Profiling stamps: a CG launch writes (start, end) wall-clock ticks of each
+// of its grid workgroups into 2 * grid slots: the slots passed (kb ==
+// nullptr), or block k of the buffer passed (replayed chunks, iteration k);
+// the reduce turns `launches` consecutive such blocks into one (start, end)
+// pair each.
 int spmv_grid(const DeviceView& v);
 int update_grid(const DeviceView& v);
 // Writes the compacted workgroup-to-tile maps of the bodies still iterating

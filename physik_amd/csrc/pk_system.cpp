@@ -124,6 +124,7 @@ struct pk_system
     // iterating (vector and SpMV tilings), rebuilt every few iterations
     pk::DevBuf<pk::RowTile> vecPerm, spmvPerm;
     pk::DevBuf<pk::GroupState> gs;
+    pk::DevBuf<int> kbase;
     // profiling: per-workgroup (start, end) ticks of every CG launch, and
     // the per-launch spans reduced from them
     pk::DevBuf<unsigned long long> rawSpmv, rawUpd, rawDir, stampSpmv, stampUpd, stampDir;
@@ -152,10 +153,35 @@ struct pk_system
     std::vector<long long> groupBlocks, groupRows;  // full BSR blocks and dynamic rows per group
     pk::Profile prof;
 
+    // CG chunks replayed from captured HIP graphs: the first chunk of a solve
+    // (full tiles) and every later one (compaction first, for a batch). A
+    // capture bakes in buffer addresses and tile shapes: `epoch` changes when
+    // they do, and the stamp buffers in use are part of the key.
+    bool useGraphs = false;
+    int epoch = 0;
+    struct ChunkGraphs
+    {
+        hipGraphExec_t first = nullptr, next = nullptr;
+        int epoch = -1;
+        const void* stamps[3] = {nullptr, nullptr, nullptr};
+    } chunks;
+
+    void drop_graphs()
+    {
+        for (hipGraphExec_t* g : {&chunks.first, &chunks.next})
+            if (*g)
+            {
+                (void)hipGraphExecDestroy(*g);
+                *g = nullptr;
+            }
+        chunks.epoch = -1;
+    }
+
     ~pk_system()
     {
         if (stream)
             (void)hipStreamSynchronize(stream);
+        drop_graphs();
         for (auto e : prof.pool)
             (void)hipEventDestroy(e);
         for (auto e : pollEv)
@@ -212,6 +238,7 @@ struct pk_system
         dv.q = q.p;
         dv.gs = gs.p;
         dv.active = active.p;
+        dv.kbase = kbase.p;
         dv.vec = vecTiles.view();
         dv.spmv = spmvTiles.view();
         dv.pa = pa.p;
@@ -399,64 +426,135 @@ static int run_assembly(pk_system* s, double dt, double tol, int maxIt, bool wit
     return PK_OK;
 }
 
+// Captures one CG chunk: kChunk iterations (direction, SpMV, update each) at
+// iterations *kbase + 0 .. kChunk-1, then *kbase += kChunk; a batch's later
+// chunks first rebuild the compacted tile maps.
+constexpr int kChunk = kCompactEvery;
+
+template <class M>
+static int capture_chunk(pk_system* s, const DeviceView& dv, bool compacted, unsigned long long* stS,
+                         unsigned long long* stU, unsigned long long* stD, hipGraphExec_t& exec)
+{
+    PK_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    DeviceView cv = dv;
+    if (compacted)
+    {
+        launch_compact(dv, s->vecPerm.p, s->spmvPerm.p);
+        cv.vec.perm = s->vecPerm.p;
+        cv.spmv.perm = s->spmvPerm.p;
+    }
+    for (int o = 0; o < kChunk; ++o)
+    {
+        launch_dir<M>(cv, dv.kbase, o, stD);
+        launch_spmv<M>(cv, dv.kbase, o, s->lanesPerRow, s->unroll, stS);
+        launch_update<M>(cv, dv.kbase, o, stU);
+    }
+    launch_advance(dv, dv.kbase, kChunk);
+    const hipError_t launchErr = hipGetLastError();
+    hipGraph_t g = nullptr;
+    const hipError_t endErr = hipStreamEndCapture(s->stream, &g);
+    if (launchErr != hipSuccess || endErr != hipSuccess)
+    {
+        if (g)
+            (void)hipGraphDestroy(g);
+        set_error(std::string("CG chunk capture failed: ") +
+                  hipGetErrorString(launchErr != hipSuccess ? launchErr : endErr));
+        return PK_ERR_HIP;
+    }
+    const hipError_t instErr = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    PK_HIP(instErr);
+    return PK_OK;
+}
+
 // The CG launches up to maxIt; lastIters (the previous solve's count of the
-// same kind) sizes the first chunk.
+// same kind) sizes the first run of launches before the host starts polling.
+// Launched one by one, or (useGraphs) replayed kChunk iterations at a time
+// from captured graphs: the same kernels in the same order either way, so the
+// results are identical.
 template <class M>
 static int run_cg(pk_system* s, int maxItArg, int lastIters)
 {
     const DeviceView dv = s->view();
     const int maxIt = std::max(0, maxItArg);
     const int gS = spmv_grid(dv), gU = update_grid(dv);
+    // launches k = 0 .. maxIt, rounded up to whole chunks
+    const int launches = (maxIt + kChunk) / kChunk * kChunk;
     unsigned long long* stS = nullptr;
     unsigned long long* stU = nullptr;
     unsigned long long* stD = nullptr;
     if (s->prof.on)
     {
-        const size_t launches = static_cast<size_t>(maxIt + 1);
-        if (s->rawSpmv.n < launches * 2 * gS || s->rawUpd.n < launches * 2 * gU || s->rawDir.n < launches * 2 * gU ||
-            s->stampSpmv.n < 2 * launches)
+        const size_t n = static_cast<size_t>(launches);
+        if (s->rawSpmv.n < n * 2 * gS || s->rawUpd.n < n * 2 * gU || s->rawDir.n < n * 2 * gU ||
+            s->stampSpmv.n < 2 * n)
         {
             int rc;
-            if ((rc = s->rawSpmv.alloc(launches * 2 * gS)) != PK_OK) return rc;
-            if ((rc = s->rawUpd.alloc(launches * 2 * gU)) != PK_OK) return rc;
-            if ((rc = s->rawDir.alloc(launches * 2 * gU)) != PK_OK) return rc;
-            if ((rc = s->stampSpmv.alloc(2 * launches)) != PK_OK) return rc;
-            if ((rc = s->stampUpd.alloc(2 * launches)) != PK_OK) return rc;
-            if ((rc = s->stampDir.alloc(2 * launches)) != PK_OK) return rc;
+            if ((rc = s->rawSpmv.alloc(n * 2 * gS)) != PK_OK) return rc;
+            if ((rc = s->rawUpd.alloc(n * 2 * gU)) != PK_OK) return rc;
+            if ((rc = s->rawDir.alloc(n * 2 * gU)) != PK_OK) return rc;
+            if ((rc = s->stampSpmv.alloc(2 * n)) != PK_OK) return rc;
+            if ((rc = s->stampUpd.alloc(2 * n)) != PK_OK) return rc;
+            if ((rc = s->stampDir.alloc(2 * n)) != PK_OK) return rc;
         }
         stS = s->rawSpmv.p;
         stU = s->rawUpd.p;
         stD = s->rawDir.p;
     }
+    const bool compact = s->spmvPerm.p != nullptr;
+    if (s->useGraphs)
+    {
+        auto& cg = s->chunks;
+        if (cg.epoch != s->epoch || cg.stamps[0] != stS || cg.stamps[1] != stU || cg.stamps[2] != stD)
+        {
+            s->drop_graphs();
+            int rc;
+            if ((rc = capture_chunk<M>(s, dv, false, stS, stU, stD, cg.first)) != PK_OK)
+                return rc;
+            if (compact && (rc = capture_chunk<M>(s, dv, true, stS, stU, stD, cg.next)) != PK_OK)
+                return rc;
+            cg.epoch = s->epoch;
+            cg.stamps[0] = stS;
+            cg.stamps[1] = stU;
+            cg.stamps[2] = stD;
+        }
+    }
     // launch k runs the direction, SpMV and update of iteration k (the
     // direction kernel first decides the end of k-1); k = maxIt only takes
-    // that decision
-    // a batch switches its CG launches to the compacted tile lists at the
-    // first compaction (every kCompactEvery iterations)
+    // that decision, and launches past it find every body stopped; a batch
+    // switches to the compacted tile maps at the first compaction (every
+    // kChunk iterations)
     DeviceView cv = dv;
-    const bool compact = s->spmvPerm.p != nullptr;
     int k = 0, poll = 0;
-    int chunk = std::max(4, lastIters - 4);
+    int chunk = std::max(kChunk, lastIters - 4);
     while (k <= maxIt)
     {
         const int end = std::min(maxIt + 1, k + chunk);
-        for (; k < end; ++k)
+        if (s->useGraphs)
         {
-            if (compact && k > 0 && k % kCompactEvery == 0)
+            for (; k < end; k += kChunk)
+                PK_HIP(hipGraphLaunch(k == 0 || !compact ? s->chunks.first : s->chunks.next, s->stream));
+        }
+        else
+        {
+            for (; k < end; ++k)
             {
-                launch_compact(dv, s->vecPerm.p, s->spmvPerm.p);
-                cv.vec.perm = s->vecPerm.p;
-                cv.spmv.perm = s->spmvPerm.p;
-            }
-            launch_dir<M>(cv, k, stD ? stD + 2 * static_cast<size_t>(gU) * k : nullptr);
-            if (k < maxIt)
-            {
-                launch_spmv<M>(cv, k, s->lanesPerRow, s->unroll,
-                               stS ? stS + 2 * static_cast<size_t>(gS) * k : nullptr);
-                launch_update<M>(cv, k, stU ? stU + 2 * static_cast<size_t>(gU) * k : nullptr);
+                if (compact && k > 0 && k % kCompactEvery == 0)
+                {
+                    launch_compact(dv, s->vecPerm.p, s->spmvPerm.p);
+                    cv.vec.perm = s->vecPerm.p;
+                    cv.spmv.perm = s->spmvPerm.p;
+                }
+                launch_dir<M>(cv, nullptr, k, stD ? stD + 2 * static_cast<size_t>(gU) * k : nullptr);
+                if (k < maxIt)
+                {
+                    launch_spmv<M>(cv, nullptr, k, s->lanesPerRow, s->unroll,
+                                   stS ? stS + 2 * static_cast<size_t>(gS) * k : nullptr);
+                    launch_update<M>(cv, nullptr, k, stU ? stU + 2 * static_cast<size_t>(gU) * k : nullptr);
+                }
             }
         }
-        chunk = 4;
+        chunk = kChunk;
         const int slot = poll & 1;
         PK_HIP(hipMemcpyAsync(s->pinned + slot, s->active.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
         PK_HIP(hipEventRecord(s->pollEv[slot], s->stream));
@@ -470,9 +568,11 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
     }
     if (stS)
     {
+        // graph chunks ran the SpMV and update of every launch they covered
+        const int nS = s->useGraphs ? k : std::min(k, maxIt);
         launch_stamp_reduce(s->stream, stD, gU, k, s->stampDir.p);
-        launch_stamp_reduce(s->stream, stS, gS, std::min(k, maxIt), s->stampSpmv.p);
-        launch_stamp_reduce(s->stream, stU, gU, std::min(k, maxIt), s->stampUpd.p);
+        launch_stamp_reduce(s->stream, stS, gS, nS, s->stampSpmv.p);
+        launch_stamp_reduce(s->stream, stU, gU, nS, s->stampUpd.p);
     }
     PK_HIP(hipGetLastError());
     return PK_OK;
@@ -548,12 +648,15 @@ static int alloc_work(pk_system* s)
     if ((rc = s->xs.alloc(nd3)) != PK_OK) return rc;
     if ((rc = s->r.alloc(nd3)) != PK_OK) return rc;
     if ((rc = s->active.alloc(1)) != PK_OK) return rc;
+    if ((rc = s->kbase.alloc(1)) != PK_OK) return rc;
+    ++s->epoch;
     return PK_OK;
 }
 
 static int setup_tiles(pk_system* s)
 {
     int rc;
+    ++s->epoch;
     if ((rc = build_tiles(s, s->vecTiles, s->vecRows)) != PK_OK) return rc;
     if ((rc = build_tiles(s, s->spmvTiles, s->spmvRows)) != PK_OK) return rc;
     if ((rc = s->pa.alloc(std::max(1, s->spmvTiles.count))) != PK_OK) return rc;
@@ -1075,6 +1178,16 @@ int pk_system_profile(pk_system* s, int enable)
     s->prof.spmvBlocks = s->prof.spmvRows = s->prof.spmvBodyLaunches = 0;
     s->prof.recs.clear();
     s->prof.used = 0;
+    return PK_OK;
+}
+
+int pk_system_set_graphs(pk_system* s, int enable)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    s->useGraphs = enable != 0;
+    if (!s->useGraphs)
+        s->drop_graphs();
     return PK_OK;
 }
 

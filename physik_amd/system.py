@@ -150,6 +150,11 @@ class System:
         N.check(self.lib.pk_system_kernel_time(self.h, kernel, C.byref(n), C.byref(ms)), "pk_system_kernel_time")
         return n.value, ms.value
 
+    def set_graphs(self, enable=True):
+        """Replay each solve's CG from captured HIP graphs, four iterations
+        per replay (identical results to launching each kernel)."""
+        N.check(self.lib.pk_system_set_graphs(self.h, int(enable)), "pk_system_set_graphs")
+
     def spmv_work(self):
         """(full BSR blocks, dynamic rows, body-launches) the profiled SpMV
         launches multiplied; stopped bodies' rows are not counted."""
