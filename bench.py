@@ -6,7 +6,9 @@
                     [--no-autotune | --shape L,R,V,U]
 
 Default: the batched differentiable workload (128 bodies, 3.2 M tets, FP32
-storage with FP64 dots and residual), forward + adjoint. A step is one
+storage of the matrix's upper triangle with FP64 dots and residual), forward
++ adjoint; beside it the same run with the full matrix stored (identical
+results, the full-matrix SpMV). A step is one
 implicit substep forward (element corotation, assembly, block-Jacobi PCG to
 rtol 1e-6, integration) and its reverse (checkpoint-segment replay, adjoint
 PCG to rtol 1e-7, corotational adjoint with dR/dF) over every body; K steps
@@ -56,7 +58,7 @@ DESCRIPTION = {
                 "one random face fixed, v0 ~ N(0, 0.5 m/s), dt=1/60, rtol 1e-6, one CG per body",
 }
 DIFFERENTIABLE = ("batch128", "diff32")
-DEFAULT_PRECISION = {"batch128": "fp32"}
+DEFAULT_PRECISION = {"batch128": "fp32u"}
 DEFAULT_STEPS = {"batch128": 40, "diff32": 50, "twist64": 50, "beam10": 100, "batch256": 100}
 
 
@@ -66,13 +68,20 @@ def workload(name):
             "batch128": W.batch128}[name]()
 
 
-def spmv_bytes(blocks, rows, body_launches, s):
-    """Algorithmic bytes of SpMV work q = A d: every multiplied 3x3 block once
-    (9 values of s bytes + int32 column), per row its int32 row pointer, one
-    d entry read and one q entry written (3 values each), and the closing row
-    pointer of each body in each launch. One launch over a whole system is
-    spmv_bytes(nnzb_full, nd, bodies, s)."""
-    return blocks * (9 * s + 4) + rows * (4 + 2 * 3 * s) + 4 * body_launches
+def spmv_bytes(blocks, rows, body_launches, s, upper_blocks=None):
+    """Algorithmic bytes of SpMV work q = A d.
+
+    Full storage: every multiplied 3x3 block once (9 values of s bytes +
+    int32 column), per row its int32 row pointer, one d entry read and one q
+    entry written (3 values each), and the closing row pointer of each body in
+    each launch. Upper-only storage (upper_blocks = the stored blocks among
+    them): every stored block once (its record also serves the transposed
+    product of its mirror row), an 8-byte (slot, column) entry per lower
+    block, per row an 8-byte pair of row pointers and d, q. One launch over a
+    whole system is spmv_bytes(nnzb_full, nd, bodies, s[, nnzb_upper])."""
+    if upper_blocks is None:
+        return blocks * (9 * s + 4) + rows * (4 + 2 * 3 * s) + 4 * body_launches
+    return upper_blocks * (9 * s + 4) + (blocks - upper_blocks) * 8 + rows * (8 + 2 * 3 * s) + 8 * body_launches
 
 
 def measured_traffic(workload_name, precision):
@@ -96,9 +105,12 @@ def roofline(s, name, prec_name):
     from physik_amd import native as N
     n_spmv, ms_spmv = s.kernel_time(N.K_SPMV)
     esize = 8 if s.precision == N.PK_FP64 else 4
-    blocks, rows, body_launches = s.spmv_work()
-    total = spmv_bytes(blocks, rows, body_launches, esize)
-    full = spmv_bytes(s.nnzb_full, s.nd, s.groups, esize)
+    upper = s.precision == N.PK_FP32_UPPER
+    blocks, ublocks, rows, body_launches = s.spmv_work()
+    total = spmv_bytes(blocks, rows, body_launches, esize, ublocks if upper else None)
+    full = spmv_bytes(s.nnzb_full, s.nd, s.groups, esize, s.nnzb_upper if upper else None)
+    # the same work in full storage, for comparison across formats
+    full_format = spmv_bytes(blocks, rows, body_launches, esize)
     achieved = total / (ms_spmv * 1e-3) / 1e9 if n_spmv and ms_spmv > 0 else 0.0
     measured = measured_traffic(name, prec_name)
     return {
@@ -110,7 +122,9 @@ def roofline(s, name, prec_name):
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": None if measured is None else measured[0],
         "traffic_vs_algorithmic": None if measured is None else measured[0] / measured[1],
+        "storage": "upper triangle" if upper else "full",
         "bytes_per_launch": total / max(n_spmv, 1),
+        "full_storage_bytes_per_launch": full_format / max(n_spmv, 1),
         "bytes_per_full_launch": full,
         "active_fraction": total / (full * max(n_spmv, 1)),
         "avg_launch_us": ms_spmv / max(n_spmv, 1) * 1e3,
@@ -127,7 +141,7 @@ def prepare(name, prec_name, args):
     from physik_amd import native as N
     from physik_amd import workloads as W
     w = workload(name)
-    prec = N.PK_FP64 if prec_name == "fp64" else N.PK_FP32_STORAGE
+    prec = {"fp64": N.PK_FP64, "fp32": N.PK_FP32_STORAGE, "fp32u": N.PK_FP32_UPPER}[prec_name]
     s = W.make_system(w, prec)
     if args.shape:
         s.tune(*[int(a) for a in args.shape.split(",")])
@@ -203,7 +217,7 @@ def run_fwd_adj(name, prec_name, steps, warmup, args):
         "backward_ms": (t3 - t2) * 1e3,
         "precision": prec_name,
         "tets": w.n_tets, "nodes": w.n_nodes, "bodies": w.n_bodies, "dynamic_rows": s.nd,
-        "nnzb_full": s.nnzb_full, "checkpoint_every": every,
+        "nnzb_full": s.nnzb_full, "nnzb_upper": s.nnzb_upper, "checkpoint_every": every,
         "loss": loss, "grad_log_young_norm": float(np.linalg.norm(gE)), "grad_v0_norm": float(np.linalg.norm(gv)),
         "cg_iterations_last_forward": int(s.results()[0].max()),
         "adjoint_cg_iterations": {"mean": float(np.mean(adj_its)), "max": int(np.max(adj_its))},
@@ -318,8 +332,11 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batch128", choices=list(DESCRIPTION))
-    ap.add_argument("--precision", default=None, choices=["fp64", "fp32"])
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "fp32u"],
+                    help="fp32: FP32 storage of the full matrix; fp32u: of its upper triangle only")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain single-body step beside the headline")
+    ap.add_argument("--no-full-storage", action="store_true",
+                    help="skip the full-matrix FP32 run beside an upper-triangle (fp32u) headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default CG launch shape")
@@ -348,7 +365,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec_name == "fp64" else "f32 storage, f64 dots",
+        "dtype": {"fp64": "f64", "fp32": "f32 storage, f64 dots",
+                  "fp32u": "f32 storage (upper triangle), f64 dots"}[prec_name],
         "data": "synthetic (seeded meshes, no checkpoint files)",
         "config": {"workload": head["workload"], "mode": "forward+adjoint" if diff else "forward",
                    "parallelism": "single GPU",
@@ -357,6 +375,12 @@ def main():
         "roofline": head["roofline"],
         "kernels_ms": head["kernels_ms"],
     }
+    if diff and prec_name == "fp32u" and not args.no_full_storage:
+        # the same run with the full matrix stored (identical results): its
+        # SpMV moves the whole matrix, the headline's reads the upper half
+        full, _ = run_fwd_adj(name, "fp32", steps, args.warmup, args)
+        line["full_storage_step"] = {k: full[k] for k in ("value", "ms_per_step", "precision", "cg_launch_shape",
+                                                          "roofline", "kernels_ms")}
     if not args.no_plain and name != "twist64":
         plain, _ = run_plain("twist64", "fp64", DEFAULT_STEPS["twist64"], args.warmup, args)
         line["plain_step"] = plain

@@ -137,7 +137,8 @@ extern "C"
     /* ---- additive entry points (no reference counterpart) ---- */
 
     /* Storage precision of the GPU system: 0 = FP64 (default), 1 = FP32
-     * storage with FP64 dots and residual update. */
+     * storage with FP64 dots and residual update, 2 = the same with only the
+     * upper triangle of the matrix stored (identical results to 1). */
     PHYSIK_API int PHYSIK_AMD_SetPrecision(PhysiK::WorldHandle world, int precision);
     /* PHYSIK_Step that returns a status instead of throwing: 0 ok, 3 when a
      * free node has no mass (the reference throws), 1 bad handle, 2 GPU error. */

@@ -200,7 +200,7 @@ __global__ __launch_bounds__(NT) void k_adj_element(int T, const int4* __restric
                                                     double dt, double* __restrict__ elem,
                                                     double* __restrict__ gradLogE)
 {
-    const int t = blockIdx.x * NT + threadIdx.x;
+    const int t = xcd_block() * NT + static_cast<int>(threadIdx.x);
     if (t >= T || !live[t])
         return;
     const int4 n4 = tets[t];
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(NT) void k_adj_nodes(int N, int T, const int* __res
                                                   const int* __restrict__ ok, double dt, double* __restrict__ xbar,
                                                   double* __restrict__ vbar)
 {
-    const int n = blockIdx.x * NT + threadIdx.x;
+    const int n = xcd_block() * NT + static_cast<int>(threadIdx.x);
     if (n >= N || !ok[nodeGroup[n]])
         return;
     double xb[3], vb[3];
@@ -356,14 +356,16 @@ void launch_adj_rhs(const DeviceView& v, const int* ok, const double* xbar, cons
 void launch_adj_element(const DeviceView& v, const int* ok, const double* vnext, double dt, double* gradLogE)
 {
     if (v.T > 0)
-        hipLaunchKernelGGL(k_adj_element, dim3((v.T + NT - 1) / NT), dim3(NT), 0, v.stream, v.T, v.tets, v.live,
+        hipLaunchKernelGGL(k_adj_element, dim3(xcd_grid((v.T + NT - 1) / NT)), dim3(NT), 0, v.stream, v.T, v.tets,
+                           v.live,
                            v.prm, v.nodeGroup, ok, v.nodeRow, v.x, vnext, v.xs, dt, v.adj, gradLogE);
 }
 
 void launch_adj_nodes(const DeviceView& v, const int* ok, double dt, double* xbar, double* vbar)
 {
     if (v.N > 0)
-        hipLaunchKernelGGL(k_adj_nodes, dim3((v.N + NT - 1) / NT), dim3(NT), 0, v.stream, v.N, v.T, v.astart, v.alist,
+        hipLaunchKernelGGL(k_adj_nodes, dim3(xcd_grid((v.N + NT - 1) / NT)), dim3(NT), 0, v.stream, v.N, v.T,
+                           v.astart, v.alist,
                            v.adj, v.nodeRow, v.nodeGroup, ok, dt, xbar, vbar);
 }
 

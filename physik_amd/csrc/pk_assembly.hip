@@ -199,7 +199,7 @@ __global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ 
                                                 const double* __restrict__ v, const int* __restrict__ cpos,
                                                 double* __restrict__ basis, double* __restrict__ corner)
 {
-    const int t = blockIdx.x * NT + threadIdx.x;
+    const int t = xcd_block() * NT + static_cast<int>(threadIdx.x);
     if (t >= T || !live[t])
         return;
     const size_t TT = static_cast<size_t>(T);
@@ -306,9 +306,10 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
                                                  const int* __restrict__ rowGroup, const double* __restrict__ basis,
                                                  const double* __restrict__ prm, int T,
                                                  const double* __restrict__ mrow, double dt2, M* __restrict__ vals,
-                                                 long long nnzf, M* __restrict__ minv, int nd, GroupState* gs)
+                                                 long long nnzf, M* __restrict__ minv, int nd, GroupState* gs,
+                                                 int upperOnly)
 {
-    const int s = (blockIdx.x * NT + threadIdx.x) / kSlotLanes;
+    const int s = (xcd_block() * NT + static_cast<int>(threadIdx.x)) / kSlotLanes;
     const int lane = threadIdx.x % kSlotLanes;
     if (s >= nnzu)
         return;  // whole lane groups
@@ -360,11 +361,13 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
         a[8] = m + a[8];
     }
     const size_t F = static_cast<size_t>(nnzf);
-    const int full = uFull[s];
+    // full storage: the block at its full index and its transpose at the
+    // mirror; upper-only storage: the record of the slot itself
+    const int full = upperOnly ? s : uFull[s];
 #pragma unroll
     for (int e = 0; e < 9; ++e)
         vals[BsrStore<M>::at(F, full, e)] = static_cast<M>(a[e]);
-    if (mirror >= 0)
+    if (mirror >= 0 && !upperOnly)
     {
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -579,7 +582,8 @@ void launch_reset(const DeviceView& v, double tol, int maxIt)
 void launch_element(const DeviceView& v)
 {
     if (v.T > 0)
-        hipLaunchKernelGGL(k_element, dim3((v.T + NT - 1) / NT), dim3(NT), 0, v.stream, v.T, v.tets, v.live, v.prm,
+        hipLaunchKernelGGL(k_element, dim3(xcd_grid((v.T + NT - 1) / NT)), dim3(NT), 0, v.stream, v.T, v.tets, v.live,
+                           v.prm,
                            v.x, v.v, v.cpos, v.basis, v.corner);
 }
 
@@ -588,10 +592,11 @@ void launch_assemble(const DeviceView& v, double dt)
 {
     const int n = static_cast<int>(v.nnzu);
     if (n > 0)
-        hipLaunchKernelGGL((k_assemble<M>), dim3((n * kSlotLanes + NT - 1) / NT), dim3(NT), 0, v.stream, n, v.cstart,
+        hipLaunchKernelGGL((k_assemble<M>), dim3(xcd_grid((static_cast<long long>(n) * kSlotLanes + NT - 1) / NT)),
+                           dim3(NT), 0, v.stream, n, v.cstart,
                            v.clist,
                            v.uFull, v.uMirror, v.slotRow, v.rowGroup, v.basis, v.prm, v.T, v.mrow, dt * dt,
-                           static_cast<M*>(v.vals), v.nnzf, static_cast<M*>(v.minv), v.nd, v.gs);
+                           static_cast<M*>(v.vals), v.nnzf, static_cast<M*>(v.minv), v.nd, v.gs, v.upper);
 }
 
 template <class M>

@@ -26,6 +26,8 @@
 // before any is reduced, to keep more bytes in flight per wave.
 #include "pk_device.h"
 
+#include <type_traits>
+
 namespace pk
 {
 
@@ -319,6 +321,152 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
         pa[t] = pd;
 }
 
+// Block at position p of a row's full-order block list in upper-only FP32
+// storage: a lower position (p < nl) is the transpose of the upper record
+// lower[lb + p].x, with column lower[lb + p].y; an upper position is record
+// ub + p - nl, with its column in word 9. The transpose is taken by selecting
+// elements, so the products formed from `a` are exactly those of full storage.
+__device__ inline void load_upper_block(const float* __restrict__ rec, const int2* __restrict__ lower, int lb, int ub,
+                                        int nl, int p, float* a, int& col)
+{
+    const bool tr = p < nl;
+    int2 e = make_int2(0, 0);
+    if (tr)
+        e = lower[lb + p];
+    const int slot = tr ? e.x : ub + p - nl;
+    const float2* r2 = reinterpret_cast<const float2*>(rec) + 5 * static_cast<size_t>(slot);
+    const float2 w0 = r2[0], w1 = r2[1], w2 = r2[2], w3 = r2[3], w4 = r2[4];
+    const float w[9] = {w0.x, w0.y, w1.x, w1.y, w2.x, w2.y, w3.x, w3.y, w4.x};
+    col = tr ? e.y : __float_as_int(w4.y);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            a[3 * i + j] = tr ? w[3 * j + i] : w[3 * i + j];
+}
+
+// spmv_body over upper-only FP32 storage: the same lanes, positions, products
+// and reduction order as the full-storage body, each upper record read from
+// HBM by its own row and again, transposed, by the row of its mirror (at most
+// a layer of rows later on the same XCD, so mostly from L2). (Issuing the
+// loads of two positions per row before their use was slower: the extra
+// registers cost more occupancy than the overlap gained.)
+template <int LPR, int U>
+__device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __restrict__ rowptr,
+                                       const int2* __restrict__ lower, const float* __restrict__ rec,
+                                       const float* __restrict__ d, float* __restrict__ q, double* __restrict__ pa,
+                                       const GroupState* gs)
+{
+    int t;
+    RowTile tl;
+    if (!tile_for(st, t, tl))
+        return;
+    const int stopAt = read_uniform(&gs[tl.group].stopAt);
+    if (stopAt >= 0 && stopAt <= k)
+        return;
+
+    const int lane = threadIdx.x % LPR;
+    const int sub = threadIdx.x / LPR;
+    constexpr int RPP = NT / LPR;
+    double pd = 0.0;
+    for (int row0 = tl.rowBegin + sub; row0 < tl.rowEnd; row0 += RPP * U)
+    {
+        int kb[U], ke[U], lb[U], ub[U], nl[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int row = row0 + u * RPP;
+            const bool live = row < tl.rowEnd;
+            const int2 r0 = live ? rowptr[row] : make_int2(0, 0);
+            const int2 r1 = live ? rowptr[row + 1] : make_int2(0, 0);
+            lb[u] = r0.x;
+            ub[u] = r0.y;
+            nl[u] = r1.x - r0.x;
+            kb[u] = lane;
+            ke[u] = (r1.x - r0.x) + (r1.y - r0.y);
+        }
+        int col[U];
+        float a[U][9];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const bool has = kb[u] < ke[u];
+            load_upper_block(rec, lower, has ? lb[u] : 0, has ? ub[u] : 0, has ? nl[u] : 0, has ? kb[u] : 0, a[u],
+                             col[u]);
+        }
+        double y[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int c = col[u];
+            const double d0 = static_cast<double>(d[3 * c]), d1 = static_cast<double>(d[3 * c + 1]),
+                         d2 = static_cast<double>(d[3 * c + 2]);
+            const bool has = kb[u] < ke[u];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                y[u][i] = has ? static_cast<double>(a[u][3 * i]) * d0 + static_cast<double>(a[u][3 * i + 1]) * d1 +
+                                    static_cast<double>(a[u][3 * i + 2]) * d2
+                              : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            for (int kk = kb[u] + LPR; kk < ke[u]; kk += LPR)
+            {
+                float b[9];
+                int c;
+                load_upper_block(rec, lower, lb[u], ub[u], nl[u], kk, b, c);
+                const double d0 = static_cast<double>(d[3 * c]), d1 = static_cast<double>(d[3 * c + 1]),
+                             d2 = static_cast<double>(d[3 * c + 2]);
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    y[u][i] += static_cast<double>(b[3 * i]) * d0 + static_cast<double>(b[3 * i + 1]) * d1 +
+                               static_cast<double>(b[3 * i + 2]) * d2;
+            }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+#pragma unroll
+            for (int o = LPR / 2; o > 0; o >>= 1)
+            {
+                y[u][0] += __shfl_xor(y[u][0], o, LPR);
+                y[u][1] += __shfl_xor(y[u][1], o, LPR);
+                y[u][2] += __shfl_xor(y[u][2], o, LPR);
+            }
+            const int row = row0 + u * RPP;
+            if (lane == 0 && row < tl.rowEnd)
+            {
+                const float q0 = static_cast<float>(y[u][0]), q1 = static_cast<float>(y[u][1]),
+                            q2 = static_cast<float>(y[u][2]);
+                q[3 * row] = q0;
+                q[3 * row + 1] = q1;
+                q[3 * row + 2] = q2;
+                pd += static_cast<double>(d[3 * row]) * static_cast<double>(q0) +
+                      static_cast<double>(d[3 * row + 1]) * static_cast<double>(q1) +
+                      static_cast<double>(d[3 * row + 2]) * static_cast<double>(q2);
+            }
+        }
+    }
+    double u1 = 0.0, u2 = 0.0;
+    block_sum3(pd, u1, u2);
+    if (threadIdx.x == 0)
+        pa[t] = pd;
+}
+
+template <int LPR, int U>
+__global__ __launch_bounds__(NT) void k_spmv_upper(const int* kb, int k0, TileView st, const int2* __restrict__ rowptr,
+                                                   const int2* __restrict__ lower, const float* __restrict__ rec,
+                                                   const float* __restrict__ d, float* __restrict__ q,
+                                                   double* __restrict__ pa, const GroupState* gs,
+                                                   unsigned long long* stamps)
+{
+    const LaunchIndex li = launch_index(kb, k0, stamps);
+    const int k = li.k;
+    unsigned long long* stamp = li.stamp;
+    stamp_begin(stamp);
+    spmv_body_upper<LPR, U>(k, st, rowptr, lower, rec, d, q, pa, gs);
+    stamp_end(stamp);
+}
+
 template <class M, int LPR, int U>
 __global__ __launch_bounds__(NT) void k_spmv(const int* kb, int k0, TileView st, const int* __restrict__ fstart,
                                              const int* __restrict__ fcol, const M* __restrict__ vals,
@@ -475,6 +623,42 @@ __global__ __launch_bounds__(NT) void k_spmv_plain(int nd, const int* __restrict
         y[3 * row + i] = acc[i];
 }
 
+// y = A x over upper-only FP32 storage, one thread per row, blocks in
+// full-row order (the same sums as k_spmv_plain over full storage).
+__global__ __launch_bounds__(NT) void k_spmv_plain_upper(int nd, const int2* __restrict__ rowptr,
+                                                         const int2* __restrict__ lower,
+                                                         const float* __restrict__ rec,
+                                                         const double* __restrict__ xin, double* __restrict__ y)
+{
+    const int row = blockIdx.x * NT + threadIdx.x;
+    if (row >= nd)
+        return;
+    const int2 r0 = rowptr[row], r1 = rowptr[row + 1];
+    const int nl = r1.x - r0.x, n = nl + (r1.y - r0.y);
+    double acc[3] = {0, 0, 0};
+    for (int p = 0; p < n; ++p)
+    {
+        float a[9];
+        int c;
+        load_upper_block(rec, lower, r0.x, r0.y, nl, p, a, c);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            acc[i] += static_cast<double>(a[3 * i]) * xin[3 * c] + static_cast<double>(a[3 * i + 1]) * xin[3 * c + 1] +
+                      static_cast<double>(a[3 * i + 2]) * xin[3 * c + 2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        y[3 * row + i] = acc[i];
+}
+
+template <int LPR, int U>
+static void launch_spmv_upper_shape(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp)
+{
+    hipLaunchKernelGGL((k_spmv_upper<LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), 0, v.stream, kb,
+                       k0, v.spmv, v.rowptr, v.lower, static_cast<const float*>(v.vals),
+                       static_cast<const float*>(v.d[k0 & 1]), static_cast<float*>(v.q), v.pa, v.gs, stamp);
+}
+
 // The iteration of a launch is k0 (kb == nullptr) or *kb + k0 with *kb even,
 // so k0 alone picks the ping-pong buffers.
 template <class M, int LPR, int U>
@@ -500,6 +684,25 @@ void launch_spmv(const DeviceView& v, const int* kb, int k0, int lanesPerRow, in
 {
     if (v.spmv.count == 0)
         return;
+    if constexpr (std::is_same<M, float>::value)
+    {
+        if (v.upper)
+        {
+#define PK_SHAPE(L, U)                                      \
+    if (lanesPerRow == L && unroll == U)                    \
+    {                                                       \
+        launch_spmv_upper_shape<L, U>(v, kb, k0, stamp);    \
+        return;                                             \
+    }
+            PK_SHAPE(4, 1) PK_SHAPE(4, 2) PK_SHAPE(4, 4)
+            PK_SHAPE(8, 1) PK_SHAPE(8, 2) PK_SHAPE(8, 4)
+            PK_SHAPE(16, 1) PK_SHAPE(16, 2) PK_SHAPE(16, 4)
+            PK_SHAPE(32, 1) PK_SHAPE(32, 2)
+#undef PK_SHAPE
+            launch_spmv_upper_shape<16, 1>(v, kb, k0, stamp);
+            return;
+        }
+    }
 #define PK_SHAPE(L, U)                        \
     if (lanesPerRow == L && unroll == U)      \
     {                                         \
@@ -634,6 +837,13 @@ void launch_stamp_reduce(hipStream_t stream, const unsigned long long* raw, int 
 template <class M>
 void launch_spmv_plain(const DeviceView& v, const double* x, double* y)
 {
+    if (v.upper)
+    {
+        if (v.nd > 0)
+            hipLaunchKernelGGL(k_spmv_plain_upper, dim3((v.nd + NT - 1) / NT), dim3(NT), 0, v.stream, v.nd, v.rowptr,
+                               v.lower, static_cast<const float*>(v.vals), x, y);
+        return;
+    }
     if (v.nd > 0)
         hipLaunchKernelGGL((k_spmv_plain<M>), dim3((v.nd + NT - 1) / NT), dim3(NT), 0, v.stream, v.nd, v.fstart,
                            v.fcol, static_cast<const M*>(v.vals), v.nnzf, x, y);
@@ -655,9 +865,10 @@ __global__ __launch_bounds__(NT) void k_record_columns(long long F, const int* _
 
 void launch_record_columns(const DeviceView& v)
 {
-    if (v.nnzf > 0)
-        hipLaunchKernelGGL(k_record_columns, dim3(static_cast<unsigned>((v.nnzf + NT - 1) / NT)), dim3(NT), 0,
-                           v.stream, v.nnzf, v.fcol, static_cast<float*>(v.vals));
+    const long long n = v.upper ? v.nnzu : v.nnzf;
+    if (n > 0)
+        hipLaunchKernelGGL(k_record_columns, dim3(static_cast<unsigned>((n + NT - 1) / NT)), dim3(NT), 0, v.stream, n,
+                           v.upper ? v.ucol : v.fcol, static_cast<float*>(v.vals));
 }
 
 template void launch_spmv_plain<double>(const DeviceView&, const double*, double*);

@@ -97,6 +97,18 @@ __device__ inline int tile_of(const TileView& tv)
     return t < tv.count ? t : -1;
 }
 
+// Workgroup order for a flat launch over consecutive blocks of work: with a
+// grid of 8 * ceil(blocks / 8) workgroups, workgroup b takes block
+// (b % 8) * per + b / 8, so each XCD walks one contiguous eighth of the work
+// and neighbouring blocks share its L2 (blocks >= the real count find no work).
+__device__ inline int xcd_block()
+{
+    const int per = static_cast<int>(gridDim.x) >> 3;
+    const int b = static_cast<int>(blockIdx.x);
+    return (b & 7) * per + (b >> 3);
+}
+inline unsigned xcd_grid(long long blocks) { return 8u * static_cast<unsigned>((blocks + 7) / 8); }
+
 // The tile (id t, rows tl) of this workgroup in a view that may be compacted;
 // false when the workgroup has none.
 __device__ inline bool tile_for(const TileView& tv, int& t, RowTile& tl)
@@ -236,6 +248,15 @@ struct DeviceView
     const int* rowGroup;
     const int* fstart;
     const int* fcol;
+    // upper-only FP32 storage (upper != 0): A holds one record per upper slot;
+    // row r's blocks in full-row order are its lower blocks (c < r), each the
+    // transpose of the upper record lower[l].x with column lower[l].y for
+    // l in [rowptr[r].x, rowptr[r+1].x), then its upper slots
+    // [rowptr[r].y, rowptr[r+1].y) (columns c >= r ascending)
+    int upper;
+    const int2* rowptr;
+    const int2* lower;
+    const int* ucol;
     const int* nstart;
     const int* nlist;
     const int* rowNode;
@@ -300,7 +321,8 @@ void launch_compact(const DeviceView& v, RowTile* vecPerm, RowTile* spmvPerm);
 void launch_stamp_reduce(hipStream_t stream, const unsigned long long* raw, int grid, int launches,
                          unsigned long long* span);
 template <class M> void launch_spmv_plain(const DeviceView& v, const double* x, double* y);
-// FP32 storage: writes each block's column into word 9 of its record.
+// FP32 storage: writes each block's column into word 9 of its record (one
+// record per full block, or per upper slot for upper-only storage).
 void launch_record_columns(const DeviceView& v);
 
 // Reverse mode (pk_adjoint.hip). `ok` holds one flag per body: the forward

@@ -89,7 +89,7 @@ struct Profile
     bool on = false;
     long long launches[PK_K_COUNT] = {};
     double ms[PK_K_COUNT] = {};
-    long long spmvBlocks = 0, spmvRows = 0, spmvBodyLaunches = 0;  // work of the counted SpMV launches
+    long long spmvBlocks = 0, spmvUpper = 0, spmvRows = 0, spmvBodyLaunches = 0;  // work of the counted SpMV launches
     std::vector<hipEvent_t> pool;
     int used = 0;
     struct Rec
@@ -118,6 +118,8 @@ struct pk_system
     pk::DevBuf<char> minv;  // storage type, like the CG vectors
     pk::DevBuf<int> cstart, clist, uFull, uMirror, slotRow, rowGroup, fstart, fcol, nstart, nlist, rowNode, active;
     pk::DevBuf<int> astart, alist, nodeRow, nodeGroup, cpos;
+    pk::DevBuf<int2> rowptr, lower;  // upper-only storage: row pointers and lower blocks
+    pk::DevBuf<int> ucol;
     pk::DevBuf<char> vals, z, d0, d1, q;  // element type set by the precision
     pk::DevBuf<pk::Partial> pb0, pb1;
     // batched solves: the tile of each CG workgroup, over the bodies still
@@ -150,7 +152,7 @@ struct pk_system
     int implicit = 1;
     int lastIters = 0;
     std::vector<pk::GroupState> hostGs;
-    std::vector<long long> groupBlocks, groupRows;  // full BSR blocks and dynamic rows per group
+    std::vector<long long> groupBlocks, groupUpper, groupRows;  // full and upper BSR blocks, dynamic rows per group
     pk::Profile prof;
 
     // CG chunks replayed from captured HIP graphs: the first chunk of a solve
@@ -213,6 +215,10 @@ struct pk_system
         dv.slotRow = slotRow.p;
         dv.rowGroup = rowGroup.p;
         dv.fstart = fstart.p;
+        dv.upper = precision == PK_FP32_UPPER ? 1 : 0;
+        dv.rowptr = rowptr.p;
+        dv.lower = lower.p;
+        dv.ucol = ucol.p;
         dv.fcol = fcol.p;
         dv.nstart = nstart.p;
         dv.nlist = nlist.p;
@@ -374,6 +380,7 @@ static void harvest_profile(pk_system* s)
                     // update) multiplied this body's rows
                     const long long c = s->hostGs[g].stopAt + s->hostGs[g].stopInUpdate;
                     p.spmvBlocks += c * s->groupBlocks[g];
+                    p.spmvUpper += c * s->groupUpper[g];
                     p.spmvRows += c * s->groupRows[g];
                     p.spmvBodyLaunches += c;
                 }
@@ -630,16 +637,20 @@ static int choose_lanes(const Topology& t)
     return 32;
 }
 
+// A, M^-1 and the SpMV vectors in FP32 (full or upper-only storage).
+static bool is_f32(const pk_system* s) { return s->precision != PK_FP64; }
+
 static int alloc_work(pk_system* s)
 {
     const Topology& t = s->topo;
-    const size_t es = s->precision == PK_FP32_STORAGE ? sizeof(float) : sizeof(double);
+    const size_t es = is_f32(s) ? sizeof(float) : sizeof(double);
     const size_t nd3 = static_cast<size_t>(t.nd) * 3;
     int rc;
     if ((rc = s->basis.alloc(static_cast<size_t>(t.T) * EB_COUNT)) != PK_OK) return rc;
     if ((rc = s->corner.alloc(t.nlist.size() * N_CORNER)) != PK_OK) return rc;
-    const size_t words = s->precision == PK_FP32_STORAGE ? BsrStore<float>::kWords : BsrStore<double>::kWords;
-    if ((rc = s->vals.alloc(static_cast<size_t>(t.nnzf()) * words * es)) != PK_OK) return rc;
+    const size_t words = is_f32(s) ? BsrStore<float>::kWords : BsrStore<double>::kWords;
+    const size_t blocks = static_cast<size_t>(s->precision == PK_FP32_UPPER ? t.nnzu() : t.nnzf());
+    if ((rc = s->vals.alloc(blocks * words * es)) != PK_OK) return rc;
     if ((rc = s->z.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->d0.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->d1.alloc(nd3 * es)) != PK_OK) return rc;
@@ -696,6 +707,7 @@ static int setup_device(pk_system* s)
         for (int k = t.ustart[r]; k < t.ustart[r + 1]; ++k)
             slotRow[k] = r;
     s->groupBlocks.assign(static_cast<size_t>(t.G), 0);
+    s->groupUpper.assign(static_cast<size_t>(t.G), 0);
     s->groupRows.assign(static_cast<size_t>(t.G), 0);
     for (int g = 0; g < t.G; ++g)
     {
@@ -703,7 +715,10 @@ static int setup_device(pk_system* s)
             rowGroup[r] = g;
         s->groupRows[g] = t.groupRowEnd[g] - t.groupRowBegin[g];
         if (s->groupRows[g] > 0)
+        {
             s->groupBlocks[g] = t.fstart[t.groupRowEnd[g]] - t.fstart[t.groupRowBegin[g]];
+            s->groupUpper[g] = t.ustart[t.groupRowEnd[g]] - t.ustart[t.groupRowBegin[g]];
+        }
     }
     if ((rc = s->slotRow.upload(slotRow, st)) != PK_OK) return rc;
     if ((rc = s->rowGroup.upload(rowGroup, st)) != PK_OK) return rc;
@@ -717,9 +732,44 @@ static int setup_device(pk_system* s)
     if ((rc = s->nodeRow.upload(t.dyn, st)) != PK_OK) return rc;
     if ((rc = s->nodeGroup.upload(t.nodeGroup, st)) != PK_OK) return rc;
     if ((rc = s->cpos.upload(t.cpos, st)) != PK_OK) return rc;
+    if (s->precision == PK_FP32_UPPER)
+    {
+        // row r's lower blocks (c < r) are the compact entries
+        // [fstart[r] - ustart[r], fstart[r+1] - ustart[r+1]); the one at full
+        // index m of row r is the mirror of upper slot s (uMirror[s] == m),
+        // stored as (s, the row of s)
+        const long long nl = t.nnzf() - t.nnzu();
+        std::vector<int2> rp(static_cast<size_t>(t.nd) + 1), low(static_cast<size_t>(std::max<long long>(nl, 1)),
+                                                                 make_int2(-1, -1));
+        for (int r = 0; r <= t.nd; ++r)
+            rp[r] = make_int2(t.fstart[r] - t.ustart[r], t.ustart[r]);
+        for (long long u = 0; u < t.nnzu(); ++u)
+        {
+            const int m = t.uMirror[u];
+            if (m < 0)
+                continue;
+            const int r = t.ucol[u];
+            const long long l = static_cast<long long>(m) - t.ustart[r];
+            if (r < 0 || r >= t.nd || m < t.fstart[r] || l < rp[r].x || l >= rp[r + 1].x || low[l].x != -1)
+            {
+                set_error("upper-only storage: inconsistent mirror map");
+                return PK_ERR_ARG;
+            }
+            low[l] = make_int2(static_cast<int>(u), slotRow[u]);
+        }
+        for (long long l = 0; l < nl; ++l)
+            if (low[l].x < 0)
+            {
+                set_error("upper-only storage: a lower block has no upper mirror");
+                return PK_ERR_ARG;
+            }
+        if ((rc = s->rowptr.upload(rp, st)) != PK_OK) return rc;
+        if ((rc = s->lower.upload(low, st)) != PK_OK) return rc;
+        if ((rc = s->ucol.upload(t.ucol, st)) != PK_OK) return rc;
+    }
     if ((rc = alloc_work(s)) != PK_OK) return rc;
     launch_gdot(s->view(), s->prm.p);
-    if (s->precision == PK_FP32_STORAGE)
+    if (is_f32(s))
         launch_record_columns(s->view());
     std::vector<double> zero(static_cast<size_t>(t.N) * 3, 0.0);
     if ((rc = s->v.upload(zero, st)) != PK_OK) return rc;
@@ -747,7 +797,7 @@ int pk_device_count(void)
 
 int pk_system_create(const pk_mesh_desc* desc, int precision, pk_system** out)
 {
-    if (!desc || !out || (precision != PK_FP64 && precision != PK_FP32_STORAGE))
+    if (!desc || !out || (precision != PK_FP64 && precision != PK_FP32_STORAGE && precision != PK_FP32_UPPER))
     {
         set_error("pk_system_create: bad argument");
         return PK_ERR_ARG;
@@ -926,7 +976,7 @@ int pk_system_step(pk_system* s, double dt, int substeps, int frames)
     for (int f = 0; f < frames; ++f)
         for (int k = 0; k < substeps; ++k)
         {
-            const int rc = s->precision == PK_FP32_STORAGE ? run_substep<float>(s, h) : run_substep<double>(s, h);
+            const int rc = is_f32(s) ? run_substep<float>(s, h) : run_substep<double>(s, h);
             if (rc != PK_OK)
                 return rc;
             if (s->implicit && s->topo.nd > 0)
@@ -1099,7 +1149,7 @@ int pk_system_record(pk_system* s, double dt, int steps, int every)
         set_error("pk_system_record: no dynamic node");
         return PK_ERR_ARG;
     }
-    return s->precision == PK_FP32_STORAGE ? record_run<float>(s, dt, steps, every)
+    return is_f32(s) ? record_run<float>(s, dt, steps, every)
                                            : record_run<double>(s, dt, steps, every);
 }
 
@@ -1131,7 +1181,7 @@ int pk_system_backward(pk_system* s, const double* xbar, const double* vbar, dou
         PK_HIP(hipMemsetAsync(s->vbar.p, 0, n3 * sizeof(double), s->stream));
     if (t.T)
         PK_HIP(hipMemsetAsync(s->gradE.p, 0, static_cast<size_t>(t.T) * sizeof(double), s->stream));
-    rc = s->precision == PK_FP32_STORAGE ? backward_run<float>(s, adj_tol, adj_max_iter, adj_iterations)
+    rc = is_f32(s) ? backward_run<float>(s, adj_tol, adj_max_iter, adj_iterations)
                                          : backward_run<double>(s, adj_tol, adj_max_iter, adj_iterations);
     if (rc != PK_OK)
         return rc;
@@ -1175,7 +1225,7 @@ int pk_system_profile(pk_system* s, int enable)
         s->prof.launches[k] = 0;
         s->prof.ms[k] = 0.0;
     }
-    s->prof.spmvBlocks = s->prof.spmvRows = s->prof.spmvBodyLaunches = 0;
+    s->prof.spmvBlocks = s->prof.spmvUpper = s->prof.spmvRows = s->prof.spmvBodyLaunches = 0;
     s->prof.recs.clear();
     s->prof.used = 0;
     return PK_OK;
@@ -1191,11 +1241,13 @@ int pk_system_set_graphs(pk_system* s, int enable)
     return PK_OK;
 }
 
-int pk_system_spmv_work(pk_system* s, long long* blocks, long long* rows, long long* body_launches)
+int pk_system_spmv_work(pk_system* s, long long* blocks, long long* upper_blocks, long long* rows,
+                        long long* body_launches)
 {
     if (!s)
         return PK_ERR_ARG;
     if (blocks) *blocks = s->prof.spmvBlocks;
+    if (upper_blocks) *upper_blocks = s->prof.spmvUpper;
     if (rows) *rows = s->prof.spmvRows;
     if (body_launches) *body_launches = s->prof.spmvBodyLaunches;
     return PK_OK;
@@ -1214,7 +1266,7 @@ int pk_system_assemble(pk_system* s, double dt)
 {
     if (!s)
         return PK_ERR_ARG;
-    const int rc = s->precision == PK_FP32_STORAGE ? run_assembly<float>(s, dt, s->tol, s->maxIt, true)
+    const int rc = is_f32(s) ? run_assembly<float>(s, dt, s->tol, s->maxIt, true)
                                                    : run_assembly<double>(s, dt, s->tol, s->maxIt, true);
     if (rc != PK_OK)
         return rc;
@@ -1233,9 +1285,31 @@ int pk_system_get_matrix(pk_system* s, double* full_vals, double* rhs, double* m
     const Topology& t = s->topo;
     const size_t F = static_cast<size_t>(t.nnzf());
     PK_HIP(hipStreamSynchronize(s->stream));
-    if (full_vals && F)
+    if (full_vals && F && s->precision == PK_FP32_UPPER)
     {
-        const bool f32 = s->precision == PK_FP32_STORAGE;
+        // one record per upper slot: the block at its full index, its
+        // transpose at the mirror
+        const size_t U = static_cast<size_t>(t.nnzu());
+        std::vector<float> raw(U * BsrStore<float>::kWords);
+        PK_HIP(hipMemcpy(raw.data(), s->vals.p, raw.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t u = 0; u < U; ++u)
+        {
+            const float* w = raw.data() + BsrStore<float>::at(U, u, 0);
+            const size_t full = static_cast<size_t>(t.uFull[u]);
+            for (int e = 0; e < 9; ++e)
+                full_vals[9 * full + e] = w[e];
+            if (t.uMirror[u] >= 0)
+            {
+                const size_t m = static_cast<size_t>(t.uMirror[u]);
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        full_vals[9 * m + 3 * i + j] = w[3 * j + i];
+            }
+        }
+    }
+    else if (full_vals && F)
+    {
+        const bool f32 = is_f32(s);
         const size_t es = f32 ? 4 : 8;
         std::vector<char> raw(F * (f32 ? BsrStore<float>::kWords : BsrStore<double>::kWords) * es);
         PK_HIP(hipMemcpy(raw.data(), s->vals.p, raw.size(), hipMemcpyDeviceToHost));
@@ -1248,7 +1322,7 @@ int pk_system_get_matrix(pk_system* s, double* full_vals, double* rhs, double* m
         PK_HIP(hipMemcpy(rhs, s->r.p, static_cast<size_t>(t.nd) * 3 * sizeof(double), hipMemcpyDeviceToHost));
     if (minv && t.nd)
     {
-        const bool f32 = s->precision == PK_FP32_STORAGE;
+        const bool f32 = is_f32(s);
         std::vector<char> raw(static_cast<size_t>(t.nd) * 9 * (f32 ? 4 : 8));
         PK_HIP(hipMemcpy(raw.data(), s->minv.p, raw.size(), hipMemcpyDeviceToHost));
         for (int r = 0; r < t.nd; ++r)
@@ -1307,7 +1381,7 @@ int pk_system_spmv(pk_system* s, const double* x, double* y)
     if ((rc = dy.alloc(static_cast<size_t>(t.nd) * 3)) != PK_OK) return rc;
     PK_HIP(hipMemcpyAsync(dx.p, x, static_cast<size_t>(t.nd) * 3 * sizeof(double), hipMemcpyHostToDevice, s->stream));
     const DeviceView dv = s->view();
-    if (s->precision == PK_FP32_STORAGE)
+    if (is_f32(s))
         launch_spmv_plain<float>(dv, dx.p, dy.p);
     else
         launch_spmv_plain<double>(dv, dx.p, dy.p);

@@ -25,7 +25,7 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=512)
     ap.add_argument("--workload", default="twist64", choices=["twist64", "beam10", "batch256", "diff32", "batch128"])
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp32u"])
     ap.add_argument("--shape", default=None,
                     help="CG launch shape lanes,spmv_rows,vector_rows,unroll (default: the system's own)")
     args = ap.parse_args()
@@ -34,17 +34,17 @@ def main():
     nbytes = args.copy_mb * (1 << 20)
     N.check(lib.pk_diag_copy(nbytes, 3, C.byref(ms)), "pk_diag_copy")
     w = getattr(W, args.workload)()
-    s = W.make_system(w, N.PK_FP64 if args.precision == "fp64" else N.PK_FP32_STORAGE)
+    s = W.make_system(w, {"fp64": N.PK_FP64, "fp32": N.PK_FP32_STORAGE, "fp32u": N.PK_FP32_UPPER}[args.precision])
     if args.shape:
         s.tune(*[int(a) for a in args.shape.split(",")])
     s.profile(True)
     s.step(w.dt, 1, args.steps)
     it = s.results()[0]
-    blocks, rows, body_launches = s.spmv_work()
+    blocks, ublocks, rows, body_launches = s.spmv_work()
     print(json.dumps({"copy_bytes": nbytes, "copy_ms": ms.value, "copy_GBps": 2 * nbytes / ms.value / 1e6,
                       "workload": args.workload, "steps": args.steps, "cg_iterations": int(it.max()),
                       "nnzb_full": s.nnzb_full, "dynamic_rows": s.nd, "tets": s.t, "bodies": s.groups,
-                      "spmv_launches": s.kernel_time(N.K_SPMV)[0], "spmv_blocks": blocks, "spmv_rows": rows,
+                      "spmv_launches": s.kernel_time(N.K_SPMV)[0], "spmv_blocks": blocks, "spmv_upper_blocks": ublocks, "spmv_rows": rows, "nnzb_upper": s.nnzb_upper,
                       "spmv_body_launches": body_launches, "shape": s.get_tune()}), flush=True)
 
 

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libphysik_amd.so")
 
 PK_OK, PK_ERR_ARG, PK_ERR_HIP, PK_ERR_MASS, PK_ERR_NO_DEVICE = 0, 1, 2, 3, 4
-PK_FP64, PK_FP32_STORAGE = 0, 1
+PK_FP64, PK_FP32_STORAGE, PK_FP32_UPPER = 0, 1, 2
 GROUP_INTEGRATED, GROUP_SOLVE_FAILED, GROUP_NONFINITE, GROUP_NO_DYNAMIC = 0, 1, 2, 3
 K_ELEMENT, K_ASSEMBLE, K_RHS, K_SPMV, K_UPDATE, K_DIRECTION, K_INTEGRATE = range(7)
 KERNEL_NAMES = ["element", "assemble", "rhs", "spmv", "update", "direction", "integrate"]
@@ -69,7 +69,7 @@ PK_SIGNATURES = {
     "pk_system_kernel_time": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
     "pk_system_set_graphs": (C.c_int, [_vp, C.c_int]),
     "pk_system_spmv_work": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
-                                      C.POINTER(C.c_longlong)]),
+                                      C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "pk_system_assemble": (C.c_int, [_vp, C.c_double]),
     "pk_system_get_matrix": (C.c_int, [_vp, _vp, _vp, _vp]),
     "pk_system_get_elements": (C.c_int, [_vp, _vp, _vp, _vp]),

@@ -156,9 +156,10 @@ class System:
         N.check(self.lib.pk_system_set_graphs(self.h, int(enable)), "pk_system_set_graphs")
 
     def spmv_work(self):
-        """(full BSR blocks, dynamic rows, body-launches) the profiled SpMV
-        launches multiplied; stopped bodies' rows are not counted."""
-        v = [C.c_longlong() for _ in range(3)]
+        """(full BSR blocks, upper-triangle blocks among them, dynamic rows,
+        body-launches) the profiled SpMV launches multiplied; stopped bodies'
+        rows are not counted."""
+        v = [C.c_longlong() for _ in range(4)]
         N.check(self.lib.pk_system_spmv_work(self.h, *[C.byref(x) for x in v]), "pk_system_spmv_work")
         return tuple(x.value for x in v)
 

@@ -292,7 +292,7 @@ def test_spmv_work_counts_only_bodies_still_iterating():
     s.profile(True)
     s.step(w.dt, 1, 2)
     n_s, _ = s.kernel_time(N.K_SPMV)
-    blocks, rows, bl = s.spmv_work()
+    blocks, _, rows, bl = s.spmv_work()
     assert bl == n_s and blocks == n_s * s.nnzb_full and rows == n_s * s.nd
     # bodies that converge at different iterations: a body takes part in its
     # iterations' launches (one more when its loop ended in the update)
@@ -301,7 +301,7 @@ def test_spmv_work_counts_only_bodies_still_iterating():
     s.profile(True)
     s.step(w.dt)
     n_s, _ = s.kernel_time(N.K_SPMV)
-    blocks, rows, bl = s.spmv_work()
+    blocks, _, rows, bl = s.spmv_work()
     it = s.results()[0]
     pat = Pattern(w.rest, w.tets, node_fixed=w.fixed, node_group=w.node_group)
     per_blocks, per_rows = [], []
