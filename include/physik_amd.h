@@ -138,6 +138,14 @@ int pk_system_step(pk_system* sys, double dt, int substeps, int frames);
 int pk_system_group_results(pk_system* sys, int* iterations, double* residual_norm, int* converged,
                             int* status);
 
+/* Reverse-mode memory (no reference counterpart). pk_system_backward replays
+ * each checkpoint segment forward and then reverses it substep by substep;
+ * when the segment's matrices (A and M^-1 of every substep) fit max_bytes
+ * (default 8 GiB), the replay keeps them and the reverse steps solve with
+ * them instead of assembling again. 0 turns this off. Gradients are identical
+ * either way. */
+int pk_system_set_matrix_store(pk_system* sys, long long max_bytes);
+
 /* CG launch mode (no reference counterpart). enable != 0: every solve replays
  * its CG four iterations at a time from HIP graphs captured on first use
  * (direction, SpMV and update launches, plus a batch's tile compaction),

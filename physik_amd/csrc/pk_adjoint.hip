@@ -353,6 +353,32 @@ void launch_adj_rhs(const DeviceView& v, const int* ok, const double* xbar, cons
                        static_cast<M*>(v.d[0]), v.pb[1]);
 }
 
+// The assembly's per-body rejection flags, saved after a replayed substep's
+// assembly and put back before the reverse step reuses its matrix.
+__global__ void k_save_failed(const GroupState* __restrict__ gs, int G, int* __restrict__ out)
+{
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x)
+        out[g] = gs[g].failed;
+}
+
+__global__ void k_restore_failed(GroupState* __restrict__ gs, int G, const int* __restrict__ in)
+{
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x)
+        gs[g].failed = in[g];
+}
+
+void launch_save_failed(const DeviceView& v, int* out)
+{
+    if (v.G > 0)
+        hipLaunchKernelGGL(k_save_failed, dim3((v.G + NT - 1) / NT), dim3(NT), 0, v.stream, v.gs, v.G, out);
+}
+
+void launch_restore_failed(const DeviceView& v, const int* in)
+{
+    if (v.G > 0)
+        hipLaunchKernelGGL(k_restore_failed, dim3((v.G + NT - 1) / NT), dim3(NT), 0, v.stream, v.gs, v.G, in);
+}
+
 void launch_adj_element(const DeviceView& v, const int* ok, const double* vnext, double dt, double* gradLogE)
 {
     if (v.T > 0)

@@ -330,6 +330,9 @@ void launch_record_columns(const DeviceView& v);
 void launch_record_ok(const DeviceView& v, int* ok);
 template <class M>
 void launch_adj_rhs(const DeviceView& v, const int* ok, const double* xbar, const double* vbar, double dt);
+// Saves / restores the per-body assembly rejection flags (G ints).
+void launch_save_failed(const DeviceView& v, int* out);
+void launch_restore_failed(const DeviceView& v, const int* in);
 void launch_adj_element(const DeviceView& v, const int* ok, const double* vnext, double dt, double* gradLogE);
 void launch_adj_nodes(const DeviceView& v, const int* ok, double dt, double* xbar, double* vbar);
 

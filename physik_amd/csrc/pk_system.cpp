@@ -62,6 +62,15 @@ struct DevBuf
         PK_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
         return PK_OK;
     }
+    // alloc, unless the buffer already holds exactly `count` elements
+    int ensure(size_t count) { return (p && n == count) ? PK_OK : alloc(count); }
+    void free()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
     int upload(const std::vector<T>& h, hipStream_t st)
     {
         const int rc = alloc(h.size());
@@ -140,6 +149,15 @@ struct pk_system
     // integrate flags of that segment, the adjoints and the gradient
     pk::DevBuf<double> tapeX, tapeV, segX, segV, xbar, vbar, gradE, xEnd, vEnd;
     pk::DevBuf<int> segOk;
+    // the matrices of the segment being reversed (A and M^-1 of every
+    // substep, and the assembly's rejection flags), kept from its replay when
+    // they fit storeCap bytes, so the reverse step need not assemble again
+    pk::DevBuf<char> segVals, segMinv;
+    pk::DevBuf<int> segFailed;
+    long long storeCap = 8LL << 30;
+    int segColumns = -1;  // epoch whose FP32 record columns segVals holds
+    char* valsView = nullptr;  // when set, the matrix the kernels use
+    char* minvView = nullptr;
     double tapeDt = 0.0;
     int tapeSteps = 0, tapeEvery = 0;
     bool tapeValid = false;
@@ -166,6 +184,7 @@ struct pk_system
         hipGraphExec_t first = nullptr, next = nullptr;
         int epoch = -1;
         const void* stamps[3] = {nullptr, nullptr, nullptr};
+        const void* matrix[2] = {nullptr, nullptr};
     } chunks;
 
     void drop_graphs()
@@ -234,10 +253,10 @@ struct pk_system
         dv.corner = corner.p;
         dv.adj = adj.p;
         dv.cpos = cpos.p;
-        dv.minv = minv.p;
+        dv.minv = minvView ? minvView : minv.p;
         dv.xs = xs.p;
         dv.r = r.p;
-        dv.vals = vals.p;
+        dv.vals = valsView ? valsView : vals.p;
         dv.z = z.p;
         dv.d[0] = d0.p;
         dv.d[1] = d1.p;
@@ -512,7 +531,8 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
     if (s->useGraphs)
     {
         auto& cg = s->chunks;
-        if (cg.epoch != s->epoch || cg.stamps[0] != stS || cg.stamps[1] != stU || cg.stamps[2] != stD)
+        if (cg.epoch != s->epoch || cg.stamps[0] != stS || cg.stamps[1] != stU || cg.stamps[2] != stD ||
+            cg.matrix[0] != dv.vals || cg.matrix[1] != dv.minv)
         {
             s->drop_graphs();
             int rc;
@@ -524,6 +544,8 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
             cg.stamps[0] = stS;
             cg.stamps[1] = stU;
             cg.stamps[2] = stD;
+            cg.matrix[0] = dv.vals;
+            cg.matrix[1] = dv.minv;
         }
     }
     // launch k runs the direction, SpMV and update of iteration k (the
@@ -585,13 +607,17 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
     return PK_OK;
 }
 
+// One substep; `failedOut` (reverse-mode replays) receives the assembly's
+// per-body rejection flags.
 template <class M>
-static int run_substep(pk_system* s, double dt)
+static int run_substep(pk_system* s, double dt, int* failedOut = nullptr)
 {
     int rc = run_assembly<M>(s, dt, s->tol, s->maxIt, true);
     if (rc != PK_OK || s->topo.nd == 0)
         return rc;
     const DeviceView dv = s->view();
+    if (failedOut)
+        launch_save_failed(dv, failedOut);
     if (s->implicit)
     {
         if ((rc = run_cg<M>(s, s->maxIt, s->lastIters)) != PK_OK)
@@ -1010,9 +1036,9 @@ static int dcopy(pk_system* s, double* dst, const double* src, size_t n)
 // One forward substep of a replay: the plain substep, then the per-body
 // integrate flags into `ok`.
 template <class M>
-static int replay_substep(pk_system* s, double dt, int* ok)
+static int replay_substep(pk_system* s, double dt, int* ok, int* failedOut = nullptr)
 {
-    int rc = run_substep<M>(s, dt);
+    int rc = run_substep<M>(s, dt, failedOut);
     if (rc != PK_OK)
         return rc;
     launch_record_ok(s->view(), ok);
@@ -1032,7 +1058,14 @@ static int reverse_substep(pk_system* s, double dt, int j, double adjTol, int ad
     int rc;
     if ((rc = dcopy(s, s->x.p, s->segX.p + j * n3, n3)) != PK_OK) return rc;
     if ((rc = dcopy(s, s->v.p, s->segV.p + j * n3, n3)) != PK_OK) return rc;
-    if ((rc = run_assembly<M>(s, dt, adjTol, adjMaxIt, false)) != PK_OK)
+    if (s->valsView)
+    {
+        // A(x_j) and its rejection flags as the replay left them
+        const DeviceView sv = s->view();
+        launch_reset(sv, adjTol, adjMaxIt);
+        launch_restore_failed(sv, s->segFailed.p + static_cast<size_t>(j) * s->topo.G);
+    }
+    else if ((rc = run_assembly<M>(s, dt, adjTol, adjMaxIt, false)) != PK_OK)
         return rc;
     const DeviceView dv = s->view();
     launch_adj_rhs<M>(dv, ok, s->xbar.p, s->vbar.p, dt);
@@ -1094,6 +1127,43 @@ static int backward_run(pk_system* s, double adjTol, int adjMaxIt, int* adjIters
     if ((rc = s->vEnd.alloc(n3)) != PK_OK) return rc;
     if ((rc = dcopy(s, s->xEnd.p, s->x.p, n3)) != PK_OK) return rc;
     if ((rc = dcopy(s, s->vEnd.p, s->v.p, n3)) != PK_OK) return rc;
+    // keep the segment's matrices from the replay when they fit
+    const size_t valsBytes = s->vals.n, minvBytes = s->minv.n;
+    bool store = t.nd > 0 && s->storeCap > 0 &&
+                 static_cast<double>(every) * static_cast<double>(valsBytes + minvBytes) <=
+                     static_cast<double>(s->storeCap);
+    if (store)
+    {
+        if (!(s->segVals.p && s->segVals.n == every * valsBytes))
+            s->segColumns = -1;  // new memory: its record columns are not written yet
+        if (s->segVals.ensure(every * valsBytes) != PK_OK || s->segMinv.ensure(every * minvBytes) != PK_OK ||
+            s->segFailed.ensure(static_cast<size_t>(every) * std::max(1, t.G)) != PK_OK)
+        {
+            // no room: assemble again in every reverse step instead
+            (void)hipGetLastError();
+            s->segVals.free();
+            s->segMinv.free();
+            s->segFailed.free();
+            store = false;
+        }
+    }
+    if (store)
+    {
+        if (is_f32(s) && s->segColumns != s->epoch)
+        {
+            for (int j = 0; j < every; ++j)
+            {
+                s->valsView = s->segVals.p + j * valsBytes;
+                launch_record_columns(s->view());
+            }
+            s->valsView = nullptr;
+            s->segColumns = s->epoch;
+        }
+    }
+    auto slot = [&](int j) {
+        s->valsView = store ? s->segVals.p + j * valsBytes : nullptr;
+        s->minvView = store ? s->segMinv.p + j * minvBytes : nullptr;
+    };
     const int nseg = (steps + every - 1) / every;
     for (int seg = nseg - 1; seg >= 0; --seg)
     {
@@ -1104,15 +1174,23 @@ static int backward_run(pk_system* s, double adjTol, int adjMaxIt, int* adjIters
         if ((rc = dcopy(s, s->segV.p, s->v.p, n3)) != PK_OK) return rc;
         for (int j = 0; j < len; ++j)
         {
-            if ((rc = replay_substep<M>(s, dt, s->segOk.p + static_cast<size_t>(j) * t.G)) != PK_OK)
+            slot(j);
+            rc = replay_substep<M>(s, dt, s->segOk.p + static_cast<size_t>(j) * t.G,
+                                   store ? s->segFailed.p + static_cast<size_t>(j) * t.G : nullptr);
+            s->valsView = s->minvView = nullptr;
+            if (rc != PK_OK)
                 return rc;
             if ((rc = dcopy(s, s->segX.p + (j + 1) * n3, s->x.p, n3)) != PK_OK) return rc;
             if ((rc = dcopy(s, s->segV.p + (j + 1) * n3, s->v.p, n3)) != PK_OK) return rc;
         }
         for (int j = len - 1; j >= 0; --j)
-            if ((rc = reverse_substep<M>(s, dt, j, adjTol, adjMaxIt, adjIters ? adjIters + n0 + j : nullptr)) !=
-                PK_OK)
+        {
+            slot(j);
+            rc = reverse_substep<M>(s, dt, j, adjTol, adjMaxIt, adjIters ? adjIters + n0 + j : nullptr);
+            s->valsView = s->minvView = nullptr;
+            if (rc != PK_OK)
                 return rc;
+        }
     }
     if ((rc = dcopy(s, s->x.p, s->xEnd.p, n3)) != PK_OK) return rc;
     if ((rc = dcopy(s, s->v.p, s->vEnd.p, n3)) != PK_OK) return rc;
@@ -1228,6 +1306,21 @@ int pk_system_profile(pk_system* s, int enable)
     s->prof.spmvBlocks = s->prof.spmvUpper = s->prof.spmvRows = s->prof.spmvBodyLaunches = 0;
     s->prof.recs.clear();
     s->prof.used = 0;
+    return PK_OK;
+}
+
+int pk_system_set_matrix_store(pk_system* s, long long max_bytes)
+{
+    if (!s || max_bytes < 0)
+        return PK_ERR_ARG;
+    s->storeCap = max_bytes;
+    if (max_bytes == 0)
+    {
+        s->segVals.free();
+        s->segMinv.free();
+        s->segFailed.free();
+        s->segColumns = -1;
+    }
     return PK_OK;
 }
 

@@ -68,6 +68,7 @@ PK_SIGNATURES = {
     "pk_system_profile": (C.c_int, [_vp, C.c_int]),
     "pk_system_kernel_time": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
     "pk_system_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "pk_system_set_matrix_store": (C.c_int, [_vp, C.c_longlong]),
     "pk_system_spmv_work": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                       C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "pk_system_assemble": (C.c_int, [_vp, C.c_double]),

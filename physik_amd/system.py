@@ -150,6 +150,11 @@ class System:
         N.check(self.lib.pk_system_kernel_time(self.h, kernel, C.byref(n), C.byref(ms)), "pk_system_kernel_time")
         return n.value, ms.value
 
+    def set_matrix_store(self, max_bytes):
+        """Bytes the reverse mode may keep for a checkpoint segment's matrices
+        (default 8 GiB); 0 re-assembles every reverse substep instead."""
+        N.check(self.lib.pk_system_set_matrix_store(self.h, int(max_bytes)), "pk_system_set_matrix_store")
+
     def set_graphs(self, enable=True):
         """Replay each solve's CG from captured HIP graphs, four iterations
         per replay (identical results to launching each kernel)."""

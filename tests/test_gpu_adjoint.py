@@ -238,3 +238,29 @@ def test_diff16_gradients_match_the_cpu_oracle_adjoint():
     assert rel(x, xo) <= 1e-11
     assert rel(gE, oE) <= 1e-8 and rel(gv, ov) <= 1e-8 and rel(gx, ox) <= 1e-8, (rel(gE, oE), rel(gv, ov), rel(gx, ox))
     assert np.abs(its - oits).max() <= 1
+
+
+@pytest.mark.parametrize("precision", [N.PK_FP64, N.PK_FP32_STORAGE, N.PK_FP32_UPPER])
+def test_segment_matrix_store_gives_identical_gradients(precision):
+    # the reverse steps solve with the matrices the replay kept, or assemble
+    # them again (store off): the same matrices, so bit-identical gradients;
+    # 7 substeps checkpointed every 3 leave a shorter last segment
+    w = W.batch128(bodies=3, lo=3, hi=5, h=0.1, steps=7)
+    w.young = w.young / 10.0
+    print(f"case matrix store, precision {precision}", flush=True)
+    out = []
+    for cap, graphs in ((0, False), (1 << 30, False), (1 << 30, True), (1 << 30, False)):
+        s = W.make_system(w, precision)
+        s.tune(8, 32, 64, 2)
+        s.set_matrix_store(cap)
+        s.set_graphs(graphs)
+        s.record(w.dt, w.steps, 3)
+        x, _ = s.state()
+        _, xbar = W.loss_and_seed(w, x)
+        out.append(s.backward(xbar, None, tol=w.adj_tol, max_iter=w.max_iter))
+        out.append(s.backward(xbar, None, tol=w.adj_tol, max_iter=w.max_iter))  # the kept buffers, reused
+        s.close()
+    for other in out[1:]:
+        for u, v in zip(out[0], other):
+            assert np.array_equal(u, v)
+
