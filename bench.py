@@ -8,7 +8,8 @@
 Default: the batched differentiable workload (128 bodies, 3.2 M tets, FP32
 storage of the matrix's upper triangle with FP64 dots and residual), forward
 + adjoint; beside it the same run with the full matrix stored (identical
-results, the full-matrix SpMV). A step is one
+results, the full-matrix SpMV). FP64 workloads also store the upper triangle
+by default (--precision fp64 for full storage). A step is one
 implicit substep forward (element corotation, assembly, block-Jacobi PCG to
 rtol 1e-6, integration) and its reverse (checkpoint-segment replay, adjoint
 PCG to rtol 1e-7, corotational adjoint with dR/dF) over every body; K steps
@@ -58,7 +59,10 @@ DESCRIPTION = {
                 "one random face fixed, v0 ~ N(0, 0.5 m/s), dt=1/60, rtol 1e-6, one CG per body",
 }
 DIFFERENTIABLE = ("batch128", "diff32")
-DEFAULT_PRECISION = {"batch128": "fp32u"}
+# FP32 storage where the workload names it, FP64 elsewhere; both with the
+# upper triangle stored (results identical to full storage, fewer bytes)
+DEFAULT_PRECISION = {"batch128": "fp32u", "twist64": "fp64u", "batch256": "fp64u", "diff32": "fp64u",
+                     "beam10": "fp64u"}
 DEFAULT_STEPS = {"batch128": 40, "diff32": 50, "twist64": 50, "beam10": 100, "batch256": 100}
 
 
@@ -75,13 +79,14 @@ def spmv_bytes(blocks, rows, body_launches, s, upper_blocks=None):
     int32 column), per row its int32 row pointer, one d entry read and one q
     entry written (3 values each), and the closing row pointer of each body in
     each launch. Upper-only storage (upper_blocks = the stored blocks among
-    them): every stored block once (its record also serves the transposed
-    product of its mirror row), an 8-byte (slot, column) entry per lower
-    block, per row an 8-byte pair of row pointers and d, q. One launch over a
-    whole system is spmv_bytes(nnzb_full, nd, bodies, s[, nnzb_upper])."""
+    them): every stored record once (10 words of s bytes: 9 values and the
+    column; it also serves the transposed product of its mirror row), an
+    8-byte (slot, column) entry per lower block, per row an 8-byte pair of row
+    pointers and d, q. One launch over a whole system is
+    spmv_bytes(nnzb_full, nd, bodies, s[, nnzb_upper])."""
     if upper_blocks is None:
         return blocks * (9 * s + 4) + rows * (4 + 2 * 3 * s) + 4 * body_launches
-    return upper_blocks * (9 * s + 4) + (blocks - upper_blocks) * 8 + rows * (8 + 2 * 3 * s) + 8 * body_launches
+    return upper_blocks * 10 * s + (blocks - upper_blocks) * 8 + rows * (8 + 2 * 3 * s) + 8 * body_launches
 
 
 def measured_traffic(workload_name, precision):
@@ -104,8 +109,8 @@ def roofline(s, name, prec_name):
     ended, so late launches move less) over their summed in-kernel spans."""
     from physik_amd import native as N
     n_spmv, ms_spmv = s.kernel_time(N.K_SPMV)
-    esize = 8 if s.precision == N.PK_FP64 else 4
-    upper = s.precision == N.PK_FP32_UPPER
+    esize = 8 if s.precision in (N.PK_FP64, N.PK_FP64_UPPER) else 4
+    upper = s.precision in (N.PK_FP32_UPPER, N.PK_FP64_UPPER)
     blocks, ublocks, rows, body_launches = s.spmv_work()
     total = spmv_bytes(blocks, rows, body_launches, esize, ublocks if upper else None)
     full = spmv_bytes(s.nnzb_full, s.nd, s.groups, esize, s.nnzb_upper if upper else None)
@@ -141,7 +146,7 @@ def prepare(name, prec_name, args):
     from physik_amd import native as N
     from physik_amd import workloads as W
     w = workload(name)
-    prec = {"fp64": N.PK_FP64, "fp32": N.PK_FP32_STORAGE, "fp32u": N.PK_FP32_UPPER}[prec_name]
+    prec = {"fp64": N.PK_FP64, "fp32": N.PK_FP32_STORAGE, "fp32u": N.PK_FP32_UPPER, "fp64u": N.PK_FP64_UPPER}[prec_name]
     s = W.make_system(w, prec)
     if args.shape:
         s.tune(*[int(a) for a in args.shape.split(",")])
@@ -332,8 +337,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batch128", choices=list(DESCRIPTION))
-    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "fp32u"],
-                    help="fp32: FP32 storage of the full matrix; fp32u: of its upper triangle only")
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "fp32u", "fp64u"],
+                    help="fp32: FP32 storage of the full matrix; fp32u, fp64u: of its upper triangle only")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain single-body step beside the headline")
     ap.add_argument("--no-full-storage", action="store_true",
                     help="skip the full-matrix FP32 run beside an upper-triangle (fp32u) headline")
@@ -365,7 +370,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32 storage, f64 dots",
+        "dtype": {"fp64": "f64", "fp64u": "f64 (upper triangle stored)", "fp32": "f32 storage, f64 dots",
                   "fp32u": "f32 storage (upper triangle), f64 dots"}[prec_name],
         "data": "synthetic (seeded meshes, no checkpoint files)",
         "config": {"workload": head["workload"], "mode": "forward+adjoint" if diff else "forward",
@@ -382,7 +387,7 @@ def main():
         line["full_storage_step"] = {k: full[k] for k in ("value", "ms_per_step", "precision", "cg_launch_shape",
                                                           "roofline", "kernels_ms")}
     if not args.no_plain and name != "twist64":
-        plain, _ = run_plain("twist64", "fp64", DEFAULT_STEPS["twist64"], args.warmup, args)
+        plain, _ = run_plain("twist64", DEFAULT_PRECISION["twist64"], DEFAULT_STEPS["twist64"], args.warmup, args)
         line["plain_step"] = plain
         line["vs_plain_step"] = head["value"] / plain["value"]
     if not args.no_cpu_baseline:

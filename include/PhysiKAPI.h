@@ -138,7 +138,8 @@ extern "C"
 
     /* Storage precision of the GPU system: 0 = FP64 (default), 1 = FP32
      * storage with FP64 dots and residual update, 2 = the same with only the
-     * upper triangle of the matrix stored (identical results to 1). */
+     * upper triangle of the matrix stored (identical results to 1), 3 = FP64
+     * with only the upper triangle stored (identical results to 0). */
     PHYSIK_API int PHYSIK_AMD_SetPrecision(PhysiK::WorldHandle world, int precision);
     /* PHYSIK_Step that returns a status instead of throwing: 0 ok, 3 when a
      * free node has no mass (the reference throws), 1 bad handle, 2 GPU error. */

@@ -47,10 +47,12 @@ enum pk_precision {
     PK_FP64 = 0,          /* everything in FP64 */
     PK_FP32_STORAGE = 1,  /* A, M^-1 and the SpMV vectors stored in FP32; dots,
                              residual update, solution and state in FP64 */
-    PK_FP32_UPPER = 2     /* as PK_FP32_STORAGE, with A stored as its upper
+    PK_FP32_UPPER = 2,    /* as PK_FP32_STORAGE, with A stored as its upper
                              triangle only (each lower block read as the
                              transpose of its mirror); results identical to
                              PK_FP32_STORAGE, about 35 % fewer SpMV bytes */
+    PK_FP64_UPPER = 3     /* as PK_FP64, with A stored as its upper triangle
+                             only; results identical to PK_FP64 */
 };
 
 /* Per-group step outcome (pk_system_group_results). */

@@ -806,7 +806,7 @@ void PHYSIK_SetNodeVelocity(PhysiK::WorldHandle world, int i, float x, float y, 
 int PHYSIK_AMD_SetPrecision(PhysiK::WorldHandle world, int precision)
 {
     World* w = W(world);
-    if (!w || (precision != PK_FP64 && precision != PK_FP32_STORAGE && precision != PK_FP32_UPPER))
+    if (!w || precision < PK_FP64 || precision > PK_FP64_UPPER)
         return 0;
     if (w->precision != precision)
         w->topoDirty = true;

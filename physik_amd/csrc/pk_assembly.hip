@@ -366,7 +366,7 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
     const int full = upperOnly ? s : uFull[s];
 #pragma unroll
     for (int e = 0; e < 9; ++e)
-        vals[BsrStore<M>::at(F, full, e)] = static_cast<M>(a[e]);
+        vals[upperOnly ? upper_at(full, e) : BsrStore<M>::at(F, full, e)] = static_cast<M>(a[e]);
     if (mirror >= 0 && !upperOnly)
     {
 #pragma unroll

@@ -321,23 +321,42 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
         pa[t] = pd;
 }
 
-// Block at position p of a row's full-order block list in upper-only FP32
+// Block at position p of a row's full-order block list in upper-only
 // storage: a lower position (p < nl) is the transpose of the upper record
 // lower[lb + p].x, with column lower[lb + p].y; an upper position is record
 // ub + p - nl, with its column in word 9. The transpose is taken by selecting
 // elements, so the products formed from `a` are exactly those of full storage.
-__device__ inline void load_upper_block(const float* __restrict__ rec, const int2* __restrict__ lower, int lb, int ub,
-                                        int nl, int p, float* a, int& col)
+__device__ inline void load_record(const float* __restrict__ rec, int slot, float* w, int& col)
+{
+    const float2* r2 = reinterpret_cast<const float2*>(rec) + 5 * static_cast<size_t>(slot);
+    const float2 w0 = r2[0], w1 = r2[1], w2 = r2[2], w3 = r2[3], w4 = r2[4];
+    w[0] = w0.x; w[1] = w0.y; w[2] = w1.x; w[3] = w1.y; w[4] = w2.x;
+    w[5] = w2.y; w[6] = w3.x; w[7] = w3.y; w[8] = w4.x;
+    col = __float_as_int(w4.y);
+}
+
+__device__ inline void load_record(const double* __restrict__ rec, int slot, double* w, int& col)
+{
+    const double2* r2 = reinterpret_cast<const double2*>(rec) + 5 * static_cast<size_t>(slot);
+    const double2 w0 = r2[0], w1 = r2[1], w2 = r2[2], w3 = r2[3], w4 = r2[4];
+    w[0] = w0.x; w[1] = w0.y; w[2] = w1.x; w[3] = w1.y; w[4] = w2.x;
+    w[5] = w2.y; w[6] = w3.x; w[7] = w3.y; w[8] = w4.x;
+    col = static_cast<int>(__double_as_longlong(w4.y));
+}
+
+template <class M>
+__device__ inline void load_upper_block(const M* __restrict__ rec, const int2* __restrict__ lower, int lb, int ub,
+                                        int nl, int p, M* a, int& col)
 {
     const bool tr = p < nl;
     int2 e = make_int2(0, 0);
     if (tr)
         e = lower[lb + p];
     const int slot = tr ? e.x : ub + p - nl;
-    const float2* r2 = reinterpret_cast<const float2*>(rec) + 5 * static_cast<size_t>(slot);
-    const float2 w0 = r2[0], w1 = r2[1], w2 = r2[2], w3 = r2[3], w4 = r2[4];
-    const float w[9] = {w0.x, w0.y, w1.x, w1.y, w2.x, w2.y, w3.x, w3.y, w4.x};
-    col = tr ? e.y : __float_as_int(w4.y);
+    M w[9];
+    int rc;
+    load_record(rec, slot, w, rc);
+    col = tr ? e.y : rc;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -351,10 +370,10 @@ __device__ inline void load_upper_block(const float* __restrict__ rec, const int
 // a layer of rows later on the same XCD, so mostly from L2). (Issuing the
 // loads of two positions per row before their use was slower: the extra
 // registers cost more occupancy than the overlap gained.)
-template <int LPR, int U>
+template <class M, int LPR, int U>
 __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __restrict__ rowptr,
-                                       const int2* __restrict__ lower, const float* __restrict__ rec,
-                                       const float* __restrict__ d, float* __restrict__ q, double* __restrict__ pa,
+                                       const int2* __restrict__ lower, const M* __restrict__ rec,
+                                       const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
                                        const GroupState* gs)
 {
     int t;
@@ -386,7 +405,7 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
             ke[u] = (r1.x - r0.x) + (r1.y - r0.y);
         }
         int col[U];
-        float a[U][9];
+        M a[U][9];
 #pragma unroll
         for (int u = 0; u < U; ++u)
         {
@@ -412,7 +431,7 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
         for (int u = 0; u < U; ++u)
             for (int kk = kb[u] + LPR; kk < ke[u]; kk += LPR)
             {
-                float b[9];
+                M b[9];
                 int c;
                 load_upper_block(rec, lower, lb[u], ub[u], nl[u], kk, b, c);
                 const double d0 = static_cast<double>(d[3 * c]), d1 = static_cast<double>(d[3 * c + 1]),
@@ -435,8 +454,7 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
             const int row = row0 + u * RPP;
             if (lane == 0 && row < tl.rowEnd)
             {
-                const float q0 = static_cast<float>(y[u][0]), q1 = static_cast<float>(y[u][1]),
-                            q2 = static_cast<float>(y[u][2]);
+                const M q0 = static_cast<M>(y[u][0]), q1 = static_cast<M>(y[u][1]), q2 = static_cast<M>(y[u][2]);
                 q[3 * row] = q0;
                 q[3 * row + 1] = q1;
                 q[3 * row + 2] = q2;
@@ -452,10 +470,10 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
         pa[t] = pd;
 }
 
-template <int LPR, int U>
+template <class M, int LPR, int U>
 __global__ __launch_bounds__(NT) void k_spmv_upper(const int* kb, int k0, TileView st, const int2* __restrict__ rowptr,
-                                                   const int2* __restrict__ lower, const float* __restrict__ rec,
-                                                   const float* __restrict__ d, float* __restrict__ q,
+                                                   const int2* __restrict__ lower, const M* __restrict__ rec,
+                                                   const M* __restrict__ d, M* __restrict__ q,
                                                    double* __restrict__ pa, const GroupState* gs,
                                                    unsigned long long* stamps)
 {
@@ -463,7 +481,7 @@ __global__ __launch_bounds__(NT) void k_spmv_upper(const int* kb, int k0, TileVi
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    spmv_body_upper<LPR, U>(k, st, rowptr, lower, rec, d, q, pa, gs);
+    spmv_body_upper<M, LPR, U>(k, st, rowptr, lower, rec, d, q, pa, gs);
     stamp_end(stamp);
 }
 
@@ -625,9 +643,9 @@ __global__ __launch_bounds__(NT) void k_spmv_plain(int nd, const int* __restrict
 
 // y = A x over upper-only FP32 storage, one thread per row, blocks in
 // full-row order (the same sums as k_spmv_plain over full storage).
+template <class M>
 __global__ __launch_bounds__(NT) void k_spmv_plain_upper(int nd, const int2* __restrict__ rowptr,
-                                                         const int2* __restrict__ lower,
-                                                         const float* __restrict__ rec,
+                                                         const int2* __restrict__ lower, const M* __restrict__ rec,
                                                          const double* __restrict__ xin, double* __restrict__ y)
 {
     const int row = blockIdx.x * NT + threadIdx.x;
@@ -638,7 +656,7 @@ __global__ __launch_bounds__(NT) void k_spmv_plain_upper(int nd, const int2* __r
     double acc[3] = {0, 0, 0};
     for (int p = 0; p < n; ++p)
     {
-        float a[9];
+        M a[9];
         int c;
         load_upper_block(rec, lower, r0.x, r0.y, nl, p, a, c);
 #pragma unroll
@@ -651,12 +669,12 @@ __global__ __launch_bounds__(NT) void k_spmv_plain_upper(int nd, const int2* __r
         y[3 * row + i] = acc[i];
 }
 
-template <int LPR, int U>
+template <class M, int LPR, int U>
 static void launch_spmv_upper_shape(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp)
 {
-    hipLaunchKernelGGL((k_spmv_upper<LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), 0, v.stream, kb,
-                       k0, v.spmv, v.rowptr, v.lower, static_cast<const float*>(v.vals),
-                       static_cast<const float*>(v.d[k0 & 1]), static_cast<float*>(v.q), v.pa, v.gs, stamp);
+    hipLaunchKernelGGL((k_spmv_upper<M, LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), 0, v.stream,
+                       kb, k0, v.spmv, v.rowptr, v.lower, static_cast<const M*>(v.vals),
+                       static_cast<const M*>(v.d[k0 & 1]), static_cast<M*>(v.q), v.pa, v.gs, stamp);
 }
 
 // The iteration of a launch is k0 (kb == nullptr) or *kb + k0 with *kb even,
@@ -684,24 +702,21 @@ void launch_spmv(const DeviceView& v, const int* kb, int k0, int lanesPerRow, in
 {
     if (v.spmv.count == 0)
         return;
-    if constexpr (std::is_same<M, float>::value)
+    if (v.upper)
     {
-        if (v.upper)
-        {
 #define PK_SHAPE(L, U)                                      \
     if (lanesPerRow == L && unroll == U)                    \
     {                                                       \
-        launch_spmv_upper_shape<L, U>(v, kb, k0, stamp);    \
+        launch_spmv_upper_shape<M, L, U>(v, kb, k0, stamp); \
         return;                                             \
     }
-            PK_SHAPE(4, 1) PK_SHAPE(4, 2) PK_SHAPE(4, 4)
-            PK_SHAPE(8, 1) PK_SHAPE(8, 2) PK_SHAPE(8, 4)
-            PK_SHAPE(16, 1) PK_SHAPE(16, 2) PK_SHAPE(16, 4)
-            PK_SHAPE(32, 1) PK_SHAPE(32, 2)
+        PK_SHAPE(4, 1) PK_SHAPE(4, 2) PK_SHAPE(4, 4)
+        PK_SHAPE(8, 1) PK_SHAPE(8, 2) PK_SHAPE(8, 4)
+        PK_SHAPE(16, 1) PK_SHAPE(16, 2) PK_SHAPE(16, 4)
+        PK_SHAPE(32, 1) PK_SHAPE(32, 2)
 #undef PK_SHAPE
-            launch_spmv_upper_shape<16, 1>(v, kb, k0, stamp);
-            return;
-        }
+        launch_spmv_upper_shape<M, 16, 1>(v, kb, k0, stamp);
+        return;
     }
 #define PK_SHAPE(L, U)                        \
     if (lanesPerRow == L && unroll == U)      \
@@ -840,8 +855,8 @@ void launch_spmv_plain(const DeviceView& v, const double* x, double* y)
     if (v.upper)
     {
         if (v.nd > 0)
-            hipLaunchKernelGGL(k_spmv_plain_upper, dim3((v.nd + NT - 1) / NT), dim3(NT), 0, v.stream, v.nd, v.rowptr,
-                               v.lower, static_cast<const float*>(v.vals), x, y);
+            hipLaunchKernelGGL(k_spmv_plain_upper<M>, dim3((v.nd + NT - 1) / NT), dim3(NT), 0, v.stream, v.nd,
+                               v.rowptr, v.lower, static_cast<const M*>(v.vals), x, y);
         return;
     }
     if (v.nd > 0)
@@ -863,12 +878,25 @@ __global__ __launch_bounds__(NT) void k_record_columns(long long F, const int* _
         vals[BsrStore<float>::at(static_cast<size_t>(F), static_cast<size_t>(k), 9)] = __int_as_float(fcol[k]);
 }
 
-void launch_record_columns(const DeviceView& v)
+__global__ __launch_bounds__(NT) void k_record_columns64(long long U, const int* __restrict__ ucol,
+                                                         double* __restrict__ vals)
+{
+    const long long k = static_cast<long long>(blockIdx.x) * NT + threadIdx.x;
+    if (k < U)
+        vals[upper_at(static_cast<size_t>(k), 9)] = __longlong_as_double(static_cast<long long>(ucol[k]));
+}
+
+void launch_record_columns(const DeviceView& v, bool f32)
 {
     const long long n = v.upper ? v.nnzu : v.nnzf;
-    if (n > 0)
-        hipLaunchKernelGGL(k_record_columns, dim3(static_cast<unsigned>((n + NT - 1) / NT)), dim3(NT), 0, v.stream, n,
-                           v.upper ? v.ucol : v.fcol, static_cast<float*>(v.vals));
+    if (n <= 0)
+        return;
+    const dim3 grid(static_cast<unsigned>((n + NT - 1) / NT));
+    if (f32)
+        hipLaunchKernelGGL(k_record_columns, grid, dim3(NT), 0, v.stream, n, v.upper ? v.ucol : v.fcol,
+                           static_cast<float*>(v.vals));
+    else if (v.upper)
+        hipLaunchKernelGGL(k_record_columns64, grid, dim3(NT), 0, v.stream, n, v.ucol, static_cast<double*>(v.vals));
 }
 
 template void launch_spmv_plain<double>(const DeviceView&, const double*, double*);

@@ -90,6 +90,11 @@ template <> struct BsrStore<float>
     __host__ __device__ static size_t at(size_t, size_t k, int e) { return 10 * k + static_cast<size_t>(e); }
 };
 
+// Upper-only storage, both precisions: one record of 10 words per upper slot,
+// the 9 values then the column (int bits of a float, or int64 bits of a
+// double), so a transposed read of a mirror is five 8- or 16-byte loads.
+__host__ __device__ inline size_t upper_at(size_t k, int e) { return 10 * k + static_cast<size_t>(e); }
+
 __device__ inline int tile_of(const TileView& tv)
 {
     const int b = static_cast<int>(blockIdx.x);
@@ -321,9 +326,10 @@ void launch_compact(const DeviceView& v, RowTile* vecPerm, RowTile* spmvPerm);
 void launch_stamp_reduce(hipStream_t stream, const unsigned long long* raw, int grid, int launches,
                          unsigned long long* span);
 template <class M> void launch_spmv_plain(const DeviceView& v, const double* x, double* y);
-// FP32 storage: writes each block's column into word 9 of its record (one
-// record per full block, or per upper slot for upper-only storage).
-void launch_record_columns(const DeviceView& v);
+// Record storage (FP32, or upper-only in either precision): writes each
+// block's column into word 9 of its record (one record per full block, or
+// per upper slot for upper-only storage).
+void launch_record_columns(const DeviceView& v, bool f32);
 
 // Reverse mode (pk_adjoint.hip). `ok` holds one flag per body: the forward
 // substep integrated. The CG of A lam = u runs through launch_spmv/update.

@@ -234,7 +234,7 @@ struct pk_system
         dv.slotRow = slotRow.p;
         dv.rowGroup = rowGroup.p;
         dv.fstart = fstart.p;
-        dv.upper = precision == PK_FP32_UPPER ? 1 : 0;
+        dv.upper = (precision == PK_FP32_UPPER || precision == PK_FP64_UPPER) ? 1 : 0;
         dv.rowptr = rowptr.p;
         dv.lower = lower.p;
         dv.ucol = ucol.p;
@@ -664,7 +664,9 @@ static int choose_lanes(const Topology& t)
 }
 
 // A, M^-1 and the SpMV vectors in FP32 (full or upper-only storage).
-static bool is_f32(const pk_system* s) { return s->precision != PK_FP64; }
+static bool is_f32(const pk_system* s) { return s->precision == PK_FP32_STORAGE || s->precision == PK_FP32_UPPER; }
+// A stored as its upper triangle, one record per slot.
+static bool is_upper(const pk_system* s) { return s->precision == PK_FP32_UPPER || s->precision == PK_FP64_UPPER; }
 
 static int alloc_work(pk_system* s)
 {
@@ -674,8 +676,8 @@ static int alloc_work(pk_system* s)
     int rc;
     if ((rc = s->basis.alloc(static_cast<size_t>(t.T) * EB_COUNT)) != PK_OK) return rc;
     if ((rc = s->corner.alloc(t.nlist.size() * N_CORNER)) != PK_OK) return rc;
-    const size_t words = is_f32(s) ? BsrStore<float>::kWords : BsrStore<double>::kWords;
-    const size_t blocks = static_cast<size_t>(s->precision == PK_FP32_UPPER ? t.nnzu() : t.nnzf());
+    const size_t words = (is_f32(s) || is_upper(s)) ? 10 : BsrStore<double>::kWords;
+    const size_t blocks = static_cast<size_t>(is_upper(s) ? t.nnzu() : t.nnzf());
     if ((rc = s->vals.alloc(blocks * words * es)) != PK_OK) return rc;
     if ((rc = s->z.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->d0.alloc(nd3 * es)) != PK_OK) return rc;
@@ -758,7 +760,7 @@ static int setup_device(pk_system* s)
     if ((rc = s->nodeRow.upload(t.dyn, st)) != PK_OK) return rc;
     if ((rc = s->nodeGroup.upload(t.nodeGroup, st)) != PK_OK) return rc;
     if ((rc = s->cpos.upload(t.cpos, st)) != PK_OK) return rc;
-    if (s->precision == PK_FP32_UPPER)
+    if (is_upper(s))
     {
         // row r's lower blocks (c < r) are the compact entries
         // [fstart[r] - ustart[r], fstart[r+1] - ustart[r+1]); the one at full
@@ -795,8 +797,8 @@ static int setup_device(pk_system* s)
     }
     if ((rc = alloc_work(s)) != PK_OK) return rc;
     launch_gdot(s->view(), s->prm.p);
-    if (is_f32(s))
-        launch_record_columns(s->view());
+    if (is_f32(s) || is_upper(s))
+        launch_record_columns(s->view(), is_f32(s));
     std::vector<double> zero(static_cast<size_t>(t.N) * 3, 0.0);
     if ((rc = s->v.upload(zero, st)) != PK_OK) return rc;
     if ((rc = s->x.alloc(static_cast<size_t>(t.N) * 3)) != PK_OK) return rc;
@@ -823,7 +825,7 @@ int pk_device_count(void)
 
 int pk_system_create(const pk_mesh_desc* desc, int precision, pk_system** out)
 {
-    if (!desc || !out || (precision != PK_FP64 && precision != PK_FP32_STORAGE && precision != PK_FP32_UPPER))
+    if (!desc || !out || precision < PK_FP64 || precision > PK_FP64_UPPER)
     {
         set_error("pk_system_create: bad argument");
         return PK_ERR_ARG;
@@ -1149,12 +1151,12 @@ static int backward_run(pk_system* s, double adjTol, int adjMaxIt, int* adjIters
     }
     if (store)
     {
-        if (is_f32(s) && s->segColumns != s->epoch)
+        if ((is_f32(s) || is_upper(s)) && s->segColumns != s->epoch)
         {
             for (int j = 0; j < every; ++j)
             {
                 s->valsView = s->segVals.p + j * valsBytes;
-                launch_record_columns(s->view());
+                launch_record_columns(s->view(), is_f32(s));
             }
             s->valsView = nullptr;
             s->segColumns = s->epoch;
@@ -1378,16 +1380,21 @@ int pk_system_get_matrix(pk_system* s, double* full_vals, double* rhs, double* m
     const Topology& t = s->topo;
     const size_t F = static_cast<size_t>(t.nnzf());
     PK_HIP(hipStreamSynchronize(s->stream));
-    if (full_vals && F && s->precision == PK_FP32_UPPER)
+    if (full_vals && F && is_upper(s))
     {
         // one record per upper slot: the block at its full index, its
         // transpose at the mirror
         const size_t U = static_cast<size_t>(t.nnzu());
-        std::vector<float> raw(U * BsrStore<float>::kWords);
-        PK_HIP(hipMemcpy(raw.data(), s->vals.p, raw.size() * sizeof(float), hipMemcpyDeviceToHost));
+        const bool f32 = is_f32(s);
+        std::vector<char> bytes(U * 10 * (f32 ? 4 : 8));
+        PK_HIP(hipMemcpy(bytes.data(), s->vals.p, bytes.size(), hipMemcpyDeviceToHost));
+        std::vector<double> raw(U * 10);
+        for (size_t i = 0; i < raw.size(); ++i)
+            raw[i] = f32 ? reinterpret_cast<const float*>(bytes.data())[i]
+                         : reinterpret_cast<const double*>(bytes.data())[i];
         for (size_t u = 0; u < U; ++u)
         {
-            const float* w = raw.data() + BsrStore<float>::at(U, u, 0);
+            const double* w = raw.data() + upper_at(u, 0);
             const size_t full = static_cast<size_t>(t.uFull[u]);
             for (int e = 0; e < 9; ++e)
                 full_vals[9 * full + e] = w[e];

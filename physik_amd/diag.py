@@ -25,7 +25,7 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=512)
     ap.add_argument("--workload", default="twist64", choices=["twist64", "beam10", "batch256", "diff32", "batch128"])
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp32u"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp32u", "fp64u"])
     ap.add_argument("--shape", default=None,
                     help="CG launch shape lanes,spmv_rows,vector_rows,unroll (default: the system's own)")
     args = ap.parse_args()
@@ -34,7 +34,8 @@ def main():
     nbytes = args.copy_mb * (1 << 20)
     N.check(lib.pk_diag_copy(nbytes, 3, C.byref(ms)), "pk_diag_copy")
     w = getattr(W, args.workload)()
-    s = W.make_system(w, {"fp64": N.PK_FP64, "fp32": N.PK_FP32_STORAGE, "fp32u": N.PK_FP32_UPPER}[args.precision])
+    s = W.make_system(w, {"fp64": N.PK_FP64, "fp32": N.PK_FP32_STORAGE, "fp32u": N.PK_FP32_UPPER,
+                          "fp64u": N.PK_FP64_UPPER}[args.precision])
     if args.shape:
         s.tune(*[int(a) for a in args.shape.split(",")])
     s.profile(True)

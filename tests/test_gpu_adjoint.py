@@ -240,7 +240,7 @@ def test_diff16_gradients_match_the_cpu_oracle_adjoint():
     assert np.abs(its - oits).max() <= 1
 
 
-@pytest.mark.parametrize("precision", [N.PK_FP64, N.PK_FP32_STORAGE, N.PK_FP32_UPPER])
+@pytest.mark.parametrize("precision", [N.PK_FP64, N.PK_FP32_STORAGE, N.PK_FP32_UPPER, N.PK_FP64_UPPER])
 def test_segment_matrix_store_gives_identical_gradients(precision):
     # the reverse steps solve with the matrices the replay kept, or assemble
     # them again (store off): the same matrices, so bit-identical gradients;

@@ -1,9 +1,10 @@
-"""Upper-only FP32 storage (PK_FP32_UPPER) against full FP32 storage.
+"""Upper-only storage (PK_FP32_UPPER, PK_FP64_UPPER) against full storage of
+the same precision.
 
 Each lower block is read as the transpose of its upper mirror, with the same
 lanes, products and reduction order, so every bar here is bit-identity with
-PK_FP32_STORAGE: matrix, SpMV, states, iteration counts, residuals and
-gradients. Full FP32 storage is itself held to the FP64 oracle
+PK_FP32_STORAGE (PK_FP64): matrix, SpMV, states, iteration counts, residuals
+and gradients. Full storage is itself held to the FP64 oracle
 (tests/test_gpu_step.py, tests/test_gpu_adjoint.py).
 """
 
@@ -30,9 +31,12 @@ def twisted(n=(8, 5, 5), seed=12):
     return W._single("upper", rest, tets, 1e5, 0.4, 1000.0, fixed, x0, np.zeros_like(rest), 1.0 / 60.0, 6, 1e-7, 500)
 
 
-def pair(w, shape=None):
-    a = W.make_system(w, N.PK_FP32_STORAGE)
-    b = W.make_system(w, N.PK_FP32_UPPER)
+PAIRS = [(N.PK_FP32_STORAGE, N.PK_FP32_UPPER), (N.PK_FP64, N.PK_FP64_UPPER)]
+
+
+def pair(w, shape=None, precisions=PAIRS[0]):
+    a = W.make_system(w, precisions[0])
+    b = W.make_system(w, precisions[1])
     if shape:
         a.tune(*shape)
         b.tune(*shape)
@@ -51,10 +55,11 @@ def same_steps(a, b, w, steps):
             assert np.array_equal(u, v), f"step {n} results"
 
 
-def test_upper_matrix_and_spmv_equal_full_storage():
+@pytest.mark.parametrize("precisions", PAIRS)
+def test_upper_matrix_and_spmv_equal_full_storage(precisions):
     w = twisted()
-    print("case matrix and spmv", flush=True)
-    a, b = pair(w)
+    print(f"case matrix and spmv {precisions}", flush=True)
+    a, b = pair(w, precisions=precisions)
     assert b.nnzb_upper < b.nnzb_full
     a.assemble(w.dt)
     b.assemble(w.dt)
@@ -64,25 +69,28 @@ def test_upper_matrix_and_spmv_equal_full_storage():
     assert np.array_equal(a.spmv(x), b.spmv(x))
 
 
-def test_upper_single_body_steps_equal_full_storage():
+@pytest.mark.parametrize("precisions", PAIRS)
+def test_upper_single_body_steps_equal_full_storage(precisions):
     w = twisted()
-    print("case single body", flush=True)
-    a, b = pair(w)
+    print(f"case single body {precisions}", flush=True)
+    a, b = pair(w, precisions=precisions)
     same_steps(a, b, w, w.steps)
 
 
+@pytest.mark.parametrize("precisions", PAIRS)
 @pytest.mark.parametrize("lanes,unroll", SHAPES)
-def test_every_upper_launch_shape_equals_full_storage(lanes, unroll):
+def test_every_upper_launch_shape_equals_full_storage(lanes, unroll, precisions):
     w = twisted(n=(6, 4, 4), seed=13)
-    print(f"case shape {lanes},{unroll}", flush=True)
-    a, b = pair(w, (lanes, 32, 64, unroll))
+    print(f"case shape {lanes},{unroll} {precisions}", flush=True)
+    a, b = pair(w, (lanes, 32, 64, unroll), precisions)
     same_steps(a, b, w, 2)
 
 
-def test_upper_batch_steps_equal_full_storage_and_count_stored_blocks():
+@pytest.mark.parametrize("precisions", PAIRS)
+def test_upper_batch_steps_equal_full_storage_and_count_stored_blocks(precisions):
     w = W.batch128(bodies=10, lo=3, hi=8, h=0.1, steps=4)
-    print("case batch", flush=True)
-    a, b = pair(w, (8, 32, 64, 2))
+    print(f"case batch {precisions}", flush=True)
+    a, b = pair(w, (8, 32, 64, 2), precisions)
     a.profile(True)
     b.profile(True)
     same_steps(a, b, w, w.steps)
@@ -94,12 +102,13 @@ def test_upper_batch_steps_equal_full_storage_and_count_stored_blocks():
     assert (upper - rows) * 2 + rows == blocks
 
 
-def test_upper_graphs_and_adjoint_equal_full_storage():
+@pytest.mark.parametrize("precisions", PAIRS)
+def test_upper_graphs_and_adjoint_equal_full_storage(precisions):
     w = W.batch128(bodies=3, lo=3, hi=5, h=0.1, steps=5)
     w.young = w.young / 10.0
-    print("case graphs and adjoint", flush=True)
+    print(f"case graphs and adjoint {precisions}", flush=True)
     out = []
-    for precision in (N.PK_FP32_STORAGE, N.PK_FP32_UPPER):
+    for precision in precisions:
         s = W.make_system(w, precision)
         s.tune(16, 32, 64, 2)
         s.set_graphs(True)
