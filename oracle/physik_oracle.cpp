@@ -654,6 +654,60 @@ struct World
         M3<T> k;
     };
 
+    // Transient point connections (World.cpp:126-131, PhysiKAPI.cpp:1441-1475):
+    // a spring from the barycentric point of four nodes to a target, added
+    // before a frame and applied in every substep of it, then cleared
+    // (World.cpp:36-72, 727-745).
+    struct Conn
+    {
+        int n[4];
+        T w[4];
+        V3<T> target;
+        T k, c;
+    };
+    std::vector<Conn> conns;
+
+    bool conn_nodes_valid(const Conn& c) const  // World.cpp:748-760
+    {
+        for (int a = 0; a < 4; ++a)
+            if (c.n[a] < 0 || c.n[a] >= N || !nodeActive[c.n[a]])
+                return false;
+        return true;
+    }
+
+    // PointConnection::UpdateSystem (PointConnection.cpp:54-130): forces and
+    // k w_a w_b I blocks pushed after the components' and gravity's
+    void push_connections(std::vector<std::pair<int, V3<T>>>& forcePush, std::vector<Block>& blocks) const
+    {
+        for (const Conn& c : conns)
+        {
+            if (!conn_nodes_valid(c) || !finite(c.k) || !finite(c.c) || c.k < T(0) || c.c < T(0))
+                continue;
+            const V3<T> point =
+                add(add(add(scl(x[c.n[0]], c.w[0]), scl(x[c.n[1]], c.w[1])), scl(x[c.n[2]], c.w[2])),
+                    scl(x[c.n[3]], c.w[3]));
+            const V3<T> vel =
+                add(add(add(scl(v[c.n[0]], c.w[0]), scl(v[c.n[1]], c.w[1])), scl(v[c.n[2]], c.w[2])),
+                    scl(v[c.n[3]], c.w[3]));
+            if (!finite(point) || !finite(vel) || !finite(c.target))
+                continue;
+            const V3<T> f = add(scl(sub(c.target, point), c.k), scl(vel, -c.c));
+            for (int a = 0; a < 4; ++a)
+                if (c.w[a] != T(0))
+                    forcePush.push_back({c.n[a], scl(f, c.w[a])});
+            if (c.k <= T(0))
+                continue;
+            for (int a = 0; a < 4; ++a)
+            {
+                if (c.w[a] == T(0))
+                    continue;
+                for (int b = 0; b < 4; ++b)
+                    if (c.w[b] != T(0))
+                        blocks.push_back({c.n[a], c.n[b], diag3(c.k * c.w[a] * c.w[b])});
+            }
+        }
+    }
+
     void rebuild_rest(const std::vector<V3<T>>& rest)
     {
         cache.resize(tets.size());
@@ -784,6 +838,8 @@ struct World
                 continue;
             forcePush.push_back({i, scl(gravity, mass[i])});
         }
+        // World::AssembleConnectionSystems (World.cpp:602-611)
+        push_connections(forcePush, blocks);
         // SolverData::PrecomputeImplicitSolve (:126-167)
         force.assign(static_cast<size_t>(N), V3<T>{});
         for (const auto& f : forcePush)
@@ -960,8 +1016,13 @@ struct World
     // reference throws std::runtime_error from here).
     int step(T frameDt)
     {
-        if (frameDt < T(0) || frameDt == T(0))
+        if (frameDt < T(0))
             return 0;
+        if (frameDt == T(0))
+        {
+            conns.clear();  // World.cpp:41-46
+            return 0;
+        }
         const int steps = std::max(1, substeps);
         const T dt = frameDt / static_cast<T>(steps);
         for (int i = 0; i < steps; ++i)
@@ -984,6 +1045,7 @@ struct World
                 integrate_explicit(dt);
             }
         }
+        conns.clear();  // World::ClearFrameConnections (World.cpp:70, 742-746)
         return 0;
     }
 };
@@ -1371,6 +1433,36 @@ void* orc_create(int precision, int node_count, const double* rest, int tet_coun
 }
 
 void orc_destroy(void* p) { delete static_cast<Handle*>(p); }
+
+// PHYSIK_AddPointConnection (PhysiKAPI.cpp:1441-1475): kept only when all
+// four nodes exist and are active (World.cpp:126-131); applied in every
+// substep of the next frame.
+void orc_add_point_connection(void* p, const int* nodes, const double* weights, const double* target,
+                              double stiffness, double damping)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        typename std::remove_reference<decltype(w)>::type::Conn c;
+        for (int a = 0; a < 4; ++a)
+        {
+            c.n[a] = nodes[a];
+            c.w[a] = static_cast<T>(weights[a]);
+        }
+        c.target = oracle::V3<T>{static_cast<T>(target[0]), static_cast<T>(target[1]), static_cast<T>(target[2])};
+        c.k = static_cast<T>(stiffness);
+        c.c = static_cast<T>(damping);
+        if (w.conn_nodes_valid(c))
+            w.conns.push_back(c);
+    });
+}
+
+int orc_point_connection_count(void* p)
+{
+    auto* h = static_cast<Handle*>(p);
+    int n = 0;
+    ORC_DISPATCH(h, { n = static_cast<int>(w.conns.size()); });
+    return n;
+}
 
 void orc_set_params(void* p, double gx, double gy, double gz, int substeps, double tol, int max_iter, int jacobi,
                     int implicit)

@@ -181,6 +181,24 @@ void ref_get_state(void* p, float* x, float* v)
     }
 }
 
+// PHYSIK_AddPointConnection with the component's local node ids mapped to
+// world nodes (PHYSIK_GetTetMeshGlobalNodeIndex).
+void ref_add_point_connection(void* p, const int* nodes, const float* w, const float* target, float stiffness,
+                              float damping)
+{
+    auto* h = static_cast<RefHandle*>(p);
+    int g[4];
+    for (int a = 0; a < 4; ++a)
+        g[a] = PHYSIK_GetTetMeshGlobalNodeIndex(h->world, h->comp, nodes[a]);
+    PHYSIK_AddPointConnection(h->world, g[0], g[1], g[2], g[3], w[0], w[1], w[2], w[3], target[0], target[1],
+                              target[2], stiffness, damping);
+}
+
+int ref_point_connection_count(void* p)
+{
+    return PHYSIK_GetPointConnectionCount(static_cast<RefHandle*>(p)->world);
+}
+
 // PHYSIK_Step; returns 1 when the reference throws (World.cpp:576).
 int ref_step(void* p, float dt)
 {

@@ -65,6 +65,9 @@ def oracle_lib():
         lib.orc_adjoint.restype = C.c_int
         lib.orc_polar_vjp.argtypes = [_dp, _dp, _dp, _dp]
         lib.orc_polar_vjp.restype = C.c_int
+        lib.orc_add_point_connection.argtypes = [C.c_void_p, _ip, _dp, _dp, C.c_double, C.c_double]
+        lib.orc_point_connection_count.argtypes = [C.c_void_p]
+        lib.orc_point_connection_count.restype = C.c_int
         _oracle = lib
     return _oracle
 
@@ -99,6 +102,9 @@ def ref_lib():
         lib.ref_get_elements.argtypes = [C.c_void_p, _bp, _fp, _fp]
         lib.ref_get_rest.argtypes = [C.c_void_p, _fp, _fp, _fp]
         lib.ref_spmv.argtypes = [C.c_int, _ip, _ip, _fp, _fp, _fp]
+        lib.ref_add_point_connection.argtypes = [C.c_void_p, _ip, _fp, _fp, C.c_float, C.c_float]
+        lib.ref_point_connection_count.argtypes = [C.c_void_p]
+        lib.ref_point_connection_count.restype = C.c_int
         _ref = lib
     return _ref
 
@@ -165,6 +171,15 @@ class Oracle:
 
     def step(self, dt):
         return self.lib.orc_step(self.h, dt)
+
+    def add_point_connection(self, nodes, weights, target, stiffness, damping):
+        """A spring from the barycentric point of four nodes to a target for
+        every substep of the next step (PHYSIK_AddPointConnection)."""
+        self.lib.orc_add_point_connection(self.h, _c(nodes, np.int32), _c(weights, np.float64),
+                                          _c(target, np.float64), float(stiffness), float(damping))
+
+    def point_connection_count(self):
+        return self.lib.orc_point_connection_count(self.h)
 
     def cg(self):
         it, res, conv = C.c_int(), C.c_double(), C.c_int()
@@ -305,6 +320,13 @@ class Reference:
 
     def step(self, dt):
         return self.lib.ref_step(self.h, dt)
+
+    def add_point_connection(self, nodes, weights, target, stiffness, damping):
+        self.lib.ref_add_point_connection(self.h, _c(nodes, np.int32), _c(weights, np.float32),
+                                          _c(target, np.float32), float(stiffness), float(damping))
+
+    def point_connection_count(self):
+        return self.lib.ref_point_connection_count(self.h)
 
     def cg(self):
         it, res, conv = C.c_int(), C.c_float(), C.c_int()

@@ -200,7 +200,69 @@ def small_cases():
     print("small cases:", list(cases))
 
 
+def connection_frames(rest, tets, frames):
+    """Per-frame point connections (PHYSIK_AddPointConnection) for the
+    connections fixture: a point inside one tet pulled toward a moving target,
+    a pair of nodes of another tet plus a zero-weight node, and a spring over
+    nodes far apart in the mesh (outside the FEM pattern), one of them fixed,
+    with a repeated node."""
+    t0, t1 = tets[10], tets[30]
+    out = []
+    for f in range(frames):
+        s = 0.01 * f
+        out.append([
+            (list(t0), [0.1, 0.2, 0.3, 0.4], list(rest[t0].mean(0) + [0.0, s, 0.05]), 500.0, 2.0),
+            ([int(t1[0]), int(t1[1]), 3, 0], [0.5, 0.5, 0.0, 0.0], list(rest[t1[0]] + [0.02, 0.0, -s]), 800.0, 0.0),
+            ([5, len(rest) - 1, 7, 7], [0.25, 0.25, 0.3, 0.2], [0.2 + s, 0.15, 0.1], 300.0, 1.0),
+        ])
+    return out
+
+
+def connections():
+    """A five-tet beam stepped with point connections added before every
+    frame, implicit (2 substeps per frame) and explicit."""
+    n = (4, 3, 3)
+    rest, tets = mg.box_mesh(*n, 0.1, "five")
+    fixed = np.zeros(len(rest), np.uint8)
+    fixed[mg.face_nodes(*n, 0)] = 1
+    frames = 8
+    conn = connection_frames(rest, tets, frames)
+    out = {}
+    for mode, implicit, dt in (("implicit", True, 1.0 / 60.0), ("explicit", False, 1.0 / 2000.0)):
+        ref = Reference(rest, tets, 1000.0, 1e5, 0.3, 0.2)
+        ref.set_params(G, 2, 1e-6, 200, implicit=implicit)
+        ref.set_fixed(fixed)
+        xs, vs, its, counts = [], [], [], []
+        for f in range(frames):
+            for nodes, w, tgt, k, c in conn[f]:
+                ref.add_point_connection(nodes, w, tgt, k, c)
+            counts.append(ref.point_connection_count())
+            assert ref.step(np.float32(dt)) == 0
+            x, v = ref.state()
+            xs.append(x)
+            vs.append(v)
+            its.append(ref.cg()[0])
+        assert ref.point_connection_count() == 0
+        ref.close()
+        out[f"{mode}_x"] = np.array(xs)
+        out[f"{mode}_v"] = np.array(vs)
+        out[f"{mode}_cg_iterations"] = np.array(its, np.int32)
+        out[f"{mode}_counts"] = np.array(counts, np.int32)
+        out[f"{mode}_dt"] = np.float32(dt)
+    out["rest"] = f32(rest)
+    out["tets"] = tets.astype(np.int32)
+    out["fixed"] = fixed
+    out["nodes"] = np.array([[c[0] for c in fr] for fr in conn], np.int32)
+    out["weights"] = f32([[c[1] for c in fr] for fr in conn])
+    out["targets"] = f32([[c[2] for c in fr] for fr in conn])
+    out["stiffness"] = f32([[c[3] for c in fr] for fr in conn])
+    out["damping"] = f32([[c[4] for c in fr] for fr in conn])
+    np.savez_compressed(os.path.join(HERE, "connections_fp32.npz"), **out)
+    print("connections:", out["implicit_cg_iterations"], out["implicit_counts"])
+
+
 if __name__ == "__main__":
     config1()
     elements()
     small_cases()
+    connections()

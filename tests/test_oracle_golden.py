@@ -95,3 +95,27 @@ def test_small_cases_bitwise(golden_dir, name):
         assert conv == bool(c["cg_converged"][s])
         assert np.array_equal(x.astype(np.float32), c["x"][s]), f"{name}: positions differ at step {s}"
         assert np.array_equal(v.astype(np.float32), c["v"][s]), f"{name}: velocities differ at step {s}"
+
+
+@pytest.mark.parametrize("mode", ["implicit", "explicit"])
+def test_point_connections_bitwise(golden_dir, mode):
+    # PHYSIK_AddPointConnection before every frame (PointConnection.cpp:54-130,
+    # World.cpp:32-73): forces and k w_a w_b I blocks, including a spring over
+    # nodes outside the FEM pattern, a fixed node and a repeated node
+    z = load(golden_dir, "connections_fp32.npz")
+    o = Oracle(32, z["rest"].astype(np.float64), z["tets"], 1e5, 0.3, 0.2, 1000.0)
+    o.set_params((0.0, 0.0, -9.81), 2, 1e-6, 200, implicit=mode == "implicit")
+    o.set_fixed(z["fixed"])
+    for f in range(len(z[f"{mode}_x"])):
+        for c in range(z["nodes"].shape[1]):
+            o.add_point_connection(z["nodes"][f, c], z["weights"][f, c].astype(np.float64),
+                                   z["targets"][f, c].astype(np.float64), float(z["stiffness"][f, c]),
+                                   float(z["damping"][f, c]))
+        assert o.point_connection_count() == z[f"{mode}_counts"][f]
+        assert o.step(float(z[f"{mode}_dt"])) == 0
+        assert o.point_connection_count() == 0
+        x, v = o.state()
+        if mode == "implicit":
+            assert o.cg()[0] == z["implicit_cg_iterations"][f]
+        assert np.array_equal(x.astype(np.float32), z[f"{mode}_x"][f]), f"positions differ at frame {f}"
+        assert np.array_equal(v.astype(np.float32), z[f"{mode}_v"][f]), f"velocities differ at frame {f}"
