@@ -125,6 +125,11 @@ extern "C"
                                          int tetIndex);
     PHYSIK_API int PHYSIK_GetActiveTetCount(PhysiK::WorldHandle world,                     /* :332 */
                                             PhysiK::ComponentHandle component);
+    PHYSIK_API int PHYSIK_GetPointConnectionCount(PhysiK::WorldHandle world);             /* :353 */
+    PHYSIK_API void PHYSIK_AddPointConnection(PhysiK::WorldHandle world, int node0, int node1, /* :355 */
+                                              int node2, int node3, float barycentricX, float barycentricY,
+                                              float barycentricZ, float barycentricW, float targetX,
+                                              float targetY, float targetZ, float stiffness, float damping);
     PHYSIK_API void PHYSIK_GetNodePosition(PhysiK::WorldHandle world, int nodeIndex,       /* :371 */
                                            float* outX, float* outY, float* outZ);
     PHYSIK_API void PHYSIK_GetNodeVelocity(PhysiK::WorldHandle world, int nodeIndex,       /* :378 */

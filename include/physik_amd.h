@@ -91,6 +91,10 @@ typedef struct pk_mesh_desc {
     const unsigned char* node_active;/* node_count, or NULL for all active */
     const int* node_group;           /* node_count non-decreasing group ids starting at 0,
                                         or NULL for a single group */
+    int extra_pair_count;            /* node pairs added to the BSR pattern beside the
+                                        tets' (for point connections between nodes no
+                                        tet joins, SolverData.cpp:313-401) */
+    const int* extra_pairs;          /* extra_pair_count x 2 node ids, or NULL */
 } pk_mesh_desc;
 
 /* Message of the last failed call on this thread. */
@@ -139,6 +143,21 @@ int pk_system_step(pk_system* sys, double dt, int substeps, int frames);
 /* Outcome of the last substep, one entry per group (any pointer may be NULL). */
 int pk_system_group_results(pk_system* sys, int* iterations, double* residual_norm, int* converged,
                             int* status);
+
+/* Transient point springs (PHYSIK_AddPointConnection, PointConnection.cpp:
+ * 54-130): `count` connections, each pulling the barycentric point
+ * sum_a weights[4i+a] x[nodes[4i+a]] toward targets[3i..3i+2] with stiffness
+ * k and damping c (force k (target - point) - c (point velocity), spread to
+ * the nodes by weight; k w_a w_b I blocks into the implicit matrix). They act
+ * in every substep of every following pk_system_step until replaced; count 0
+ * clears them. As in the reference, a connection is ignored when a node is out
+ * of range or inactive, k or c is negative or non-finite, or the target is
+ * non-finite. Every pair of dynamic nodes a connection couples must lie in the
+ * system's BSR pattern (pk_mesh_desc.extra_pairs adds pairs): PK_ERR_ARG
+ * otherwise, with the connections left as they were. The reverse mode does
+ * not cover connections: pk_system_record fails while any is set. */
+int pk_system_set_connections(pk_system* sys, int count, const int* nodes, const double* weights,
+                              const double* targets, const double* stiffness, const double* damping);
 
 /* Reverse-mode memory (no reference counterpart). pk_system_backward replays
  * each checkpoint segment forward and then reverses it substep by substep;

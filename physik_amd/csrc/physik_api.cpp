@@ -13,6 +13,7 @@
 #include <string>
 #include <unordered_map>
 #include <unordered_set>
+#include <set>
 #include <vector>
 
 #include "PhysiKAPI.h"
@@ -171,6 +172,17 @@ struct World
     bool lastConverged = false;
     int lastStatus = PK_GROUP_NO_DYNAMIC;
 
+    // point connections added for the next frame (World.cpp:126-131), and the
+    // node pairs they have needed in the BSR pattern beyond the tets' (the
+    // system is rebuilt with them, SolverData.cpp:389-405)
+    struct Conn
+    {
+        int n[4];
+        double w[4], target[3], k, c;
+    };
+    std::vector<Conn> conns;
+    std::set<std::pair<int, int>> extraPairs;
+
     pk_system* sys = nullptr;
     bool topoDirty = true;     // rebuild the device system before the next step
     bool hostNewer = false;    // host state changed since the last upload
@@ -278,6 +290,15 @@ struct World
         d.tet_active = tact.data();
         d.node_fixed = fixed.data();
         d.node_active = nodeActive.data();
+        std::vector<int> extra;
+        for (const auto& pr : extraPairs)
+            if (pr.first < N && pr.second < N)
+            {
+                extra.push_back(pr.first);
+                extra.push_back(pr.second);
+            }
+        d.extra_pair_count = static_cast<int>(extra.size() / 2);
+        d.extra_pairs = extra.empty() ? nullptr : extra.data();
         // Rest geometry comes from each component's own rest positions.
         std::vector<double> rest(x);
         for (const auto& cp : components)
@@ -301,10 +322,46 @@ struct World
         return PK_OK;
     }
 
+    // Hands the frame's connections to the system; when they couple nodes the
+    // pattern lacks, rebuilds it with those pairs first.
+    int apply_connections()
+    {
+        auto upload = [&]() {
+            std::vector<int> n;
+            std::vector<double> w, tg, k, c;
+            for (const Conn& cn : conns)
+            {
+                n.insert(n.end(), cn.n, cn.n + 4);
+                w.insert(w.end(), cn.w, cn.w + 4);
+                tg.insert(tg.end(), cn.target, cn.target + 3);
+                k.push_back(cn.k);
+                c.push_back(cn.c);
+            }
+            return pk_system_set_connections(sys, static_cast<int>(conns.size()), n.data(), w.data(), tg.data(),
+                                             k.data(), c.data());
+        };
+        int rc = upload();
+        if (rc != PK_ERR_ARG)
+            return rc;
+        for (const Conn& cn : conns)
+            for (int a = 0; a < 4; ++a)
+                for (int b = a + 1; b < 4; ++b)
+                    if (cn.w[a] != 0.0 && cn.w[b] != 0.0 && cn.n[a] != cn.n[b])
+                        extraPairs.insert({std::min(cn.n[a], cn.n[b]), std::max(cn.n[a], cn.n[b])});
+        if ((rc = rebuild()) != PK_OK)
+            return rc;
+        return upload();
+    }
+
     int step(float frameDt)  // World.cpp:32-73
     {
-        if (frameDt < 0.0f || frameDt == 0.0f)
+        if (frameDt < 0.0f)
             return PK_OK;
+        if (frameDt == 0.0f)
+        {
+            conns.clear();  // World.cpp:41-46
+            return PK_OK;
+        }
         if (topoDirty || !sys)
         {
             const int rc = rebuild();
@@ -318,6 +375,14 @@ struct World
                 return rc;
             hostNewer = false;
         }
+        // the connections first: a pattern they need rebuilds the system,
+        // which the settings below must then reach
+        if (!conns.empty())
+        {
+            const int rc = apply_connections();
+            if (rc != PK_OK)
+                return rc;
+        }
         const int steps = std::max(1, substeps);
         pk_system_set_gravity(sys, gravity[0], gravity[1], gravity[2]);
         pk_system_set_mode(sys, implicit ? 1 : 0);
@@ -325,8 +390,11 @@ struct World
         if (rc != PK_OK)
             return rc;
         rc = pk_system_step(sys, static_cast<double>(frameDt), steps, 1);
+        if (!conns.empty())
+            pk_system_set_connections(sys, 0, nullptr, nullptr, nullptr, nullptr, nullptr);
         if (rc != PK_OK)
-            return rc;
+            return rc;  // the reference leaves the frame's connections when it throws
+        conns.clear();  // World::ClearFrameConnections (World.cpp:70)
         deviceNewer = true;
         if (implicit)
         {
@@ -755,6 +823,39 @@ int PHYSIK_GetActiveTetCount(PhysiK::WorldHandle world, PhysiK::ComponentHandle 
     if (!c)
         return 0;
     return static_cast<int>(std::count(c->tetActive.begin(), c->tetActive.end(), 1));
+}
+
+int PHYSIK_GetPointConnectionCount(PhysiK::WorldHandle world)
+{
+    World* w = W(world);
+    return w ? static_cast<int>(w->conns.size()) : 0;
+}
+
+// PhysiKAPI.cpp:1441-1475 -> World::AddPointConnection (World.cpp:126-131):
+// kept only when all four nodes exist and are active.
+void PHYSIK_AddPointConnection(PhysiK::WorldHandle world, int node0, int node1, int node2, int node3,
+                               float barycentricX, float barycentricY, float barycentricZ, float barycentricW,
+                               float targetX, float targetY, float targetZ, float stiffness, float damping)
+{
+    World* w = W(world);
+    if (!w)
+        return;
+    const int n[4] = {node0, node1, node2, node3};
+    for (int a = 0; a < 4; ++a)
+        if (!valid_node(w, n[a]) || !w->nodeActive[n[a]])
+            return;
+    World::Conn c;
+    std::copy(n, n + 4, c.n);
+    c.w[0] = barycentricX;
+    c.w[1] = barycentricY;
+    c.w[2] = barycentricZ;
+    c.w[3] = barycentricW;
+    c.target[0] = targetX;
+    c.target[1] = targetY;
+    c.target[2] = targetZ;
+    c.k = stiffness;
+    c.c = damping;
+    w->conns.push_back(c);
 }
 
 void PHYSIK_GetNodePosition(PhysiK::WorldHandle world, int i, float* outX, float* outY, float* outZ)

@@ -307,7 +307,7 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
                                                  const double* __restrict__ prm, int T,
                                                  const double* __restrict__ mrow, double dt2, M* __restrict__ vals,
                                                  long long nnzf, M* __restrict__ minv, int nd, GroupState* gs,
-                                                 int upperOnly)
+                                                 int upperOnly, const double* __restrict__ cslot)
 {
     const int s = (xcd_block() * NT + static_cast<int>(threadIdx.x)) / kSlotLanes;
     const int lane = threadIdx.x % kSlotLanes;
@@ -348,6 +348,14 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
     }
     if (lane != 0)
         return;
+    if (cslot)
+    {
+        // point connections' k w_a w_b I blocks (PointConnection.cpp:110-128)
+        const double cs = cslot[s];
+        a[0] += cs;
+        a[4] += cs;
+        a[8] += cs;
+    }
 #pragma unroll
     for (int e = 0; e < 9; ++e)
         a[e] *= dt2;
@@ -424,7 +432,8 @@ __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__
                                             const double* __restrict__ mrow, double gx, double gy, double gz,
                                             double dt, double dt2, const M* __restrict__ minv, int nd,
                                             double* __restrict__ xs, double* __restrict__ rv, M* __restrict__ zv,
-                                            M* __restrict__ d0, Partial* __restrict__ part)
+                                            M* __restrict__ d0, Partial* __restrict__ part,
+                                            const double* __restrict__ crhs)
 {
     const int t = tile_of(tv);
     if (t < 0)
@@ -471,7 +480,7 @@ __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__
         double b[3], z[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            b[i] = (f[i] + gv[i] * m) * dt - kv[i] * dt2;
+            b[i] = (f[i] + gv[i] * m) * dt - kv[i] * dt2 + (crhs ? crhs[3 * row + i] : 0.0);
         if (!(fin(b[0]) && fin(b[1]) && fin(b[2])))
             flag |= 1;
         apply_minv(minv, static_cast<size_t>(nd), row, b, z);
@@ -539,7 +548,7 @@ __global__ __launch_bounds__(NT) void k_explicit(TileView tv, const int* __restr
                                                  const int* __restrict__ nstart, const double* __restrict__ corner,
                                                  const double* __restrict__ mrow,
                                                  double gx, double gy, double gz, double dt, double* __restrict__ x,
-                                                 double* __restrict__ v)
+                                                 double* __restrict__ v, const double* __restrict__ crhs)
 {
     const int t = tile_of(tv);
     if (t < 0)
@@ -558,6 +567,12 @@ __global__ __launch_bounds__(NT) void k_explicit(TileView tv, const int* __restr
                 f[1] += e1;
                 f[2] += e2;
             }
+        }
+        if (crhs)
+        {
+            f[0] += crhs[3 * row];
+            f[1] += crhs[3 * row + 1];
+            f[2] += crhs[3 * row + 2];
         }
         const double m = mrow[row];
         const double gv[3] = {gx, gy, gz};
@@ -596,7 +611,8 @@ void launch_assemble(const DeviceView& v, double dt)
                            dim3(NT), 0, v.stream, n, v.cstart,
                            v.clist,
                            v.uFull, v.uMirror, v.slotRow, v.rowGroup, v.basis, v.prm, v.T, v.mrow, dt * dt,
-                           static_cast<M*>(v.vals), v.nnzf, static_cast<M*>(v.minv), v.nd, v.gs, v.upper);
+                           static_cast<M*>(v.vals), v.nnzf, static_cast<M*>(v.minv), v.nd, v.gs, v.upper,
+                           v.nconn > 0 ? v.cslot : nullptr);
 }
 
 template <class M>
@@ -605,7 +621,7 @@ void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt)
     if (v.vec.count > 0)
         hipLaunchKernelGGL((k_rhs<M>), dim3(grid_of(v.vec)), dim3(NT), 0, v.stream, v.vec, v.nstart,
                            v.corner, v.mrow, gx, gy, gz, dt, dt * dt, static_cast<const M*>(v.minv), v.nd, v.xs, v.r, static_cast<M*>(v.z),
-                           static_cast<M*>(v.d[0]), v.pb[1]);
+                           static_cast<M*>(v.d[0]), v.pb[1], v.nconn > 0 ? v.crhs : nullptr);
 }
 
 void launch_integrate(const DeviceView& v, double dt, int commit)
@@ -619,7 +635,111 @@ void launch_explicit(const DeviceView& v, double gx, double gy, double gz, doubl
 {
     if (v.vec.count > 0)
         hipLaunchKernelGGL(k_explicit, dim3(grid_of(v.vec)), dim3(NT), 0, v.stream, v.vec, v.rowNode, v.nstart,
-                           v.corner, v.mrow, gx, gy, gz, dt, v.x, v.v);
+                           v.corner, v.mrow, gx, gy, gz, dt, v.x, v.v, v.nconn > 0 ? v.crhs : nullptr);
+}
+
+// Point connections of a substep (PointConnection.cpp:54-130): the
+// barycentric point's position and velocity and the force
+// k (target - point) + (-c) velocity, in the reference's order of terms. A
+// connection whose point, velocity or target is not finite pushes no force
+// (ok = 0; its blocks stay in the matrix).
+__global__ __launch_bounds__(NT) void k_conn_eval(int n, const int4* __restrict__ nodes,
+                                                  const double* __restrict__ data, const double* __restrict__ x,
+                                                  const double* __restrict__ v, double* __restrict__ eval)
+{
+    const int c = blockIdx.x * NT + threadIdx.x;
+    if (c >= n)
+        return;
+    const int4 nd = nodes[c];
+    const double* d = data + 9 * static_cast<size_t>(c);
+    double p[3], u[3];
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        p[i] = ((x[3 * nd.x + i] * d[0] + x[3 * nd.y + i] * d[1]) + x[3 * nd.z + i] * d[2]) + x[3 * nd.w + i] * d[3];
+        u[i] = ((v[3 * nd.x + i] * d[0] + v[3 * nd.y + i] * d[1]) + v[3 * nd.z + i] * d[2]) + v[3 * nd.w + i] * d[3];
+        ok = ok && fin(p[i]) && fin(u[i]) && fin(d[4 + i]);
+    }
+    double* e = eval + 7 * static_cast<size_t>(c);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        e[i] = ok ? (d[4 + i] - p[i]) * d[7] + u[i] * (-d[8]) : 0.0;
+        e[3 + i] = ok ? u[i] : 0.0;
+    }
+    e[6] = ok ? 1.0 : 0.0;
+}
+
+// Each touched dynamic row's share of its connections, summed in connection
+// order: w_a (dt F - dt^2 k u) for the implicit right-hand side (the force
+// and the -dt^2 K v of the k w_a w_b I blocks over all four nodes, with u the
+// point velocity), w_a F as a force for the explicit step.
+__global__ __launch_bounds__(NT) void k_conn_rows(int n, const int* __restrict__ rows, const int* __restrict__ start,
+                                                  const int2* __restrict__ entries, const double* __restrict__ data,
+                                                  const double* __restrict__ eval, double dt, double dt2, int implicit,
+                                                  double* __restrict__ crhs)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= n)
+        return;
+    double acc[3] = {0, 0, 0};
+    for (int k = start[i]; k < start[i + 1]; ++k)
+    {
+        const int2 en = entries[k];
+        const double w = data[9 * static_cast<size_t>(en.x) + en.y];
+        const double kc = data[9 * static_cast<size_t>(en.x) + 7];
+        const double* e = eval + 7 * static_cast<size_t>(en.x);
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            acc[j] += implicit ? w * (dt * e[j] - dt2 * kc * e[3 + j]) : w * e[j];
+    }
+    const int r = rows[i];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        crhs[3 * r + j] = acc[j];
+}
+
+__global__ __launch_bounds__(NT) void k_slot_scatter(int n, const int* __restrict__ idx, const double* __restrict__ val,
+                                                     double* __restrict__ cslot)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i < n)
+        cslot[idx[i]] = val ? val[i] : 0.0;
+}
+
+__global__ __launch_bounds__(NT) void k_zero_rows(int n, const int* __restrict__ rows, double* __restrict__ crhs)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i < n)
+    {
+        const int r = rows[i];
+        crhs[3 * r] = crhs[3 * r + 1] = crhs[3 * r + 2] = 0.0;
+    }
+}
+
+void launch_connections(const DeviceView& v, double dt, int implicit)
+{
+    if (v.nconn <= 0)
+        return;
+    hipLaunchKernelGGL(k_conn_eval, dim3((v.nconn + NT - 1) / NT), dim3(NT), 0, v.stream, v.nconn, v.connNodes,
+                       v.connData, v.x, v.v, v.connEval);
+    if (v.nconnRows > 0)
+        hipLaunchKernelGGL(k_conn_rows, dim3((v.nconnRows + NT - 1) / NT), dim3(NT), 0, v.stream, v.nconnRows,
+                           v.connRows, v.connRowStart, v.connEntries, v.connData, v.connEval, dt, dt * dt, implicit,
+                           v.crhs);
+}
+
+void launch_slot_scatter(hipStream_t stream, int n, const int* idx, const double* val, double* cslot)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_slot_scatter, dim3((n + NT - 1) / NT), dim3(NT), 0, stream, n, idx, val, cslot);
+}
+
+void launch_zero_rows(hipStream_t stream, int n, const int* rows, double* crhs)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_zero_rows, dim3((n + NT - 1) / NT), dim3(NT), 0, stream, n, rows, crhs);
 }
 
 void launch_gdot(const DeviceView& v, double* prm)

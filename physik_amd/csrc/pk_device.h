@@ -287,6 +287,17 @@ struct DeviceView
     GroupState* gs;
     int* active;
     int* kbase;        // base iteration of a replayed CG chunk (zeroed by the reset)
+    // point connections (none when nconn == 0)
+    int nconn;
+    const int4* connNodes;
+    const double* connData;  // 9 per connection: weights (4), target (3), k, c
+    double* connEval;        // 7 per connection: force (3), point velocity (3), ok
+    int nconnRows;           // dynamic rows the connections touch
+    const int* connRows;
+    const int* connRowStart; // nconnRows + 1 offsets into connEntries
+    const int2* connEntries; // (connection, corner a) in connection order
+    double* crhs;            // nd x 3: the connections' right-hand side (or force) per row
+    const double* cslot;     // per upper slot: sum of k w_a w_b (null without connections)
     // reductions
     TileView vec, spmv;
     double* pa;        // per SpMV tile: d.q
@@ -306,6 +317,14 @@ template <class M> void launch_assemble(const DeviceView& v, double dt);
 template <class M> void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt);
 void launch_integrate(const DeviceView& v, double dt, int commit);
 void launch_explicit(const DeviceView& v, double gx, double gy, double gz, double dt);
+// Point connections: per-connection force and point velocity at the current
+// state, then each touched row's right-hand side terms (implicit) or force
+// (explicit) into crhs.
+void launch_connections(const DeviceView& v, double dt, int implicit);
+// cslot[idx[i]] = val ? val[i] : 0 for i < n.
+void launch_slot_scatter(hipStream_t stream, int n, const int* idx, const double* val, double* cslot);
+// crhs[3 r .. 3 r + 2] = 0 for the n rows listed.
+void launch_zero_rows(hipStream_t stream, int n, const int* rows, double* crhs);
 template <class M> void launch_dir(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp);
 template <class M>
 void launch_spmv(const DeviceView& v, const int* kb, int k0, int lanesPerRow, int unroll,

@@ -13,6 +13,8 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <map>
+#include <array>
 #include <vector>
 
 #include "physik_amd.h"
@@ -173,6 +175,14 @@ struct pk_system
     std::vector<long long> groupBlocks, groupUpper, groupRows;  // full and upper BSR blocks, dynamic rows per group
     pk::Profile prof;
 
+    // point connections in force (pk_system_set_connections): their data, the
+    // rows and upper slots they touch (kept to clear them when replaced)
+    int nconn = 0, nconnRows = 0, nslots = 0;
+    pk::DevBuf<int4> connNodes;
+    pk::DevBuf<double> connData, connEval, crhs, cslot, slotVal;
+    pk::DevBuf<int> connRows, connRowStart, slotIdx;
+    pk::DevBuf<int2> connEntries;
+
     // CG chunks replayed from captured HIP graphs: the first chunk of a solve
     // (full tiles) and every later one (compaction first, for a batch). A
     // capture bakes in buffer addresses and tile shapes: `epoch` changes when
@@ -264,6 +274,16 @@ struct pk_system
         dv.gs = gs.p;
         dv.active = active.p;
         dv.kbase = kbase.p;
+        dv.nconn = nconn;
+        dv.connNodes = connNodes.p;
+        dv.connData = connData.p;
+        dv.connEval = connEval.p;
+        dv.nconnRows = nconnRows;
+        dv.connRows = connRows.p;
+        dv.connRowStart = connRowStart.p;
+        dv.connEntries = connEntries.p;
+        dv.crhs = crhs.p;
+        dv.cslot = cslot.p;
         dv.vec = vecTiles.view();
         dv.spmv = spmvTiles.view();
         dv.pa = pa.p;
@@ -437,6 +457,7 @@ static int run_assembly(pk_system* s, double dt, double tol, int maxIt, bool wit
         Bracket br(s, PK_K_ELEMENT);
         launch_element(dv);
     }
+    launch_connections(dv, dt, s->implicit);
     if (s->topo.nd == 0)
         return PK_OK;
     {
@@ -1216,6 +1237,11 @@ int pk_system_record(pk_system* s, double dt, int steps, int every)
         set_error("pk_system_record: the reverse mode covers the implicit integrator only");
         return PK_ERR_ARG;
     }
+    if (s->nconn > 0)
+    {
+        set_error("pk_system_record: the reverse mode does not cover point connections");
+        return PK_ERR_ARG;
+    }
     if (s->topo.massErrorNode >= 0)
     {
         set_error("Non-fixed node has no physical mass contribution. nodeIndex=" +
@@ -1308,6 +1334,142 @@ int pk_system_profile(pk_system* s, int enable)
     s->prof.spmvBlocks = s->prof.spmvUpper = s->prof.spmvRows = s->prof.spmvBodyLaunches = 0;
     s->prof.recs.clear();
     s->prof.used = 0;
+    return PK_OK;
+}
+
+int pk_system_set_connections(pk_system* s, int count, const int* nodes, const double* weights,
+                              const double* targets, const double* stiffness, const double* damping)
+{
+    if (!s || count < 0 || (count > 0 && (!nodes || !weights || !targets || !stiffness || !damping)))
+    {
+        set_error("pk_system_set_connections: bad argument");
+        return PK_ERR_ARG;
+    }
+    const Topology& t = s->topo;
+    // the connections the reference keeps (World.cpp:126-131, 748-760;
+    // PointConnection.cpp:58-66, 81-84)
+    std::vector<int4> hn;
+    std::vector<double> hd;
+    for (int i = 0; i < count; ++i)
+    {
+        bool ok = std::isfinite(stiffness[i]) && std::isfinite(damping[i]) && stiffness[i] >= 0.0 &&
+                  damping[i] >= 0.0;
+        for (int a = 0; a < 4; ++a)
+        {
+            const int n = nodes[4 * i + a];
+            ok = ok && n >= 0 && n < t.N && t.nodeActive[n];
+        }
+        for (int j = 0; j < 3; ++j)
+            ok = ok && std::isfinite(targets[3 * i + j]);
+        if (!ok)
+            continue;
+        hn.push_back(make_int4(nodes[4 * i], nodes[4 * i + 1], nodes[4 * i + 2], nodes[4 * i + 3]));
+        for (int a = 0; a < 4; ++a)
+            hd.push_back(weights[4 * i + a]);
+        for (int j = 0; j < 3; ++j)
+            hd.push_back(targets[3 * i + j]);
+        hd.push_back(stiffness[i]);
+        hd.push_back(damping[i]);
+    }
+    const int nc = static_cast<int>(hn.size());
+    // the upper slots their k w_a w_b I blocks land in (each dynamic pair
+    // once, as the reference skips a lower block whose mirror is present,
+    // SolverData.cpp:440-446) and the rows their terms reach
+    std::map<int, double> slots;
+    std::vector<std::array<int, 3>> ent;  // (row, connection, a)
+    auto find_slot = [&](int r, int c) {
+        const auto b = t.ucol.begin() + t.ustart[r], e = t.ucol.begin() + t.ustart[r + 1];
+        const auto it = std::lower_bound(b, e, c);
+        return (it != e && *it == c) ? static_cast<int>(it - t.ucol.begin()) : -1;
+    };
+    for (int ci = 0; ci < nc; ++ci)
+    {
+        const int n4[4] = {hn[ci].x, hn[ci].y, hn[ci].z, hn[ci].w};
+        const double* d = hd.data() + 9 * static_cast<size_t>(ci);
+        for (int a = 0; a < 4; ++a)
+            if (d[a] != 0.0 && t.dyn[n4[a]] >= 0)
+                ent.push_back({t.dyn[n4[a]], ci, a});
+        if (d[7] <= 0.0)
+            continue;
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b)
+            {
+                if (d[a] == 0.0 || d[b] == 0.0)
+                    continue;
+                const int ra = t.dyn[n4[a]], rb = t.dyn[n4[b]];
+                if (ra < 0 || rb < 0 || ra > rb)
+                    continue;
+                const int sl = find_slot(ra, rb);
+                if (sl < 0)
+                {
+                    set_error("pk_system_set_connections: nodes " + std::to_string(n4[a]) + " and " +
+                              std::to_string(n4[b]) + " are not coupled in the BSR pattern (add the pair to "
+                              "pk_mesh_desc.extra_pairs)");
+                    return PK_ERR_ARG;
+                }
+                slots[sl] += d[7] * d[a] * d[b];
+            }
+    }
+    std::stable_sort(ent.begin(), ent.end(), [](const auto& p, const auto& q) { return p[0] < q[0]; });
+    std::vector<int> rows, rstart(1, 0);
+    std::vector<int2> entries;
+    for (size_t k = 0; k < ent.size(); ++k)
+    {
+        if (rows.empty() || rows.back() != ent[k][0])
+        {
+            if (!rows.empty())
+                rstart.push_back(static_cast<int>(entries.size()));
+            rows.push_back(ent[k][0]);
+        }
+        entries.push_back(make_int2(ent[k][1], ent[k][2]));
+    }
+    if (!rows.empty())
+        rstart.push_back(static_cast<int>(entries.size()));
+    std::vector<int> sidx;
+    std::vector<double> sval;
+    for (const auto& kv : slots)
+    {
+        sidx.push_back(kv.first);
+        sval.push_back(kv.second);
+    }
+
+    hipStream_t st = s->stream;
+    int rc;
+    // clear what the previous connections set
+    if (s->nslots > 0)
+        launch_slot_scatter(st, s->nslots, s->slotIdx.p, nullptr, s->cslot.p);
+    if (s->nconnRows > 0)
+        launch_zero_rows(st, s->nconnRows, s->connRows.p, s->crhs.p);
+    PK_HIP(hipGetLastError());
+    PK_HIP(hipStreamSynchronize(st));
+    s->nconn = s->nconnRows = s->nslots = 0;
+    if (nc == 0)
+        return PK_OK;
+    if (!s->cslot.p && t.nnzu() > 0)
+    {
+        if ((rc = s->cslot.alloc(static_cast<size_t>(t.nnzu()))) != PK_OK) return rc;
+        PK_HIP(hipMemsetAsync(s->cslot.p, 0, s->cslot.n * sizeof(double), st));
+    }
+    if (!s->crhs.p && t.nd > 0)
+    {
+        if ((rc = s->crhs.alloc(static_cast<size_t>(t.nd) * 3)) != PK_OK) return rc;
+        PK_HIP(hipMemsetAsync(s->crhs.p, 0, s->crhs.n * sizeof(double), st));
+    }
+    if ((rc = s->connNodes.upload(hn, st)) != PK_OK) return rc;
+    if ((rc = s->connData.upload(hd, st)) != PK_OK) return rc;
+    if ((rc = s->connEval.alloc(7 * static_cast<size_t>(nc))) != PK_OK) return rc;
+    if ((rc = s->connRows.upload(rows, st)) != PK_OK) return rc;
+    if ((rc = s->connRowStart.upload(rstart, st)) != PK_OK) return rc;
+    if ((rc = s->connEntries.upload(entries, st)) != PK_OK) return rc;
+    if ((rc = s->slotIdx.upload(sidx, st)) != PK_OK) return rc;
+    if ((rc = s->slotVal.upload(sval, st)) != PK_OK) return rc;
+    launch_slot_scatter(st, static_cast<int>(sidx.size()), s->slotIdx.p, s->slotVal.p, s->cslot.p);
+    PK_HIP(hipGetLastError());
+    PK_HIP(hipStreamSynchronize(st));
+    s->nconn = nc;
+    s->nconnRows = static_cast<int>(rows.size());
+    s->nslots = static_cast<int>(sidx.size());
+    s->tapeValid = false;
     return PK_OK;
 }
 

@@ -156,6 +156,20 @@ int build_topology(const pk_mesh_desc& d, Topology& topo, std::string& err, bool
 
     auto fixed = [&](int n) { return d.node_fixed && d.node_fixed[n]; };
     auto nodeActive = [&](int n) { return !d.node_active || d.node_active[n]; };
+    topo.nodeActive.resize(static_cast<size_t>(N));
+    for (int n = 0; n < N; ++n)
+        topo.nodeActive[n] = nodeActive(n) ? 1 : 0;
+    if (d.extra_pair_count < 0 || (d.extra_pair_count > 0 && !d.extra_pairs))
+    {
+        err = "mesh descriptor: bad extra pairs";
+        return PK_ERR_ARG;
+    }
+    for (int e = 0; e < 2 * d.extra_pair_count; ++e)
+        if (d.extra_pairs[e] < 0 || d.extra_pairs[e] >= N)
+        {
+            err = "extra pair " + std::to_string(e / 2) + " references a node out of range";
+            return PK_ERR_ARG;
+        }
     auto tetActive = [&](int t) { return !d.tet_active || d.tet_active[t]; };
 
     topo.live.assign(static_cast<size_t>(T), 0);
@@ -227,6 +241,12 @@ int build_topology(const pk_mesh_desc& d, Topology& topo, std::string& err, bool
                     ++cnt[std::min(da, db)];
             }
         }
+    for (int e = 0; e < d.extra_pair_count; ++e)
+    {
+        const int da = topo.dyn[d.extra_pairs[2 * e]], db = topo.dyn[d.extra_pairs[2 * e + 1]];
+        if (da >= 0 && db >= 0)
+            ++cnt[std::min(da, db)];
+    }
     std::vector<long long> raw_start(static_cast<size_t>(nd) + 1, 0);
     for (int r = 0; r < nd; ++r)
         raw_start[r + 1] = raw_start[r] + cnt[r];
@@ -247,6 +267,12 @@ int build_topology(const pk_mesh_desc& d, Topology& topo, std::string& err, bool
                     raw[fillp[std::min(da, db)]++] = std::max(da, db);
             }
         }
+    for (int e = 0; e < d.extra_pair_count; ++e)
+    {
+        const int da = topo.dyn[d.extra_pairs[2 * e]], db = topo.dyn[d.extra_pairs[2 * e + 1]];
+        if (da >= 0 && db >= 0)
+            raw[fillp[std::min(da, db)]++] = std::max(da, db);
+    }
     topo.ustart.assign(static_cast<size_t>(nd) + 1, 0);
     topo.ucol.clear();
     topo.ucol.reserve(raw.size() / 2 + 1);

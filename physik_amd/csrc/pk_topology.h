@@ -80,6 +80,7 @@ struct Topology
     std::vector<int> cpos;               // 4 T: row-list index of corner a of tet t, or -1
     std::vector<int> astart, alist;      // every node -> packed (tet, corner) over live tets, increasing tet
     std::vector<int> nodeGroup;          // N -> body
+    std::vector<uint8_t> nodeActive;     // N
     std::vector<int> scatter;            // optional T*16 upper slot of each local pair, -1 if not dynamic
 
     long long nnzu() const { return static_cast<long long>(ucol.size()); }

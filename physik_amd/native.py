@@ -30,6 +30,7 @@ class MeshDesc(C.Structure):
         ("node_count", C.c_int), ("rest", _vp), ("tet_count", C.c_int), ("tets", _vp),
         ("young", _vp), ("poisson", _vp), ("damping", _vp), ("density", _vp),
         ("tet_active", _vp), ("node_fixed", _vp), ("node_active", _vp), ("node_group", _vp),
+        ("extra_pair_count", C.c_int), ("extra_pairs", _vp),
     ]
 
 
@@ -68,6 +69,7 @@ PK_SIGNATURES = {
     "pk_system_profile": (C.c_int, [_vp, C.c_int]),
     "pk_system_kernel_time": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
     "pk_system_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "pk_system_set_connections": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "pk_system_set_matrix_store": (C.c_int, [_vp, C.c_longlong]),
     "pk_system_spmv_work": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                       C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
@@ -128,6 +130,9 @@ PHYSIK_SIGNATURES = {
     "PHYSIK_SetTetActive": (None, [_W, _H, C.c_int, C.c_int]),
     "PHYSIK_DeactivateTet": (None, [_W, _H, C.c_int]),
     "PHYSIK_GetActiveTetCount": (C.c_int, [_W, _H]),
+    "PHYSIK_GetPointConnectionCount": (C.c_int, [_W]),
+    "PHYSIK_AddPointConnection": (None, [_W, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                         C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]),
     "PHYSIK_GetNodePosition": (None, [_W, C.c_int, _fp, _fp, _fp]),
     "PHYSIK_GetNodeVelocity": (None, [_W, C.c_int, _fp, _fp, _fp]),
     "PHYSIK_SetNodePosition": (None, [_W, C.c_int, C.c_float, C.c_float, C.c_float]),

@@ -18,7 +18,7 @@ class _Desc:
     """Keeps the numpy arrays behind a MeshDesc alive."""
 
     def __init__(self, rest, tets, young, poisson, damping, density, tet_active=None, node_fixed=None,
-                 node_active=None, node_group=None):
+                 node_active=None, node_group=None, extra_pairs=None):
         self.rest = N.as_f64(rest).reshape(-1, 3)
         self.tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
         t = self.tets.shape[0]
@@ -31,9 +31,13 @@ class _Desc:
         u8 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.uint8)  # noqa: E731
         self.tet_active, self.node_fixed, self.node_active = u8(tet_active), u8(node_fixed), u8(node_active)
         self.node_group = None if node_group is None else np.ascontiguousarray(node_group, dtype=np.int32)
+        self.extra_pairs = (None if extra_pairs is None or len(extra_pairs) == 0
+                            else np.ascontiguousarray(extra_pairs, dtype=np.int32).reshape(-1, 2))
+        n_extra = 0 if self.extra_pairs is None else self.extra_pairs.shape[0]
         self.c = N.MeshDesc(self.rest.shape[0], N.ptr(self.rest), t, N.ptr(self.tets), N.ptr(self.young),
                             N.ptr(self.poisson), N.ptr(self.damping), N.ptr(self.density), N.ptr(self.tet_active),
-                            N.ptr(self.node_fixed), N.ptr(self.node_active), N.ptr(self.node_group))
+                            N.ptr(self.node_fixed), N.ptr(self.node_active), N.ptr(self.node_group), n_extra,
+                            N.ptr(self.extra_pairs))
 
 
 class Pattern:
@@ -149,6 +153,23 @@ class System:
         n, ms = C.c_longlong(), C.c_double()
         N.check(self.lib.pk_system_kernel_time(self.h, kernel, C.byref(n), C.byref(ms)), "pk_system_kernel_time")
         return n.value, ms.value
+
+    def set_connections(self, nodes=None, weights=None, targets=None, stiffness=None, damping=None):
+        """Point springs acting in every following substep (PHYSIK_AddPointConnection
+        semantics); no arguments clears them. nodes (C, 4) ints, weights (C, 4),
+        targets (C, 3), stiffness (C,), damping (C,)."""
+        if nodes is None or len(nodes) == 0:
+            N.check(self.lib.pk_system_set_connections(self.h, 0, None, None, None, None, None),
+                    "pk_system_set_connections")
+            return
+        nd = np.ascontiguousarray(nodes, dtype=np.int32).reshape(-1, 4)
+        c = nd.shape[0]
+        w = N.as_f64(weights).reshape(c, 4)
+        tg = N.as_f64(targets).reshape(c, 3)
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(stiffness, np.float64), (c,)))
+        dm = np.ascontiguousarray(np.broadcast_to(np.asarray(damping, np.float64), (c,)))
+        N.check(self.lib.pk_system_set_connections(self.h, c, N.ptr(nd), N.ptr(w), N.ptr(tg), N.ptr(k), N.ptr(dm)),
+                "pk_system_set_connections")
 
     def set_matrix_store(self, max_bytes):
         """Bytes the reverse mode may keep for a checkpoint segment's matrices

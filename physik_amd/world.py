@@ -100,6 +100,16 @@ class World:
     def SetNodeVelocity(self, i, x, y, z):
         self.lib.PHYSIK_SetNodeVelocity(self.h, i, x, y, z)
 
+    # transient connections
+    def AddPointConnection(self, nodes, weights, target, stiffness, damping):
+        """PHYSIK_AddPointConnection: a spring from the barycentric point of four
+        world nodes to `target`, for every substep of the next Step."""
+        self.lib.PHYSIK_AddPointConnection(self.h, *[int(n) for n in nodes], *[float(w) for w in weights],
+                                           *[float(t) for t in target], float(stiffness), float(damping))
+
+    def GetPointConnectionCount(self):
+        return self.lib.PHYSIK_GetPointConnectionCount(self.h)
+
     # meshes and components
     def GenerateTetMesh(self, positions, tets):
         p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
