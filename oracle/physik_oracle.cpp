@@ -675,14 +675,85 @@ struct World
         return true;
     }
 
+    // Collision spheres (PHYSIK_CreateCollisionSphereComponent, PhysiKAPI.cpp:685-700),
+    // in creation order, with their connection settings
+    // (CollisionSphereComponent.cpp:86-110).
+    struct Sphere
+    {
+        V3<T> c;
+        T r, k, d;
+    };
+    std::vector<Sphere> spheres;
+    int lastContacts = 0;  // sphere contacts of the last assembly
+
+    // CollisionSphereComponent::UpdateSystem (CollisionSphereComponent.cpp:112-193)
+    // + AddPointConnectionFromContact (:48-66): per substep, every active tet
+    // whose centroid lies inside a sphere gets a spring from the centroid to
+    // the sphere's surface along the centre-to-centroid direction. The
+    // reference appends these after the frame's connections and drops them at
+    // the end of the substep (World.cpp:54-67, 727-740).
+    std::vector<Conn> sphere_contacts() const
+    {
+        std::vector<Conn> out;
+        const T w = T(0.25f);
+        for (const Sphere& s : spheres)
+        {
+            if (s.r <= T(0))
+                continue;
+            for (const auto& t : tets)
+            {
+                if (!t.active)
+                    continue;
+                bool ok = true;
+                for (int a = 0; a < 4; ++a)
+                    if (t.n[a] < 0 || t.n[a] >= N)
+                        ok = false;
+                if (!ok)
+                    continue;
+                const V3<T> p =
+                    add(add(add(scl(x[t.n[0]], w), scl(x[t.n[1]], w)), scl(x[t.n[2]], w)), scl(x[t.n[3]], w));
+                const V3<T> d = sub(p, s.c);
+                const T dist = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+                if (dist >= s.r)  // a NaN distance goes on, as in the reference
+                    continue;
+                // NormalizeOrFallback (:27-36)
+                const V3<T> nrm = dist > T(0.00001f) ? V3<T>{d.x / dist, d.y / dist, d.z / dist} : V3<T>{T(0), T(0), T(1)};
+                const T depth = s.r - dist;
+                if (depth <= T(0))
+                    continue;
+                Conn c;
+                for (int a = 0; a < 4; ++a)
+                {
+                    c.n[a] = t.n[a];
+                    c.w[a] = w;
+                }
+                c.target = add(p, scl(nrm, depth));
+                c.k = s.k;
+                c.c = s.d;
+                if (conn_nodes_valid(c))  // World::AddPointConnection (World.cpp:126-131)
+                    out.push_back(c);
+            }
+        }
+        return out;
+    }
+
     // PointConnection::UpdateSystem (PointConnection.cpp:54-130): forces and
-    // k w_a w_b I blocks pushed after the components' and gravity's
-    void push_connections(std::vector<std::pair<int, V3<T>>>& forcePush, std::vector<Block>& blocks) const
+    // k w_a w_b I blocks pushed after the components' and gravity's, the
+    // frame's connections first, then the substep's sphere contacts
+    void push_connections(std::vector<std::pair<int, V3<T>>>& forcePush, std::vector<Block>& blocks,
+                          const std::vector<Conn>& contacts) const
     {
         for (const Conn& c : conns)
+            push_connection(c, forcePush, blocks);
+        for (const Conn& c : contacts)
+            push_connection(c, forcePush, blocks);
+    }
+
+    void push_connection(const Conn& c, std::vector<std::pair<int, V3<T>>>& forcePush, std::vector<Block>& blocks) const
+    {
         {
             if (!conn_nodes_valid(c) || !finite(c.k) || !finite(c.c) || c.k < T(0) || c.c < T(0))
-                continue;
+                return;
             const V3<T> point =
                 add(add(add(scl(x[c.n[0]], c.w[0]), scl(x[c.n[1]], c.w[1])), scl(x[c.n[2]], c.w[2])),
                     scl(x[c.n[3]], c.w[3]));
@@ -690,13 +761,13 @@ struct World
                 add(add(add(scl(v[c.n[0]], c.w[0]), scl(v[c.n[1]], c.w[1])), scl(v[c.n[2]], c.w[2])),
                     scl(v[c.n[3]], c.w[3]));
             if (!finite(point) || !finite(vel) || !finite(c.target))
-                continue;
+                return;
             const V3<T> f = add(scl(sub(c.target, point), c.k), scl(vel, -c.c));
             for (int a = 0; a < 4; ++a)
                 if (c.w[a] != T(0))
                     forcePush.push_back({c.n[a], scl(f, c.w[a])});
             if (c.k <= T(0))
-                continue;
+                return;
             for (int a = 0; a < 4; ++a)
             {
                 if (c.w[a] == T(0))
@@ -742,6 +813,9 @@ struct World
     // throws), 2 when the implicit assembly rejects non-finite input.
     int precompute(T dt)
     {
+        // the spheres run before the tet meshes (ComponentExecutionPriority.h:5-17)
+        const std::vector<Conn> contacts = sphere_contacts();
+        lastContacts = static_cast<int>(contacts.size());
         // TetMeshPhysicsComponent::UpdateSystem (:293-354)
         std::vector<std::pair<int, T>> massPush;
         std::vector<std::pair<int, V3<T>>> forcePush;
@@ -839,7 +913,7 @@ struct World
             forcePush.push_back({i, scl(gravity, mass[i])});
         }
         // World::AssembleConnectionSystems (World.cpp:602-611)
-        push_connections(forcePush, blocks);
+        push_connections(forcePush, blocks, contacts);
         // SolverData::PrecomputeImplicitSolve (:126-167)
         force.assign(static_cast<size_t>(N), V3<T>{});
         for (const auto& f : forcePush)
@@ -1456,6 +1530,41 @@ void orc_add_point_connection(void* p, const int* nodes, const double* weights, 
     });
 }
 
+// PHYSIK_CreateCollisionSphereComponent (PhysiKAPI.cpp:685-700; the radius
+// clamped at 0, CollisionSphereComponent.cpp:69-77) and
+// PHYSIK_SetCollisionSphereConnectionSettings (:86-110: a non-finite value
+// becomes 0, then max(0, value)) for `count` spheres in creation order,
+// replacing the previous ones.
+void orc_set_spheres(void* p, int count, const double* centers, const double* radii, const double* stiffness,
+                     const double* damping)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, {
+        using S = typename std::remove_reference<decltype(w)>::type::Sphere;
+        auto sanitize = [](T value) { return std::isfinite(value) ? std::max(T(0), value) : T(0); };
+        w.spheres.clear();
+        for (int i = 0; i < count; ++i)
+        {
+            S s;
+            s.c = oracle::V3<T>{static_cast<T>(centers[3 * i]), static_cast<T>(centers[3 * i + 1]),
+                                static_cast<T>(centers[3 * i + 2])};
+            s.r = std::max(T(0), static_cast<T>(radii[i]));
+            s.k = sanitize(static_cast<T>(stiffness[i]));
+            s.d = sanitize(static_cast<T>(damping[i]));
+            w.spheres.push_back(s);
+        }
+    });
+}
+
+// Sphere contacts generated by the last substep's assembly.
+int orc_last_contact_count(void* p)
+{
+    auto* h = static_cast<Handle*>(p);
+    int n = 0;
+    ORC_DISPATCH(h, { n = w.lastContacts; });
+    return n;
+}
+
 int orc_point_connection_count(void* p)
 {
     auto* h = static_cast<Handle*>(p);
@@ -1732,6 +1841,8 @@ int orc_adjoint(void* p, double dt, int steps, const double* xbar, const double*
     if (h->precision != 64)
         return -1;
     auto& w = *h->w64;
+    if (!w.conns.empty() || !w.spheres.empty())
+        return -1;  // the reverse substep has no connection terms
     std::vector<oracle::V3<double>> xb(static_cast<size_t>(w.N)), vb(static_cast<size_t>(w.N));
     for (int i = 0; i < w.N; ++i)
     {

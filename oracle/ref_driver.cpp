@@ -54,6 +54,7 @@ struct RefHandle
     int nodeCount = 0;
     int tetCount = 0;
     std::unique_ptr<PhysiK::SolverData> sd;
+    std::vector<PhysiK::ComponentHandle> spheres;
 };
 
 PhysiK::World* W(RefHandle* h) { return static_cast<PhysiK::World*>(h->world); }
@@ -192,6 +193,57 @@ void ref_add_point_connection(void* p, const int* nodes, const float* w, const f
         g[a] = PHYSIK_GetTetMeshGlobalNodeIndex(h->world, h->comp, nodes[a]);
     PHYSIK_AddPointConnection(h->world, g[0], g[1], g[2], g[3], w[0], w[1], w[2], w[3], target[0], target[1],
                               target[2], stiffness, damping);
+}
+
+// PHYSIK_CreateCollisionSphereComponent + PHYSIK_SetCollisionSphereConnectionSettings;
+// returns the sphere's index for the calls below.
+int ref_add_sphere(void* p, float x, float y, float z, float radius, float stiffness, float damping)
+{
+    auto* h = static_cast<RefHandle*>(p);
+    const PhysiK::ComponentHandle c = PHYSIK_CreateCollisionSphereComponent(h->world, x, y, z, radius);
+    PHYSIK_SetCollisionSphereConnectionSettings(h->world, c, stiffness, damping);
+    h->spheres.push_back(c);
+    return static_cast<int>(h->spheres.size()) - 1;
+}
+
+// PHYSIK_GetCollisionSphereConnectionSettings
+void ref_sphere_settings(void* p, int sphere, float* stiffness, float* damping)
+{
+    auto* h = static_cast<RefHandle*>(p);
+    PHYSIK_GetCollisionSphereConnectionSettings(h->world, h->spheres[sphere], stiffness, damping);
+}
+
+// PHYSIK_SetCollisionComponentKinematicTarget (taken at the next PHYSIK_Step)
+void ref_move_sphere(void* p, int sphere, float x, float y, float z)
+{
+    auto* h = static_cast<RefHandle*>(p);
+    PHYSIK_SetCollisionComponentKinematicTarget(h->world, h->spheres[sphere], x, y, z);
+}
+
+// PHYSIK_GetCollisionSphereOverlaps: per overlap 8 ints (geometry type,
+// primitive, nodes 0-3, node mask, node count) and 5 floats (centre, radius,
+// minimum node distance). Returns the count written.
+int ref_sphere_overlaps(void* p, int sphere, int max_overlaps, int* ints, float* floats)
+{
+    auto* h = static_cast<RefHandle*>(p);
+    std::vector<PhysikCollisionSphereOverlap> o(static_cast<size_t>(std::max(1, max_overlaps)));
+    const int n = PHYSIK_GetCollisionSphereOverlaps(h->world, h->spheres[sphere], o.data(), max_overlaps);
+    for (int i = 0; i < n; ++i)
+    {
+        const PhysikCollisionSphereOverlap& e = o[i];
+        const int iv[8] = {e.geometryType, e.primitiveIndex, e.node0, e.node1, e.node2, e.node3,
+                           e.overlappedNodeMask, e.overlappedNodeCount};
+        const float fv[5] = {e.sphereCenterX, e.sphereCenterY, e.sphereCenterZ, e.sphereRadius, e.minDistance};
+        std::copy(iv, iv + 8, ints + 8 * i);
+        std::copy(fv, fv + 5, floats + 5 * i);
+    }
+    return n;
+}
+
+int ref_sphere_overlap_count(void* p, int sphere)
+{
+    auto* h = static_cast<RefHandle*>(p);
+    return PHYSIK_GetCollisionSphereOverlapCount(h->world, h->spheres[sphere]);
 }
 
 int ref_point_connection_count(void* p)

@@ -68,6 +68,9 @@ def oracle_lib():
         lib.orc_add_point_connection.argtypes = [C.c_void_p, _ip, _dp, _dp, C.c_double, C.c_double]
         lib.orc_point_connection_count.argtypes = [C.c_void_p]
         lib.orc_point_connection_count.restype = C.c_int
+        lib.orc_set_spheres.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
+        lib.orc_last_contact_count.argtypes = [C.c_void_p]
+        lib.orc_last_contact_count.restype = C.c_int
         _oracle = lib
     return _oracle
 
@@ -105,6 +108,14 @@ def ref_lib():
         lib.ref_add_point_connection.argtypes = [C.c_void_p, _ip, _fp, _fp, C.c_float, C.c_float]
         lib.ref_point_connection_count.argtypes = [C.c_void_p]
         lib.ref_point_connection_count.restype = C.c_int
+        lib.ref_add_sphere.argtypes = [C.c_void_p] + [C.c_float] * 6
+        lib.ref_add_sphere.restype = C.c_int
+        lib.ref_sphere_settings.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        lib.ref_move_sphere.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float]
+        lib.ref_sphere_overlaps.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _fp]
+        lib.ref_sphere_overlaps.restype = C.c_int
+        lib.ref_sphere_overlap_count.argtypes = [C.c_void_p, C.c_int]
+        lib.ref_sphere_overlap_count.restype = C.c_int
         _ref = lib
     return _ref
 
@@ -180,6 +191,18 @@ class Oracle:
 
     def point_connection_count(self):
         return self.lib.orc_point_connection_count(self.h)
+
+    def last_contact_count(self):
+        """Sphere contacts generated by the last substep's assembly."""
+        return self.lib.orc_last_contact_count(self.h)
+
+    def set_spheres(self, centers=(), radii=(), stiffness=(), damping=()):
+        """Collision spheres in creation order (PHYSIK_CreateCollisionSphereComponent
+        + PHYSIK_SetCollisionSphereConnectionSettings); none removes them."""
+        n = len(radii)
+        c = _c(np.reshape(centers, (n, 3)) if n else np.zeros((1, 3)), np.float64)
+        r, k, d = (_c(a if n else [0.0], np.float64) for a in (radii, stiffness, damping))
+        self.lib.orc_set_spheres(self.h, n, c, r, k, d)
 
     def cg(self):
         it, res, conv = C.c_int(), C.c_double(), C.c_int()
@@ -327,6 +350,27 @@ class Reference:
 
     def point_connection_count(self):
         return self.lib.ref_point_connection_count(self.h)
+
+    def add_sphere(self, center, radius, stiffness=1000.0, damping=10.0):
+        return self.lib.ref_add_sphere(self.h, *(float(c) for c in center), radius, stiffness, damping)
+
+    def sphere_settings(self, sphere):
+        k, d = C.c_float(), C.c_float()
+        self.lib.ref_sphere_settings(self.h, sphere, C.byref(k), C.byref(d))
+        return k.value, d.value
+
+    def move_sphere(self, sphere, center):
+        self.lib.ref_move_sphere(self.h, sphere, *(float(c) for c in center))
+
+    def sphere_overlaps(self, sphere, max_overlaps=None):
+        """(ints (n, 8): type, tet, nodes 0-3, node mask, node count;
+        floats (n, 5): centre, radius, minimum node distance)."""
+        if max_overlaps is None:
+            max_overlaps = self.lib.ref_sphere_overlap_count(self.h, sphere)
+        ints = np.zeros((max(1, max_overlaps), 8), np.int32)
+        floats = np.zeros((max(1, max_overlaps), 5), np.float32)
+        n = self.lib.ref_sphere_overlaps(self.h, sphere, max_overlaps, ints, floats)
+        return ints[:n], floats[:n]
 
     def cg(self):
         it, res, conv = C.c_int(), C.c_float(), C.c_int()

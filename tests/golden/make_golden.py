@@ -261,8 +261,81 @@ def connections():
     print("connections:", out["implicit_cg_iterations"], out["implicit_counts"])
 
 
+def sphere_scene(frames):
+    """Per-frame collision spheres (centre, radius, stiffness, damping) for the
+    spheres fixture: one under the beam's free end, into which the beam sags,
+    and one moved in from the side by a kinematic target every frame."""
+    return [[((0.32, 0.15, -0.12), 0.2, 1500.0, 4.0), ((0.22, 0.42 - 0.015 * f, 0.15), 0.13, 2500.0, 0.0)]
+            for f in range(frames)]
+
+
+def spheres():
+    """A five-tet beam stepped against two collision spheres
+    (PHYSIK_CreateCollisionSphereComponent, a kinematic target per frame),
+    implicit (2 substeps per frame) and explicit, with the overlap query after
+    the last frame; and overlap queries on the rest mesh with two tets
+    deactivated."""
+    n = (4, 3, 3)
+    rest, tets = mg.box_mesh(*n, 0.1, "five")
+    fixed = np.zeros(len(rest), np.uint8)
+    fixed[mg.face_nodes(*n, 0)] = 1
+    frames = 10
+    scene = sphere_scene(frames)
+    out = {}
+    for mode, implicit, dt in (("implicit", True, 1.0 / 60.0), ("explicit", False, 1.0 / 2000.0)):
+        ref = Reference(rest, tets, 1000.0, 1e5, 0.3, 0.2)
+        ref.set_params(G, 2, 1e-6, 200, implicit=implicit)
+        ref.set_fixed(fixed)
+        ids = [ref.add_sphere(c, r, k, d) for c, r, k, d in scene[0]]
+        xs, vs, its = [], [], []
+        for f in range(frames):
+            for i, (c, _, _, _) in zip(ids, scene[f]):
+                ref.move_sphere(i, c)
+            assert ref.step(np.float32(dt)) == 0
+            assert ref.point_connection_count() == 0
+            x, v = ref.state()
+            xs.append(x)
+            vs.append(v)
+            its.append(ref.cg()[0])
+        for i in ids:
+            ints, floats = ref.sphere_overlaps(i)
+            out[f"{mode}_overlaps{i}_ints"] = ints
+            out[f"{mode}_overlaps{i}_floats"] = floats
+        ref.close()
+        out[f"{mode}_x"] = np.array(xs)
+        out[f"{mode}_v"] = np.array(vs)
+        out[f"{mode}_cg_iterations"] = np.array(its, np.int32)
+        out[f"{mode}_dt"] = np.float32(dt)
+    # overlap queries on the rest mesh: a sphere centred on a node, one that
+    # covers a corner of the beam, one of zero radius; tets 3 and 17 inactive
+    query = [((0.1, 0.1, 0.1), 0.05), ((0.0, 0.3, 0.3), 0.17), ((0.2, 0.1, 0.1), 0.0)]
+    active = np.ones(len(tets), np.uint8)
+    active[[3, 17]] = 0
+    ref = Reference(rest, tets, 1000.0, 1e5, 0.3, 0.2)
+    ref.set_tet_active(active)
+    for i, (c, r) in enumerate(query):
+        j = ref.add_sphere(c, r)
+        assert j == i
+        assert ref.sphere_settings(j) == (1000.0, 10.0)
+        ints, floats = ref.sphere_overlaps(j)
+        out[f"query{i}_ints"] = ints
+        out[f"query{i}_floats"] = floats
+    ref.close()
+    out["query_centers"] = f32([q[0] for q in query])
+    out["query_radii"] = f32([q[1] for q in query])
+    out["query_active"] = active
+    out["rest"] = f32(rest)
+    out["tets"] = tets.astype(np.int32)
+    out["fixed"] = fixed
+    out["centers"] = f32([[s[0] for s in fr] for fr in scene])
+    out["radii"] = f32([[s[1] for s in fr] for fr in scene])
+    out["stiffness"] = f32([[s[2] for s in fr] for fr in scene])
+    out["damping"] = f32([[s[3] for s in fr] for fr in scene])
+    np.savez_compressed(os.path.join(HERE, "spheres_fp32.npz"), **out)
+    print("spheres:", out["implicit_cg_iterations"], [out[f"query{i}_ints"].shape[0] for i in range(3)])
+
+
 if __name__ == "__main__":
-    config1()
-    elements()
-    small_cases()
-    connections()
+    # every fixture, or only those named on the command line
+    for name in sys.argv[1:] or ["config1", "elements", "small_cases", "connections", "spheres"]:
+        globals()[name]()

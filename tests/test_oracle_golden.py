@@ -98,6 +98,37 @@ def test_small_cases_bitwise(golden_dir, name):
 
 
 @pytest.mark.parametrize("mode", ["implicit", "explicit"])
+def test_sphere_contacts_bitwise(golden_dir, mode):
+    # two collision spheres, one moved every frame (CollisionSphereComponent.cpp:112-193,
+    # CollisionComponent.cpp:5-15): per-substep centroid contacts turned into
+    # point connections; without the spheres the trajectory differs
+    z = load(golden_dir, "spheres_fp32.npz")
+    runs = []
+    for with_spheres in (True, False):
+        o = Oracle(32, z["rest"].astype(np.float64), z["tets"], 1e5, 0.3, 0.2, 1000.0)
+        o.set_params((0.0, 0.0, -9.81), 2, 1e-6, 200, implicit=mode == "implicit")
+        o.set_fixed(z["fixed"])
+        xs, contacts = [], []
+        for f in range(len(z[f"{mode}_x"])):
+            if with_spheres:
+                o.set_spheres(z["centers"][f].astype(np.float64), z["radii"][f].astype(np.float64),
+                              z["stiffness"][f].astype(np.float64), z["damping"][f].astype(np.float64))
+            assert o.step(float(z[f"{mode}_dt"])) == 0
+            assert o.point_connection_count() == 0
+            contacts.append(o.last_contact_count())
+            x, v = o.state()
+            xs.append(x.astype(np.float32))
+            if with_spheres:
+                if mode == "implicit":
+                    assert o.cg()[0] == z["implicit_cg_iterations"][f]
+                assert np.array_equal(x.astype(np.float32), z[f"{mode}_x"][f]), f"positions differ at frame {f}"
+                assert np.array_equal(v.astype(np.float32), z[f"{mode}_v"][f]), f"velocities differ at frame {f}"
+        runs.append((xs, contacts))
+    assert min(runs[0][1]) >= 10 and max(runs[1][1]) == 0
+    assert not np.array_equal(runs[0][0][-1], runs[1][0][-1])
+
+
+@pytest.mark.parametrize("mode", ["implicit", "explicit"])
 def test_point_connections_bitwise(golden_dir, mode):
     # PHYSIK_AddPointConnection before every frame (PointConnection.cpp:54-130,
     # World.cpp:32-73): forces and k w_a w_b I blocks, including a spring over
