@@ -19,9 +19,8 @@
  *     implicit Euler;
  *   - a failed solve leaves the state of that substep untouched.
  *
- * Not provided (out of scope: render, gameplay and collision features):
- * collision spheres, scripts, surface extraction/visuals, visual meshes, tet
- * mesh mappers, point connections.
+ * Not provided (out of scope: render and gameplay features): scripts,
+ * surface extraction/visuals, visual meshes, tet mesh mappers.
  */
 #ifndef PHYSIK_AMD_PHYSIKAPI_H
 #define PHYSIK_AMD_PHYSIKAPI_H
@@ -52,6 +51,36 @@ struct ComponentHandle
     bool IsValid() const { return index != 0xFFFFFFFFu && generation != 0u; }
 };
 }  // namespace PhysiK
+
+/* include/PhysiK/API/PhysiKAPI.h:24-31 */
+enum PhysikOverlapGeometryType
+{
+    PHYSIK_OverlapGeometry_Unknown = 0,
+    PHYSIK_OverlapGeometry_Tetrahedron = 1,
+    PHYSIK_OverlapGeometry_Triangle = 2,
+    PHYSIK_OverlapGeometry_Sphere = 3,
+    PHYSIK_OverlapGeometry_Node = 4
+};
+
+/* include/PhysiK/API/PhysiKAPI.h:33-53: one active tet of a tet-mesh physics
+ * component with at least one node inside the sphere. */
+struct PhysikCollisionSphereOverlap
+{
+    int geometryType;
+    PhysiK::ComponentHandle component;
+    int primitiveIndex;
+    int node0;
+    int node1;
+    int node2;
+    int node3;
+    int overlappedNodeMask;
+    int overlappedNodeCount;
+    float sphereCenterX;
+    float sphereCenterY;
+    float sphereCenterZ;
+    float sphereRadius;
+    float minDistance;
+};
 
 /* include/PhysiK/API/PhysiKAPI.h:16-22 */
 struct PhysikMaterialDesc
@@ -109,6 +138,14 @@ extern "C"
     PHYSIK_API void PHYSIK_SetTetMeshMaterial(PhysiK::WorldHandle world,                   /* :154 */
                                               PhysiK::ComponentHandle component,
                                               const PhysikMaterialDesc* material);
+    PHYSIK_API PhysiK::ComponentHandle PHYSIK_CreateCollisionSphereComponent(              /* :159 */
+        PhysiK::WorldHandle world, float x, float y, float z, float radius);
+    PHYSIK_API void PHYSIK_SetCollisionSphereConnectionSettings(PhysiK::WorldHandle world, /* :288 */
+                                                                PhysiK::ComponentHandle sphereComponent,
+                                                                float stiffness, float damping);
+    PHYSIK_API void PHYSIK_GetCollisionSphereConnectionSettings(PhysiK::WorldHandle world, /* :294 */
+                                                                PhysiK::ComponentHandle sphereComponent,
+                                                                float* outStiffness, float* outDamping);
     PHYSIK_API void PHYSIK_DestroyComponent(PhysiK::WorldHandle world,                     /* :300 */
                                             PhysiK::ComponentHandle component);
     PHYSIK_API int PHYSIK_IsComponentHandleValid(PhysiK::WorldHandle world,                /* :304 */
@@ -125,6 +162,14 @@ extern "C"
                                          int tetIndex);
     PHYSIK_API int PHYSIK_GetActiveTetCount(PhysiK::WorldHandle world,                     /* :332 */
                                             PhysiK::ComponentHandle component);
+    PHYSIK_API int PHYSIK_GetCollisionSphereOverlapCount(PhysiK::WorldHandle world,        /* :336 */
+                                                         PhysiK::ComponentHandle sphereComponent);
+    PHYSIK_API int PHYSIK_GetCollisionSphereOverlaps(PhysiK::WorldHandle world,            /* :340 */
+                                                     PhysiK::ComponentHandle sphereComponent,
+                                                     PhysikCollisionSphereOverlap* outOverlaps, int maxOverlaps);
+    PHYSIK_API void PHYSIK_SetCollisionComponentKinematicTarget(PhysiK::WorldHandle world, /* :346 */
+                                                                PhysiK::ComponentHandle component, float x,
+                                                                float y, float z);
     PHYSIK_API int PHYSIK_GetPointConnectionCount(PhysiK::WorldHandle world);             /* :353 */
     PHYSIK_API void PHYSIK_AddPointConnection(PhysiK::WorldHandle world, int node0, int node1, /* :355 */
                                               int node2, int node3, float barycentricX, float barycentricY,

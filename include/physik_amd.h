@@ -159,6 +159,21 @@ int pk_system_group_results(pk_system* sys, int* iterations, double* residual_no
 int pk_system_set_connections(pk_system* sys, int count, const int* nodes, const double* weights,
                               const double* targets, const double* stiffness, const double* damping);
 
+/* Collision spheres (CollisionSphereComponent::UpdateSystem,
+ * CollisionSphereComponent.cpp:112-193): in every substep of every following
+ * pk_system_step, each active tet with four active nodes whose centroid p lies
+ * inside a sphere (distance d < radius) gets a spring with weights 1/4 from p
+ * to p + n (radius - d), n = (p - centre) / d (or (0, 0, 1) when d <= 1e-5),
+ * of the sphere's stiffness and damping, for that substep only; the contacts
+ * are found on the device from the substep's starting state and act on top of
+ * the point connections. `count` spheres: centers 3 x count, radii, stiffness
+ * and damping finite and >= 0 (PK_ERR_ARG otherwise; the PHYSIK_* layer
+ * clamps as the reference does, PhysiKAPI.cpp:685-700, 1180-1194). A sphere of
+ * radius 0 touches nothing. count 0 removes them. The reverse mode does not
+ * cover spheres: pk_system_record fails while any is set. */
+int pk_system_set_spheres(pk_system* sys, int count, const double* centers, const double* radii,
+                          const double* stiffness, const double* damping);
+
 /* Reverse-mode memory (no reference counterpart). pk_system_backward replays
  * each checkpoint segment forward and then reverses it substep by substep;
  * when the segment's matrices (A and M^-1 of every substep) fit max_bytes

@@ -151,6 +151,16 @@ struct Component
     std::vector<uint8_t> tetActive;
     std::vector<double> young;  // per tet
     Material material;
+    // a collision sphere instead of a tet mesh (CollisionSphereComponent.h,
+    // CollisionComponent.h:10-28): centre, radius clamped at 0, connection
+    // settings (defaults 1000 and 10), and a kinematic target taken at the
+    // next step
+    bool sphere = false;
+    float center[3] = {0.f, 0.f, 0.f};
+    float radius = 0.f;
+    float stiffness = 1000.f, damping = 10.f;
+    bool hasTarget = false;
+    float target[3] = {0.f, 0.f, 0.f};
 };
 
 struct World
@@ -182,6 +192,7 @@ struct World
     };
     std::vector<Conn> conns;
     std::set<std::pair<int, int>> extraPairs;
+    bool sysSpheres = false;  // the device system holds collision spheres
 
     pk_system* sys = nullptr;
     bool topoDirty = true;     // rebuild the device system before the next step
@@ -246,12 +257,14 @@ struct World
     {
         if (!get(h))
             return;
+        const bool sphere = components[h.index]->sphere;
         components[h.index].reset();
         ++generations[h.index];
         if (generations[h.index] == 0u)
             generations[h.index] = 1u;
         freeSlots.push_back(h.index);
-        topoDirty = true;
+        if (!sphere)
+            topoDirty = true;  // spheres reach the system at every step
     }
 
     // Rebuilds the device system from every active physics component.
@@ -319,7 +332,34 @@ struct World
         topoDirty = false;
         hostNewer = false;
         deviceNewer = false;
+        sysSpheres = false;
         return PK_OK;
+    }
+
+    // Hands the collision spheres to the system, in component order. A sphere
+    // of radius 0 touches nothing (CollisionSphereComponent.cpp:120-123), nor
+    // does one with a non-finite centre or radius: every contact it could make
+    // has a non-finite target, which PointConnection::UpdateSystem skips.
+    int apply_spheres()
+    {
+        std::vector<double> c, r, k, d;
+        for (const auto& cp : components)
+        {
+            if (!cp || !cp->sphere || !cp->active || !(cp->radius > 0.f) || !std::isfinite(cp->radius))
+                continue;
+            if (!std::isfinite(cp->center[0]) || !std::isfinite(cp->center[1]) || !std::isfinite(cp->center[2]))
+                continue;
+            c.insert(c.end(), cp->center, cp->center + 3);
+            r.push_back(cp->radius);
+            k.push_back(cp->stiffness);
+            d.push_back(cp->damping);
+        }
+        if (r.empty() && !sysSpheres)
+            return PK_OK;
+        const int rc = pk_system_set_spheres(sys, static_cast<int>(r.size()), c.data(), r.data(), k.data(), d.data());
+        if (rc == PK_OK)
+            sysSpheres = !r.empty();
+        return rc;
     }
 
     // Hands the frame's connections to the system; when they couple nodes the
@@ -357,6 +397,14 @@ struct World
     {
         if (frameDt < 0.0f)
             return PK_OK;
+        // World::PreUpdateComponents (World.cpp:38, 527-537): a collision
+        // component takes its kinematic target (CollisionComponent.cpp:5-15)
+        for (auto& cp : components)
+            if (cp && cp->sphere && cp->active && cp->hasTarget)
+            {
+                std::copy(cp->target, cp->target + 3, cp->center);
+                cp->hasTarget = false;
+            }
         if (frameDt == 0.0f)
         {
             conns.clear();  // World.cpp:41-46
@@ -376,13 +424,15 @@ struct World
             hostNewer = false;
         }
         // the connections first: a pattern they need rebuilds the system,
-        // which the settings below must then reach
+        // which the spheres and settings below must then reach
         if (!conns.empty())
         {
             const int rc = apply_connections();
             if (rc != PK_OK)
                 return rc;
         }
+        if (const int rc = apply_spheres(); rc != PK_OK)
+            return rc;
         const int steps = std::max(1, substeps);
         pk_system_set_gravity(sys, gravity[0], gravity[1], gravity[2]);
         pk_system_set_mode(sys, implicit ? 1 : 0);
@@ -420,7 +470,95 @@ GeneratedMesh* M(PhysiK::GeneratedTetMeshHandle h)
 
 bool valid_node(World* w, int i) { return w && i >= 0 && i < w->nodeCount(); }
 
-Component* tet_mesh(World* w, PhysiK::ComponentHandle h) { return w ? w->get(h) : nullptr; }
+Component* tet_mesh(World* w, PhysiK::ComponentHandle h)
+{
+    Component* c = w ? w->get(h) : nullptr;
+    return (c && !c->sphere) ? c : nullptr;
+}
+
+Component* sphere_of(World* w, PhysiK::ComponentHandle h)  // AsCollisionSphere (PhysiKAPI.cpp:101-112)
+{
+    Component* c = w ? w->get(h) : nullptr;
+    return (c && c->sphere) ? c : nullptr;
+}
+
+// SanitizeConnectionValue (CollisionSphereComponent.cpp:38-46)
+float sanitize_connection_value(float value) { return std::isfinite(value) ? std::max(0.0f, value) : 0.0f; }
+
+// CollisionSphereComponent::QueryOverlaps (CollisionSphereComponent.cpp:195-285):
+// every active tet of an active tet-mesh physics component, in component
+// order, with at least one node within the radius, in float as the reference
+// computes it from its float positions.
+std::vector<PhysikCollisionSphereOverlap> query_overlaps(World* w, PhysiK::ComponentHandle h)
+{
+    std::vector<PhysikCollisionSphereOverlap> out;
+    const Component* s = sphere_of(w, h);
+    if (!s || !s->active || s->radius <= 0.0f)
+        return out;
+    w->pull();
+    const float r2 = s->radius * s->radius;
+    for (size_t ci = 0; ci < w->components.size(); ++ci)
+    {
+        const auto& cp = w->components[ci];
+        if (!cp || !cp->physics || !cp->active)
+            continue;
+        for (size_t t = 0; t < cp->tetActive.size(); ++t)
+        {
+            if (!cp->tetActive[t])
+                continue;
+            int nodes[4];
+            bool ok = true;
+            for (int a = 0; a < 4; ++a)
+            {
+                nodes[a] = cp->globalBegin + cp->tets[4 * t + a];
+                ok = ok && valid_node(w, nodes[a]);
+            }
+            if (!ok)
+                continue;
+            int mask = 0, count = 0;
+            float minDistance = 0.0f;
+            bool hasDistance = false;
+            for (int a = 0; a < 4; ++a)
+            {
+                const float dx = static_cast<float>(w->x[3 * nodes[a]]) - s->center[0];
+                const float dy = static_cast<float>(w->x[3 * nodes[a] + 1]) - s->center[1];
+                const float dz = static_cast<float>(w->x[3 * nodes[a] + 2]) - s->center[2];
+                const float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+                const float dsq = (xx + yy) + zz;
+                const float distance = std::sqrt(std::max(0.0f, dsq));
+                if (!hasDistance || distance < minDistance)
+                {
+                    minDistance = distance;
+                    hasDistance = true;
+                }
+                if (dsq <= r2)
+                {
+                    mask |= 1 << a;
+                    ++count;
+                }
+            }
+            if (count <= 0)
+                continue;
+            PhysikCollisionSphereOverlap o{};
+            o.geometryType = PHYSIK_OverlapGeometry_Tetrahedron;
+            o.component = PhysiK::ComponentHandle{static_cast<std::uint32_t>(ci), w->generations[ci]};
+            o.primitiveIndex = static_cast<int>(t);
+            o.node0 = nodes[0];
+            o.node1 = nodes[1];
+            o.node2 = nodes[2];
+            o.node3 = nodes[3];
+            o.overlappedNodeMask = mask;
+            o.overlappedNodeCount = count;
+            o.sphereCenterX = s->center[0];
+            o.sphereCenterY = s->center[1];
+            o.sphereCenterZ = s->center[2];
+            o.sphereRadius = s->radius;
+            o.minDistance = minDistance;
+            out.push_back(o);
+        }
+    }
+    return out;
+}
 
 Component* physics(World* w, PhysiK::ComponentHandle h)
 {
@@ -823,6 +961,77 @@ int PHYSIK_GetActiveTetCount(PhysiK::WorldHandle world, PhysiK::ComponentHandle 
     if (!c)
         return 0;
     return static_cast<int>(std::count(c->tetActive.begin(), c->tetActive.end(), 1));
+}
+
+// PhysiKAPI.cpp:685-700, CollisionSphereComponent::Create (CollisionSphereComponent.cpp:69-77)
+PhysiK::ComponentHandle PHYSIK_CreateCollisionSphereComponent(PhysiK::WorldHandle world, float x, float y, float z,
+                                                              float radius)
+{
+    World* w = W(world);
+    if (!w)
+        return PhysiK::ComponentHandle{};
+    auto c = std::make_unique<Component>();
+    c->sphere = true;
+    c->center[0] = x;
+    c->center[1] = y;
+    c->center[2] = z;
+    c->radius = std::max(0.0f, radius);
+    return w->add(std::move(c));
+}
+
+// PhysiKAPI.cpp:1180-1194 -> CollisionSphereComponent::SetConnectionSettings (:86-105)
+void PHYSIK_SetCollisionSphereConnectionSettings(PhysiK::WorldHandle world, PhysiK::ComponentHandle h,
+                                                 float stiffness, float damping)
+{
+    Component* c = sphere_of(W(world), h);
+    if (!c)
+        return;
+    c->stiffness = sanitize_connection_value(stiffness);
+    c->damping = sanitize_connection_value(damping);
+}
+
+// PhysiKAPI.cpp:1196-1216
+void PHYSIK_GetCollisionSphereConnectionSettings(PhysiK::WorldHandle world, PhysiK::ComponentHandle h,
+                                                 float* outStiffness, float* outDamping)
+{
+    const Component* c = sphere_of(W(world), h);
+    if (!c)
+        return;
+    if (outStiffness)
+        *outStiffness = c->stiffness;
+    if (outDamping)
+        *outDamping = c->damping;
+}
+
+// PhysiKAPI.cpp:1376-1383
+int PHYSIK_GetCollisionSphereOverlapCount(PhysiK::WorldHandle world, PhysiK::ComponentHandle h)
+{
+    return static_cast<int>(query_overlaps(W(world), h).size());
+}
+
+// PhysiKAPI.cpp:1385-1405
+int PHYSIK_GetCollisionSphereOverlaps(PhysiK::WorldHandle world, PhysiK::ComponentHandle h,
+                                      PhysikCollisionSphereOverlap* outOverlaps, int maxOverlaps)
+{
+    if (!outOverlaps || maxOverlaps <= 0)
+        return 0;
+    const std::vector<PhysikCollisionSphereOverlap> o = query_overlaps(W(world), h);
+    const int n = std::min(maxOverlaps, static_cast<int>(o.size()));
+    std::copy(o.begin(), o.begin() + n, outOverlaps);
+    return n;
+}
+
+// PhysiKAPI.cpp:1407-1427: the position is taken at the next PHYSIK_Step
+void PHYSIK_SetCollisionComponentKinematicTarget(PhysiK::WorldHandle world, PhysiK::ComponentHandle h, float x,
+                                                 float y, float z)
+{
+    Component* c = sphere_of(W(world), h);
+    if (!c)
+        return;
+    c->target[0] = x;
+    c->target[1] = y;
+    c->target[2] = z;
+    c->hasTarget = true;
 }
 
 int PHYSIK_GetPointConnectionCount(PhysiK::WorldHandle world)

@@ -612,7 +612,7 @@ void launch_assemble(const DeviceView& v, double dt)
                            v.clist,
                            v.uFull, v.uMirror, v.slotRow, v.rowGroup, v.basis, v.prm, v.T, v.mrow, dt * dt,
                            static_cast<M*>(v.vals), v.nnzf, static_cast<M*>(v.minv), v.nd, v.gs, v.upper,
-                           v.nconn > 0 ? v.cslot : nullptr);
+                           v.slotAdd);
 }
 
 template <class M>
@@ -621,7 +621,7 @@ void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt)
     if (v.vec.count > 0)
         hipLaunchKernelGGL((k_rhs<M>), dim3(grid_of(v.vec)), dim3(NT), 0, v.stream, v.vec, v.nstart,
                            v.corner, v.mrow, gx, gy, gz, dt, dt * dt, static_cast<const M*>(v.minv), v.nd, v.xs, v.r, static_cast<M*>(v.z),
-                           static_cast<M*>(v.d[0]), v.pb[1], v.nconn > 0 ? v.crhs : nullptr);
+                           static_cast<M*>(v.d[0]), v.pb[1], v.rowAdd);
 }
 
 void launch_integrate(const DeviceView& v, double dt, int commit)
@@ -635,7 +635,7 @@ void launch_explicit(const DeviceView& v, double gx, double gy, double gz, doubl
 {
     if (v.vec.count > 0)
         hipLaunchKernelGGL(k_explicit, dim3(grid_of(v.vec)), dim3(NT), 0, v.stream, v.vec, v.rowNode, v.nstart,
-                           v.corner, v.mrow, gx, gy, gz, dt, v.x, v.v, v.nconn > 0 ? v.crhs : nullptr);
+                           v.corner, v.mrow, gx, gy, gz, dt, v.x, v.v, v.rowAdd);
 }
 
 // Point connections of a substep (PointConnection.cpp:54-130): the
@@ -728,6 +728,131 @@ void launch_connections(const DeviceView& v, double dt, int implicit)
         hipLaunchKernelGGL(k_conn_rows, dim3((v.nconnRows + NT - 1) / NT), dim3(NT), 0, v.stream, v.nconnRows,
                            v.connRows, v.connRowStart, v.connEntries, v.connData, v.connEval, dt, dt * dt, implicit,
                            v.crhs);
+}
+
+// CollisionSphereComponent::UpdateSystem (CollisionSphereComponent.cpp:112-193)
+// and AddPointConnectionFromContact (:48-66) for one candidate tet: for every
+// sphere of positive radius holding the tet's centroid p (distance d < radius,
+// a NaN distance included, as in the reference), a connection with weights
+// 1/4 to the target p + n (radius - d), n = (p - centre)/d or (0, 0, 1) for
+// d <= 1e-5, handled as PointConnection::UpdateSystem (PointConnection.cpp:54-130)
+// handles it: nothing when p, the centroid velocity u or the target is not
+// finite; else the force F = k (target - p) + (-c) u and the blocks
+// k/16 I. Writes sum k and, per corner before the 1/4 weight,
+// sum (dt F - dt^2 k u) (implicit) or sum F (explicit).
+__global__ __launch_bounds__(NT) void k_sphere_contacts(int n, const int* __restrict__ cand,
+                                                        const int4* __restrict__ tets, const double* __restrict__ x,
+                                                        const double* __restrict__ v, int nsph,
+                                                        const double* __restrict__ sph, double dt, int implicit,
+                                                        double* __restrict__ cdat)
+{
+    const int i = xcd_block() * NT + static_cast<int>(threadIdx.x);
+    if (i >= n)
+        return;
+    const int4 nd = tets[cand[i]];
+    const double w = 0.25;
+    double p[3], u[3];
+    bool fin3 = true;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+    {
+        p[j] = ((x[3 * nd.x + j] * w + x[3 * nd.y + j] * w) + x[3 * nd.z + j] * w) + x[3 * nd.w + j] * w;
+        u[j] = ((v[3 * nd.x + j] * w + v[3 * nd.y + j] * w) + v[3 * nd.z + j] * w) + v[3 * nd.w + j] * w;
+        fin3 = fin3 && fin(p[j]) && fin(u[j]);
+    }
+    double ksum = 0.0, g[3] = {0.0, 0.0, 0.0};
+    const double dt2 = dt * dt;
+    for (int s = 0; s < nsph; ++s)
+    {
+        const double* sp = sph + 6 * s;
+        const double r = sp[3];
+        if (!(r > 0.0))
+            continue;
+        const double d[3] = {p[0] - sp[0], p[1] - sp[1], p[2] - sp[2]};
+        const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (dist >= r)
+            continue;
+        const bool unit = dist > static_cast<double>(0.00001f);
+        const double depth = r - dist;
+        if (depth <= 0.0)
+            continue;
+        double t[3];
+        bool ok = fin3;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            const double nj = unit ? d[j] / dist : (j == 2 ? 1.0 : 0.0);
+            t[j] = p[j] + nj * depth;
+            ok = ok && fin(t[j]);
+        }
+        if (!ok)
+            continue;
+        const double k = sp[4], c = sp[5];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            const double F = (t[j] - p[j]) * k + u[j] * (-c);
+            g[j] += implicit ? dt * F - dt2 * k * u[j] : F;
+        }
+        ksum += k;
+    }
+    double* o = cdat + 4 * static_cast<size_t>(i);
+    o[0] = ksum;
+    o[1] = g[0];
+    o[2] = g[1];
+    o[3] = g[2];
+}
+
+// sslot[s] = base[s] + (1/16) sum of the candidates' k over the slot's list
+// (each ordered corner pair of a tet adds k w_a w_b = k/16).
+__global__ __launch_bounds__(NT) void k_sphere_slots(long long n, const int* __restrict__ start,
+                                                     const int* __restrict__ list, const double* __restrict__ cdat,
+                                                     const double* __restrict__ base, double* __restrict__ sslot)
+{
+    const long long s = static_cast<long long>(xcd_block()) * NT + threadIdx.x;
+    if (s >= n)
+        return;
+    double acc = 0.0;
+    for (int k = start[s]; k < start[s + 1]; ++k)
+        acc += cdat[4 * static_cast<size_t>(list[k])];
+    sslot[s] = (base ? base[s] : 0.0) + 0.0625 * acc;
+}
+
+// srhs[3 r + j] = base[3 r + j] + (1/4) sum of the candidates' terms over the
+// row's list.
+__global__ __launch_bounds__(NT) void k_sphere_rows(int n, const int* __restrict__ start, const int* __restrict__ list,
+                                                    const double* __restrict__ cdat, const double* __restrict__ base,
+                                                    double* __restrict__ srhs)
+{
+    const int r = xcd_block() * NT + static_cast<int>(threadIdx.x);
+    if (r >= n)
+        return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int k = start[r]; k < start[r + 1]; ++k)
+    {
+        const double* c = cdat + 4 * static_cast<size_t>(list[k]);
+        acc[0] += c[1];
+        acc[1] += c[2];
+        acc[2] += c[3];
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        srhs[3 * r + j] = (base ? base[3 * r + j] : 0.0) + 0.25 * acc[j];
+}
+
+void launch_spheres(const DeviceView& v, double dt, int implicit)
+{
+    if (v.nsph <= 0)
+        return;
+    if (v.ncand > 0)
+        hipLaunchKernelGGL(k_sphere_contacts, dim3(xcd_grid((v.ncand + NT - 1) / NT)), dim3(NT), 0, v.stream, v.ncand,
+                           v.cand, v.tets, v.x, v.v, v.nsph, v.sph, dt, implicit, v.cdat);
+    if (v.nnzu > 0)
+        hipLaunchKernelGGL(k_sphere_slots, dim3(xcd_grid((v.nnzu + NT - 1) / NT)), dim3(NT), 0,
+                           v.stream, v.nnzu, v.csStart, v.csList, v.cdat, v.nconn > 0 ? v.cslot : nullptr, v.sslot);
+    if (v.nd > 0)
+        hipLaunchKernelGGL(k_sphere_rows, dim3(xcd_grid((v.nd + NT - 1) / NT)), dim3(NT), 0, v.stream, v.nd,
+                           v.crStart, v.crList, v.cdat, v.nconn > 0 ? v.crhs : nullptr, v.srhs);
 }
 
 void launch_slot_scatter(hipStream_t stream, int n, const int* idx, const double* val, double* cslot)

@@ -298,6 +298,25 @@ struct DeviceView
     const int2* connEntries; // (connection, corner a) in connection order
     double* crhs;            // nd x 3: the connections' right-hand side (or force) per row
     const double* cslot;     // per upper slot: sum of k w_a w_b (null without connections)
+    // collision spheres (none when nsph == 0): per substep, each candidate
+    // tet's contact terms, then their sums per upper slot and per row on top
+    // of the point connections'
+    int nsph;
+    const double* sph;       // 6 per sphere: centre (3), radius, k, c
+    int ncand;               // candidate tets: active, four active nodes
+    const int* cand;
+    double* cdat;            // 4 per candidate: sum of k, rhs term (or force) xyz
+    const int* csStart;      // nnzu + 1 offsets into csList
+    const int* csList;       // upper slot -> candidates, once per ordered pair landing there
+    const int* crStart;      // nd + 1 offsets into crList
+    const int* crList;       // dynamic row -> candidates, once per corner on that row
+    double* sslot;           // per upper slot: cslot + the contacts' k/16
+    double* srhs;            // nd x 3: crhs + the contacts' terms / 4
+    // what the assembly adds per upper slot and the right-hand side (or the
+    // explicit force) per row: sslot/srhs with spheres, else cslot/crhs with
+    // connections, else null
+    const double* slotAdd;
+    const double* rowAdd;
     // reductions
     TileView vec, spmv;
     double* pa;        // per SpMV tile: d.q
@@ -321,6 +340,9 @@ void launch_explicit(const DeviceView& v, double gx, double gy, double gz, doubl
 // state, then each touched row's right-hand side terms (implicit) or force
 // (explicit) into crhs.
 void launch_connections(const DeviceView& v, double dt, int implicit);
+// Collision spheres: each candidate tet's contacts at the current state, then
+// sslot and srhs (no-op without spheres). After launch_connections.
+void launch_spheres(const DeviceView& v, double dt, int implicit);
 // cslot[idx[i]] = val ? val[i] : 0 for i < n.
 void launch_slot_scatter(hipStream_t stream, int n, const int* idx, const double* val, double* cslot);
 // crhs[3 r .. 3 r + 2] = 0 for the n rows listed.

@@ -183,6 +183,12 @@ struct pk_system
     pk::DevBuf<int> connRows, connRowStart, slotIdx;
     pk::DevBuf<int2> connEntries;
 
+    // collision spheres in force (pk_system_set_spheres); the candidate tets
+    // and their slot and row lists are built on first use (ncand < 0 before)
+    int nsph = 0, ncand = -1;
+    pk::DevBuf<double> sph, cdat, sslot, srhs;
+    pk::DevBuf<int> cand, csStart, csList, crStart, crList;
+
     // CG chunks replayed from captured HIP graphs: the first chunk of a solve
     // (full tiles) and every later one (compaction first, for a batch). A
     // capture bakes in buffer addresses and tile shapes: `epoch` changes when
@@ -284,6 +290,19 @@ struct pk_system
         dv.connEntries = connEntries.p;
         dv.crhs = crhs.p;
         dv.cslot = cslot.p;
+        dv.nsph = nsph;
+        dv.sph = sph.p;
+        dv.ncand = std::max(0, ncand);
+        dv.cand = cand.p;
+        dv.cdat = cdat.p;
+        dv.csStart = csStart.p;
+        dv.csList = csList.p;
+        dv.crStart = crStart.p;
+        dv.crList = crList.p;
+        dv.sslot = sslot.p;
+        dv.srhs = srhs.p;
+        dv.slotAdd = nsph > 0 ? sslot.p : (nconn > 0 ? cslot.p : nullptr);
+        dv.rowAdd = nsph > 0 ? srhs.p : (nconn > 0 ? crhs.p : nullptr);
         dv.vec = vecTiles.view();
         dv.spmv = spmvTiles.view();
         dv.pa = pa.p;
@@ -458,6 +477,7 @@ static int run_assembly(pk_system* s, double dt, double tol, int maxIt, bool wit
         launch_element(dv);
     }
     launch_connections(dv, dt, s->implicit);
+    launch_spheres(dv, dt, s->implicit);
     if (s->topo.nd == 0)
         return PK_OK;
     {
@@ -1237,9 +1257,9 @@ int pk_system_record(pk_system* s, double dt, int steps, int every)
         set_error("pk_system_record: the reverse mode covers the implicit integrator only");
         return PK_ERR_ARG;
     }
-    if (s->nconn > 0)
+    if (s->nconn > 0 || s->nsph > 0)
     {
-        set_error("pk_system_record: the reverse mode does not cover point connections");
+        set_error("pk_system_record: the reverse mode does not cover point connections or collision spheres");
         return PK_ERR_ARG;
     }
     if (s->topo.massErrorNode >= 0)
@@ -1469,6 +1489,147 @@ int pk_system_set_connections(pk_system* s, int count, const int* nodes, const d
     s->nconn = nc;
     s->nconnRows = static_cast<int>(rows.size());
     s->nslots = static_cast<int>(sidx.size());
+    s->tapeValid = false;
+    return PK_OK;
+}
+
+// The tets collision spheres can touch (active, four active nodes: the
+// contacts the reference keeps, CollisionSphereComponent.cpp:141-162,
+// World.cpp:126-131) and, for each, the upper slots of its ordered corner
+// pairs (r_a <= r_b, both dynamic) and the rows of its dynamic corners, as
+// per-slot and per-row lists in candidate order.
+static int build_contact_lists(pk_system* s)
+{
+    const Topology& t = s->topo;
+    std::vector<int> cand;
+    for (int tt = 0; tt < t.T; ++tt)
+    {
+        if (!t.tetActive[tt])
+            continue;
+        bool ok = true;
+        for (int a = 0; a < 4; ++a)
+        {
+            const int n = t.tets[4 * static_cast<size_t>(tt) + a];
+            ok = ok && n >= 0 && n < t.N && t.nodeActive[n];
+        }
+        if (ok)
+            cand.push_back(tt);
+    }
+    const long long nnzu = t.nnzu();
+    if (static_cast<long long>(cand.size()) * 16 >= (1LL << 31))
+    {
+        set_error("pk_system_set_spheres: too many tets for the contact lists");
+        return PK_ERR_ARG;
+    }
+    auto find_slot = [&](int r, int c) {
+        const auto b = t.ucol.begin() + t.ustart[r], e = t.ucol.begin() + t.ustart[r + 1];
+        const auto it = std::lower_bound(b, e, c);
+        return (it != e && *it == c) ? static_cast<int>(it - t.ucol.begin()) : -1;
+    };
+    std::vector<int> sstart(static_cast<size_t>(nnzu) + 1, 0), rstart(static_cast<size_t>(t.nd) + 1, 0);
+    std::vector<int> slotOf;  // per (candidate, ordered pair) in fill order
+    for (size_t i = 0; i < cand.size(); ++i)
+    {
+        const int* n4 = t.tets.data() + 4 * static_cast<size_t>(cand[i]);
+        for (int a = 0; a < 4; ++a)
+        {
+            const int ra = t.dyn[n4[a]];
+            if (ra < 0)
+                continue;
+            ++rstart[ra + 1];
+            for (int b = 0; b < 4; ++b)
+            {
+                const int rb = t.dyn[n4[b]];
+                if (rb < 0 || ra > rb)
+                    continue;
+                const int sl = find_slot(ra, rb);
+                if (sl < 0)
+                {
+                    set_error("pk_system_set_spheres: a tet's corner pair is missing from the BSR pattern");
+                    return PK_ERR_ARG;
+                }
+                slotOf.push_back(sl);
+                ++sstart[sl + 1];
+            }
+        }
+    }
+    for (long long k = 0; k < nnzu; ++k)
+        sstart[k + 1] += sstart[k];
+    for (int r = 0; r < t.nd; ++r)
+        rstart[r + 1] += rstart[r];
+    std::vector<int> slist(static_cast<size_t>(sstart[nnzu])), rlist(static_cast<size_t>(rstart[t.nd]));
+    {
+        std::vector<int> sfill(sstart.begin(), sstart.end() - 1), rfill(rstart.begin(), rstart.end() - 1);
+        size_t q = 0;
+        for (size_t i = 0; i < cand.size(); ++i)
+        {
+            const int* n4 = t.tets.data() + 4 * static_cast<size_t>(cand[i]);
+            for (int a = 0; a < 4; ++a)
+            {
+                const int ra = t.dyn[n4[a]];
+                if (ra < 0)
+                    continue;
+                rlist[rfill[ra]++] = static_cast<int>(i);
+                for (int b = 0; b < 4; ++b)
+                {
+                    const int rb = t.dyn[n4[b]];
+                    if (rb < 0 || ra > rb)
+                        continue;
+                    slist[sfill[slotOf[q++]]++] = static_cast<int>(i);
+                }
+            }
+        }
+    }
+    hipStream_t st = s->stream;
+    int rc;
+    if ((rc = s->cand.upload(cand, st)) != PK_OK) return rc;
+    if ((rc = s->csStart.upload(sstart, st)) != PK_OK) return rc;
+    if ((rc = s->csList.upload(slist, st)) != PK_OK) return rc;
+    if ((rc = s->crStart.upload(rstart, st)) != PK_OK) return rc;
+    if ((rc = s->crList.upload(rlist, st)) != PK_OK) return rc;
+    if ((rc = s->cdat.alloc(4 * std::max<size_t>(1, cand.size()))) != PK_OK) return rc;
+    if ((rc = s->sslot.alloc(static_cast<size_t>(std::max<long long>(1, nnzu)))) != PK_OK) return rc;
+    if ((rc = s->srhs.alloc(3 * static_cast<size_t>(std::max(1, t.nd)))) != PK_OK) return rc;
+    PK_HIP(hipStreamSynchronize(st));
+    s->ncand = static_cast<int>(cand.size());
+    return PK_OK;
+}
+
+int pk_system_set_spheres(pk_system* s, int count, const double* centers, const double* radii,
+                          const double* stiffness, const double* damping)
+{
+    if (!s || count < 0 || (count > 0 && (!centers || !radii || !stiffness || !damping)))
+    {
+        set_error("pk_system_set_spheres: bad argument");
+        return PK_ERR_ARG;
+    }
+    std::vector<double> hs;
+    for (int i = 0; i < count; ++i)
+    {
+        const double v[6] = {centers[3 * i], centers[3 * i + 1], centers[3 * i + 2], radii[i], stiffness[i],
+                             damping[i]};
+        for (int j = 0; j < 6; ++j)
+            if (!std::isfinite(v[j]) || (j >= 3 && v[j] < 0.0))
+            {
+                set_error("pk_system_set_spheres: sphere " + std::to_string(i) +
+                          " needs a finite centre and finite radius, stiffness and damping >= 0");
+                return PK_ERR_ARG;
+            }
+        hs.insert(hs.end(), v, v + 6);
+    }
+    int rc;
+    if (count > 0 && s->ncand < 0 && (rc = build_contact_lists(s)) != PK_OK)
+        return rc;
+    PK_HIP(hipStreamSynchronize(s->stream));
+    s->nsph = 0;
+    if (count == 0)
+        return PK_OK;
+    // kept when the count is unchanged (spheres that move every frame)
+    if ((rc = s->sph.ensure(hs.size())) != PK_OK)
+        return rc;
+    PK_HIP(hipMemcpyAsync(s->sph.p, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    PK_HIP(hipStreamSynchronize(s->stream));
+    s->nsph = count;
     s->tapeValid = false;
     return PK_OK;
 }

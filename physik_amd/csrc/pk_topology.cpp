@@ -172,6 +172,9 @@ int build_topology(const pk_mesh_desc& d, Topology& topo, std::string& err, bool
         }
     auto tetActive = [&](int t) { return !d.tet_active || d.tet_active[t]; };
 
+    topo.tetActive.assign(static_cast<size_t>(T), 0);
+    for (int t = 0; t < T; ++t)
+        topo.tetActive[t] = tetActive(t) ? 1 : 0;
     topo.live.assign(static_cast<size_t>(T), 0);
     topo.mass.assign(static_cast<size_t>(N), 0.0);
     for (int t = 0; t < T; ++t)

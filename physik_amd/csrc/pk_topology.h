@@ -63,6 +63,7 @@ struct Topology
     int N = 0, T = 0, G = 1;
     std::vector<int> tets;         // 4T
     std::vector<uint8_t> live;     // T: the tet produces an element (active, valid, V > 0)
+    std::vector<uint8_t> tetActive;  // T: the tet's active flag
     std::vector<double> prm;       // P_COUNT * T
     std::vector<double> poisson;   // T, kept to refresh Lame parameters when E changes
     std::vector<double> mass;      // N lumped masses (fixed nodes get none)

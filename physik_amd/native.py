@@ -46,6 +46,22 @@ class PhysikMaterialDesc(C.Structure):
                 ("damping", C.c_float)]
 
 
+# PhysikOverlapGeometryType (the reference's PhysiKAPI.h:24-31)
+PHYSIK_OverlapGeometry_Unknown = 0
+PHYSIK_OverlapGeometry_Tetrahedron = 1
+PHYSIK_OverlapGeometry_Triangle = 2
+PHYSIK_OverlapGeometry_Sphere = 3
+PHYSIK_OverlapGeometry_Node = 4
+
+
+class PhysikCollisionSphereOverlap(C.Structure):
+    _fields_ = [("geometryType", C.c_int), ("component", ComponentHandle), ("primitiveIndex", C.c_int),
+                ("node0", C.c_int), ("node1", C.c_int), ("node2", C.c_int), ("node3", C.c_int),
+                ("overlappedNodeMask", C.c_int), ("overlappedNodeCount", C.c_int),
+                ("sphereCenterX", C.c_float), ("sphereCenterY", C.c_float), ("sphereCenterZ", C.c_float),
+                ("sphereRadius", C.c_float), ("minDistance", C.c_float)]
+
+
 _lib = None
 
 PK_SIGNATURES = {
@@ -70,6 +86,7 @@ PK_SIGNATURES = {
     "pk_system_kernel_time": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
     "pk_system_set_graphs": (C.c_int, [_vp, C.c_int]),
     "pk_system_set_connections": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "pk_system_set_spheres": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
     "pk_system_set_matrix_store": (C.c_int, [_vp, C.c_longlong]),
     "pk_system_spmv_work": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                       C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
@@ -130,6 +147,12 @@ PHYSIK_SIGNATURES = {
     "PHYSIK_SetTetActive": (None, [_W, _H, C.c_int, C.c_int]),
     "PHYSIK_DeactivateTet": (None, [_W, _H, C.c_int]),
     "PHYSIK_GetActiveTetCount": (C.c_int, [_W, _H]),
+    "PHYSIK_CreateCollisionSphereComponent": (_H, [_W, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "PHYSIK_SetCollisionSphereConnectionSettings": (None, [_W, _H, C.c_float, C.c_float]),
+    "PHYSIK_GetCollisionSphereConnectionSettings": (None, [_W, _H, _fp, _fp]),
+    "PHYSIK_GetCollisionSphereOverlapCount": (C.c_int, [_W, _H]),
+    "PHYSIK_GetCollisionSphereOverlaps": (C.c_int, [_W, _H, C.POINTER(PhysikCollisionSphereOverlap), C.c_int]),
+    "PHYSIK_SetCollisionComponentKinematicTarget": (None, [_W, _H, C.c_float, C.c_float, C.c_float]),
     "PHYSIK_GetPointConnectionCount": (C.c_int, [_W]),
     "PHYSIK_AddPointConnection": (None, [_W, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                          C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]),

@@ -171,6 +171,22 @@ class System:
         N.check(self.lib.pk_system_set_connections(self.h, c, N.ptr(nd), N.ptr(w), N.ptr(tg), N.ptr(k), N.ptr(dm)),
                 "pk_system_set_connections")
 
+    def set_spheres(self, centers=None, radii=None, stiffness=1000.0, damping=10.0):
+        """Collision spheres acting in every following substep (contacts from
+        the tets' centroids, CollisionSphereComponent semantics); no arguments
+        removes them. centers (S, 3), radii (S,), stiffness and damping (S,) or
+        scalars."""
+        if radii is None or len(np.atleast_1d(radii)) == 0:
+            N.check(self.lib.pk_system_set_spheres(self.h, 0, None, None, None, None), "pk_system_set_spheres")
+            return
+        r = N.as_f64(np.atleast_1d(radii))
+        n = r.shape[0]
+        c = N.as_f64(centers).reshape(n, 3)
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(stiffness, np.float64), (n,)))
+        dm = np.ascontiguousarray(np.broadcast_to(np.asarray(damping, np.float64), (n,)))
+        N.check(self.lib.pk_system_set_spheres(self.h, n, N.ptr(c), N.ptr(r), N.ptr(k), N.ptr(dm)),
+                "pk_system_set_spheres")
+
     def set_matrix_store(self, max_bytes):
         """Bytes the reverse mode may keep for a checkpoint segment's matrices
         (default 8 GiB); 0 re-assembles every reverse substep instead."""

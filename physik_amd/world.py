@@ -110,6 +110,32 @@ class World:
     def GetPointConnectionCount(self):
         return self.lib.PHYSIK_GetPointConnectionCount(self.h)
 
+    # collision spheres
+    def CreateCollisionSphereComponent(self, x, y, z, radius):
+        return self.lib.PHYSIK_CreateCollisionSphereComponent(self.h, x, y, z, radius)
+
+    def SetCollisionSphereConnectionSettings(self, comp, stiffness, damping):
+        self.lib.PHYSIK_SetCollisionSphereConnectionSettings(self.h, comp, stiffness, damping)
+
+    def GetCollisionSphereConnectionSettings(self, comp, stiffness=-1.0, damping=-1.0):
+        """(stiffness, damping); the given defaults come back when the handle
+        is not a sphere (the call leaves its outputs untouched)."""
+        k, d = C.c_float(stiffness), C.c_float(damping)
+        self.lib.PHYSIK_GetCollisionSphereConnectionSettings(self.h, comp, C.byref(k), C.byref(d))
+        return k.value, d.value
+
+    def SetCollisionComponentKinematicTarget(self, comp, x, y, z):
+        self.lib.PHYSIK_SetCollisionComponentKinematicTarget(self.h, comp, x, y, z)
+
+    def GetCollisionSphereOverlapCount(self, comp):
+        return self.lib.PHYSIK_GetCollisionSphereOverlapCount(self.h, comp)
+
+    def GetCollisionSphereOverlaps(self, comp, max_overlaps):
+        """The overlap records (PhysikCollisionSphereOverlap) the call writes."""
+        out = (N.PhysikCollisionSphereOverlap * max(1, max_overlaps))()
+        n = self.lib.PHYSIK_GetCollisionSphereOverlaps(self.h, comp, out, max_overlaps)
+        return list(out[:n])
+
     # meshes and components
     def GenerateTetMesh(self, positions, tets):
         p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)

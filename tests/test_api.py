@@ -3,6 +3,7 @@ activation. None of these calls touches the GPU; they mirror the reference's
 own API tests (tests/PointConnectionTests.cpp in the reference tree)."""
 
 import math
+import os
 
 import numpy as np
 import pytest
@@ -136,3 +137,154 @@ def test_node_setters_follow_reference_semantics():
     w.SetNodeVelocity(i, 4.0, 5.0, 6.0)
     w.SetNodeFixed(i, 1)  # fixing zeroes the velocity (World.cpp:403-417)
     assert w.IsNodeFixed(i) == 1 and np.array_equal(w.GetNodeVelocity(i), [0, 0, 0])
+
+
+# --- collision spheres (host side: components, settings, overlap queries) ---
+
+def test_collision_sphere_settings_defaults_and_clamps():
+    # PointConnectionTests.cpp:817-857 (SanitizeConnectionValue, CollisionSphereComponent.cpp:38-46)
+    w = World()
+    s = w.CreateCollisionSphereComponent(0.0, 0.0, 0.0, 0.5)
+    assert s.valid() and w.IsComponentHandleValid(s)
+    assert w.GetCollisionSphereConnectionSettings(s) == (1000.0, 10.0)
+    w.SetCollisionSphereConnectionSettings(s, 2500.0, 12.5)
+    assert w.GetCollisionSphereConnectionSettings(s) == (2500.0, 12.5)
+    w.SetCollisionSphereConnectionSettings(s, -10.0, -2.0)
+    assert w.GetCollisionSphereConnectionSettings(s) == (0.0, 0.0)
+    w.SetCollisionSphereConnectionSettings(s, float("nan"), float("inf"))
+    assert w.GetCollisionSphereConnectionSettings(s) == (0.0, 0.0)
+    # a sphere is not a tet mesh
+    assert w.GetTetMeshTetCount(s) == 0 and w.GetTetMeshGlobalNodeBeginIndex(s) == -1
+
+
+def test_collision_sphere_settings_invalid_inputs():
+    # PointConnectionTests.cpp:880-914
+    import ctypes as C
+    from physik_amd import native
+    w = World()
+    c = single_tet(w, E=25.0, density=1.0)
+    s = w.CreateCollisionSphereComponent(0.0, 0.0, 0.0, 0.5)
+    w.lib.PHYSIK_SetCollisionSphereConnectionSettings(None, s, 10.0, 1.0)
+    w.SetCollisionSphereConnectionSettings(native.ComponentHandle(), 10.0, 1.0)
+    w.SetCollisionSphereConnectionSettings(c, 10.0, 1.0)
+    k, d = C.c_float(-1.0), C.c_float(-1.0)
+    w.lib.PHYSIK_GetCollisionSphereConnectionSettings(None, s, C.byref(k), C.byref(d))
+    assert (k.value, d.value) == (-1.0, -1.0)
+    assert w.GetCollisionSphereConnectionSettings(c) == (-1.0, -1.0)
+    w.lib.PHYSIK_GetCollisionSphereConnectionSettings(w.h, s, C.byref(k), None)
+    assert k.value == 1000.0
+    w.lib.PHYSIK_GetCollisionSphereConnectionSettings(w.h, s, None, C.byref(d))
+    assert d.value == 10.0
+    # the radius is clamped at 0 (CollisionSphereComponent.cpp:69-77); a null world gives an invalid handle
+    assert not w.lib.PHYSIK_CreateCollisionSphereComponent(None, 0.0, 0.0, 0.0, 1.0).valid()
+    z = w.CreateCollisionSphereComponent(0.0, 0.0, 0.0, -3.0)
+    assert w.GetCollisionSphereOverlapCount(z) == 0
+
+
+def two_tets(w):
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, -1]], np.float32)
+    m = w.GenerateTetMesh(pos, [[0, 1, 2, 3], [0, 1, 2, 4]])
+    c = w.CreateTetMeshPhysicsComponent(m, material(1.0, 0.0, 0.3, 0.0))
+    w.DestroyGeneratedTetMesh(m)
+    return c
+
+
+def test_collision_sphere_overlap_queries_follow_the_reference_tests():
+    # PointConnectionTests.cpp:1801-1983
+    from physik_amd import native
+    w = World()
+    t = single_tet(w, E=25.0, density=1.0)
+    nodes = [w.GetTetMeshGlobalNodeIndex(t, i) for i in range(4)]
+    s = w.CreateCollisionSphereComponent(0.0, 0.0, 0.0, 0.05)
+    assert w.GetCollisionSphereOverlapCount(s) == 1
+    (o,) = w.GetCollisionSphereOverlaps(s, 1)
+    assert o.geometryType == native_tet() and o.primitiveIndex == 0
+    assert (o.component.index, o.component.generation) == (t.index, t.generation)
+    assert [o.node0, o.node1, o.node2, o.node3] == nodes
+    assert (o.overlappedNodeMask, o.overlappedNodeCount) == (1, 1)
+    assert o.sphereCenterX == 0.0 and abs(o.sphereRadius - 0.05) < 1e-7 and o.minDistance == 0.0
+    # invalid inputs give 0
+    assert w.lib.PHYSIK_GetCollisionSphereOverlapCount(None, s) == 0
+    assert w.GetCollisionSphereOverlapCount(native.ComponentHandle()) == 0
+    assert w.GetCollisionSphereOverlapCount(t) == 0
+    assert w.lib.PHYSIK_GetCollisionSphereOverlaps(w.h, s, None, 1) == 0
+    assert w.GetCollisionSphereOverlaps(s, 0) == [] and w.GetCollisionSphereOverlaps(t, 1) == []
+    # the mesh destroyed: nothing to overlap
+    w.DestroyComponent(t)
+    assert w.GetCollisionSphereOverlapCount(s) == 0
+
+    # world node mapping: a node added first shifts the mesh's nodes
+    w = World()
+    w.AddNode(100.0, 100.0, 100.0)
+    t = single_tet(w, E=25.0, density=1.0)
+    s = w.CreateCollisionSphereComponent(0.0, 0.0, 0.0, 0.05)
+    (o,) = w.GetCollisionSphereOverlaps(s, 1)
+    assert o.node0 == w.GetTetMeshGlobalNodeIndex(t, 0) != 0
+    # far away, zero radius: none
+    assert w.GetCollisionSphereOverlapCount(w.CreateCollisionSphereComponent(10.0, 10.0, 10.0, 0.5)) == 0
+    assert w.GetCollisionSphereOverlapCount(w.CreateCollisionSphereComponent(0.0, 0.0, 0.0, 0.0)) == 0
+    # mask and count
+    (o,) = w.GetCollisionSphereOverlaps(w.CreateCollisionSphereComponent(0.5, 0.0, 0.0, 0.51), 1)
+    assert (o.overlappedNodeMask, o.overlappedNodeCount) == (3, 2) and abs(o.minDistance - 0.5) < 1e-6
+
+    # inactive tets are skipped; count and fill agree
+    w = World()
+    c = two_tets(w)
+    s = w.CreateCollisionSphereComponent(0.0, 0.0, 0.0, 0.05)
+    assert w.GetCollisionSphereOverlapCount(s) == 2
+    assert len(w.GetCollisionSphereOverlaps(s, 1)) == 1
+    both = w.GetCollisionSphereOverlaps(s, 2)
+    assert len(both) == 2 and both[0].primitiveIndex != both[1].primitiveIndex
+    low = w.CreateCollisionSphereComponent(0.0, 0.0, -1.0, 0.05)
+    assert w.GetCollisionSphereOverlapCount(low) == 1
+    w.DeactivateTet(c, 1)
+    assert w.GetCollisionSphereOverlapCount(low) == 0
+
+
+def native_tet():
+    from physik_amd import native
+    return native.PHYSIK_OverlapGeometry_Tetrahedron
+
+
+def test_collision_sphere_overlaps_match_the_reference_fixture(golden_dir):
+    # the reference's overlap query on a beam with two tets deactivated
+    # (tests/golden/spheres_fp32.npz, from the compiled reference): every
+    # field bit for bit
+    z = np.load(os.path.join(golden_dir, "spheres_fp32.npz"), allow_pickle=False)
+    w = World()
+    m = w.GenerateTetMesh(z["rest"], z["tets"])
+    c = w.CreateTetMeshPhysicsComponent(m, material(1000.0, 1e5, 0.3, 0.2))
+    w.DestroyGeneratedTetMesh(m)
+    for t in np.flatnonzero(z["query_active"] == 0):
+        w.DeactivateTet(c, int(t))
+    for i, (center, radius) in enumerate(zip(z["query_centers"], z["query_radii"])):
+        s = w.CreateCollisionSphereComponent(*[float(v) for v in center], float(radius))
+        got = w.GetCollisionSphereOverlaps(s, 1000)
+        ints, floats = z[f"query{i}_ints"], z[f"query{i}_floats"]
+        assert len(got) == len(ints) == w.GetCollisionSphereOverlapCount(s)
+        for o, iv, fv in zip(got, ints, floats):
+            assert [o.geometryType, o.primitiveIndex, o.node0, o.node1, o.node2, o.node3, o.overlappedNodeMask,
+                    o.overlappedNodeCount] == iv.tolist()
+            assert np.array_equal(np.float32([o.sphereCenterX, o.sphereCenterY, o.sphereCenterZ, o.sphereRadius,
+                                              o.minDistance]), fv)
+            assert (o.component.index, o.component.generation) == (c.index, c.generation)
+
+
+def test_collision_sphere_kinematic_target_is_taken_at_the_next_step():
+    # PHYSIK_SetCollisionComponentKinematicTarget (PhysiKAPI.cpp:1407-1427):
+    # the target becomes the position in the next Step's PreUpdate
+    # (CollisionComponent.cpp:5-15), also for Step(0) (World.cpp:38-46)
+    w = World()
+    single_tet(w, E=25.0, density=1.0)
+    s = w.CreateCollisionSphereComponent(10.0, 10.0, 10.0, 0.05)
+    assert w.GetCollisionSphereOverlapCount(s) == 0
+    w.SetCollisionComponentKinematicTarget(s, 0.0, 0.0, 0.0)
+    assert w.GetCollisionSphereOverlapCount(s) == 0
+    w.Step(-1.0)  # a negative step returns before the components' PreUpdate
+    assert w.GetCollisionSphereOverlapCount(s) == 0
+    w.Step(0.0)
+    assert w.GetCollisionSphereOverlapCount(s) == 1
+    (o,) = w.GetCollisionSphereOverlaps(s, 1)
+    assert (o.sphereCenterX, o.sphereCenterY, o.sphereCenterZ) == (0.0, 0.0, 0.0)
+    w.DestroyComponent(s)
+    assert not w.IsComponentHandleValid(s) and w.GetCollisionSphereOverlapCount(s) == 0
