@@ -139,7 +139,8 @@ def roofline(s, name, prec_name):
 
 def kernels_ms(s):
     from physik_amd import native as N
-    return {N.KERNEL_NAMES[k]: round(s.kernel_time(k)[1], 3) for k in range(7)}
+    return {N.KERNEL_NAMES[k]: round(s.kernel_time(k)[1], 3) for k in range(len(N.KERNEL_NAMES))
+            if s.kernel_time(k)[0] > 0 or k != N.K_CONTACT}
 
 
 def prepare(name, prec_name, args):

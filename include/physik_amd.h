@@ -74,7 +74,8 @@ enum pk_kernel {
     PK_K_UPDATE = 4,      /* x, r update, z = M^-1 r, r.z, r.r */
     PK_K_DIRECTION = 5,   /* d = z + beta d */
     PK_K_INTEGRATE = 6,   /* v += dv, x += dt v (check + commit) */
-    PK_K_COUNT = 7
+    PK_K_CONTACT = 7,     /* point connections and collision-sphere contacts */
+    PK_K_COUNT = 8
 };
 
 typedef struct pk_mesh_desc {

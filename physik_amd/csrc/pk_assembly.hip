@@ -738,13 +738,13 @@ void launch_connections(const DeviceView& v, double dt, int implicit)
 // d <= 1e-5, handled as PointConnection::UpdateSystem (PointConnection.cpp:54-130)
 // handles it: nothing when p, the centroid velocity u or the target is not
 // finite; else the force F = k (target - p) + (-c) u and the blocks
-// k/16 I. Writes sum k and, per corner before the 1/4 weight,
-// sum (dt F - dt^2 k u) (implicit) or sum F (explicit).
+// k/16 I. Writes ck = sum k and, per corner before the 1/4 weight,
+// cg = sum (dt F - dt^2 k u) (implicit) or sum F (explicit).
 __global__ __launch_bounds__(NT) void k_sphere_contacts(int n, const int* __restrict__ cand,
                                                         const int4* __restrict__ tets, const double* __restrict__ x,
                                                         const double* __restrict__ v, int nsph,
                                                         const double* __restrict__ sph, double dt, int implicit,
-                                                        double* __restrict__ cdat)
+                                                        double* __restrict__ ck, double* __restrict__ cg)
 {
     const int i = xcd_block() * NT + static_cast<int>(threadIdx.x);
     if (i >= n)
@@ -796,48 +796,71 @@ __global__ __launch_bounds__(NT) void k_sphere_contacts(int n, const int* __rest
         }
         ksum += k;
     }
-    double* o = cdat + 4 * static_cast<size_t>(i);
-    o[0] = ksum;
-    o[1] = g[0];
-    o[2] = g[1];
-    o[3] = g[2];
+    ck[i] = ksum;
+    cg[3 * static_cast<size_t>(i)] = g[0];
+    cg[3 * static_cast<size_t>(i) + 1] = g[1];
+    cg[3 * static_cast<size_t>(i) + 2] = g[2];
 }
 
-// sslot[s] = base[s] + (1/16) sum of the candidates' k over the slot's list
-// (each ordered corner pair of a tet adds k w_a w_b = k/16).
+// Lanes per upper slot and per row of the two sums below: a slot gathers about
+// 7 candidates on a tet mesh, a row about 24; each lane group walks its list
+// with consecutive lanes on consecutive entries and combines the lanes'
+// partial sums in a fixed order.
+constexpr int kSphereSlotLanes = 4;
+constexpr int kSphereRowLanes = 8;
+
+// sslot[s] = base[s] + (1/16) sum of ck over the slot's list (each ordered
+// corner pair of a tet adds k w_a w_b = k/16).
 __global__ __launch_bounds__(NT) void k_sphere_slots(long long n, const int* __restrict__ start,
-                                                     const int* __restrict__ list, const double* __restrict__ cdat,
+                                                     const int* __restrict__ list, const double* __restrict__ ck,
                                                      const double* __restrict__ base, double* __restrict__ sslot)
 {
-    const long long s = static_cast<long long>(xcd_block()) * NT + threadIdx.x;
+    const long long s = (static_cast<long long>(xcd_block()) * NT + threadIdx.x) / kSphereSlotLanes;
+    const int lane = threadIdx.x % kSphereSlotLanes;
     if (s >= n)
-        return;
+        return;  // whole lane groups
     double acc = 0.0;
-    for (int k = start[s]; k < start[s + 1]; ++k)
-        acc += cdat[4 * static_cast<size_t>(list[k])];
-    sslot[s] = (base ? base[s] : 0.0) + 0.0625 * acc;
+    const int k1 = start[s + 1];
+    for (int k = start[s] + lane; k < k1; k += kSphereSlotLanes)
+        acc += ck[list[k]];
+#pragma unroll
+    for (int o = kSphereSlotLanes / 2; o > 0; o >>= 1)
+        acc += __shfl_xor(acc, o, kSphereSlotLanes);
+    if (lane == 0)
+        sslot[s] = (base ? base[s] : 0.0) + 0.0625 * acc;
 }
 
-// srhs[3 r + j] = base[3 r + j] + (1/4) sum of the candidates' terms over the
-// row's list.
+// srhs[3 r + j] = base[3 r + j] + (1/4) sum of cg over the row's list.
 __global__ __launch_bounds__(NT) void k_sphere_rows(int n, const int* __restrict__ start, const int* __restrict__ list,
-                                                    const double* __restrict__ cdat, const double* __restrict__ base,
+                                                    const double* __restrict__ cg, const double* __restrict__ base,
                                                     double* __restrict__ srhs)
 {
-    const int r = xcd_block() * NT + static_cast<int>(threadIdx.x);
+    const int r = (xcd_block() * NT + static_cast<int>(threadIdx.x)) / kSphereRowLanes;
+    const int lane = threadIdx.x % kSphereRowLanes;
     if (r >= n)
-        return;
+        return;  // whole lane groups
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int k = start[r]; k < start[r + 1]; ++k)
+    const int k1 = start[r + 1];
+    for (int k = start[r] + lane; k < k1; k += kSphereRowLanes)
     {
-        const double* c = cdat + 4 * static_cast<size_t>(list[k]);
-        acc[0] += c[1];
-        acc[1] += c[2];
-        acc[2] += c[3];
+        const double* c = cg + 3 * static_cast<size_t>(list[k]);
+        acc[0] += c[0];
+        acc[1] += c[1];
+        acc[2] += c[2];
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j)
-        srhs[3 * r + j] = (base ? base[3 * r + j] : 0.0) + 0.25 * acc[j];
+    {
+#pragma unroll
+        for (int o = kSphereRowLanes / 2; o > 0; o >>= 1)
+            acc[j] += __shfl_xor(acc[j], o, kSphereRowLanes);
+    }
+    if (lane == 0)
+    {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            srhs[3 * r + j] = (base ? base[3 * r + j] : 0.0) + 0.25 * acc[j];
+    }
 }
 
 void launch_spheres(const DeviceView& v, double dt, int implicit)
@@ -846,13 +869,14 @@ void launch_spheres(const DeviceView& v, double dt, int implicit)
         return;
     if (v.ncand > 0)
         hipLaunchKernelGGL(k_sphere_contacts, dim3(xcd_grid((v.ncand + NT - 1) / NT)), dim3(NT), 0, v.stream, v.ncand,
-                           v.cand, v.tets, v.x, v.v, v.nsph, v.sph, dt, implicit, v.cdat);
+                           v.cand, v.tets, v.x, v.v, v.nsph, v.sph, dt, implicit, v.ck, v.cg);
     if (v.nnzu > 0)
-        hipLaunchKernelGGL(k_sphere_slots, dim3(xcd_grid((v.nnzu + NT - 1) / NT)), dim3(NT), 0,
-                           v.stream, v.nnzu, v.csStart, v.csList, v.cdat, v.nconn > 0 ? v.cslot : nullptr, v.sslot);
+        hipLaunchKernelGGL(k_sphere_slots, dim3(xcd_grid((v.nnzu * kSphereSlotLanes + NT - 1) / NT)), dim3(NT), 0,
+                           v.stream, v.nnzu, v.csStart, v.csList, v.ck, v.nconn > 0 ? v.cslot : nullptr, v.sslot);
     if (v.nd > 0)
-        hipLaunchKernelGGL(k_sphere_rows, dim3(xcd_grid((v.nd + NT - 1) / NT)), dim3(NT), 0, v.stream, v.nd,
-                           v.crStart, v.crList, v.cdat, v.nconn > 0 ? v.crhs : nullptr, v.srhs);
+        hipLaunchKernelGGL(k_sphere_rows,
+                           dim3(xcd_grid((static_cast<long long>(v.nd) * kSphereRowLanes + NT - 1) / NT)), dim3(NT),
+                           0, v.stream, v.nd, v.crStart, v.crList, v.cg, v.nconn > 0 ? v.crhs : nullptr, v.srhs);
 }
 
 void launch_slot_scatter(hipStream_t stream, int n, const int* idx, const double* val, double* cslot)

@@ -305,7 +305,8 @@ struct DeviceView
     const double* sph;       // 6 per sphere: centre (3), radius, k, c
     int ncand;               // candidate tets: active, four active nodes
     const int* cand;
-    double* cdat;            // 4 per candidate: sum of k, rhs term (or force) xyz
+    double* ck;              // per candidate: sum of k
+    double* cg;              // 3 per candidate: rhs term (or force) before the 1/4 weight
     const int* csStart;      // nnzu + 1 offsets into csList
     const int* csList;       // upper slot -> candidates, once per ordered pair landing there
     const int* crStart;      // nd + 1 offsets into crList

@@ -186,7 +186,7 @@ struct pk_system
     // collision spheres in force (pk_system_set_spheres); the candidate tets
     // and their slot and row lists are built on first use (ncand < 0 before)
     int nsph = 0, ncand = -1;
-    pk::DevBuf<double> sph, cdat, sslot, srhs;
+    pk::DevBuf<double> sph, ck, cg, sslot, srhs;
     pk::DevBuf<int> cand, csStart, csList, crStart, crList;
 
     // CG chunks replayed from captured HIP graphs: the first chunk of a solve
@@ -294,7 +294,8 @@ struct pk_system
         dv.sph = sph.p;
         dv.ncand = std::max(0, ncand);
         dv.cand = cand.p;
-        dv.cdat = cdat.p;
+        dv.ck = ck.p;
+        dv.cg = cg.p;
         dv.csStart = csStart.p;
         dv.csList = csList.p;
         dv.crStart = crStart.p;
@@ -476,8 +477,12 @@ static int run_assembly(pk_system* s, double dt, double tol, int maxIt, bool wit
         Bracket br(s, PK_K_ELEMENT);
         launch_element(dv);
     }
-    launch_connections(dv, dt, s->implicit);
-    launch_spheres(dv, dt, s->implicit);
+    if (s->nconn > 0 || s->nsph > 0)
+    {
+        Bracket br(s, PK_K_CONTACT);
+        launch_connections(dv, dt, s->implicit);
+        launch_spheres(dv, dt, s->implicit);
+    }
     if (s->topo.nd == 0)
         return PK_OK;
     {
@@ -1587,7 +1592,8 @@ static int build_contact_lists(pk_system* s)
     if ((rc = s->csList.upload(slist, st)) != PK_OK) return rc;
     if ((rc = s->crStart.upload(rstart, st)) != PK_OK) return rc;
     if ((rc = s->crList.upload(rlist, st)) != PK_OK) return rc;
-    if ((rc = s->cdat.alloc(4 * std::max<size_t>(1, cand.size()))) != PK_OK) return rc;
+    if ((rc = s->ck.alloc(std::max<size_t>(1, cand.size()))) != PK_OK) return rc;
+    if ((rc = s->cg.alloc(3 * std::max<size_t>(1, cand.size()))) != PK_OK) return rc;
     if ((rc = s->sslot.alloc(static_cast<size_t>(std::max<long long>(1, nnzu)))) != PK_OK) return rc;
     if ((rc = s->srhs.alloc(3 * static_cast<size_t>(std::max(1, t.nd)))) != PK_OK) return rc;
     PK_HIP(hipStreamSynchronize(st));

@@ -2,12 +2,16 @@
 
     python -m physik_amd.diag [--copy-mb 512] [--workload twist64] [--steps 1]
                               [--precision fp64|fp32] [--shape L,R,V,U]
+                              [--spheres S]
 
 Run under ``rocprofv3 --pmc FETCH_SIZE`` (and again with WRITE_SIZE): the copy
 kernel moves exactly copy-mb MiB each way, which calibrates the counters; the
 implicit steps that follow give the per-dispatch counts of every step kernel.
 The JSON line it prints carries the work the SpMV launches did (blocks and
-rows of the bodies still iterating), to set the counters against.
+rows of the bodies still iterating), to set the counters against. With
+--spheres S, S collision spheres sit along the diagonal of the mesh's bounding
+box (radius a quarter of its smallest side) and the line adds the contact
+kernels' time and the share of tets whose starting centroid is in contact.
 """
 
 from __future__ import annotations
@@ -15,6 +19,8 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+
+import numpy as np
 
 from . import native as N
 from . import workloads as W
@@ -28,6 +34,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp32u", "fp64u"])
     ap.add_argument("--shape", default=None,
                     help="CG launch shape lanes,spmv_rows,vector_rows,unroll (default: the system's own)")
+    ap.add_argument("--spheres", type=int, default=0)
     args = ap.parse_args()
     lib = N.lib()
     ms = C.c_double()
@@ -38,15 +45,27 @@ def main():
                           "fp64u": N.PK_FP64_UPPER}[args.precision])
     if args.shape:
         s.tune(*[int(a) for a in args.shape.split(",")])
+    extra = {}
+    if args.spheres > 0:
+        lo, hi = w.x0.min(0), w.x0.max(0)
+        centers = lo + (np.arange(args.spheres)[:, None] + 0.5) / args.spheres * (hi - lo)
+        radius = 0.25 * float((hi - lo).min())
+        s.set_spheres(centers, np.full(args.spheres, radius), 1e4, 10.0)
+        cent = w.x0[w.tets].mean(1)
+        inside = (np.linalg.norm(cent[:, None, :] - centers[None], axis=2) < radius).any(1)
+        extra = {"spheres": args.spheres, "sphere_radius": radius, "contact_tet_share": float(inside.mean())}
     s.profile(True)
     s.step(w.dt, 1, args.steps)
+    if args.spheres > 0:
+        extra.update({k: s.kernel_time(i) for k, i in (("contact", N.K_CONTACT), ("element", N.K_ELEMENT),
+                                                       ("assemble", N.K_ASSEMBLE), ("rhs", N.K_RHS))})
     it = s.results()[0]
     blocks, ublocks, rows, body_launches = s.spmv_work()
     print(json.dumps({"copy_bytes": nbytes, "copy_ms": ms.value, "copy_GBps": 2 * nbytes / ms.value / 1e6,
                       "workload": args.workload, "steps": args.steps, "cg_iterations": int(it.max()),
                       "nnzb_full": s.nnzb_full, "dynamic_rows": s.nd, "tets": s.t, "bodies": s.groups,
                       "spmv_launches": s.kernel_time(N.K_SPMV)[0], "spmv_blocks": blocks, "spmv_upper_blocks": ublocks, "spmv_rows": rows, "nnzb_upper": s.nnzb_upper,
-                      "spmv_body_launches": body_launches, "shape": s.get_tune()}), flush=True)
+                      "spmv_body_launches": body_launches, "shape": s.get_tune(), **extra}), flush=True)
 
 
 if __name__ == "__main__":

@@ -17,8 +17,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libphysik_amd.so")
 PK_OK, PK_ERR_ARG, PK_ERR_HIP, PK_ERR_MASS, PK_ERR_NO_DEVICE = 0, 1, 2, 3, 4
 PK_FP64, PK_FP32_STORAGE, PK_FP32_UPPER, PK_FP64_UPPER = 0, 1, 2, 3
 GROUP_INTEGRATED, GROUP_SOLVE_FAILED, GROUP_NONFINITE, GROUP_NO_DYNAMIC = 0, 1, 2, 3
-K_ELEMENT, K_ASSEMBLE, K_RHS, K_SPMV, K_UPDATE, K_DIRECTION, K_INTEGRATE = range(7)
-KERNEL_NAMES = ["element", "assemble", "rhs", "spmv", "update", "direction", "integrate"]
+K_ELEMENT, K_ASSEMBLE, K_RHS, K_SPMV, K_UPDATE, K_DIRECTION, K_INTEGRATE, K_CONTACT = range(8)
+KERNEL_NAMES = ["element", "assemble", "rhs", "spmv", "update", "direction", "integrate", "contact"]
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
