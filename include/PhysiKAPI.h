@@ -197,6 +197,10 @@ extern "C"
     /* Outcome of the last substep's solve: 0 integrated, 1 solve failed,
      * 2 non-finite update rejected, 3 no dynamic node. */
     PHYSIK_API int PHYSIK_AMD_GetLastStepStatus(PhysiK::WorldHandle world);
+    /* How many times the world has built its device system. PHYSIK_SetTetActive
+     * switches a tet in place at the next step while the set of active nodes
+     * stays the same; any other topology change builds the system again. */
+    PHYSIK_API int PHYSIK_AMD_GetSystemBuildCount(PhysiK::WorldHandle world);
     /* Bulk node state in FP64 (count nodes, xyz triples). Copy returns the
      * number of nodes written; Set writes positions and/or velocities as given
      * (positions do not reset velocities here). */

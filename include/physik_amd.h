@@ -40,7 +40,8 @@ enum pk_status {
     PK_ERR_HIP = 2,       /* a HIP runtime call failed (message in pk_last_error) */
     PK_ERR_MASS = 3,      /* a free active node has no mass: the reference throws
                              std::runtime_error here (World.cpp:560-580) */
-    PK_ERR_NO_DEVICE = 4  /* no GPU visible */
+    PK_ERR_NO_DEVICE = 4, /* no GPU visible */
+    PK_ERR_REBUILD = 5    /* the change needs the system created again (nothing was changed) */
 };
 
 enum pk_precision {
@@ -159,6 +160,20 @@ int pk_system_group_results(pk_system* sys, int* iterations, double* residual_no
  * not cover connections: pk_system_record fails while any is set. */
 int pk_system_set_connections(pk_system* sys, int count, const int* nodes, const double* weights,
                               const double* targets, const double* stiffness, const double* damping);
+
+/* Switches tets on or off in place (TetMeshComponent::SetTetActive,
+ * TetMeshComponent.cpp:153-169): count tet ids and their new active flags (the
+ * last entry for a tet wins). A tet switched off stops producing forces,
+ * stiffness, lumped mass and sphere contacts; its node pairs stay in the BSR
+ * pattern as zero blocks, as the reference keeps its pattern
+ * (PointConnectionTests.cpp:2780-2803). Masses are summed again as the
+ * creation sums them, so the state after a switch equals that of a system
+ * created with the new flags up to the order of additions in the CG. Node
+ * activity is not changed here. PK_ERR_REBUILD, with nothing changed, when
+ * the switch would change which nodes are dynamic (a free node left without
+ * mass, or one gaining its first) or turn on a tet that was not live when the
+ * system was created: create the system again with the new flags then. */
+int pk_system_set_tets_active(pk_system* sys, int count, const int* tets, const unsigned char* active);
 
 /* Collision spheres (CollisionSphereComponent::UpdateSystem,
  * CollisionSphereComponent.cpp:112-193): in every substep of every following

@@ -741,6 +741,7 @@ void launch_connections(const DeviceView& v, double dt, int implicit)
 // k/16 I. Writes ck = sum k and, per corner before the 1/4 weight,
 // cg = sum (dt F - dt^2 k u) (implicit) or sum F (explicit).
 __global__ __launch_bounds__(NT) void k_sphere_contacts(int n, const int* __restrict__ cand,
+                                                        const unsigned char* __restrict__ tact,
                                                         const int4* __restrict__ tets, const double* __restrict__ x,
                                                         const double* __restrict__ v, int nsph,
                                                         const double* __restrict__ sph, double dt, int implicit,
@@ -749,7 +750,15 @@ __global__ __launch_bounds__(NT) void k_sphere_contacts(int n, const int* __rest
     const int i = xcd_block() * NT + static_cast<int>(threadIdx.x);
     if (i >= n)
         return;
-    const int4 nd = tets[cand[i]];
+    const int tt = cand[i];
+    if (!tact[tt])
+    {
+        // switched off: no contacts (CollisionSphereComponent.cpp:143-146)
+        ck[i] = 0.0;
+        cg[3 * static_cast<size_t>(i)] = cg[3 * static_cast<size_t>(i) + 1] = cg[3 * static_cast<size_t>(i) + 2] = 0.0;
+        return;
+    }
+    const int4 nd = tets[tt];
     const double w = 0.25;
     double p[3], u[3];
     bool fin3 = true;
@@ -869,7 +878,7 @@ void launch_spheres(const DeviceView& v, double dt, int implicit)
         return;
     if (v.ncand > 0)
         hipLaunchKernelGGL(k_sphere_contacts, dim3(xcd_grid((v.ncand + NT - 1) / NT)), dim3(NT), 0, v.stream, v.ncand,
-                           v.cand, v.tets, v.x, v.v, v.nsph, v.sph, dt, implicit, v.ck, v.cg);
+                           v.cand, v.tact, v.tets, v.x, v.v, v.nsph, v.sph, dt, implicit, v.ck, v.cg);
     if (v.nnzu > 0)
         hipLaunchKernelGGL(k_sphere_slots, dim3(xcd_grid((v.nnzu * kSphereSlotLanes + NT - 1) / NT)), dim3(NT), 0,
                            v.stream, v.nnzu, v.csStart, v.csList, v.ck, v.nconn > 0 ? v.cslot : nullptr, v.sslot);
@@ -877,6 +886,61 @@ void launch_spheres(const DeviceView& v, double dt, int implicit)
         hipLaunchKernelGGL(k_sphere_rows,
                            dim3(xcd_grid((static_cast<long long>(v.nd) * kSphereRowLanes + NT - 1) / NT)), dim3(NT),
                            0, v.stream, v.nd, v.crStart, v.crList, v.cg, v.nconn > 0 ? v.crhs : nullptr, v.srhs);
+}
+
+// A tet switched off in place: its element outputs that the gather lists
+// still reach (basis planes, its corner entries, its reverse-mode planes)
+// become zeros, which stay since the element kernels skip tets not live.
+__global__ __launch_bounds__(NT) void k_zero_tets(int n, const int* __restrict__ ids, int T,
+                                                  const int* __restrict__ cpos, double* __restrict__ basis,
+                                                  double* __restrict__ corner, double* __restrict__ adj)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= n)
+        return;
+    const int t = ids[i];
+    const size_t TT = static_cast<size_t>(T);
+    for (int k = 0; k < EB_COUNT; ++k)
+        basis[k * TT + t] = 0.0;
+    for (int a = 0; a < 4; ++a)
+    {
+        const int pos = cpos[a * TT + t];
+        if (pos >= 0)
+            for (int e = 0; e < N_CORNER; ++e)
+                corner[static_cast<size_t>(pos) * N_CORNER + e] = 0.0;
+    }
+    if (adj)
+        for (int k = 0; k < EP_ADJ_COUNT; ++k)
+            adj[(EP_ADJ_X + k) * TT + t] = 0.0;
+}
+
+// live[ids[i]] = liveVal[i], tact[ids[i]] = actVal[i].
+__global__ __launch_bounds__(NT) void k_tet_flags(int n, const int* __restrict__ ids,
+                                                  const unsigned char* __restrict__ liveVal,
+                                                  const unsigned char* __restrict__ actVal,
+                                                  unsigned char* __restrict__ live, unsigned char* __restrict__ tact)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i < n)
+    {
+        live[ids[i]] = liveVal[i];
+        tact[ids[i]] = actVal[i];
+    }
+}
+
+void launch_zero_tets(const DeviceView& v, int n, const int* ids)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_zero_tets, dim3((n + NT - 1) / NT), dim3(NT), 0, v.stream, n, ids, v.T, v.cpos, v.basis,
+                           v.corner, v.adj);
+}
+
+void launch_tet_flags(hipStream_t stream, int n, const int* ids, const unsigned char* liveVal,
+                      const unsigned char* actVal, unsigned char* live, unsigned char* tact)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_tet_flags, dim3((n + NT - 1) / NT), dim3(NT), 0, stream, n, ids, liveVal, actVal, live,
+                           tact);
 }
 
 void launch_slot_scatter(hipStream_t stream, int n, const int* idx, const double* val, double* cslot)

@@ -303,6 +303,7 @@ struct DeviceView
     // of the point connections'
     int nsph;
     const double* sph;       // 6 per sphere: centre (3), radius, k, c
+    const unsigned char* tact;  // T: the tet's active flag
     int ncand;               // candidate tets: active, four active nodes
     const int* cand;
     double* ck;              // per candidate: sum of k
@@ -344,6 +345,11 @@ void launch_connections(const DeviceView& v, double dt, int implicit);
 // Collision spheres: each candidate tet's contacts at the current state, then
 // sslot and srhs (no-op without spheres). After launch_connections.
 void launch_spheres(const DeviceView& v, double dt, int implicit);
+// Zeros the element outputs of the n tets listed (switched off in place).
+void launch_zero_tets(const DeviceView& v, int n, const int* ids);
+// live[ids[i]] = liveVal[i] and tact[ids[i]] = actVal[i] for i < n.
+void launch_tet_flags(hipStream_t stream, int n, const int* ids, const unsigned char* liveVal,
+                      const unsigned char* actVal, unsigned char* live, unsigned char* tact);
 // cslot[idx[i]] = val ? val[i] : 0 for i < n.
 void launch_slot_scatter(hipStream_t stream, int n, const int* idx, const double* val, double* cslot);
 // crhs[3 r .. 3 r + 2] = 0 for the n rows listed.

@@ -14,6 +14,7 @@
 #include <memory>
 #include <string>
 #include <map>
+#include <set>
 #include <array>
 #include <vector>
 
@@ -124,7 +125,7 @@ struct pk_system
     hipStream_t stream = nullptr;
 
     pk::DevBuf<int4> tets;
-    pk::DevBuf<unsigned char> live;
+    pk::DevBuf<unsigned char> live, tact;
     pk::DevBuf<double> prm, x, v, basis, corner, adj, mrow, xs, r, pa;
     pk::DevBuf<char> minv;  // storage type, like the CG vectors
     pk::DevBuf<int> cstart, clist, uFull, uMirror, slotRow, rowGroup, fstart, fcol, nstart, nlist, rowNode, active;
@@ -242,6 +243,7 @@ struct pk_system
         dv.nnzf = topo.nnzf();
         dv.tets = tets.p;
         dv.live = live.p;
+        dv.tact = tact.p;
         dv.prm = prm.p;
         dv.cstart = cstart.p;
         dv.clist = clist.p;
@@ -767,6 +769,8 @@ static int setup_device(pk_system* s)
     if ((rc = s->tets.upload(tv, st)) != PK_OK) return rc;
     std::vector<unsigned char> live(t.live.begin(), t.live.end());
     if ((rc = s->live.upload(live, st)) != PK_OK) return rc;
+    std::vector<unsigned char> tact(t.tetActive.begin(), t.tetActive.end());
+    if ((rc = s->tact.upload(tact, st)) != PK_OK) return rc;
     if ((rc = s->prm.upload(t.prm, st)) != PK_OK) return rc;
     std::vector<double> mrow(static_cast<size_t>(t.nd));
     for (int r = 0; r < t.nd; ++r)
@@ -1301,7 +1305,12 @@ int pk_system_backward(pk_system* s, const double* xbar, const double* vbar, dou
     if ((rc = s->vbar.alloc(n3)) != PK_OK) return rc;
     if ((rc = s->gradE.alloc(static_cast<size_t>(t.T))) != PK_OK) return rc;
     if (s->adj.n < static_cast<size_t>(t.T) * EP_ADJ_COUNT)
+    {
         if ((rc = s->adj.alloc(static_cast<size_t>(t.T) * EP_ADJ_COUNT)) != PK_OK) return rc;
+        // tets switched off in place are still in the node gather lists and
+        // the reverse element kernel skips them: their planes must read 0
+        PK_HIP(hipMemsetAsync(s->adj.p, 0, s->adj.n * sizeof(double), s->stream));
+    }
     if (xbar)
         PK_HIP(hipMemcpyAsync(s->xbar.p, xbar, n3 * sizeof(double), hipMemcpyHostToDevice, s->stream));
     else
@@ -1498,19 +1507,17 @@ int pk_system_set_connections(pk_system* s, int count, const int* nodes, const d
     return PK_OK;
 }
 
-// The tets collision spheres can touch (active, four active nodes: the
-// contacts the reference keeps, CollisionSphereComponent.cpp:141-162,
-// World.cpp:126-131) and, for each, the upper slots of its ordered corner
-// pairs (r_a <= r_b, both dynamic) and the rows of its dynamic corners, as
-// per-slot and per-row lists in candidate order.
+// The tets collision spheres can touch (four active nodes: the contacts the
+// reference keeps, CollisionSphereComponent.cpp:141-162, World.cpp:126-131;
+// the contact kernel skips those switched off) and, for each, the upper
+// slots of its ordered corner pairs (r_a <= r_b, both dynamic) and the rows
+// of its dynamic corners, as per-slot and per-row lists in candidate order.
 static int build_contact_lists(pk_system* s)
 {
     const Topology& t = s->topo;
     std::vector<int> cand;
     for (int tt = 0; tt < t.T; ++tt)
     {
-        if (!t.tetActive[tt])
-            continue;
         bool ok = true;
         for (int a = 0; a < 4; ++a)
         {
@@ -1636,6 +1643,114 @@ int pk_system_set_spheres(pk_system* s, int count, const double* centers, const 
     PK_HIP(hipMemcpyAsync(s->sph.p, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
     PK_HIP(hipStreamSynchronize(s->stream));
     s->nsph = count;
+    s->tapeValid = false;
+    return PK_OK;
+}
+
+int pk_system_set_tets_active(pk_system* s, int count, const int* tets, const unsigned char* active)
+{
+    if (!s || count < 0 || (count > 0 && (!tets || !active)))
+    {
+        set_error("pk_system_set_tets_active: bad argument");
+        return PK_ERR_ARG;
+    }
+    Topology& t = s->topo;
+    std::map<int, uint8_t> want;  // the last entry for a tet wins
+    for (int i = 0; i < count; ++i)
+    {
+        if (tets[i] < 0 || tets[i] >= t.T)
+        {
+            set_error("pk_system_set_tets_active: tet " + std::to_string(tets[i]) + " out of range");
+            return PK_ERR_ARG;
+        }
+        want[tets[i]] = active[i] ? 1 : 0;
+    }
+    const double* vol = t.prm.data() + static_cast<size_t>(P_VOL) * t.T;
+    std::vector<int> ids;
+    std::vector<unsigned char> liveVal, actVal;
+    std::vector<int> off;  // listed tets switched off: their outputs become zeros
+    std::set<int> nodes;
+    for (const auto& kv : want)
+    {
+        const int id = kv.first;
+        if (t.tetActive[id] == kv.second)
+            continue;
+        const uint8_t live = (kv.second && vol[id] > 0.0) ? 1 : 0;
+        if (live && !t.listed[id])
+        {
+            set_error("pk_system_set_tets_active: tet " + std::to_string(id) +
+                      " was not live when the system was created; create it again");
+            return PK_ERR_REBUILD;
+        }
+        ids.push_back(id);
+        liveVal.push_back(live);
+        actVal.push_back(kv.second);
+        if (!live && t.listed[id])
+            off.push_back(id);
+        for (int a = 0; a < 4; ++a)
+            nodes.insert(t.tets[4 * static_cast<size_t>(id) + a]);
+    }
+    if (ids.empty())
+        return PK_OK;
+    // lumped masses of the touched nodes as the creation sums them (tet order
+    // over the tets live now, TetMeshPhysicsComponent.cpp:127-152); a node
+    // that would join or leave the dynamic set changes the system's rows
+    std::vector<uint8_t> liveNow(t.live);
+    for (size_t i = 0; i < ids.size(); ++i)
+        liveNow[ids[i]] = liveVal[i];
+    std::vector<std::pair<int, double>> masses;
+    for (const int n : nodes)
+    {
+        if (n < 0 || n >= t.N || t.nodeFixed[n])
+            continue;
+        double m = 0.0;
+        for (int k = t.astart[n]; k < t.astart[n + 1]; ++k)
+        {
+            const int tt = t.alist[k] >> 2;
+            if (liveNow[tt])
+                m += t.tetShare[tt];
+        }
+        const bool dyn = t.nodeActive[n] && std::isfinite(m) && m > 0.0;
+        if (dyn != (t.dyn[n] >= 0))
+        {
+            set_error("pk_system_set_tets_active: node " + std::to_string(n) +
+                      (dyn ? " joins" : " leaves") + " the dynamic set; create the system again");
+            return PK_ERR_REBUILD;
+        }
+        masses.push_back({n, m});
+    }
+    std::vector<int> rowIdx;
+    std::vector<double> rowMass;
+    for (const auto& nm : masses)
+    {
+        t.mass[nm.first] = nm.second;
+        if (t.dyn[nm.first] >= 0)  // a node outside the dynamic set has no row
+        {
+            rowIdx.push_back(t.dyn[nm.first]);
+            rowMass.push_back(nm.second);
+        }
+    }
+    for (size_t i = 0; i < ids.size(); ++i)
+    {
+        t.live[ids[i]] = liveVal[i];
+        t.tetActive[ids[i]] = actVal[i];
+    }
+    hipStream_t st = s->stream;
+    int rc;
+    pk::DevBuf<int> dIds, dOff, dRows;
+    pk::DevBuf<unsigned char> dLive, dAct;
+    pk::DevBuf<double> dMass;
+    if ((rc = dIds.upload(ids, st)) != PK_OK) return rc;
+    if ((rc = dLive.upload(liveVal, st)) != PK_OK) return rc;
+    if ((rc = dAct.upload(actVal, st)) != PK_OK) return rc;
+    if ((rc = dOff.upload(off, st)) != PK_OK) return rc;
+    if ((rc = dRows.upload(rowIdx, st)) != PK_OK) return rc;
+    if ((rc = dMass.upload(rowMass, st)) != PK_OK) return rc;
+    launch_tet_flags(st, static_cast<int>(ids.size()), dIds.p, dLive.p, dAct.p, s->live.p, s->tact.p);
+    launch_slot_scatter(st, static_cast<int>(rowIdx.size()), dRows.p, dMass.p, s->mrow.p);
+    launch_zero_tets(s->view(), static_cast<int>(off.size()), dOff.p);
+    PK_HIP(hipGetLastError());
+    PK_HIP(hipStreamSynchronize(st));
     s->tapeValid = false;
     return PK_OK;
 }

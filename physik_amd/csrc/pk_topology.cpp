@@ -157,8 +157,12 @@ int build_topology(const pk_mesh_desc& d, Topology& topo, std::string& err, bool
     auto fixed = [&](int n) { return d.node_fixed && d.node_fixed[n]; };
     auto nodeActive = [&](int n) { return !d.node_active || d.node_active[n]; };
     topo.nodeActive.resize(static_cast<size_t>(N));
+    topo.nodeFixed.assign(static_cast<size_t>(N), 0);
     for (int n = 0; n < N; ++n)
+    {
         topo.nodeActive[n] = nodeActive(n) ? 1 : 0;
+        topo.nodeFixed[n] = fixed(n) ? 1 : 0;
+    }
     if (d.extra_pair_count < 0 || (d.extra_pair_count > 0 && !d.extra_pairs))
     {
         err = "mesh descriptor: bad extra pairs";
@@ -176,18 +180,23 @@ int build_topology(const pk_mesh_desc& d, Topology& topo, std::string& err, bool
     for (int t = 0; t < T; ++t)
         topo.tetActive[t] = tetActive(t) ? 1 : 0;
     topo.live.assign(static_cast<size_t>(T), 0);
+    topo.tetShare.assign(static_cast<size_t>(T), 0.0);
     topo.mass.assign(static_cast<size_t>(N), 0.0);
     for (int t = 0; t < T; ++t)
     {
+        const double rho = std::max(0.0, d.density[t]);
+        if (std::isfinite(rho) && std::isfinite(vol[t]) && vol[t] > 0.0)
+        {
+            const double nm = rho * vol[t] * static_cast<double>(0.25f);
+            if (std::isfinite(nm) && nm > 0.0)
+                topo.tetShare[t] = nm;
+        }
         if (!tetActive(t))
             continue;
         if (vol[t] > 0.0)
             topo.live[t] = 1;
-        const double rho = std::max(0.0, d.density[t]);
-        if (!std::isfinite(rho) || !std::isfinite(vol[t]) || !(vol[t] > 0.0))
-            continue;
-        const double nm = rho * vol[t] * static_cast<double>(0.25f);
-        if (!std::isfinite(nm) || !(nm > 0.0))
+        const double nm = topo.tetShare[t];
+        if (!(nm > 0.0))
             continue;
         for (int a = 0; a < 4; ++a)
         {
@@ -196,6 +205,7 @@ int build_topology(const pk_mesh_desc& d, Topology& topo, std::string& err, bool
                 topo.mass[n] += nm;
         }
     }
+    topo.listed = topo.live;
     topo.massErrorNode = -1;
     for (int n = 0; n < N; ++n)
         if (nodeActive(n) && !fixed(n) && !(std::isfinite(topo.mass[n]) && topo.mass[n] > 0.0))

@@ -64,6 +64,8 @@ struct Topology
     std::vector<int> tets;         // 4T
     std::vector<uint8_t> live;     // T: the tet produces an element (active, valid, V > 0)
     std::vector<uint8_t> tetActive;  // T: the tet's active flag
+    std::vector<uint8_t> listed;     // T: live when the gather lists were built (in them)
+    std::vector<double> tetShare;    // T: the lumped mass each corner gets from the tet while live
     std::vector<double> prm;       // P_COUNT * T
     std::vector<double> poisson;   // T, kept to refresh Lame parameters when E changes
     std::vector<double> mass;      // N lumped masses (fixed nodes get none)
@@ -82,6 +84,7 @@ struct Topology
     std::vector<int> astart, alist;      // every node -> packed (tet, corner) over live tets, increasing tet
     std::vector<int> nodeGroup;          // N -> body
     std::vector<uint8_t> nodeActive;     // N
+    std::vector<uint8_t> nodeFixed;      // N
     std::vector<int> scatter;            // optional T*16 upper slot of each local pair, -1 if not dynamic
 
     long long nnzu() const { return static_cast<long long>(ucol.size()); }

@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libphysik_amd.so")
 
-PK_OK, PK_ERR_ARG, PK_ERR_HIP, PK_ERR_MASS, PK_ERR_NO_DEVICE = 0, 1, 2, 3, 4
+PK_OK, PK_ERR_ARG, PK_ERR_HIP, PK_ERR_MASS, PK_ERR_NO_DEVICE, PK_ERR_REBUILD = 0, 1, 2, 3, 4, 5
 PK_FP64, PK_FP32_STORAGE, PK_FP32_UPPER, PK_FP64_UPPER = 0, 1, 2, 3
 GROUP_INTEGRATED, GROUP_SOLVE_FAILED, GROUP_NONFINITE, GROUP_NO_DYNAMIC = 0, 1, 2, 3
 K_ELEMENT, K_ASSEMBLE, K_RHS, K_SPMV, K_UPDATE, K_DIRECTION, K_INTEGRATE, K_CONTACT = range(8)
@@ -87,6 +87,7 @@ PK_SIGNATURES = {
     "pk_system_set_graphs": (C.c_int, [_vp, C.c_int]),
     "pk_system_set_connections": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "pk_system_set_spheres": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "pk_system_set_tets_active": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "pk_system_set_matrix_store": (C.c_int, [_vp, C.c_longlong]),
     "pk_system_spmv_work": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                       C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
@@ -153,6 +154,7 @@ PHYSIK_SIGNATURES = {
     "PHYSIK_GetCollisionSphereOverlapCount": (C.c_int, [_W, _H]),
     "PHYSIK_GetCollisionSphereOverlaps": (C.c_int, [_W, _H, C.POINTER(PhysikCollisionSphereOverlap), C.c_int]),
     "PHYSIK_SetCollisionComponentKinematicTarget": (None, [_W, _H, C.c_float, C.c_float, C.c_float]),
+    "PHYSIK_AMD_GetSystemBuildCount": (C.c_int, [_W]),
     "PHYSIK_GetPointConnectionCount": (C.c_int, [_W]),
     "PHYSIK_AddPointConnection": (None, [_W, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                          C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]),

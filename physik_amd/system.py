@@ -187,6 +187,18 @@ class System:
         N.check(self.lib.pk_system_set_spheres(self.h, n, N.ptr(c), N.ptr(r), N.ptr(k), N.ptr(dm)),
                 "pk_system_set_spheres")
 
+    def set_tets_active(self, tets, active):
+        """Switches tets on or off in place. Returns False, with nothing
+        changed, when the switch would change the dynamic nodes (or turn on a
+        tet not live at creation): the system must then be created again."""
+        ids = np.ascontiguousarray(np.atleast_1d(tets), dtype=np.int32)
+        flags = np.ascontiguousarray(np.broadcast_to(np.asarray(active, np.uint8), ids.shape))
+        rc = self.lib.pk_system_set_tets_active(self.h, len(ids), N.ptr(ids), N.ptr(flags))
+        if rc == N.PK_ERR_REBUILD:
+            return False
+        N.check(rc, "pk_system_set_tets_active")
+        return True
+
     def set_matrix_store(self, max_bytes):
         """Bytes the reverse mode may keep for a checkpoint segment's matrices
         (default 8 GiB); 0 re-assembles every reverse substep instead."""

@@ -189,6 +189,9 @@ class World:
     def DeactivateTet(self, comp, t):
         self.lib.PHYSIK_DeactivateTet(self.h, comp, t)
 
+    def GetSystemBuildCount(self):
+        return self.lib.PHYSIK_AMD_GetSystemBuildCount(self.h)
+
     def GetActiveTetCount(self, comp):
         return self.lib.PHYSIK_GetActiveTetCount(self.h, comp)
 
