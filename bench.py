@@ -161,6 +161,30 @@ def prepare(name, prec_name, args):
     return w, s, dict(zip(("lanes_per_row", "spmv_rows", "vector_rows", "unroll"), shape))
 
 
+def frame_export(s, reps=5):
+    """The per-frame readback after a step, measured three ways (median of
+    `reps`): the FP64 state downloaded into new host arrays
+    (pk_system_get_state, 48 B per node), the float frame export copied out
+    (pk_system_export_begin + export_read, 24 B per node through pinned
+    memory), and the export alone up to the moment the host may read it."""
+    def med(f):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f()
+            ts.append(time.perf_counter() - t0)
+        return 1e3 * sorted(ts)[len(ts) // 2]
+
+    s.state()
+    s.export_begin()
+    s.export_read()
+    fp64 = med(s.state)
+    exp = med(lambda: (s.export_begin(), s.export_read()))
+    wait = med(lambda: (s.export_begin(), s.export_read(0, 0)))
+    return {"nodes": s.n, "fp64_download_ms": fp64, "float_export_ms": exp, "float_export_wait_ms": wait,
+            "float_export_GBps": 24 * s.n / (wait * 1e6)}
+
+
 def run_plain(name, prec_name, steps, warmup, args):
     """K plain implicit steps from the workload's initial state."""
     w, s, shape = prepare(name, prec_name, args)
@@ -188,6 +212,7 @@ def run_plain(name, prec_name, steps, warmup, args):
         "kernels_ms": kernels_ms(s),
     }
     s.profile(False)
+    out["frame_export"] = frame_export(s)
     s.close()
     return out, w
 

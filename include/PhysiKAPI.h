@@ -215,6 +215,23 @@ extern "C"
     /* Per-tet Young's modulus of a physics component (count = its tet count). */
     PHYSIK_API int PHYSIK_AMD_SetTetMeshYoungModuli(PhysiK::WorldHandle world, PhysiK::ComponentHandle component,
                                                     const double* youngModuli, int count);
+    /* Frame readback (the per-frame readers of the reference: visual
+     * components in World::PostUpdate, World.cpp:539-549, and a host engine's
+     * PHYSIK_GetNodePosition/Velocity loop). After each PHYSIK_Step the GPU
+     * converts the node state to float triples and copies them into pinned
+     * host memory without the host waiting; the first getter of the frame
+     * waits for that copy, and the getters then read it. mask: 1 positions,
+     * 2 velocities, 3 both (the default), 0 none (getters then download the
+     * FP64 state). Returns 1, or 0 for a bad handle or mask. */
+    PHYSIK_API int PHYSIK_AMD_SetFrameExport(PhysiK::WorldHandle world, int mask);
+    /* Bulk PHYSIK_GetNodePosition / PHYSIK_GetNodeVelocity: nodes
+     * [firstNode, firstNode + count) into out (clipped to the node count),
+     * the same float values the per-node getters return. Returns the number
+     * of nodes written (0 for a bad handle or range). */
+    PHYSIK_API int PHYSIK_AMD_GetNodePositions(PhysiK::WorldHandle world, int firstNode, int count,
+                                               PhysiK::Vec3* outPositions);
+    PHYSIK_API int PHYSIK_AMD_GetNodeVelocities(PhysiK::WorldHandle world, int firstNode, int count,
+                                                PhysiK::Vec3* outVelocities);
     /* Message of the last error raised by this library on this thread. */
     PHYSIK_API const char* PHYSIK_AMD_GetLastError();
 }

@@ -41,7 +41,14 @@ enum pk_status {
     PK_ERR_MASS = 3,      /* a free active node has no mass: the reference throws
                              std::runtime_error here (World.cpp:560-580) */
     PK_ERR_NO_DEVICE = 4, /* no GPU visible */
-    PK_ERR_REBUILD = 5    /* the change needs the system created again (nothing was changed) */
+    PK_ERR_REBUILD = 5,   /* the change needs the system created again (nothing was changed) */
+    PK_ERR_STATE = 6      /* nothing to read: no frame export since the state last changed */
+};
+
+/* What a frame export carries (pk_system_export_begin). */
+enum pk_export {
+    PK_EXPORT_POSITIONS = 1,
+    PK_EXPORT_VELOCITIES = 2
 };
 
 enum pk_precision {
@@ -127,6 +134,25 @@ int pk_system_get_tune(const pk_system* sys, int* lanes_per_row, int* spmv_rows,
 /* Node state, node_count x 3 each; either pointer may be NULL. */
 int pk_system_set_state(pk_system* sys, const double* x, const double* v);
 int pk_system_get_state(pk_system* sys, double* x, double* v);
+
+/* Frame export: the readback after a step for the reference's per-frame
+ * readers (visual components in World::PostUpdate, World.cpp:539-549, and a
+ * host engine's PHYSIK_GetNodePosition/Velocity loop, PhysiKAPI.cpp:1510-1545,
+ * which return float). pk_system_export_begin enqueues on the system's stream
+ * a device conversion of the node state to the reference's float Vec3 layout
+ * (3 floats per node, each the value static_cast<float> gives) and its copy
+ * into pinned host memory that the system owns, and returns without waiting;
+ * `what` is PK_EXPORT_POSITIONS, PK_EXPORT_VELOCITIES or both. 12 bytes per
+ * node and quantity cross PCIe instead of pk_system_get_state's 24.
+ * pk_system_export_view waits for the copy and points at the float arrays
+ * (node_count x 3; pass NULL for a quantity not wanted); the pointers stay
+ * valid until the next export or the destruction of the system.
+ * pk_system_export_read waits likewise and copies nodes [first, first+count).
+ * Both return PK_ERR_STATE when the quantity asked for was not exported since
+ * x or v last changed (a step, pk_system_set_state, record or backward). */
+int pk_system_export_begin(pk_system* sys, int what);
+int pk_system_export_view(pk_system* sys, const float** positions, const float** velocities);
+int pk_system_export_read(pk_system* sys, int first, int count, float* positions, float* velocities);
 
 int pk_system_set_gravity(pk_system* sys, double gx, double gy, double gz);
 /* CG settings, applied to every group (ConjugateGradientSettings,

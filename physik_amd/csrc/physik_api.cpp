@@ -207,6 +207,13 @@ struct World
     bool topoDirty = true;     // rebuild the device system before the next step
     bool hostNewer = false;    // host state changed since the last upload
     bool deviceNewer = false;  // device state changed since the last download
+    // frame export: what each step exports for the float getters (default
+    // both), what the last step did export, and the float arrays once waited on
+    int exportMask = PK_EXPORT_POSITIONS | PK_EXPORT_VELOCITIES;
+    int frameMask = 0;
+    bool frameWaited = false;
+    const float* frameX = nullptr;
+    const float* frameV = nullptr;
 
     ~World()
     {
@@ -221,6 +228,26 @@ struct World
             return;
         if (pk_system_get_state(sys, x.data(), v.data()) == PK_OK)
             deviceNewer = false;
+    }
+    // The float node state of the last step's frame export (`which` is
+    // PK_EXPORT_POSITIONS or PK_EXPORT_VELOCITIES) while the device holds the
+    // newest state; nullptr sends the getter to the host mirror (pull()).
+    // Each value is static_cast<float> of the FP64 state, as from the mirror.
+    const float* frame(int which)
+    {
+        if (!deviceNewer || !sys || !(frameMask & which))
+            return nullptr;
+        if (!frameWaited)
+        {
+            const bool px = (frameMask & PK_EXPORT_POSITIONS) != 0, pv = (frameMask & PK_EXPORT_VELOCITIES) != 0;
+            if (pk_system_export_view(sys, px ? &frameX : nullptr, pv ? &frameV : nullptr) != PK_OK)
+            {
+                frameMask = 0;
+                return nullptr;
+            }
+            frameWaited = true;
+        }
+        return which == PK_EXPORT_POSITIONS ? frameX : frameV;
     }
     void touch_host()
     {
@@ -471,6 +498,8 @@ struct World
         int rc = pk_system_set_solver(sys, tolerance, maxIterations, 1);
         if (rc != PK_OK)
             return rc;
+        frameMask = 0;
+        frameWaited = false;
         rc = pk_system_step(sys, static_cast<double>(frameDt), steps, 1);
         if (!conns.empty())
             pk_system_set_connections(sys, 0, nullptr, nullptr, nullptr, nullptr, nullptr);
@@ -478,6 +507,10 @@ struct World
             return rc;  // the reference leaves the frame's connections when it throws
         conns.clear();  // World::ClearFrameConnections (World.cpp:70)
         deviceNewer = true;
+        // the readback for the frame's readers starts now and overlaps the
+        // host's work until the first getter waits for it
+        if (exportMask && pk_system_export_begin(sys, exportMask) == PK_OK)
+            frameMask = exportMask;
         if (implicit)
         {
             int it = 0, conv = 0, status = 0;
@@ -1142,6 +1175,13 @@ void PHYSIK_GetNodePosition(PhysiK::WorldHandle world, int i, float* outX, float
     World* w = W(world);
     if (!valid_node(w, i))
         return;
+    if (const float* f = w->frame(PK_EXPORT_POSITIONS))
+    {
+        if (outX) *outX = f[3 * i];
+        if (outY) *outY = f[3 * i + 1];
+        if (outZ) *outZ = f[3 * i + 2];
+        return;
+    }
     w->pull();
     if (outX) *outX = static_cast<float>(w->x[3 * i]);
     if (outY) *outY = static_cast<float>(w->x[3 * i + 1]);
@@ -1153,6 +1193,13 @@ void PHYSIK_GetNodeVelocity(PhysiK::WorldHandle world, int i, float* outX, float
     World* w = W(world);
     if (!valid_node(w, i))
         return;
+    if (const float* f = w->frame(PK_EXPORT_VELOCITIES))
+    {
+        if (outX) *outX = f[3 * i];
+        if (outY) *outY = f[3 * i + 1];
+        if (outZ) *outZ = f[3 * i + 2];
+        return;
+    }
     w->pull();
     if (outX) *outX = static_cast<float>(w->v[3 * i]);
     if (outY) *outY = static_cast<float>(w->v[3 * i + 1]);
@@ -1231,6 +1278,51 @@ int PHYSIK_AMD_SetNodeStateF64(PhysiK::WorldHandle world, const double* position
     if (velocities)
         std::memcpy(w->v.data(), velocities, sizeof(double) * 3 * n);
     return n;
+}
+
+int PHYSIK_AMD_SetFrameExport(PhysiK::WorldHandle world, int mask)
+{
+    World* w = W(world);
+    if (!w || mask < 0 || mask > (PK_EXPORT_POSITIONS | PK_EXPORT_VELOCITIES))
+        return 0;
+    w->exportMask = mask;
+    return 1;
+}
+
+static_assert(sizeof(PhysiK::Vec3) == 3 * sizeof(float), "Vec3 is three packed floats");
+
+namespace
+{
+// Nodes [first, first + count) of positions (which = PK_EXPORT_POSITIONS) or
+// velocities as float triples: the frame export while the device holds the
+// newest state, else the host mirror cast to float.
+int copy_node_floats(World* w, int which, int first, int count, PhysiK::Vec3* out)
+{
+    if (!w || !out || first < 0 || count <= 0 || first >= w->nodeCount())
+        return 0;
+    const int n = std::min(count, w->nodeCount() - first);
+    float* dst = &out[0].x;
+    if (const float* f = w->frame(which))
+    {
+        std::memcpy(dst, f + 3 * static_cast<size_t>(first), sizeof(float) * 3 * static_cast<size_t>(n));
+        return n;
+    }
+    w->pull();
+    const std::vector<double>& src = which == PK_EXPORT_POSITIONS ? w->x : w->v;
+    for (size_t k = 0; k < 3 * static_cast<size_t>(n); ++k)
+        dst[k] = static_cast<float>(src[3 * static_cast<size_t>(first) + k]);
+    return n;
+}
+}  // namespace
+
+int PHYSIK_AMD_GetNodePositions(PhysiK::WorldHandle world, int firstNode, int count, PhysiK::Vec3* outPositions)
+{
+    return copy_node_floats(W(world), PK_EXPORT_POSITIONS, firstNode, count, outPositions);
+}
+
+int PHYSIK_AMD_GetNodeVelocities(PhysiK::WorldHandle world, int firstNode, int count, PhysiK::Vec3* outVelocities)
+{
+    return copy_node_floats(W(world), PK_EXPORT_VELOCITIES, firstNode, count, outVelocities);
 }
 
 const char* PHYSIK_AMD_GetLastError() { return g_apiError.c_str(); }

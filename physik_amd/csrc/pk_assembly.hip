@@ -708,6 +708,33 @@ __global__ __launch_bounds__(NT) void k_slot_scatter(int n, const int* __restric
         cslot[idx[i]] = val ? val[i] : 0.0;
 }
 
+// Frame export: the node state as the reference's float Vec3 triples
+// (Vec3.h:7-12, the value static_cast<float> gives, round to nearest even).
+// Positions fill out[0 .. n3), velocities out[n3 .. 2 n3); each lane converts
+// two consecutive doubles so a wave reads 1 KiB and writes 512 B contiguous.
+__global__ __launch_bounds__(NT) void k_export_frame(long long n3, const double* __restrict__ x,
+                                                     const double* __restrict__ v, float* __restrict__ out)
+{
+    const long long pairs = (n3 + 1) / 2;
+    const long long stride = static_cast<long long>(gridDim.x) * NT;
+    for (long long p = static_cast<long long>(blockIdx.x) * NT + threadIdx.x; p < 2 * pairs; p += stride)
+    {
+        const bool vel = p >= pairs;
+        const long long i = 2 * (vel ? p - pairs : p);
+        const double* src = vel ? v : x;
+        float* dst = out + (vel ? n3 : 0);
+        if (src == nullptr)
+            continue;
+        if (i + 1 < n3)
+        {
+            dst[i] = static_cast<float>(src[i]);
+            dst[i + 1] = static_cast<float>(src[i + 1]);
+        }
+        else
+            dst[i] = static_cast<float>(src[i]);
+    }
+}
+
 __global__ __launch_bounds__(NT) void k_zero_rows(int n, const int* __restrict__ rows, double* __restrict__ crhs)
 {
     const int i = blockIdx.x * NT + threadIdx.x;
@@ -953,6 +980,16 @@ void launch_zero_rows(hipStream_t stream, int n, const int* rows, double* crhs)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_zero_rows, dim3((n + NT - 1) / NT), dim3(NT), 0, stream, n, rows, crhs);
+}
+
+void launch_export_frame(hipStream_t stream, long long n3, const double* x, const double* v, float* out)
+{
+    if (n3 <= 0)
+        return;
+    const long long work = 2 * ((n3 + 1) / 2);
+    const long long blocks = (work + NT - 1) / NT;
+    const long long grid = blocks < 8192 ? blocks : 8192;
+    hipLaunchKernelGGL(k_export_frame, dim3(static_cast<unsigned>(grid)), dim3(NT), 0, stream, n3, x, v, out);
 }
 
 void launch_gdot(const DeviceView& v, double* prm)

@@ -202,6 +202,27 @@ __global__ __launch_bounds__(NT) void k_dir(const int* kb, int k0, TileView vt, 
     stamp_end(stamp);
 }
 
+// Row i of a 3x3 block times (d0, d1, d2) with the rounding pinned by explicit
+// fused multiply-adds: a_i0 d0, then + a_i1 d1, then + a_i2 d2. Every SpMV
+// form (full or upper-triangle storage, any launch shape) forms its products
+// through this, so they agree bit for bit whatever the compiler would
+// otherwise contract.
+template <class M>
+__device__ inline double brow(const M* a, int i, double d0, double d1, double d2)
+{
+    return __builtin_fma(static_cast<double>(a[3 * i + 2]), d2,
+                         __builtin_fma(static_cast<double>(a[3 * i + 1]), d1, static_cast<double>(a[3 * i]) * d0));
+}
+
+// d_row . q_row for a tile's partial, pinned the same way.
+template <class M>
+__device__ inline double dot_dq(const M* __restrict__ d, int row, M q0, M q1, M q2)
+{
+    return __builtin_fma(static_cast<double>(d[3 * row + 2]), static_cast<double>(q2),
+                         __builtin_fma(static_cast<double>(d[3 * row + 1]), static_cast<double>(q1),
+                                       static_cast<double>(d[3 * row]) * static_cast<double>(q0)));
+}
+
 // Block k of the full BSR: its 9 values (row-major) and its column.
 __device__ inline void load_block(const double* __restrict__ vals, size_t F, const int* __restrict__ fcol, int k,
                                   double* a, int& col)
@@ -273,9 +294,7 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
             const bool has = kb[u] < ke[u];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                y[u][i] = has ? static_cast<double>(a[u][3 * i]) * d0 + static_cast<double>(a[u][3 * i + 1]) * d1 +
-                                    static_cast<double>(a[u][3 * i + 2]) * d2
-                              : 0.0;
+                y[u][i] = has ? brow(a[u], i, d0, d1, d2) : 0.0;
         }
         // rows longer than LPR blocks
 #pragma unroll
@@ -289,8 +308,7 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
                              d2 = static_cast<double>(d[3 * c + 2]);
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
-                    y[u][i] += static_cast<double>(b[3 * i]) * d0 + static_cast<double>(b[3 * i + 1]) * d1 +
-                               static_cast<double>(b[3 * i + 2]) * d2;
+                    y[u][i] += brow(b, i, d0, d1, d2);
             }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -309,9 +327,7 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
                 q[3 * row] = q0;
                 q[3 * row + 1] = q1;
                 q[3 * row + 2] = q2;
-                pd += static_cast<double>(d[3 * row]) * static_cast<double>(q0) +
-                      static_cast<double>(d[3 * row + 1]) * static_cast<double>(q1) +
-                      static_cast<double>(d[3 * row + 2]) * static_cast<double>(q2);
+                pd += dot_dq(d, row, q0, q1, q2);
             }
         }
     }
@@ -364,12 +380,15 @@ __device__ inline void load_upper_block(const M* __restrict__ rec, const int2* _
             a[3 * i + j] = tr ? w[3 * j + i] : w[3 * i + j];
 }
 
-// spmv_body over upper-only FP32 storage: the same lanes, positions, products
-// and reduction order as the full-storage body, each upper record read from
-// HBM by its own row and again, transposed, by the row of its mirror (at most
-// a layer of rows later on the same XCD, so mostly from L2). (Issuing the
-// loads of two positions per row before their use was slower: the extra
-// registers cost more occupancy than the overlap gained.)
+// spmv_body over upper-only storage: the same lanes, positions, products and
+// reduction order as the full-storage body, each upper record read by its own
+// row and again, transposed, by the row of its mirror (at most a layer of
+// rows later on the same XCD). With tiles of several passes about half of the
+// mirror reads miss L2 (the two reads land in different passes of their
+// tiles): the launch moves ~1.5x its upper-storage bytes. Tiles of one pass,
+// or every position of a row loaded up front, keep them in L2 (1.0x) but ran
+// slower on MI355X: this kernel is bound by its chain of dependent loads and
+// by the address unit, not by HBM bytes, and those variants lose occupancy.
 template <class M, int LPR, int U>
 __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __restrict__ rowptr,
                                        const int2* __restrict__ lower, const M* __restrict__ rec,
@@ -423,9 +442,7 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
             const bool has = kb[u] < ke[u];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                y[u][i] = has ? static_cast<double>(a[u][3 * i]) * d0 + static_cast<double>(a[u][3 * i + 1]) * d1 +
-                                    static_cast<double>(a[u][3 * i + 2]) * d2
-                              : 0.0;
+                y[u][i] = has ? brow(a[u], i, d0, d1, d2) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -438,8 +455,7 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
                              d2 = static_cast<double>(d[3 * c + 2]);
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
-                    y[u][i] += static_cast<double>(b[3 * i]) * d0 + static_cast<double>(b[3 * i + 1]) * d1 +
-                               static_cast<double>(b[3 * i + 2]) * d2;
+                    y[u][i] += brow(b, i, d0, d1, d2);
             }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -458,9 +474,7 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
                 q[3 * row] = q0;
                 q[3 * row + 1] = q1;
                 q[3 * row + 2] = q2;
-                pd += static_cast<double>(d[3 * row]) * static_cast<double>(q0) +
-                      static_cast<double>(d[3 * row + 1]) * static_cast<double>(q1) +
-                      static_cast<double>(d[3 * row + 2]) * static_cast<double>(q2);
+                pd += dot_dq(d, row, q0, q1, q2);
             }
         }
     }
@@ -633,8 +647,7 @@ __global__ __launch_bounds__(NT) void k_spmv_plain(int nd, const int* __restrict
         load_block(vals, F, fcol, k, a, c);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            acc[i] += static_cast<double>(a[3 * i]) * xin[3 * c] + static_cast<double>(a[3 * i + 1]) * xin[3 * c + 1] +
-                      static_cast<double>(a[3 * i + 2]) * xin[3 * c + 2];
+            acc[i] += brow(a, i, xin[3 * c], xin[3 * c + 1], xin[3 * c + 2]);
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -661,8 +674,7 @@ __global__ __launch_bounds__(NT) void k_spmv_plain_upper(int nd, const int2* __r
         load_upper_block(rec, lower, r0.x, r0.y, nl, p, a, c);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            acc[i] += static_cast<double>(a[3 * i]) * xin[3 * c] + static_cast<double>(a[3 * i + 1]) * xin[3 * c + 1] +
-                      static_cast<double>(a[3 * i + 2]) * xin[3 * c + 2];
+            acc[i] += brow(a, i, xin[3 * c], xin[3 * c + 1], xin[3 * c + 2]);
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)

@@ -354,6 +354,9 @@ void launch_tet_flags(hipStream_t stream, int n, const int* ids, const unsigned 
 void launch_slot_scatter(hipStream_t stream, int n, const int* idx, const double* val, double* cslot);
 // crhs[3 r .. 3 r + 2] = 0 for the n rows listed.
 void launch_zero_rows(hipStream_t stream, int n, const int* rows, double* crhs);
+// out[i] = (float) x[i] and out[n3 + i] = (float) v[i] for i < n3 (a null
+// source is skipped).
+void launch_export_frame(hipStream_t stream, long long n3, const double* x, const double* v, float* out);
 template <class M> void launch_dir(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp);
 template <class M>
 void launch_spmv(const DeviceView& v, const int* kb, int k0, int lanesPerRow, int unroll,

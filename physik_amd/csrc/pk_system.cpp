@@ -147,6 +147,16 @@ struct pk_system
     int* pinned = nullptr;
     hipEvent_t pollEv[2] = {nullptr, nullptr};
 
+    // frame export (pk_system_export_begin): float copies of x and v, made on
+    // the device and copied into pinned host memory; the copy completes at
+    // frameEv. stateVer changes with every write of x or v, so an export made
+    // before it is stale.
+    float* framePinned = nullptr;
+    pk::DevBuf<float> frameDev;
+    hipEvent_t frameEv = nullptr;
+    int frameWhat = 0;
+    unsigned long long stateVer = 0, frameVer = ~0ull;
+
     // reverse mode: checkpoints (x, v) every `tapeEvery` substeps of a
     // recorded run, the states of the segment being reversed, per-body
     // integrate flags of that segment, the adjoints and the gradient
@@ -227,6 +237,10 @@ struct pk_system
                 (void)hipEventDestroy(e);
         if (pinned)
             (void)hipHostFree(pinned);
+        if (framePinned)
+            (void)hipHostFree(framePinned);
+        if (frameEv)
+            (void)hipEventDestroy(frameEv);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -964,6 +978,7 @@ int pk_system_set_state(pk_system* s, const double* x, const double* v)
 {
     if (!s)
         return PK_ERR_ARG;
+    ++s->stateVer;
     const size_t bytes = static_cast<size_t>(s->topo.N) * 3 * sizeof(double);
     if (x && bytes)
         PK_HIP(hipMemcpyAsync(s->x.p, x, bytes, hipMemcpyHostToDevice, s->stream));
@@ -983,6 +998,89 @@ int pk_system_get_state(pk_system* s, double* x, double* v)
     if (v && bytes)
         PK_HIP(hipMemcpyAsync(v, s->v.p, bytes, hipMemcpyDeviceToHost, s->stream));
     PK_HIP(hipStreamSynchronize(s->stream));
+    return PK_OK;
+}
+
+int pk_system_export_begin(pk_system* s, int what)
+{
+    if (!s || what < 1 || what > (PK_EXPORT_POSITIONS | PK_EXPORT_VELOCITIES))
+    {
+        set_error("pk_system_export_begin: what is PK_EXPORT_POSITIONS, PK_EXPORT_VELOCITIES or both");
+        return PK_ERR_ARG;
+    }
+    const size_t n3 = static_cast<size_t>(s->topo.N) * 3;
+    s->frameVer = ~0ull;
+    s->frameWhat = 0;
+    if (n3 == 0)
+        return PK_OK;
+    if (!s->framePinned)
+        PK_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->framePinned), 2 * n3 * sizeof(float),
+                             hipHostMallocDefault));
+    if (!s->frameEv)
+        PK_HIP(hipEventCreateWithFlags(&s->frameEv, hipEventDisableTiming));
+    int rc;
+    if ((rc = s->frameDev.ensure(2 * n3)) != PK_OK)
+        return rc;
+    const bool px = (what & PK_EXPORT_POSITIONS) != 0, pv = (what & PK_EXPORT_VELOCITIES) != 0;
+    pk::launch_export_frame(s->stream, static_cast<long long>(n3), px ? s->x.p : nullptr, pv ? s->v.p : nullptr,
+                            s->frameDev.p);
+    PK_HIP(hipGetLastError());
+    if (px && pv)
+        PK_HIP(hipMemcpyAsync(s->framePinned, s->frameDev.p, 2 * n3 * sizeof(float), hipMemcpyDeviceToHost,
+                              s->stream));
+    else
+    {
+        const size_t off = px ? 0 : n3;
+        PK_HIP(hipMemcpyAsync(s->framePinned + off, s->frameDev.p + off, n3 * sizeof(float), hipMemcpyDeviceToHost,
+                              s->stream));
+    }
+    PK_HIP(hipEventRecord(s->frameEv, s->stream));
+    s->frameWhat = what;
+    s->frameVer = s->stateVer;
+    return PK_OK;
+}
+
+int pk_system_export_view(pk_system* s, const float** positions, const float** velocities)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    if (positions)
+        *positions = nullptr;
+    if (velocities)
+        *velocities = nullptr;
+    const bool wantX = positions != nullptr, wantV = velocities != nullptr;
+    if (s->frameVer != s->stateVer || (wantX && !(s->frameWhat & PK_EXPORT_POSITIONS)) ||
+        (wantV && !(s->frameWhat & PK_EXPORT_VELOCITIES)))
+    {
+        set_error("pk_system_export_view: no export of that kind since the state last changed");
+        return PK_ERR_STATE;
+    }
+    const size_t n3 = static_cast<size_t>(s->topo.N) * 3;
+    if (n3 > 0)
+        PK_HIP(hipEventSynchronize(s->frameEv));
+    if (wantX)
+        *positions = s->framePinned;
+    if (wantV)
+        *velocities = n3 > 0 ? s->framePinned + n3 : s->framePinned;
+    return PK_OK;
+}
+
+int pk_system_export_read(pk_system* s, int first, int count, float* positions, float* velocities)
+{
+    if (!s || first < 0 || count < 0 || static_cast<long long>(first) + count > s->topo.N)
+    {
+        set_error("pk_system_export_read: node range outside the system");
+        return PK_ERR_ARG;
+    }
+    const float *px = nullptr, *pv = nullptr;
+    const int rc = pk_system_export_view(s, positions ? &px : nullptr, velocities ? &pv : nullptr);
+    if (rc != PK_OK)
+        return rc;
+    const size_t off = 3 * static_cast<size_t>(first), len = 3 * static_cast<size_t>(count) * sizeof(float);
+    if (positions && len)
+        std::memcpy(positions, px + off, len);
+    if (velocities && len)
+        std::memcpy(velocities, pv + off, len);
     return PK_OK;
 }
 
@@ -1050,6 +1148,7 @@ int pk_system_step(pk_system* s, double dt, int substeps, int frames)
     }
     if (!(dt > 0.0))
         return PK_OK;  // World::Step does nothing for dt <= 0
+    ++s->stateVer;
     const double h = dt / substeps;
     for (int f = 0; f < frames; ++f)
         for (int k = 0; k < substeps; ++k)
@@ -1284,6 +1383,7 @@ int pk_system_record(pk_system* s, double dt, int steps, int every)
         set_error("pk_system_record: no dynamic node");
         return PK_ERR_ARG;
     }
+    ++s->stateVer;
     return is_f32(s) ? record_run<float>(s, dt, steps, every)
                                            : record_run<double>(s, dt, steps, every);
 }
@@ -1298,6 +1398,7 @@ int pk_system_backward(pk_system* s, const double* xbar, const double* vbar, dou
         set_error("pk_system_backward: no recorded run (call pk_system_record after the last change)");
         return PK_ERR_ARG;
     }
+    ++s->stateVer;  // the replays write x and v (the end state is restored, bit for bit)
     const Topology& t = s->topo;
     const size_t n3 = static_cast<size_t>(t.N) * 3;
     int rc;

@@ -14,7 +14,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libphysik_amd.so")
 
-PK_OK, PK_ERR_ARG, PK_ERR_HIP, PK_ERR_MASS, PK_ERR_NO_DEVICE, PK_ERR_REBUILD = 0, 1, 2, 3, 4, 5
+PK_OK, PK_ERR_ARG, PK_ERR_HIP, PK_ERR_MASS, PK_ERR_NO_DEVICE, PK_ERR_REBUILD, PK_ERR_STATE = 0, 1, 2, 3, 4, 5, 6
+PK_EXPORT_POSITIONS, PK_EXPORT_VELOCITIES = 1, 2
 PK_FP64, PK_FP32_STORAGE, PK_FP32_UPPER, PK_FP64_UPPER = 0, 1, 2, 3
 GROUP_INTEGRATED, GROUP_SOLVE_FAILED, GROUP_NONFINITE, GROUP_NO_DYNAMIC = 0, 1, 2, 3
 K_ELEMENT, K_ASSEMBLE, K_RHS, K_SPMV, K_UPDATE, K_DIRECTION, K_INTEGRATE, K_CONTACT = range(8)
@@ -76,6 +77,9 @@ PK_SIGNATURES = {
                                      C.POINTER(C.c_int)]),
     "pk_system_set_state": (C.c_int, [_vp, _vp, _vp]),
     "pk_system_get_state": (C.c_int, [_vp, _vp, _vp]),
+    "pk_system_export_begin": (C.c_int, [_vp, C.c_int]),
+    "pk_system_export_view": (C.c_int, [_vp, _vp, _vp]),
+    "pk_system_export_read": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
     "pk_system_set_gravity": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),
     "pk_system_set_solver": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
     "pk_system_set_mode": (C.c_int, [_vp, C.c_int]),
@@ -169,6 +173,9 @@ PHYSIK_SIGNATURES = {
     "PHYSIK_AMD_SetNodeStateF64": (C.c_int, [_W, _vp, _vp, C.c_int]),
     "PHYSIK_AMD_GenerateTetMeshF64": (_vp, [_vp, C.c_int, _vp, C.c_int]),
     "PHYSIK_AMD_SetTetMeshYoungModuli": (C.c_int, [_W, _H, _vp, C.c_int]),
+    "PHYSIK_AMD_SetFrameExport": (C.c_int, [_W, C.c_int]),
+    "PHYSIK_AMD_GetNodePositions": (C.c_int, [_W, C.c_int, C.c_int, _vp]),
+    "PHYSIK_AMD_GetNodeVelocities": (C.c_int, [_W, C.c_int, C.c_int, _vp]),
     "PHYSIK_AMD_GetLastError": (C.c_char_p, []),
 }
 

@@ -117,6 +117,27 @@ class System:
         N.check(self.lib.pk_system_get_state(self.h, N.ptr(x), N.ptr(v)), "pk_system_get_state")
         return x, v
 
+    def export_begin(self, positions=True, velocities=True):
+        """Starts the float frame export (pk_system_export_begin); returns at once."""
+        what = (N.PK_EXPORT_POSITIONS if positions else 0) | (N.PK_EXPORT_VELOCITIES if velocities else 0)
+        N.check(self.lib.pk_system_export_begin(self.h, what), "pk_system_export_begin")
+
+    def export_read(self, first=0, count=None, positions=True, velocities=True):
+        """Waits for the frame export and returns float32 (count, 3) arrays
+        (None for a quantity not asked for); raises on PK_ERR_STATE."""
+        count = self.n - first if count is None else count
+        x = np.zeros((count, 3), dtype=np.float32) if positions else None
+        v = np.zeros((count, 3), dtype=np.float32) if velocities else None
+        N.check(self.lib.pk_system_export_read(self.h, first, count, N.ptr(x), N.ptr(v)), "pk_system_export_read")
+        return x, v
+
+    def export_status(self, first=0, count=None, positions=True, velocities=True):
+        """Status of pk_system_export_read for the same arguments (no raise)."""
+        count = self.n - first if count is None else count
+        x = np.zeros((max(count, 1), 3), dtype=np.float32) if positions else None
+        v = np.zeros((max(count, 1), 3), dtype=np.float32) if velocities else None
+        return self.lib.pk_system_export_read(self.h, first, count, N.ptr(x), N.ptr(v))
+
     def set_gravity(self, g):
         N.check(self.lib.pk_system_set_gravity(self.h, *[float(a) for a in g]), "pk_system_set_gravity")
 

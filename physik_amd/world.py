@@ -206,6 +206,20 @@ class World:
         y = np.ascontiguousarray(young, dtype=np.float64)
         return self.lib.PHYSIK_AMD_SetTetMeshYoungModuli(self.h, comp, N.ptr(y), y.size)
 
+    def SetFrameExport(self, mask):
+        return self.lib.PHYSIK_AMD_SetFrameExport(self.h, mask)
+
+    def GetNodePositions(self, first, count):
+        """PHYSIK_AMD_GetNodePositions: float32 (n, 3) array of the nodes written."""
+        out = np.zeros((max(count, 0), 3), dtype=np.float32)
+        n = self.lib.PHYSIK_AMD_GetNodePositions(self.h, first, count, N.ptr(out) if count > 0 else None)
+        return out[:n]
+
+    def GetNodeVelocities(self, first, count):
+        out = np.zeros((max(count, 0), 3), dtype=np.float32)
+        n = self.lib.PHYSIK_AMD_GetNodeVelocities(self.h, first, count, N.ptr(out) if count > 0 else None)
+        return out[:n]
+
     def node_state(self, count):
         x = np.zeros((count, 3))
         v = np.zeros((count, 3))
