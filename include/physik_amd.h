@@ -131,6 +131,15 @@ int pk_system_sizes(const pk_system* sys, int* nodes, int* tets, int* groups, in
 int pk_system_tune(pk_system* sys, int lanes_per_row, int spmv_rows, int vector_rows, int unroll);
 int pk_system_get_tune(const pk_system* sys, int* lanes_per_row, int* spmv_rows, int* vector_rows, int* unroll);
 
+/* SpMV staging (no reference counterpart): enable != 0 (the default) lets
+ * each SpMV workgroup copy the window of the direction vector its rows
+ * reference into LDS (up to 20 KB) and read the direction blocks from there;
+ * tiles whose window is wider gather from global memory. Results are
+ * identical either way. get reports the setting and the rows a workgroup
+ * stages (0: none). */
+int pk_system_set_spmv_staging(pk_system* sys, int enable);
+int pk_system_get_spmv_staging(const pk_system* sys, int* enabled, int* rows);
+
 /* Node state, node_count x 3 each; either pointer may be NULL. */
 int pk_system_set_state(pk_system* sys, const double* x, const double* v);
 int pk_system_get_state(pk_system* sys, double* x, double* v);

@@ -214,13 +214,55 @@ __device__ inline double brow(const M* a, int i, double d0, double d1, double d2
                          __builtin_fma(static_cast<double>(a[3 * i + 1]), d1, static_cast<double>(a[3 * i]) * d0));
 }
 
-// d_row . q_row for a tile's partial, pinned the same way.
+// Where an SpMV reads the direction vector d: global memory, or the window
+// of rows [base, base + len) of it that the workgroup staged in LDS.
 template <class M>
-__device__ inline double dot_dq(const M* __restrict__ d, int row, M q0, M q1, M q2)
+struct GlobalDir
 {
-    return __builtin_fma(static_cast<double>(d[3 * row + 2]), static_cast<double>(q2),
-                         __builtin_fma(static_cast<double>(d[3 * row + 1]), static_cast<double>(q1),
-                                       static_cast<double>(d[3 * row]) * static_cast<double>(q0)));
+    const M* __restrict__ d;
+    __device__ double at(int c, int j) const { return static_cast<double>(d[3 * c + j]); }
+};
+
+template <class M>
+struct StagedDir
+{
+    const M* lds;
+    int base;
+    __device__ double at(int c, int j) const { return static_cast<double>(lds[3 * (c - base) + j]); }
+};
+
+// d_row . q_row for a tile's partial, pinned the same way.
+template <class M, class Dir>
+__device__ inline double dot_dq(const Dir& d, int row, M q0, M q1, M q2)
+{
+    return __builtin_fma(d.at(row, 2), static_cast<double>(q2),
+                         __builtin_fma(d.at(row, 1), static_cast<double>(q1), d.at(row, 0) * static_cast<double>(q0)));
+}
+
+// Dynamic LDS of the SpMV launches: the staged window of d.
+extern __shared__ __align__(16) unsigned char pk_stage_lds[];
+
+// Copies rows [w.x, w.y] of d into LDS with coalesced loads (four in flight
+// per thread) and waits for the workgroup; returns the staged view.
+template <class M>
+__device__ inline StagedDir<M> stage_window(const M* __restrict__ d, int2 w)
+{
+    M* lds = reinterpret_cast<M*>(pk_stage_lds);
+    const M* src = d + 3 * static_cast<size_t>(w.x);
+    const int n = 3 * (w.y - w.x + 1);
+    for (int i0 = threadIdx.x; i0 < n; i0 += 4 * NT)
+    {
+        M v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            v[j] = i0 + j * NT < n ? src[i0 + j * NT] : M(0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (i0 + j * NT < n)
+                lds[i0 + j * NT] = v[j];
+    }
+    __syncthreads();
+    return StagedDir<M>{lds, w.x};
 }
 
 // Block k of the full BSR: its 9 values (row-major) and its column.
@@ -243,22 +285,11 @@ __device__ inline void load_block(const float* __restrict__ vals, size_t, const 
     col = __float_as_int(w4.y);
 }
 
-// q = A d_k for the tile's rows and the tile's d.q partial. A body whose loop
-// ended at or before k (decided by k_dir(k) or earlier) is skipped.
-template <class M, int LPR, int U>
-__device__ inline void spmv_body(int k, const TileView& st, const int* __restrict__ fstart,
-                                 const int* __restrict__ fcol, const M* __restrict__ vals, long long nnzf,
-                                 const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
-                                 const GroupState* gs)
+// The rows of one tile (full storage); returns the lane's d.q partial.
+template <class M, int LPR, int U, class Dir>
+__device__ inline double full_rows(const RowTile& tl, const int* __restrict__ fstart, const int* __restrict__ fcol,
+                                   const M* __restrict__ vals, long long nnzf, const Dir& d, M* __restrict__ q)
 {
-    int t;
-    RowTile tl;
-    if (!tile_for(st, t, tl))
-        return;
-    const int stopAt = read_uniform(&gs[tl.group].stopAt);
-    if (stopAt >= 0 && stopAt <= k)
-        return;
-
     const int lane = threadIdx.x % LPR;
     const int sub = threadIdx.x / LPR;
     constexpr int RPP = NT / LPR;
@@ -288,10 +319,10 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
 #pragma unroll
         for (int u = 0; u < U; ++u)
         {
-            const int c = col[u];
-            const double d0 = static_cast<double>(d[3 * c]), d1 = static_cast<double>(d[3 * c + 1]),
-                         d2 = static_cast<double>(d[3 * c + 2]);
             const bool has = kb[u] < ke[u];
+            // a lane with no block reads row tl.rowBegin's column, which any window holds
+            const int c = has ? col[u] : tl.rowBegin;
+            const double d0 = d.at(c, 0), d1 = d.at(c, 1), d2 = d.at(c, 2);
 #pragma unroll
             for (int i = 0; i < 3; ++i)
                 y[u][i] = has ? brow(a[u], i, d0, d1, d2) : 0.0;
@@ -304,8 +335,7 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
                 M b[9];
                 int c;
                 load_block(vals, F, fcol, kk, b, c);
-                const double d0 = static_cast<double>(d[3 * c]), d1 = static_cast<double>(d[3 * c + 1]),
-                             d2 = static_cast<double>(d[3 * c + 2]);
+                const double d0 = d.at(c, 0), d1 = d.at(c, 1), d2 = d.at(c, 2);
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
                     y[u][i] += brow(b, i, d0, d1, d2);
@@ -331,6 +361,32 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
             }
         }
     }
+    return pd;
+}
+
+// q = A d_k for the tile's rows and the tile's d.q partial. A body whose loop
+// ended at or before k (decided by k_dir(k) or earlier) is skipped. The
+// direction blocks come from the tile's window of d staged in LDS when it
+// fits the launch's capacity (as in spmv_body_upper), else from global memory.
+template <class M, int LPR, int U>
+__device__ inline void spmv_body(int k, const TileView& st, const int* __restrict__ fstart,
+                                 const int* __restrict__ fcol, const M* __restrict__ vals, long long nnzf,
+                                 const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
+                                 const GroupState* gs, const int2* __restrict__ win, int winCap)
+{
+    int t;
+    RowTile tl;
+    if (!tile_for(st, t, tl))
+        return;
+    const int stopAt = read_uniform(&gs[tl.group].stopAt);
+    if (stopAt >= 0 && stopAt <= k)
+        return;
+    double pd;
+    const int2 w = winCap > 0 ? win[t] : make_int2(0, -1);
+    if (winCap > 0 && w.y - w.x + 1 <= winCap)  // uniform: one tile per workgroup
+        pd = full_rows<M, LPR, U>(tl, fstart, fcol, vals, nnzf, stage_window(d, w), q);
+    else
+        pd = full_rows<M, LPR, U>(tl, fstart, fcol, vals, nnzf, GlobalDir<M>{d}, q);
     double u1 = 0.0, u2 = 0.0;
     block_sum3(pd, u1, u2);
     if (threadIdx.x == 0)
@@ -389,20 +445,12 @@ __device__ inline void load_upper_block(const M* __restrict__ rec, const int2* _
 // or every position of a row loaded up front, keep them in L2 (1.0x) but ran
 // slower on MI355X: this kernel is bound by its chain of dependent loads and
 // by the address unit, not by HBM bytes, and those variants lose occupancy.
-template <class M, int LPR, int U>
-__device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __restrict__ rowptr,
-                                       const int2* __restrict__ lower, const M* __restrict__ rec,
-                                       const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
-                                       const GroupState* gs)
+// The rows of one tile (upper-only storage); returns the lane's d.q partial.
+template <class M, int LPR, int U, class Dir>
+__device__ inline double upper_rows(const RowTile& tl, const int2* __restrict__ rowptr,
+                                    const int2* __restrict__ lower, const M* __restrict__ rec, const Dir& d,
+                                    M* __restrict__ q)
 {
-    int t;
-    RowTile tl;
-    if (!tile_for(st, t, tl))
-        return;
-    const int stopAt = read_uniform(&gs[tl.group].stopAt);
-    if (stopAt >= 0 && stopAt <= k)
-        return;
-
     const int lane = threadIdx.x % LPR;
     const int sub = threadIdx.x / LPR;
     constexpr int RPP = NT / LPR;
@@ -436,10 +484,10 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
 #pragma unroll
         for (int u = 0; u < U; ++u)
         {
-            const int c = col[u];
-            const double d0 = static_cast<double>(d[3 * c]), d1 = static_cast<double>(d[3 * c + 1]),
-                         d2 = static_cast<double>(d[3 * c + 2]);
             const bool has = kb[u] < ke[u];
+            // a lane with no block reads row tl.rowBegin's column, which any window holds
+            const int c = has ? col[u] : tl.rowBegin;
+            const double d0 = d.at(c, 0), d1 = d.at(c, 1), d2 = d.at(c, 2);
 #pragma unroll
             for (int i = 0; i < 3; ++i)
                 y[u][i] = has ? brow(a[u], i, d0, d1, d2) : 0.0;
@@ -451,8 +499,7 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
                 M b[9];
                 int c;
                 load_upper_block(rec, lower, lb[u], ub[u], nl[u], kk, b, c);
-                const double d0 = static_cast<double>(d[3 * c]), d1 = static_cast<double>(d[3 * c + 1]),
-                             d2 = static_cast<double>(d[3 * c + 2]);
+                const double d0 = d.at(c, 0), d1 = d.at(c, 1), d2 = d.at(c, 2);
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
                     y[u][i] += brow(b, i, d0, d1, d2);
@@ -478,6 +525,35 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
             }
         }
     }
+    return pd;
+}
+
+// q = A d_k for one tile of upper-only storage and its d.q partial. When the
+// tile's column window win[t] (every column its rows reference, their own
+// rows included) fits the launch's staging capacity, the workgroup first
+// copies that window of d into LDS with coalesced loads and reads every
+// direction block from there: the scattered gathers leave the address unit
+// and the chain of dependent global loads. Otherwise it gathers from global
+// memory. Products and sums are the same either way.
+template <class M, int LPR, int U>
+__device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __restrict__ rowptr,
+                                       const int2* __restrict__ lower, const M* __restrict__ rec,
+                                       const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
+                                       const GroupState* gs, const int2* __restrict__ win, int winCap)
+{
+    int t;
+    RowTile tl;
+    if (!tile_for(st, t, tl))
+        return;
+    const int stopAt = read_uniform(&gs[tl.group].stopAt);
+    if (stopAt >= 0 && stopAt <= k)
+        return;
+    double pd;
+    const int2 w = winCap > 0 ? win[t] : make_int2(0, -1);
+    if (winCap > 0 && w.y - w.x + 1 <= winCap)  // uniform: one tile per workgroup
+        pd = upper_rows<M, LPR, U>(tl, rowptr, lower, rec, stage_window(d, w), q);
+    else
+        pd = upper_rows<M, LPR, U>(tl, rowptr, lower, rec, GlobalDir<M>{d}, q);
     double u1 = 0.0, u2 = 0.0;
     block_sum3(pd, u1, u2);
     if (threadIdx.x == 0)
@@ -489,13 +565,14 @@ __global__ __launch_bounds__(NT) void k_spmv_upper(const int* kb, int k0, TileVi
                                                    const int2* __restrict__ lower, const M* __restrict__ rec,
                                                    const M* __restrict__ d, M* __restrict__ q,
                                                    double* __restrict__ pa, const GroupState* gs,
+                                                   const int2* __restrict__ win, int winCap,
                                                    unsigned long long* stamps)
 {
     const LaunchIndex li = launch_index(kb, k0, stamps);
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    spmv_body_upper<M, LPR, U>(k, st, rowptr, lower, rec, d, q, pa, gs);
+    spmv_body_upper<M, LPR, U>(k, st, rowptr, lower, rec, d, q, pa, gs, win, winCap);
     stamp_end(stamp);
 }
 
@@ -504,13 +581,14 @@ __global__ __launch_bounds__(NT) void k_spmv(const int* kb, int k0, TileView st,
                                              const int* __restrict__ fcol, const M* __restrict__ vals,
                                              long long nnzf, const M* __restrict__ d, M* __restrict__ q,
                                              double* __restrict__ pa, const GroupState* gs,
+                                             const int2* __restrict__ win, int winCap,
                                              unsigned long long* stamps)
 {
     const LaunchIndex li = launch_index(kb, k0, stamps);
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    spmv_body<M, LPR, U>(k, st, fstart, fcol, vals, nnzf, d, q, pa, gs);
+    spmv_body<M, LPR, U>(k, st, fstart, fcol, vals, nnzf, d, q, pa, gs, win, winCap);
     stamp_end(stamp);
 }
 
@@ -684,9 +762,11 @@ __global__ __launch_bounds__(NT) void k_spmv_plain_upper(int nd, const int2* __r
 template <class M, int LPR, int U>
 static void launch_spmv_upper_shape(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp)
 {
-    hipLaunchKernelGGL((k_spmv_upper<M, LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), 0, v.stream,
-                       kb, k0, v.spmv, v.rowptr, v.lower, static_cast<const M*>(v.vals),
-                       static_cast<const M*>(v.d[k0 & 1]), static_cast<M*>(v.q), v.pa, v.gs, stamp);
+    const size_t lds = 3 * sizeof(M) * static_cast<size_t>(v.winCap);
+    hipLaunchKernelGGL((k_spmv_upper<M, LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds,
+                       v.stream, kb, k0, v.spmv, v.rowptr, v.lower, static_cast<const M*>(v.vals),
+                       static_cast<const M*>(v.d[k0 & 1]), static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap,
+                       stamp);
 }
 
 // The iteration of a launch is k0 (kb == nullptr) or *kb + k0 with *kb even,
@@ -694,9 +774,11 @@ static void launch_spmv_upper_shape(const DeviceView& v, const int* kb, int k0, 
 template <class M, int LPR, int U>
 static void launch_spmv_shape(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp)
 {
-    hipLaunchKernelGGL((k_spmv<M, LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), 0, v.stream, kb,
-                       k0, v.spmv, v.fstart, v.fcol, static_cast<const M*>(v.vals), v.nnzf,
-                       static_cast<const M*>(v.d[k0 & 1]), static_cast<M*>(v.q), v.pa, v.gs, stamp);
+    const size_t lds = 3 * sizeof(M) * static_cast<size_t>(v.winCap);
+    hipLaunchKernelGGL((k_spmv<M, LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds, v.stream,
+                       kb, k0, v.spmv, v.fstart, v.fcol, static_cast<const M*>(v.vals), v.nnzf,
+                       static_cast<const M*>(v.d[k0 & 1]), static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap,
+                       stamp);
 }
 
 template <class M>

@@ -319,6 +319,11 @@ struct DeviceView
     // connections, else null
     const double* slotAdd;
     const double* rowAdd;
+    // SpMV staging: per SpMV tile the rows [x, y] of d its blocks reference;
+    // a tile whose window holds at most winCap rows reads d from LDS (winCap
+    // 0: never)
+    const int2* spmvWin;
+    int winCap;
     // reductions
     TileView vec, spmv;
     double* pa;        // per SpMV tile: d.q

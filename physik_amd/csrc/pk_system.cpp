@@ -144,6 +144,11 @@ struct pk_system
     pk::DevBuf<unsigned long long> rawSpmv, rawUpd, rawDir, stampSpmv, stampUpd, stampDir;
     double wallClockKHz = 0.0;
     pk::Tiles vecTiles, spmvTiles;
+    // SpMV staging (upper-only storage): the column window of every SpMV tile
+    // and the rows of d a workgroup stages in LDS (0: no staging)
+    pk::DevBuf<int2> spmvWin;
+    int winCap = 0;
+    bool staging = true;
     int* pinned = nullptr;
     hipEvent_t pollEv[2] = {nullptr, nullptr};
 
@@ -322,6 +327,8 @@ struct pk_system
         dv.rowAdd = nsph > 0 ? srhs.p : (nconn > 0 ? crhs.p : nullptr);
         dv.vec = vecTiles.view();
         dv.spmv = spmvTiles.view();
+        dv.spmvWin = spmvWin.p;
+        dv.winCap = winCap;
         dv.pa = pa.p;
         dv.pb[0] = pb0.p;
         dv.pb[1] = pb1.p;
@@ -754,12 +761,50 @@ static int alloc_work(pk_system* s)
     return PK_OK;
 }
 
+static bool is_f32(const pk_system* s);
+
+// LDS an SpMV workgroup may take for its staged window of d: at the
+// kernels' 7 waves per SIMD, 7 workgroups share a CU's 160 KB.
+constexpr size_t kStageBytes = 20 * 1024;
+
+// Column windows of the SpMV tiles (same order as build_tiles) and the
+// staging capacity: the widest window, if it fits kStageBytes, else the
+// widest that does; tiles beyond it gather from global memory.
+static int setup_windows(pk_system* s)
+{
+    const Topology& t = s->topo;
+    s->winCap = 0;
+    if (!s->staging || t.nd == 0)
+        return PK_OK;
+    std::vector<int2> win;
+    int widest = 0;
+    for (int g = 0; g < t.G; ++g)
+        for (int r0 = t.groupRowBegin[g]; r0 < t.groupRowEnd[g]; r0 += s->spmvRows)
+        {
+            const int r1 = std::min(r0 + s->spmvRows, t.groupRowEnd[g]);
+            int lo = r0, hi = r1 - 1;  // rows ascending within a row: first and last column bound it
+            for (int r = r0; r < r1; ++r)
+                if (t.fstart[r + 1] > t.fstart[r])
+                {
+                    lo = std::min(lo, t.fcol[t.fstart[r]]);
+                    hi = std::max(hi, t.fcol[t.fstart[r + 1] - 1]);
+                }
+            win.push_back(make_int2(lo, hi));
+            widest = std::max(widest, hi - lo + 1);
+        }
+    const size_t bytesPerRow = 3 * (is_f32(s) ? sizeof(float) : sizeof(double));
+    const int cap = static_cast<int>(kStageBytes / bytesPerRow);
+    s->winCap = std::min(widest, cap);
+    return s->spmvWin.upload(win, s->stream);
+}
+
 static int setup_tiles(pk_system* s)
 {
     int rc;
     ++s->epoch;
     if ((rc = build_tiles(s, s->vecTiles, s->vecRows)) != PK_OK) return rc;
     if ((rc = build_tiles(s, s->spmvTiles, s->spmvRows)) != PK_OK) return rc;
+    if ((rc = setup_windows(s)) != PK_OK) return rc;
     if ((rc = s->pa.alloc(std::max(1, s->spmvTiles.count))) != PK_OK) return rc;
     if ((rc = s->pb0.alloc(std::max(1, s->vecTiles.count))) != PK_OK) return rc;
     if ((rc = s->pb1.alloc(std::max(1, s->vecTiles.count))) != PK_OK) return rc;
@@ -961,6 +1006,29 @@ int pk_system_tune(pk_system* s, int lanes_per_row, int spmv_rows, int vector_ro
     s->vecRows = vector_rows;
     s->unroll = unroll;
     return setup_tiles(s);
+}
+
+int pk_system_set_spmv_staging(pk_system* s, int enable)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    s->tapeValid = false;  // a recorded run no longer replays the same launches
+    s->staging = enable != 0;
+    ++s->epoch;
+    const int rc = setup_windows(s);
+    if (rc != PK_OK)
+        return rc;
+    PK_HIP(hipStreamSynchronize(s->stream));
+    return PK_OK;
+}
+
+int pk_system_get_spmv_staging(const pk_system* s, int* enabled, int* rows)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    if (enabled) *enabled = s->staging ? 1 : 0;
+    if (rows) *rows = s->winCap;
+    return PK_OK;
 }
 
 int pk_system_get_tune(const pk_system* s, int* lanes_per_row, int* spmv_rows, int* vector_rows, int* unroll)

@@ -35,6 +35,7 @@ def main():
     ap.add_argument("--shape", default=None,
                     help="CG launch shape lanes,spmv_rows,vector_rows,unroll (default: the system's own)")
     ap.add_argument("--spheres", type=int, default=0)
+    ap.add_argument("--no-staging", action="store_true", help="SpMV gathers d from global memory (upper storage)")
     args = ap.parse_args()
     lib = N.lib()
     ms = C.c_double()
@@ -45,6 +46,8 @@ def main():
                           "fp64u": N.PK_FP64_UPPER}[args.precision])
     if args.shape:
         s.tune(*[int(a) for a in args.shape.split(",")])
+    if args.no_staging:
+        s.set_staging(False)
     extra = {}
     if args.spheres > 0:
         lo, hi = w.x0.min(0), w.x0.max(0)
@@ -69,7 +72,7 @@ def main():
                       "nnzb_full": s.nnzb_full, "dynamic_rows": s.nd, "tets": s.t, "bodies": s.groups,
                       "spmv_launches": s.kernel_time(N.K_SPMV)[0], "spmv_blocks": blocks, "spmv_upper_blocks": ublocks, "spmv_rows": rows, "nnzb_upper": s.nnzb_upper,
                       "spmv_body_launches": body_launches, "shape": s.get_tune(),
-                      "spmv_us": 1e3 * ms_spmv / max(n_spmv, 1), "update_us": 1e3 * ms_upd / max(n_upd, 1),
+                      "staging": s.get_staging(), "spmv_us": 1e3 * ms_spmv / max(n_spmv, 1), "update_us": 1e3 * ms_upd / max(n_upd, 1),
                       "direction_us": 1e3 * ms_dir / max(n_dir, 1), **extra}), flush=True)
 
 

@@ -101,6 +101,16 @@ class System:
         from .autotune import autotune
         return autotune(self, dt, cache=cache, verbose=verbose)
 
+    def set_staging(self, enable=True):
+        """SpMV direction windows staged in LDS (upper storage; identical results)."""
+        N.check(self.lib.pk_system_set_spmv_staging(self.h, int(enable)), "pk_system_set_spmv_staging")
+
+    def get_staging(self):
+        """(enabled, rows a workgroup stages)."""
+        e, r = C.c_int(), C.c_int()
+        N.check(self.lib.pk_system_get_spmv_staging(self.h, C.byref(e), C.byref(r)), "pk_system_get_spmv_staging")
+        return bool(e.value), r.value
+
     def get_tune(self):
         v = [C.c_int() for _ in range(4)]
         N.check(self.lib.pk_system_get_tune(self.h, *[C.byref(x) for x in v]), "pk_system_get_tune")

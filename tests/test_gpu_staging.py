@@ -1,0 +1,112 @@
+"""SpMV direction windows staged in LDS (full and upper-triangle storage).
+
+A workgroup whose tile's column window fits the staging capacity copies that
+window of d into LDS and reads every direction block from there; a wider tile
+gathers from global memory. The products and sums are the same, so a system
+with staging on must step bit for bit like the same system with it off, at
+every launch shape, in a batch whose bodies put some tiles on each side of
+the capacity.
+"""
+
+import numpy as np
+import pytest
+
+from physik_amd import meshgen as mg
+from physik_amd import native as N
+from physik_amd import workloads as W
+from physik_amd.system import Pattern
+
+pytestmark = pytest.mark.gpu
+
+STAGE_BYTES = 20 * 1024  # the library's per-workgroup staging budget
+
+
+def mixed_batch(seed=21):
+    """Three bodies: a narrow band, a mid one, and a 30 x 30 cell slab whose
+    planes of 961 nodes give tile windows of ~2000 rows (beyond the FP32
+    capacity of 1706 rows and the FP64 one of 853)."""
+    rng = mg.Pcg32(seed)
+    rests, tetss, fixeds, v0s, groups, body_nodes, body_tets = [], [], [], [], [], [], []
+    n_off = t_off = 0
+    for b, (nx, ny, nz) in enumerate([(4, 3, 3), (8, 6, 5), (30, 30, 2)]):
+        h = 0.05
+        rest, tets = mg.box_mesh(nx, ny, nz, h, "kuhn")
+        rest = mg.jitter(rest, h, 0.1, rng)
+        fixed = np.zeros(len(rest), np.uint8)
+        fixed[mg.face_nodes(nx, ny, nz, 0)] = 1
+        v0 = 0.2 * rng.normal(rest.size).reshape(-1, 3)
+        v0[fixed == 1] = 0.0
+        rests.append(rest)
+        tetss.append(tets + n_off)
+        fixeds.append(fixed)
+        v0s.append(v0)
+        groups.append(np.full(len(rest), b, np.int32))
+        body_nodes.append((n_off, n_off + len(rest)))
+        body_tets.append((t_off, t_off + len(tets)))
+        n_off += len(rest)
+        t_off += len(tets)
+    rest = np.concatenate(rests)
+    tets = np.concatenate(tetss).astype(np.int32)
+    t = len(tets)
+    return W.Workload("mixed", rest, tets, np.full(t, 2e5), np.full(t, 0.35), np.zeros(t), np.full(t, 1000.0),
+                      np.concatenate(fixeds), rest.copy(), np.concatenate(v0s), np.concatenate(groups), 1.0 / 60.0,
+                      4, 1e-7, 600, W.G, body_nodes, body_tets)
+
+
+def windows(w, rows_per_tile):
+    """Column window widths of the SpMV tiles, as the library forms them."""
+    p = Pattern(w.rest, w.tets, node_fixed=w.fixed, node_group=w.node_group)
+    dyn_group = w.node_group[np.flatnonzero(p.dyn >= 0)]
+    widths = []
+    for g in range(w.n_bodies):
+        rows = np.flatnonzero(dyn_group == g)
+        for r0 in range(rows[0], rows[-1] + 1, rows_per_tile):
+            r1 = min(r0 + rows_per_tile, rows[-1] + 1)
+            lo = min(r0, p.full_col[p.full_start[r0:r1]].min())
+            hi = max(r1 - 1, p.full_col[p.full_start[r0 + 1:r1 + 1] - 1].max())
+            widths.append(hi - lo + 1)
+    return np.array(widths)
+
+
+@pytest.mark.parametrize("precision", [N.PK_FP32_UPPER, N.PK_FP64_UPPER, N.PK_FP32_STORAGE, N.PK_FP64])
+@pytest.mark.parametrize("shape", [(8, 128, 512, 2), (4, 64, 256, 1), (16, 32, 256, 4), (32, 256, 1024, 2)])
+def test_staged_windows_step_like_global_gathers(precision, shape):
+    w = mixed_batch()
+    a = W.make_system(w, precision)
+    b = W.make_system(w, precision)
+    a.tune(*shape)
+    b.tune(*shape)
+    b.set_staging(False)
+    enabled, rows = a.get_staging()
+    f32 = precision in (N.PK_FP32_UPPER, N.PK_FP32_STORAGE)
+    cap = STAGE_BYTES // (3 * (4 if f32 else 8))
+    widths = windows(w, shape[1])
+    print(f"precision {precision} shape {shape}: staged rows {rows}, windows {widths.min()}..{widths.max()}",
+          flush=True)
+    assert enabled and rows == min(widths.max(), cap)
+    assert (widths <= rows).any() and (widths > rows).any(), "the batch must put tiles on both sides"
+    assert b.get_staging() == (False, 0)
+    for n in range(w.steps):
+        a.step(w.dt)
+        b.step(w.dt)
+        xa, va = a.state()
+        xb, vb = b.state()
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb), f"step {n}"
+        for u, v in zip(a.results(), b.results()):
+            assert np.array_equal(u, v), f"step {n} results"
+    assert a.results()[0].min() > 5  # the solves did iterate
+    a.close()
+    b.close()
+
+
+def test_staging_is_switchable():
+    w = mixed_batch()
+    for precision in (N.PK_FP32_STORAGE, N.PK_FP32_UPPER):
+        s = W.make_system(w, precision)
+        enabled, rows = s.get_staging()
+        assert enabled and rows == 1706  # the slab's windows are wider than the capacity
+        s.set_staging(False)
+        assert s.get_staging() == (False, 0)
+        s.set_staging(True)
+        assert s.get_staging() == (True, rows)
+        s.close()
