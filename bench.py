@@ -98,7 +98,7 @@ def measured_traffic(workload_name, precision):
     try:
         with open(path) as f:
             entry = json.load(f).get(f"{workload_name}/{precision}")
-        return None if entry is None else (float(entry["bytes"]), float(entry["algorithmic"]))
+        return None if entry is None else (float(entry["bytes"]), float(entry["algorithmic"]), entry.get("shape"))
     except (OSError, ValueError, KeyError, TypeError):
         return None
 
@@ -127,6 +127,7 @@ def roofline(s, name, prec_name):
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": None if measured is None else measured[0],
         "traffic_vs_algorithmic": None if measured is None else measured[0] / measured[1],
+        "traffic_shape": None if measured is None else measured[2],  # CG launch shape of the counter passes
         "storage": "upper triangle" if upper else "full",
         "bytes_per_launch": total / max(n_spmv, 1),
         "full_storage_bytes_per_launch": full_format / max(n_spmv, 1),

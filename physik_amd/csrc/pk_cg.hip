@@ -603,6 +603,30 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
     RowTile tl;
     if (!tile_for(vt, t, tl))
         return;
+    // the thread's first kPre rows (d, q, x, r, M^-1) are in flight while the
+    // stop flag and the d.q partials are read
+    constexpr int kPre = 2;
+    const size_t N = static_cast<size_t>(nd);
+    const int row0 = tl.rowBegin + static_cast<int>(threadIdx.x);
+    M dpre[kPre][3], qpre[kPre][3], mpre[kPre][9];
+    double xpre[kPre][3], rpre[kPre][3];
+#pragma unroll
+    for (int j = 0; j < kPre; ++j)
+    {
+        const int row = row0 + j * NT;
+        const int rr = row < tl.rowEnd ? row : tl.rowBegin;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            dpre[j][i] = dcur[3 * rr + i];
+            qpre[j][i] = q[3 * rr + i];
+            xpre[j][i] = xs[3 * rr + i];
+            rpre[j][i] = rv[3 * rr + i];
+        }
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            mpre[j][e] = minv[e * N + rr];
+    }
     const int g = tl.group;
     GroupState& s = gs[g];
     const int stopAt = read_uniform(&s.stopAt);
@@ -621,24 +645,22 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
             record_stop(s, active, k, 1, k, delta <= s.target, norm_of(rr), false);
         return;
     }
-    const size_t N = static_cast<size_t>(nd);
     double pz = 0.0, pr = 0.0;
     int flag = 0;
-    for (int row = tl.rowBegin + threadIdx.x; row < tl.rowEnd; row += NT)
-    {
+    // one row: x += alpha d, r -= alpha q, z = M^-1 r, and the partials
+    auto one = [&](int row, const M* dd, const M* qq, const double* xx, const double* rq, const M* mm) {
         double r[3], z[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
         {
-            xs[3 * row + i] = xs[3 * row + i] + static_cast<double>(dcur[3 * row + i]) * alpha;
-            r[i] = rv[3 * row + i] + static_cast<double>(q[3 * row + i]) * (-alpha);
+            xs[3 * row + i] = xx[i] + static_cast<double>(dd[i]) * alpha;
+            r[i] = rq[i] + static_cast<double>(qq[i]) * (-alpha);
             rv[3 * row + i] = r[i];
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            z[i] = static_cast<double>(minv[(3 * i) * N + row]) * r[0] +
-                   static_cast<double>(minv[(3 * i + 1) * N + row]) * r[1] +
-                   static_cast<double>(minv[(3 * i + 2) * N + row]) * r[2];
+            z[i] = static_cast<double>(mm[3 * i]) * r[0] + static_cast<double>(mm[3 * i + 1]) * r[1] +
+                   static_cast<double>(mm[3 * i + 2]) * r[2];
         if (!(fin(z[0]) && fin(z[1]) && fin(z[2])))
             flag = 1;
         M zs[3];
@@ -650,6 +672,27 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
         }
         pz += r[0] * static_cast<double>(zs[0]) + r[1] * static_cast<double>(zs[1]) + r[2] * static_cast<double>(zs[2]);
         pr += r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    };
+#pragma unroll
+    for (int j = 0; j < kPre; ++j)
+        if (row0 + j * NT < tl.rowEnd)
+            one(row0 + j * NT, dpre[j], qpre[j], xpre[j], rpre[j], mpre[j]);
+    for (int row = row0 + kPre * NT; row < tl.rowEnd; row += NT)
+    {
+        M dd[3], qq[3], mm[9];
+        double xx[3], rq[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            dd[i] = dcur[3 * row + i];
+            qq[i] = q[3 * row + i];
+            xx[i] = xs[3 * row + i];
+            rq[i] = rv[3 * row + i];
+        }
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            mm[e] = minv[e * N + row];
+        one(row, dd, qq, xx, rq, mm);
     }
     double u = 0.0;
     block_sum3(pz, pr, u);
