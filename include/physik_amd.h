@@ -132,13 +132,16 @@ int pk_system_tune(pk_system* sys, int lanes_per_row, int spmv_rows, int vector_
 int pk_system_get_tune(const pk_system* sys, int* lanes_per_row, int* spmv_rows, int* vector_rows, int* unroll);
 
 /* SpMV staging (no reference counterpart): enable != 0 (the default) lets
- * each SpMV workgroup copy the window of the direction vector its rows
- * reference into LDS (up to 20 KB) and read the direction blocks from there;
- * tiles whose window is wider gather from global memory. Results are
- * identical either way. get reports the setting and the rows a workgroup
- * stages (0: none). */
+ * each SpMV workgroup copy the rows of the direction vector its tile
+ * references into LDS (up to 20 KB) and read the direction blocks from there.
+ * When every tile's column window [first, last] fits, tiles stage their
+ * windows; otherwise they stage up to four segments of rows (the columns cut
+ * at their three widest gaps), and tiles that still need more than 20 KB
+ * gather from global memory. Results are identical either way. get reports
+ * the setting, the rows a workgroup stages (0: none) and whether segments are
+ * in use. */
 int pk_system_set_spmv_staging(pk_system* sys, int enable);
-int pk_system_get_spmv_staging(const pk_system* sys, int* enabled, int* rows);
+int pk_system_get_spmv_staging(const pk_system* sys, int* enabled, int* rows, int* segmented);
 
 /* Node state, node_count x 3 each; either pointer may be NULL. */
 int pk_system_set_state(pk_system* sys, const double* x, const double* v);

@@ -26,6 +26,7 @@
 // before any is reduced, to keep more bytes in flight per wave.
 #include "pk_device.h"
 
+#include <climits>
 #include <type_traits>
 
 namespace pk
@@ -214,8 +215,8 @@ __device__ inline double brow(const M* a, int i, double d0, double d1, double d2
                          __builtin_fma(static_cast<double>(a[3 * i + 1]), d1, static_cast<double>(a[3 * i]) * d0));
 }
 
-// Where an SpMV reads the direction vector d: global memory, or the window
-// of rows [base, base + len) of it that the workgroup staged in LDS.
+// Where an SpMV reads the direction vector d: global memory, or the
+// segments of it that the workgroup staged in LDS.
 template <class M>
 struct GlobalDir
 {
@@ -223,12 +224,24 @@ struct GlobalDir
     __device__ double at(int c, int j) const { return static_cast<double>(d[3 * c + j]); }
 };
 
-template <class M>
+// LDS row of column c: c + sh0, plus (SEG: segments) the deltas of every
+// later segment whose start c has reached. (Written as masked deltas, not a
+// chain of selects, which the compiler turns into a lookup in scratch
+// memory.) Without SEG the tile's staging is one contiguous window.
+template <class M, bool SEG>
 struct StagedDir
 {
     const M* lds;
-    int base;
-    __device__ double at(int c, int j) const { return static_cast<double>(lds[3 * (c - base) + j]); }
+    int lo1, lo2, lo3;  // starts of segments 1..3 (unused: INT_MAX)
+    int sh0;            // -lo0: LDS row of column c in segment 0 is c + sh0
+    int dl1, dl2, dl3;  // shift of segment s minus that of segment s-1 (unused: 0)
+    __device__ int at_row(int c) const
+    {
+        if (!SEG)
+            return c + sh0;
+        return c + sh0 + (c >= lo1 ? dl1 : 0) + (c >= lo2 ? dl2 : 0) + (c >= lo3 ? dl3 : 0);
+    }
+    __device__ double at(int c, int j) const { return static_cast<double>(lds[3 * at_row(c) + j]); }
 };
 
 // d_row . q_row for a tile's partial, pinned the same way.
@@ -239,30 +252,42 @@ __device__ inline double dot_dq(const Dir& d, int row, M q0, M q1, M q2)
                          __builtin_fma(d.at(row, 1), static_cast<double>(q1), d.at(row, 0) * static_cast<double>(q0)));
 }
 
-// Dynamic LDS of the SpMV launches: the staged window of d.
+// Dynamic LDS of the SpMV launches: the staged segments of d.
 extern __shared__ __align__(16) unsigned char pk_stage_lds[];
 
-// Copies rows [w.x, w.y] of d into LDS with coalesced loads (four in flight
-// per thread) and waits for the workgroup; returns the staged view.
-template <class M>
-__device__ inline StagedDir<M> stage_window(const M* __restrict__ d, int2 w)
+// Copies the tile's segments of d into LDS with coalesced loads and waits
+// for the workgroup; returns the staged view. LDS element i belongs to
+// segment s with 3 end[s-1] <= i < 3 end[s] and holds d element i - 3 shift[s].
+template <class M, bool SEG>
+__device__ inline StagedDir<M, SEG> stage_segments(const M* __restrict__ d, const StageSegs& seg)
 {
     M* lds = reinterpret_cast<M*>(pk_stage_lds);
-    const M* src = d + 3 * static_cast<size_t>(w.x);
-    const int n = 3 * (w.y - w.x + 1);
+    const int4 lo = seg.lo, end = seg.end;
+    // shifts: segment s's column lo_s sits at LDS row end_{s-1}
+    const int sh0 = -lo.x;
+    const int sh1 = lo.y == INT_MAX ? sh0 : end.x - lo.y;
+    const int sh2 = lo.z == INT_MAX ? sh1 : end.y - lo.z;
+    const int sh3 = lo.w == INT_MAX ? sh2 : end.z - lo.w;
+    const StagedDir<M, SEG> v{lds, lo.y, lo.z, lo.w, sh0, sh1 - sh0, sh2 - sh1, sh3 - sh2};
+    const int n = 3 * end.w, e0 = 3 * end.x, e1 = 3 * end.y, e2 = 3 * end.z;
     for (int i0 = threadIdx.x; i0 < n; i0 += 4 * NT)
     {
-        M v[4];
+        M x[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            v[j] = i0 + j * NT < n ? src[i0 + j * NT] : M(0);
+        {
+            const int i = i0 + j * NT;
+            const int sh = SEG ? v.sh0 + (i >= e0 ? v.dl1 : 0) + (i >= e1 ? v.dl2 : 0) + (i >= e2 ? v.dl3 : 0)
+                               : v.sh0;
+            x[j] = i < n ? d[i - 3 * sh] : M(0);
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             if (i0 + j * NT < n)
-                lds[i0 + j * NT] = v[j];
+                lds[i0 + j * NT] = x[j];
     }
     __syncthreads();
-    return StagedDir<M>{lds, w.x};
+    return v;
 }
 
 // Block k of the full BSR: its 9 values (row-major) and its column.
@@ -287,8 +312,10 @@ __device__ inline void load_block(const float* __restrict__ vals, size_t, const 
 
 // The rows of one tile (full storage); returns the lane's d.q partial.
 template <class M, int LPR, int U, class Dir>
-__device__ inline double full_rows(const RowTile& tl, const int* __restrict__ fstart, const int* __restrict__ fcol,
-                                   const M* __restrict__ vals, long long nnzf, const Dir& d, M* __restrict__ q)
+__device__ __attribute__((always_inline)) inline double full_rows(const RowTile& tl, const int* __restrict__ fstart,
+                                                                  const int* __restrict__ fcol,
+                                                                  const M* __restrict__ vals, long long nnzf, Dir d,
+                                                                  M* __restrict__ q)
 {
     const int lane = threadIdx.x % LPR;
     const int sub = threadIdx.x / LPR;
@@ -320,7 +347,7 @@ __device__ inline double full_rows(const RowTile& tl, const int* __restrict__ fs
         for (int u = 0; u < U; ++u)
         {
             const bool has = kb[u] < ke[u];
-            // a lane with no block reads row tl.rowBegin's column, which any window holds
+            // a lane with no block reads row tl.rowBegin's column, which every staging holds
             const int c = has ? col[u] : tl.rowBegin;
             const double d0 = d.at(c, 0), d1 = d.at(c, 1), d2 = d.at(c, 2);
 #pragma unroll
@@ -366,25 +393,28 @@ __device__ inline double full_rows(const RowTile& tl, const int* __restrict__ fs
 
 // q = A d_k for the tile's rows and the tile's d.q partial. A body whose loop
 // ended at or before k (decided by k_dir(k) or earlier) is skipped. The
-// direction blocks come from the tile's window of d staged in LDS when it
-// fits the launch's capacity (as in spmv_body_upper), else from global memory.
-template <class M, int LPR, int U>
+// direction blocks come from the tile's segments of d staged in LDS when
+// they fit the launch's capacity (as in spmv_body_upper), else from global
+// memory.
+template <class M, int LPR, int U, bool SEG>
 __device__ inline void spmv_body(int k, const TileView& st, const int* __restrict__ fstart,
                                  const int* __restrict__ fcol, const M* __restrict__ vals, long long nnzf,
                                  const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
-                                 const GroupState* gs, const int2* __restrict__ win, int winCap)
+                                 const GroupState* gs, const StageSegs* __restrict__ win, int winCap)
 {
     int t;
     RowTile tl;
     if (!tile_for(st, t, tl))
         return;
+    StageSegs sg{make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
+    if (winCap > 0)
+        sg = win[t];  // in flight with the stop flag
     const int stopAt = read_uniform(&gs[tl.group].stopAt);
     if (stopAt >= 0 && stopAt <= k)
         return;
     double pd;
-    const int2 w = winCap > 0 ? win[t] : make_int2(0, -1);
-    if (winCap > 0 && w.y - w.x + 1 <= winCap)  // uniform: one tile per workgroup
-        pd = full_rows<M, LPR, U>(tl, fstart, fcol, vals, nnzf, stage_window(d, w), q);
+    if (winCap > 0 && sg.end.w <= winCap)  // uniform: one tile per workgroup
+        pd = full_rows<M, LPR, U>(tl, fstart, fcol, vals, nnzf, stage_segments<M, SEG>(d, sg), q);
     else
         pd = full_rows<M, LPR, U>(tl, fstart, fcol, vals, nnzf, GlobalDir<M>{d}, q);
     double u1 = 0.0, u2 = 0.0;
@@ -447,9 +477,10 @@ __device__ inline void load_upper_block(const M* __restrict__ rec, const int2* _
 // by the address unit, not by HBM bytes, and those variants lose occupancy.
 // The rows of one tile (upper-only storage); returns the lane's d.q partial.
 template <class M, int LPR, int U, class Dir>
-__device__ inline double upper_rows(const RowTile& tl, const int2* __restrict__ rowptr,
-                                    const int2* __restrict__ lower, const M* __restrict__ rec, const Dir& d,
-                                    M* __restrict__ q)
+__device__ __attribute__((always_inline)) inline double upper_rows(const RowTile& tl, const int2* __restrict__ rowptr,
+                                                                   const int2* __restrict__ lower,
+                                                                   const M* __restrict__ rec, Dir d,
+                                                                   M* __restrict__ q)
 {
     const int lane = threadIdx.x % LPR;
     const int sub = threadIdx.x / LPR;
@@ -485,7 +516,7 @@ __device__ inline double upper_rows(const RowTile& tl, const int2* __restrict__ 
         for (int u = 0; u < U; ++u)
         {
             const bool has = kb[u] < ke[u];
-            // a lane with no block reads row tl.rowBegin's column, which any window holds
+            // a lane with no block reads row tl.rowBegin's column, which every staging holds
             const int c = has ? col[u] : tl.rowBegin;
             const double d0 = d.at(c, 0), d1 = d.at(c, 1), d2 = d.at(c, 2);
 #pragma unroll
@@ -529,29 +560,32 @@ __device__ inline double upper_rows(const RowTile& tl, const int2* __restrict__ 
 }
 
 // q = A d_k for one tile of upper-only storage and its d.q partial. When the
-// tile's column window win[t] (every column its rows reference, their own
-// rows included) fits the launch's staging capacity, the workgroup first
-// copies that window of d into LDS with coalesced loads and reads every
-// direction block from there: the scattered gathers leave the address unit
-// and the chain of dependent global loads. Otherwise it gathers from global
-// memory. Products and sums are the same either way.
-template <class M, int LPR, int U>
+// tile's segments win[t] (up to four runs of rows covering every column its
+// rows reference, their own rows included: on a grid, the layers below, at
+// and above the tile) fit the launch's staging capacity, the workgroup first
+// copies them from d into LDS with coalesced loads and reads every direction
+// block from there: the scattered gathers leave the address unit and the
+// chain of dependent global loads. Otherwise it gathers from global memory.
+// Products and sums are the same either way.
+template <class M, int LPR, int U, bool SEG>
 __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __restrict__ rowptr,
                                        const int2* __restrict__ lower, const M* __restrict__ rec,
                                        const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
-                                       const GroupState* gs, const int2* __restrict__ win, int winCap)
+                                       const GroupState* gs, const StageSegs* __restrict__ win, int winCap)
 {
     int t;
     RowTile tl;
     if (!tile_for(st, t, tl))
         return;
+    StageSegs sg{make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
+    if (winCap > 0)
+        sg = win[t];  // in flight with the stop flag
     const int stopAt = read_uniform(&gs[tl.group].stopAt);
     if (stopAt >= 0 && stopAt <= k)
         return;
     double pd;
-    const int2 w = winCap > 0 ? win[t] : make_int2(0, -1);
-    if (winCap > 0 && w.y - w.x + 1 <= winCap)  // uniform: one tile per workgroup
-        pd = upper_rows<M, LPR, U>(tl, rowptr, lower, rec, stage_window(d, w), q);
+    if (winCap > 0 && sg.end.w <= winCap)  // uniform: one tile per workgroup
+        pd = upper_rows<M, LPR, U>(tl, rowptr, lower, rec, stage_segments<M, SEG>(d, sg), q);
     else
         pd = upper_rows<M, LPR, U>(tl, rowptr, lower, rec, GlobalDir<M>{d}, q);
     double u1 = 0.0, u2 = 0.0;
@@ -560,35 +594,35 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
         pa[t] = pd;
 }
 
-template <class M, int LPR, int U>
+template <class M, int LPR, int U, bool SEG>
 __global__ __launch_bounds__(NT) void k_spmv_upper(const int* kb, int k0, TileView st, const int2* __restrict__ rowptr,
                                                    const int2* __restrict__ lower, const M* __restrict__ rec,
                                                    const M* __restrict__ d, M* __restrict__ q,
                                                    double* __restrict__ pa, const GroupState* gs,
-                                                   const int2* __restrict__ win, int winCap,
+                                                   const StageSegs* __restrict__ win, int winCap,
                                                    unsigned long long* stamps)
 {
     const LaunchIndex li = launch_index(kb, k0, stamps);
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    spmv_body_upper<M, LPR, U>(k, st, rowptr, lower, rec, d, q, pa, gs, win, winCap);
+    spmv_body_upper<M, LPR, U, SEG>(k, st, rowptr, lower, rec, d, q, pa, gs, win, winCap);
     stamp_end(stamp);
 }
 
-template <class M, int LPR, int U>
+template <class M, int LPR, int U, bool SEG>
 __global__ __launch_bounds__(NT) void k_spmv(const int* kb, int k0, TileView st, const int* __restrict__ fstart,
                                              const int* __restrict__ fcol, const M* __restrict__ vals,
                                              long long nnzf, const M* __restrict__ d, M* __restrict__ q,
                                              double* __restrict__ pa, const GroupState* gs,
-                                             const int2* __restrict__ win, int winCap,
+                                             const StageSegs* __restrict__ win, int winCap,
                                              unsigned long long* stamps)
 {
     const LaunchIndex li = launch_index(kb, k0, stamps);
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    spmv_body<M, LPR, U>(k, st, fstart, fcol, vals, nnzf, d, q, pa, gs, win, winCap);
+    spmv_body<M, LPR, U, SEG>(k, st, fstart, fcol, vals, nnzf, d, q, pa, gs, win, winCap);
     stamp_end(stamp);
 }
 
@@ -806,10 +840,10 @@ template <class M, int LPR, int U>
 static void launch_spmv_upper_shape(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp)
 {
     const size_t lds = 3 * sizeof(M) * static_cast<size_t>(v.winCap);
-    hipLaunchKernelGGL((k_spmv_upper<M, LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds,
-                       v.stream, kb, k0, v.spmv, v.rowptr, v.lower, static_cast<const M*>(v.vals),
-                       static_cast<const M*>(v.d[k0 & 1]), static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap,
-                       stamp);
+    auto kernel = v.stageSegs ? k_spmv_upper<M, LPR, U, true> : k_spmv_upper<M, LPR, U, false>;
+    hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds, v.stream, kb, k0,
+                       v.spmv, v.rowptr, v.lower, static_cast<const M*>(v.vals), static_cast<const M*>(v.d[k0 & 1]),
+                       static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap, stamp);
 }
 
 // The iteration of a launch is k0 (kb == nullptr) or *kb + k0 with *kb even,
@@ -818,8 +852,9 @@ template <class M, int LPR, int U>
 static void launch_spmv_shape(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp)
 {
     const size_t lds = 3 * sizeof(M) * static_cast<size_t>(v.winCap);
-    hipLaunchKernelGGL((k_spmv<M, LPR, U>), dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds, v.stream,
-                       kb, k0, v.spmv, v.fstart, v.fcol, static_cast<const M*>(v.vals), v.nnzf,
+    auto kernel = v.stageSegs ? k_spmv<M, LPR, U, true> : k_spmv<M, LPR, U, false>;
+    hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds, v.stream, kb, k0,
+                       v.spmv, v.fstart, v.fcol, static_cast<const M*>(v.vals), v.nnzf,
                        static_cast<const M*>(v.d[k0 & 1]), static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap,
                        stamp);
 }

@@ -45,6 +45,16 @@ struct Partial
     int flag, pad;
 };
 
+// The rows of d an SpMV tile references, as up to four ascending,
+// disjoint segments of rows [lo[s], lo[s] + len[s]) packed one after another
+// in LDS: segment s starts at LDS row end[s-1] (0 for s = 0), and end[3] is
+// the rows staged in all. An unused segment has lo = INT_MAX and length 0.
+struct StageSegs
+{
+    int4 lo;
+    int4 end;
+};
+
 // A tiled launch: tiles of consecutive rows, grouped by body. Workgroup b runs
 // tile (b % 8) * per + b / 8 so that each of the 8 XCDs (workgroups are
 // dispatched round-robin over them) walks one contiguous eighth of the rows
@@ -319,11 +329,12 @@ struct DeviceView
     // connections, else null
     const double* slotAdd;
     const double* rowAdd;
-    // SpMV staging: per SpMV tile the rows [x, y] of d its blocks reference;
-    // a tile whose window holds at most winCap rows reads d from LDS (winCap
-    // 0: never)
-    const int2* spmvWin;
+    // SpMV staging: per SpMV tile the segments of d its blocks reference; a
+    // tile whose segments hold at most winCap rows in all reads d from LDS
+    // (winCap 0: never)
+    const StageSegs* spmvWin;
     int winCap;
+    int stageSegs;  // 0: every tile's staging is one window (segment 0 only)
     // reductions
     TileView vec, spmv;
     double* pa;        // per SpMV tile: d.q

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -146,8 +147,9 @@ struct pk_system
     pk::Tiles vecTiles, spmvTiles;
     // SpMV staging (upper-only storage): the column window of every SpMV tile
     // and the rows of d a workgroup stages in LDS (0: no staging)
-    pk::DevBuf<int2> spmvWin;
+    pk::DevBuf<pk::StageSegs> spmvWin;
     int winCap = 0;
+    int stageSegs = 0;
     bool staging = true;
     int* pinned = nullptr;
     hipEvent_t pollEv[2] = {nullptr, nullptr};
@@ -329,6 +331,7 @@ struct pk_system
         dv.spmv = spmvTiles.view();
         dv.spmvWin = spmvWin.p;
         dv.winCap = winCap;
+        dv.stageSegs = stageSegs;
         dv.pa = pa.p;
         dv.pb[0] = pb0.p;
         dv.pb[1] = pb1.p;
@@ -767,35 +770,77 @@ static bool is_f32(const pk_system* s);
 // kernels' 7 waves per SIMD, 7 workgroups share a CU's 160 KB.
 constexpr size_t kStageBytes = 20 * 1024;
 
-// Column windows of the SpMV tiles (same order as build_tiles) and the
-// staging capacity: the widest window, if it fits kStageBytes, else the
-// widest that does; tiles beyond it gather from global memory.
+// Staging of the SpMV tiles (same order as build_tiles). Each tile's
+// columns span a window [first, last]; when every window fits kStageBytes the
+// tiles stage their windows (one segment, the cheapest lookup). Otherwise
+// each tile stages its distinct columns cut at the three widest gaps between
+// them into up to four segments (on a grid: the layers below, at and above
+// the tile), and tiles whose segments still exceed the capacity gather from
+// global memory. winCap is the widest staging kept, at most the capacity.
 static int setup_windows(pk_system* s)
 {
     const Topology& t = s->topo;
     s->winCap = 0;
+    s->stageSegs = 0;
     if (!s->staging || t.nd == 0)
         return PK_OK;
-    std::vector<int2> win;
-    int widest = 0;
+    const size_t bytesPerRow = 3 * (is_f32(s) ? sizeof(float) : sizeof(double));
+    const int cap = static_cast<int>(kStageBytes / bytesPerRow);
+    std::vector<pk::StageSegs> wins, segs;
+    std::vector<int> cols;
+    std::vector<std::pair<int, int>> gaps;  // (gap rows, index of the column before it)
+    int widestWin = 0, widestSeg = 0;
     for (int g = 0; g < t.G; ++g)
         for (int r0 = t.groupRowBegin[g]; r0 < t.groupRowEnd[g]; r0 += s->spmvRows)
         {
             const int r1 = std::min(r0 + s->spmvRows, t.groupRowEnd[g]);
-            int lo = r0, hi = r1 - 1;  // rows ascending within a row: first and last column bound it
+            cols.assign(t.fcol.begin() + t.fstart[r0], t.fcol.begin() + t.fstart[r1]);
             for (int r = r0; r < r1; ++r)
-                if (t.fstart[r + 1] > t.fstart[r])
+                cols.push_back(r);
+            std::sort(cols.begin(), cols.end());
+            cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+            const int len = cols.back() - cols.front() + 1;
+            wins.push_back(pk::StageSegs{make_int4(cols.front(), INT_MAX, INT_MAX, INT_MAX),
+                                         make_int4(len, len, len, len)});
+            widestWin = std::max(widestWin, len);
+            gaps.clear();
+            for (size_t i = 0; i + 1 < cols.size(); ++i)
+                if (cols[i + 1] - cols[i] > 1)
+                    gaps.push_back({cols[i + 1] - cols[i] - 1, static_cast<int>(i)});
+            const size_t cuts = std::min<size_t>(3, gaps.size());
+            std::partial_sort(gaps.begin(), gaps.begin() + cuts, gaps.end(),
+                              [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+                                  return a.first != b.first ? a.first > b.first : a.second < b.second;
+                              });
+            std::vector<int> at;  // index of the last column of each segment
+            for (size_t c = 0; c < cuts; ++c)
+                at.push_back(gaps[c].second);
+            std::sort(at.begin(), at.end());
+            at.push_back(static_cast<int>(cols.size()) - 1);
+            int lo[4] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX}, end[4] = {0, 0, 0, 0};
+            int first = 0, total = 0;
+            for (size_t k = 0; k < 4; ++k)
+            {
+                if (k < at.size())
                 {
-                    lo = std::min(lo, t.fcol[t.fstart[r]]);
-                    hi = std::max(hi, t.fcol[t.fstart[r + 1] - 1]);
+                    lo[k] = cols[first];
+                    total += cols[at[k]] - cols[first] + 1;
+                    first = at[k] + 1;
                 }
-            win.push_back(make_int2(lo, hi));
-            widest = std::max(widest, hi - lo + 1);
+                end[k] = total;
+            }
+            segs.push_back(pk::StageSegs{make_int4(lo[0], lo[1], lo[2], lo[3]),
+                                         make_int4(end[0], end[1], end[2], end[3])});
+            widestSeg = std::max(widestSeg, total);
         }
-    const size_t bytesPerRow = 3 * (is_f32(s) ? sizeof(float) : sizeof(double));
-    const int cap = static_cast<int>(kStageBytes / bytesPerRow);
-    s->winCap = std::min(widest, cap);
-    return s->spmvWin.upload(win, s->stream);
+    if (widestWin <= cap)
+    {
+        s->winCap = widestWin;
+        return s->spmvWin.upload(wins, s->stream);
+    }
+    s->stageSegs = 1;
+    s->winCap = std::min(widestSeg, cap);
+    return s->spmvWin.upload(segs, s->stream);
 }
 
 static int setup_tiles(pk_system* s)
@@ -1022,12 +1067,13 @@ int pk_system_set_spmv_staging(pk_system* s, int enable)
     return PK_OK;
 }
 
-int pk_system_get_spmv_staging(const pk_system* s, int* enabled, int* rows)
+int pk_system_get_spmv_staging(const pk_system* s, int* enabled, int* rows, int* segmented)
 {
     if (!s)
         return PK_ERR_ARG;
     if (enabled) *enabled = s->staging ? 1 : 0;
     if (rows) *rows = s->winCap;
+    if (segmented) *segmented = s->stageSegs;
     return PK_OK;
 }
 

@@ -106,10 +106,11 @@ class System:
         N.check(self.lib.pk_system_set_spmv_staging(self.h, int(enable)), "pk_system_set_spmv_staging")
 
     def get_staging(self):
-        """(enabled, rows a workgroup stages)."""
-        e, r = C.c_int(), C.c_int()
-        N.check(self.lib.pk_system_get_spmv_staging(self.h, C.byref(e), C.byref(r)), "pk_system_get_spmv_staging")
-        return bool(e.value), r.value
+        """(enabled, rows a workgroup stages, segments in use)."""
+        e, r, g = C.c_int(), C.c_int(), C.c_int()
+        N.check(self.lib.pk_system_get_spmv_staging(self.h, C.byref(e), C.byref(r), C.byref(g)),
+                "pk_system_get_spmv_staging")
+        return bool(e.value), r.value, bool(g.value)
 
     def get_tune(self):
         v = [C.c_int() for _ in range(4)]

@@ -1,11 +1,12 @@
-"""SpMV direction windows staged in LDS (full and upper-triangle storage).
+"""SpMV direction segments staged in LDS (full and upper-triangle storage).
 
-A workgroup whose tile's column window fits the staging capacity copies that
-window of d into LDS and reads every direction block from there; a wider tile
-gathers from global memory. The products and sums are the same, so a system
-with staging on must step bit for bit like the same system with it off, at
-every launch shape, in a batch whose bodies put some tiles on each side of
-the capacity.
+When every SpMV tile's column window fits the capacity, tiles copy their
+windows of d into LDS; otherwise they copy up to four segments of d, and a
+tile needing more than the capacity gathers from global memory. The products
+and sums are the same, so a system with staging on must step bit for bit like
+the same system with it off, at every launch shape, both in a batch whose
+windows all fit and in one whose segments put tiles on each side of the
+capacity.
 """
 
 import numpy as np
@@ -21,14 +22,15 @@ pytestmark = pytest.mark.gpu
 STAGE_BYTES = 20 * 1024  # the library's per-workgroup staging budget
 
 
-def mixed_batch(seed=21):
-    """Three bodies: a narrow band, a mid one, and a 30 x 30 cell slab whose
-    planes of 961 nodes give tile windows of ~2000 rows (beyond the FP32
-    capacity of 1706 rows and the FP64 one of 853)."""
+def mixed_batch(seed=21, bar=True):
+    """Two small boxes, whose tiles' windows fit, and (bar) a 700 x 2 x 2 cell
+    bar, whose tiles reference columns in clusters about 700 rows apart, so
+    that even their four segments need 2,200 rows or more (beyond the FP32
+    capacity of 1,706 rows and the FP64 one of 853)."""
     rng = mg.Pcg32(seed)
     rests, tetss, fixeds, v0s, groups, body_nodes, body_tets = [], [], [], [], [], [], []
     n_off = t_off = 0
-    for b, (nx, ny, nz) in enumerate([(4, 3, 3), (8, 6, 5), (30, 30, 2)]):
+    for b, (nx, ny, nz) in enumerate([(4, 3, 3), (8, 6, 5)] + ([(700, 2, 2)] if bar else [])):
         h = 0.05
         rest, tets = mg.box_mesh(nx, ny, nz, h, "kuhn")
         rest = mg.jitter(rest, h, 0.1, rng)
@@ -53,39 +55,63 @@ def mixed_batch(seed=21):
                       4, 1e-7, 600, W.G, body_nodes, body_tets)
 
 
+def staged_rows(cols):
+    """Rows the library stages for a tile referencing `cols`: the distinct
+    columns cut at their three widest gaps (ties: the earlier gap) into up to
+    four runs, each staged from its first to its last column."""
+    cols = np.unique(cols)
+    gaps = [(cols[i + 1] - cols[i] - 1, i) for i in range(len(cols) - 1) if cols[i + 1] - cols[i] > 1]
+    gaps.sort(key=lambda g: (-g[0], g[1]))
+    ends = sorted(g[1] for g in gaps[:3]) + [len(cols) - 1]
+    total, first = 0, 0
+    for e in ends:
+        total += cols[e] - cols[first] + 1
+        first = e + 1
+    return total
+
+
 def windows(w, rows_per_tile):
-    """Column window widths of the SpMV tiles, as the library forms them."""
+    """Window widths and segment totals of every SpMV tile, as the library
+    forms them."""
     p = Pattern(w.rest, w.tets, node_fixed=w.fixed, node_group=w.node_group)
     dyn_group = w.node_group[np.flatnonzero(p.dyn >= 0)]
-    widths = []
+    wins, segs = [], []
     for g in range(w.n_bodies):
         rows = np.flatnonzero(dyn_group == g)
         for r0 in range(rows[0], rows[-1] + 1, rows_per_tile):
             r1 = min(r0 + rows_per_tile, rows[-1] + 1)
-            lo = min(r0, p.full_col[p.full_start[r0:r1]].min())
-            hi = max(r1 - 1, p.full_col[p.full_start[r0 + 1:r1 + 1] - 1].max())
-            widths.append(hi - lo + 1)
-    return np.array(widths)
+            cols = np.concatenate([p.full_col[p.full_start[r0]:p.full_start[r1]], np.arange(r0, r1)])
+            wins.append(cols.max() - cols.min() + 1)
+            segs.append(staged_rows(cols))
+    return np.array(wins), np.array(segs)
 
 
+@pytest.mark.parametrize("bar", [True, False])
 @pytest.mark.parametrize("precision", [N.PK_FP32_UPPER, N.PK_FP64_UPPER, N.PK_FP32_STORAGE, N.PK_FP64])
 @pytest.mark.parametrize("shape", [(8, 128, 512, 2), (4, 64, 256, 1), (16, 32, 256, 4), (32, 256, 1024, 2)])
-def test_staged_windows_step_like_global_gathers(precision, shape):
-    w = mixed_batch()
+def test_staged_tiles_step_like_global_gathers(precision, shape, bar):
+    w = mixed_batch(bar=bar)
     a = W.make_system(w, precision)
     b = W.make_system(w, precision)
     a.tune(*shape)
     b.tune(*shape)
     b.set_staging(False)
-    enabled, rows = a.get_staging()
+    enabled, rows, segmented = a.get_staging()
     f32 = precision in (N.PK_FP32_UPPER, N.PK_FP32_STORAGE)
     cap = STAGE_BYTES // (3 * (4 if f32 else 8))
-    widths = windows(w, shape[1])
-    print(f"precision {precision} shape {shape}: staged rows {rows}, windows {widths.min()}..{widths.max()}",
-          flush=True)
-    assert enabled and rows == min(widths.max(), cap)
-    assert (widths <= rows).any() and (widths > rows).any(), "the batch must put tiles on both sides"
-    assert b.get_staging() == (False, 0)
+    wins, segs = windows(w, shape[1])
+    print(f"precision {precision} shape {shape} bar {bar}: staged rows {rows}, segmented {segmented}, windows "
+          f"{wins.min()}..{wins.max()}, segments {segs.min()}..{segs.max()}", flush=True)
+    assert enabled
+    if wins.max() <= cap:
+        assert not segmented and rows == wins.max()
+    else:
+        assert segmented and rows == min(segs.max(), cap)
+    if bar:
+        assert segmented and (segs <= rows).any() and (segs > rows).any(), "tiles on both sides of the capacity"
+    else:
+        assert not segmented
+    assert b.get_staging() == (False, 0, False)
     for n in range(w.steps):
         a.step(w.dt)
         b.step(w.dt)
@@ -103,10 +129,10 @@ def test_staging_is_switchable():
     w = mixed_batch()
     for precision in (N.PK_FP32_STORAGE, N.PK_FP32_UPPER):
         s = W.make_system(w, precision)
-        enabled, rows = s.get_staging()
-        assert enabled and rows == 1706  # the slab's windows are wider than the capacity
+        enabled, rows, segmented = s.get_staging()
+        assert enabled and segmented and rows == 1706  # the bar's tiles need more than the capacity
         s.set_staging(False)
-        assert s.get_staging() == (False, 0)
+        assert s.get_staging() == (False, 0, False)
         s.set_staging(True)
-        assert s.get_staging() == (True, rows)
+        assert s.get_staging() == (True, rows, True)
         s.close()
