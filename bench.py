@@ -72,7 +72,7 @@ def workload(name):
             "batch128": W.batch128}[name]()
 
 
-def spmv_bytes(blocks, rows, body_launches, s, upper_blocks=None):
+def spmv_bytes(blocks, rows, body_launches, s, upper_blocks=None, fused=False):
     """Algorithmic bytes of SpMV work q = A d.
 
     Full storage: every multiplied 3x3 block once (9 values of s bytes +
@@ -82,11 +82,14 @@ def spmv_bytes(blocks, rows, body_launches, s, upper_blocks=None):
     them): every stored record once (10 words of s bytes: 9 values and the
     column; it also serves the transposed product of its mirror row), an
     8-byte (slot, column) entry per lower block, per row an 8-byte pair of row
-    pointers and d, q. One launch over a whole system is
+    pointers and d, q. With the direction update fused into the SpMV, a row
+    reads z and d_{k-1} and writes d_k instead of reading d_k (two more vector
+    passes). One launch over a whole system is
     spmv_bytes(nnzb_full, nd, bodies, s[, nnzb_upper])."""
+    vec = (4 if fused else 2) * 3 * s
     if upper_blocks is None:
-        return blocks * (9 * s + 4) + rows * (4 + 2 * 3 * s) + 4 * body_launches
-    return upper_blocks * 10 * s + (blocks - upper_blocks) * 8 + rows * (8 + 2 * 3 * s) + 8 * body_launches
+        return blocks * (9 * s + 4) + rows * (4 + vec) + 4 * body_launches
+    return upper_blocks * 10 * s + (blocks - upper_blocks) * 8 + rows * (8 + vec) + 8 * body_launches
 
 
 def measured_traffic(workload_name, precision):
@@ -112,15 +115,16 @@ def roofline(s, name, prec_name):
     esize = 8 if s.precision in (N.PK_FP64, N.PK_FP64_UPPER) else 4
     upper = s.precision in (N.PK_FP32_UPPER, N.PK_FP64_UPPER)
     blocks, ublocks, rows, body_launches = s.spmv_work()
-    total = spmv_bytes(blocks, rows, body_launches, esize, ublocks if upper else None)
-    full = spmv_bytes(s.nnzb_full, s.nd, s.groups, esize, s.nnzb_upper if upper else None)
+    _, staged_rows, segmented, fused = s.get_staging()
+    total = spmv_bytes(blocks, rows, body_launches, esize, ublocks if upper else None, fused)
+    full = spmv_bytes(s.nnzb_full, s.nd, s.groups, esize, s.nnzb_upper if upper else None, fused)
     # the same work in full storage, for comparison across formats
-    full_format = spmv_bytes(blocks, rows, body_launches, esize)
+    full_format = spmv_bytes(blocks, rows, body_launches, esize, None, fused)
     achieved = total / (ms_spmv * 1e-3) / 1e9 if n_spmv and ms_spmv > 0 else 0.0
     measured = measured_traffic(name, prec_name)
     return {
         "bound": "hbm",
-        "kernel": "k_spmv (q = A d_k on the full BSR, d.q partial per tile)",
+        "kernel": "k_spmv (q = A d_k, d.q partial per tile; d_k = z_k + beta d_k-1 formed in it when fused)",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -129,6 +133,7 @@ def roofline(s, name, prec_name):
         "traffic_vs_algorithmic": None if measured is None else measured[0] / measured[1],
         "traffic_shape": None if measured is None else measured[2],  # CG launch shape of the counter passes
         "storage": "upper triangle" if upper else "full",
+        "staging": {"rows": staged_rows, "segments": segmented, "direction_fused": fused},
         "bytes_per_launch": total / max(n_spmv, 1),
         "full_storage_bytes_per_launch": full_format / max(n_spmv, 1),
         "bytes_per_full_launch": full,

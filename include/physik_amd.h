@@ -131,17 +131,22 @@ int pk_system_sizes(const pk_system* sys, int* nodes, int* tets, int* groups, in
 int pk_system_tune(pk_system* sys, int lanes_per_row, int spmv_rows, int vector_rows, int unroll);
 int pk_system_get_tune(const pk_system* sys, int* lanes_per_row, int* spmv_rows, int* vector_rows, int* unroll);
 
-/* SpMV staging (no reference counterpart): enable != 0 (the default) lets
- * each SpMV workgroup copy the rows of the direction vector its tile
- * references into LDS (up to 20 KB) and read the direction blocks from there.
- * When every tile's column window [first, last] fits, tiles stage their
- * windows; otherwise they stage up to four segments of rows (the columns cut
- * at their three widest gaps), and tiles that still need more than 20 KB
- * gather from global memory. Results are identical either way. get reports
- * the setting, the rows a workgroup stages (0: none) and whether segments are
- * in use. */
-int pk_system_set_spmv_staging(pk_system* sys, int enable);
-int pk_system_get_spmv_staging(const pk_system* sys, int* enabled, int* rows, int* segmented);
+/* SpMV staging (no reference counterpart). mode 1 (the default) lets each
+ * SpMV workgroup copy the rows of the direction vector its tile references
+ * into LDS (up to 20 KB) and read the direction blocks from there: when every
+ * tile's column window [first, last] fits, tiles stage their windows;
+ * otherwise they stage up to four segments of rows (the columns cut at their
+ * three widest gaps), and tiles that still need more than 20 KB gather from
+ * global memory. mode 2 does the same and, when every tile stages, also folds
+ * the direction update into the SpMV: each workgroup takes the stop rules and
+ * beta itself, stages z + beta d_{k-1} and writes d_k for its own rows, so a
+ * CG iteration is two launches instead of three (on MI355X this is slower
+ * than mode 1: every SpMV workgroup repeats the reduction prologue). mode 0
+ * gathers from global memory. Results are identical in every mode. get
+ * reports the mode, the rows a workgroup stages (0: none), whether segments
+ * are in use and whether the direction update is fused. */
+int pk_system_set_spmv_staging(pk_system* sys, int mode);
+int pk_system_get_spmv_staging(const pk_system* sys, int* mode, int* rows, int* segmented, int* fused);
 
 /* Node state, node_count x 3 each; either pointer may be NULL. */
 int pk_system_set_state(pk_system* sys, const double* x, const double* v);

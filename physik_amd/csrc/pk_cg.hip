@@ -258,8 +258,33 @@ extern __shared__ __align__(16) unsigned char pk_stage_lds[];
 // Copies the tile's segments of d into LDS with coalesced loads and waits
 // for the workgroup; returns the staged view. LDS element i belongs to
 // segment s with 3 end[s-1] <= i < 3 end[s] and holds d element i - 3 shift[s].
+// What a staging copy reads for element gi of d: d itself (mode 0), or, when
+// the SpMV also does the direction update of iteration k > 0 (mode 1),
+// z_k + beta d_{k-1}, formed exactly as k_dir forms it and stored to d_k for
+// the tile's own rows [own0, own1) (each row has one owner tile).
+template <class M>
+struct DirSource
+{
+    const M* __restrict__ d;
+    const M* __restrict__ z;
+    const M* __restrict__ dprev;
+    M* dcur;
+    double beta;
+    int mode;
+    int own0, own1;
+    __device__ M get(int gi) const
+    {
+        if (mode == 0)
+            return d[gi];
+        const M v = direction(z[gi], dprev[gi], beta);
+        if (gi >= own0 && gi < own1)
+            dcur[gi] = v;
+        return v;
+    }
+};
+
 template <class M, bool SEG>
-__device__ inline StagedDir<M, SEG> stage_segments(const M* __restrict__ d, const StageSegs& seg)
+__device__ inline StagedDir<M, SEG> stage_segments(const DirSource<M>& src, const StageSegs& seg)
 {
     M* lds = reinterpret_cast<M*>(pk_stage_lds);
     const int4 lo = seg.lo, end = seg.end;
@@ -279,7 +304,7 @@ __device__ inline StagedDir<M, SEG> stage_segments(const M* __restrict__ d, cons
             const int i = i0 + j * NT;
             const int sh = SEG ? v.sh0 + (i >= e0 ? v.dl1 : 0) + (i >= e1 ? v.dl2 : 0) + (i >= e2 ? v.dl3 : 0)
                                : v.sh0;
-            x[j] = i < n ? d[i - 3 * sh] : M(0);
+            x[j] = i < n ? src.get(i - 3 * sh) : M(0);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
@@ -308,6 +333,43 @@ __device__ inline void load_block(const float* __restrict__ vals, size_t, const 
     a[0] = w0.x; a[1] = w0.y; a[2] = w1.x; a[3] = w1.y; a[4] = w2.x;
     a[5] = w2.y; a[6] = w3.x; a[7] = w3.y; a[8] = w4.x;
     col = __float_as_int(w4.y);
+}
+
+// The start of an SpMV workgroup, with `fuse` (every tile's staging fits and
+// there is no direction launch): the stop rules for the end of iteration k-1
+// and beta, taken as k_dir takes them (each workgroup from the same partials;
+// the body's first tile records them), then the direction source of the
+// staging copy. Without `fuse`: k_dir(k) has decided, and the body is skipped
+// when its loop ended at or before k. Returns false when the workgroup has
+// nothing to do.
+template <class M>
+__device__ inline bool spmv_start(int k, int t, const RowTile& tl, const TileView& st, const TileView& vt,
+                                  int fuse, const M* __restrict__ d, const M* __restrict__ z,
+                                  const M* __restrict__ dprev, M* dcur, const Partial* __restrict__ pbPrev,
+                                  GroupState* gs, int* active, DirSource<M>& src)
+{
+    GroupState& s = gs[tl.group];
+    double beta = 0.0;
+    if (fuse)
+    {
+        if (read_uniform(&s.stopAt) >= 0)
+            return false;
+        double rz, rr;
+        int flags;
+        sum_partials(pbPrev, vt.groupBegin[tl.group], vt.groupCount[tl.group], rz, rr, flags);
+        if (stop_rules(k, s, rz, rr, flags, t == st.groupBegin[tl.group] && threadIdx.x == 0, active, beta))
+            return false;
+    }
+    else
+    {
+        const int stopAt = read_uniform(&s.stopAt);
+        if (stopAt >= 0 && stopAt <= k)
+            return false;
+    }
+    // at k = 0 the right-hand side kernel wrote d_0
+    src = DirSource<M>{fuse && k > 0 ? dcur : d, z, dprev, dcur, beta, fuse && k > 0 ? 1 : 0,
+                       3 * tl.rowBegin, 3 * tl.rowEnd};
+    return true;
 }
 
 // The rows of one tile (full storage); returns the lane's d.q partial.
@@ -400,7 +462,9 @@ template <class M, int LPR, int U, bool SEG>
 __device__ inline void spmv_body(int k, const TileView& st, const int* __restrict__ fstart,
                                  const int* __restrict__ fcol, const M* __restrict__ vals, long long nnzf,
                                  const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
-                                 const GroupState* gs, const StageSegs* __restrict__ win, int winCap)
+                                 GroupState* gs, const StageSegs* __restrict__ win, int winCap, int fuse,
+                                 const TileView& vt, const M* __restrict__ z, const M* __restrict__ dprev, M* dcur,
+                                 const Partial* __restrict__ pbPrev, int* active)
 {
     int t;
     RowTile tl;
@@ -409,12 +473,12 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
     StageSegs sg{make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
     if (winCap > 0)
         sg = win[t];  // in flight with the stop flag
-    const int stopAt = read_uniform(&gs[tl.group].stopAt);
-    if (stopAt >= 0 && stopAt <= k)
+    DirSource<M> src;
+    if (!spmv_start(k, t, tl, st, vt, fuse, d, z, dprev, dcur, pbPrev, gs, active, src))
         return;
     double pd;
     if (winCap > 0 && sg.end.w <= winCap)  // uniform: one tile per workgroup
-        pd = full_rows<M, LPR, U>(tl, fstart, fcol, vals, nnzf, stage_segments<M, SEG>(d, sg), q);
+        pd = full_rows<M, LPR, U>(tl, fstart, fcol, vals, nnzf, stage_segments<M, SEG>(src, sg), q);
     else
         pd = full_rows<M, LPR, U>(tl, fstart, fcol, vals, nnzf, GlobalDir<M>{d}, q);
     double u1 = 0.0, u2 = 0.0;
@@ -571,7 +635,9 @@ template <class M, int LPR, int U, bool SEG>
 __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __restrict__ rowptr,
                                        const int2* __restrict__ lower, const M* __restrict__ rec,
                                        const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
-                                       const GroupState* gs, const StageSegs* __restrict__ win, int winCap)
+                                       GroupState* gs, const StageSegs* __restrict__ win, int winCap, int fuse,
+                                       const TileView& vt, const M* __restrict__ z, const M* __restrict__ dprev,
+                                       M* dcur, const Partial* __restrict__ pbPrev, int* active)
 {
     int t;
     RowTile tl;
@@ -580,12 +646,12 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
     StageSegs sg{make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
     if (winCap > 0)
         sg = win[t];  // in flight with the stop flag
-    const int stopAt = read_uniform(&gs[tl.group].stopAt);
-    if (stopAt >= 0 && stopAt <= k)
+    DirSource<M> src;
+    if (!spmv_start(k, t, tl, st, vt, fuse, d, z, dprev, dcur, pbPrev, gs, active, src))
         return;
     double pd;
     if (winCap > 0 && sg.end.w <= winCap)  // uniform: one tile per workgroup
-        pd = upper_rows<M, LPR, U>(tl, rowptr, lower, rec, stage_segments<M, SEG>(d, sg), q);
+        pd = upper_rows<M, LPR, U>(tl, rowptr, lower, rec, stage_segments<M, SEG>(src, sg), q);
     else
         pd = upper_rows<M, LPR, U>(tl, rowptr, lower, rec, GlobalDir<M>{d}, q);
     double u1 = 0.0, u2 = 0.0;
@@ -598,15 +664,19 @@ template <class M, int LPR, int U, bool SEG>
 __global__ __launch_bounds__(NT) void k_spmv_upper(const int* kb, int k0, TileView st, const int2* __restrict__ rowptr,
                                                    const int2* __restrict__ lower, const M* __restrict__ rec,
                                                    const M* __restrict__ d, M* __restrict__ q,
-                                                   double* __restrict__ pa, const GroupState* gs,
-                                                   const StageSegs* __restrict__ win, int winCap,
+                                                   double* __restrict__ pa, GroupState* gs,
+                                                   const StageSegs* __restrict__ win, int winCap, int fuse,
+                                                   TileView vt, const M* __restrict__ z,
+                                                   const M* __restrict__ dprev, M* dcur,
+                                                   const Partial* __restrict__ pbPrev, int* active,
                                                    unsigned long long* stamps)
 {
     const LaunchIndex li = launch_index(kb, k0, stamps);
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    spmv_body_upper<M, LPR, U, SEG>(k, st, rowptr, lower, rec, d, q, pa, gs, win, winCap);
+    spmv_body_upper<M, LPR, U, SEG>(k, st, rowptr, lower, rec, d, q, pa, gs, win, winCap, fuse, vt, z, dprev, dcur,
+                                    pbPrev, active);
     stamp_end(stamp);
 }
 
@@ -614,15 +684,18 @@ template <class M, int LPR, int U, bool SEG>
 __global__ __launch_bounds__(NT) void k_spmv(const int* kb, int k0, TileView st, const int* __restrict__ fstart,
                                              const int* __restrict__ fcol, const M* __restrict__ vals,
                                              long long nnzf, const M* __restrict__ d, M* __restrict__ q,
-                                             double* __restrict__ pa, const GroupState* gs,
-                                             const StageSegs* __restrict__ win, int winCap,
+                                             double* __restrict__ pa, GroupState* gs,
+                                             const StageSegs* __restrict__ win, int winCap, int fuse, TileView vt,
+                                             const M* __restrict__ z, const M* __restrict__ dprev, M* dcur,
+                                             const Partial* __restrict__ pbPrev, int* active,
                                              unsigned long long* stamps)
 {
     const LaunchIndex li = launch_index(kb, k0, stamps);
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    spmv_body<M, LPR, U, SEG>(k, st, fstart, fcol, vals, nnzf, d, q, pa, gs, win, winCap);
+    spmv_body<M, LPR, U, SEG>(k, st, fstart, fcol, vals, nnzf, d, q, pa, gs, win, winCap, fuse, vt, z, dprev, dcur,
+                              pbPrev, active);
     stamp_end(stamp);
 }
 
@@ -843,7 +916,9 @@ static void launch_spmv_upper_shape(const DeviceView& v, const int* kb, int k0, 
     auto kernel = v.stageSegs ? k_spmv_upper<M, LPR, U, true> : k_spmv_upper<M, LPR, U, false>;
     hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds, v.stream, kb, k0,
                        v.spmv, v.rowptr, v.lower, static_cast<const M*>(v.vals), static_cast<const M*>(v.d[k0 & 1]),
-                       static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap, stamp);
+                       static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap, v.fuseDir, v.vec,
+                       static_cast<const M*>(v.z), static_cast<const M*>(v.d[(k0 + 1) & 1]),
+                       static_cast<M*>(v.d[k0 & 1]), v.pb[(k0 + 1) & 1], v.active, stamp);
 }
 
 // The iteration of a launch is k0 (kb == nullptr) or *kb + k0 with *kb even,
@@ -856,7 +931,8 @@ static void launch_spmv_shape(const DeviceView& v, const int* kb, int k0, unsign
     hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds, v.stream, kb, k0,
                        v.spmv, v.fstart, v.fcol, static_cast<const M*>(v.vals), v.nnzf,
                        static_cast<const M*>(v.d[k0 & 1]), static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap,
-                       stamp);
+                       v.fuseDir, v.vec, static_cast<const M*>(v.z), static_cast<const M*>(v.d[(k0 + 1) & 1]),
+                       static_cast<M*>(v.d[k0 & 1]), v.pb[(k0 + 1) & 1], v.active, stamp);
 }
 
 template <class M>

@@ -335,6 +335,7 @@ struct DeviceView
     const StageSegs* spmvWin;
     int winCap;
     int stageSegs;  // 0: every tile's staging is one window (segment 0 only)
+    int fuseDir;    // every tile stages: the SpMV also does the direction update (no k_dir launch)
     // reductions
     TileView vec, spmv;
     double* pa;        // per SpMV tile: d.q

@@ -150,7 +150,8 @@ struct pk_system
     pk::DevBuf<pk::StageSegs> spmvWin;
     int winCap = 0;
     int stageSegs = 0;
-    bool staging = true;
+    int fuseDir = 0;
+    int stagingMode = 1;  // 0 off, 1 staging, 2 staging with the direction update fused where every tile stages
     int* pinned = nullptr;
     hipEvent_t pollEv[2] = {nullptr, nullptr};
 
@@ -332,6 +333,7 @@ struct pk_system
         dv.spmvWin = spmvWin.p;
         dv.winCap = winCap;
         dv.stageSegs = stageSegs;
+        dv.fuseDir = fuseDir;
         dv.pa = pa.p;
         dv.pb[0] = pb0.p;
         dv.pb[1] = pb1.p;
@@ -475,7 +477,7 @@ static void harvest_profile(pk_system* s)
                     add(PK_K_SPMV, a, k);
                 if (k < updWork)
                     add(PK_K_UPDATE, b, k);
-                if (k < dirWork)
+                if (k < dirWork && !s->fuseDir)
                     add(PK_K_DIRECTION, c, k);
             }
         }
@@ -543,7 +545,8 @@ static int capture_chunk(pk_system* s, const DeviceView& dv, bool compacted, uns
     }
     for (int o = 0; o < kChunk; ++o)
     {
-        launch_dir<M>(cv, dv.kbase, o, stD);
+        if (!dv.fuseDir)
+            launch_dir<M>(cv, dv.kbase, o, stD);
         launch_spmv<M>(cv, dv.kbase, o, s->lanesPerRow, s->unroll, stS);
         launch_update<M>(cv, dv.kbase, o, stU);
     }
@@ -646,13 +649,15 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
                     cv.vec.perm = s->vecPerm.p;
                     cv.spmv.perm = s->spmvPerm.p;
                 }
-                launch_dir<M>(cv, nullptr, k, stD ? stD + 2 * static_cast<size_t>(gU) * k : nullptr);
-                if (k < maxIt)
-                {
+                // fused: the SpMV takes the stop decision of k (also at
+                // k = maxIt) and forms d_k; otherwise k_dir does
+                if (!dv.fuseDir)
+                    launch_dir<M>(cv, nullptr, k, stD ? stD + 2 * static_cast<size_t>(gU) * k : nullptr);
+                if (k < maxIt || dv.fuseDir)
                     launch_spmv<M>(cv, nullptr, k, s->lanesPerRow, s->unroll,
                                    stS ? stS + 2 * static_cast<size_t>(gS) * k : nullptr);
+                if (k < maxIt)
                     launch_update<M>(cv, nullptr, k, stU ? stU + 2 * static_cast<size_t>(gU) * k : nullptr);
-                }
             }
         }
         chunk = kChunk;
@@ -671,7 +676,8 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
     {
         // graph chunks ran the SpMV and update of every launch they covered
         const int nS = s->useGraphs ? k : std::min(k, maxIt);
-        launch_stamp_reduce(s->stream, stD, gU, k, s->stampDir.p);
+        if (!dv.fuseDir)
+            launch_stamp_reduce(s->stream, stD, gU, k, s->stampDir.p);
         launch_stamp_reduce(s->stream, stS, gS, nS, s->stampSpmv.p);
         launch_stamp_reduce(s->stream, stU, gU, nS, s->stampUpd.p);
     }
@@ -782,7 +788,8 @@ static int setup_windows(pk_system* s)
     const Topology& t = s->topo;
     s->winCap = 0;
     s->stageSegs = 0;
-    if (!s->staging || t.nd == 0)
+    s->fuseDir = 0;
+    if (s->stagingMode == 0 || t.nd == 0)
         return PK_OK;
     const size_t bytesPerRow = 3 * (is_f32(s) ? sizeof(float) : sizeof(double));
     const int cap = static_cast<int>(kStageBytes / bytesPerRow);
@@ -836,10 +843,12 @@ static int setup_windows(pk_system* s)
     if (widestWin <= cap)
     {
         s->winCap = widestWin;
+        s->fuseDir = s->stagingMode == 2 ? 1 : 0;
         return s->spmvWin.upload(wins, s->stream);
     }
     s->stageSegs = 1;
     s->winCap = std::min(widestSeg, cap);
+    s->fuseDir = s->stagingMode == 2 && widestSeg <= cap ? 1 : 0;
     return s->spmvWin.upload(segs, s->stream);
 }
 
@@ -1053,12 +1062,15 @@ int pk_system_tune(pk_system* s, int lanes_per_row, int spmv_rows, int vector_ro
     return setup_tiles(s);
 }
 
-int pk_system_set_spmv_staging(pk_system* s, int enable)
+int pk_system_set_spmv_staging(pk_system* s, int mode)
 {
-    if (!s)
+    if (!s || mode < 0 || mode > 2)
+    {
+        set_error("pk_system_set_spmv_staging: mode 0 (off), 1 (staging) or 2 (staging, direction fused)");
         return PK_ERR_ARG;
+    }
     s->tapeValid = false;  // a recorded run no longer replays the same launches
-    s->staging = enable != 0;
+    s->stagingMode = mode;
     ++s->epoch;
     const int rc = setup_windows(s);
     if (rc != PK_OK)
@@ -1067,13 +1079,14 @@ int pk_system_set_spmv_staging(pk_system* s, int enable)
     return PK_OK;
 }
 
-int pk_system_get_spmv_staging(const pk_system* s, int* enabled, int* rows, int* segmented)
+int pk_system_get_spmv_staging(const pk_system* s, int* mode, int* rows, int* segmented, int* fused)
 {
     if (!s)
         return PK_ERR_ARG;
-    if (enabled) *enabled = s->staging ? 1 : 0;
+    if (mode) *mode = s->stagingMode;
     if (rows) *rows = s->winCap;
     if (segmented) *segmented = s->stageSegs;
+    if (fused) *fused = s->fuseDir;
     return PK_OK;
 }
 

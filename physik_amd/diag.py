@@ -35,7 +35,8 @@ def main():
     ap.add_argument("--shape", default=None,
                     help="CG launch shape lanes,spmv_rows,vector_rows,unroll (default: the system's own)")
     ap.add_argument("--spheres", type=int, default=0)
-    ap.add_argument("--no-staging", action="store_true", help="SpMV gathers d from global memory (upper storage)")
+    ap.add_argument("--staging", type=int, default=1, choices=[0, 1, 2],
+                    help="SpMV staging of d: 0 off, 1 on, 2 on with the direction update fused")
     args = ap.parse_args()
     lib = N.lib()
     ms = C.c_double()
@@ -46,8 +47,7 @@ def main():
                           "fp64u": N.PK_FP64_UPPER}[args.precision])
     if args.shape:
         s.tune(*[int(a) for a in args.shape.split(",")])
-    if args.no_staging:
-        s.set_staging(False)
+    s.set_staging(args.staging)
     extra = {}
     if args.spheres > 0:
         lo, hi = w.x0.min(0), w.x0.max(0)

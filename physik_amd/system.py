@@ -101,16 +101,17 @@ class System:
         from .autotune import autotune
         return autotune(self, dt, cache=cache, verbose=verbose)
 
-    def set_staging(self, enable=True):
-        """SpMV direction windows staged in LDS (upper storage; identical results)."""
-        N.check(self.lib.pk_system_set_spmv_staging(self.h, int(enable)), "pk_system_set_spmv_staging")
+    def set_staging(self, mode=1):
+        """SpMV staging of d in LDS: 0 off, 1 on, 2 on with the direction update
+        fused into the SpMV where every tile stages (identical results)."""
+        N.check(self.lib.pk_system_set_spmv_staging(self.h, int(mode)), "pk_system_set_spmv_staging")
 
     def get_staging(self):
-        """(enabled, rows a workgroup stages, segments in use)."""
-        e, r, g = C.c_int(), C.c_int(), C.c_int()
-        N.check(self.lib.pk_system_get_spmv_staging(self.h, C.byref(e), C.byref(r), C.byref(g)),
+        """(mode, rows a workgroup stages, segments in use, direction fused)."""
+        m, r, g, f = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        N.check(self.lib.pk_system_get_spmv_staging(self.h, C.byref(m), C.byref(r), C.byref(g), C.byref(f)),
                 "pk_system_get_spmv_staging")
-        return bool(e.value), r.value, bool(g.value)
+        return m.value, r.value, bool(g.value), bool(f.value)
 
     def get_tune(self):
         v = [C.c_int() for _ in range(4)]

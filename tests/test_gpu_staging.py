@@ -86,32 +86,35 @@ def windows(w, rows_per_tile):
     return np.array(wins), np.array(segs)
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("bar", [True, False])
 @pytest.mark.parametrize("precision", [N.PK_FP32_UPPER, N.PK_FP64_UPPER, N.PK_FP32_STORAGE, N.PK_FP64])
 @pytest.mark.parametrize("shape", [(8, 128, 512, 2), (4, 64, 256, 1), (16, 32, 256, 4), (32, 256, 1024, 2)])
-def test_staged_tiles_step_like_global_gathers(precision, shape, bar):
+def test_staged_tiles_step_like_global_gathers(precision, shape, bar, mode):
     w = mixed_batch(bar=bar)
     a = W.make_system(w, precision)
     b = W.make_system(w, precision)
     a.tune(*shape)
     b.tune(*shape)
-    b.set_staging(False)
-    enabled, rows, segmented = a.get_staging()
+    a.set_staging(mode)
+    b.set_staging(0)
+    got_mode, rows, segmented, fused = a.get_staging()
     f32 = precision in (N.PK_FP32_UPPER, N.PK_FP32_STORAGE)
     cap = STAGE_BYTES // (3 * (4 if f32 else 8))
     wins, segs = windows(w, shape[1])
-    print(f"precision {precision} shape {shape} bar {bar}: staged rows {rows}, segmented {segmented}, windows "
-          f"{wins.min()}..{wins.max()}, segments {segs.min()}..{segs.max()}", flush=True)
-    assert enabled
+    print(f"precision {precision} shape {shape} bar {bar} mode {mode}: staged rows {rows}, segmented {segmented}, "
+          f"fused {fused}, windows {wins.min()}..{wins.max()}, segments {segs.min()}..{segs.max()}", flush=True)
+    assert got_mode == mode
     if wins.max() <= cap:
         assert not segmented and rows == wins.max()
     else:
         assert segmented and rows == min(segs.max(), cap)
     if bar:
         assert segmented and (segs <= rows).any() and (segs > rows).any(), "tiles on both sides of the capacity"
+        assert not fused  # a tile that gathers from global memory needs d_k from a direction launch
     else:
-        assert not segmented
-    assert b.get_staging() == (False, 0, False)
+        assert not segmented and fused == (mode == 2)
+    assert b.get_staging() == (0, 0, False, False)
     for n in range(w.steps):
         a.step(w.dt)
         b.step(w.dt)
@@ -129,10 +132,18 @@ def test_staging_is_switchable():
     w = mixed_batch()
     for precision in (N.PK_FP32_STORAGE, N.PK_FP32_UPPER):
         s = W.make_system(w, precision)
-        enabled, rows, segmented = s.get_staging()
-        assert enabled and segmented and rows == 1706  # the bar's tiles need more than the capacity
-        s.set_staging(False)
-        assert s.get_staging() == (False, 0, False)
-        s.set_staging(True)
-        assert s.get_staging() == (True, rows, True)
+        mode, rows, segmented, fused = s.get_staging()
+        assert mode == 1 and segmented and not fused and rows == 1706  # the bar's tiles need more than the capacity
+        s.set_staging(2)
+        assert s.get_staging() == (2, rows, True, False)  # a tile gathers from global memory: no fusing
+        s.set_staging(0)
+        assert s.get_staging() == (0, 0, False, False)
+        s.set_staging(1)
+        assert s.get_staging() == (1, rows, True, False)
+        assert s.lib.pk_system_set_spmv_staging(s.h, 3) == N.PK_ERR_ARG
         s.close()
+    s = W.make_system(mixed_batch(bar=False), N.PK_FP32_UPPER)
+    assert s.get_staging()[1:] == (s.get_staging()[1], False, False)
+    s.set_staging(2)
+    assert s.get_staging()[3]  # every tile stages: the direction update is fused
+    s.close()
