@@ -199,15 +199,30 @@ __global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ 
                                                 const double* __restrict__ v, const int* __restrict__ cpos,
                                                 double* __restrict__ basis, double* __restrict__ corner)
 {
-    const int t = xcd_block() * NT + static_cast<int>(threadIdx.x);
-    if (t >= T || !live[t])
-        return;
+    const int t0 = xcd_block() * NT;
+    const int t = t0 + static_cast<int>(threadIdx.x);
     const size_t TT = static_cast<size_t>(T);
+    // the workgroup's basis records go out as one contiguous run: each thread
+    // puts its record in LDS, then the workgroup copies the run with
+    // consecutive 16-byte stores (a tet that is not live stores zeros, which
+    // is what it holds: it contributes nothing)
+    __shared__ double2 stage[NT * EB_COUNT / 2];
     ElementOut o;
-    element_compute(t, TT, tets, prm, x, v, o);
+    o.valid = false;
+    const bool work = t < T && live[t];
+    if (work)
+        element_compute(t, TT, tets, prm, x, v, o);
 #pragma unroll
-    for (int k = 0; k < EB_COUNT; ++k)
-        basis[k * TT + t] = o.valid ? o.basis[k] : 0.0;
+    for (int k = 0; k < EB_COUNT / 2; ++k)
+        stage[threadIdx.x * (EB_COUNT / 2) + k] =
+            o.valid ? make_double2(o.basis[2 * k], o.basis[2 * k + 1]) : make_double2(0.0, 0.0);
+    __syncthreads();
+    const int nrec = min(NT, T - t0);
+    double2* dst = reinterpret_cast<double2*>(basis + static_cast<size_t>(t0) * EB_COUNT);
+    for (int i = threadIdx.x; i < nrec * (EB_COUNT / 2); i += NT)
+        dst[i] = stage[i];
+    if (!work)
+        return;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
     {
@@ -247,22 +262,24 @@ __global__ __launch_bounds__(NT) void k_gdot(int T, double* __restrict__ prm)
     }
 }
 
-// Block R K_ab R^T of local pair p of tet t from its basis planes.
+// Block R K_ab R^T of local pair p of tet t from its basis record.
 __device__ inline void element_block(const double* __restrict__ basis, const double* __restrict__ prm, size_t TT,
                                      int t, int p, double* K)
 {
     int a, b;
     pair_of(p, a, b);
+    const double* rec = basis + static_cast<size_t>(t) * EB_COUNT;
     double rga[3], rgb[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
-        rga[i] = basis[(EB_RG + 3 * a + i) * TT + t];
-        rgb[i] = basis[(EB_RG + 3 * b + i) * TT + t];
+        rga[i] = rec[EB_RG + 3 * a + i];
+        rgb[i] = rec[EB_RG + 3 * b + i];
     }
-    const double r00 = basis[(EB_RRT + 0) * TT + t], r01 = basis[(EB_RRT + 1) * TT + t],
-                 r02 = basis[(EB_RRT + 2) * TT + t], r11 = basis[(EB_RRT + 3) * TT + t],
-                 r12 = basis[(EB_RRT + 4) * TT + t], r22 = basis[(EB_RRT + 5) * TT + t];
+    // R R^T: 48 bytes on a 16-byte boundary (the record is 144 bytes)
+    const double2* rr = reinterpret_cast<const double2*>(rec + EB_RRT);
+    const double2 rr0 = rr[0], rr1 = rr[1], rr2 = rr[2];
+    const double r00 = rr0.x, r01 = rr0.y, r02 = rr1.x, r11 = rr1.y, r12 = rr2.x, r22 = rr2.y;
     const double RRt[9] = {r00, r01, r02, r01, r11, r12, r02, r12, r22};
     rotated_block(rga, rgb, RRt, prm[P_VOL * TT + t], prm[P_LAMBDA * TT + t], prm[P_MU * TT + t],
                   prm[(P_GDOT + p) * TT + t], K);
@@ -928,7 +945,7 @@ __global__ __launch_bounds__(NT) void k_zero_tets(int n, const int* __restrict__
     const int t = ids[i];
     const size_t TT = static_cast<size_t>(T);
     for (int k = 0; k < EB_COUNT; ++k)
-        basis[k * TT + t] = 0.0;
+        basis[static_cast<size_t>(t) * EB_COUNT + k] = 0.0;
     for (int a = 0; a < 4; ++a)
     {
         const int pos = cpos[a * TT + t];

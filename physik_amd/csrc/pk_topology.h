@@ -29,10 +29,12 @@ enum TetParam
 };
 
 // Element output of one substep:
-//   basis   18 planes, basis[k * T + t]: R g_a (k = 3a + i), then R R^T (xx,
-//           xy, xz, yy, yz, zz). The blocks R K_ab R^T are not stored; the
-//           slot gather rebuilds each from these with the closed form the
-//           element kernel also uses (coalesced writes, 144 B per tet).
+//   basis   one 144-byte record per tet, basis[t * EB_COUNT + k]: R g_a
+//           (k = 3a + i), then R R^T (xx, xy, xz, yy, yz, zz). The blocks
+//           R K_ab R^T are not stored; the slot gather rebuilds each from
+//           these with the closed form the element kernel also uses, and
+//           finds all of a contribution's basis values in the tet's one or
+//           two cache lines.
 //   corner  6 doubles per (dynamic row, tet, corner) in the order of the row
 //           lists: the element force f_a, then sum_b (R K_ab R^T) v_b, written
 //           at cpos[a * T + t], so the right-hand side reads a contiguous run.
