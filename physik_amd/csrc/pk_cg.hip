@@ -699,7 +699,7 @@ __global__ __launch_bounds__(NT) void k_spmv(const int* kb, int k0, TileView st,
     stamp_end(stamp);
 }
 
-template <class M>
+template <class M, bool LONG>
 __device__ inline void update_body(int k, const TileView& vt, const TileView& st, const M* __restrict__ minv,
                                    int nd, const M* __restrict__ dcur, const M* __restrict__ q,
                                    double* __restrict__ xs, double* __restrict__ rv, M* __restrict__ zv,
@@ -739,7 +739,8 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
     const int stopAt = read_uniform(&s.stopAt);
     if (stopAt >= 0 && stopAt <= k)
         return;
-    const double dq = sum_doubles(pa, st.groupBegin[g], st.groupCount[g]);
+    const double dq = LONG ? sum_doubles_long(pa, st.groupBegin[g], st.groupCount[g])
+                           : sum_doubles(pa, st.groupBegin[g], st.groupCount[g]);
     const double delta = s.delta[k & 1];
     const double alpha = delta / dq;
     if (!fin(dq) || dq <= 0.0 || !fin(alpha))
@@ -808,7 +809,7 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
         pbCur[t] = Partial{pz, pr, flag, 0};
 }
 
-template <class M>
+template <class M, bool LONG>
 __global__ __launch_bounds__(NT) void k_update(const int* kb, int k0, TileView vt, TileView st,
                                                const M* __restrict__ minv, int nd, const M* __restrict__ dcur,
                                                const M* __restrict__ q, double* __restrict__ xs,
@@ -821,7 +822,7 @@ __global__ __launch_bounds__(NT) void k_update(const int* kb, int k0, TileView v
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    update_body<M>(k, vt, st, minv, nd, dcur, q, xs, rv, zv, pa, pbCur, pbPrev, gs, active);
+    update_body<M, LONG>(k, vt, st, minv, nd, dcur, q, xs, rv, zv, pa, pbCur, pbPrev, gs, active);
     stamp_end(stamp);
 }
 
@@ -985,7 +986,8 @@ void launch_update(const DeviceView& v, const int* kb, int k0, unsigned long lon
 {
     if (v.vec.count == 0)
         return;
-    hipLaunchKernelGGL((k_update<M>), dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, kb, k0,
+    auto kernel = v.longSums ? k_update<M, true> : k_update<M, false>;
+    hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, kb, k0,
                        v.vec, v.spmv, static_cast<const M*>(v.minv), v.nd, static_cast<const M*>(v.d[k0 & 1]),
                        static_cast<const M*>(v.q), v.xs, v.r, static_cast<M*>(v.z), v.pa, v.pb[k0 & 1],
                        v.pb[(k0 + 1) & 1], v.gs, v.active, stamp);

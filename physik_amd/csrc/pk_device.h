@@ -193,6 +193,29 @@ __device__ inline void sum_partials(const Partial* p, int begin, int count, doub
     flag = __syncthreads_or(f);
 }
 
+// sum_doubles for long lists (a large body's SpMV partials): eight loads in
+// flight per thread, the adds in the same order, so the total is
+// bit-identical to sum_doubles'.
+__device__ inline double sum_doubles_long(const double* p, int begin, int count)
+{
+    double x = 0.0, y = 0.0, z = 0.0;
+    int i = threadIdx.x;
+    for (; i + 7 * NT < count; i += 8 * NT)
+    {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            v[j] = p[begin + i + j * NT];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            x += v[j];
+    }
+    for (; i < count; i += NT)
+        x += p[begin + i];
+    block_sum3(x, y, z);
+    return x;
+}
+
 __device__ inline double sum_doubles(const double* p, int begin, int count)
 {
     double x = 0.0, y = 0.0, z = 0.0;
@@ -336,7 +359,9 @@ struct DeviceView
     int winCap;
     int stageSegs;  // 0: every tile's staging is one window (segment 0 only)
     int fuseDir;    // every tile stages: the SpMV also does the direction update (no k_dir launch)
-    // reductions
+    // reductions; longSums: some body has 4 * NT SpMV tiles or more (its
+    // update workgroups sum long partial lists)
+    int longSums;
     TileView vec, spmv;
     double* pa;        // per SpMV tile: d.q
     Partial* pb[2];    // per vector tile: (r.z, r.r, flags), by iteration parity

@@ -92,6 +92,7 @@ struct Tiles
     DevBuf<int> groupBegin, groupCount;
     int count = 0;
     int per = 0;
+    int longest = 0;             // most tiles of one group
     std::vector<int> hostBegin;  // first tile of each group
 
     TileView view() const { return TileView{tiles.p, groupBegin.p, groupCount.p, count, per}; }
@@ -330,6 +331,7 @@ struct pk_system
         dv.rowAdd = nsph > 0 ? srhs.p : (nconn > 0 ? crhs.p : nullptr);
         dv.vec = vecTiles.view();
         dv.spmv = spmvTiles.view();
+        dv.longSums = spmvTiles.longest >= 4 * pk::NT ? 1 : 0;
         dv.spmvWin = spmvWin.p;
         dv.winCap = winCap;
         dv.stageSegs = stageSegs;
@@ -358,6 +360,7 @@ static int build_tiles(pk_system* s, Tiles& tt, int rowsPerTile)
     }
     tt.count = static_cast<int>(tiles.size());
     tt.per = (tt.count + 7) / 8;
+    tt.longest = gc.empty() ? 0 : *std::max_element(gc.begin(), gc.end());
     tt.hostBegin = gb;
     int rc;
     if ((rc = tt.tiles.upload(tiles, s->stream)) != PK_OK) return rc;
