@@ -95,7 +95,7 @@ def test_export_of_many_nodes_covers_every_value():
     s.close()
 
 
-def _physik_beam(n=3):
+def _physik_beam(n=3, mode=1):
     from physik_amd import meshgen as mg
     rest, tets = mg.box_mesh(n, n, n, 0.1, "five")
     w = World()
@@ -104,7 +104,7 @@ def _physik_beam(n=3):
     w.DestroyGeneratedTetMesh(mesh)
     for i in mg.face_nodes(n, n, n, 0):
         w.SetNodeFixed(int(i), 1)
-    w.SetSolverMode(1)
+    w.SetSolverMode(mode)
     w.SetConjugateGradientTolerance(1e-6)
     w.SetConjugateGradientMaxIterations(200)
     return w, comp, len(rest)
@@ -152,4 +152,21 @@ def test_physik_bulk_getters_clip_and_refuse_bad_ranges():
     assert w.SetFrameExport(4) == 0 and w.SetFrameExport(-1) == 0
     px, _ = _per_node(w, n)
     assert np.array_equal(w.GetNodePositions(n - 3, 10), px[n - 3:])
+    w.close()
+
+
+def test_physik_frame_export_after_explicit_steps():
+    # the explicit integrator (the reference's default mode) exports the same way
+    w, comp, n = _physik_beam(mode=0)
+    w.SetGravity(0.0, 0.0, -9.81)
+    w.SetSubstepCount(8)
+    for step in range(3):
+        w.Step(1.0 / 600.0)
+        px, pv = _per_node(w, n)
+        bx, bv = w.GetNodePositions(0, n), w.GetNodeVelocities(0, n)
+        x, v = w.node_state(n)
+        print(f"explicit step {step}: max |v| {np.abs(v).max():.3e}", flush=True)
+        assert np.abs(v).max() > 0.0  # gravity moved the free nodes
+        assert np.array_equal(px, x.astype(np.float32)) and np.array_equal(pv, v.astype(np.float32))
+        assert np.array_equal(bx, px) and np.array_equal(bv, pv)
     w.close()
