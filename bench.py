@@ -132,6 +132,10 @@ def roofline(s, name, prec_name):
         "traffic": None if measured is None else measured[0],
         "traffic_vs_algorithmic": None if measured is None else measured[0] / measured[1],
         "traffic_shape": None if measured is None else measured[2],  # CG launch shape of the counter passes
+        # the counters' bytes per launch over this run's average launch time:
+        # the rate the kernel actually moves data at (a plain copy reaches ~5.3 TB/s)
+        "traffic_rate_GBps": (None if measured is None or not n_spmv or ms_spmv <= 0
+                              else measured[0] / (ms_spmv / n_spmv * 1e-3) / 1e9),
         "storage": "upper triangle" if upper else "full",
         "staging": {"rows": staged_rows, "segments": segmented, "direction_fused": fused},
         "bytes_per_launch": total / max(n_spmv, 1),
