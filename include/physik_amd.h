@@ -148,6 +148,16 @@ int pk_system_get_tune(const pk_system* sys, int* lanes_per_row, int* spmv_rows,
 int pk_system_set_spmv_staging(pk_system* sys, int mode);
 int pk_system_get_spmv_staging(const pk_system* sys, int* mode, int* rows, int* segmented, int* fused);
 
+/* Pencil SpMV tiles (no reference counterpart; off by default): enable != 0
+ * makes each SpMV tile of a body whose rows come in layers (a stride S found
+ * from the pattern: the median offset of the rows' lower neighbours in the
+ * layer below) pair a run of spmv_rows / 2 rows with the run S rows later,
+ * so that a row's transposed mirror records are read by the same workgroup
+ * in the same pass. Rows without such a stride get plain tiles. The SpMV
+ * partial sums change their grouping, so results agree with plain tiles to
+ * rounding, not bit for bit; full and upper storage stay bit-identical. */
+int pk_system_set_spmv_pencils(pk_system* sys, int enable);
+
 /* Node state, node_count x 3 each; either pointer may be NULL. */
 int pk_system_set_state(pk_system* sys, const double* x, const double* v);
 int pk_system_get_state(pk_system* sys, double* x, double* v);

@@ -374,7 +374,8 @@ __device__ inline bool spmv_start(int k, int t, const RowTile& tl, const TileVie
 
 // The rows of one tile (full storage); returns the lane's d.q partial.
 template <class M, int LPR, int U, class Dir>
-__device__ __attribute__((always_inline)) inline double full_rows(const RowTile& tl, const int* __restrict__ fstart,
+__device__ __attribute__((always_inline)) inline double full_rows(const RowTile& tl, int2 c2,
+                                                                  const int* __restrict__ fstart,
                                                                   const int* __restrict__ fcol,
                                                                   const M* __restrict__ vals, long long nnzf, Dir d,
                                                                   M* __restrict__ q)
@@ -384,14 +385,18 @@ __device__ __attribute__((always_inline)) inline double full_rows(const RowTile&
     constexpr int RPP = NT / LPR;
     const size_t F = static_cast<size_t>(nnzf);
     double pd = 0.0;
-    for (int row0 = tl.rowBegin + sub; row0 < tl.rowEnd; row0 += RPP * U)
+    // position i of the tile: row rowBegin + i of its first run, then
+    // c2.x + (i - len1) of its second (pencil tiles)
+    const int len1 = tl.rowEnd - tl.rowBegin, nrows = len1 + c2.y;
+    for (int i0 = sub; i0 < nrows; i0 += RPP * U)
     {
         int kb[U], ke[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
         {
-            const int row = row0 + u * RPP;
-            const bool live = row < tl.rowEnd;
+            const int i = i0 + u * RPP;
+            const int row = i < len1 ? tl.rowBegin + i : c2.x + (i - len1);
+            const bool live = i < nrows;
             kb[u] = live ? fstart[row] + lane : 0;
             ke[u] = live ? fstart[row + 1] : 0;
         }
@@ -439,8 +444,9 @@ __device__ __attribute__((always_inline)) inline double full_rows(const RowTile&
                 y[u][1] += __shfl_xor(y[u][1], o, LPR);
                 y[u][2] += __shfl_xor(y[u][2], o, LPR);
             }
-            const int row = row0 + u * RPP;
-            if (lane == 0 && row < tl.rowEnd)
+            const int i = i0 + u * RPP;
+            const int row = i < len1 ? tl.rowBegin + i : c2.x + (i - len1);
+            if (lane == 0 && i < nrows)
             {
                 const M q0 = static_cast<M>(y[u][0]), q1 = static_cast<M>(y[u][1]), q2 = static_cast<M>(y[u][2]);
                 q[3 * row] = q0;
@@ -462,7 +468,8 @@ template <class M, int LPR, int U, bool SEG>
 __device__ inline void spmv_body(int k, const TileView& st, const int* __restrict__ fstart,
                                  const int* __restrict__ fcol, const M* __restrict__ vals, long long nnzf,
                                  const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
-                                 GroupState* gs, const StageSegs* __restrict__ win, int winCap, int fuse,
+                                 GroupState* gs, const StageSegs* __restrict__ win, int winCap,
+                                 const int2* __restrict__ chunk2, int fuse,
                                  const TileView& vt, const M* __restrict__ z, const M* __restrict__ dprev, M* dcur,
                                  const Partial* __restrict__ pbPrev, int* active)
 {
@@ -473,14 +480,15 @@ __device__ inline void spmv_body(int k, const TileView& st, const int* __restric
     StageSegs sg{make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
     if (winCap > 0)
         sg = win[t];  // in flight with the stop flag
+    const int2 c2 = chunk2 ? chunk2[t] : make_int2(0, 0);
     DirSource<M> src;
     if (!spmv_start(k, t, tl, st, vt, fuse, d, z, dprev, dcur, pbPrev, gs, active, src))
         return;
     double pd;
     if (winCap > 0 && sg.end.w <= winCap)  // uniform: one tile per workgroup
-        pd = full_rows<M, LPR, U>(tl, fstart, fcol, vals, nnzf, stage_segments<M, SEG>(src, sg), q);
+        pd = full_rows<M, LPR, U>(tl, c2, fstart, fcol, vals, nnzf, stage_segments<M, SEG>(src, sg), q);
     else
-        pd = full_rows<M, LPR, U>(tl, fstart, fcol, vals, nnzf, GlobalDir<M>{d}, q);
+        pd = full_rows<M, LPR, U>(tl, c2, fstart, fcol, vals, nnzf, GlobalDir<M>{d}, q);
     double u1 = 0.0, u2 = 0.0;
     block_sum3(pd, u1, u2);
     if (threadIdx.x == 0)
@@ -541,7 +549,8 @@ __device__ inline void load_upper_block(const M* __restrict__ rec, const int2* _
 // by the address unit, not by HBM bytes, and those variants lose occupancy.
 // The rows of one tile (upper-only storage); returns the lane's d.q partial.
 template <class M, int LPR, int U, class Dir>
-__device__ __attribute__((always_inline)) inline double upper_rows(const RowTile& tl, const int2* __restrict__ rowptr,
+__device__ __attribute__((always_inline)) inline double upper_rows(const RowTile& tl, int2 c2,
+                                                                   const int2* __restrict__ rowptr,
                                                                    const int2* __restrict__ lower,
                                                                    const M* __restrict__ rec, Dir d,
                                                                    M* __restrict__ q)
@@ -550,14 +559,18 @@ __device__ __attribute__((always_inline)) inline double upper_rows(const RowTile
     const int sub = threadIdx.x / LPR;
     constexpr int RPP = NT / LPR;
     double pd = 0.0;
-    for (int row0 = tl.rowBegin + sub; row0 < tl.rowEnd; row0 += RPP * U)
+    // position i of the tile: row rowBegin + i of its first run, then
+    // c2.x + (i - len1) of its second (pencil tiles)
+    const int len1 = tl.rowEnd - tl.rowBegin, nrows = len1 + c2.y;
+    for (int i0 = sub; i0 < nrows; i0 += RPP * U)
     {
         int kb[U], ke[U], lb[U], ub[U], nl[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
         {
-            const int row = row0 + u * RPP;
-            const bool live = row < tl.rowEnd;
+            const int i = i0 + u * RPP;
+            const int row = i < len1 ? tl.rowBegin + i : c2.x + (i - len1);
+            const bool live = i < nrows;
             const int2 r0 = live ? rowptr[row] : make_int2(0, 0);
             const int2 r1 = live ? rowptr[row + 1] : make_int2(0, 0);
             lb[u] = r0.x;
@@ -609,8 +622,9 @@ __device__ __attribute__((always_inline)) inline double upper_rows(const RowTile
                 y[u][1] += __shfl_xor(y[u][1], o, LPR);
                 y[u][2] += __shfl_xor(y[u][2], o, LPR);
             }
-            const int row = row0 + u * RPP;
-            if (lane == 0 && row < tl.rowEnd)
+            const int i = i0 + u * RPP;
+            const int row = i < len1 ? tl.rowBegin + i : c2.x + (i - len1);
+            if (lane == 0 && i < nrows)
             {
                 const M q0 = static_cast<M>(y[u][0]), q1 = static_cast<M>(y[u][1]), q2 = static_cast<M>(y[u][2]);
                 q[3 * row] = q0;
@@ -635,7 +649,8 @@ template <class M, int LPR, int U, bool SEG>
 __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __restrict__ rowptr,
                                        const int2* __restrict__ lower, const M* __restrict__ rec,
                                        const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
-                                       GroupState* gs, const StageSegs* __restrict__ win, int winCap, int fuse,
+                                       GroupState* gs, const StageSegs* __restrict__ win, int winCap,
+                                 const int2* __restrict__ chunk2, int fuse,
                                        const TileView& vt, const M* __restrict__ z, const M* __restrict__ dprev,
                                        M* dcur, const Partial* __restrict__ pbPrev, int* active)
 {
@@ -646,14 +661,15 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
     StageSegs sg{make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
     if (winCap > 0)
         sg = win[t];  // in flight with the stop flag
+    const int2 c2 = chunk2 ? chunk2[t] : make_int2(0, 0);
     DirSource<M> src;
     if (!spmv_start(k, t, tl, st, vt, fuse, d, z, dprev, dcur, pbPrev, gs, active, src))
         return;
     double pd;
     if (winCap > 0 && sg.end.w <= winCap)  // uniform: one tile per workgroup
-        pd = upper_rows<M, LPR, U>(tl, rowptr, lower, rec, stage_segments<M, SEG>(src, sg), q);
+        pd = upper_rows<M, LPR, U>(tl, c2, rowptr, lower, rec, stage_segments<M, SEG>(src, sg), q);
     else
-        pd = upper_rows<M, LPR, U>(tl, rowptr, lower, rec, GlobalDir<M>{d}, q);
+        pd = upper_rows<M, LPR, U>(tl, c2, rowptr, lower, rec, GlobalDir<M>{d}, q);
     double u1 = 0.0, u2 = 0.0;
     block_sum3(pd, u1, u2);
     if (threadIdx.x == 0)
@@ -665,7 +681,8 @@ __global__ __launch_bounds__(NT) void k_spmv_upper(const int* kb, int k0, TileVi
                                                    const int2* __restrict__ lower, const M* __restrict__ rec,
                                                    const M* __restrict__ d, M* __restrict__ q,
                                                    double* __restrict__ pa, GroupState* gs,
-                                                   const StageSegs* __restrict__ win, int winCap, int fuse,
+                                                   const StageSegs* __restrict__ win, int winCap,
+                                                   const int2* __restrict__ chunk2, int fuse,
                                                    TileView vt, const M* __restrict__ z,
                                                    const M* __restrict__ dprev, M* dcur,
                                                    const Partial* __restrict__ pbPrev, int* active,
@@ -675,7 +692,8 @@ __global__ __launch_bounds__(NT) void k_spmv_upper(const int* kb, int k0, TileVi
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    spmv_body_upper<M, LPR, U, SEG>(k, st, rowptr, lower, rec, d, q, pa, gs, win, winCap, fuse, vt, z, dprev, dcur,
+    spmv_body_upper<M, LPR, U, SEG>(k, st, rowptr, lower, rec, d, q, pa, gs, win, winCap, chunk2, fuse, vt, z, dprev,
+                                    dcur,
                                     pbPrev, active);
     stamp_end(stamp);
 }
@@ -685,7 +703,8 @@ __global__ __launch_bounds__(NT) void k_spmv(const int* kb, int k0, TileView st,
                                              const int* __restrict__ fcol, const M* __restrict__ vals,
                                              long long nnzf, const M* __restrict__ d, M* __restrict__ q,
                                              double* __restrict__ pa, GroupState* gs,
-                                             const StageSegs* __restrict__ win, int winCap, int fuse, TileView vt,
+                                             const StageSegs* __restrict__ win, int winCap,
+                                             const int2* __restrict__ chunk2, int fuse, TileView vt,
                                              const M* __restrict__ z, const M* __restrict__ dprev, M* dcur,
                                              const Partial* __restrict__ pbPrev, int* active,
                                              unsigned long long* stamps)
@@ -694,7 +713,8 @@ __global__ __launch_bounds__(NT) void k_spmv(const int* kb, int k0, TileView st,
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    spmv_body<M, LPR, U, SEG>(k, st, fstart, fcol, vals, nnzf, d, q, pa, gs, win, winCap, fuse, vt, z, dprev, dcur,
+    spmv_body<M, LPR, U, SEG>(k, st, fstart, fcol, vals, nnzf, d, q, pa, gs, win, winCap, chunk2, fuse, vt, z, dprev,
+                              dcur,
                               pbPrev, active);
     stamp_end(stamp);
 }
@@ -917,7 +937,7 @@ static void launch_spmv_upper_shape(const DeviceView& v, const int* kb, int k0, 
     auto kernel = v.stageSegs ? k_spmv_upper<M, LPR, U, true> : k_spmv_upper<M, LPR, U, false>;
     hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds, v.stream, kb, k0,
                        v.spmv, v.rowptr, v.lower, static_cast<const M*>(v.vals), static_cast<const M*>(v.d[k0 & 1]),
-                       static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap, v.fuseDir, v.vec,
+                       static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap, v.spmvChunk2, v.fuseDir, v.vec,
                        static_cast<const M*>(v.z), static_cast<const M*>(v.d[(k0 + 1) & 1]),
                        static_cast<M*>(v.d[k0 & 1]), v.pb[(k0 + 1) & 1], v.active, stamp);
 }
@@ -932,7 +952,7 @@ static void launch_spmv_shape(const DeviceView& v, const int* kb, int k0, unsign
     hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds, v.stream, kb, k0,
                        v.spmv, v.fstart, v.fcol, static_cast<const M*>(v.vals), v.nnzf,
                        static_cast<const M*>(v.d[k0 & 1]), static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap,
-                       v.fuseDir, v.vec, static_cast<const M*>(v.z), static_cast<const M*>(v.d[(k0 + 1) & 1]),
+                       v.spmvChunk2, v.fuseDir, v.vec, static_cast<const M*>(v.z), static_cast<const M*>(v.d[(k0 + 1) & 1]),
                        static_cast<M*>(v.d[k0 & 1]), v.pb[(k0 + 1) & 1], v.active, stamp);
 }
 

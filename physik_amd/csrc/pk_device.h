@@ -356,6 +356,7 @@ struct DeviceView
     // tile whose segments hold at most winCap rows in all reads d from LDS
     // (winCap 0: never)
     const StageSegs* spmvWin;
+    const int2* spmvChunk2;  // pencil tiles: per SpMV tile a second run of rows (begin, count); null: none
     int winCap;
     int stageSegs;  // 0: every tile's staging is one window (segment 0 only)
     int fuseDir;    // every tile stages: the SpMV also does the direction update (no k_dir launch)

@@ -149,6 +149,13 @@ struct pk_system
     // SpMV staging (upper-only storage): the column window of every SpMV tile
     // and the rows of d a workgroup stages in LDS (0: no staging)
     pk::DevBuf<pk::StageSegs> spmvWin;
+    // pencil SpMV tiles (pk_system_set_spmv_pencils): per SpMV tile a second
+    // run of rows (begin, count) a layer after its first; host copies of the
+    // SpMV tiling for the staging set-up
+    bool pencils = false;
+    pk::DevBuf<int2> spmvChunk2;
+    std::vector<pk::RowTile> spmvHostTiles;
+    std::vector<int2> spmvHostChunk2;
     int winCap = 0;
     int stageSegs = 0;
     int fuseDir = 0;
@@ -333,6 +340,7 @@ struct pk_system
         dv.spmv = spmvTiles.view();
         dv.longSums = spmvTiles.longest >= 4 * pk::NT ? 1 : 0;
         dv.spmvWin = spmvWin.p;
+        dv.spmvChunk2 = pencils ? spmvChunk2.p : nullptr;
         dv.winCap = winCap;
         dv.stageSegs = stageSegs;
         dv.fuseDir = fuseDir;
@@ -367,6 +375,87 @@ static int build_tiles(pk_system* s, Tiles& tt, int rowsPerTile)
     if ((rc = tt.groupBegin.upload(gb, s->stream)) != PK_OK) return rc;
     if ((rc = tt.groupCount.upload(gc, s->stream)) != PK_OK) return rc;
     return PK_OK;
+}
+
+// Layer stride of rows [b, e) of one body: for each row, the centre of its
+// lower neighbours more than half its lower extent away (the layer below),
+// as an offset; the median over the rows. 0 when the rows have no such
+// neighbours.
+static int layer_stride(const Topology& t, int b, int e)
+{
+    std::vector<int> centres;
+    for (int r = b; r < e; ++r)
+    {
+        const int k0 = t.fstart[r], k1 = t.fstart[r + 1];
+        if (k1 == k0 || t.fcol[k0] >= r)
+            continue;
+        const int ext = r - t.fcol[k0];
+        if (ext < 2)
+            continue;
+        int lo = INT_MAX, hi = 0;
+        for (int k = k0; k < k1 && t.fcol[k] < r; ++k)
+        {
+            const int o = r - t.fcol[k];
+            if (2 * o > ext)
+            {
+                lo = std::min(lo, o);
+                hi = std::max(hi, o);
+            }
+        }
+        if (hi > 0)
+            centres.push_back((lo + hi) / 2);
+    }
+    if (centres.empty())
+        return 0;
+    std::nth_element(centres.begin(), centres.begin() + centres.size() / 2, centres.end());
+    return centres[centres.size() / 2];
+}
+
+// The SpMV tiling. Plain: runs of spmvRows consecutive rows. Pencils: where a
+// body's layer stride S is known, each tile pairs a run of spmvRows / 2 rows
+// with the run S rows later (blocks of 2 S rows split into such pairs), so a
+// row's mirror records in the layer below are read by the same workgroup in
+// the same pass; rows left over at a body's end get plain tiles.
+static int build_spmv_tiles(pk_system* s)
+{
+    const Topology& t = s->topo;
+    std::vector<RowTile>& tiles = s->spmvHostTiles;
+    std::vector<int2>& chunk2 = s->spmvHostChunk2;
+    tiles.clear();
+    chunk2.clear();
+    const int R = s->spmvRows, w = std::max(1, s->spmvRows / 2);
+    std::vector<int> gb(static_cast<size_t>(t.G), 0), gc(static_cast<size_t>(t.G), 0);
+    for (int g = 0; g < t.G; ++g)
+    {
+        gb[g] = static_cast<int>(tiles.size());
+        const int b = t.groupRowBegin[g], e = t.groupRowEnd[g];
+        int r = b;
+        const int S = s->pencils ? layer_stride(t, b, e) : 0;
+        if (S >= w)
+            for (; r + 2 * S <= e; r += 2 * S)
+                for (int j = r; j < r + S; j += w)
+                {
+                    const int n = std::min(w, r + S - j);
+                    tiles.push_back(RowTile{g, j, j + n, 0});
+                    chunk2.push_back(make_int2(j + S, n));
+                }
+        for (; r < e; r += R)
+        {
+            tiles.push_back(RowTile{g, r, std::min(r + R, e), 0});
+            chunk2.push_back(make_int2(0, 0));
+        }
+        gc[g] = static_cast<int>(tiles.size()) - gb[g];
+    }
+    Tiles& tt = s->spmvTiles;
+    tt.count = static_cast<int>(tiles.size());
+    tt.per = (tt.count + 7) / 8;
+    tt.longest = gc.empty() ? 0 : *std::max_element(gc.begin(), gc.end());
+    tt.hostBegin = gb;
+    int rc;
+    if ((rc = tt.tiles.upload(tiles, s->stream)) != PK_OK) return rc;
+    if ((rc = tt.groupBegin.upload(gb, s->stream)) != PK_OK) return rc;
+    if ((rc = tt.groupCount.upload(gc, s->stream)) != PK_OK) return rc;
+    return s->spmvChunk2.upload(chunk2, s->stream);
 }
 
 static int upload_groups(pk_system* s)
@@ -800,13 +889,17 @@ static int setup_windows(pk_system* s)
     std::vector<int> cols;
     std::vector<std::pair<int, int>> gaps;  // (gap rows, index of the column before it)
     int widestWin = 0, widestSeg = 0;
-    for (int g = 0; g < t.G; ++g)
-        for (int r0 = t.groupRowBegin[g]; r0 < t.groupRowEnd[g]; r0 += s->spmvRows)
+    for (size_t ti = 0; ti < s->spmvHostTiles.size(); ++ti)
         {
-            const int r1 = std::min(r0 + s->spmvRows, t.groupRowEnd[g]);
-            cols.assign(t.fcol.begin() + t.fstart[r0], t.fcol.begin() + t.fstart[r1]);
-            for (int r = r0; r < r1; ++r)
-                cols.push_back(r);
+            const pk::RowTile& tl = s->spmvHostTiles[ti];
+            const int2 c2 = s->spmvHostChunk2[ti];
+            cols.clear();
+            for (const int2 run : {make_int2(tl.rowBegin, tl.rowEnd), make_int2(c2.x, c2.x + c2.y)})
+            {
+                cols.insert(cols.end(), t.fcol.begin() + t.fstart[run.x], t.fcol.begin() + t.fstart[run.y]);
+                for (int r = run.x; r < run.y; ++r)
+                    cols.push_back(r);
+            }
             std::sort(cols.begin(), cols.end());
             cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
             const int len = cols.back() - cols.front() + 1;
@@ -846,12 +939,12 @@ static int setup_windows(pk_system* s)
     if (widestWin <= cap)
     {
         s->winCap = widestWin;
-        s->fuseDir = s->stagingMode == 2 ? 1 : 0;
+        s->fuseDir = s->stagingMode == 2 && !s->pencils ? 1 : 0;
         return s->spmvWin.upload(wins, s->stream);
     }
     s->stageSegs = 1;
     s->winCap = std::min(widestSeg, cap);
-    s->fuseDir = s->stagingMode == 2 && widestSeg <= cap ? 1 : 0;
+    s->fuseDir = s->stagingMode == 2 && widestSeg <= cap && !s->pencils ? 1 : 0;
     return s->spmvWin.upload(segs, s->stream);
 }
 
@@ -860,7 +953,7 @@ static int setup_tiles(pk_system* s)
     int rc;
     ++s->epoch;
     if ((rc = build_tiles(s, s->vecTiles, s->vecRows)) != PK_OK) return rc;
-    if ((rc = build_tiles(s, s->spmvTiles, s->spmvRows)) != PK_OK) return rc;
+    if ((rc = build_spmv_tiles(s)) != PK_OK) return rc;
     if ((rc = setup_windows(s)) != PK_OK) return rc;
     if ((rc = s->pa.alloc(std::max(1, s->spmvTiles.count))) != PK_OK) return rc;
     if ((rc = s->pb0.alloc(std::max(1, s->vecTiles.count))) != PK_OK) return rc;
@@ -1080,6 +1173,15 @@ int pk_system_set_spmv_staging(pk_system* s, int mode)
         return rc;
     PK_HIP(hipStreamSynchronize(s->stream));
     return PK_OK;
+}
+
+int pk_system_set_spmv_pencils(pk_system* s, int enable)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    s->pencils = enable != 0;
+    s->tapeValid = false;
+    return setup_tiles(s);
 }
 
 int pk_system_get_spmv_staging(const pk_system* s, int* mode, int* rows, int* segmented, int* fused)

@@ -76,6 +76,7 @@ PK_SIGNATURES = {
     "pk_system_get_tune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]),
     "pk_system_set_spmv_staging": (C.c_int, [_vp, C.c_int]),
+    "pk_system_set_spmv_pencils": (C.c_int, [_vp, C.c_int]),
     "pk_system_get_spmv_staging": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                              C.POINTER(C.c_int)]),
     "pk_system_set_state": (C.c_int, [_vp, _vp, _vp]),

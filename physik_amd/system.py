@@ -106,6 +106,10 @@ class System:
         fused into the SpMV where every tile stages (identical results)."""
         N.check(self.lib.pk_system_set_spmv_staging(self.h, int(mode)), "pk_system_set_spmv_staging")
 
+    def set_pencils(self, enable=True):
+        """Pencil SpMV tiles: a run of rows paired with the run a layer later."""
+        N.check(self.lib.pk_system_set_spmv_pencils(self.h, int(enable)), "pk_system_set_spmv_pencils")
+
     def get_staging(self):
         """(mode, rows a workgroup stages, segments in use, direction fused)."""
         m, r, g, f = C.c_int(), C.c_int(), C.c_int(), C.c_int()
