@@ -291,6 +291,13 @@ int pk_system_get_matrix(pk_system* sys, double* full_vals, double* rhs, double*
  * (tet_count x 12); zero for tets that contribute nothing. Any pointer may be
  * NULL. */
 int pk_system_get_elements(pk_system* sys, double* blocks, double* forces, double* kv);
+/* Per-tet rotation of the corotation at the current state (tet_count x 9,
+ * row-major R) and how ExtractRotationPolar (FEMModel.cpp:115-163) found it
+ * (tet_count): 0 the Newton limit, 1 the Newton limit with column 2 negated
+ * (det < 0), 2 the Identity fallback (non-finite or |det| <= 1e-8), -1 the tet
+ * contributes nothing (inactive, zero volume or non-finite F; R reads 0).
+ * Either pointer may be NULL. */
+int pk_system_get_rotations(pk_system* sys, double* rotations, int* kind);
 /* y = A x with the assembled full BSR (dynamic_rows x 3). */
 int pk_system_spmv(pk_system* sys, const double* x, double* y);
 /* Solution dv of the last solve (dynamic_rows x 3). */

@@ -60,6 +60,8 @@ struct ElementOut
     double K[10][9];     // R K_ab R^T for the stored pairs (row-major)
     double f[4][3];      // element forces (elastic + damping)
     double kv[4][3];     // sum_b (R K_ab R^T) v_b
+    double R[9];         // the rotation (inspection only)
+    int kind;            // how polar() found R: 0 Newton, 1 column 2 negated, 2 Identity
 };
 
 __device__ inline void element_compute(int t, size_t TT, const int4* __restrict__ tets,
@@ -105,7 +107,10 @@ __device__ inline void element_compute(int t, size_t TT, const int4* __restrict_
     if (!o.valid)
         return;  // the reference skips the tet this substep
     double R[9];
-    polar(F, R);
+    polar(F, R, &o.kind);
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+        o.R[e] = R[e];
     double g[4][3];
     shape_gradients(Dinv, g);
     // u_a = R^T (x_a - x_0) - (X_a - X_0); u_0 = 0
@@ -285,8 +290,9 @@ __device__ inline void element_block(const double* __restrict__ basis, const dou
                   prm[(P_GDOT + p) * TT + t], K);
 }
 
-// The element outputs of every tet as one record (10 blocks, 4 forces, 4 K v:
-// 114 doubles), for inspection; zero for tets that contribute nothing.
+// The element outputs of every tet as one record (10 blocks, 4 forces, 4 K v,
+// the rotation R and how polar() found it: EREC_COUNT doubles), for
+// inspection; zero (and kind -1) for tets that contribute nothing.
 __global__ __launch_bounds__(NT) void k_element_records(int T, const int4* __restrict__ tets,
                                                         const unsigned char* __restrict__ live,
                                                         const double* __restrict__ prm, const double* __restrict__ x,
@@ -299,16 +305,19 @@ __global__ __launch_bounds__(NT) void k_element_records(int T, const int4* __res
     o.valid = false;
     if (live[t])
         element_compute(t, static_cast<size_t>(T), tets, prm, x, v, o);
-    double* r = out + 114 * static_cast<size_t>(t);
+    double* r = out + EREC_COUNT * static_cast<size_t>(t);
     for (int p = 0; p < 10; ++p)
         for (int e = 0; e < 9; ++e)
-            r[9 * p + e] = o.valid ? o.K[p][e] : 0.0;
+            r[EREC_K + 9 * p + e] = o.valid ? o.K[p][e] : 0.0;
     for (int a = 0; a < 4; ++a)
         for (int i = 0; i < 3; ++i)
         {
-            r[90 + 3 * a + i] = o.valid ? o.f[a][i] : 0.0;
-            r[102 + 3 * a + i] = o.valid ? o.kv[a][i] : 0.0;
+            r[EREC_F + 3 * a + i] = o.valid ? o.f[a][i] : 0.0;
+            r[EREC_KV + 3 * a + i] = o.valid ? o.kv[a][i] : 0.0;
         }
+    for (int e = 0; e < 9; ++e)
+        r[EREC_R + e] = o.valid ? o.R[e] : 0.0;
+    r[EREC_KIND] = o.valid ? static_cast<double>(o.kind) : -1.0;
 }
 
 // Lanes per upper slot in the assembly gather: a node's star has ~20

@@ -2213,34 +2213,67 @@ int pk_system_get_matrix(pk_system* s, double* full_vals, double* rhs, double* m
     return PK_OK;
 }
 
-int pk_system_get_elements(pk_system* s, double* blocks, double* forces, double* kv)
+namespace
 {
-    if (!s)
-        return PK_ERR_ARG;
-    const Topology& t = s->topo;
+// Runs the inspection kernel and returns every tet's EREC_COUNT record.
+int element_records(pk_system* s, std::vector<double>& raw)
+{
+    const size_t T = static_cast<size_t>(s->topo.T);
+    raw.clear();
     PK_HIP(hipStreamSynchronize(s->stream));
-    const size_t T = static_cast<size_t>(t.T);
     if (T == 0)
         return PK_OK;
     DevBuf<double> tmp;
     int rc;
-    if ((rc = tmp.alloc(T * 114)) != PK_OK)
+    if ((rc = tmp.alloc(T * EREC_COUNT)) != PK_OK)
         return rc;
     launch_element_records(s->view(), tmp.p);
     PK_HIP(hipGetLastError());
     PK_HIP(hipStreamSynchronize(s->stream));
-    std::vector<double> raw(T * 114);
+    raw.resize(T * EREC_COUNT);
     PK_HIP(hipMemcpy(raw.data(), tmp.p, raw.size() * sizeof(double), hipMemcpyDeviceToHost));
+    return PK_OK;
+}
+}  // namespace
+
+int pk_system_get_elements(pk_system* s, double* blocks, double* forces, double* kv)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    std::vector<double> raw;
+    const int rc = element_records(s, raw);
+    if (rc != PK_OK)
+        return rc;
+    const size_t T = static_cast<size_t>(s->topo.T);
     for (size_t i = 0; i < T; ++i)
     {
-        const double* r = raw.data() + 114 * i;
+        const double* r = raw.data() + EREC_COUNT * i;
         for (int k = 0; k < 90; ++k)
-            if (blocks) blocks[90 * i + k] = r[k];
+            if (blocks) blocks[90 * i + k] = r[EREC_K + k];
         for (int k = 0; k < 12; ++k)
         {
-            if (forces) forces[12 * i + k] = r[90 + k];
-            if (kv) kv[12 * i + k] = r[102 + k];
+            if (forces) forces[12 * i + k] = r[EREC_F + k];
+            if (kv) kv[12 * i + k] = r[EREC_KV + k];
         }
+    }
+    return PK_OK;
+}
+
+int pk_system_get_rotations(pk_system* s, double* rotations, int* kind)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    std::vector<double> raw;
+    const int rc = element_records(s, raw);
+    if (rc != PK_OK)
+        return rc;
+    const size_t T = static_cast<size_t>(s->topo.T);
+    for (size_t i = 0; i < T; ++i)
+    {
+        const double* r = raw.data() + EREC_COUNT * i;
+        for (int k = 0; k < 9; ++k)
+            if (rotations) rotations[9 * i + k] = r[EREC_R + k];
+        if (kind) kind[i] = static_cast<int>(r[EREC_KIND]);
     }
     return PK_OK;
 }

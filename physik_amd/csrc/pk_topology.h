@@ -52,6 +52,18 @@ enum ElemLayout
     EP_ADJ_COUNT = 24
 };
 
+// Inspection record of one tet (pk_system_get_elements / get_rotations):
+// the 10 blocks (row-major), forces, K v, R (row-major) and the polar kind.
+enum ElemRecord
+{
+    EREC_K = 0,
+    EREC_F = 90,
+    EREC_KV = 102,
+    EREC_R = 114,
+    EREC_KIND = 123,
+    EREC_COUNT = 124
+};
+
 // Packed gather entry for one upper slot: tet << 5 | pair << 1 | transpose.
 inline int pack_contrib(int tet, int pair, int transpose) { return (tet << 5) | (pair << 1) | transpose; }
 // Packed rhs entry for one dynamic row: tet << 2 | corner.

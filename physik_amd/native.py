@@ -102,6 +102,7 @@ PK_SIGNATURES = {
     "pk_system_assemble": (C.c_int, [_vp, C.c_double]),
     "pk_system_get_matrix": (C.c_int, [_vp, _vp, _vp, _vp]),
     "pk_system_get_elements": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "pk_system_get_rotations": (C.c_int, [_vp, _vp, _vp]),
     "pk_system_spmv": (C.c_int, [_vp, _dp, _dp]),
     "pk_system_get_solution": (C.c_int, [_vp, _dp]),
     "pk_system_record": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),

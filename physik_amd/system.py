@@ -272,6 +272,14 @@ class System:
                 "pk_system_get_elements")
         return blocks, forces, kv
 
+    def rotations(self):
+        """(R per tet (t, 3, 3), polar kind per tet: 0 Newton, 1 column 2
+        negated, 2 Identity fallback, -1 no contribution)."""
+        R = np.zeros((self.t, 3, 3))
+        kind = np.zeros(self.t, np.int32)
+        N.check(self.lib.pk_system_get_rotations(self.h, N.ptr(R), N.ptr(kind)), "pk_system_get_rotations")
+        return R, kind
+
     def spmv(self, x):
         xa = N.as_f64(x).reshape(-1)
         y = np.zeros(self.nd * 3)
