@@ -38,6 +38,9 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
+from physik_amd.roofline import assembly_bytes as _assembly_bytes  # noqa: E402
+from physik_amd.roofline import precision_sizes, spmv_bytes, spmv_format_bytes  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0
 METRIC = "tet·steps/s fwd+adjoint batched vs plain step, 1 GPU, rtol 1e-6; SpMV % HBM"
 
@@ -66,43 +69,35 @@ DEFAULT_PRECISION = {"batch128": "fp32u", "twist64": "fp64u", "batch256": "fp64u
 DEFAULT_STEPS = {"batch128": 40, "diff32": 50, "twist64": 50, "beam10": 100, "batch256": 100}
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def workload(name):
     from physik_amd import workloads as W
     return {"twist64": W.twist64, "beam10": W.beam10, "batch256": W.batch256, "diff32": W.diff32,
             "batch128": W.batch128}[name]()
 
 
-def spmv_bytes(blocks, rows, body_launches, s, upper_blocks=None, fused=False):
-    """Algorithmic bytes of SpMV work q = A d.
-
-    Full storage: every multiplied 3x3 block once (9 values of s bytes +
-    int32 column), per row its int32 row pointer, one d entry read and one q
-    entry written (3 values each), and the closing row pointer of each body in
-    each launch. Upper-only storage (upper_blocks = the stored blocks among
-    them): every stored record once (10 words of s bytes: 9 values and the
-    column; it also serves the transposed product of its mirror row), an
-    8-byte (slot, column) entry per lower block, per row an 8-byte pair of row
-    pointers and d, q. With the direction update fused into the SpMV, a row
-    reads z and d_{k-1} and writes d_k instead of reading d_k (two more vector
-    passes). One launch over a whole system is
-    spmv_bytes(nnzb_full, nd, bodies, s[, nnzb_upper])."""
-    vec = (4 if fused else 2) * 3 * s
-    if upper_blocks is None:
-        return blocks * (9 * s + 4) + rows * (4 + vec) + 4 * body_launches
-    return upper_blocks * 10 * s + (blocks - upper_blocks) * 8 + rows * (8 + vec) + 8 * body_launches
-
-
 def measured_traffic(workload_name, precision):
-    """HBM bytes per SpMV launch from the committed counter passes
-    (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size) and
-    the algorithmic bytes per launch of that same profiled run, or None when
+    """HBM bytes per launch from the committed counter passes
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size): the
+    SpMV's bytes, the algorithmic bytes of that same profiled run and its
+    launch shape, and the assembly kernels' bytes per substep, or None when
     no pass covers this workload and precision."""
     path = os.path.join(ROOT, "profiles", "spmv_traffic.json")
     try:
         with open(path) as f:
             entry = json.load(f).get(f"{workload_name}/{precision}")
-        return None if entry is None else (float(entry["bytes"]), float(entry["algorithmic"]), entry.get("shape"))
-    except (OSError, ValueError, KeyError, TypeError):
+    except (OSError, ValueError):
+        return None
+    if entry is None:
+        return None
+    try:
+        return {"bytes": float(entry["bytes"]), "algorithmic": float(entry["algorithmic"]),
+                "shape": entry.get("shape"), "assembly": entry.get("assembly"), "source": entry.get("source")}
+    except (KeyError, TypeError, ValueError):
         return None
 
 
@@ -116,12 +111,16 @@ def roofline(s, name, prec_name):
     upper = s.precision in (N.PK_FP32_UPPER, N.PK_FP64_UPPER)
     blocks, ublocks, rows, body_launches = s.spmv_work()
     _, staged_rows, segmented, fused = s.get_staging()
-    total = spmv_bytes(blocks, rows, body_launches, esize, ublocks if upper else None, fused)
+    ub = ublocks if upper else None
+    total = spmv_bytes(blocks, rows, body_launches, esize, ub, fused)
+    fmt = spmv_format_bytes(blocks, rows, body_launches, esize, ub, fused)
     full = spmv_bytes(s.nnzb_full, s.nd, s.groups, esize, s.nnzb_upper if upper else None, fused)
     # the same work in full storage, for comparison across formats
     full_format = spmv_bytes(blocks, rows, body_launches, esize, None, fused)
-    achieved = total / (ms_spmv * 1e-3) / 1e9 if n_spmv and ms_spmv > 0 else 0.0
+    sec = ms_spmv * 1e-3
+    achieved = total / sec / 1e9 if n_spmv and ms_spmv > 0 else 0.0
     measured = measured_traffic(name, prec_name)
+    per = max(n_spmv, 1)
     return {
         "bound": "hbm",
         "kernel": "k_spmv (q = A d_k, d.q partial per tile; d_k = z_k + beta d_k-1 formed in it when fused)",
@@ -129,21 +128,63 @@ def roofline(s, name, prec_name):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None if measured is None else measured[0],
-        "traffic_vs_algorithmic": None if measured is None else measured[0] / measured[1],
-        "traffic_shape": None if measured is None else measured[2],  # CG launch shape of the counter passes
+        "traffic": None if measured is None else measured["bytes"],
+        "traffic_vs_algorithmic": None if measured is None else measured["bytes"] / measured["algorithmic"],
+        "traffic_shape": None if measured is None else measured["shape"],  # CG launch shape of the counter passes
         # the counters' bytes per launch over this run's average launch time:
         # the rate the kernel actually moves data at (a plain copy reaches ~5.3 TB/s)
         "traffic_rate_GBps": (None if measured is None or not n_spmv or ms_spmv <= 0
-                              else measured[0] / (ms_spmv / n_spmv * 1e-3) / 1e9),
+                              else measured["bytes"] / (ms_spmv / per * 1e-3) / 1e9),
         "storage": "upper triangle" if upper else "full",
         "staging": {"rows": staged_rows, "segments": segmented, "direction_fused": fused},
-        "bytes_per_launch": total / max(n_spmv, 1),
-        "full_storage_bytes_per_launch": full_format / max(n_spmv, 1),
+        "bytes_per_launch": total / per,
+        # what the storage format adds to the algorithmic bytes (upper
+        # storage: the mirror list, the second row pointer, FP64 column words)
+        "overhead_bytes_per_launch": (fmt - total) / per,
+        "format_rate_GBps": fmt / sec / 1e9 if n_spmv and ms_spmv > 0 else 0.0,
+        "full_storage_bytes_per_launch": full_format / per,
         "bytes_per_full_launch": full,
-        "active_fraction": total / (full * max(n_spmv, 1)),
-        "avg_launch_us": ms_spmv / max(n_spmv, 1) * 1e3,
+        "active_fraction": total / (full * per),
+        "avg_launch_us": ms_spmv / per * 1e3,
         "launches": n_spmv,
+    }
+
+
+def assembly_bytes(s, w):
+    """Algorithmic bytes of one assembly of system s (physik_amd.roofline)."""
+    sA, upper = precision_sizes(s.precision)
+    return _assembly_bytes(w.n_tets, w.n_nodes, s.nnzb_upper if upper else s.nnzb_full, s.nd, sA)
+
+
+def assembly_roofline(s, w, name, prec_name):
+    """Assembly roofline over the timed region: assembly_bytes per assembly
+    over the summed times of the element, slot-gather and right-hand-side
+    launches (HIP events on the system's stream)."""
+    from physik_amd import native as N
+    n_el, ms_el = s.kernel_time(N.K_ELEMENT)
+    _, ms_as = s.kernel_time(N.K_ASSEMBLE)
+    _, ms_rhs = s.kernel_time(N.K_RHS)
+    ms = ms_el + ms_as + ms_rhs
+    b = assembly_bytes(s, w)
+    achieved = b * n_el / (ms * 1e-3) / 1e9 if n_el and ms > 0 else 0.0
+    measured = measured_traffic(name, prec_name)
+    asm = None if measured is None else measured.get("assembly")
+    traffic = None if not asm else sum(float(asm[k]) for k in ("element", "assemble", "rhs"))
+    return {
+        "bound": "hbm",
+        "kernel": "k_element + k_assemble + k_rhs (one assembly: corotation, slot gather, right-hand side)",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "bytes_per_assembly": b,
+        "assemblies": n_el,
+        "ms_per_assembly": ms / max(n_el, 1),
+        "ms_per_assembly_by_kernel": {"element": ms_el / max(n_el, 1), "assemble": ms_as / max(n_el, 1),
+                                      "rhs": ms_rhs / max(n_el, 1)},
+        "traffic": traffic,
+        "traffic_by_kernel": asm,
+        "traffic_vs_algorithmic": None if traffic is None else traffic / b,
     }
 
 
@@ -195,7 +236,7 @@ def frame_export(s, reps=5):
             "float_export_GBps": 24 * s.n / (wait * 1e6)}
 
 
-def run_plain(name, prec_name, steps, warmup, args):
+def run_plain(name, prec_name, steps, warmup, args, export=True):
     """K plain implicit steps from the workload's initial state."""
     w, s, shape = prepare(name, prec_name, args)
     for _ in range(warmup):
@@ -219,10 +260,13 @@ def run_plain(name, prec_name, steps, warmup, args):
         "cg_launch_shape": shape,
         "cg_launch": args.cg_launch,
         "roofline": roofline(s, name, prec_name),
+        "assembly_roofline": assembly_roofline(s, w, name, prec_name),
         "kernels_ms": kernels_ms(s),
     }
     s.profile(False)
-    out["frame_export"] = frame_export(s)
+    if export:
+        out["frame_export"] = frame_export(s)
+    out["final_state"] = s.state()
     s.close()
     return out, w
 
@@ -265,9 +309,12 @@ def run_fwd_adj(name, prec_name, steps, warmup, args):
         "cg_launch_shape": shape,
         "cg_launch": args.cg_launch,
         "roofline": roofline(s, name, prec_name),
+        "assembly_roofline": assembly_roofline(s, w, name, prec_name),
         "kernels_ms": kernels_ms(s),
     }
     s.profile(False)
+    out["final_state"] = (x, s.state()[1])
+    out["gradients"] = (gE, gv)
     s.close()
     return out, w
 
@@ -367,6 +414,141 @@ def cpu_reference_forward(w, name, steps, budget_s):
     }
 
 
+def cpu_info():
+    """Host CPU model, logical CPUs this process may use, and nproc."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "nproc": usable, "os_cpu_count": os.cpu_count()}
+
+
+def oracle_error(w, head, budget_s):
+    """FP64-oracle error of the GPU run per body, on the smallest bodies
+    that fit the budget (at least one): relative L2 of the final x and v
+    after the run's K substeps, and, for a differentiable workload, of
+    dL/dlog E and dL/dv0 against the oracle adjoint seeded from the oracle's
+    own final state."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import checkers
+    from physik_amd import workloads as W
+
+    def rel(a, b, floor=0.0):
+        nb = max(np.linalg.norm(b), floor)
+        return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+    x, v = head["final_state"]
+    grads = head.get("gradients")
+    steps = head["steps"]
+    order = sorted(range(w.n_bodies), key=lambda b: w.body_tets[b][1] - w.body_tets[b][0]) if w.n_bodies > 1 \
+        else [0]
+    rows, t0 = [], time.perf_counter()
+    for b in order:
+        wb = w.body(b) if w.n_bodies > 1 else w
+        n0, n1 = w.body_nodes[b]
+        t_0, t_1 = w.body_tets[b]
+        o = _oracle(checkers, wb)
+        for _ in range(steps):
+            o.step(wb.dt)
+        xo, vo = o.state()
+        o.close()
+        # v relative to the larger of the final and initial velocity norms: a
+        # body that has come to rest keeps v near 0, where a plain relative
+        # error measures the CG tolerance only
+        row = {"body": b, "tets": t_1 - t_0, "x": rel(x[n0:n1], xo),
+               "v": rel(v[n0:n1], vo, np.linalg.norm(wb.v0))}
+        if grads is not None:
+            a = _oracle(checkers, wb)
+            oE, _, ov, _ = a.adjoint(wb.dt, steps, W.loss_and_seed(wb, xo)[1], None, tol=wb.adj_tol,
+                                     max_iter=wb.max_iter)
+            a.close()
+            row["grad_log_young"] = rel(grads[0][t_0:t_1], oE)
+            row["grad_v0"] = rel(grads[1][n0:n1], ov)
+        rows.append(row)
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    keys = [k for k in ("x", "v", "grad_log_young", "grad_v0") if k in rows[0]]
+    return {
+        "reference": "FP64 oracle restatement of the reference step (and its adjoint), same inputs, "
+                     f"{steps} substeps, forward rtol {w.tol:g}" + (f", adjoint rtol {w.adj_tol:g}" if grads else ""),
+        "sample": f"the {len(rows)} smallest of {w.n_bodies} bodies" if w.n_bodies > 1 else "the whole body",
+        "metric": "relative L2 per body (v: over max(|v_S|, |v_0|) of the oracle)",
+        **{k: {"max": max(r[k] for r in rows), "median": float(np.median([r[k] for r in rows]))} for k in keys},
+        "per_body": rows,
+        "seconds": time.perf_counter() - t0,
+    }
+
+
+def _ref_worker(job):
+    """One worker of the process-parallel CPU figure: the reference's FP32
+    forward over its share of the bodies, `steps` substeps each, timing only
+    the steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import checkers
+    el, tet_steps = 0.0, 0
+    for body in job["bodies"]:
+        rest, tets, young, poisson, damping, density, fixed, x0, v0, dt, tol, max_iter, gravity = body
+        if checkers.reference_available():
+            sim = checkers.Reference(rest, tets, float(density[0]), float(young[0]), float(poisson[0]),
+                                     float(damping[0]))
+            if not (young == young[0]).all():
+                sim.set_young(young)
+        else:
+            sim = checkers.Oracle(32, rest, tets, young, poisson, damping, density)
+        sim.set_params(gravity, 1, tol, max_iter)
+        sim.set_fixed(fixed)
+        sim.set_state(x0, v0)
+        t0 = time.perf_counter()
+        for _ in range(job["steps"]):
+            sim.step(dt)
+        el += time.perf_counter() - t0
+        tet_steps += len(tets) * job["steps"]
+        sim.close()
+    return el, tet_steps
+
+
+def cpu_parallel_upper_bound(w, name, steps, workers):
+    """Every body of the batch as its own World on `workers` host processes
+    (bodies spread by longest-processing-time on tet count): total
+    tet-steps over the slowest worker's step time. An upper bound for a
+    host engine that runs one reference World per body per core, not the
+    reference itself (which has no threading)."""
+    import multiprocessing as mp
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import checkers
+    loads = [0] * workers
+    jobs = [{"bodies": [], "steps": steps} for _ in range(workers)]
+    for b in sorted(range(w.n_bodies), key=lambda b: -(w.body_tets[b][1] - w.body_tets[b][0])):
+        wb = w.body(b)
+        k = loads.index(min(loads))
+        loads[k] += wb.n_tets
+        jobs[k]["bodies"].append((wb.rest, wb.tets, wb.young, wb.poisson, wb.damping, wb.density, wb.fixed, wb.x0,
+                                  wb.v0, wb.dt, wb.tol, wb.max_iter, wb.gravity))
+    ctx = mp.get_context("spawn")  # no fork of a process that holds the GPU
+    with ctx.Pool(workers) as pool:
+        res = pool.map(_ref_worker, jobs)
+    slowest = max(el for el, _ in res)
+    total = sum(ts for _, ts in res)
+    return {
+        "value": total / max(slowest, 1e-9),
+        "unit": "tet·steps/s",
+        "cores": workers,
+        "kind": "reference" if checkers.reference_available() else "port",
+        "label": "upper bound, not the reference: one World per body, bodies spread over host processes",
+        "sample": f"all {w.n_bodies} bodies of {name}, {steps} substeps each from the initial state, FP32 forward, "
+                  f"{workers} processes, slowest {slowest:.1f} s of step time; setup excluded",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,6 +566,10 @@ def main():
     ap.add_argument("--shape", default=None, help="CG launch shape lanes,spmv_rows,vector_rows,unroll (no autotune)")
     ap.add_argument("--cg-launch", default="kernels", choices=["kernels", "graphs"],
                     help="issue each CG launch from the host, or replay captured 4-iteration HIP graphs")
+    ap.add_argument("--no-oracle-error", action="store_true", help="skip the per-body FP64-oracle error")
+    ap.add_argument("--oracle-budget", type=float, default=10.0, help="seconds of oracle work for the error sample")
+    ap.add_argument("--no-batch-forward", action="store_true", help="skip the batch256 forward sub-record")
+    ap.add_argument("--no-cpu-parallel", action="store_true", help="skip the process-parallel CPU upper bound")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the batched and single-body "
@@ -394,7 +580,9 @@ def main():
     prec_name = args.precision or DEFAULT_PRECISION.get(name, "fp64")
     steps = args.steps or DEFAULT_STEPS[name]
     diff = name in DIFFERENTIABLE
+    progress(f"headline: {name} {prec_name}, {steps} steps")
     head, w = (run_fwd_adj if diff else run_plain)(name, prec_name, steps, args.warmup, args)
+    internal = ("final_state", "gradients")
     line = {
         "metric": METRIC,
         "value": head["value"],
@@ -412,26 +600,58 @@ def main():
         "config": {"workload": head["workload"], "mode": "forward+adjoint" if diff else "forward",
                    "parallelism": "single GPU",
                    **{k: v for k, v in head.items() if k not in ("workload", "value", "unit", "roofline",
-                                                                 "kernels_ms", "ms_per_step", "steps")}},
+                                                                 "assembly_roofline", "kernels_ms", "ms_per_step",
+                                                                 "steps", *internal)}},
         "roofline": head["roofline"],
+        "assembly_roofline": head["assembly_roofline"],
         "kernels_ms": head["kernels_ms"],
     }
+    if not args.no_oracle_error:
+        progress("oracle error of the headline run")
+        line["oracle_error"] = oracle_error(w, head, args.oracle_budget)
     if diff and prec_name == "fp32u" and not args.no_full_storage:
         # the same run with the full matrix stored (identical results): its
         # SpMV moves the whole matrix, the headline's reads the upper half
+        progress("full-storage run")
         full, _ = run_fwd_adj(name, "fp32", steps, args.warmup, args)
         line["full_storage_step"] = {k: full[k] for k in ("value", "ms_per_step", "precision", "cg_launch_shape",
-                                                          "roofline", "kernels_ms")}
+                                                          "roofline", "assembly_roofline", "kernels_ms")}
     if not args.no_plain and name != "twist64":
+        progress("plain step: twist64")
         plain, _ = run_plain("twist64", DEFAULT_PRECISION["twist64"], DEFAULT_STEPS["twist64"], args.warmup, args)
+        for k in internal:
+            plain.pop(k, None)
         line["plain_step"] = plain
         line["vs_plain_step"] = head["value"] / plain["value"]
+    if not args.no_batch_forward and name != "batch256":
+        # the batched forward workload (256 bodies, FP64), launched kernel by
+        # kernel and replayed from captured HIP graphs
+        progress("batched forward: batch256, kernels")
+        bf, wbf = run_plain("batch256", DEFAULT_PRECISION["batch256"], DEFAULT_STEPS["batch256"], args.warmup, args,
+                            export=False)
+        rec = {k: v for k, v in bf.items() if k not in internal}
+        if not args.no_oracle_error:
+            rec["oracle_error"] = oracle_error(wbf, bf, args.oracle_budget / 2)
+        gargs = argparse.Namespace(**{**vars(args), "cg_launch": "graphs"})
+        progress("batched forward: batch256, graphs")
+        bg, _ = run_plain("batch256", DEFAULT_PRECISION["batch256"], DEFAULT_STEPS["batch256"], args.warmup, gargs,
+                          export=False)
+        rec["graphs"] = {k: bg[k] for k in ("value", "ms_per_step", "cg_launch", "roofline", "kernels_ms")}
+        line["batch_forward"] = rec
     if not args.no_cpu_baseline:
+        progress("CPU baselines")
+        info = cpu_info()
         if diff:
             line["cpu_baseline"] = cpu_fwd_adj(w, name, steps, args.cpu_budget)
             line["cpu_reference_forward"] = cpu_reference_forward(w, name, steps, args.cpu_budget / 2)
         else:
             line["cpu_baseline"] = cpu_reference_forward(w, name, steps, args.cpu_budget)
+        line["cpu_baseline"].update({"cpu_model": info["model"], "nproc": info["nproc"]})
+        if w.n_bodies > 1 and not args.no_cpu_parallel:
+            workers = max(1, min(16, info["nproc"] or 1))
+            progress(f"CPU process-parallel figure: {workers} processes")
+            line["cpu_parallel_upper_bound"] = cpu_parallel_upper_bound(w, name, min(steps, 2), workers)
+            line["cpu_parallel_upper_bound"].update({"cpu_model": info["model"], "nproc": info["nproc"]})
     print(json.dumps(line), flush=True)
 
 

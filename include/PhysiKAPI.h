@@ -135,6 +135,12 @@ extern "C"
     PHYSIK_API int PHYSIK_GetTetMeshGlobalNodeIndex(PhysiK::WorldHandle world,             /* :149 */
                                                     PhysiK::ComponentHandle tetMeshPhysicsHandle,
                                                     int localNodeIndex);
+    PHYSIK_API int PHYSIK_SetTetMeshLocalCurrentPosition(PhysiK::WorldHandle world,        /* :236 */
+                                                         PhysiK::ComponentHandle tetMesh, int localNodeIndex,
+                                                         float x, float y, float z);
+    PHYSIK_API int PHYSIK_GetTetMeshLocalCurrentPosition(PhysiK::WorldHandle world,        /* :244 */
+                                                         PhysiK::ComponentHandle tetMesh, int localNodeIndex,
+                                                         float* outX, float* outY, float* outZ);
     PHYSIK_API void PHYSIK_SetTetMeshMaterial(PhysiK::WorldHandle world,                   /* :154 */
                                               PhysiK::ComponentHandle component,
                                               const PhysikMaterialDesc* material);

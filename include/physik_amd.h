@@ -161,6 +161,11 @@ int pk_system_set_spmv_pencils(pk_system* sys, int enable);
 /* Node state, node_count x 3 each; either pointer may be NULL. */
 int pk_system_set_state(pk_system* sys, const double* x, const double* v);
 int pk_system_get_state(pk_system* sys, double* x, double* v);
+/* The same with device pointers on the system's device (no host round trip):
+ * the caller's writes of x and v must be complete before the call (e.g. its
+ * stream synchronised); the copies are complete when the call returns. */
+int pk_system_set_state_device(pk_system* sys, const double* x, const double* v);
+int pk_system_get_state_device(pk_system* sys, double* x, double* v);
 
 /* Frame export: the readback after a step for the reference's per-frame
  * readers (visual components in World::PostUpdate, World.cpp:539-549, and a
@@ -329,13 +334,18 @@ int pk_system_get_solution(pk_system* sys, double* dv);
 int pk_system_record(pk_system* sys, double dt, int steps, int every);
 int pk_system_backward(pk_system* sys, const double* xbar, const double* vbar, double adj_tol, int adj_max_iter,
                        double* grad_log_young, double* grad_x0, double* grad_v0, int* adj_iterations);
-
-/* ---- diagnostics ---- */
-
-/* Copies `bytes` (a multiple of 32) between two fresh device buffers `reps`
- * times with a plain streaming kernel and returns the average time per copy.
- * Used to calibrate the HBM byte counters of the profiler. */
-int pk_diag_copy(long long bytes, int reps, double* avg_ms);
+/* pk_system_backward with xbar, vbar and the three gradient arrays as device
+ * pointers on the system's device (adj_iterations stays a host pointer);
+ * results are identical. Same completion rules as pk_system_set_state_device. */
+int pk_system_backward_device(pk_system* sys, const double* xbar, const double* vbar, double adj_tol,
+                              int adj_max_iter, double* grad_log_young, double* grad_x0, double* grad_v0,
+                              int* adj_iterations);
+/* Identifies the recorded run pk_system_backward would reverse: a count of
+ * this system's successful pk_system_record calls (so a later recording has
+ * a larger id), or 0 when no valid recording exists (none made, or discarded
+ * by a later change). A caller holding an id can detect that the system has
+ * recorded another run since. */
+int pk_system_tape_id(const pk_system* sys, long long* id);
 
 /* ---- host-only topology (no GPU needed) ---- */
 

@@ -19,8 +19,10 @@ ROOT = os.path.dirname(HERE)
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libphysik_amd.so")
 SOURCES = ["csrc/pk_assembly.hip", "csrc/pk_cg.hip", "csrc/pk_system.cpp", "csrc/pk_topology.cpp",
-           "csrc/physik_api.cpp", "csrc/pk_diag.hip",
-           "csrc/pk_adjoint.hip"]
+           "csrc/physik_api.cpp", "csrc/pk_adjoint.hip"]
+# the counter-calibration copy kernel of physik_amd/diag.py: a library of its own
+DIAG_LIB = os.path.join(LIB_DIR, "libphysik_diag.so")
+DIAG_SOURCES = ["csrc/pk_diag.hip"]
 ARCH = os.environ.get("PHYSIK_AMD_ARCH", "gfx950")
 
 
@@ -31,30 +33,37 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (set HIPCC)")
 
 
-def command(out: str = LIB) -> list[str]:
+def command(out: str = LIB, sources=None) -> list[str]:
     return [
         hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
         # packed FP32 ops are kept off for all device code
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-fvisibility=hidden",
         "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
-        *[os.path.join(HERE, s) for s in SOURCES],
+        *[os.path.join(HERE, s) for s in (SOURCES if sources is None else sources)],
         "-o", out,
     ]
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(LIB_DIR, exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = command(tmp)
+def _compile(out: str, sources, verbose: bool) -> str:
+    tmp = out + ".tmp"
+    cmd = command(tmp, sources)
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError(f"hipcc failed with status {res.returncode}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
+
+
+def build(verbose: bool = False) -> str:
+    """Builds the product library and the diagnostics library; returns the
+    product library's path."""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    _compile(DIAG_LIB, DIAG_SOURCES, verbose)
+    return _compile(LIB, SOURCES, verbose)
 
 
 if __name__ == "__main__":

@@ -152,6 +152,15 @@ struct Component
     std::vector<double> young;  // per tet
     Material material;
     int tetBase = -1;  // index of the component's first tet in the device system
+    // the component's own copy of its current node positions
+    // (TetMeshComponent nodePositions: the rest positions at creation,
+    // PHYSIK_SetTetMeshLocalCurrentPosition writes it); a physics component
+    // copies the world's positions into it in every substep and after a step
+    // with dt > 0 (TetMeshPhysicsComponent.cpp:234-252, 299, 356-362).
+    // currentFromWorld: the copy equals the world's present positions and is
+    // read from there until the world state changes on the host.
+    std::vector<float> current;
+    bool currentFromWorld = false;
     // a collision sphere instead of a tet mesh (CollisionSphereComponent.h,
     // CollisionComponent.h:10-28): centre, radius clamped at 0, connection
     // settings (defaults 1000 and 10), and a kinematic target taken at the
@@ -252,7 +261,37 @@ struct World
     void touch_host()
     {
         pull();
+        materialize_current();
         hostNewer = true;
+    }
+    // float world position of node g, as PHYSIK_GetNodePosition returns it
+    void node_position(int g, float* out)
+    {
+        if (const float* f = frame(PK_EXPORT_POSITIONS))
+        {
+            std::copy(f + 3 * g, f + 3 * g + 3, out);
+            return;
+        }
+        pull();
+        for (int k = 0; k < 3; ++k)
+            out[k] = static_cast<float>(x[3 * g + k]);
+    }
+    // The copies that still read the world take its positions now, before
+    // the host changes the world state.
+    void materialize_current(Component& c)
+    {
+        if (!c.currentFromWorld)
+            return;
+        c.currentFromWorld = false;
+        for (int i = 0; i < c.globalCount && 3 * i + 2 < static_cast<int>(c.current.size()); ++i)
+            if (c.globalBegin + i < nodeCount())
+                node_position(c.globalBegin + i, c.current.data() + 3 * i);
+    }
+    void materialize_current()
+    {
+        for (auto& cp : components)
+            if (cp)
+                materialize_current(*cp);
     }
     int add_node(double px, double py, double pz)
     {
@@ -451,6 +490,12 @@ struct World
             conns.clear();  // World.cpp:41-46
             return PK_OK;
         }
+        // every active physics component copies the world's positions in its
+        // first UpdateSystem and again in PostUpdate: from here on its copy
+        // is the world's state (also when the step throws after the copy)
+        for (auto& cp : components)
+            if (cp && cp->physics && cp->active)
+                cp->currentFromWorld = true;
         if (!topoDirty && sys && !pendingTets.empty())
         {
             // tets switched since the last step, in place when the system can
@@ -679,6 +724,7 @@ std::unique_ptr<Component> component_from(const GeneratedMesh& m)  // TetMeshCom
         c->tets.insert(c->tets.end(), m.tets.begin() + t, m.tets.begin() + t + 4);
     }
     c->tetActive.assign(c->tets.size() / 4, 1);
+    c->current.assign(m.positions.begin(), m.positions.end());
     return c;
 }
 
@@ -919,6 +965,38 @@ PhysiK::ComponentHandle PHYSIK_CreateTetMeshPhysicsComponent(PhysiK::WorldHandle
         w->add_node(c->rest[3 * i], c->rest[3 * i + 1], c->rest[3 * i + 2]);
     w->topoDirty = true;
     return w->add(std::move(c));
+}
+
+int PHYSIK_SetTetMeshLocalCurrentPosition(PhysiK::WorldHandle world, PhysiK::ComponentHandle h, int local, float x,
+                                          float y, float z)  // PhysiKAPI.cpp:992-1016
+{
+    World* w = W(world);
+    Component* c = tet_mesh(w, h);
+    if (!c || local < 0 || 3 * local + 2 >= static_cast<int>(c->current.size()))
+        return 0;
+    w->materialize_current(*c);
+    c->current[3 * local] = x;
+    c->current[3 * local + 1] = y;
+    c->current[3 * local + 2] = z;
+    return 1;
+}
+
+int PHYSIK_GetTetMeshLocalCurrentPosition(PhysiK::WorldHandle world, PhysiK::ComponentHandle h, int local,
+                                          float* outX, float* outY, float* outZ)  // PhysiKAPI.cpp:1018-1053
+{
+    World* w = W(world);
+    if (!w || !outX || !outY || !outZ)
+        return 0;
+    Component* c = tet_mesh(w, h);
+    if (!c || local < 0 || 3 * local + 2 >= static_cast<int>(c->current.size()))
+        return 0;
+    float p[3] = {c->current[3 * local], c->current[3 * local + 1], c->current[3 * local + 2]};
+    if (c->currentFromWorld && local < c->globalCount && c->globalBegin + local < w->nodeCount())
+        w->node_position(c->globalBegin + local, p);
+    *outX = p[0];
+    *outY = p[1];
+    *outZ = p[2];
+    return 1;
 }
 
 int PHYSIK_GetTetMeshGlobalNodeBeginIndex(PhysiK::WorldHandle world, PhysiK::ComponentHandle h)

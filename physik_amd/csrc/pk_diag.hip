@@ -1,15 +1,20 @@
-// Diagnostics: a plain streaming copy kernel of known size, used to calibrate
-// HBM byte counters (FETCH_SIZE, WRITE_SIZE) against the bytes a kernel must
-// move.
+// Diagnostics library (libphysik_diag.so, not part of the product library):
+// a plain streaming copy kernel of known size, used to calibrate the HBM byte
+// counters (FETCH_SIZE, WRITE_SIZE) against the bytes a kernel must move.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 
-#include "physik_amd.h"
+#include "pk_diag.h"
 
 namespace pk
 {
-void set_error(const std::string& msg);
+namespace
+{
+thread_local std::string g_diag_error;
+void set_error(const std::string& msg) { g_diag_error = msg; }
+}  // namespace
 
 __global__ __launch_bounds__(256) void k_copy(const double4* __restrict__ src, double4* __restrict__ dst, size_t n)
 {
@@ -18,13 +23,15 @@ __global__ __launch_bounds__(256) void k_copy(const double4* __restrict__ src, d
 }
 }  // namespace pk
 
+extern "C" const char* pk_diag_last_error(void) { return pk::g_diag_error.c_str(); }
+
 extern "C" int pk_diag_copy(long long bytes, int reps, double* avg_ms)
 {
     using namespace pk;
     if (bytes <= 0 || reps <= 0 || bytes % 32)
     {
         set_error("pk_diag_copy: bytes must be a positive multiple of 32");
-        return PK_ERR_ARG;
+        return 1;
     }
     const size_t n = static_cast<size_t>(bytes) / 32;
     double4 *a = nullptr, *b = nullptr;
@@ -33,7 +40,7 @@ extern "C" int pk_diag_copy(long long bytes, int reps, double* avg_ms)
     {
         if (a) (void)hipFree(a);
         set_error("pk_diag_copy: allocation failed");
-        return PK_ERR_HIP;
+        return 2;
     }
     (void)hipMemset(a, 0, bytes);
     (void)hipEventCreate(&e0);
@@ -53,5 +60,5 @@ extern "C" int pk_diag_copy(long long bytes, int reps, double* avg_ms)
     (void)hipEventDestroy(e1);
     (void)hipFree(a);
     (void)hipFree(b);
-    return hipGetLastError() == hipSuccess ? PK_OK : PK_ERR_HIP;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
 }

@@ -190,6 +190,7 @@ struct pk_system
     double tapeDt = 0.0;
     int tapeSteps = 0, tapeEvery = 0;
     bool tapeValid = false;
+    long long tapeId = 0;  // counts the recorded runs of this system
     int lastItersAdj = 0;
 
     double gravity[3] = {0, 0, 0};
@@ -1206,30 +1207,62 @@ int pk_system_get_tune(const pk_system* s, int* lanes_per_row, int* spmv_rows, i
     return PK_OK;
 }
 
-int pk_system_set_state(pk_system* s, const double* x, const double* v)
+}  // extern "C"
+
+namespace pk
 {
-    if (!s)
-        return PK_ERR_ARG;
+// x and v in or out, host or device pointers (kind H2D/D2H or D2D).
+static int copy_state_in(pk_system* s, const double* x, const double* v, hipMemcpyKind kind)
+{
     ++s->stateVer;
     const size_t bytes = static_cast<size_t>(s->topo.N) * 3 * sizeof(double);
     if (x && bytes)
-        PK_HIP(hipMemcpyAsync(s->x.p, x, bytes, hipMemcpyHostToDevice, s->stream));
+        PK_HIP(hipMemcpyAsync(s->x.p, x, bytes, kind, s->stream));
     if (v && bytes)
-        PK_HIP(hipMemcpyAsync(s->v.p, v, bytes, hipMemcpyHostToDevice, s->stream));
+        PK_HIP(hipMemcpyAsync(s->v.p, v, bytes, kind, s->stream));
     PK_HIP(hipStreamSynchronize(s->stream));
     return PK_OK;
 }
 
-int pk_system_get_state(pk_system* s, double* x, double* v)
+static int copy_state_out(pk_system* s, double* x, double* v, hipMemcpyKind kind)
 {
-    if (!s)
-        return PK_ERR_ARG;
     const size_t bytes = static_cast<size_t>(s->topo.N) * 3 * sizeof(double);
     if (x && bytes)
-        PK_HIP(hipMemcpyAsync(x, s->x.p, bytes, hipMemcpyDeviceToHost, s->stream));
+        PK_HIP(hipMemcpyAsync(x, s->x.p, bytes, kind, s->stream));
     if (v && bytes)
-        PK_HIP(hipMemcpyAsync(v, s->v.p, bytes, hipMemcpyDeviceToHost, s->stream));
+        PK_HIP(hipMemcpyAsync(v, s->v.p, bytes, kind, s->stream));
     PK_HIP(hipStreamSynchronize(s->stream));
+    return PK_OK;
+}
+}  // namespace pk
+
+extern "C" {
+
+int pk_system_set_state(pk_system* s, const double* x, const double* v)
+{
+    return s ? copy_state_in(s, x, v, hipMemcpyHostToDevice) : PK_ERR_ARG;
+}
+
+int pk_system_get_state(pk_system* s, double* x, double* v)
+{
+    return s ? copy_state_out(s, x, v, hipMemcpyDeviceToHost) : PK_ERR_ARG;
+}
+
+int pk_system_set_state_device(pk_system* s, const double* x, const double* v)
+{
+    return s ? copy_state_in(s, x, v, hipMemcpyDeviceToDevice) : PK_ERR_ARG;
+}
+
+int pk_system_get_state_device(pk_system* s, double* x, double* v)
+{
+    return s ? copy_state_out(s, x, v, hipMemcpyDeviceToDevice) : PK_ERR_ARG;
+}
+
+int pk_system_tape_id(const pk_system* s, long long* id)
+{
+    if (!s || !id)
+        return PK_ERR_ARG;
+    *id = s->tapeValid ? s->tapeId : 0;
     return PK_OK;
 }
 
@@ -1471,10 +1504,10 @@ static int record_run(pk_system* s, double dt, int steps, int every)
     const size_t n3 = static_cast<size_t>(s->topo.N) * 3;
     const int nck = steps / every + 1;
     int rc;
-    if ((rc = s->tapeX.alloc(nck * n3)) != PK_OK) return rc;
-    if ((rc = s->tapeV.alloc(nck * n3)) != PK_OK) return rc;
-    if ((rc = s->segOk.alloc(static_cast<size_t>(every) * std::max(1, s->topo.G))) != PK_OK) return rc;
     s->tapeValid = false;
+    if ((rc = s->tapeX.ensure(nck * n3)) != PK_OK) return rc;
+    if ((rc = s->tapeV.ensure(nck * n3)) != PK_OK) return rc;
+    if ((rc = s->segOk.ensure(static_cast<size_t>(every) * std::max(1, s->topo.G))) != PK_OK) return rc;
     if ((rc = dcopy(s, s->tapeX.p, s->x.p, n3)) != PK_OK) return rc;
     if ((rc = dcopy(s, s->tapeV.p, s->v.p, n3)) != PK_OK) return rc;
     for (int n = 0; n < steps; ++n)
@@ -1493,6 +1526,7 @@ static int record_run(pk_system* s, double dt, int steps, int every)
     s->tapeSteps = steps;
     s->tapeEvery = every;
     s->tapeValid = true;
+    ++s->tapeId;
     return PK_OK;
 }
 
@@ -1620,8 +1654,12 @@ int pk_system_record(pk_system* s, double dt, int steps, int every)
                                            : record_run<double>(s, dt, steps, every);
 }
 
-int pk_system_backward(pk_system* s, const double* xbar, const double* vbar, double adj_tol, int adj_max_iter,
-                       double* grad_log_young, double* grad_x0, double* grad_v0, int* adj_iterations)
+}  // extern "C"
+
+namespace pk
+{
+static int backward_entry(pk_system* s, const double* xbar, const double* vbar, double adj_tol, int adj_max_iter,
+                          double* grad_log_young, double* grad_x0, double* grad_v0, int* adj_iterations, bool device)
 {
     if (!s)
         return PK_ERR_ARG;
@@ -1630,13 +1668,15 @@ int pk_system_backward(pk_system* s, const double* xbar, const double* vbar, dou
         set_error("pk_system_backward: no recorded run (call pk_system_record after the last change)");
         return PK_ERR_ARG;
     }
+    const hipMemcpyKind in = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind out = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     ++s->stateVer;  // the replays write x and v (the end state is restored, bit for bit)
     const Topology& t = s->topo;
     const size_t n3 = static_cast<size_t>(t.N) * 3;
     int rc;
-    if ((rc = s->xbar.alloc(n3)) != PK_OK) return rc;
-    if ((rc = s->vbar.alloc(n3)) != PK_OK) return rc;
-    if ((rc = s->gradE.alloc(static_cast<size_t>(t.T))) != PK_OK) return rc;
+    if ((rc = s->xbar.ensure(n3)) != PK_OK) return rc;
+    if ((rc = s->vbar.ensure(n3)) != PK_OK) return rc;
+    if ((rc = s->gradE.ensure(static_cast<size_t>(t.T))) != PK_OK) return rc;
     if (s->adj.n < static_cast<size_t>(t.T) * EP_ADJ_COUNT)
     {
         if ((rc = s->adj.alloc(static_cast<size_t>(t.T) * EP_ADJ_COUNT)) != PK_OK) return rc;
@@ -1645,29 +1685,47 @@ int pk_system_backward(pk_system* s, const double* xbar, const double* vbar, dou
         PK_HIP(hipMemsetAsync(s->adj.p, 0, s->adj.n * sizeof(double), s->stream));
     }
     if (xbar)
-        PK_HIP(hipMemcpyAsync(s->xbar.p, xbar, n3 * sizeof(double), hipMemcpyHostToDevice, s->stream));
+        PK_HIP(hipMemcpyAsync(s->xbar.p, xbar, n3 * sizeof(double), in, s->stream));
     else
         PK_HIP(hipMemsetAsync(s->xbar.p, 0, n3 * sizeof(double), s->stream));
     if (vbar)
-        PK_HIP(hipMemcpyAsync(s->vbar.p, vbar, n3 * sizeof(double), hipMemcpyHostToDevice, s->stream));
+        PK_HIP(hipMemcpyAsync(s->vbar.p, vbar, n3 * sizeof(double), in, s->stream));
     else
         PK_HIP(hipMemsetAsync(s->vbar.p, 0, n3 * sizeof(double), s->stream));
     if (t.T)
         PK_HIP(hipMemsetAsync(s->gradE.p, 0, static_cast<size_t>(t.T) * sizeof(double), s->stream));
     rc = is_f32(s) ? backward_run<float>(s, adj_tol, adj_max_iter, adj_iterations)
-                                         : backward_run<double>(s, adj_tol, adj_max_iter, adj_iterations);
+                   : backward_run<double>(s, adj_tol, adj_max_iter, adj_iterations);
     if (rc != PK_OK)
         return rc;
     if (grad_log_young && t.T)
-        PK_HIP(hipMemcpy(grad_log_young, s->gradE.p, static_cast<size_t>(t.T) * sizeof(double),
-                         hipMemcpyDeviceToHost));
+        PK_HIP(hipMemcpyAsync(grad_log_young, s->gradE.p, static_cast<size_t>(t.T) * sizeof(double), out,
+                              s->stream));
     if (grad_x0 && n3)
-        PK_HIP(hipMemcpy(grad_x0, s->xbar.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+        PK_HIP(hipMemcpyAsync(grad_x0, s->xbar.p, n3 * sizeof(double), out, s->stream));
     if (grad_v0 && n3)
-        PK_HIP(hipMemcpy(grad_v0, s->vbar.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+        PK_HIP(hipMemcpyAsync(grad_v0, s->vbar.p, n3 * sizeof(double), out, s->stream));
+    PK_HIP(hipStreamSynchronize(s->stream));
     return PK_OK;
 }
+}  // namespace pk
 
+extern "C" {
+
+int pk_system_backward(pk_system* s, const double* xbar, const double* vbar, double adj_tol, int adj_max_iter,
+                       double* grad_log_young, double* grad_x0, double* grad_v0, int* adj_iterations)
+{
+    return backward_entry(s, xbar, vbar, adj_tol, adj_max_iter, grad_log_young, grad_x0, grad_v0, adj_iterations,
+                          false);
+}
+
+int pk_system_backward_device(pk_system* s, const double* xbar, const double* vbar, double adj_tol,
+                              int adj_max_iter, double* grad_log_young, double* grad_x0, double* grad_v0,
+                              int* adj_iterations)
+{
+    return backward_entry(s, xbar, vbar, adj_tol, adj_max_iter, grad_log_young, grad_x0, grad_v0, adj_iterations,
+                          true);
+}
 
 int pk_system_group_results(pk_system* s, int* iterations, double* residual, int* converged, int* status)
 {

@@ -19,11 +19,25 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import os
 
 import numpy as np
 
 from . import native as N
 from . import workloads as W
+
+
+def diag_lib():
+    """The diagnostics library (pk_diag.h), built beside the product library."""
+    from .build import DIAG_LIB
+    if not os.path.exists(DIAG_LIB):
+        raise RuntimeError(f"{DIAG_LIB} is missing: run python -m physik_amd.build")
+    lib = C.CDLL(DIAG_LIB)
+    lib.pk_diag_copy.restype = C.c_int
+    lib.pk_diag_copy.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double)]
+    lib.pk_diag_last_error.restype = C.c_char_p
+    lib.pk_diag_last_error.argtypes = []
+    return lib
 
 
 def main():
@@ -39,10 +53,11 @@ def main():
     ap.add_argument("--staging", type=int, default=1, choices=[0, 1, 2],
                     help="SpMV staging of d: 0 off, 1 on, 2 on with the direction update fused")
     args = ap.parse_args()
-    lib = N.lib()
+    diag = diag_lib()
     ms = C.c_double()
     nbytes = args.copy_mb * (1 << 20)
-    N.check(lib.pk_diag_copy(nbytes, 3, C.byref(ms)), "pk_diag_copy")
+    if diag.pk_diag_copy(nbytes, 3, C.byref(ms)) != 0:
+        raise RuntimeError("pk_diag_copy failed: " + diag.pk_diag_last_error().decode())
     w = getattr(W, args.workload)()
     s = W.make_system(w, {"fp64": N.PK_FP64, "fp32": N.PK_FP32_STORAGE, "fp32u": N.PK_FP32_UPPER,
                           "fp64u": N.PK_FP64_UPPER}[args.precision])
@@ -74,7 +89,7 @@ def main():
     n_el, ms_el = s.kernel_time(N.K_ELEMENT)
     print(json.dumps({"copy_bytes": nbytes, "copy_ms": ms.value, "copy_GBps": 2 * nbytes / ms.value / 1e6,
                       "workload": args.workload, "steps": args.steps, "cg_iterations": int(it.max()),
-                      "nnzb_full": s.nnzb_full, "dynamic_rows": s.nd, "tets": s.t, "bodies": s.groups,
+                      "nnzb_full": s.nnzb_full, "dynamic_rows": s.nd, "tets": s.t, "nodes": s.n, "bodies": s.groups,
                       "spmv_launches": s.kernel_time(N.K_SPMV)[0], "spmv_blocks": blocks, "spmv_upper_blocks": ublocks, "spmv_rows": rows, "nnzb_upper": s.nnzb_upper,
                       "spmv_body_launches": body_launches, "shape": s.get_tune(),
                       "staging": s.get_staging(), "spmv_us": 1e3 * ms_spmv / max(n_spmv, 1), "update_us": 1e3 * ms_upd / max(n_upd, 1),

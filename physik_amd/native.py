@@ -81,6 +81,9 @@ PK_SIGNATURES = {
                                              C.POINTER(C.c_int)]),
     "pk_system_set_state": (C.c_int, [_vp, _vp, _vp]),
     "pk_system_get_state": (C.c_int, [_vp, _vp, _vp]),
+    "pk_system_set_state_device": (C.c_int, [_vp, _vp, _vp]),
+    "pk_system_get_state_device": (C.c_int, [_vp, _vp, _vp]),
+    "pk_system_tape_id": (C.c_int, [_vp, C.POINTER(C.c_longlong)]),
     "pk_system_export_begin": (C.c_int, [_vp, C.c_int]),
     "pk_system_export_view": (C.c_int, [_vp, _vp, _vp]),
     "pk_system_export_read": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
@@ -107,7 +110,7 @@ PK_SIGNATURES = {
     "pk_system_get_solution": (C.c_int, [_vp, _dp]),
     "pk_system_record": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
     "pk_system_backward": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp, _vp, _vp]),
-    "pk_diag_copy": (C.c_int, [C.c_longlong, C.c_int, C.POINTER(C.c_double)]),
+    "pk_system_backward_device": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp, _vp, _vp]),
     "pk_pattern_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(_vp)]),
     "pk_pattern_destroy": (None, [_vp]),
     "pk_pattern_sizes": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
@@ -149,6 +152,8 @@ PHYSIK_SIGNATURES = {
     "PHYSIK_GetTetMeshGlobalNodeCount": (C.c_int, [_W, _H]),
     "PHYSIK_GetTetMeshGlobalNodeIndex": (C.c_int, [_W, _H, C.c_int]),
     "PHYSIK_SetTetMeshMaterial": (None, [_W, _H, C.POINTER(PhysikMaterialDesc)]),
+    "PHYSIK_SetTetMeshLocalCurrentPosition": (C.c_int, [_W, _H, C.c_int, C.c_float, C.c_float, C.c_float]),
+    "PHYSIK_GetTetMeshLocalCurrentPosition": (C.c_int, [_W, _H, C.c_int, _fp, _fp, _fp]),
     "PHYSIK_DestroyComponent": (None, [_W, _H]),
     "PHYSIK_IsComponentHandleValid": (C.c_int, [_W, _H]),
     "PHYSIK_GetTetMeshTetCount": (C.c_int, [_W, _H]),

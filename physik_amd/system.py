@@ -133,6 +133,21 @@ class System:
         N.check(self.lib.pk_system_get_state(self.h, N.ptr(x), N.ptr(v)), "pk_system_get_state")
         return x, v
 
+    def set_state_device(self, x_ptr=None, v_ptr=None):
+        """Node state from device memory: x_ptr, v_ptr are addresses of
+        node_count x 3 float64 arrays on the system's device (or None)."""
+        N.check(self.lib.pk_system_set_state_device(self.h, x_ptr, v_ptr), "pk_system_set_state_device")
+
+    def get_state_device(self, x_ptr=None, v_ptr=None):
+        """Node state into device memory (addresses as set_state_device)."""
+        N.check(self.lib.pk_system_get_state_device(self.h, x_ptr, v_ptr), "pk_system_get_state_device")
+
+    def tape_id(self):
+        """Id of the recorded run backward() would reverse (0: none)."""
+        v = C.c_longlong()
+        N.check(self.lib.pk_system_tape_id(self.h, C.byref(v)), "pk_system_tape_id")
+        return v.value
+
     def export_begin(self, positions=True, velocities=True):
         """Starts the float frame export (pk_system_export_begin); returns at once."""
         what = (N.PK_EXPORT_POSITIONS if positions else 0) | (N.PK_EXPORT_VELOCITIES if velocities else 0)
@@ -310,6 +325,16 @@ class System:
         N.check(self.lib.pk_system_backward(self.h, N.ptr(xb), N.ptr(vb), tol, int(max_iter), N.ptr(gE), N.ptr(gx),
                                             N.ptr(gv), N.ptr(its)), "pk_system_backward")
         return gE, gx, gv, its[:getattr(self, "_steps", 0)]
+
+
+    def backward_device(self, xbar_ptr=None, vbar_ptr=None, grad_log_young_ptr=None, grad_x0_ptr=None,
+                        grad_v0_ptr=None, tol=1e-6, max_iter=1024):
+        """backward() with device addresses for the seeds and the gradients;
+        returns the adjoint CG iterations per substep."""
+        its = np.zeros(max(getattr(self, "_steps", 0), 1), np.int32)
+        N.check(self.lib.pk_system_backward_device(self.h, xbar_ptr, vbar_ptr, tol, int(max_iter), grad_log_young_ptr,
+                                                   grad_x0_ptr, grad_v0_ptr, N.ptr(its)), "pk_system_backward_device")
+        return its[:getattr(self, "_steps", 0)]
 
 
 def device_count() -> int:

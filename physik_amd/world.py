@@ -168,6 +168,15 @@ class World:
     def SetTetMeshMaterial(self, comp, mat):
         self.lib.PHYSIK_SetTetMeshMaterial(self.h, comp, C.byref(mat))
 
+    def SetTetMeshLocalCurrentPosition(self, comp, local, x, y, z):
+        return self.lib.PHYSIK_SetTetMeshLocalCurrentPosition(self.h, comp, local, x, y, z)
+
+    def GetTetMeshLocalCurrentPosition(self, comp, local):
+        """(status, (x, y, z)) of PHYSIK_GetTetMeshLocalCurrentPosition."""
+        v = [C.c_float() for _ in range(3)]
+        rc = self.lib.PHYSIK_GetTetMeshLocalCurrentPosition(self.h, comp, local, *[C.byref(a) for a in v])
+        return rc, tuple(a.value for a in v)
+
     def DestroyComponent(self, comp):
         self.lib.PHYSIK_DestroyComponent(self.h, comp)
 

@@ -79,3 +79,42 @@ def test_autograd_gradients_match_central_differences():
     print(f"v0 direction: adjoint {ad:.9e} fd {fd:.9e}", flush=True)
     assert abs(ad - fd) <= 1e-5 * abs(fd)
     s.close()
+
+
+def test_autograd_on_device_tensors_equals_the_host_path():
+    # tensors on the GPU cross the boundary as device pointers: the outputs
+    # and gradients must be bit-identical to the host-tensor path (a linear
+    # loss, so the seeds torch forms are the same constants on both devices)
+    w = small(tol=1e-12)
+    s = W.make_system(w, N.PK_FP64)
+    rng = np.random.default_rng(5)
+    cx, cv = rng.normal(size=w.x0.shape), rng.normal(size=w.v0.shape)
+    out = []
+    for dev in ("cpu", "cuda"):
+        log_e = torch.tensor(np.log(w.young), requires_grad=True, device=dev)
+        v0 = torch.tensor(w.v0.copy(), requires_grad=True, device=dev)
+        x0 = torch.tensor(w.x0.copy(), requires_grad=True, device=dev)
+        xS, vS = ImplicitRollout.apply(s, log_e, x0, v0, w.dt, w.steps, 3, 1e-12, 4000)
+        assert xS.device.type == dev and vS.device.type == dev
+        loss = (xS * torch.from_numpy(cx).to(dev)).sum() + (vS * torch.from_numpy(cv).to(dev)).sum()
+        loss.backward()
+        out.append([t.detach().cpu().numpy() for t in (xS, vS, log_e.grad, x0.grad, v0.grad)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+    s.close()
+
+
+def test_backward_of_an_overwritten_recording_raises():
+    # two rollouts on one system before the first backward: the first
+    # backward would reverse the second run, so it must refuse
+    w = small(tol=1e-10)
+    s = W.make_system(w, N.PK_FP64)
+    log_e = torch.tensor(np.log(w.young), requires_grad=True)
+    v0 = torch.tensor(w.v0.copy(), requires_grad=True)
+    loss1, _ = rollout_loss(s, w, log_e, v0)
+    loss2, _ = rollout_loss(s, w, log_e, v0)
+    with pytest.raises(RuntimeError, match="recorded another run"):
+        loss1.backward()
+    loss2.backward()  # the latest run still has its recording
+    assert np.isfinite(log_e.grad.numpy()).all()
+    s.close()
