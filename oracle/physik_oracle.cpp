@@ -1231,6 +1231,55 @@ int pcg_t(int n, int nnz, const int* rs, const int* ci, const double* vals, cons
     *converged = res.converged ? 1 : 0;
     return static_cast<int>(xv.size()) == n ? res.iterations : -1 - res.iterations;
 }
+
+// CurrentLinearSolver::Solve (LinearSolver.cpp:93-131) with its block-Jacobi
+// preconditioner (BuildInversePreconditioner, LinearSolver.cpp:30-85): the
+// inverse of each diagonal block, the solve refused (solution cleared) when a
+// diagonal block is missing or non-finite, its determinant is non-finite or
+// |det| <= 1e-8, or its inverse is non-finite; Identity blocks when Jacobi is
+// off; maxIterations <= 0 means 3 n. Returns the iterations, or -1 -
+// iterations when the solution was cleared.
+template <class T>
+int linear_solve_t(int n, int nnz, const int* rs, const int* ci, const double* vals, const double* b, int max_it,
+                   double tol, int jacobi, double* x, double* residual, int* converged)
+{
+    oracle::Bsr<T> A;
+    build_bsr(A, n, nnz, rs, ci, vals);
+    *residual = 0.0;
+    *converged = 0;
+    std::vector<oracle::M3<T>> mv(static_cast<size_t>(std::max(0, n)));
+    for (int blk = 0; blk < n; ++blk)
+    {
+        if (!jacobi)
+        {
+            mv[blk] = oracle::M3<T>{};  // Mat3::Identity()
+            continue;
+        }
+        const int idx = A.find(blk, blk);
+        if (idx < 0)
+            return -1;
+        const oracle::M3<T>& D = A.val[static_cast<size_t>(idx)];
+        if (!oracle::finite(D))
+            return -1;
+        const T d = oracle::det(D);
+        if (!oracle::finite(d) || std::abs(d) <= T(1.0e-8f))
+            return -1;
+        const oracle::M3<T> I = oracle::inv(D);
+        if (!oracle::finite(I))
+            return -1;
+        mv[blk] = I;
+    }
+    std::vector<oracle::V3<T>> bv(static_cast<size_t>(std::max(0, n))), xv;
+    for (int i = 0; i < n; ++i)
+        bv[i] = {static_cast<T>(b[3 * i]), static_cast<T>(b[3 * i + 1]), static_cast<T>(b[3 * i + 2])};
+    const int maxIt = max_it > 0 ? max_it : 3 * n;
+    const auto res = oracle::pcg(xv, A, bv, maxIt, static_cast<T>(tol), mv);
+    for (size_t i = 0; i < xv.size() && i < static_cast<size_t>(n); ++i)
+        oracle::put3(x + 3 * i, xv[i]);
+    *residual = static_cast<double>(res.residual);
+    *converged = res.converged ? 1 : 0;
+    return static_cast<int>(xv.size()) == n ? res.iterations : -1 - res.iterations;
+}
 }  // namespace oracle
 
 
@@ -1826,6 +1875,17 @@ int orc_pcg(int precision, int n, int nnz, const int* row_start, const int* col_
     if (precision == 32)
         return oracle::pcg_t<float>(n, nnz, row_start, col_index, vals, b, minv, max_it, tol, x, residual, converged);
     return oracle::pcg_t<double>(n, nnz, row_start, col_index, vals, b, minv, max_it, tol, x, residual, converged);
+}
+
+// The LinearSolver plugin's solve on a symmetric-upper BSR (see linear_solve_t).
+int orc_linear_solve(int precision, int n, int nnz, const int* row_start, const int* col_index, const double* vals,
+                     const double* b, int max_it, double tol, int jacobi, double* x, double* residual, int* converged)
+{
+    if (precision == 32)
+        return oracle::linear_solve_t<float>(n, nnz, row_start, col_index, vals, b, max_it, tol, jacobi, x, residual,
+                                             converged);
+    return oracle::linear_solve_t<double>(n, nnz, row_start, col_index, vals, b, max_it, tol, jacobi, x, residual,
+                                          converged);
 }
 
 // Reverse mode (FP64 worlds only): runs `steps` implicit substeps of length dt

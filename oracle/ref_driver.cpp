@@ -43,6 +43,7 @@
 #include "PhysiK/Components/TetMeshPhysicsComponent.h"
 #include "PhysiK/Core/Physics/FEM/FEMModel.h"
 #include "PhysiK/Core/Solvers/Linear/ConjugateGradientSolver.h"
+#include "PhysiK/Core/Solvers/Linear/LinearSolver.h"
 #include "PhysiK/Math/SparseBlockMatrix.h"
 
 namespace
@@ -411,5 +412,39 @@ void ref_spmv(int n, const int* row_start, const int* col_index, const float* va
         y[3 * i + 1] = yv[i].y;
         y[3 * i + 2] = yv[i].z;
     }
+}
+
+// The reference's LinearSolver plugin slot (GetCurrentLinearSolver().Solve,
+// LinearSolver.cpp:93-136) on a symmetric-upper BSR. Returns the iterations,
+// or -1 - iterations when the solver left the solution vector empty.
+int ref_linear_solve(int n, const int* row_start, const int* col_index, const float* vals, const float* b,
+                     int max_iterations, float tolerance, int use_jacobi, float* x, float* residual, int* converged)
+{
+    std::vector<std::pair<int, int>> coords;
+    for (int r = 0; r < n; ++r)
+        for (int k = row_start[r]; k < row_start[r + 1]; ++k)
+            coords.push_back({r, col_index[k]});
+    PhysiK::SparseBlockMatrix A;
+    A.BuildPattern(n, coords);
+    for (int r = 0; r < n; ++r)
+        for (int k = row_start[r]; k < row_start[r + 1]; ++k)
+            A.AddBlock(r, col_index[k], get_rowmajor(vals + 9 * k));
+    std::vector<PhysiK::Vec3> bv(static_cast<size_t>(n)), xv;
+    for (int i = 0; i < n; ++i)
+        bv[i] = PhysiK::Vec3{b[3 * i], b[3 * i + 1], b[3 * i + 2]};
+    PhysiK::LinearSolveSettings settings;
+    settings.maxIterations = max_iterations;
+    settings.tolerance = tolerance;
+    settings.useJacobiPreconditioner = use_jacobi != 0;
+    const PhysiK::LinearSolveResult res = PhysiK::GetCurrentLinearSolver().Solve(A, bv, xv, settings);
+    for (size_t i = 0; i < xv.size() && i < static_cast<size_t>(n); ++i)
+    {
+        x[3 * i + 0] = xv[i].x;
+        x[3 * i + 1] = xv[i].y;
+        x[3 * i + 2] = xv[i].z;
+    }
+    *residual = res.residualNorm;
+    *converged = res.converged ? 1 : 0;
+    return static_cast<int>(xv.size()) == n ? res.iterations : -1 - res.iterations;
 }
 }

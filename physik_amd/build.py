@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libphysik_amd.so")
 SOURCES = ["csrc/pk_assembly.hip", "csrc/pk_cg.hip", "csrc/pk_system.cpp", "csrc/pk_topology.cpp",
-           "csrc/physik_api.cpp", "csrc/pk_adjoint.hip"]
+           "csrc/physik_api.cpp", "csrc/pk_adjoint.hip", "csrc/pk_linear.hip"]
 # the counter-calibration copy kernel of physik_amd/diag.py: a library of its own
 DIAG_LIB = os.path.join(LIB_DIR, "libphysik_diag.so")
 DIAG_SOURCES = ["csrc/pk_diag.hip"]

@@ -63,6 +63,12 @@ class PhysikCollisionSphereOverlap(C.Structure):
                 ("sphereRadius", C.c_float), ("minDistance", C.c_float)]
 
 
+class LinearResult(C.Structure):
+    """pk_linear_result (include/physik_amd_linear.h)."""
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("solution_valid", C.c_int), ("pad", C.c_int),
+                ("residual_norm", C.c_double)]
+
+
 _lib = None
 
 PK_SIGNATURES = {
@@ -111,6 +117,10 @@ PK_SIGNATURES = {
     "pk_system_record": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
     "pk_system_backward": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp, _vp, _vp]),
     "pk_system_backward_device": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp, _vp, _vp]),
+    "pk_linear_solve": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_float, C.c_int,
+                                  C.POINTER(LinearResult)]),
+    "pk_bsr_pcg": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_int, _vp,
+                             C.POINTER(LinearResult)]),
     "pk_pattern_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(_vp)]),
     "pk_pattern_destroy": (None, [_vp]),
     "pk_pattern_sizes": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),

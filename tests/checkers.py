@@ -63,6 +63,9 @@ def oracle_lib():
         lib.orc_adjoint.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp, _dp,
                                     _ip]
         lib.orc_adjoint.restype = C.c_int
+        lib.orc_linear_solve.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, C.c_int, C.c_double,
+                                         C.c_int, _dp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        lib.orc_linear_solve.restype = C.c_int
         lib.orc_polar_vjp.argtypes = [_dp, _dp, _dp, _dp]
         lib.orc_polar_vjp.restype = C.c_int
         lib.orc_add_point_connection.argtypes = [C.c_void_p, _ip, _dp, _dp, C.c_double, C.c_double]
@@ -105,6 +108,9 @@ def ref_lib():
         lib.ref_get_elements.argtypes = [C.c_void_p, _bp, _fp, _fp]
         lib.ref_get_rest.argtypes = [C.c_void_p, _fp, _fp, _fp]
         lib.ref_spmv.argtypes = [C.c_int, _ip, _ip, _fp, _fp, _fp]
+        lib.ref_linear_solve.argtypes = [C.c_int, _ip, _ip, _fp, _fp, C.c_int, C.c_float, C.c_int, _fp,
+                                         C.POINTER(C.c_float), C.POINTER(C.c_int)]
+        lib.ref_linear_solve.restype = C.c_int
         lib.ref_add_point_connection.argtypes = [C.c_void_p, _ip, _fp, _fp, C.c_float, C.c_float]
         lib.ref_point_connection_count.argtypes = [C.c_void_p]
         lib.ref_point_connection_count.restype = C.c_int
@@ -292,6 +298,34 @@ def oracle_pcg(row_start, col, vals, b, minv, max_it, tol, precision=64):
                               _c(vals, np.float64).reshape(-1), _c(b, np.float64).reshape(-1),
                               None if mv is None else mv.ctypes.data, max_it, tol, x, C.byref(res), C.byref(conv))
     return x.reshape(-1, 3), it, res.value, bool(conv.value)
+
+
+def oracle_linear_solve(row_start, col, vals, b, max_it, tol, jacobi=True, precision=64):
+    """LinearSolver::Solve restated (block-Jacobi PCG on a symmetric-upper
+    BSR): (x or None when the solution was cleared, iterations, residual,
+    converged)."""
+    n = len(row_start) - 1
+    x = np.zeros(max(n, 1) * 3)
+    res = C.c_double()
+    conv = C.c_int()
+    it = oracle_lib().orc_linear_solve(precision, n, len(col), _c(row_start, np.int32), _c(col, np.int32),
+                                       _c(vals, np.float64).reshape(-1), _c(b, np.float64).reshape(-1), int(max_it),
+                                       float(tol), int(jacobi), x, C.byref(res), C.byref(conv))
+    sol = x[:3 * n].reshape(-1, 3) if it >= 0 else None
+    return sol, (it if it >= 0 else -1 - it), res.value, bool(conv.value)
+
+
+def ref_linear_solve(row_start, col, vals, b, max_it, tol, jacobi=True):
+    """The reference's own GetCurrentLinearSolver().Solve (FP32)."""
+    n = len(row_start) - 1
+    x = np.zeros(max(n, 1) * 3, np.float32)
+    res = C.c_float()
+    conv = C.c_int()
+    it = ref_lib().ref_linear_solve(n, _c(row_start, np.int32), _c(col, np.int32), _c(vals, np.float32).reshape(-1),
+                                    _c(b, np.float32).reshape(-1), int(max_it), float(tol), int(jacobi), x,
+                                    C.byref(res), C.byref(conv))
+    sol = x[:3 * n].reshape(-1, 3) if it >= 0 else None
+    return sol, (it if it >= 0 else -1 - it), res.value, bool(conv.value)
 
 
 class Reference:
