@@ -42,6 +42,12 @@ typedef struct pk_linear_result {
     double residual_norm;
 } pk_linear_result;
 
+/* Largest system pk_bsr_pcg_noalloc solves: its work arrays live in a
+ * file-scope device array sized for this many rows (44,053,760 bytes: 168
+ * bytes per row plus the partial sums), which every GPU the library runs on
+ * holds until the process ends. */
+#define PK_LINEAR_NOALLOC_MAX_ROWS 262144
+
 /* The plugin entry, host memory in the reference's own layout: a
  * SparseBlockMatrix of block_count block rows (row_start: block_count + 1
  * offsets, col_index: row_start[block_count] block columns, values: one
@@ -67,6 +73,21 @@ int pk_linear_solve(int block_count, const int* row_start, const int* col_index,
  * 0. */
 int pk_bsr_pcg(int rows, const int* row_start, const int* col, const double* vals, const double* b, double* x,
                int max_iterations, double tolerance, int use_jacobi, void* stream, pk_linear_result* result);
+
+/* pk_bsr_pcg without any allocation: no hipMalloc, hipFree or allocating
+ * library call. No counterpart in the reference; it is for callers that
+ * cannot allocate (code between HIP graph captures, serving loops, harnesses
+ * that forbid device allocation). Identical kernels and results as
+ * pk_bsr_pcg; the work arrays are one file-scope device array per GPU
+ * (PK_LINEAR_NOALLOC_MAX_ROWS rows at most: larger systems are refused with
+ * PK_ERR_ARG), located on first use on each device. Every call on one GPU
+ * shares that array: calls that overlap in time on one GPU (from several
+ * host threads or streams) race and must be serialised by the caller. Each
+ * call writes every work value it reads, so nothing carries over between
+ * calls. */
+int pk_bsr_pcg_noalloc(int rows, const int* row_start, const int* col, const double* vals, const double* b,
+                       double* x, int max_iterations, double tolerance, int use_jacobi, void* stream,
+                       pk_linear_result* result);
 
 #pragma GCC visibility pop
 

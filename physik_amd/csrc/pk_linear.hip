@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -498,9 +499,83 @@ bool linear_trivial(int rows, double tol, pk_linear_result* res, int* status)
 
 }  // namespace pk
 
+// The work arrays of pk_bsr_pcg_noalloc: one copy per GPU, held until the
+// process ends. Host code never passes `scratch` itself (its host-side name
+// is not a device address); it uses the address hipGetSymbolAddress gives on
+// each device.
+constexpr size_t kNoallocBytes = pk::lin::work_bytes(PK_LINEAR_NOALLOC_MAX_ROWS);
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+namespace pk
+{
+namespace
+{
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call on that device (with it current) and kept.
+char* scratch_on_current_device()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    {
+        set_error("pk_bsr_pcg_noalloc: no current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev])
+    {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+        {
+            set_error(std::string("pk_bsr_pcg_noalloc: hipGetSymbolAddress(scratch) failed: ") +
+                      hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch[dev] = static_cast<char*>(p);
+    }
+    return g_scratch[dev];
+}
+}  // namespace
+}  // namespace pk
+
 using namespace pk;
 
 extern "C" {
+
+int pk_bsr_pcg_noalloc(int rows, const int* row_start, const int* col, const double* vals, const double* b,
+                       double* x, int max_iterations, double tolerance, int use_jacobi, void* stream,
+                       pk_linear_result* result)
+{
+    if (!result)
+    {
+        set_error("pk_bsr_pcg_noalloc: result is NULL");
+        return PK_ERR_ARG;
+    }
+    if (rows > PK_LINEAR_NOALLOC_MAX_ROWS)
+    {
+        *result = pk_linear_result{};
+        set_error("pk_bsr_pcg_noalloc: " + std::to_string(rows) + " rows exceed the " +
+                  std::to_string(PK_LINEAR_NOALLOC_MAX_ROWS) + " its work arrays hold (use pk_bsr_pcg)");
+        return PK_ERR_ARG;
+    }
+    int status;
+    if (linear_trivial(rows, tolerance, result, &status))
+        return status;
+    if (!row_start || !col || !vals || !b || !x)
+    {
+        set_error("pk_bsr_pcg_noalloc: a device pointer is NULL");
+        return PK_ERR_ARG;
+    }
+    char* base = scratch_on_current_device();
+    if (!base)
+        return PK_ERR_HIP;
+    return linear_solve_on(base, rows, row_start, col, vals, b, x, max_iterations, tolerance, use_jacobi,
+                           static_cast<hipStream_t>(stream), result);
+}
 
 int pk_bsr_pcg(int rows, const int* row_start, const int* col, const double* vals, const double* b, double* x,
                int max_iterations, double tolerance, int use_jacobi, void* stream, pk_linear_result* result)

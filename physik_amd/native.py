@@ -121,6 +121,8 @@ PK_SIGNATURES = {
                                   C.POINTER(LinearResult)]),
     "pk_bsr_pcg": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_int, _vp,
                              C.POINTER(LinearResult)]),
+    "pk_bsr_pcg_noalloc": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_int, _vp,
+                                     C.POINTER(LinearResult)]),
     "pk_pattern_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(_vp)]),
     "pk_pattern_destroy": (None, [_vp]),
     "pk_pattern_sizes": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
