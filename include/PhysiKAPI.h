@@ -221,6 +221,13 @@ extern "C"
     /* Per-tet Young's modulus of a physics component (count = its tet count). */
     PHYSIK_API int PHYSIK_AMD_SetTetMeshYoungModuli(PhysiK::WorldHandle world, PhysiK::ComponentHandle component,
                                                     const double* youngModuli, int count);
+    /* FEM model of a physics component: the reference's
+     * TetMeshPhysicsComponentDesc::femModel (TetMeshPhysicsComponent.h:13-17),
+     * which its C ABI cannot set. 0 Linear, 1 Corotational (the default), 2
+     * NeoHookean (no elastic contribution, as FEMModel::ComputeForces).
+     * Returns 1, or 0 for a bad handle or model. */
+    PHYSIK_API int PHYSIK_AMD_SetTetMeshFemModel(PhysiK::WorldHandle world, PhysiK::ComponentHandle component,
+                                                 int femModel);
     /* Frame readback (the per-frame readers of the reference: visual
      * components in World::PostUpdate, World.cpp:539-549, and a host engine's
      * PHYSIK_GetNodePosition/Velocity loop). After each PHYSIK_Step the GPU

@@ -192,6 +192,14 @@ int pk_system_set_gravity(pk_system* sys, double gx, double gy, double gz);
 int pk_system_set_solver(pk_system* sys, double tolerance, int max_iterations, int use_jacobi);
 /* 1 = implicit Euler (default), 0 = explicit Euler (World.cpp:697-725). */
 int pk_system_set_mode(pk_system* sys, int implicit);
+/* FEM model per tet (tet_count values, the reference's FemModel,
+ * FEMModel.h:40-45): 0 linear (FEMModel::ComputeElasticForces,
+ * FEMModel.cpp:559-612: K1 with R = I, no deformation-gradient check), 1
+ * corotational (the default, TetMeshPhysicsComponent.h:13-17), 2 NeoHookean,
+ * for which the reference's FEMModel::ComputeForces produces no element
+ * contribution (the tet keeps its lumped mass). The reverse mode covers the
+ * corotational model only: pk_system_record fails while any tet has another. */
+int pk_system_set_fem_models(pk_system* sys, const unsigned char* models);
 /* Per-tet Young's modulus (tet_count values); topology is unchanged. */
 int pk_system_set_young(pk_system* sys, const double* young);
 
@@ -299,8 +307,9 @@ int pk_system_get_elements(pk_system* sys, double* blocks, double* forces, doubl
 /* Per-tet rotation of the corotation at the current state (tet_count x 9,
  * row-major R) and how ExtractRotationPolar (FEMModel.cpp:115-163) found it
  * (tet_count): 0 the Newton limit, 1 the Newton limit with column 2 negated
- * (det < 0), 2 the Identity fallback (non-finite or |det| <= 1e-8), -1 the tet
- * contributes nothing (inactive, zero volume or non-finite F; R reads 0).
+ * (det < 0), 2 the Identity fallback (non-finite or |det| <= 1e-8), 3 the
+ * linear model (R = I), -1 the tet contributes nothing (inactive, zero
+ * volume, non-finite F or no elastic model; R reads 0).
  * Either pointer may be NULL. */
 int pk_system_get_rotations(pk_system* sys, double* rotations, int* kind);
 /* y = A x with the assembled full BSR (dynamic_rows x 3). */

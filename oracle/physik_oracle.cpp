@@ -333,7 +333,7 @@ void corotated_u(T u[12], const Tet<T>& t, const std::vector<V3<T>>& x, const M3
 // R * K_ab * R^T, built from Ke entries).
 template <class T>
 void element(Element<T>& out, const Tet<T>& t, const FemCache<T>& c, const std::vector<V3<T>>& v, const T u[12],
-             const M3<T>& R)
+             const M3<T>& R, bool rotate = true)
 {
     T strain[6], stress[6], fint[12];
     for (int r = 0; r < 6; ++r)  // :273-284
@@ -359,7 +359,8 @@ void element(Element<T>& out, const Tet<T>& t, const FemCache<T>& c, const std::
     for (int a = 0; a < 4; ++a)
     {
         V3<T> fe{-fint[3 * a + 0], -fint[3 * a + 1], -fint[3 * a + 2]};
-        fe = mul(R, fe);
+        if (rotate)  // the linear model passes no rotation (FEMModel.cpp:403-406)
+            fe = mul(R, fe);
         const V3<T> damp = scl(v[t.n[a]], -t.damping);
         out.f[a] = add(fe, damp);
     }
@@ -371,9 +372,22 @@ void element(Element<T>& out, const Tet<T>& t, const FemCache<T>& c, const std::
             for (int i = 0; i < 3; ++i)
                 for (int j = 0; j < 3; ++j)
                     set(blk, i, j, c.Ke[3 * a + i][3 * b + j]);
-            out.k[a][b] = mul(mul(R, blk), Rt);
+            out.k[a][b] = rotate ? mul(mul(R, blk), Rt) : blk;  // AssembleStiffness, :336-375
         }
     out.R = R;
+}
+
+// FEMModel.cpp:227-245: u_a = x_a - X_a (the linear model's displacement).
+template <class T>
+void linear_u(T u[12], const Tet<T>& t, const std::vector<V3<T>>& x)
+{
+    for (int a = 0; a < 4; ++a)
+    {
+        const V3<T> d = sub(x[t.n[a]], t.restPos[a]);
+        u[3 * a + 0] = d.x;
+        u[3 * a + 1] = d.y;
+        u[3 * a + 2] = d.z;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -629,6 +643,10 @@ struct World
     int maxIt = 128;
     T tol = T(1.0e-4f);
     bool jacobi = true;
+    // the FEM model of the world's tets (FemModel, FEMModel.h:40-45): 0 Linear,
+    // 1 Corotational (the default, TetMeshPhysicsComponent.h:13-17), 2
+    // NeoHookean, for which FEMModel::ComputeForces produces nothing
+    int femModel = 1;
     CgResult<T> last;
 
     // FEM-level pattern over world nodes (TetMeshPhysicsComponent.cpp:36-60,
@@ -856,17 +874,28 @@ struct World
             for (int a = 0; a < 4; ++a)
                 if (t.n[a] < 0 || t.n[a] >= N)
                     valid = false;
-            if (!valid || t.vol <= T(0))
+            if (!valid || t.vol <= T(0) || femModel == 2)
                 continue;
-            const M3<T> ds = cols(sub(x[t.n[1]], x[t.n[0]]), sub(x[t.n[2]], x[t.n[0]]), sub(x[t.n[3]], x[t.n[0]]));
-            const M3<T> F = mul(ds, t.dmInv);
-            if (!finite(F))
-                continue;
-            const M3<T> R = polar(F);
-            T u[12];
-            corotated_u(u, t, x, R);
             Element<T>& e = elems[ti];
-            element(e, t, cache[ti], v, u, R);
+            T u[12];
+            if (femModel == 0)
+            {
+                // FEMModel::ComputeElasticForces (:580-612): no deformation
+                // gradient, no rotation
+                linear_u(u, t, x);
+                element(e, t, cache[ti], v, u, M3<T>{}, false);
+            }
+            else
+            {
+                const M3<T> ds =
+                    cols(sub(x[t.n[1]], x[t.n[0]]), sub(x[t.n[2]], x[t.n[0]]), sub(x[t.n[3]], x[t.n[0]]));
+                const M3<T> F = mul(ds, t.dmInv);
+                if (!finite(F))
+                    continue;
+                const M3<T> R = polar(F);
+                corotated_u(u, t, x, R);
+                element(e, t, cache[ti], v, u, R);
+            }
             elemValid[ti] = 1;
             for (int a = 0; a < 4; ++a)  // TMPC.cpp:86-96
                 forcePush.push_back({t.n[a], e.f[a]});
@@ -1695,6 +1724,14 @@ void orc_set_node_active(void* p, const unsigned char* active)
 
 // Per-tet Young's modulus; rebuilds the element cache like
 // TetMeshPhysicsComponent::RebuildTetFemCache (:211-219).
+// TetMeshPhysicsComponentDesc::femModel (TetMeshPhysicsComponent.h:13-17):
+// 0 Linear, 1 Corotational, 2 NeoHookean (no elastic contribution).
+void orc_set_fem_model(void* p, int model)
+{
+    auto* h = static_cast<Handle*>(p);
+    ORC_DISPATCH(h, { w.femModel = model; });
+}
+
 void orc_set_young(void* p, const double* young)
 {
     auto* h = static_cast<Handle*>(p);
@@ -1901,8 +1938,8 @@ int orc_adjoint(void* p, double dt, int steps, const double* xbar, const double*
     if (h->precision != 64)
         return -1;
     auto& w = *h->w64;
-    if (!w.conns.empty() || !w.spheres.empty())
-        return -1;  // the reverse substep has no connection terms
+    if (!w.conns.empty() || !w.spheres.empty() || w.femModel != 1)
+        return -1;  // the reverse substep has no connection terms and covers the corotational model only
     std::vector<oracle::V3<double>> xb(static_cast<size_t>(w.N)), vb(static_cast<size_t>(w.N));
     for (int i = 0; i < w.N; ++i)
     {

@@ -355,7 +355,7 @@ void ref_get_elements(void* p, unsigned char* valid, float* f, float* K)
     for (int t = 0; t < h->tetCount; ++t)
     {
         std::vector<PhysiK::TetElementContribution> out;
-        PhysiK::FEMModel::ComputeForces(PhysiK::FemModel::Corotational, {c->tets[t]}, {c->tetFemCache[t]},
+        PhysiK::FEMModel::ComputeForces(c->GetFemModel(), {c->tets[t]}, {c->tetFemCache[t]},
                                         c->nodePositions, c->nodeVelocities, out);
         valid[t] = out.empty() ? 0 : 1;
         if (out.empty())
@@ -388,6 +388,14 @@ void ref_get_rest(void* p, float* vol, float* dminv, float* grad)
             grad[12 * t + 3 * a + 2] = tet.shapeFunctionGradients[a].z;
         }
     }
+}
+
+// TetMeshPhysicsComponentDesc::femModel of the driver's component (the C ABI
+// has no setter; FemModel, FEMModel.h:40-45).
+void ref_set_fem_model(void* p, int model)
+{
+    auto* h = static_cast<RefHandle*>(p);
+    C(h)->SetFemModel(static_cast<PhysiK::FemModel>(model));
 }
 
 // Symmetric-upper BSR SpMV (SparseBlockMatrix::Multiply, :209-268).

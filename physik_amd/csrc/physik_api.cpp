@@ -151,6 +151,7 @@ struct Component
     std::vector<uint8_t> tetActive;
     std::vector<double> young;  // per tet
     Material material;
+    int femModel = 1;  // TetMeshPhysicsComponentDesc::femModel: 0 Linear, 1 Corotational, 2 NeoHookean
     int tetBase = -1;  // index of the component's first tet in the device system
     // the component's own copy of its current node positions
     // (TetMeshComponent nodePositions: the rest positions at creation,
@@ -351,7 +352,8 @@ struct World
         const int N = nodeCount();
         std::vector<int> tets;
         std::vector<double> young, poisson, damping, density;
-        std::vector<uint8_t> tact;
+        std::vector<uint8_t> tact, models;
+        bool mixed = false;
         pendingTets.clear();
         for (const auto& cp : components)
         {
@@ -359,8 +361,10 @@ struct World
                 continue;
             Component& c = *cp;
             c.tetBase = static_cast<int>(young.size());
+            mixed = mixed || c.femModel != 1;
             for (size_t t = 0; t < c.tetActive.size(); ++t)
             {
+                models.push_back(static_cast<uint8_t>(c.femModel));
                 for (int a = 0; a < 4; ++a)
                     tets.push_back(c.globalBegin + c.tets[4 * t + a]);
                 young.push_back(c.young[t]);
@@ -406,6 +410,8 @@ struct World
         if (rc != PK_OK)
             return rc;
         ++builds;
+        if (mixed && (rc = pk_system_set_fem_models(sys, models.data())) != PK_OK)
+            return rc;
         rc = pk_system_set_state(sys, x.data(), v.data());
         if (rc != PK_OK)
             return rc;
@@ -1039,6 +1045,18 @@ int PHYSIK_AMD_SetTetMeshYoungModuli(PhysiK::WorldHandle world, PhysiK::Componen
         return 0;
     c->young.assign(young, young + count);
     w->topoDirty = true;
+    return 1;
+}
+
+int PHYSIK_AMD_SetTetMeshFemModel(PhysiK::WorldHandle world, PhysiK::ComponentHandle h, int femModel)
+{
+    World* w = W(world);
+    Component* c = physics(w, h);
+    if (!c || femModel < 0 || femModel > 2)
+        return 0;
+    if (c->femModel != femModel)
+        w->topoDirty = true;
+    c->femModel = femModel;
     return 1;
 }
 

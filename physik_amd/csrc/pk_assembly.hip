@@ -61,13 +61,18 @@ struct ElementOut
     double f[4][3];      // element forces (elastic + damping)
     double kv[4][3];     // sum_b (R K_ab R^T) v_b
     double R[9];         // the rotation (inspection only)
-    int kind;            // how polar() found R: 0 Newton, 1 column 2 negated, 2 Identity
+    int kind;            // how polar() found R: 0 Newton, 1 column 2 negated, 2 Identity; 3 linear model
 };
 
 __device__ inline void element_compute(int t, size_t TT, const int4* __restrict__ tets,
                                        const double* __restrict__ prm, const double* __restrict__ x,
-                                       const double* __restrict__ v, ElementOut& o)
+                                       const double* __restrict__ v, ElementOut& o, int model = FEM_COROTATIONAL)
 {
+    if (model == FEM_NONE)
+    {
+        o.valid = false;
+        return;
+    }
     const int4 n4 = tets[t];
     const int ids[4] = {n4.x, n4.y, n4.z, n4.w};
     double xs[4][3], vs[4][3];
@@ -103,11 +108,25 @@ __device__ inline void element_compute(int t, size_t TT, const int4* __restrict_
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             F[3 * i + j] = ds[0][i] * Dinv[j] + ds[1][i] * Dinv[3 + j] + ds[2][i] * Dinv[6 + j];
-    o.valid = fin9(F);
-    if (!o.valid)
-        return;  // the reference skips the tet this substep
     double R[9];
-    polar(F, R, &o.kind);
+    if (model == FEM_LINEAR)
+    {
+        // FEMModel::ComputeElasticForces: no deformation gradient check and
+        // no rotation; u_a = (x_a - x_0) - (X_a - X_0) is the displacement
+        // x_a - X_a relative to corner 0 (B maps both to the same strain)
+        o.valid = true;
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            R[e] = (e % 4 == 0) ? 1.0 : 0.0;
+        o.kind = 3;
+    }
+    else
+    {
+        o.valid = fin9(F);
+        if (!o.valid)
+            return;  // the reference skips the tet this substep
+        polar(F, R, &o.kind);
+    }
 #pragma unroll
     for (int e = 0; e < 9; ++e)
         o.R[e] = R[e];
@@ -200,6 +219,7 @@ __device__ inline void element_compute(int t, size_t TT, const int4* __restrict_
 // force and K v at its place in the row lists.
 __global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ tets,
                                                 const unsigned char* __restrict__ live,
+                                                const unsigned char* __restrict__ fem,
                                                 const double* __restrict__ prm, const double* __restrict__ x,
                                                 const double* __restrict__ v, const int* __restrict__ cpos,
                                                 double* __restrict__ basis, double* __restrict__ corner)
@@ -216,7 +236,7 @@ __global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ 
     o.valid = false;
     const bool work = t < T && live[t];
     if (work)
-        element_compute(t, TT, tets, prm, x, v, o);
+        element_compute(t, TT, tets, prm, x, v, o, fem ? fem[t] : FEM_COROTATIONAL);
 #pragma unroll
     for (int k = 0; k < EB_COUNT / 2; ++k)
         stage[threadIdx.x * (EB_COUNT / 2) + k] =
@@ -295,6 +315,7 @@ __device__ inline void element_block(const double* __restrict__ basis, const dou
 // inspection; zero (and kind -1) for tets that contribute nothing.
 __global__ __launch_bounds__(NT) void k_element_records(int T, const int4* __restrict__ tets,
                                                         const unsigned char* __restrict__ live,
+                                                        const unsigned char* __restrict__ fem,
                                                         const double* __restrict__ prm, const double* __restrict__ x,
                                                         const double* __restrict__ v, double* __restrict__ out)
 {
@@ -304,7 +325,7 @@ __global__ __launch_bounds__(NT) void k_element_records(int T, const int4* __res
     ElementOut o;
     o.valid = false;
     if (live[t])
-        element_compute(t, static_cast<size_t>(T), tets, prm, x, v, o);
+        element_compute(t, static_cast<size_t>(T), tets, prm, x, v, o, fem ? fem[t] : FEM_COROTATIONAL);
     double* r = out + EREC_COUNT * static_cast<size_t>(t);
     for (int p = 0; p < 10; ++p)
         for (int e = 0; e < 9; ++e)
@@ -624,7 +645,7 @@ void launch_element(const DeviceView& v)
 {
     if (v.T > 0)
         hipLaunchKernelGGL(k_element, dim3(xcd_grid((v.T + NT - 1) / NT)), dim3(NT), 0, v.stream, v.T, v.tets, v.live,
-                           v.prm,
+                           v.fem, v.prm,
                            v.x, v.v, v.cpos, v.basis, v.corner);
 }
 
@@ -1028,7 +1049,7 @@ void launch_element_records(const DeviceView& v, double* out)
 {
     if (v.T > 0)
         hipLaunchKernelGGL(k_element_records, dim3((v.T + NT - 1) / NT), dim3(NT), 0, v.stream, v.T, v.tets, v.live,
-                           v.prm, v.x, v.v, out);
+                           v.fem, v.prm, v.x, v.v, out);
 }
 
 template void launch_assemble<double>(const DeviceView&, double);

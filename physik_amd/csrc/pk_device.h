@@ -9,6 +9,14 @@ namespace pk
 
 constexpr int NT = 256;  // threads per workgroup for every kernel
 
+// FEM model of a tet (the reference's FemModel, FEMModel.h:40-45).
+enum FemModelId
+{
+    FEM_LINEAR = 0,        // K1 with R = I (FEMModel.cpp:559-612)
+    FEM_COROTATIONAL = 1,  // the default
+    FEM_NONE = 2           // NeoHookean: FEMModel::ComputeForces produces nothing for it
+};
+
 // Per-group CG state. No workgroup reads a field that another workgroup of
 // the same launch may write: delta is double-buffered by iteration parity,
 // and stop decisions are taken identically by every workgroup of the group
@@ -277,6 +285,7 @@ struct DeviceView
     // topology
     const int4* tets;
     const unsigned char* live;
+    const unsigned char* fem;  // per-tet FEM model (FemModel values), or nullptr: all corotational
     const double* prm;
     const int* cstart;
     const int* clist;

@@ -128,6 +128,7 @@ struct pk_system
 
     pk::DevBuf<int4> tets;
     pk::DevBuf<unsigned char> live, tact;
+    pk::DevBuf<unsigned char> fem;  // per-tet FEM model; not allocated while every tet is corotational
     pk::DevBuf<double> prm, x, v, basis, corner, adj, mrow, xs, r, pa;
     pk::DevBuf<char> minv;  // storage type, like the CG vectors
     pk::DevBuf<int> cstart, clist, uFull, uMirror, slotRow, rowGroup, fstart, fcol, nstart, nlist, rowNode, active;
@@ -274,6 +275,7 @@ struct pk_system
         dv.nnzf = topo.nnzf();
         dv.tets = tets.p;
         dv.live = live.p;
+        dv.fem = fem.p;
         dv.tact = tact.p;
         dv.prm = prm.p;
         dv.cstart = cstart.p;
@@ -1380,6 +1382,36 @@ int pk_system_set_mode(pk_system* s, int implicit)
     return PK_OK;
 }
 
+int pk_system_set_fem_models(pk_system* s, const unsigned char* models)
+{
+    if (!s || (!models && s->topo.T > 0))
+        return PK_ERR_ARG;
+    const size_t T = static_cast<size_t>(s->topo.T);
+    bool allCorot = true;
+    for (size_t t = 0; t < T; ++t)
+    {
+        if (models[t] > FEM_NONE)
+        {
+            set_error("pk_system_set_fem_models: a model is not 0 (linear), 1 (corotational) or 2 (none)");
+            return PK_ERR_ARG;
+        }
+        allCorot = allCorot && models[t] == FEM_COROTATIONAL;
+    }
+    s->tapeValid = false;  // a recorded run no longer replays the same map
+    ++s->epoch;            // captured graphs hold the element launch's arguments
+    if (allCorot)
+    {
+        s->fem.free();
+        return PK_OK;
+    }
+    int rc;
+    if ((rc = s->fem.ensure(T)) != PK_OK)
+        return rc;
+    PK_HIP(hipMemcpyAsync(s->fem.p, models, T, hipMemcpyHostToDevice, s->stream));
+    PK_HIP(hipStreamSynchronize(s->stream));
+    return PK_OK;
+}
+
 int pk_system_set_young(pk_system* s, const double* young)
 {
     if (!s || !young)
@@ -1634,6 +1666,11 @@ int pk_system_record(pk_system* s, double dt, int steps, int every)
     if (s->nconn > 0 || s->nsph > 0)
     {
         set_error("pk_system_record: the reverse mode does not cover point connections or collision spheres");
+        return PK_ERR_ARG;
+    }
+    if (s->fem.p)
+    {
+        set_error("pk_system_record: the reverse mode covers the corotational FEM model only");
         return PK_ERR_ARG;
     }
     if (s->topo.massErrorNode >= 0)

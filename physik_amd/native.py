@@ -97,6 +97,7 @@ PK_SIGNATURES = {
     "pk_system_set_solver": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
     "pk_system_set_mode": (C.c_int, [_vp, C.c_int]),
     "pk_system_set_young": (C.c_int, [_vp, _dp]),
+    "pk_system_set_fem_models": (C.c_int, [_vp, _vp]),
     "pk_system_step": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
     "pk_system_group_results": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "pk_system_profile": (C.c_int, [_vp, C.c_int]),
@@ -194,6 +195,7 @@ PHYSIK_SIGNATURES = {
     "PHYSIK_AMD_CopyNodeStateF64": (C.c_int, [_W, _vp, _vp, C.c_int]),
     "PHYSIK_AMD_SetNodeStateF64": (C.c_int, [_W, _vp, _vp, C.c_int]),
     "PHYSIK_AMD_GenerateTetMeshF64": (_vp, [_vp, C.c_int, _vp, C.c_int]),
+    "PHYSIK_AMD_SetTetMeshFemModel": (C.c_int, [_W, _H, C.c_int]),
     "PHYSIK_AMD_SetTetMeshYoungModuli": (C.c_int, [_W, _H, _vp, C.c_int]),
     "PHYSIK_AMD_SetFrameExport": (C.c_int, [_W, C.c_int]),
     "PHYSIK_AMD_GetNodePositions": (C.c_int, [_W, C.c_int, C.c_int, _vp]),

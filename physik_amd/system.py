@@ -178,6 +178,11 @@ class System:
     def set_mode(self, implicit=True):
         N.check(self.lib.pk_system_set_mode(self.h, int(implicit)), "pk_system_set_mode")
 
+    def set_fem_models(self, models):
+        """FEM model per tet (or one for all): 0 linear, 1 corotational, 2 none."""
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(models, np.uint8), (self.t,)))
+        N.check(self.lib.pk_system_set_fem_models(self.h, N.ptr(m)), "pk_system_set_fem_models")
+
     def set_young(self, young):
         y = np.ascontiguousarray(np.broadcast_to(np.asarray(young, np.float64), (self.t,)))
         N.check(self.lib.pk_system_set_young(self.h, y), "pk_system_set_young")

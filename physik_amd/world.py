@@ -168,6 +168,9 @@ class World:
     def SetTetMeshMaterial(self, comp, mat):
         self.lib.PHYSIK_SetTetMeshMaterial(self.h, comp, C.byref(mat))
 
+    def AMD_SetTetMeshFemModel(self, comp, model):
+        return self.lib.PHYSIK_AMD_SetTetMeshFemModel(self.h, comp, model)
+
     def SetTetMeshLocalCurrentPosition(self, comp, local, x, y, z):
         return self.lib.PHYSIK_SetTetMeshLocalCurrentPosition(self.h, comp, local, x, y, z)
 

@@ -43,6 +43,7 @@ def oracle_lib():
         lib.orc_set_tet_active.argtypes = [C.c_void_p, _bp]
         lib.orc_set_node_active.argtypes = [C.c_void_p, _bp]
         lib.orc_set_young.argtypes = [C.c_void_p, _dp]
+        lib.orc_set_fem_model.argtypes = [C.c_void_p, C.c_int]
         lib.orc_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         lib.orc_get_state.argtypes = [C.c_void_p, _dp, _dp]
         lib.orc_step.argtypes = [C.c_void_p, C.c_double]
@@ -96,6 +97,7 @@ def ref_lib():
         lib.ref_set_fixed.argtypes = [C.c_void_p, _bp]
         lib.ref_set_tet_active.argtypes = [C.c_void_p, _bp]
         lib.ref_set_young.argtypes = [C.c_void_p, _fp]
+        lib.ref_set_fem_model.argtypes = [C.c_void_p, C.c_int]
         lib.ref_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ref_get_state.argtypes = [C.c_void_p, _fp, _fp]
         lib.ref_step.argtypes = [C.c_void_p, C.c_float]
@@ -173,6 +175,10 @@ class Oracle:
 
     def set_young(self, young):
         self.lib.orc_set_young(self.h, _c(young, np.float64))
+
+    def set_fem_model(self, model):
+        """0 Linear, 1 Corotational (default), 2 NeoHookean (FemModel, FEMModel.h:40-45)."""
+        self.lib.orc_set_fem_model(self.h, int(model))
 
     def set_state(self, x=None, v=None):
         xa = None if x is None else _c(x, np.float64)
@@ -362,6 +368,9 @@ class Reference:
 
     def set_young(self, young):
         self.lib.ref_set_young(self.h, _c(young, np.float32))
+
+    def set_fem_model(self, model):
+        self.lib.ref_set_fem_model(self.h, int(model))
 
     def set_state(self, x=None, v=None):
         xa = None if x is None else _c(x, np.float32)

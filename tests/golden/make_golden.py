@@ -132,6 +132,62 @@ def elements():
     print("elements:", len(cases), "cases")
 
 
+def linear():
+    """The linear FEM model (FemModel::Linear, FEMModel.cpp:559-612) under
+    the same shapes: single tets in assorted states (elements) and small
+    meshes stepped implicitly and explicitly (trajectories)."""
+    out = {}
+    rng = mg.Pcg32(17)
+    unit = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float64)
+    tets1 = np.array([[0, 1, 2, 3]], np.int32)
+    el = {k: [] for k in ("rest", "x", "v", "young", "poisson", "damping", "valid", "f", "K")}
+    for k in range(8):
+        rest = unit * (0.2 + 0.8 * rng.uniform(1)[0]) + (rng.uniform(12).reshape(4, 3) - 0.5) * 0.05
+        x = rest + (rng.uniform(12).reshape(4, 3) - 0.5) * (0.02 if k < 4 else 0.4)  # small and large strain
+        if k == 7:
+            x = rest @ np.array([[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]]).T  # rigid rotation
+        v = (rng.uniform(12).reshape(4, 3) - 0.5)
+        E, nu, c = 10 ** (2 + 3 * rng.uniform(1)[0]), 0.2 + 0.28 * rng.uniform(1)[0], rng.uniform(1)[0]
+        ref = Reference(rest, tets1, 1.0, E, nu, c)
+        ref.set_fem_model(0)
+        ref.set_state(x, v)
+        valid, fe, ke = ref.elements()
+        ref.close()
+        for key, val in (("rest", f32(rest)), ("x", f32(x)), ("v", f32(v)), ("young", np.float32(E)),
+                         ("poisson", np.float32(nu)), ("damping", np.float32(c)), ("valid", valid[0]),
+                         ("f", fe[0]), ("K", ke[0])):
+            el[key].append(val)
+    out.update({f"elements__{k}": np.array(v) for k, v in el.items()})
+    rest, tets = mg.box_mesh(3, 2, 2, 0.2, "kuhn")
+    rest = mg.jitter(rest, 0.2, 0.1, mg.Pcg32(9))
+    fixed = np.zeros(len(rest), np.uint8)
+    fixed[mg.face_nodes(3, 2, 2, 0)] = 1
+    for mode, name in ((1, "implicit"), (0, "explicit")):
+        ref = Reference(rest, tets, 500.0, 2e4, 0.35, 0.05)
+        ref.set_fem_model(0)
+        ref.set_params(G, 2, 1e-6, 256, implicit=bool(mode))
+        ref.set_fixed(fixed)
+        x0 = rest.copy()
+        x0[~fixed.astype(bool)] += 0.01 * (rng.uniform(3 * int((fixed == 0).sum())).reshape(-1, 3) - 0.5)
+        ref.set_state(x0, None)
+        x_init, v_init = ref.state()
+        xs, vs, its = [], [], []
+        dt = np.float32(1.0 / 60.0 if mode else 1.0 / 2000.0)
+        for _ in range(6):
+            assert ref.step(dt) == 0
+            x, v = ref.state()
+            xs.append(x)
+            vs.append(v)
+            its.append(ref.cg()[0])
+        ref.close()
+        out.update({f"{name}__rest": f32(rest), f"{name}__tets": tets, f"{name}__fixed": fixed,
+                    f"{name}__x_init": x_init, f"{name}__v_init": v_init, f"{name}__dt": dt,
+                    f"{name}__x": np.array(xs), f"{name}__v": np.array(vs),
+                    f"{name}__cg_iterations": np.array(its, np.int32)})
+    np.savez_compressed(os.path.join(HERE, "linear_fp32.npz"), **out)
+    print("linear: 8 elements, implicit and explicit trajectories")
+
+
 def small_cases():
     """Tiny meshes stepped a few frames under assorted settings."""
     cases = {}
@@ -337,5 +393,5 @@ def spheres():
 
 if __name__ == "__main__":
     # every fixture, or only those named on the command line
-    for name in sys.argv[1:] or ["config1", "elements", "small_cases", "connections", "spheres"]:
+    for name in sys.argv[1:] or ["config1", "elements", "small_cases", "connections", "spheres", "linear"]:
         globals()[name]()
