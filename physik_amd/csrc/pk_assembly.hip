@@ -52,216 +52,241 @@ __global__ void k_reset(GroupState* gs, int G, int* active, int* kbase, double t
         *active = cnt;
 }
 
-// Everything one tet contributes to a substep, in registers.
-struct ElementOut
-{
-    bool valid;          // F finite: otherwise the tet contributes nothing
-    double basis[EB_COUNT];  // R g_a, then R R^T (xx, xy, xz, yy, yz, zz)
-    double K[10][9];     // R K_ab R^T for the stored pairs (row-major)
-    double f[4][3];      // element forces (elastic + damping)
-    double kv[4][3];     // sum_b (R K_ab R^T) v_b
-    double R[9];         // the rotation (inspection only)
-    int kind;            // how polar() found R: 0 Newton, 1 column 2 negated, 2 Identity; 3 linear model
-};
-
-__device__ inline void element_compute(int t, size_t TT, const int4* __restrict__ tets,
-                                       const double* __restrict__ prm, const double* __restrict__ x,
-                                       const double* __restrict__ v, ElementOut& o, int model = FEM_COROTATIONAL)
+// Everything one tet contributes to a substep (FEMModel.cpp:614-679 with
+// :377-424, :336-375 and :115-163), streamed to a sink as it is formed so that
+// no output waits in registers for the others:
+//   sink.rotation(R, kind)   R (row-major) and how it was found: 0 the Newton
+//                            polar limit, 1 with column 2 negated, 2 Identity
+//                            fallback, 3 the linear model (R = I)
+//   sink.basis(k, value)     the 18 basis values: R g_a (k = 3a + i), then
+//                            R R^T (xx, xy, xz, yy, yz, zz)
+//   sink.corner(a, f, kv)    corner a's element force (elastic + damping) and
+//                            (K v)_a = sum_b R K_ab R^T v_b
+// Both corner terms have the form V R M g_a with a 3x3 M: the stress
+// sigma(R^T (x_a - x_0) - (X_a - X_0)) for the force, and
+// S(H) = lambda tr(H) I + mu (H + H^T) with H = R^T sum_b v_b g_b^T for K v,
+// which is sum_b R Ke_ab R^T v_b written out (Ke_ab = V (lambda g_a g_b^T +
+// mu g_b g_a^T + mu (g_a . g_b) I)) for any R. Returns false when the tet
+// contributes nothing this substep (the sink is then not called).
+template <class Sink>
+__device__ __attribute__((always_inline)) inline bool element_eval(int t, size_t TT, const int4* __restrict__ tets,
+                                                                   const double* __restrict__ prm,
+                                                                   const double* __restrict__ x,
+                                                                   const double* __restrict__ v, int model,
+                                                                   Sink& sink)
 {
     if (model == FEM_NONE)
-    {
-        o.valid = false;
-        return;
-    }
+        return false;
     const int4 n4 = tets[t];
     const int ids[4] = {n4.x, n4.y, n4.z, n4.w};
-    double xs[4][3], vs[4][3];
+    double ds[3][3];  // ds[k] = x_{k+1} - x_0
+    {
+        const double x0[3] = {x[3 * ids[0]], x[3 * ids[0] + 1], x[3 * ids[0] + 2]};
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-        {
-            xs[a][i] = x[3 * ids[a] + i];
-            vs[a][i] = v[3 * ids[a] + i];
-        }
-    double Dinv[9], Erest[9];
+            for (int i = 0; i < 3; ++i)
+                ds[k][i] = x[3 * ids[k + 1] + i] - x0[i];
+    }
+    double Dinv[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k)
-    {
         Dinv[k] = prm[(P_DMINV + k) * TT + t];
-        Erest[k] = prm[(P_EDGE + k) * TT + t];
-    }
-    const double vol = prm[P_VOL * TT + t];
-    const double lam = prm[P_LAMBDA * TT + t];
-    const double mu = prm[P_MU * TT + t];
-    const double damp = prm[P_DAMP * TT + t];
-
-    double ds[3][3];  // ds[k] = x_{k+1} - x_0
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            ds[k][i] = xs[k + 1][i] - xs[0][i];
-    double F[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            F[3 * i + j] = ds[0][i] * Dinv[j] + ds[1][i] * Dinv[3 + j] + ds[2][i] * Dinv[6 + j];
     double R[9];
+    int kind;
     if (model == FEM_LINEAR)
     {
-        // FEMModel::ComputeElasticForces: no deformation gradient check and
-        // no rotation; u_a = (x_a - x_0) - (X_a - X_0) is the displacement
-        // x_a - X_a relative to corner 0 (B maps both to the same strain)
-        o.valid = true;
+        // FEMModel::ComputeElasticForces: no deformation-gradient check and
+        // no rotation; u_a = (x_a - x_0) - (X_a - X_0) below is the
+        // displacement x_a - X_a relative to corner 0 (B maps both to the
+        // same strain)
 #pragma unroll
         for (int e = 0; e < 9; ++e)
             R[e] = (e % 4 == 0) ? 1.0 : 0.0;
-        o.kind = 3;
+        kind = 3;
     }
     else
     {
-        o.valid = fin9(F);
-        if (!o.valid)
-            return;  // the reference skips the tet this substep
-        polar(F, R, &o.kind);
-    }
+        double F[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e)
-        o.R[e] = R[e];
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                F[3 * i + j] = ds[0][i] * Dinv[j] + ds[1][i] * Dinv[3 + j] + ds[2][i] * Dinv[6 + j];
+        if (!fin9(F))
+            return false;  // the reference skips the tet this substep
+        polar(F, R, &kind);
+    }
+    sink.rotation(R, kind);
     double g[4][3];
     shape_gradients(Dinv, g);
-    // u_a = R^T (x_a - x_0) - (X_a - X_0); u_0 = 0
-    double u[4][3];
 #pragma unroll
-    for (int a = 1; a < 4; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            u[a][i] = (R[i] * ds[a - 1][0] + R[3 + i] * ds[a - 1][1] + R[6 + i] * ds[a - 1][2]) -
-                      Erest[3 * (a - 1) + i];
-    // engineering strain and stress (FEMModel.cpp:174-225, 386-389)
-    double exx = 0, eyy = 0, ezz = 0, gxy = 0, gyz = 0, gxz = 0;
-#pragma unroll
-    for (int a = 1; a < 4; ++a)
+            sink.basis(EB_RG + 3 * a + i, R[3 * i] * g[a][0] + R[3 * i + 1] * g[a][1] + R[3 * i + 2] * g[a][2]);
+    sink.basis(EB_RRT + 0, R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+    sink.basis(EB_RRT + 1, R[0] * R[3] + R[1] * R[4] + R[2] * R[5]);
+    sink.basis(EB_RRT + 2, R[0] * R[6] + R[1] * R[7] + R[2] * R[8]);
+    sink.basis(EB_RRT + 3, R[3] * R[3] + R[4] * R[4] + R[5] * R[5]);
+    sink.basis(EB_RRT + 4, R[3] * R[6] + R[4] * R[7] + R[5] * R[8]);
+    sink.basis(EB_RRT + 5, R[6] * R[6] + R[7] * R[7] + R[8] * R[8]);
+    const double lam = prm[P_LAMBDA * TT + t];
+    const double mu = prm[P_MU * TT + t];
+    // engineering strain of u_a = R^T ds_a - (X_a - X_0), u_0 = 0, and the
+    // stress (FEMModel.cpp:174-225, 386-389)
+    double Pf[9];
     {
-        exx += g[a][0] * u[a][0];
-        eyy += g[a][1] * u[a][1];
-        ezz += g[a][2] * u[a][2];
-        gxy += g[a][1] * u[a][0] + g[a][0] * u[a][1];
-        gyz += g[a][2] * u[a][1] + g[a][1] * u[a][2];
-        gxz += g[a][2] * u[a][0] + g[a][0] * u[a][2];
-    }
-    const double tr = exx + eyy + ezz;
-    const double sxx = lam * tr + 2.0 * mu * exx;
-    const double syy = lam * tr + 2.0 * mu * eyy;
-    const double szz = lam * tr + 2.0 * mu * ezz;
-    const double txy = mu * gxy, tyz = mu * gyz, txz = mu * gxz;
-    // f_a = R (-V B_a^T sigma) - c v_a
+        double exx = 0, eyy = 0, ezz = 0, gxy = 0, gyz = 0, gxz = 0;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
-    {
-        const double fx = -vol * (sxx * g[a][0] + txy * g[a][1] + txz * g[a][2]);
-        const double fy = -vol * (txy * g[a][0] + syy * g[a][1] + tyz * g[a][2]);
-        const double fz = -vol * (txz * g[a][0] + tyz * g[a][1] + szz * g[a][2]);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            o.f[a][i] = (R[3 * i] * fx + R[3 * i + 1] * fy + R[3 * i + 2] * fz) - damp * vs[a][i];
-    }
-    double rg[4][3];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            rg[a][i] = R[3 * i] * g[a][0] + R[3 * i + 1] * g[a][1] + R[3 * i + 2] * g[a][2];
-    double RRt[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            RRt[3 * i + j] = R[3 * i] * R[3 * j] + R[3 * i + 1] * R[3 * j + 1] + R[3 * i + 2] * R[3 * j + 2];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            o.basis[EB_RG + 3 * a + i] = rg[a][i];
-    o.basis[EB_RRT + 0] = RRt[0];
-    o.basis[EB_RRT + 1] = RRt[1];
-    o.basis[EB_RRT + 2] = RRt[2];
-    o.basis[EB_RRT + 3] = RRt[4];
-    o.basis[EB_RRT + 4] = RRt[5];
-    o.basis[EB_RRT + 5] = RRt[8];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            o.kv[a][i] = 0.0;
-#pragma unroll
-    for (int p = 0; p < 10; ++p)
-    {
-        int a, b;
-        pair_of(p, a, b);
-        double* K = o.K[p];
-        rotated_block(rg[a], rg[b], RRt, vol, lam, mu, gdot(g[a], g[b]), K);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            o.kv[a][i] += K[3 * i] * vs[b][0] + K[3 * i + 1] * vs[b][1] + K[3 * i + 2] * vs[b][2];
-        if (a != b)
+        for (int a = 1; a < 4; ++a)
         {
+            double u[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                o.kv[b][i] += K[i] * vs[a][0] + K[3 + i] * vs[a][1] + K[6 + i] * vs[a][2];
+                u[i] = (R[i] * ds[a - 1][0] + R[3 + i] * ds[a - 1][1] + R[6 + i] * ds[a - 1][2]) -
+                       prm[(P_EDGE + 3 * (a - 1) + i) * TT + t];
+            exx += g[a][0] * u[0];
+            eyy += g[a][1] * u[1];
+            ezz += g[a][2] * u[2];
+            gxy += g[a][1] * u[0] + g[a][0] * u[1];
+            gyz += g[a][2] * u[1] + g[a][1] * u[2];
+            gxz += g[a][2] * u[0] + g[a][0] * u[2];
         }
+        const double tr = exx + eyy + ezz;
+        const double sg[9] = {lam * tr + 2.0 * mu * exx, mu * gxy, mu * gxz,
+                              mu * gxy, lam * tr + 2.0 * mu * eyy, mu * gyz,
+                              mu * gxz, mu * gyz, lam * tr + 2.0 * mu * ezz};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Pf[3 * i + j] = R[3 * i] * sg[j] + R[3 * i + 1] * sg[3 + j] + R[3 * i + 2] * sg[6 + j];
     }
+    double vs[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            vs[a][i] = v[3 * ids[a] + i];
+    double Pk[9];
+    {
+        double W[9];  // sum_b v_b g_b^T
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                W[3 * i + j] = ((vs[0][i] * g[0][j] + vs[1][i] * g[1][j]) + vs[2][i] * g[2][j]) + vs[3][i] * g[3][j];
+        double H[9];  // R^T W
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                H[3 * i + j] = R[i] * W[j] + R[3 + i] * W[3 + j] + R[6 + i] * W[6 + j];
+        const double lt = lam * (H[0] + H[4] + H[8]);
+        const double S[9] = {lt + 2.0 * mu * H[0], mu * (H[1] + H[3]), mu * (H[2] + H[6]),
+                             mu * (H[1] + H[3]), lt + 2.0 * mu * H[4], mu * (H[5] + H[7]),
+                             mu * (H[2] + H[6]), mu * (H[5] + H[7]), lt + 2.0 * mu * H[8]};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Pk[3 * i + j] = R[3 * i] * S[j] + R[3 * i + 1] * S[3 + j] + R[3 * i + 2] * S[6 + j];
+    }
+    const double vol = prm[P_VOL * TT + t];
+    const double damp = prm[P_DAMP * TT + t];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+    {
+        double f[3], kv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            f[i] = -vol * (Pf[3 * i] * g[a][0] + Pf[3 * i + 1] * g[a][1] + Pf[3 * i + 2] * g[a][2]) - damp * vs[a][i];
+            kv[i] = vol * (Pk[3 * i] * g[a][0] + Pk[3 * i + 1] * g[a][1] + Pk[3 * i + 2] * g[a][2]);
+        }
+        sink.corner(a, f, kv);
+    }
+    return true;
 }
 
-// Writes the tet's basis planes (R g_a, R R^T) and each dynamic corner's
-// force and K v at its place in the row lists.
+// The sink of the step's element kernel: basis values into the thread's LDS
+// record, and per dynamic corner one entry at its place in the row lists:
+// implicit, dt f - dt^2 (K v) (the force part dropped when it is not finite,
+// as SolverData::PrecomputeImplicitSolve skips a non-finite push,
+// SolverData.cpp:145-154); explicit, the force f.
+struct StepSink
+{
+    double* rec;  // the thread's 18 LDS slots
+    const int* cpos;
+    size_t TT;
+    int t;
+    double* entries;
+    double dt, dt2;
+    int implicit;
+
+    __device__ void rotation(const double*, int) {}
+    __device__ void basis(int k, double val) { rec[k] = val; }
+    __device__ void corner_at(int a, const double* f, const double* kv)
+    {
+        const int pos = cpos[a * TT + t];
+        if (pos < 0)
+            return;
+        double* out = entries + static_cast<size_t>(pos) * N_CORNER;
+        if (!implicit)
+        {
+            out[0] = f[0];
+            out[1] = f[1];
+            out[2] = f[2];
+            return;
+        }
+        const bool ff = fin(f[0]) && fin(f[1]) && fin(f[2]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            out[i] = (ff ? f[i] * dt : 0.0) - kv[i] * dt2;
+    }
+    __device__ void corner(int a, const double* f, const double* kv) { corner_at(a, f, kv); }
+};
+
+// Writes the tet's basis record and each dynamic corner's entry.
 __global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ tets,
                                                 const unsigned char* __restrict__ live,
                                                 const unsigned char* __restrict__ fem,
                                                 const double* __restrict__ prm, const double* __restrict__ x,
                                                 const double* __restrict__ v, const int* __restrict__ cpos,
-                                                double* __restrict__ basis, double* __restrict__ corner)
+                                                double* __restrict__ basis, double* __restrict__ corner, double dt,
+                                                int implicit)
 {
     const int t0 = xcd_block() * NT;
     const int t = t0 + static_cast<int>(threadIdx.x);
     const size_t TT = static_cast<size_t>(T);
     // the workgroup's basis records go out as one contiguous run: each thread
     // puts its record in LDS, then the workgroup copies the run with
-    // consecutive 16-byte stores (a tet that is not live stores zeros, which
-    // is what it holds: it contributes nothing)
+    // consecutive 16-byte stores (a tet that contributes nothing stores
+    // zeros, which is what it holds)
     __shared__ double2 stage[NT * EB_COUNT / 2];
-    ElementOut o;
-    o.valid = false;
+    double* rec = reinterpret_cast<double*>(stage) + threadIdx.x * EB_COUNT;
     const bool work = t < T && live[t];
-    if (work)
-        element_compute(t, TT, tets, prm, x, v, o, fem ? fem[t] : FEM_COROTATIONAL);
+    StepSink sink{rec, cpos, TT, t, corner, dt, dt * dt, implicit};
+    const bool ok = work && element_eval(t, TT, tets, prm, x, v, fem ? fem[t] : FEM_COROTATIONAL, sink);
+    if (!ok)
+    {
 #pragma unroll
-    for (int k = 0; k < EB_COUNT / 2; ++k)
-        stage[threadIdx.x * (EB_COUNT / 2) + k] =
-            o.valid ? make_double2(o.basis[2 * k], o.basis[2 * k + 1]) : make_double2(0.0, 0.0);
+        for (int k = 0; k < EB_COUNT; ++k)
+            rec[k] = 0.0;
+        if (work)
+        {
+            const double z[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+                sink.corner_at(a, z, z);
+        }
+    }
     __syncthreads();
     const int nrec = min(NT, T - t0);
     double2* dst = reinterpret_cast<double2*>(basis + static_cast<size_t>(t0) * EB_COUNT);
     for (int i = threadIdx.x; i < nrec * (EB_COUNT / 2); i += NT)
         dst[i] = stage[i];
-    if (!work)
-        return;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-    {
-        const int pos = cpos[a * TT + t];
-        if (pos < 0)
-            continue;
-        double* out = corner + static_cast<size_t>(pos) * N_CORNER;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-        {
-            out[i] = o.valid ? o.f[a][i] : 0.0;
-            out[N_KV + i] = o.valid ? o.kv[a][i] : 0.0;
-        }
-    }
 }
 
 // g_a . g_b of the 10 local pairs into the parameter planes (rest data, once
@@ -287,13 +312,12 @@ __global__ __launch_bounds__(NT) void k_gdot(int T, double* __restrict__ prm)
     }
 }
 
-// Block R K_ab R^T of local pair p of tet t from its basis record.
-__device__ inline void element_block(const double* __restrict__ basis, const double* __restrict__ prm, size_t TT,
-                                     int t, int p, double* K)
+// Block R K_ab R^T of local pair p of tet t from its basis record rec.
+__device__ inline void element_block_from(const double* __restrict__ rec, const double* __restrict__ prm, size_t TT,
+                                          int t, int p, double* K)
 {
     int a, b;
     pair_of(p, a, b);
-    const double* rec = basis + static_cast<size_t>(t) * EB_COUNT;
     double rga[3], rgb[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -310,9 +334,44 @@ __device__ inline void element_block(const double* __restrict__ basis, const dou
                   prm[(P_GDOT + p) * TT + t], K);
 }
 
+// Block R K_ab R^T of local pair p of tet t from the basis planes.
+__device__ inline void element_block(const double* __restrict__ basis, const double* __restrict__ prm, size_t TT,
+                                     int t, int p, double* K)
+{
+    element_block_from(basis + static_cast<size_t>(t) * EB_COUNT, prm, TT, t, p, K);
+}
+
+// The inspection sink: everything into registers.
+struct RecordSink
+{
+    double basisv[EB_COUNT];
+    double f[4][3], kv[4][3];
+    double R[9];
+    int kind;
+
+    __device__ void rotation(const double* r, int k)
+    {
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            R[e] = r[e];
+        kind = k;
+    }
+    __device__ void basis(int k, double val) { basisv[k] = val; }
+    __device__ void corner(int a, const double* ff, const double* kk)
+    {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            f[a][i] = ff[i];
+            kv[a][i] = kk[i];
+        }
+    }
+};
+
 // The element outputs of every tet as one record (10 blocks, 4 forces, 4 K v,
 // the rotation R and how polar() found it: EREC_COUNT doubles), for
-// inspection; zero (and kind -1) for tets that contribute nothing.
+// inspection; zero (and kind -1) for tets that contribute nothing. The blocks
+// are formed from the basis values exactly as the slot gather forms them.
 __global__ __launch_bounds__(NT) void k_element_records(int T, const int4* __restrict__ tets,
                                                         const unsigned char* __restrict__ live,
                                                         const unsigned char* __restrict__ fem,
@@ -322,23 +381,27 @@ __global__ __launch_bounds__(NT) void k_element_records(int T, const int4* __res
     const int t = blockIdx.x * NT + threadIdx.x;
     if (t >= T)
         return;
-    ElementOut o;
-    o.valid = false;
-    if (live[t])
-        element_compute(t, static_cast<size_t>(T), tets, prm, x, v, o, fem ? fem[t] : FEM_COROTATIONAL);
+    const size_t TT = static_cast<size_t>(T);
+    RecordSink o;
+    const bool ok = live[t] && element_eval(t, TT, tets, prm, x, v, fem ? fem[t] : FEM_COROTATIONAL, o);
     double* r = out + EREC_COUNT * static_cast<size_t>(t);
     for (int p = 0; p < 10; ++p)
+    {
+        double K[9];
+        if (ok)
+            element_block_from(o.basisv, prm, TT, t, p, K);
         for (int e = 0; e < 9; ++e)
-            r[EREC_K + 9 * p + e] = o.valid ? o.K[p][e] : 0.0;
+            r[EREC_K + 9 * p + e] = ok ? K[e] : 0.0;
+    }
     for (int a = 0; a < 4; ++a)
         for (int i = 0; i < 3; ++i)
         {
-            r[EREC_F + 3 * a + i] = o.valid ? o.f[a][i] : 0.0;
-            r[EREC_KV + 3 * a + i] = o.valid ? o.kv[a][i] : 0.0;
+            r[EREC_F + 3 * a + i] = ok ? o.f[a][i] : 0.0;
+            r[EREC_KV + 3 * a + i] = ok ? o.kv[a][i] : 0.0;
         }
     for (int e = 0; e < 9; ++e)
-        r[EREC_R + e] = o.valid ? o.R[e] : 0.0;
-    r[EREC_KIND] = o.valid ? static_cast<double>(o.kind) : -1.0;
+        r[EREC_R + e] = ok ? o.R[e] : 0.0;
+    r[EREC_KIND] = ok ? static_cast<double>(o.kind) : -1.0;
 }
 
 // Lanes per upper slot in the assembly gather: a node's star has ~20
@@ -496,30 +559,21 @@ __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__
     {
         const int row = row0 + static_cast<int>(threadIdx.x) / RL;
         const bool live = row < tl.rowEnd;
-        double f[3] = {0, 0, 0}, kv[3] = {0, 0, 0};
+        // each corner entry is dt f - dt^2 (K v) of one tet (k_element)
+        double e3[3] = {0, 0, 0};
         const int k1 = live ? nstart[row + 1] : 0;
         for (int k = live ? nstart[row] + lane : 0; k < k1; k += RL)
         {
             const double* e = corner + static_cast<size_t>(k) * N_CORNER;
-            const double f0 = e[0], f1 = e[1], f2 = e[2];
-            if (fin(f0) && fin(f1) && fin(f2))  // non-finite pushes are skipped (SolverData.cpp:145-154)
-            {
-                f[0] += f0;
-                f[1] += f1;
-                f[2] += f2;
-            }
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                kv[i] += e[N_KV + i];
+                e3[i] += e[i];
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int o = RL / 2; o > 0; o >>= 1)
-            {
-                f[i] += __shfl_xor(f[i], o, RL);
-                kv[i] += __shfl_xor(kv[i], o, RL);
-            }
+                e3[i] += __shfl_xor(e3[i], o, RL);
         if (!live || lane != 0)
             continue;
         const double m = mrow[row];
@@ -527,7 +581,7 @@ __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__
         double b[3], z[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            b[i] = (f[i] + gv[i] * m) * dt - kv[i] * dt2 + (crhs ? crhs[3 * row + i] : 0.0);
+            b[i] = e3[i] + (gv[i] * m) * dt + (crhs ? crhs[3 * row + i] : 0.0);
         if (!(fin(b[0]) && fin(b[1]) && fin(b[2])))
             flag |= 1;
         apply_minv(minv, static_cast<size_t>(nd), row, b, z);
@@ -641,12 +695,11 @@ void launch_reset(const DeviceView& v, double tol, int maxIt)
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(NT), 0, v.stream, v.gs, v.G, v.active, v.kbase, tol, maxIt);
 }
 
-void launch_element(const DeviceView& v)
+void launch_element(const DeviceView& v, double dt, int implicit)
 {
     if (v.T > 0)
         hipLaunchKernelGGL(k_element, dim3(xcd_grid((v.T + NT - 1) / NT)), dim3(NT), 0, v.stream, v.T, v.tets, v.live,
-                           v.fem, v.prm,
-                           v.x, v.v, v.cpos, v.basis, v.corner);
+                           v.fem, v.prm, v.x, v.v, v.cpos, v.basis, v.corner, dt, implicit);
 }
 
 template <class M>

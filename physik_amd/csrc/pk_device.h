@@ -385,7 +385,7 @@ void launch_gdot(const DeviceView& v, double* prm);
 // Every tet's element outputs at the current state as 114-double records
 // (10 blocks, 4 forces, 4 K v), for inspection.
 void launch_element_records(const DeviceView& v, double* out);
-void launch_element(const DeviceView& v);
+void launch_element(const DeviceView& v, double dt, int implicit);
 template <class M> void launch_assemble(const DeviceView& v, double dt);
 template <class M> void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt);
 void launch_integrate(const DeviceView& v, double dt, int commit);

@@ -598,7 +598,7 @@ static int run_assembly(pk_system* s, double dt, double tol, int maxIt, bool wit
     launch_reset(dv, tol, maxIt);
     {
         Bracket br(s, PK_K_ELEMENT);
-        launch_element(dv);
+        launch_element(dv, dt, s->implicit);
     }
     if (s->nconn > 0 || s->nsph > 0)
     {

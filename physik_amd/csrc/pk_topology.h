@@ -35,9 +35,10 @@ enum TetParam
 //           these with the closed form the element kernel also uses, and
 //           finds all of a contribution's basis values in the tet's one or
 //           two cache lines.
-//   corner  6 doubles per (dynamic row, tet, corner) in the order of the row
-//           lists: the element force f_a, then sum_b (R K_ab R^T) v_b, written
-//           at cpos[a * T + t], so the right-hand side reads a contiguous run.
+//   corner  3 doubles per (dynamic row, tet, corner) in the order of the row
+//           lists, written at cpos[a * T + t] so that the right-hand side
+//           reads a contiguous run: implicit, dt f_a - dt^2 sum_b (R K_ab R^T)
+//           v_b (the force dropped when not finite); explicit, f_a.
 // Reverse mode keeps its per-tet corner terms in planes adj[k * T + t]:
 // dPhi/dx_a (k = 3a + i), then dPhi/dv_a (k = 12 + 3a + i).
 enum ElemLayout
@@ -45,8 +46,7 @@ enum ElemLayout
     EB_RG = 0,
     EB_RRT = 12,
     EB_COUNT = 18,
-    N_CORNER = 6,     // doubles per corner entry
-    N_KV = 3,         // offset of K v in a corner entry
+    N_CORNER = 3,     // doubles per corner entry
     EP_ADJ_X = 0,
     EP_ADJ_V = 12,
     EP_ADJ_COUNT = 24

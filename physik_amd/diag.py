@@ -87,6 +87,7 @@ def main():
     n_dir, ms_dir = s.kernel_time(N.K_DIRECTION)
     n_as, ms_as = s.kernel_time(N.K_ASSEMBLE)
     n_el, ms_el = s.kernel_time(N.K_ELEMENT)
+    n_rhs, ms_rhs = s.kernel_time(N.K_RHS)
     print(json.dumps({"copy_bytes": nbytes, "copy_ms": ms.value, "copy_GBps": 2 * nbytes / ms.value / 1e6,
                       "workload": args.workload, "steps": args.steps, "cg_iterations": int(it.max()),
                       "nnzb_full": s.nnzb_full, "dynamic_rows": s.nd, "tets": s.t, "nodes": s.n, "bodies": s.groups,
@@ -94,7 +95,8 @@ def main():
                       "spmv_body_launches": body_launches, "shape": s.get_tune(),
                       "staging": s.get_staging(), "spmv_us": 1e3 * ms_spmv / max(n_spmv, 1), "update_us": 1e3 * ms_upd / max(n_upd, 1),
                       "direction_us": 1e3 * ms_dir / max(n_dir, 1), "assemble_us": 1e3 * ms_as / max(n_as, 1),
-                      "element_us": 1e3 * ms_el / max(n_el, 1), **extra}), flush=True)
+                      "element_us": 1e3 * ms_el / max(n_el, 1), "rhs_us": 1e3 * ms_rhs / max(n_rhs, 1), **extra}),
+          flush=True)
 
 
 if __name__ == "__main__":
