@@ -312,9 +312,28 @@ __global__ __launch_bounds__(NT) void k_gdot(int T, double* __restrict__ prm)
     }
 }
 
-// Block R K_ab R^T of local pair p of tet t from its basis record rec.
-__device__ inline void element_block_from(const double* __restrict__ rec, const double* __restrict__ prm, size_t TT,
-                                          int t, int p, double* K)
+// The stiffness record of each tet from the parameter planes (at creation,
+// after k_gdot, and whenever lambda and mu change).
+__global__ __launch_bounds__(NT) void k_stiff_records(int T, const double* __restrict__ prm, double* __restrict__ kdat)
+{
+    const int t = blockIdx.x * NT + threadIdx.x;
+    if (t >= T)
+        return;
+    const size_t TT = static_cast<size_t>(T);
+    const double vol = prm[P_VOL * TT + t];
+    const double vm = vol * prm[P_MU * TT + t];
+    double* k = kdat + static_cast<size_t>(t) * KD_COUNT;
+    k[KD_VLAM] = vol * prm[P_LAMBDA * TT + t];
+    k[KD_VMU] = vm;
+#pragma unroll
+    for (int p = 0; p < 10; ++p)
+        k[KD_VMUG + p] = vm * prm[(P_GDOT + p) * TT + t];
+}
+
+// Block R K_ab R^T of local pair p of a tet from its basis record rec and
+// its stiffness record kd.
+__device__ inline void element_block_from(const double* __restrict__ rec, const double* __restrict__ kd, int p,
+                                          double* K)
 {
     int a, b;
     pair_of(p, a, b);
@@ -330,15 +349,15 @@ __device__ inline void element_block_from(const double* __restrict__ rec, const 
     const double2 rr0 = rr[0], rr1 = rr[1], rr2 = rr[2];
     const double r00 = rr0.x, r01 = rr0.y, r02 = rr1.x, r11 = rr1.y, r12 = rr2.x, r22 = rr2.y;
     const double RRt[9] = {r00, r01, r02, r01, r11, r12, r02, r12, r22};
-    rotated_block(rga, rgb, RRt, prm[P_VOL * TT + t], prm[P_LAMBDA * TT + t], prm[P_MU * TT + t],
-                  prm[(P_GDOT + p) * TT + t], K);
+    const double2 f = *reinterpret_cast<const double2*>(kd + KD_VLAM);
+    rotated_block_scaled(rga, rgb, RRt, f.x, f.y, kd[KD_VMUG + p], K);
 }
 
-// Block R K_ab R^T of local pair p of tet t from the basis planes.
-__device__ inline void element_block(const double* __restrict__ basis, const double* __restrict__ prm, size_t TT,
-                                     int t, int p, double* K)
+// Block R K_ab R^T of local pair p of tet t from the basis and stiffness records.
+__device__ inline void element_block(const double* __restrict__ basis, const double* __restrict__ kdat, int t, int p,
+                                     double* K)
 {
-    element_block_from(basis + static_cast<size_t>(t) * EB_COUNT, prm, TT, t, p, K);
+    element_block_from(basis + static_cast<size_t>(t) * EB_COUNT, kdat + static_cast<size_t>(t) * KD_COUNT, p, K);
 }
 
 // The inspection sink: everything into registers.
@@ -375,7 +394,8 @@ struct RecordSink
 __global__ __launch_bounds__(NT) void k_element_records(int T, const int4* __restrict__ tets,
                                                         const unsigned char* __restrict__ live,
                                                         const unsigned char* __restrict__ fem,
-                                                        const double* __restrict__ prm, const double* __restrict__ x,
+                                                        const double* __restrict__ prm,
+                                                        const double* __restrict__ kdat, const double* __restrict__ x,
                                                         const double* __restrict__ v, double* __restrict__ out)
 {
     const int t = blockIdx.x * NT + threadIdx.x;
@@ -389,7 +409,7 @@ __global__ __launch_bounds__(NT) void k_element_records(int T, const int4* __res
     {
         double K[9];
         if (ok)
-            element_block_from(o.basisv, prm, TT, t, p, K);
+            element_block_from(o.basisv, kdat + static_cast<size_t>(t) * KD_COUNT, p, K);
         for (int e = 0; e < 9; ++e)
             r[EREC_K + 9 * p + e] = ok ? K[e] : 0.0;
     }
@@ -409,30 +429,34 @@ __global__ __launch_bounds__(NT) void k_element_records(int T, const int4* __res
 // wave idle while its diagonal slots finish.
 constexpr int kSlotLanes = 4;
 
+// Index of the stored local pair (a, b), a <= b (pair_of's inverse).
+__device__ constexpr int kPairOf[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 8}, {3, 6, 8, 9}};
+
+// Lanes per dynamic row in the assembly gather: a node's star holds ~24
+// tets, each of the row's ~7 off-diagonal upper slots ~5.
+constexpr int kRowGatherLanes = 16;
+
+// A = dt^2 K (+ contacts) of off-diagonal upper slot s from its contribution
+// list, summed by the 4 lanes of a sub-group (lane-strided, then shuffles: a
+// fixed order) and written by the sub-group's first lane to both triangles
+// (SolverData.cpp:422-450).
 template <class M>
-__global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict__ cstart, const int* __restrict__ clist,
-                                                 const int* __restrict__ uFull,
-                                                 const int* __restrict__ uMirror, const int* __restrict__ slotRow,
-                                                 const int* __restrict__ rowGroup, const double* __restrict__ basis,
-                                                 const double* __restrict__ prm, int T,
-                                                 const double* __restrict__ mrow, double dt2, M* __restrict__ vals,
-                                                 long long nnzf, M* __restrict__ minv, int nd, GroupState* gs,
-                                                 int upperOnly, const double* __restrict__ cslot)
+__device__ inline void gather_slot(int s, int lane4, const int* __restrict__ cstart, const int* __restrict__ clist,
+                                   const int* __restrict__ uFull, const int* __restrict__ uMirror,
+                                   const double* __restrict__ basis, const double* __restrict__ kdat,
+                                   double dt2, M* __restrict__ vals, size_t F, int upperOnly,
+                                   const double* __restrict__ cslot, int& failed)
 {
-    const int s = (xcd_block() * NT + static_cast<int>(threadIdx.x)) / kSlotLanes;
-    const int lane = threadIdx.x % kSlotLanes;
-    if (s >= nnzu)
-        return;  // whole lane groups
     double a[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e)
         a[e] = 0.0;
     const int k1 = cstart[s + 1];
-    for (int k = cstart[s] + lane; k < k1; k += kSlotLanes)
+    for (int k = cstart[s] + lane4; k < k1; k += kSlotLanes)
     {
         const int c = clist[k];
         double b[9];
-        element_block(basis, prm, static_cast<size_t>(T), c >> 5, (c >> 1) & 15, b);
+        element_block(basis, kdat, c >> 5, (c >> 1) & 15, b);
         if (c & 1)
         {
 #pragma unroll
@@ -448,19 +472,15 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
                 a[e] += b[e];
         }
     }
-    // the group's partial sums, combined in a fixed order
 #pragma unroll
     for (int e = 0; e < 9; ++e)
-    {
 #pragma unroll
         for (int o = kSlotLanes / 2; o > 0; o >>= 1)
             a[e] += __shfl_xor(a[e], o, kSlotLanes);
-    }
-    if (lane != 0)
+    if (lane4 != 0)
         return;
     if (cslot)
     {
-        // point connections' k w_a w_b I blocks (PointConnection.cpp:110-128)
         const double cs = cslot[s];
         a[0] += cs;
         a[4] += cs;
@@ -469,36 +489,95 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
 #pragma unroll
     for (int e = 0; e < 9; ++e)
         a[e] *= dt2;
-    const int mirror = uMirror[s];
-    const int r = slotRow[s];
-    if (mirror < 0)
-    {
-        const double m = mrow[r];
-        a[0] = m + a[0];
-        a[4] = m + a[4];
-        a[8] = m + a[8];
-    }
-    const size_t F = static_cast<size_t>(nnzf);
     // full storage: the block at its full index and its transpose at the
     // mirror; upper-only storage: the record of the slot itself
     const int full = upperOnly ? s : uFull[s];
 #pragma unroll
     for (int e = 0; e < 9; ++e)
         vals[upperOnly ? upper_at(full, e) : BsrStore<M>::at(F, full, e)] = static_cast<M>(a[e]);
-    if (mirror >= 0 && !upperOnly)
+    if (!upperOnly)
     {
+        const int mirror = uMirror[s];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
                 vals[BsrStore<M>::at(F, mirror, 3 * i + j)] = static_cast<M>(a[3 * j + i]);
     }
-    const bool ok = fin9(a);
-    const int g = rowGroup[r];
-    if (!ok)
-        gs[g].failed = 1;  // a non-finite stiffness block rejects the assembly
-    if (mirror < 0)
+    if (!fin9(a))
+        failed = 1;  // a non-finite stiffness block rejects the assembly
+}
+
+// The assembly gather, atomic-free: kRowGatherLanes lanes per dynamic row r.
+// First the diagonal block over the row's corner list (all lanes, in tet
+// order): A_rr = m I + dt^2 (sum_a R K_aa R^T + contacts) and its block-Jacobi
+// inverse (SolverData.cpp:407-420, 260-306); then the row's off-diagonal
+// upper slots, four at a time, each by a sub-group of 4 lanes. A row's blocks
+// all come from the tets around node r, so they are read while in L2. A
+// non-finite block or a refused inverse marks the body's solve failed.
+template <class M>
+__global__ __launch_bounds__(NT) void k_assemble(int nnzu, int nd, const int* __restrict__ cstart,
+                                                 const int* __restrict__ clist, const int* __restrict__ nstart,
+                                                 const int* __restrict__ nlist, const int* __restrict__ diagSlot,
+                                                 const int* __restrict__ uFull, const int* __restrict__ uMirror,
+                                                 const int* __restrict__ rowGroup, const double* __restrict__ basis,
+                                                 const double* __restrict__ kdat,
+                                                 const double* __restrict__ mrow, double dt2, M* __restrict__ vals,
+                                                 long long nnzf, M* __restrict__ minv, GroupState* gs, int upperOnly,
+                                                 const double* __restrict__ cslot)
+{
+    const int row = (xcd_block() * NT + static_cast<int>(threadIdx.x)) / kRowGatherLanes;
+    const int lane = threadIdx.x % kRowGatherLanes;
+    if (row >= nd)
+        return;  // whole lane groups
+    const size_t F = static_cast<size_t>(nnzf);
+    double a[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+        a[e] = 0.0;
+    const int k1 = nstart[row + 1];
+    for (int k = nstart[row] + lane; k < k1; k += kRowGatherLanes)
     {
+        const int c = nlist[k];
+        double b[9];
+        element_block(basis, kdat, c >> 2, kPairOf[c & 3][c & 3], b);
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            a[e] += b[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+#pragma unroll
+        for (int o = kRowGatherLanes / 2; o > 0; o >>= 1)
+            a[e] += __shfl_xor(a[e], o, kRowGatherLanes);
+    const int sd = diagSlot[row];
+    const int se = row + 1 < nd ? diagSlot[row + 1] : nnzu;
+    const int g = rowGroup[row];
+    int failed = 0;
+    if (lane == 0)
+    {
+        if (cslot)
+        {
+            // point connections' and contacts' k w_a w_b I (PointConnection.cpp:110-128)
+            const double cs = cslot[sd];
+            a[0] += cs;
+            a[4] += cs;
+            a[8] += cs;
+        }
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            a[e] *= dt2;
+        const double m = mrow[row];
+        a[0] = m + a[0];
+        a[4] = m + a[4];
+        a[8] = m + a[8];
+        const int full = upperOnly ? sd : uFull[sd];
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            vals[upperOnly ? upper_at(full, e) : BsrStore<M>::at(F, full, e)] = static_cast<M>(a[e]);
+        const bool ok = fin9(a);
+        if (!ok)
+            failed = 1;
         double mi[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
         if (gs[g].jacobi)
         {
@@ -514,12 +593,20 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, const int* __restrict
                 pok = fin9(mi);
             }
             if (!pok)
-                gs[g].failed = 1;  // BuildInversePreconditioner fails the solve
+                failed = 1;  // BuildInversePreconditioner fails the solve
         }
 #pragma unroll
         for (int e = 0; e < 9; ++e)
-            minv[static_cast<size_t>(e) * nd + r] = static_cast<M>(mi[e]);
+            minv[static_cast<size_t>(e) * nd + row] = static_cast<M>(mi[e]);
     }
+    // the row's off-diagonal slots (sd + 1 .. se - 1), sub-group lane / 4 on
+    // slots sd + 1 + sub, sd + 5 + sub, ...
+    const int sub = lane / kSlotLanes;
+    for (int s = sd + 1 + sub; s < se; s += kRowGatherLanes / kSlotLanes)
+        gather_slot(s, lane % kSlotLanes, cstart, clist, uFull, uMirror, basis, kdat, dt2, vals, F, upperOnly,
+                    cslot, failed);
+    if (failed)
+        gs[g].failed = 1;
 }
 
 // Lanes per dynamic row in the right-hand side gather (a node has ~20 corners).
@@ -536,11 +623,15 @@ __device__ inline void apply_minv(const M* __restrict__ minv, size_t nd, int row
                static_cast<double>(minv[(3 * i + 2) * nd + row]) * r[2];
 }
 
+// b = dt (f + m g) - dt^2 K v per dynamic row from its corner entries (each
+// dt f - dt^2 (K v) of one tet, k_element), then the CG start x = 0, r = b,
+// z = d = M^-1 r with the (r.z, r.r) partials (ConjugateGradientSolver.cpp:
+// 151-164).
 template <class M>
 __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__ nstart,
                                             const double* __restrict__ corner,
                                             const double* __restrict__ mrow, double gx, double gy, double gz,
-                                            double dt, double dt2, const M* __restrict__ minv, int nd,
+                                            double dt, const M* __restrict__ minv, int nd,
                                             double* __restrict__ xs, double* __restrict__ rv, M* __restrict__ zv,
                                             M* __restrict__ d0, Partial* __restrict__ part,
                                             const double* __restrict__ crhs)
@@ -559,7 +650,6 @@ __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__
     {
         const int row = row0 + static_cast<int>(threadIdx.x) / RL;
         const bool live = row < tl.rowEnd;
-        // each corner entry is dt f - dt^2 (K v) of one tet (k_element)
         double e3[3] = {0, 0, 0};
         const int k1 = live ? nstart[row + 1] : 0;
         for (int k = live ? nstart[row] + lane : 0; k < k1; k += RL)
@@ -706,21 +796,20 @@ template <class M>
 void launch_assemble(const DeviceView& v, double dt)
 {
     const int n = static_cast<int>(v.nnzu);
-    if (n > 0)
-        hipLaunchKernelGGL((k_assemble<M>), dim3(xcd_grid((static_cast<long long>(n) * kSlotLanes + NT - 1) / NT)),
-                           dim3(NT), 0, v.stream, n, v.cstart,
-                           v.clist,
-                           v.uFull, v.uMirror, v.slotRow, v.rowGroup, v.basis, v.prm, v.T, v.mrow, dt * dt,
-                           static_cast<M*>(v.vals), v.nnzf, static_cast<M*>(v.minv), v.nd, v.gs, v.upper,
-                           v.slotAdd);
+    if (n <= 0 || v.nd <= 0)
+        return;
+    const long long groups = (static_cast<long long>(v.nd) * kRowGatherLanes + NT - 1) / NT;
+    hipLaunchKernelGGL((k_assemble<M>), dim3(xcd_grid(groups)), dim3(NT), 0, v.stream, n, v.nd, v.cstart, v.clist,
+                       v.nstart, v.nlist, v.diagSlot, v.uFull, v.uMirror, v.rowGroup, v.basis, v.kdat, v.mrow,
+                       dt * dt, static_cast<M*>(v.vals), v.nnzf, static_cast<M*>(v.minv), v.gs, v.upper, v.slotAdd);
 }
 
 template <class M>
 void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt)
 {
     if (v.vec.count > 0)
-        hipLaunchKernelGGL((k_rhs<M>), dim3(grid_of(v.vec)), dim3(NT), 0, v.stream, v.vec, v.nstart,
-                           v.corner, v.mrow, gx, gy, gz, dt, dt * dt, static_cast<const M*>(v.minv), v.nd, v.xs, v.r, static_cast<M*>(v.z),
+        hipLaunchKernelGGL((k_rhs<M>), dim3(grid_of(v.vec)), dim3(NT), 0, v.stream, v.vec, v.nstart, v.corner, v.mrow,
+                           gx, gy, gz, dt, static_cast<const M*>(v.minv), v.nd, v.xs, v.r, static_cast<M*>(v.z),
                            static_cast<M*>(v.d[0]), v.pb[1], v.rowAdd);
 }
 
@@ -1092,6 +1181,12 @@ void launch_export_frame(hipStream_t stream, long long n3, const double* x, cons
     hipLaunchKernelGGL(k_export_frame, dim3(static_cast<unsigned>(grid)), dim3(NT), 0, stream, n3, x, v, out);
 }
 
+void launch_stiff_records(const DeviceView& v)
+{
+    if (v.T > 0)
+        hipLaunchKernelGGL(k_stiff_records, dim3((v.T + NT - 1) / NT), dim3(NT), 0, v.stream, v.T, v.prm, v.kdat);
+}
+
 void launch_gdot(const DeviceView& v, double* prm)
 {
     if (v.T > 0)
@@ -1102,7 +1197,7 @@ void launch_element_records(const DeviceView& v, double* out)
 {
     if (v.T > 0)
         hipLaunchKernelGGL(k_element_records, dim3((v.T + NT - 1) / NT), dim3(NT), 0, v.stream, v.T, v.tets, v.live,
-                           v.fem, v.prm, v.x, v.v, out);
+                           v.fem, v.prm, v.kdat, v.x, v.v, out);
 }
 
 template void launch_assemble<double>(const DeviceView&, double);

@@ -287,11 +287,13 @@ struct DeviceView
     const unsigned char* live;
     const unsigned char* fem;  // per-tet FEM model (FemModel values), or nullptr: all corotational
     const double* prm;
+    double* kdat;  // per-tet stiffness records (KD_COUNT doubles each)
     const int* cstart;
     const int* clist;
     const int* uFull;
     const int* uMirror;
     const int* slotRow;
+    const int* diagSlot;  // nd: the upper slot of each row's diagonal block
     const int* rowGroup;
     const int* fstart;
     const int* fcol;
@@ -385,6 +387,7 @@ void launch_gdot(const DeviceView& v, double* prm);
 // Every tet's element outputs at the current state as 114-double records
 // (10 blocks, 4 forces, 4 K v), for inspection.
 void launch_element_records(const DeviceView& v, double* out);
+void launch_stiff_records(const DeviceView& v);
 void launch_element(const DeviceView& v, double dt, int implicit);
 template <class M> void launch_assemble(const DeviceView& v, double dt);
 template <class M> void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt);

@@ -105,17 +105,17 @@ __device__ inline void shape_gradients(const double* Dinv, double g[4][3])
 
 __device__ inline double gdot(const double* ga, const double* gb) { return ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]; }
 
-// R K_ab R^T = V (lam (R g_a)(R g_b)^T + mu (R g_b)(R g_a)^T + mu (g_a.g_b) R R^T),
-// the closed form of R Ke_ab R^T (FEMModel.cpp:312-375); RRt is full 3x3.
-__device__ inline void rotated_block(const double* rga, const double* rgb, const double* RRt, double vol, double lam,
-                                     double mu, double gab, double* K)
+// R K_ab R^T from the premultiplied factors vl = V lam, vm = V mu and
+// vmg = V mu (g_a . g_b) (the tet's stiffness record); RRt is full 3x3.
+__device__ inline void rotated_block_scaled(const double* rga, const double* rgb, const double* RRt, double vl,
+                                            double vm, double vmg, double* K)
 {
-    const double mg = mu * gab;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            K[3 * i + j] = vol * (lam * rga[i] * rgb[j] + mu * rgb[i] * rga[j] + mg * RRt[3 * i + j]);
+            K[3 * i + j] = vl * rga[i] * rgb[j] + vm * rgb[i] * rga[j] + vmg * RRt[3 * i + j];
 }
+
 
 }  // namespace pk

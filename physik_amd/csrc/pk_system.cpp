@@ -129,10 +129,10 @@ struct pk_system
     pk::DevBuf<int4> tets;
     pk::DevBuf<unsigned char> live, tact;
     pk::DevBuf<unsigned char> fem;  // per-tet FEM model; not allocated while every tet is corotational
-    pk::DevBuf<double> prm, x, v, basis, corner, adj, mrow, xs, r, pa;
+    pk::DevBuf<double> prm, kdat, x, v, basis, corner, adj, mrow, xs, r, pa;
     pk::DevBuf<char> minv;  // storage type, like the CG vectors
     pk::DevBuf<int> cstart, clist, uFull, uMirror, slotRow, rowGroup, fstart, fcol, nstart, nlist, rowNode, active;
-    pk::DevBuf<int> astart, alist, nodeRow, nodeGroup, cpos;
+    pk::DevBuf<int> astart, alist, nodeRow, nodeGroup, cpos, diagSlot;
     pk::DevBuf<int2> rowptr, lower;  // upper-only storage: row pointers and lower blocks
     pk::DevBuf<int> ucol;
     pk::DevBuf<char> vals, z, d0, d1, q;  // element type set by the precision
@@ -278,11 +278,13 @@ struct pk_system
         dv.fem = fem.p;
         dv.tact = tact.p;
         dv.prm = prm.p;
+        dv.kdat = kdat.p;
         dv.cstart = cstart.p;
         dv.clist = clist.p;
         dv.uFull = uFull.p;
         dv.uMirror = uMirror.p;
         dv.slotRow = slotRow.p;
+        dv.diagSlot = diagSlot.p;
         dv.rowGroup = rowGroup.p;
         dv.fstart = fstart.p;
         dv.upper = (precision == PK_FP32_UPPER || precision == PK_FP64_UPPER) ? 1 : 0;
@@ -590,29 +592,39 @@ static void harvest_profile(pk_system* s)
     p.used = 0;
 }
 
-// reset -> element -> assemble (-> rhs with the CG start when withRhs).
+// reset -> element (-> contacts) -> off-diagonal slots -> rows. `mode`:
+// kAsmExplicit the element forces only (explicit integration);
+// kAsmImplicit the matrix, the right-hand side and the CG start;
+// kAsmMatrix the matrix only (a reverse step brings its own right-hand side).
+enum AssemblyMode
+{
+    kAsmExplicit = 0,
+    kAsmImplicit = 1,
+    kAsmMatrix = 2
+};
+
 template <class M>
-static int run_assembly(pk_system* s, double dt, double tol, int maxIt, bool withRhs)
+static int run_assembly(pk_system* s, double dt, double tol, int maxIt, int mode)
 {
     const DeviceView dv = s->view();
     launch_reset(dv, tol, maxIt);
     {
         Bracket br(s, PK_K_ELEMENT);
-        launch_element(dv, dt, s->implicit);
+        launch_element(dv, dt, mode != kAsmExplicit);
     }
     if (s->nconn > 0 || s->nsph > 0)
     {
         Bracket br(s, PK_K_CONTACT);
-        launch_connections(dv, dt, s->implicit);
-        launch_spheres(dv, dt, s->implicit);
+        launch_connections(dv, dt, mode != kAsmExplicit);
+        launch_spheres(dv, dt, mode != kAsmExplicit);
     }
-    if (s->topo.nd == 0)
+    if (s->topo.nd == 0 || mode == kAsmExplicit)
         return PK_OK;
     {
         Bracket br(s, PK_K_ASSEMBLE);
         launch_assemble<M>(dv, dt);
     }
-    if (withRhs)
+    if (mode == kAsmImplicit)
     {
         Bracket br(s, PK_K_RHS);
         launch_rhs<M>(dv, s->gravity[0], s->gravity[1], s->gravity[2], dt);
@@ -785,7 +797,7 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
 template <class M>
 static int run_substep(pk_system* s, double dt, int* failedOut = nullptr)
 {
-    int rc = run_assembly<M>(s, dt, s->tol, s->maxIt, true);
+    int rc = run_assembly<M>(s, dt, s->tol, s->maxIt, s->implicit ? kAsmImplicit : kAsmExplicit);
     if (rc != PK_OK || s->topo.nd == 0)
         return rc;
     const DeviceView dv = s->view();
@@ -1011,6 +1023,19 @@ static int setup_device(pk_system* s)
         }
     }
     if ((rc = s->slotRow.upload(slotRow, st)) != PK_OK) return rc;
+    // the diagonal slot of each row: the row's first upper slot (its column
+    // is the row itself, the smallest column >= the row)
+    std::vector<int> diagSlot(static_cast<size_t>(t.nd));
+    for (int r = 0; r < t.nd; ++r)
+    {
+        if (t.ustart[r] >= t.ustart[r + 1] || t.ucol[t.ustart[r]] != r)
+        {
+            set_error("pk_system_create: a dynamic row has no diagonal block in its pattern");
+            return PK_ERR_ARG;
+        }
+        diagSlot[r] = t.ustart[r];
+    }
+    if ((rc = s->diagSlot.upload(diagSlot, st)) != PK_OK) return rc;
     if ((rc = s->rowGroup.upload(rowGroup, st)) != PK_OK) return rc;
     if ((rc = s->fstart.upload(t.fstart, st)) != PK_OK) return rc;
     if ((rc = s->fcol.upload(t.fcol, st)) != PK_OK) return rc;
@@ -1059,6 +1084,8 @@ static int setup_device(pk_system* s)
     }
     if ((rc = alloc_work(s)) != PK_OK) return rc;
     launch_gdot(s->view(), s->prm.p);
+    if ((rc = s->kdat.alloc(static_cast<size_t>(t.T) * KD_COUNT)) != PK_OK) return rc;
+    launch_stiff_records(s->view());
     if (is_f32(s) || is_upper(s))
         launch_record_columns(s->view(), is_f32(s));
     std::vector<double> zero(static_cast<size_t>(t.N) * 3, 0.0);
@@ -1426,6 +1453,8 @@ int pk_system_set_young(pk_system* s, const double* young)
     // lambda and mu are adjacent planes of the parameter block
     PK_HIP(hipMemcpyAsync(s->prm.p + P_LAMBDA * T, t.prm.data() + P_LAMBDA * T, 2 * T * sizeof(double),
                           hipMemcpyHostToDevice, s->stream));
+    launch_stiff_records(s->view());
+    PK_HIP(hipGetLastError());
     PK_HIP(hipStreamSynchronize(s->stream));
     return PK_OK;
 }
@@ -1513,7 +1542,7 @@ static int reverse_substep(pk_system* s, double dt, int j, double adjTol, int ad
         launch_reset(sv, adjTol, adjMaxIt);
         launch_restore_failed(sv, s->segFailed.p + static_cast<size_t>(j) * s->topo.G);
     }
-    else if ((rc = run_assembly<M>(s, dt, adjTol, adjMaxIt, false)) != PK_OK)
+    else if ((rc = run_assembly<M>(s, dt, adjTol, adjMaxIt, kAsmMatrix)) != PK_OK)
         return rc;
     const DeviceView dv = s->view();
     launch_adj_rhs<M>(dv, ok, s->xbar.p, s->vbar.p, dt);
@@ -2233,8 +2262,8 @@ int pk_system_assemble(pk_system* s, double dt)
 {
     if (!s)
         return PK_ERR_ARG;
-    const int rc = is_f32(s) ? run_assembly<float>(s, dt, s->tol, s->maxIt, true)
-                                                   : run_assembly<double>(s, dt, s->tol, s->maxIt, true);
+    const int rc = is_f32(s) ? run_assembly<float>(s, dt, s->tol, s->maxIt, kAsmImplicit)
+                             : run_assembly<double>(s, dt, s->tol, s->maxIt, kAsmImplicit);
     if (rc != PK_OK)
         return rc;
     PK_HIP(hipStreamSynchronize(s->stream));

@@ -52,6 +52,19 @@ enum ElemLayout
     EP_ADJ_COUNT = 24
 };
 
+// Constant stiffness factors of one tet, one 96-byte record per tet
+// kdat[t * KD_COUNT + k] (built on the device from the parameter planes, again
+// when Young's moduli change): V lambda, V mu, then V mu (g_a . g_b) for the
+// 10 stored pairs. The slot gather reads a contribution's factors from one
+// record instead of four parameter planes.
+enum StiffRecord
+{
+    KD_VLAM = 0,
+    KD_VMU = 1,
+    KD_VMUG = 2,
+    KD_COUNT = 12
+};
+
 // Inspection record of one tet (pk_system_get_elements / get_rotations):
 // the 10 blocks (row-major), forces, K v, R (row-major) and the polar kind.
 enum ElemRecord
