@@ -110,20 +110,21 @@ def roofline(s, name, prec_name):
     esize = 8 if s.precision in (N.PK_FP64, N.PK_FP64_UPPER) else 4
     upper = s.precision in (N.PK_FP32_UPPER, N.PK_FP64_UPPER)
     blocks, ublocks, rows, body_launches = s.spmv_work()
-    _, staged_rows, segmented, fused = s.get_staging()
+    _, staged_rows, segmented = s.get_staging()
     ub = ublocks if upper else None
-    total = spmv_bytes(blocks, rows, body_launches, esize, ub, fused)
-    fmt = spmv_format_bytes(blocks, rows, body_launches, esize, ub, fused)
-    full = spmv_bytes(s.nnzb_full, s.nd, s.groups, esize, s.nnzb_upper if upper else None, fused)
+    total = spmv_bytes(blocks, rows, body_launches, esize, ub)
+    fmt = spmv_format_bytes(blocks, rows, body_launches, esize, ub)
+    full = spmv_bytes(s.nnzb_full, s.nd, s.groups, esize, s.nnzb_upper if upper else None)
     # the same work in full storage, for comparison across formats
-    full_format = spmv_bytes(blocks, rows, body_launches, esize, None, fused)
+    full_format = spmv_bytes(blocks, rows, body_launches, esize, None)
     sec = ms_spmv * 1e-3
     achieved = total / sec / 1e9 if n_spmv and ms_spmv > 0 else 0.0
     measured = measured_traffic(name, prec_name)
     per = max(n_spmv, 1)
     return {
         "bound": "hbm",
-        "kernel": "k_spmv (q = A d_k, d.q partial per tile; d_k = z_k + beta d_k-1 formed in it when fused)",
+        "kernel": ("k_spmv_upper (sliced upper-only storage, one lane per row)" if upper else "k_spmv")
+                  + " (q = A d_k and the d.q partial per tile)",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -135,11 +136,11 @@ def roofline(s, name, prec_name):
         # the rate the kernel actually moves data at (a plain copy reaches ~5.3 TB/s)
         "traffic_rate_GBps": (None if measured is None or not n_spmv or ms_spmv <= 0
                               else measured["bytes"] / (ms_spmv / per * 1e-3) / 1e9),
-        "storage": "upper triangle" if upper else "full",
-        "staging": {"rows": staged_rows, "segments": segmented, "direction_fused": fused},
+        "storage": "upper triangle, sliced" if upper else "full",
+        "staging": {"rows": staged_rows, "segments": segmented},
         "bytes_per_launch": total / per,
         # what the storage format adds to the algorithmic bytes (upper
-        # storage: the mirror list, the second row pointer, FP64 column words)
+        # storage: the mirror list and FP64 column words, less the row pointers)
         "overhead_bytes_per_launch": (fmt - total) / per,
         "format_rate_GBps": fmt / sec / 1e9 if n_spmv and ms_spmv > 0 else 0.0,
         "full_storage_bytes_per_launch": full_format / per,

@@ -122,11 +122,15 @@ void pk_system_destroy(pk_system* sys);
 int pk_system_sizes(const pk_system* sys, int* nodes, int* tets, int* groups, int* dynamic_rows,
                     long long* nnzb_upper, long long* nnzb_full);
 
-/* Launch shape of the CG kernels: lanes per BSR row in the SpMV (4, 8, 16 or
- * 32), rows per SpMV workgroup, rows per vector-kernel workgroup, and rows
+/* Launch shape of the CG kernels: lanes per BSR row in the SpMV (1, 4, 8, 16
+ * or 32), rows per SpMV workgroup, rows per vector-kernel workgroup, and rows
  * each SpMV lane group loads before reducing (1, 2, or 4 up to 16 lanes).
- * Results of a body do not depend on the other bodies of the system, only on
- * these numbers. A default is chosen from the row lengths at creation; the
+ * Upper-only storage (PK_FP32_UPPER, PK_FP64_UPPER) runs one lane per row
+ * over slices of 64 rows: lanes_per_row must be 1, spmv_rows a multiple of
+ * 64, and unroll is not used (any accepted value runs the same). Results
+ * of a body do not depend on the other bodies of the system, only on these
+ * numbers; upper-only storage gives the results of full storage at one lane
+ * per row. A default is chosen from the row lengths at creation; the
  * autotuner (physik_amd/autotune.py) searches them. */
 int pk_system_tune(pk_system* sys, int lanes_per_row, int spmv_rows, int vector_rows, int unroll);
 int pk_system_get_tune(const pk_system* sys, int* lanes_per_row, int* spmv_rows, int* vector_rows, int* unroll);
@@ -137,26 +141,11 @@ int pk_system_get_tune(const pk_system* sys, int* lanes_per_row, int* spmv_rows,
  * tile's column window [first, last] fits, tiles stage their windows;
  * otherwise they stage up to four segments of rows (the columns cut at their
  * three widest gaps), and tiles that still need more than 20 KB gather from
- * global memory. mode 2 does the same and, when every tile stages, also folds
- * the direction update into the SpMV: each workgroup takes the stop rules and
- * beta itself, stages z + beta d_{k-1} and writes d_k for its own rows, so a
- * CG iteration is two launches instead of three (on MI355X this is slower
- * than mode 1: every SpMV workgroup repeats the reduction prologue). mode 0
- * gathers from global memory. Results are identical in every mode. get
- * reports the mode, the rows a workgroup stages (0: none), whether segments
- * are in use and whether the direction update is fused. */
+ * global memory. mode 0 gathers from global memory. Results are identical in
+ * every mode. get reports the mode, the rows a workgroup stages (0: none)
+ * and whether segments are in use. */
 int pk_system_set_spmv_staging(pk_system* sys, int mode);
-int pk_system_get_spmv_staging(const pk_system* sys, int* mode, int* rows, int* segmented, int* fused);
-
-/* Pencil SpMV tiles (no reference counterpart; off by default): enable != 0
- * makes each SpMV tile of a body whose rows come in layers (a stride S found
- * from the pattern: the median offset of the rows' lower neighbours in the
- * layer below) pair a run of spmv_rows / 2 rows with the run S rows later,
- * so that a row's transposed mirror records are read by the same workgroup
- * in the same pass. Rows without such a stride get plain tiles. The SpMV
- * partial sums change their grouping, so results agree with plain tiles to
- * rounding, not bit for bit; full and upper storage stay bit-identical. */
-int pk_system_set_spmv_pencils(pk_system* sys, int enable);
+int pk_system_get_spmv_staging(const pk_system* sys, int* mode, int* rows, int* segmented);
 
 /* Node state, node_count x 3 each; either pointer may be NULL. */
 int pk_system_set_state(pk_system* sys, const double* x, const double* v);

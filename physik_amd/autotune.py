@@ -7,7 +7,8 @@ row lengths fill the lanes. The best shape depends on the row-length
 distribution, the storage precision and the number of rows, so it is measured
 rather than guessed:
 
-  stage 1  lanes per row, with (128 rows, unroll 2);
+  stage 1  lanes per row, with (128 rows, unroll 2) (upper-only storage:
+           one lane per row, no search);
   stage 2  rows per workgroup x unroll, with the best lane count;
   stage 3  rows per vector-kernel workgroup: the grid of the direction and
            update kernels, whose every workgroup first re-sums the other
@@ -87,8 +88,11 @@ def autotune(system, dt, cache=None, verbose=False):
             print(f"autotune {shape}: spmv {spmv * 1e3:.1f} us, direction + update {upd * 1e3:.1f} us", flush=True)
         return spmv, upd
 
-    best_l = min(LANES, key=lambda lanes: trial((lanes, 128, vec0, 2))[0])
-    cands = [(best_l, r, vec0, u) for r in ROWS for u in UNROLL if _valid(best_l, u)]
+    # upper-only storage runs one lane per row over 64-row slices: no lane stage
+    upper = system.precision in (N.PK_FP32_UPPER, N.PK_FP64_UPPER)
+    best_l = 1 if upper else min(LANES, key=lambda lanes: trial((lanes, 128, vec0, 2))[0])
+    # upper-only storage does not use unroll
+    cands = [(best_l, r, vec0, u) for r in ROWS for u in ((1,) if upper else UNROLL) if _valid(best_l, u)]
     best = min(cands, key=lambda s: trial(s)[0])
     best = min(((best[0], best[1], vr, best[3]) for vr in VEC_ROWS), key=lambda s: sum(trial(s)))
     system.tune(*best)

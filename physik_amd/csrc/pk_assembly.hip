@@ -490,11 +490,11 @@ __device__ inline void gather_slot(int s, int lane4, const int* __restrict__ cst
     for (int e = 0; e < 9; ++e)
         a[e] *= dt2;
     // full storage: the block at its full index and its transpose at the
-    // mirror; upper-only storage: the record of the slot itself
-    const int full = upperOnly ? s : uFull[s];
+    // mirror; upper-only storage: the slot's element
+    const int full = uFull[s];
 #pragma unroll
     for (int e = 0; e < 9; ++e)
-        vals[upperOnly ? upper_at(full, e) : BsrStore<M>::at(F, full, e)] = static_cast<M>(a[e]);
+        vals[upperOnly ? SliceStore<M>::word(F, full, e) : BsrStore<M>::at(F, full, e)] = static_cast<M>(a[e]);
     if (!upperOnly)
     {
         const int mirror = uMirror[s];
@@ -515,6 +515,8 @@ __device__ inline void gather_slot(int s, int lane4, const int* __restrict__ cst
 // upper slots, four at a time, each by a sub-group of 4 lanes. A row's blocks
 // all come from the tets around node r, so they are read while in L2. A
 // non-finite block or a refused inverse marks the body's solve failed.
+// uFull maps each upper slot to its full index (full storage, nnzf blocks)
+// or to its element (sliced upper-only storage, nnzf then holds nE).
 template <class M>
 __global__ __launch_bounds__(NT) void k_assemble(int nnzu, int nd, const int* __restrict__ cstart,
                                                  const int* __restrict__ clist, const int* __restrict__ nstart,
@@ -571,10 +573,10 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, int nd, const int* __
         a[0] = m + a[0];
         a[4] = m + a[4];
         a[8] = m + a[8];
-        const int full = upperOnly ? sd : uFull[sd];
+        const int full = uFull[sd];
 #pragma unroll
         for (int e = 0; e < 9; ++e)
-            vals[upperOnly ? upper_at(full, e) : BsrStore<M>::at(F, full, e)] = static_cast<M>(a[e]);
+            vals[upperOnly ? SliceStore<M>::word(F, full, e) : BsrStore<M>::at(F, full, e)] = static_cast<M>(a[e]);
         const bool ok = fin9(a);
         if (!ok)
             failed = 1;
@@ -799,9 +801,11 @@ void launch_assemble(const DeviceView& v, double dt)
     if (n <= 0 || v.nd <= 0)
         return;
     const long long groups = (static_cast<long long>(v.nd) * kRowGatherLanes + NT - 1) / NT;
+    // upper-only storage: slots map to their elements (of nE)
     hipLaunchKernelGGL((k_assemble<M>), dim3(xcd_grid(groups)), dim3(NT), 0, v.stream, n, v.nd, v.cstart, v.clist,
-                       v.nstart, v.nlist, v.diagSlot, v.uFull, v.uMirror, v.rowGroup, v.basis, v.kdat, v.mrow,
-                       dt * dt, static_cast<M*>(v.vals), v.nnzf, static_cast<M*>(v.minv), v.gs, v.upper, v.slotAdd);
+                       v.nstart, v.nlist, v.diagSlot, v.upper ? v.slotEll : v.uFull, v.uMirror, v.rowGroup, v.basis,
+                       v.kdat, v.mrow, dt * dt, static_cast<M*>(v.vals), v.upper ? v.nE : v.nnzf,
+                       static_cast<M*>(v.minv), v.gs, v.upper, v.slotAdd);
 }
 
 template <class M>

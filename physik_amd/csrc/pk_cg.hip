@@ -258,33 +258,8 @@ extern __shared__ __align__(16) unsigned char pk_stage_lds[];
 // Copies the tile's segments of d into LDS with coalesced loads and waits
 // for the workgroup; returns the staged view. LDS element i belongs to
 // segment s with 3 end[s-1] <= i < 3 end[s] and holds d element i - 3 shift[s].
-// What a staging copy reads for element gi of d: d itself (mode 0), or, when
-// the SpMV also does the direction update of iteration k > 0 (mode 1),
-// z_k + beta d_{k-1}, formed exactly as k_dir forms it and stored to d_k for
-// the tile's own rows [own0, own1) (each row has one owner tile).
-template <class M>
-struct DirSource
-{
-    const M* __restrict__ d;
-    const M* __restrict__ z;
-    const M* __restrict__ dprev;
-    M* dcur;
-    double beta;
-    int mode;
-    int own0, own1;
-    __device__ M get(int gi) const
-    {
-        if (mode == 0)
-            return d[gi];
-        const M v = direction(z[gi], dprev[gi], beta);
-        if (gi >= own0 && gi < own1)
-            dcur[gi] = v;
-        return v;
-    }
-};
-
 template <class M, bool SEG>
-__device__ inline StagedDir<M, SEG> stage_segments(const DirSource<M>& src, const StageSegs& seg)
+__device__ inline StagedDir<M, SEG> stage_segments(const M* __restrict__ d, const StageSegs& seg)
 {
     M* lds = reinterpret_cast<M*>(pk_stage_lds);
     const int4 lo = seg.lo, end = seg.end;
@@ -304,7 +279,7 @@ __device__ inline StagedDir<M, SEG> stage_segments(const DirSource<M>& src, cons
             const int i = i0 + j * NT;
             const int sh = SEG ? v.sh0 + (i >= e0 ? v.dl1 : 0) + (i >= e1 ? v.dl2 : 0) + (i >= e2 ? v.dl3 : 0)
                                : v.sh0;
-            x[j] = i < n ? src.get(i - 3 * sh) : M(0);
+            x[j] = i < n ? d[i - 3 * sh] : M(0);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
@@ -335,59 +310,88 @@ __device__ inline void load_block(const float* __restrict__ vals, size_t, const 
     col = __float_as_int(w4.y);
 }
 
-// The start of an SpMV workgroup, with `fuse` (every tile's staging fits and
-// there is no direction launch): the stop rules for the end of iteration k-1
-// and beta, taken as k_dir takes them (each workgroup from the same partials;
-// the body's first tile records them), then the direction source of the
-// staging copy. Without `fuse`: k_dir(k) has decided, and the body is skipped
-// when its loop ended at or before k. Returns false when the workgroup has
-// nothing to do.
-template <class M>
-__device__ inline bool spmv_start(int k, int t, const RowTile& tl, const TileView& st, const TileView& vt,
-                                  int fuse, const M* __restrict__ d, const M* __restrict__ z,
-                                  const M* __restrict__ dprev, M* dcur, const Partial* __restrict__ pbPrev,
-                                  GroupState* gs, int* active, DirSource<M>& src)
+// The start of an SpMV workgroup: k_dir(k) has decided, and the body is
+// skipped when its loop ended at or before k.
+__device__ inline bool spmv_live(int k, const RowTile& tl, GroupState* gs)
 {
-    GroupState& s = gs[tl.group];
-    double beta = 0.0;
-    if (fuse)
+    const int stopAt = read_uniform(&gs[tl.group].stopAt);
+    return !(stopAt >= 0 && stopAt <= k);
+}
+
+// The rows of one tile (full storage) with one lane per row: a row's sum is
+// (its blocks below the diagonal, columns descending) + (the diagonal and
+// the blocks above it, columns ascending), the order of sliced upper-only
+// storage; fdiag holds each row's full index of its diagonal block.
+template <class M, int U, class Dir>
+__device__ __attribute__((always_inline)) inline double full_rows1(const RowTile& tl, const int* __restrict__ fstart,
+                                                                   const int* __restrict__ fcol,
+                                                                   const int* __restrict__ fdiag,
+                                                                   const M* __restrict__ vals, size_t F, Dir d,
+                                                                   M* __restrict__ q)
+{
+    double pd = 0.0;
+    const int nrows = tl.rowEnd - tl.rowBegin;
+    for (int i0 = static_cast<int>(threadIdx.x); i0 < nrows; i0 += NT * U)
     {
-        if (read_uniform(&s.stopAt) >= 0)
-            return false;
-        double rz, rr;
-        int flags;
-        sum_partials(pbPrev, vt.groupBegin[tl.group], vt.groupCount[tl.group], rz, rr, flags);
-        if (stop_rules(k, s, rz, rr, flags, t == st.groupBegin[tl.group] && threadIdx.x == 0, active, beta))
-            return false;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int i = i0 + u * NT;
+            if (i >= nrows)
+                break;
+            const int row = tl.rowBegin + i;
+            const int f0 = fstart[row], f1 = fstart[row + 1], dp = fdiag[row];
+            double l[3] = {0.0, 0.0, 0.0}, h[3] = {0.0, 0.0, 0.0};
+            for (int p = dp - 1; p >= f0; --p)
+            {
+                M a[9];
+                int c;
+                load_block(vals, F, fcol, p, a, c);
+                const double d0 = d.at(c, 0), d1 = d.at(c, 1), d2 = d.at(c, 2);
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+                    l[r] += brow(a, r, d0, d1, d2);
+            }
+            for (int p = dp; p < f1; ++p)
+            {
+                M a[9];
+                int c;
+                load_block(vals, F, fcol, p, a, c);
+                const double d0 = d.at(c, 0), d1 = d.at(c, 1), d2 = d.at(c, 2);
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+                    h[r] += brow(a, r, d0, d1, d2);
+            }
+            const M q0 = static_cast<M>(l[0] + h[0]), q1 = static_cast<M>(l[1] + h[1]),
+                    q2 = static_cast<M>(l[2] + h[2]);
+            q[3 * row] = q0;
+            q[3 * row + 1] = q1;
+            q[3 * row + 2] = q2;
+            pd += dot_dq(d, row, q0, q1, q2);
+        }
     }
-    else
-    {
-        const int stopAt = read_uniform(&s.stopAt);
-        if (stopAt >= 0 && stopAt <= k)
-            return false;
-    }
-    // at k = 0 the right-hand side kernel wrote d_0
-    src = DirSource<M>{fuse && k > 0 ? dcur : d, z, dprev, dcur, beta, fuse && k > 0 ? 1 : 0,
-                       3 * tl.rowBegin, 3 * tl.rowEnd};
-    return true;
+    return pd;
 }
 
 // The rows of one tile (full storage); returns the lane's d.q partial.
+// Lane l of a row's LPR lanes sums the row's blocks l, l + LPR, ... in that
+// order, then the lanes' sums are combined by a butterfly (LPR = 1:
+// full_rows1).
 template <class M, int LPR, int U, class Dir>
-__device__ __attribute__((always_inline)) inline double full_rows(const RowTile& tl, int2 c2,
-                                                                  const int* __restrict__ fstart,
+__device__ __attribute__((always_inline)) inline double full_rows(const RowTile& tl, const int* __restrict__ fstart,
                                                                   const int* __restrict__ fcol,
+                                                                  const int* __restrict__ fdiag,
                                                                   const M* __restrict__ vals, long long nnzf, Dir d,
                                                                   M* __restrict__ q)
 {
+    if constexpr (LPR == 1)
+        return full_rows1<M, U>(tl, fstart, fcol, fdiag, vals, static_cast<size_t>(nnzf), d, q);
     const int lane = threadIdx.x % LPR;
     const int sub = threadIdx.x / LPR;
     constexpr int RPP = NT / LPR;
     const size_t F = static_cast<size_t>(nnzf);
     double pd = 0.0;
-    // position i of the tile: row rowBegin + i of its first run, then
-    // c2.x + (i - len1) of its second (pencil tiles)
-    const int len1 = tl.rowEnd - tl.rowBegin, nrows = len1 + c2.y;
+    const int nrows = tl.rowEnd - tl.rowBegin;
     for (int i0 = sub; i0 < nrows; i0 += RPP * U)
     {
         int kb[U], ke[U];
@@ -395,7 +399,7 @@ __device__ __attribute__((always_inline)) inline double full_rows(const RowTile&
         for (int u = 0; u < U; ++u)
         {
             const int i = i0 + u * RPP;
-            const int row = i < len1 ? tl.rowBegin + i : c2.x + (i - len1);
+            const int row = tl.rowBegin + i;
             const bool live = i < nrows;
             kb[u] = live ? fstart[row] + lane : 0;
             ke[u] = live ? fstart[row + 1] : 0;
@@ -445,7 +449,7 @@ __device__ __attribute__((always_inline)) inline double full_rows(const RowTile&
                 y[u][2] += __shfl_xor(y[u][2], o, LPR);
             }
             const int i = i0 + u * RPP;
-            const int row = i < len1 ? tl.rowBegin + i : c2.x + (i - len1);
+            const int row = tl.rowBegin + i;
             if (lane == 0 && i < nrows)
             {
                 const M q0 = static_cast<M>(y[u][0]), q1 = static_cast<M>(y[u][1]), q2 = static_cast<M>(y[u][2]);
@@ -460,199 +464,19 @@ __device__ __attribute__((always_inline)) inline double full_rows(const RowTile&
 }
 
 // q = A d_k for the tile's rows and the tile's d.q partial. A body whose loop
-// ended at or before k (decided by k_dir(k) or earlier) is skipped. The
-// direction blocks come from the tile's segments of d staged in LDS when
-// they fit the launch's capacity (as in spmv_body_upper), else from global
-// memory.
-template <class M, int LPR, int U, bool SEG>
-__device__ inline void spmv_body(int k, const TileView& st, const int* __restrict__ fstart,
-                                 const int* __restrict__ fcol, const M* __restrict__ vals, long long nnzf,
-                                 const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
-                                 GroupState* gs, const StageSegs* __restrict__ win, int winCap,
-                                 const int2* __restrict__ chunk2, int fuse,
-                                 const TileView& vt, const M* __restrict__ z, const M* __restrict__ dprev, M* dcur,
-                                 const Partial* __restrict__ pbPrev, int* active)
-{
-    int t;
-    RowTile tl;
-    if (!tile_for(st, t, tl))
-        return;
-    StageSegs sg{make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
-    if (winCap > 0)
-        sg = win[t];  // in flight with the stop flag
-    const int2 c2 = chunk2 ? chunk2[t] : make_int2(0, 0);
-    DirSource<M> src;
-    if (!spmv_start(k, t, tl, st, vt, fuse, d, z, dprev, dcur, pbPrev, gs, active, src))
-        return;
-    double pd;
-    if (winCap > 0 && sg.end.w <= winCap)  // uniform: one tile per workgroup
-        pd = full_rows<M, LPR, U>(tl, c2, fstart, fcol, vals, nnzf, stage_segments<M, SEG>(src, sg), q);
-    else
-        pd = full_rows<M, LPR, U>(tl, c2, fstart, fcol, vals, nnzf, GlobalDir<M>{d}, q);
-    double u1 = 0.0, u2 = 0.0;
-    block_sum3(pd, u1, u2);
-    if (threadIdx.x == 0)
-        pa[t] = pd;
-}
-
-// Block at position p of a row's full-order block list in upper-only
-// storage: a lower position (p < nl) is the transpose of the upper record
-// lower[lb + p].x, with column lower[lb + p].y; an upper position is record
-// ub + p - nl, with its column in word 9. The transpose is taken by selecting
-// elements, so the products formed from `a` are exactly those of full storage.
-__device__ inline void load_record(const float* __restrict__ rec, int slot, float* w, int& col)
-{
-    const float2* r2 = reinterpret_cast<const float2*>(rec) + 5 * static_cast<size_t>(slot);
-    const float2 w0 = r2[0], w1 = r2[1], w2 = r2[2], w3 = r2[3], w4 = r2[4];
-    w[0] = w0.x; w[1] = w0.y; w[2] = w1.x; w[3] = w1.y; w[4] = w2.x;
-    w[5] = w2.y; w[6] = w3.x; w[7] = w3.y; w[8] = w4.x;
-    col = __float_as_int(w4.y);
-}
-
-__device__ inline void load_record(const double* __restrict__ rec, int slot, double* w, int& col)
-{
-    const double2* r2 = reinterpret_cast<const double2*>(rec) + 5 * static_cast<size_t>(slot);
-    const double2 w0 = r2[0], w1 = r2[1], w2 = r2[2], w3 = r2[3], w4 = r2[4];
-    w[0] = w0.x; w[1] = w0.y; w[2] = w1.x; w[3] = w1.y; w[4] = w2.x;
-    w[5] = w2.y; w[6] = w3.x; w[7] = w3.y; w[8] = w4.x;
-    col = static_cast<int>(__double_as_longlong(w4.y));
-}
-
-template <class M>
-__device__ inline void load_upper_block(const M* __restrict__ rec, const int2* __restrict__ lower, int lb, int ub,
-                                        int nl, int p, M* a, int& col)
-{
-    const bool tr = p < nl;
-    int2 e = make_int2(0, 0);
-    if (tr)
-        e = lower[lb + p];
-    const int slot = tr ? e.x : ub + p - nl;
-    M w[9];
-    int rc;
-    load_record(rec, slot, w, rc);
-    col = tr ? e.y : rc;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            a[3 * i + j] = tr ? w[3 * j + i] : w[3 * i + j];
-}
-
-// spmv_body over upper-only storage: the same lanes, positions, products and
-// reduction order as the full-storage body, each upper record read by its own
-// row and again, transposed, by the row of its mirror (at most a layer of
-// rows later on the same XCD). With tiles of several passes about half of the
-// mirror reads miss L2 (the two reads land in different passes of their
-// tiles): the launch moves ~1.5x its upper-storage bytes. Tiles of one pass,
-// or every position of a row loaded up front, keep them in L2 (1.0x) but ran
-// slower on MI355X: this kernel is bound by its chain of dependent loads and
-// by the address unit, not by HBM bytes, and those variants lose occupancy.
-// The rows of one tile (upper-only storage); returns the lane's d.q partial.
-template <class M, int LPR, int U, class Dir>
-__device__ __attribute__((always_inline)) inline double upper_rows(const RowTile& tl, int2 c2,
-                                                                   const int2* __restrict__ rowptr,
-                                                                   const int2* __restrict__ lower,
-                                                                   const M* __restrict__ rec, Dir d,
-                                                                   M* __restrict__ q)
-{
-    const int lane = threadIdx.x % LPR;
-    const int sub = threadIdx.x / LPR;
-    constexpr int RPP = NT / LPR;
-    double pd = 0.0;
-    // position i of the tile: row rowBegin + i of its first run, then
-    // c2.x + (i - len1) of its second (pencil tiles)
-    const int len1 = tl.rowEnd - tl.rowBegin, nrows = len1 + c2.y;
-    for (int i0 = sub; i0 < nrows; i0 += RPP * U)
-    {
-        int kb[U], ke[U], lb[U], ub[U], nl[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-        {
-            const int i = i0 + u * RPP;
-            const int row = i < len1 ? tl.rowBegin + i : c2.x + (i - len1);
-            const bool live = i < nrows;
-            const int2 r0 = live ? rowptr[row] : make_int2(0, 0);
-            const int2 r1 = live ? rowptr[row + 1] : make_int2(0, 0);
-            lb[u] = r0.x;
-            ub[u] = r0.y;
-            nl[u] = r1.x - r0.x;
-            kb[u] = lane;
-            ke[u] = (r1.x - r0.x) + (r1.y - r0.y);
-        }
-        int col[U];
-        M a[U][9];
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-        {
-            const bool has = kb[u] < ke[u];
-            load_upper_block(rec, lower, has ? lb[u] : 0, has ? ub[u] : 0, has ? nl[u] : 0, has ? kb[u] : 0, a[u],
-                             col[u]);
-        }
-        double y[U][3];
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-        {
-            const bool has = kb[u] < ke[u];
-            // a lane with no block reads row tl.rowBegin's column, which every staging holds
-            const int c = has ? col[u] : tl.rowBegin;
-            const double d0 = d.at(c, 0), d1 = d.at(c, 1), d2 = d.at(c, 2);
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                y[u][i] = has ? brow(a[u], i, d0, d1, d2) : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            for (int kk = kb[u] + LPR; kk < ke[u]; kk += LPR)
-            {
-                M b[9];
-                int c;
-                load_upper_block(rec, lower, lb[u], ub[u], nl[u], kk, b, c);
-                const double d0 = d.at(c, 0), d1 = d.at(c, 1), d2 = d.at(c, 2);
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    y[u][i] += brow(b, i, d0, d1, d2);
-            }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-        {
-#pragma unroll
-            for (int o = LPR / 2; o > 0; o >>= 1)
-            {
-                y[u][0] += __shfl_xor(y[u][0], o, LPR);
-                y[u][1] += __shfl_xor(y[u][1], o, LPR);
-                y[u][2] += __shfl_xor(y[u][2], o, LPR);
-            }
-            const int i = i0 + u * RPP;
-            const int row = i < len1 ? tl.rowBegin + i : c2.x + (i - len1);
-            if (lane == 0 && i < nrows)
-            {
-                const M q0 = static_cast<M>(y[u][0]), q1 = static_cast<M>(y[u][1]), q2 = static_cast<M>(y[u][2]);
-                q[3 * row] = q0;
-                q[3 * row + 1] = q1;
-                q[3 * row + 2] = q2;
-                pd += dot_dq(d, row, q0, q1, q2);
-            }
-        }
-    }
-    return pd;
-}
-
-// q = A d_k for one tile of upper-only storage and its d.q partial. When the
+// ended at or before k (decided by k_dir(k) or earlier) is skipped. When the
 // tile's segments win[t] (up to four runs of rows covering every column its
 // rows reference, their own rows included: on a grid, the layers below, at
 // and above the tile) fit the launch's staging capacity, the workgroup first
 // copies them from d into LDS with coalesced loads and reads every direction
-// block from there: the scattered gathers leave the address unit and the
-// chain of dependent global loads. Otherwise it gathers from global memory.
-// Products and sums are the same either way.
+// block from there; otherwise it gathers from global memory. Products and
+// sums are the same either way.
 template <class M, int LPR, int U, bool SEG>
-__device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __restrict__ rowptr,
-                                       const int2* __restrict__ lower, const M* __restrict__ rec,
-                                       const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
-                                       GroupState* gs, const StageSegs* __restrict__ win, int winCap,
-                                 const int2* __restrict__ chunk2, int fuse,
-                                       const TileView& vt, const M* __restrict__ z, const M* __restrict__ dprev,
-                                       M* dcur, const Partial* __restrict__ pbPrev, int* active)
+__device__ inline void spmv_body(int k, const TileView& st, const int* __restrict__ fstart,
+                                 const int* __restrict__ fcol, const int* __restrict__ fdiag,
+                                 const M* __restrict__ vals, long long nnzf,
+                                 const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
+                                 GroupState* gs, const StageSegs* __restrict__ win, int winCap)
 {
     int t;
     RowTile tl;
@@ -661,61 +485,246 @@ __device__ inline void spmv_body_upper(int k, const TileView& st, const int2* __
     StageSegs sg{make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
     if (winCap > 0)
         sg = win[t];  // in flight with the stop flag
-    const int2 c2 = chunk2 ? chunk2[t] : make_int2(0, 0);
-    DirSource<M> src;
-    if (!spmv_start(k, t, tl, st, vt, fuse, d, z, dprev, dcur, pbPrev, gs, active, src))
+    if (!spmv_live(k, tl, gs))
         return;
     double pd;
     if (winCap > 0 && sg.end.w <= winCap)  // uniform: one tile per workgroup
-        pd = upper_rows<M, LPR, U>(tl, c2, rowptr, lower, rec, stage_segments<M, SEG>(src, sg), q);
+        pd = full_rows<M, LPR, U>(tl, fstart, fcol, fdiag, vals, nnzf, stage_segments<M, SEG>(d, sg), q);
     else
-        pd = upper_rows<M, LPR, U>(tl, c2, rowptr, lower, rec, GlobalDir<M>{d}, q);
+        pd = full_rows<M, LPR, U>(tl, fstart, fcol, fdiag, vals, nnzf, GlobalDir<M>{d}, q);
     double u1 = 0.0, u2 = 0.0;
     block_sum3(pd, u1, u2);
     if (threadIdx.x == 0)
         pa[t] = pd;
 }
 
-template <class M, int LPR, int U, bool SEG>
-__global__ __launch_bounds__(NT) void k_spmv_upper(const int* kb, int k0, TileView st, const int2* __restrict__ rowptr,
-                                                   const int2* __restrict__ lower, const M* __restrict__ rec,
-                                                   const M* __restrict__ d, M* __restrict__ q,
-                                                   double* __restrict__ pa, GroupState* gs,
+// Element e of sliced upper-only storage: its 9 values (row-major), as two
+// 16-byte loads and one 4-byte load (FP32) or four 16-byte loads and one
+// 8-byte load (FP64), each from its own plane.
+__device__ inline void load_element(const float* __restrict__ rec, size_t nE, int e, float* a)
+{
+    const float4 g0 = reinterpret_cast<const float4*>(rec)[e];
+    const float4 g1 = reinterpret_cast<const float4*>(rec + 4 * nE)[e];
+    a[0] = g0.x; a[1] = g0.y; a[2] = g0.z; a[3] = g0.w;
+    a[4] = g1.x; a[5] = g1.y; a[6] = g1.z; a[7] = g1.w;
+    a[8] = rec[8 * nE + e];
+}
+
+__device__ inline void load_element(const double* __restrict__ rec, size_t nE, int e, double* a)
+{
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+    {
+        const double2 w = reinterpret_cast<const double2*>(rec + 2 * g * nE)[e];
+        a[2 * g] = w.x;
+        a[2 * g + 1] = w.y;
+    }
+    a[8] = rec[8 * nE + e];
+}
+
+__device__ inline bool lane_bit(unsigned long long m, int lane) { return ((m >> lane) & 1ull) != 0; }
+
+// One slice (64 rows, one lane per row) of sliced upper-only storage. Round
+// i multiplies the row's upper block at position i (i = 0: the diagonal)
+// and its lower block at position i - 1. The lower ELL runs down from the
+// diagonal, so the transpose of the mirror at offset -o sits at the position
+// the upper block at offset +o takes; on a regular grid (aligned slices) the
+// two reads of one record -- by its own row and, transposed, by its mirror's
+// row -- then fall in the same round, in this wave or in the workgroup a
+// layer of rows earlier, and the second finds the record in L2. In an
+// aligned slice a block's column is the row plus the position's offset;
+// otherwise it comes from the column planes. The loop is software-pipelined:
+// round i's masks, mirror elements (and columns) are loaded during round
+// i - 1, so each round waits on one load of values (and of d) only. The
+// row's sum is (lower blocks, columns descending) + (upper blocks, columns
+// ascending), each through brow: full storage at one lane per row sums the
+// same way, so the two storages agree bit for bit. Every load of a wave
+// reads consecutive entries of one plane, or in aligned slices the mirror
+// records of consecutive rows. Returns the lane's d.q contribution.
+template <class M, bool ALIGNED, int U, class Dir>
+__device__ __attribute__((always_inline)) inline double slice_body(const SliceView& sv, const int4& ds, int row,
+                                                                   int lane, bool live, const M* __restrict__ rec,
+                                                                   size_t nE, const Dir& d, M* __restrict__ q)
+{
+    constexpr int W = kSliceRows;
+    const int wu = ds.z & 0xffff, wl = (ds.z >> 16) & 0x7fff;
+    const int rounds = wu > wl + 1 ? wu : wl + 1;
+    const int* __restrict__ off = sv.off + ds.w;
+    const unsigned long long* __restrict__ mask = sv.mask + ds.w;
+    double l0 = 0.0, l1 = 0.0, l2 = 0.0, u0 = 0.0, u1 = 0.0, u2 = 0.0;
+    // what rounds i0 .. i0 + U - 1 need before their values: lane masks,
+    // mirror elements and (packed slices) columns
+    bool hu[U], hl[U];
+    int em[U], cu[U], cl[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+        const int i = u;
+        hu[u] = i < wu && lane_bit(mask[i], lane);
+        hl[u] = i >= 1 && i <= wl && lane_bit(mask[wu + i - 1], lane);
+        em[u] = hl[u] ? sv.mirE[ds.y + W * (i - 1) + lane] : 0;
+        cu[u] = (!ALIGNED && hu[u]) ? sv.ecol[ds.x + W * i + lane] : 0;
+        cl[u] = (!ALIGNED && hl[u]) ? sv.mirC[ds.y + W * (i - 1) + lane] : 0;
+    }
+    for (int i0 = 0; i0 < rounds; i0 += U)
+    {
+        bool hu1[U], hl1[U];
+        int em1[U], cu1[U], cl1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int i = i0 + U + u;
+            hu1[u] = i < wu && lane_bit(mask[i], lane);
+            hl1[u] = i >= 1 && i <= wl && lane_bit(mask[wu + i - 1], lane);
+            em1[u] = 0;
+            cu1[u] = 0;
+            cl1[u] = 0;
+            if (hl1[u])
+                em1[u] = sv.mirE[ds.y + W * (i - 1) + lane];
+            if (!ALIGNED && hu1[u])
+                cu1[u] = sv.ecol[ds.x + W * i + lane];
+            if (!ALIGNED && hl1[u])
+                cl1[u] = sv.mirC[ds.y + W * (i - 1) + lane];
+        }
+        M a[U][9], b[U][9];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            if (hu[u])
+                load_element(rec, nE, ds.x + W * (i0 + u) + lane, a[u]);
+            if (hl[u])
+                load_element(rec, nE, em[u], b[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int i = i0 + u;
+            if (hu[u])
+            {
+                const int c = ALIGNED ? row + off[i] : cu[u];
+                const double d0 = d.at(c, 0), d1 = d.at(c, 1), d2 = d.at(c, 2);
+                u0 += brow(a[u], 0, d0, d1, d2);
+                u1 += brow(a[u], 1, d0, d1, d2);
+                u2 += brow(a[u], 2, d0, d1, d2);
+            }
+            if (hl[u])
+            {
+                const int c = ALIGNED ? row - off[wu + i - 1] : cl[u];
+                const double d0 = d.at(c, 0), d1 = d.at(c, 1), d2 = d.at(c, 2);
+                M t[9];
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                    for (int jj = 0; jj < 3; ++jj)
+                        t[3 * ii + jj] = b[u][3 * jj + ii];
+                l0 += brow(t, 0, d0, d1, d2);
+                l1 += brow(t, 1, d0, d1, d2);
+                l2 += brow(t, 2, d0, d1, d2);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            hu[u] = hu1[u];
+            hl[u] = hl1[u];
+            em[u] = em1[u];
+            cu[u] = cu1[u];
+            cl[u] = cl1[u];
+        }
+    }
+    if (!live)
+        return 0.0;
+    const M q0 = static_cast<M>(l0 + u0), q1 = static_cast<M>(l1 + u1), q2 = static_cast<M>(l2 + u2);
+    q[3 * row] = q0;
+    q[3 * row + 1] = q1;
+    q[3 * row + 2] = q2;
+    return dot_dq(d, row, q0, q1, q2);
+}
+
+// The rows of one tile in sliced upper-only storage: wave w of the
+// workgroup takes the tile's slices w, w + NT / 64, ... (slice_body, U
+// rounds loaded at a time).
+template <class M, int U, class Dir>
+__device__ __attribute__((always_inline)) inline double slice_rows(const RowTile& tl, int slice0,
+                                                                   const SliceView& sv, const M* __restrict__ rec,
+                                                                   Dir d, M* __restrict__ q)
+{
+    constexpr int W = kSliceRows, WAVES = NT / kSliceRows;
+    const int wave = static_cast<int>(threadIdx.x) / W;
+    const int lane = static_cast<int>(threadIdx.x) % W;
+    const int nsl = (tl.rowEnd - tl.rowBegin + W - 1) / W;
+    const size_t nE = static_cast<size_t>(sv.nE);
+    double pd = 0.0;
+    for (int j = wave; j < nsl; j += WAVES)
+    {
+        const int sl = __builtin_amdgcn_readfirstlane(slice0 + j);
+        const int4 ds = sv.slices[sl];
+        const int row = tl.rowBegin + W * j + lane;
+        const bool live = row < tl.rowEnd;
+        if (ds.z < 0)  // aligned (uniform)
+            pd += slice_body<M, true, U>(sv, ds, row, lane, live, rec, nE, d, q);
+        else
+            pd += slice_body<M, false, U>(sv, ds, row, lane, live, rec, nE, d, q);
+    }
+    return pd;
+}
+
+// spmv_body over sliced upper-only storage.
+template <class M, int U, bool SEG>
+__device__ inline void spmv_body_upper(int k, const TileView& st, const SliceView& sv, const M* __restrict__ rec,
+                                       const M* __restrict__ d, M* __restrict__ q, double* __restrict__ pa,
+                                       GroupState* gs, const StageSegs* __restrict__ win, int winCap)
+{
+    int t;
+    RowTile tl;
+    if (!tile_for(st, t, tl))
+        return;
+    StageSegs sg{make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
+    if (winCap > 0)
+        sg = win[t];  // in flight with the stop flag
+    const int s0 = sv.tileSlice[t];
+    if (!spmv_live(k, tl, gs))
+        return;
+    double pd;
+    if (winCap > 0 && sg.end.w <= winCap)  // uniform: one tile per workgroup
+        pd = slice_rows<M, U>(tl, s0, sv, rec, stage_segments<M, SEG>(d, sg), q);
+    else
+        pd = slice_rows<M, U>(tl, s0, sv, rec, GlobalDir<M>{d}, q);
+    double u1 = 0.0, u2 = 0.0;
+    block_sum3(pd, u1, u2);
+    if (threadIdx.x == 0)
+        pa[t] = pd;
+}
+
+template <class M, int U, bool SEG>
+__global__ __launch_bounds__(NT) void k_spmv_upper(const int* kb, int k0, TileView st, SliceView sv,
+                                                   const M* __restrict__ rec, const M* __restrict__ d,
+                                                   M* __restrict__ q, double* __restrict__ pa, GroupState* gs,
                                                    const StageSegs* __restrict__ win, int winCap,
-                                                   const int2* __restrict__ chunk2, int fuse,
-                                                   TileView vt, const M* __restrict__ z,
-                                                   const M* __restrict__ dprev, M* dcur,
-                                                   const Partial* __restrict__ pbPrev, int* active,
                                                    unsigned long long* stamps)
 {
     const LaunchIndex li = launch_index(kb, k0, stamps);
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    spmv_body_upper<M, LPR, U, SEG>(k, st, rowptr, lower, rec, d, q, pa, gs, win, winCap, chunk2, fuse, vt, z, dprev,
-                                    dcur,
-                                    pbPrev, active);
+    spmv_body_upper<M, U, SEG>(k, st, sv, rec, d, q, pa, gs, win, winCap);
     stamp_end(stamp);
 }
 
 template <class M, int LPR, int U, bool SEG>
 __global__ __launch_bounds__(NT) void k_spmv(const int* kb, int k0, TileView st, const int* __restrict__ fstart,
-                                             const int* __restrict__ fcol, const M* __restrict__ vals,
+                                             const int* __restrict__ fcol, const int* __restrict__ fdiag,
+                                             const M* __restrict__ vals,
                                              long long nnzf, const M* __restrict__ d, M* __restrict__ q,
                                              double* __restrict__ pa, GroupState* gs,
                                              const StageSegs* __restrict__ win, int winCap,
-                                             const int2* __restrict__ chunk2, int fuse, TileView vt,
-                                             const M* __restrict__ z, const M* __restrict__ dprev, M* dcur,
-                                             const Partial* __restrict__ pbPrev, int* active,
                                              unsigned long long* stamps)
 {
     const LaunchIndex li = launch_index(kb, k0, stamps);
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    spmv_body<M, LPR, U, SEG>(k, st, fstart, fcol, vals, nnzf, d, q, pa, gs, win, winCap, chunk2, fuse, vt, z, dprev,
-                              dcur,
-                              pbPrev, active);
+    spmv_body<M, LPR, U, SEG>(k, st, fstart, fcol, fdiag, vals, nnzf, d, q, pa, gs, win, winCap);
     stamp_end(stamp);
 }
 
@@ -903,43 +912,57 @@ __global__ __launch_bounds__(NT) void k_spmv_plain(int nd, const int* __restrict
         y[3 * row + i] = acc[i];
 }
 
-// y = A x over upper-only FP32 storage, one thread per row, blocks in
-// full-row order (the same sums as k_spmv_plain over full storage).
+// y = A x over sliced upper-only storage, one thread per row, blocks in
+// full-row order (the same sums as k_spmv_plain over full storage), columns
+// from the column planes.
 template <class M>
-__global__ __launch_bounds__(NT) void k_spmv_plain_upper(int nd, const int2* __restrict__ rowptr,
-                                                         const int2* __restrict__ lower, const M* __restrict__ rec,
+__global__ __launch_bounds__(NT) void k_spmv_plain_upper(int nd, SliceView sv, const M* __restrict__ rec,
                                                          const double* __restrict__ xin, double* __restrict__ y)
 {
     const int row = blockIdx.x * NT + threadIdx.x;
     if (row >= nd)
         return;
-    const int2 r0 = rowptr[row], r1 = rowptr[row + 1];
-    const int nl = r1.x - r0.x, n = nl + (r1.y - r0.y);
+    const int2 rs = sv.rowSlice[row];  // (slice, lane)
+    const int4 ds = sv.slices[rs.x];
+    const int wu = ds.z & 0xffff, wl = (ds.z >> 16) & 0x7fff, l = rs.y;
+    const size_t n = static_cast<size_t>(sv.nE);
     double acc[3] = {0, 0, 0};
-    for (int p = 0; p < n; ++p)
-    {
-        M a[9];
-        int c;
-        load_upper_block(rec, lower, r0.x, r0.y, nl, p, a, c);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            acc[i] += brow(a, i, xin[3 * c], xin[3 * c + 1], xin[3 * c + 2]);
-    }
+    for (int k = wl - 1; k >= 0; --k)  // the lower ELL runs from the diagonal down
+        if (lane_bit(sv.mask[ds.w + wu + k], l))
+        {
+            const int idx = ds.y + kSliceRows * k + l;
+            M a[9], t[9];
+            load_element(rec, n, sv.mirE[idx], a);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    t[3 * i + j] = a[3 * j + i];
+            const int c = sv.mirC[idx];
+            for (int i = 0; i < 3; ++i)
+                acc[i] += brow(t, i, xin[3 * c], xin[3 * c + 1], xin[3 * c + 2]);
+        }
+    for (int k = 0; k < wu; ++k)
+        if (lane_bit(sv.mask[ds.w + k], l))
+        {
+            const int e = ds.x + kSliceRows * k + l;
+            M a[9];
+            load_element(rec, n, e, a);
+            const int c = sv.ecol[e];
+            for (int i = 0; i < 3; ++i)
+                acc[i] += brow(a, i, xin[3 * c], xin[3 * c + 1], xin[3 * c + 2]);
+        }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         y[3 * row + i] = acc[i];
 }
 
-template <class M, int LPR, int U>
-static void launch_spmv_upper_shape(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp)
+template <class M, int U>
+static void launch_spmv_upper(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp)
 {
     const size_t lds = 3 * sizeof(M) * static_cast<size_t>(v.winCap);
-    auto kernel = v.stageSegs ? k_spmv_upper<M, LPR, U, true> : k_spmv_upper<M, LPR, U, false>;
+    auto kernel = v.stageSegs ? k_spmv_upper<M, U, true> : k_spmv_upper<M, U, false>;
     hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds, v.stream, kb, k0,
-                       v.spmv, v.rowptr, v.lower, static_cast<const M*>(v.vals), static_cast<const M*>(v.d[k0 & 1]),
-                       static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap, v.spmvChunk2, v.fuseDir, v.vec,
-                       static_cast<const M*>(v.z), static_cast<const M*>(v.d[(k0 + 1) & 1]),
-                       static_cast<M*>(v.d[k0 & 1]), v.pb[(k0 + 1) & 1], v.active, stamp);
+                       v.spmv, v.slice, static_cast<const M*>(v.vals), static_cast<const M*>(v.d[k0 & 1]),
+                       static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap, stamp);
 }
 
 // The iteration of a launch is k0 (kb == nullptr) or *kb + k0 with *kb even,
@@ -950,10 +973,9 @@ static void launch_spmv_shape(const DeviceView& v, const int* kb, int k0, unsign
     const size_t lds = 3 * sizeof(M) * static_cast<size_t>(v.winCap);
     auto kernel = v.stageSegs ? k_spmv<M, LPR, U, true> : k_spmv<M, LPR, U, false>;
     hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds, v.stream, kb, k0,
-                       v.spmv, v.fstart, v.fcol, static_cast<const M*>(v.vals), v.nnzf,
+                       v.spmv, v.fstart, v.fcol, v.fdiag, static_cast<const M*>(v.vals), v.nnzf,
                        static_cast<const M*>(v.d[k0 & 1]), static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap,
-                       v.spmvChunk2, v.fuseDir, v.vec, static_cast<const M*>(v.z), static_cast<const M*>(v.d[(k0 + 1) & 1]),
-                       static_cast<M*>(v.d[k0 & 1]), v.pb[(k0 + 1) & 1], v.active, stamp);
+                       stamp);
 }
 
 template <class M>
@@ -966,6 +988,10 @@ void launch_dir(const DeviceView& v, const int* kb, int k0, unsigned long long* 
                        static_cast<M*>(v.d[k0 & 1]), v.pb[(k0 + 1) & 1], v.gs, v.active, stamp);
 }
 
+// Upper-only storage runs one lane per row (lanesPerRow 1) and one round
+// per pipeline step (two rounds per step ran slower on MI355X: the extra
+// registers cost more waves than the deeper pipeline gained), whatever
+// `unroll`; full storage any listed shape.
 template <class M>
 void launch_spmv(const DeviceView& v, const int* kb, int k0, int lanesPerRow, int unroll, unsigned long long* stamp)
 {
@@ -973,18 +999,7 @@ void launch_spmv(const DeviceView& v, const int* kb, int k0, int lanesPerRow, in
         return;
     if (v.upper)
     {
-#define PK_SHAPE(L, U)                                      \
-    if (lanesPerRow == L && unroll == U)                    \
-    {                                                       \
-        launch_spmv_upper_shape<M, L, U>(v, kb, k0, stamp); \
-        return;                                             \
-    }
-        PK_SHAPE(4, 1) PK_SHAPE(4, 2) PK_SHAPE(4, 4)
-        PK_SHAPE(8, 1) PK_SHAPE(8, 2) PK_SHAPE(8, 4)
-        PK_SHAPE(16, 1) PK_SHAPE(16, 2) PK_SHAPE(16, 4)
-        PK_SHAPE(32, 1) PK_SHAPE(32, 2)
-#undef PK_SHAPE
-        launch_spmv_upper_shape<M, 16, 1>(v, kb, k0, stamp);
+        launch_spmv_upper<M, 1>(v, kb, k0, stamp);
         return;
     }
 #define PK_SHAPE(L, U)                        \
@@ -993,6 +1008,7 @@ void launch_spmv(const DeviceView& v, const int* kb, int k0, int lanesPerRow, in
         launch_spmv_shape<M, L, U>(v, kb, k0, stamp); \
         return;                               \
     }
+    PK_SHAPE(1, 1) PK_SHAPE(1, 2) PK_SHAPE(1, 4)
     PK_SHAPE(4, 1) PK_SHAPE(4, 2) PK_SHAPE(4, 4)
     PK_SHAPE(8, 1) PK_SHAPE(8, 2) PK_SHAPE(8, 4)
     PK_SHAPE(16, 1) PK_SHAPE(16, 2) PK_SHAPE(16, 4)
@@ -1126,7 +1142,7 @@ void launch_spmv_plain(const DeviceView& v, const double* x, double* y)
     {
         if (v.nd > 0)
             hipLaunchKernelGGL(k_spmv_plain_upper<M>, dim3((v.nd + NT - 1) / NT), dim3(NT), 0, v.stream, v.nd,
-                               v.rowptr, v.lower, static_cast<const M*>(v.vals), x, y);
+                               v.slice, static_cast<const M*>(v.vals), x, y);
         return;
     }
     if (v.nd > 0)
@@ -1148,25 +1164,15 @@ __global__ __launch_bounds__(NT) void k_record_columns(long long F, const int* _
         vals[BsrStore<float>::at(static_cast<size_t>(F), static_cast<size_t>(k), 9)] = __int_as_float(fcol[k]);
 }
 
-__global__ __launch_bounds__(NT) void k_record_columns64(long long U, const int* __restrict__ ucol,
-                                                         double* __restrict__ vals)
-{
-    const long long k = static_cast<long long>(blockIdx.x) * NT + threadIdx.x;
-    if (k < U)
-        vals[upper_at(static_cast<size_t>(k), 9)] = __longlong_as_double(static_cast<long long>(ucol[k]));
-}
-
+// The column word of every FP32 full-storage record (upper-only storage
+// keeps its columns in a plane of their own, set at creation).
 void launch_record_columns(const DeviceView& v, bool f32)
 {
-    const long long n = v.upper ? v.nnzu : v.nnzf;
-    if (n <= 0)
+    const long long n = v.nnzf;
+    if (n <= 0 || v.upper || !f32)
         return;
     const dim3 grid(static_cast<unsigned>((n + NT - 1) / NT));
-    if (f32)
-        hipLaunchKernelGGL(k_record_columns, grid, dim3(NT), 0, v.stream, n, v.upper ? v.ucol : v.fcol,
-                           static_cast<float*>(v.vals));
-    else if (v.upper)
-        hipLaunchKernelGGL(k_record_columns64, grid, dim3(NT), 0, v.stream, n, v.ucol, static_cast<double*>(v.vals));
+    hipLaunchKernelGGL(k_record_columns, grid, dim3(NT), 0, v.stream, n, v.fcol, static_cast<float*>(v.vals));
 }
 
 template void launch_spmv_plain<double>(const DeviceView&, const double*, double*);

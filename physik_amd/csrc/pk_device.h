@@ -108,10 +108,57 @@ template <> struct BsrStore<float>
     __host__ __device__ static size_t at(size_t, size_t k, int e) { return 10 * k + static_cast<size_t>(e); }
 };
 
-// Upper-only storage, both precisions: one record of 10 words per upper slot,
-// the 9 values then the column (int bits of a float, or int64 bits of a
-// double), so a transposed read of a mirror is five 8- or 16-byte loads.
-__host__ __device__ inline size_t upper_at(size_t k, int e) { return 10 * k + static_cast<size_t>(e); }
+// Upper-only storage, both precisions, sliced for one lane per row. The
+// dynamic rows of each body are cut into slices of kSliceRows consecutive
+// rows (one wavefront). A slice keeps its rows' upper blocks as an ELL block
+// of width wu: the k-th upper block of the row at lane l (the diagonal, then
+// the columns above it ascending) is element ubase + kSliceRows * k + l. An
+// element's 9 values (row-major) are kept in planes of nE elements, so that
+// the lanes of a wave read one plane of consecutive elements as one
+// contiguous run:
+//   FP32: values 0-3 (16 bytes) | values 4-7 (16 bytes) | value 8
+//   FP64: values 0-1 | 2-3 | 4-5 | 6-7 (16 bytes each) | value 8
+// and its column is entry e of a separate int32 plane (the pattern's, set
+// once). A row's lower blocks (columns c < r, descending) are a second ELL
+// block of width wl, each entry the element of its upper mirror (c, r) (one
+// int32 plane) and c (another).
+// Slice descriptor: (ubase, lbase, wu | wl << 16 | aligned << 31, mbase);
+// bit l of mask[mbase + k] says lane l has an upper position k, bit l of
+// mask[mbase + wu + k] a lower position k. In an aligned slice position k
+// means one stencil offset for every row -- the upper block at column
+// r + off[mbase + k], the lower block at column r - off[mbase + wu + k]
+// (offsets ascending; a row without that neighbour leaves a hole) -- so
+// columns need no load, and on a regular grid the transposed mirror of a
+// lower position sits at the same position of the mirror row's slice. Slices
+// whose offsets do not line up keep each row's blocks packed from position 0
+// and read their columns from the planes. Padding elements are never read.
+constexpr int kSliceRows = 64;
+struct SliceView
+{
+    const int4* slices;
+    const unsigned long long* mask;
+    const int* off;       // stencil offsets of aligned slices, indexed like mask
+    const int* mirE;      // lower ELL: element of the upper mirror
+    const int* mirC;      // lower ELL: its column
+    const int* ecol;      // column of every element
+    const int* tileSlice; // first slice of every SpMV tile
+    const int2* rowSlice; // (slice, lane) of every row
+    long long nE;
+};
+
+template <class M>
+struct SliceStore
+{
+    static constexpr size_t kGroup = 16 / sizeof(M);  // values per 16-byte plane entry
+    static constexpr size_t kWords = 9;               // values per element
+    // index into the value array of value w (0..8) of element e
+    __host__ __device__ static size_t word(size_t nE, size_t e, int w)
+    {
+        if (w < 8)
+            return kGroup * ((static_cast<size_t>(w) / kGroup) * nE + e) + static_cast<size_t>(w) % kGroup;
+        return 8 * nE + e;
+    }
+};
 
 __device__ inline int tile_of(const TileView& tv)
 {
@@ -297,14 +344,13 @@ struct DeviceView
     const int* rowGroup;
     const int* fstart;
     const int* fcol;
-    // upper-only FP32 storage (upper != 0): A holds one record per upper slot;
-    // row r's blocks in full-row order are its lower blocks (c < r), each the
-    // transpose of the upper record lower[l].x with column lower[l].y for
-    // l in [rowptr[r].x, rowptr[r+1].x), then its upper slots
-    // [rowptr[r].y, rowptr[r+1].y) (columns c >= r ascending)
+    const int* fdiag;  // full storage: each row's full index of its diagonal block
+    // upper-only storage (upper != 0), sliced (SliceStore, SliceView), and
+    // the element of every upper slot
     int upper;
-    const int2* rowptr;
-    const int2* lower;
+    long long nE;
+    SliceView slice;
+    const int* slotEll;
     const int* ucol;
     const int* nstart;
     const int* nlist;
@@ -367,10 +413,8 @@ struct DeviceView
     // tile whose segments hold at most winCap rows in all reads d from LDS
     // (winCap 0: never)
     const StageSegs* spmvWin;
-    const int2* spmvChunk2;  // pencil tiles: per SpMV tile a second run of rows (begin, count); null: none
     int winCap;
     int stageSegs;  // 0: every tile's staging is one window (segment 0 only)
-    int fuseDir;    // every tile stages: the SpMV also does the direction update (no k_dir launch)
     // reductions; longSums: some body has 4 * NT SpMV tiles or more (its
     // update workgroups sum long partial lists)
     int longSums;

@@ -133,7 +133,16 @@ struct pk_system
     pk::DevBuf<char> minv;  // storage type, like the CG vectors
     pk::DevBuf<int> cstart, clist, uFull, uMirror, slotRow, rowGroup, fstart, fcol, nstart, nlist, rowNode, active;
     pk::DevBuf<int> astart, alist, nodeRow, nodeGroup, cpos, diagSlot;
-    pk::DevBuf<int2> rowptr, lower;  // upper-only storage: row pointers and lower blocks
+    // sliced upper-only storage: slice descriptors, lane masks, lower ELL
+    // entries, the element of each upper slot, each row's (slice, lane) and
+    // each SpMV tile's first slice
+    pk::DevBuf<int> fdiag;  // full storage: full index of each row's diagonal block
+    pk::DevBuf<int4> slices;
+    pk::DevBuf<unsigned long long> smask;
+    pk::DevBuf<int2> rowSlice;
+    pk::DevBuf<int> slotEll, tileSlice, ecol, soff, mirE, mirC;
+    std::vector<int> groupSlice;  // first slice of each body
+    long long nE = 0;
     pk::DevBuf<int> ucol;
     pk::DevBuf<char> vals, z, d0, d1, q;  // element type set by the precision
     pk::DevBuf<pk::Partial> pb0, pb1;
@@ -150,17 +159,11 @@ struct pk_system
     // SpMV staging (upper-only storage): the column window of every SpMV tile
     // and the rows of d a workgroup stages in LDS (0: no staging)
     pk::DevBuf<pk::StageSegs> spmvWin;
-    // pencil SpMV tiles (pk_system_set_spmv_pencils): per SpMV tile a second
-    // run of rows (begin, count) a layer after its first; host copies of the
-    // SpMV tiling for the staging set-up
-    bool pencils = false;
-    pk::DevBuf<int2> spmvChunk2;
+    // host copy of the SpMV tiling for the staging set-up
     std::vector<pk::RowTile> spmvHostTiles;
-    std::vector<int2> spmvHostChunk2;
     int winCap = 0;
     int stageSegs = 0;
-    int fuseDir = 0;
-    int stagingMode = 1;  // 0 off, 1 staging, 2 staging with the direction update fused where every tile stages
+    int stagingMode = 1;  // 0 off, 1 staging
     int* pinned = nullptr;
     hipEvent_t pollEv[2] = {nullptr, nullptr};
 
@@ -287,9 +290,11 @@ struct pk_system
         dv.diagSlot = diagSlot.p;
         dv.rowGroup = rowGroup.p;
         dv.fstart = fstart.p;
+        dv.fdiag = fdiag.p;
         dv.upper = (precision == PK_FP32_UPPER || precision == PK_FP64_UPPER) ? 1 : 0;
-        dv.rowptr = rowptr.p;
-        dv.lower = lower.p;
+        dv.nE = nE;
+        dv.slice = pk::SliceView{slices.p, smask.p, soff.p, mirE.p, mirC.p, ecol.p, tileSlice.p, rowSlice.p, nE};
+        dv.slotEll = slotEll.p;
         dv.ucol = ucol.p;
         dv.fcol = fcol.p;
         dv.nstart = nstart.p;
@@ -345,10 +350,8 @@ struct pk_system
         dv.spmv = spmvTiles.view();
         dv.longSums = spmvTiles.longest >= 4 * pk::NT ? 1 : 0;
         dv.spmvWin = spmvWin.p;
-        dv.spmvChunk2 = pencils ? spmvChunk2.p : nullptr;
         dv.winCap = winCap;
         dv.stageSegs = stageSegs;
-        dv.fuseDir = fuseDir;
         dv.pa = pa.p;
         dv.pb[0] = pb0.p;
         dv.pb[1] = pb1.p;
@@ -382,75 +385,33 @@ static int build_tiles(pk_system* s, Tiles& tt, int rowsPerTile)
     return PK_OK;
 }
 
-// Layer stride of rows [b, e) of one body: for each row, the centre of its
-// lower neighbours more than half its lower extent away (the layer below),
-// as an offset; the median over the rows. 0 when the rows have no such
-// neighbours.
-static int layer_stride(const Topology& t, int b, int e)
-{
-    std::vector<int> centres;
-    for (int r = b; r < e; ++r)
-    {
-        const int k0 = t.fstart[r], k1 = t.fstart[r + 1];
-        if (k1 == k0 || t.fcol[k0] >= r)
-            continue;
-        const int ext = r - t.fcol[k0];
-        if (ext < 2)
-            continue;
-        int lo = INT_MAX, hi = 0;
-        for (int k = k0; k < k1 && t.fcol[k] < r; ++k)
-        {
-            const int o = r - t.fcol[k];
-            if (2 * o > ext)
-            {
-                lo = std::min(lo, o);
-                hi = std::max(hi, o);
-            }
-        }
-        if (hi > 0)
-            centres.push_back((lo + hi) / 2);
-    }
-    if (centres.empty())
-        return 0;
-    std::nth_element(centres.begin(), centres.begin() + centres.size() / 2, centres.end());
-    return centres[centres.size() / 2];
-}
+static bool is_upper(const pk_system* s);
 
-// The SpMV tiling. Plain: runs of spmvRows consecutive rows. Pencils: where a
-// body's layer stride S is known, each tile pairs a run of spmvRows / 2 rows
-// with the run S rows later (blocks of 2 S rows split into such pairs), so a
-// row's mirror records in the layer below are read by the same workgroup in
-// the same pass; rows left over at a body's end get plain tiles.
+// The SpMV tiling: runs of spmvRows consecutive rows of one body (with
+// sliced upper-only storage spmvRows is a multiple of kSliceRows, so every
+// tile is whole slices: tileSlice holds its first).
 static int build_spmv_tiles(pk_system* s)
 {
     const Topology& t = s->topo;
     std::vector<RowTile>& tiles = s->spmvHostTiles;
-    std::vector<int2>& chunk2 = s->spmvHostChunk2;
     tiles.clear();
-    chunk2.clear();
-    const int R = s->spmvRows, w = std::max(1, s->spmvRows / 2);
-    std::vector<int> gb(static_cast<size_t>(t.G), 0), gc(static_cast<size_t>(t.G), 0);
+    const int R = s->spmvRows;
+    std::vector<int> gb(static_cast<size_t>(t.G), 0), gc(static_cast<size_t>(t.G), 0), first;
     for (int g = 0; g < t.G; ++g)
     {
         gb[g] = static_cast<int>(tiles.size());
         const int b = t.groupRowBegin[g], e = t.groupRowEnd[g];
-        int r = b;
-        const int S = s->pencils ? layer_stride(t, b, e) : 0;
-        if (S >= w)
-            for (; r + 2 * S <= e; r += 2 * S)
-                for (int j = r; j < r + S; j += w)
-                {
-                    const int n = std::min(w, r + S - j);
-                    tiles.push_back(RowTile{g, j, j + n, 0});
-                    chunk2.push_back(make_int2(j + S, n));
-                }
-        for (; r < e; r += R)
+        for (int r = b; r < e; r += R)
         {
             tiles.push_back(RowTile{g, r, std::min(r + R, e), 0});
-            chunk2.push_back(make_int2(0, 0));
+            if (is_upper(s))
+                first.push_back(s->groupSlice[g] + (r - b) / kSliceRows);
         }
         gc[g] = static_cast<int>(tiles.size()) - gb[g];
     }
+    int rc0;
+    if (is_upper(s) && (rc0 = s->tileSlice.upload(first, s->stream)) != PK_OK)
+        return rc0;
     Tiles& tt = s->spmvTiles;
     tt.count = static_cast<int>(tiles.size());
     tt.per = (tt.count + 7) / 8;
@@ -459,8 +420,7 @@ static int build_spmv_tiles(pk_system* s)
     int rc;
     if ((rc = tt.tiles.upload(tiles, s->stream)) != PK_OK) return rc;
     if ((rc = tt.groupBegin.upload(gb, s->stream)) != PK_OK) return rc;
-    if ((rc = tt.groupCount.upload(gc, s->stream)) != PK_OK) return rc;
-    return s->spmvChunk2.upload(chunk2, s->stream);
+    return tt.groupCount.upload(gc, s->stream);
 }
 
 static int upload_groups(pk_system* s)
@@ -574,7 +534,7 @@ static void harvest_profile(pk_system* s)
                     add(PK_K_SPMV, a, k);
                 if (k < updWork)
                     add(PK_K_UPDATE, b, k);
-                if (k < dirWork && !s->fuseDir)
+                if (k < dirWork)
                     add(PK_K_DIRECTION, c, k);
             }
         }
@@ -652,8 +612,7 @@ static int capture_chunk(pk_system* s, const DeviceView& dv, bool compacted, uns
     }
     for (int o = 0; o < kChunk; ++o)
     {
-        if (!dv.fuseDir)
-            launch_dir<M>(cv, dv.kbase, o, stD);
+        launch_dir<M>(cv, dv.kbase, o, stD);
         launch_spmv<M>(cv, dv.kbase, o, s->lanesPerRow, s->unroll, stS);
         launch_update<M>(cv, dv.kbase, o, stU);
     }
@@ -756,11 +715,8 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
                     cv.vec.perm = s->vecPerm.p;
                     cv.spmv.perm = s->spmvPerm.p;
                 }
-                // fused: the SpMV takes the stop decision of k (also at
-                // k = maxIt) and forms d_k; otherwise k_dir does
-                if (!dv.fuseDir)
-                    launch_dir<M>(cv, nullptr, k, stD ? stD + 2 * static_cast<size_t>(gU) * k : nullptr);
-                if (k < maxIt || dv.fuseDir)
+                launch_dir<M>(cv, nullptr, k, stD ? stD + 2 * static_cast<size_t>(gU) * k : nullptr);
+                if (k < maxIt)
                     launch_spmv<M>(cv, nullptr, k, s->lanesPerRow, s->unroll,
                                    stS ? stS + 2 * static_cast<size_t>(gS) * k : nullptr);
                 if (k < maxIt)
@@ -783,8 +739,7 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
     {
         // graph chunks ran the SpMV and update of every launch they covered
         const int nS = s->useGraphs ? k : std::min(k, maxIt);
-        if (!dv.fuseDir)
-            launch_stamp_reduce(s->stream, stD, gU, k, s->stampDir.p);
+        launch_stamp_reduce(s->stream, stD, gU, k, s->stampDir.p);
         launch_stamp_reduce(s->stream, stS, gS, nS, s->stampSpmv.p);
         launch_stamp_reduce(s->stream, stU, gU, nS, s->stampUpd.p);
     }
@@ -853,6 +808,152 @@ static bool is_f32(const pk_system* s) { return s->precision == PK_FP32_STORAGE 
 // A stored as its upper triangle, one record per slot.
 static bool is_upper(const pk_system* s) { return s->precision == PK_FP32_UPPER || s->precision == PK_FP64_UPPER; }
 
+// Sliced upper-only storage (SliceStore, SliceView): per body, slices of
+// kSliceRows rows; per slice its upper and lower ELL blocks (aligned to the
+// slice's stencil offsets where those line up, else packed), their lane
+// masks and offsets; the element and column of every upper slot; and for
+// every lower position the element of its upper mirror and the mirror's row.
+static int build_slices(pk_system* s, const std::vector<int>& slotRow)
+{
+    const Topology& t = s->topo;
+    constexpr int W = kSliceRows;
+    // a slice aligns when its distinct offsets need at most this many more
+    // positions than its longest row
+    constexpr int kAlignSlack = 2;
+    std::vector<int4> desc;
+    std::vector<unsigned long long> masks;
+    std::vector<int> offs;
+    std::vector<int2> rowSlice(static_cast<size_t>(std::max(t.nd, 1)), make_int2(0, 0));
+    std::vector<int> slotEll(static_cast<size_t>(std::max<long long>(t.nnzu(), 1)), -1);
+    // lower ELL position of each lower block, by its full index
+    std::vector<int> lowPos(static_cast<size_t>(std::max<long long>(t.nnzf(), 1)), -1);
+    s->groupSlice.assign(static_cast<size_t>(t.G), 0);
+    long long nE = 0, nL = 0;
+    std::vector<int> up, lo;
+    for (int g = 0; g < t.G; ++g)
+    {
+        s->groupSlice[g] = static_cast<int>(desc.size());
+        for (int r0 = t.groupRowBegin[g]; r0 < t.groupRowEnd[g]; r0 += W)
+        {
+            const int n = std::min(W, t.groupRowEnd[g] - r0);
+            int mu = 0, ml = 0;  // longest row's upper and lower counts
+            up.clear();
+            lo.clear();
+            for (int l = 0; l < n; ++l)
+            {
+                const int r = r0 + l;
+                const int f0 = t.fstart[r], f1 = t.fstart[r + 1];
+                const int nu = t.ustart[r + 1] - t.ustart[r], nlr = (f1 - f0) - nu;
+                mu = std::max(mu, nu);
+                ml = std::max(ml, nlr);
+                for (int k = f0; k < f1; ++k)
+                    (t.fcol[k] >= r ? up : lo).push_back(t.fcol[k] >= r ? t.fcol[k] - r : r - t.fcol[k]);
+            }
+            std::sort(up.begin(), up.end());
+            up.erase(std::unique(up.begin(), up.end()), up.end());
+            std::sort(lo.begin(), lo.end());
+            lo.erase(std::unique(lo.begin(), lo.end()), lo.end());
+            const bool aligned = static_cast<int>(up.size()) <= mu + kAlignSlack &&
+                                 static_cast<int>(lo.size()) <= ml + kAlignSlack;
+            const int wu = aligned ? static_cast<int>(up.size()) : mu;
+            const int wl = aligned ? static_cast<int>(lo.size()) : ml;
+            if (wu > 0xffff || wl > 0x7fff || nE + static_cast<long long>(W) * wu > INT_MAX ||
+                nL + static_cast<long long>(W) * wl > INT_MAX || masks.size() + wu + wl > static_cast<size_t>(INT_MAX))
+            {
+                set_error("upper-only storage: the pattern exceeds the sliced layout's limits");
+                return PK_ERR_ARG;
+            }
+            const int sl = static_cast<int>(desc.size());
+            const int mbase = static_cast<int>(masks.size());
+            desc.push_back(make_int4(static_cast<int>(nE), static_cast<int>(nL),
+                                     wu | (wl << 16) | (aligned ? static_cast<int>(0x80000000u) : 0), mbase));
+            masks.resize(masks.size() + wu + wl, 0ull);
+            offs.resize(masks.size(), 0);
+            if (aligned)
+            {
+                std::copy(up.begin(), up.end(), offs.begin() + mbase);
+                std::copy(lo.begin(), lo.end(), offs.begin() + mbase + wu);
+            }
+            for (int l = 0; l < n; ++l)
+            {
+                const int r = r0 + l;
+                rowSlice[r] = make_int2(sl, l);
+                const int f0 = t.fstart[r], f1 = t.fstart[r + 1];
+                const int nlr = (f1 - f0) - (t.ustart[r + 1] - t.ustart[r]);
+                // upper slots ustart[r] + k (columns ascending)
+                for (int k = 0; k < t.ustart[r + 1] - t.ustart[r]; ++k)
+                {
+                    const int u = t.ustart[r] + k;
+                    const int pos = aligned ? static_cast<int>(std::lower_bound(up.begin(), up.end(), t.ucol[u] - r) -
+                                                               up.begin())
+                                            : k;
+                    masks[mbase + pos] |= 1ull << l;
+                    slotEll[u] = static_cast<int>(nE + static_cast<long long>(W) * pos + l);
+                }
+                // lower blocks at full indices f0 .. f0 + nlr - 1 (columns ascending);
+                // packed: the k-th below the diagonal at position k
+                for (int m = f0; m < f0 + nlr; ++m)
+                {
+                    const int pos = aligned ? static_cast<int>(std::lower_bound(lo.begin(), lo.end(), r - t.fcol[m]) -
+                                                               lo.begin())
+                                            : (f0 + nlr - 1 - m);
+                    masks[mbase + wu + pos] |= 1ull << l;
+                    lowPos[m] = static_cast<int>(nL + static_cast<long long>(W) * pos + l);
+                }
+            }
+            nE += static_cast<long long>(W) * wu;
+            nL += static_cast<long long>(W) * wl;
+        }
+    }
+    // the lower block at full index m = uMirror[u] is the transposed mirror of
+    // upper slot u
+    std::vector<int> mirE(static_cast<size_t>(std::max<long long>(nL, 1)), -1),
+        mirC(static_cast<size_t>(std::max<long long>(nL, 1)), 0);
+    long long placed = 0;
+    for (long long u = 0; u < t.nnzu(); ++u)
+    {
+        const int m = t.uMirror[u];
+        if (m < 0)
+            continue;
+        const int idx = m < t.nnzf() ? lowPos[m] : -1;
+        if (idx < 0 || mirE[idx] != -1 || slotEll[u] < 0)
+        {
+            set_error("upper-only storage: inconsistent mirror map");
+            return PK_ERR_ARG;
+        }
+        mirE[idx] = slotEll[u];
+        mirC[idx] = slotRow[u];
+        ++placed;
+    }
+    for (long long u = 0; u < t.nnzu(); ++u)
+        if (slotEll[u] < 0)
+        {
+            set_error("upper-only storage: an upper slot outside every body's rows");
+            return PK_ERR_ARG;
+        }
+    if (placed != t.nnzf() - t.nnzu())
+    {
+        set_error("upper-only storage: a lower block has no upper mirror");
+        return PK_ERR_ARG;
+    }
+    s->nE = nE;
+    std::vector<int> ecol(static_cast<size_t>(std::max<long long>(nE, 1)), 0);
+    for (long long u = 0; u < t.nnzu(); ++u)
+        ecol[slotEll[u]] = t.ucol[u];
+    for (auto& v : mirE)
+        v = std::max(v, 0);  // padding entries (never read)
+    const hipStream_t st = s->stream;
+    int rc;
+    if ((rc = s->ecol.upload(ecol, st)) != PK_OK) return rc;
+    if ((rc = s->slices.upload(desc, st)) != PK_OK) return rc;
+    if ((rc = s->smask.upload(masks, st)) != PK_OK) return rc;
+    if ((rc = s->soff.upload(offs, st)) != PK_OK) return rc;
+    if ((rc = s->mirE.upload(mirE, st)) != PK_OK) return rc;
+    if ((rc = s->mirC.upload(mirC, st)) != PK_OK) return rc;
+    if ((rc = s->rowSlice.upload(rowSlice, st)) != PK_OK) return rc;
+    return s->slotEll.upload(slotEll, st);
+}
+
 static int alloc_work(pk_system* s)
 {
     const Topology& t = s->topo;
@@ -861,9 +962,11 @@ static int alloc_work(pk_system* s)
     int rc;
     if ((rc = s->basis.alloc(static_cast<size_t>(t.T) * EB_COUNT)) != PK_OK) return rc;
     if ((rc = s->corner.alloc(t.nlist.size() * N_CORNER)) != PK_OK) return rc;
-    const size_t words = (is_f32(s) || is_upper(s)) ? 10 : BsrStore<double>::kWords;
-    const size_t blocks = static_cast<size_t>(is_upper(s) ? t.nnzu() : t.nnzf());
+    const size_t words = is_upper(s) ? SliceStore<double>::kWords : is_f32(s) ? 10 : BsrStore<double>::kWords;
+    const size_t blocks = static_cast<size_t>(is_upper(s) ? s->nE : t.nnzf());
     if ((rc = s->vals.alloc(blocks * words * es)) != PK_OK) return rc;
+    if (is_upper(s) && blocks)  // the padding elements of the slices stay zero (never read)
+        PK_HIP(hipMemsetAsync(s->vals.p, 0, blocks * words * es, s->stream));
     if ((rc = s->z.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->d0.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->d1.alloc(nd3 * es)) != PK_OK) return rc;
@@ -895,7 +998,6 @@ static int setup_windows(pk_system* s)
     const Topology& t = s->topo;
     s->winCap = 0;
     s->stageSegs = 0;
-    s->fuseDir = 0;
     if (s->stagingMode == 0 || t.nd == 0)
         return PK_OK;
     const size_t bytesPerRow = 3 * (is_f32(s) ? sizeof(float) : sizeof(double));
@@ -907,14 +1009,9 @@ static int setup_windows(pk_system* s)
     for (size_t ti = 0; ti < s->spmvHostTiles.size(); ++ti)
         {
             const pk::RowTile& tl = s->spmvHostTiles[ti];
-            const int2 c2 = s->spmvHostChunk2[ti];
-            cols.clear();
-            for (const int2 run : {make_int2(tl.rowBegin, tl.rowEnd), make_int2(c2.x, c2.x + c2.y)})
-            {
-                cols.insert(cols.end(), t.fcol.begin() + t.fstart[run.x], t.fcol.begin() + t.fstart[run.y]);
-                for (int r = run.x; r < run.y; ++r)
-                    cols.push_back(r);
-            }
+            cols.assign(t.fcol.begin() + t.fstart[tl.rowBegin], t.fcol.begin() + t.fstart[tl.rowEnd]);
+            for (int r = tl.rowBegin; r < tl.rowEnd; ++r)
+                cols.push_back(r);
             std::sort(cols.begin(), cols.end());
             cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
             const int len = cols.back() - cols.front() + 1;
@@ -954,12 +1051,10 @@ static int setup_windows(pk_system* s)
     if (widestWin <= cap)
     {
         s->winCap = widestWin;
-        s->fuseDir = s->stagingMode == 2 && !s->pencils ? 1 : 0;
         return s->spmvWin.upload(wins, s->stream);
     }
     s->stageSegs = 1;
     s->winCap = std::min(widestSeg, cap);
-    s->fuseDir = s->stagingMode == 2 && widestSeg <= cap && !s->pencils ? 1 : 0;
     return s->spmvWin.upload(segs, s->stream);
 }
 
@@ -1036,6 +1131,13 @@ static int setup_device(pk_system* s)
         diagSlot[r] = t.ustart[r];
     }
     if ((rc = s->diagSlot.upload(diagSlot, st)) != PK_OK) return rc;
+    if (!is_upper(s))
+    {
+        std::vector<int> fdiag(static_cast<size_t>(t.nd));
+        for (int r = 0; r < t.nd; ++r)
+            fdiag[r] = t.uFull[diagSlot[r]];
+        if ((rc = s->fdiag.upload(fdiag, st)) != PK_OK) return rc;
+    }
     if ((rc = s->rowGroup.upload(rowGroup, st)) != PK_OK) return rc;
     if ((rc = s->fstart.upload(t.fstart, st)) != PK_OK) return rc;
     if ((rc = s->fcol.upload(t.fcol, st)) != PK_OK) return rc;
@@ -1049,49 +1151,21 @@ static int setup_device(pk_system* s)
     if ((rc = s->cpos.upload(t.cpos, st)) != PK_OK) return rc;
     if (is_upper(s))
     {
-        // row r's lower blocks (c < r) are the compact entries
-        // [fstart[r] - ustart[r], fstart[r+1] - ustart[r+1]); the one at full
-        // index m of row r is the mirror of upper slot s (uMirror[s] == m),
-        // stored as (s, the row of s)
-        const long long nl = t.nnzf() - t.nnzu();
-        std::vector<int2> rp(static_cast<size_t>(t.nd) + 1), low(static_cast<size_t>(std::max<long long>(nl, 1)),
-                                                                 make_int2(-1, -1));
-        for (int r = 0; r <= t.nd; ++r)
-            rp[r] = make_int2(t.fstart[r] - t.ustart[r], t.ustart[r]);
-        for (long long u = 0; u < t.nnzu(); ++u)
-        {
-            const int m = t.uMirror[u];
-            if (m < 0)
-                continue;
-            const int r = t.ucol[u];
-            const long long l = static_cast<long long>(m) - t.ustart[r];
-            if (r < 0 || r >= t.nd || m < t.fstart[r] || l < rp[r].x || l >= rp[r + 1].x || low[l].x != -1)
-            {
-                set_error("upper-only storage: inconsistent mirror map");
-                return PK_ERR_ARG;
-            }
-            low[l] = make_int2(static_cast<int>(u), slotRow[u]);
-        }
-        for (long long l = 0; l < nl; ++l)
-            if (low[l].x < 0)
-            {
-                set_error("upper-only storage: a lower block has no upper mirror");
-                return PK_ERR_ARG;
-            }
-        if ((rc = s->rowptr.upload(rp, st)) != PK_OK) return rc;
-        if ((rc = s->lower.upload(low, st)) != PK_OK) return rc;
+        if ((rc = build_slices(s, slotRow)) != PK_OK) return rc;
         if ((rc = s->ucol.upload(t.ucol, st)) != PK_OK) return rc;
     }
     if ((rc = alloc_work(s)) != PK_OK) return rc;
     launch_gdot(s->view(), s->prm.p);
     if ((rc = s->kdat.alloc(static_cast<size_t>(t.T) * KD_COUNT)) != PK_OK) return rc;
     launch_stiff_records(s->view());
-    if (is_f32(s) || is_upper(s))
-        launch_record_columns(s->view(), is_f32(s));
+    if (is_f32(s) && !is_upper(s))
+        launch_record_columns(s->view(), true);
     std::vector<double> zero(static_cast<size_t>(t.N) * 3, 0.0);
     if ((rc = s->v.upload(zero, st)) != PK_OK) return rc;
     if ((rc = s->x.alloc(static_cast<size_t>(t.N) * 3)) != PK_OK) return rc;
-    s->lanesPerRow = choose_lanes(t);
+    s->lanesPerRow = is_upper(s) ? 1 : choose_lanes(t);
+    if (is_upper(s))
+        s->spmvRows = 8 * kSliceRows;  // two slices per wave
     if ((rc = setup_tiles(s)) != PK_OK) return rc;
     return upload_groups(s);
 }
@@ -1172,12 +1246,19 @@ int pk_system_sizes(const pk_system* s, int* nodes, int* tets, int* groups, int*
 
 int pk_system_tune(pk_system* s, int lanes_per_row, int spmv_rows, int vector_rows, int unroll)
 {
-    const bool lanesOk = lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16 || lanes_per_row == 32;
+    const bool lanesOk = lanes_per_row == 1 || lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16 ||
+                         lanes_per_row == 32;
     const bool unrollOk = unroll == 1 || unroll == 2 || (unroll == 4 && lanes_per_row <= 16);
     if (!s || !lanesOk || !unrollOk || spmv_rows < 1 || vector_rows < 1)
     {
-        set_error("pk_system_tune: lanes_per_row in {4, 8, 16, 32}, unroll in {1, 2, 4} (4 only up to 16 lanes), "
+        set_error("pk_system_tune: lanes_per_row in {1, 4, 8, 16, 32}, unroll in {1, 2, 4} (4 only up to 16 lanes), "
                   "positive row counts");
+        return PK_ERR_ARG;
+    }
+    if (is_upper(s) && (lanes_per_row != 1 || spmv_rows % kSliceRows != 0))
+    {
+        set_error("pk_system_tune: upper-only storage runs one lane per row (lanes_per_row 1) over whole slices "
+                  "(spmv_rows a multiple of 64)");
         return PK_ERR_ARG;
     }
     s->tapeValid = false;  // a recorded run no longer replays bit-identically
@@ -1190,9 +1271,9 @@ int pk_system_tune(pk_system* s, int lanes_per_row, int spmv_rows, int vector_ro
 
 int pk_system_set_spmv_staging(pk_system* s, int mode)
 {
-    if (!s || mode < 0 || mode > 2)
+    if (!s || mode < 0 || mode > 1)
     {
-        set_error("pk_system_set_spmv_staging: mode 0 (off), 1 (staging) or 2 (staging, direction fused)");
+        set_error("pk_system_set_spmv_staging: mode 0 (off) or 1 (staging)");
         return PK_ERR_ARG;
     }
     s->tapeValid = false;  // a recorded run no longer replays the same launches
@@ -1205,23 +1286,13 @@ int pk_system_set_spmv_staging(pk_system* s, int mode)
     return PK_OK;
 }
 
-int pk_system_set_spmv_pencils(pk_system* s, int enable)
-{
-    if (!s)
-        return PK_ERR_ARG;
-    s->pencils = enable != 0;
-    s->tapeValid = false;
-    return setup_tiles(s);
-}
-
-int pk_system_get_spmv_staging(const pk_system* s, int* mode, int* rows, int* segmented, int* fused)
+int pk_system_get_spmv_staging(const pk_system* s, int* mode, int* rows, int* segmented)
 {
     if (!s)
         return PK_ERR_ARG;
     if (mode) *mode = s->stagingMode;
     if (rows) *rows = s->winCap;
     if (segmented) *segmented = s->stageSegs;
-    if (fused) *fused = s->fuseDir;
     return PK_OK;
 }
 
@@ -1627,7 +1698,7 @@ static int backward_run(pk_system* s, double adjTol, int adjMaxIt, int* adjIters
     }
     if (store)
     {
-        if ((is_f32(s) || is_upper(s)) && s->segColumns != s->epoch)
+        if (is_f32(s) && !is_upper(s) && s->segColumns != s->epoch)
         {
             for (int j = 0; j < every; ++j)
             {
@@ -2283,19 +2354,20 @@ int pk_system_get_matrix(pk_system* s, double* full_vals, double* rhs, double* m
     PK_HIP(hipStreamSynchronize(s->stream));
     if (full_vals && F && is_upper(s))
     {
-        // one record per upper slot: the block at its full index, its
+        // one element per upper slot: the block at its full index, its
         // transpose at the mirror
-        const size_t U = static_cast<size_t>(t.nnzu());
+        const size_t U = static_cast<size_t>(t.nnzu()), nE = static_cast<size_t>(s->nE);
         const bool f32 = is_f32(s);
-        std::vector<char> bytes(U * 10 * (f32 ? 4 : 8));
+        std::vector<char> bytes(nE * SliceStore<double>::kWords * (f32 ? 4 : 8));
         PK_HIP(hipMemcpy(bytes.data(), s->vals.p, bytes.size(), hipMemcpyDeviceToHost));
-        std::vector<double> raw(U * 10);
-        for (size_t i = 0; i < raw.size(); ++i)
-            raw[i] = f32 ? reinterpret_cast<const float*>(bytes.data())[i]
-                         : reinterpret_cast<const double*>(bytes.data())[i];
+        std::vector<int> ell(U);
+        PK_HIP(hipMemcpy(ell.data(), s->slotEll.p, U * sizeof(int), hipMemcpyDeviceToHost));
         for (size_t u = 0; u < U; ++u)
         {
-            const double* w = raw.data() + upper_at(u, 0);
+            double w[9];
+            for (int e = 0; e < 9; ++e)
+                w[e] = f32 ? reinterpret_cast<const float*>(bytes.data())[SliceStore<float>::word(nE, ell[u], e)]
+                           : reinterpret_cast<const double*>(bytes.data())[SliceStore<double>::word(nE, ell[u], e)];
             const size_t full = static_cast<size_t>(t.uFull[u]);
             for (int e = 0; e < 9; ++e)
                 full_vals[9 * full + e] = w[e];

@@ -49,9 +49,7 @@ def main():
     ap.add_argument("--shape", default=None,
                     help="CG launch shape lanes,spmv_rows,vector_rows,unroll (default: the system's own)")
     ap.add_argument("--spheres", type=int, default=0)
-    ap.add_argument("--pencils", action="store_true", help="pencil SpMV tiles")
-    ap.add_argument("--staging", type=int, default=1, choices=[0, 1, 2],
-                    help="SpMV staging of d: 0 off, 1 on, 2 on with the direction update fused")
+    ap.add_argument("--staging", type=int, default=1, choices=[0, 1], help="SpMV staging of d: 0 off, 1 on")
     args = ap.parse_args()
     diag = diag_lib()
     ms = C.c_double()
@@ -64,8 +62,6 @@ def main():
     if args.shape:
         s.tune(*[int(a) for a in args.shape.split(",")])
     s.set_staging(args.staging)
-    if args.pencils:
-        s.set_pencils(True)
     extra = {}
     if args.spheres > 0:
         lo, hi = w.x0.min(0), w.x0.max(0)

@@ -84,7 +84,7 @@ def traffic(args):
     upper = prec.endswith("u")
     n_spmv = int(diag["spmv_launches"])
     spmv_alg = spmv_bytes(diag["spmv_blocks"], diag["spmv_rows"], diag["spmv_body_launches"], s,
-                          diag["spmv_upper_blocks"] if upper else None, bool(diag["staging"][3])) / max(n_spmv, 1)
+                          diag["spmv_upper_blocks"] if upper else None) / max(n_spmv, 1)
     stored = diag["nnzb_upper"] if upper else diag["nnzb_full"]
     asm_alg = assembly_bytes(diag["tets"], diag["nodes"], stored, diag["dynamic_rows"], s)
     table, asm, spmv = [], {}, None

@@ -9,7 +9,7 @@ against the measured time (achieved GB/s) or the HBM counters (traffic).
 from __future__ import annotations
 
 
-def spmv_bytes(blocks, rows, body_launches, s, upper_blocks=None, fused=False):
+def spmv_bytes(blocks, rows, body_launches, s, upper_blocks=None):
     """Algorithmic bytes of SpMV work q = A d, format-independent: every
     stored 3x3 block the launches multiplied once (9 values of s bytes + an
     int32 column), per row one int32 row pointer, d read once and q written
@@ -17,24 +17,21 @@ def spmv_bytes(blocks, rows, body_launches, s, upper_blocks=None, fused=False):
     launch: nnzb (9 s + 4) + (N_d + 1) 4 + 2 N_d 3 s for one launch over a
     whole body. With the upper triangle stored (upper_blocks = the stored
     blocks among `blocks`), nnzb is the upper blocks: each also serves its
-    mirror's transposed product. With the direction update fused into the
-    SpMV, a row reads z and d_k-1 and writes d_k instead of reading d_k (two
-    more vector passes)."""
-    vec = (4 if fused else 2) * 3 * s
+    mirror's transposed product."""
     stored = blocks if upper_blocks is None else upper_blocks
-    return stored * (9 * s + 4) + rows * (4 + vec) + 4 * body_launches
+    return stored * (9 * s + 4) + rows * (4 + 2 * 3 * s) + 4 * body_launches
 
 
-def spmv_format_bytes(blocks, rows, body_launches, s, upper_blocks=None, fused=False):
+def spmv_format_bytes(blocks, rows, body_launches, s, upper_blocks=None):
     """Bytes the executed format itself holds for the same work: full
-    storage as spmv_bytes; upper-only storage adds an 8-byte (slot, column)
-    entry per lower block, a pair of row pointers per row (lower start, upper
-    start) and, for FP64, an 8-byte column word per record (10 words of s
-    bytes)."""
-    vec = (4 if fused else 2) * 3 * s
+    storage as spmv_bytes; sliced upper-only storage holds a record of 10
+    words of s bytes per upper block (the FP64 column word is 8 bytes) and an
+    8-byte (mirror element, column) entry per lower block, and no row
+    pointers (its slice descriptors and lane masks, 16 bytes and 8 bytes per
+    ELL column for 64 rows, about 2 bytes per row, are not counted)."""
     if upper_blocks is None:
-        return spmv_bytes(blocks, rows, body_launches, s, None, fused)
-    return upper_blocks * 10 * s + (blocks - upper_blocks) * 8 + rows * (8 + vec) + 8 * body_launches
+        return spmv_bytes(blocks, rows, body_launches, s, None)
+    return upper_blocks * 10 * s + (blocks - upper_blocks) * 8 + rows * 2 * 3 * s
 
 
 def assembly_bytes(tets, nodes, stored_blocks, rows, s_matrix):

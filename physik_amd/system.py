@@ -102,20 +102,15 @@ class System:
         return autotune(self, dt, cache=cache, verbose=verbose)
 
     def set_staging(self, mode=1):
-        """SpMV staging of d in LDS: 0 off, 1 on, 2 on with the direction update
-        fused into the SpMV where every tile stages (identical results)."""
+        """SpMV staging of d in LDS: 0 off, 1 on (identical results)."""
         N.check(self.lib.pk_system_set_spmv_staging(self.h, int(mode)), "pk_system_set_spmv_staging")
 
-    def set_pencils(self, enable=True):
-        """Pencil SpMV tiles: a run of rows paired with the run a layer later."""
-        N.check(self.lib.pk_system_set_spmv_pencils(self.h, int(enable)), "pk_system_set_spmv_pencils")
-
     def get_staging(self):
-        """(mode, rows a workgroup stages, segments in use, direction fused)."""
-        m, r, g, f = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        N.check(self.lib.pk_system_get_spmv_staging(self.h, C.byref(m), C.byref(r), C.byref(g), C.byref(f)),
+        """(mode, rows a workgroup stages, segments in use)."""
+        m, r, g = C.c_int(), C.c_int(), C.c_int()
+        N.check(self.lib.pk_system_get_spmv_staging(self.h, C.byref(m), C.byref(r), C.byref(g)),
                 "pk_system_get_spmv_staging")
-        return m.value, r.value, bool(g.value), bool(f.value)
+        return m.value, r.value, bool(g.value)
 
     def get_tune(self):
         v = [C.c_int() for _ in range(4)]

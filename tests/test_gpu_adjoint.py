@@ -20,6 +20,7 @@ import numpy as np
 import pytest
 
 from checkers import Oracle
+from launch_shapes import shape_for
 from physik_amd import meshgen as mg
 from physik_amd import native as N
 from physik_amd import workloads as W
@@ -251,7 +252,7 @@ def test_segment_matrix_store_gives_identical_gradients(precision):
     out = []
     for cap, graphs in ((0, False), (1 << 30, False), (1 << 30, True), (1 << 30, False)):
         s = W.make_system(w, precision)
-        s.tune(8, 32, 64, 2)
+        s.tune(*shape_for(precision, 8, 32, 64, 2))
         s.set_matrix_store(cap)
         s.set_graphs(graphs)
         s.record(w.dt, w.steps, 3)

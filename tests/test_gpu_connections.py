@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from checkers import Oracle
+from launch_shapes import PAIR_SHAPE
 from physik_amd import meshgen as mg
 from physik_amd import native as N
 from physik_amd import workloads as W
@@ -55,8 +56,10 @@ def far_pairs(w):
     return [(a, b) for i, a in enumerate(n) for b in n[i + 1:]]
 
 
-def run(w, precision, conn, implicit=True, extra_pairs=None):
+def run(w, precision, conn, implicit=True, extra_pairs=None, shape=None):
     s = W.make_system(w, precision) if extra_pairs is None else make_with_pairs(w, precision, extra_pairs)
+    if shape:
+        s.tune(*shape)
     s.set_mode(implicit)
     o = Oracle(64, w.rest, w.tets, w.young, w.poisson, w.damping, w.density)
     o.set_params(w.gravity, w.substeps, w.tol, w.max_iter, implicit=implicit)
@@ -109,8 +112,8 @@ def test_connection_outside_the_pattern_is_rejected_then_accepted_with_extra_pai
 def test_upper_and_fp32_storage_with_connections():
     w = beam()
     print("case connections, upper and fp32 storage", flush=True)
-    full = run(w, N.PK_FP64, frames(w), extra_pairs=far_pairs(w))
-    upper = run(w, N.PK_FP64_UPPER, frames(w), extra_pairs=far_pairs(w))
+    full = run(w, N.PK_FP64, frames(w), extra_pairs=far_pairs(w), shape=PAIR_SHAPE)
+    upper = run(w, N.PK_FP64_UPPER, frames(w), extra_pairs=far_pairs(w), shape=PAIR_SHAPE)
     for a, b in zip(full, upper):
         assert np.array_equal(a[0][0], b[0][0]) and np.array_equal(a[0][1], b[0][1]) and a[2] == b[2]
     for f, ((xg, vg), (xo, vo), _, _) in enumerate(run(w, N.PK_FP32_UPPER, frames(w), extra_pairs=far_pairs(w))):

@@ -77,15 +77,13 @@ def test_batched_graph_steps_equal_launched_steps():
     assert len(set(it.tolist())) > 1, "bodies should stop at different iterations"
 
 
-@pytest.mark.parametrize("staging", [0, 2])
+@pytest.mark.parametrize("staging", [0, 1])
 def test_graphs_recaptured_after_tune_and_profiling(staging):
     w = W.batch256(bodies=6, lo=3, hi=7, steps=4)
     print(f"case recapture, staging {staging}", flush=True)
     a, b = pair(w, shape=(16, 64, 128, 1))
     a.set_staging(staging)
     b.set_staging(staging)
-    fused = a.get_staging()[3]
-    assert fused == (staging == 2)  # small bodies: every tile stages
     same_step(a, b, w, 1)
     a.tune(4, 32, 64, 4)
     b.tune(4, 32, 64, 4)
@@ -94,11 +92,8 @@ def test_graphs_recaptured_after_tune_and_profiling(staging):
     b.profile(True)
     same_step(a, b, w, 2)
     # the stamped spans and the work they cover agree launch for launch
-    for k in (N.K_SPMV, N.K_UPDATE):
+    for k in (N.K_SPMV, N.K_UPDATE, N.K_DIRECTION):
         assert a.kernel_time(k)[0] == b.kernel_time(k)[0] > 0
-    # the fused SpMV forms the direction: no direction launch then
-    assert a.kernel_time(N.K_DIRECTION)[0] == b.kernel_time(N.K_DIRECTION)[0]
-    assert (a.kernel_time(N.K_DIRECTION)[0] == 0) == fused
     assert a.spmv_work() == b.spmv_work()
     b.set_graphs(False)
     same_step(a, b, w, 1)

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from checkers import Oracle
+from launch_shapes import PAIR_SHAPE
 from physik_amd import meshgen as mg
 from physik_amd import native as N
 from physik_amd import workloads as W
@@ -49,8 +50,10 @@ def connections(w):
             np.array([500.0]), np.array([2.0]))
 
 
-def run(w, precision, implicit=True, with_connections=False):
+def run(w, precision, implicit=True, with_connections=False, shape=None):
     s = W.make_system(w, precision)
+    if shape:
+        s.tune(*shape)
     s.set_mode(implicit)
     o = Oracle(64, w.rest, w.tets, w.young, w.poisson, w.damping, w.density)
     o.set_params(w.gravity, w.substeps, w.tol, w.max_iter, implicit=implicit)
@@ -101,8 +104,8 @@ def test_spheres_with_connections_match_the_oracle():
 def test_upper_and_fp32_storage_with_spheres():
     w = beam()
     print("case spheres, upper and fp32 storage", flush=True)
-    full = run(w, N.PK_FP64)
-    upper = run(w, N.PK_FP64_UPPER)
+    full = run(w, N.PK_FP64, shape=PAIR_SHAPE)
+    upper = run(w, N.PK_FP64_UPPER, shape=PAIR_SHAPE)
     for a, b in zip(full, upper):
         assert np.array_equal(a[0][0], b[0][0]) and np.array_equal(a[0][1], b[0][1]) and a[2] == b[2]
     for f, ((xg, vg), (xo, vo), _, _) in enumerate(run(w, N.PK_FP32_UPPER)):

@@ -17,6 +17,8 @@ from physik_amd import native as N
 from physik_amd import workloads as W
 from physik_amd.system import Pattern
 
+from launch_shapes import shape_for
+
 pytestmark = pytest.mark.gpu
 
 STAGE_BYTES = 20 * 1024  # the library's per-workgroup staging budget
@@ -86,35 +88,34 @@ def windows(w, rows_per_tile):
     return np.array(wins), np.array(segs)
 
 
-@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("bar", [True, False])
 @pytest.mark.parametrize("precision", [N.PK_FP32_UPPER, N.PK_FP64_UPPER, N.PK_FP32_STORAGE, N.PK_FP64])
 @pytest.mark.parametrize("shape", [(8, 128, 512, 2), (4, 64, 256, 1), (16, 32, 256, 4), (32, 256, 1024, 2)])
-def test_staged_tiles_step_like_global_gathers(precision, shape, bar, mode):
+def test_staged_tiles_step_like_global_gathers(precision, shape, bar):
     w = mixed_batch(bar=bar)
+    shape = shape_for(precision, *shape)
     a = W.make_system(w, precision)
     b = W.make_system(w, precision)
     a.tune(*shape)
     b.tune(*shape)
-    a.set_staging(mode)
+    a.set_staging(1)
     b.set_staging(0)
-    got_mode, rows, segmented, fused = a.get_staging()
+    got_mode, rows, segmented = a.get_staging()
     f32 = precision in (N.PK_FP32_UPPER, N.PK_FP32_STORAGE)
     cap = STAGE_BYTES // (3 * (4 if f32 else 8))
     wins, segs = windows(w, shape[1])
-    print(f"precision {precision} shape {shape} bar {bar} mode {mode}: staged rows {rows}, segmented {segmented}, "
-          f"fused {fused}, windows {wins.min()}..{wins.max()}, segments {segs.min()}..{segs.max()}", flush=True)
-    assert got_mode == mode
+    print(f"precision {precision} shape {shape} bar {bar}: staged rows {rows}, segmented {segmented}, "
+          f"windows {wins.min()}..{wins.max()}, segments {segs.min()}..{segs.max()}", flush=True)
+    assert got_mode == 1
     if wins.max() <= cap:
         assert not segmented and rows == wins.max()
     else:
         assert segmented and rows == min(segs.max(), cap)
     if bar:
         assert segmented and (segs <= rows).any() and (segs > rows).any(), "tiles on both sides of the capacity"
-        assert not fused  # a tile that gathers from global memory needs d_k from a direction launch
     else:
-        assert not segmented and fused == (mode == 2)
-    assert b.get_staging() == (0, 0, False, False)
+        assert not segmented
+    assert b.get_staging() == (0, 0, False)
     for n in range(w.steps):
         a.step(w.dt)
         b.step(w.dt)
@@ -132,18 +133,14 @@ def test_staging_is_switchable():
     w = mixed_batch()
     for precision in (N.PK_FP32_STORAGE, N.PK_FP32_UPPER):
         s = W.make_system(w, precision)
-        mode, rows, segmented, fused = s.get_staging()
-        assert mode == 1 and segmented and not fused and rows == 1706  # the bar's tiles need more than the capacity
-        s.set_staging(2)
-        assert s.get_staging() == (2, rows, True, False)  # a tile gathers from global memory: no fusing
+        mode, rows, segmented = s.get_staging()
+        assert mode == 1 and segmented and rows == 1706  # the bar's tiles need more than the capacity
         s.set_staging(0)
-        assert s.get_staging() == (0, 0, False, False)
+        assert s.get_staging() == (0, 0, False)
         s.set_staging(1)
-        assert s.get_staging() == (1, rows, True, False)
-        assert s.lib.pk_system_set_spmv_staging(s.h, 3) == N.PK_ERR_ARG
+        assert s.get_staging() == (1, rows, True)
+        assert s.lib.pk_system_set_spmv_staging(s.h, 2) == N.PK_ERR_ARG
         s.close()
     s = W.make_system(mixed_batch(bar=False), N.PK_FP32_UPPER)
-    assert s.get_staging()[1:] == (s.get_staging()[1], False, False)
-    s.set_staging(2)
-    assert s.get_staging()[3]  # every tile stages: the direction update is fused
+    assert s.get_staging()[1:] == (s.get_staging()[1], False)
     s.close()
