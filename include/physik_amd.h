@@ -211,8 +211,9 @@ int pk_system_group_results(pk_system* sys, int* iterations, double* residual_no
  * of range or inactive, k or c is negative or non-finite, or the target is
  * non-finite. Every pair of dynamic nodes a connection couples must lie in the
  * system's BSR pattern (pk_mesh_desc.extra_pairs adds pairs): PK_ERR_ARG
- * otherwise, with the connections left as they were. The reverse mode does
- * not cover connections: pk_system_record fails while any is set. */
+ * otherwise, with the connections left as they were. The reverse mode
+ * (pk_system_record / backward) differentiates their forces (the targets,
+ * stiffness and damping are constants of the run). */
 int pk_system_set_connections(pk_system* sys, int count, const int* nodes, const double* weights,
                               const double* targets, const double* stiffness, const double* damping);
 
@@ -240,8 +241,10 @@ int pk_system_set_tets_active(pk_system* sys, int count, const int* tets, const 
  * the point connections. `count` spheres: centers 3 x count, radii, stiffness
  * and damping finite and >= 0 (PK_ERR_ARG otherwise; the PHYSIK_* layer
  * clamps as the reference does, PhysiKAPI.cpp:685-700, 1180-1194). A sphere of
- * radius 0 touches nothing. count 0 removes them. The reverse mode does not
- * cover spheres: pk_system_record fails while any is set. */
+ * radius 0 touches nothing. count 0 removes them. The reverse mode
+ * differentiates the contact forces (their targets move with the centroid;
+ * which tets touch a sphere is piecewise constant and contributes no
+ * derivative). */
 int pk_system_set_spheres(pk_system* sys, int count, const double* centers, const double* radii,
                           const double* stiffness, const double* damping);
 

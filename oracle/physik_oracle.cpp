@@ -710,12 +710,13 @@ struct World
     // the sphere's surface along the centre-to-centroid direction. The
     // reference appends these after the frame's connections and drops them at
     // the end of the substep (World.cpp:54-67, 727-740).
-    std::vector<Conn> sphere_contacts() const
+    std::vector<Conn> sphere_contacts(std::vector<int>* sphereOf = nullptr) const
     {
         std::vector<Conn> out;
         const T w = T(0.25f);
-        for (const Sphere& s : spheres)
+        for (size_t si = 0; si < spheres.size(); ++si)
         {
+            const Sphere& s = spheres[si];
             if (s.r <= T(0))
                 continue;
             for (const auto& t : tets)
@@ -749,7 +750,11 @@ struct World
                 c.k = s.k;
                 c.c = s.d;
                 if (conn_nodes_valid(c))  // World::AddPointConnection (World.cpp:126-131)
+                {
                     out.push_back(c);
+                    if (sphereOf)
+                        sphereOf->push_back(static_cast<int>(si));
+                }
             }
         }
         return out;
@@ -1334,6 +1339,19 @@ int linear_solve_t(int n, int nnz, const int* rs, const int* ci, const double* v
 // lambda and mu scale with E, so d Phi / d log E_e is the elastic part of
 // Phi_e. R = polar(F) is differentiated as the exact polar factor:
 // dR = R Omega with S Omega + Omega S = R^T dF - dF^T R, S = R^T F.
+// A point connection (PointConnection.cpp:54-130) on nodes n_a with weights
+// w_a pushes the force w_a F, F = k G(p) - c u with p = sum w_a x_a,
+// u = sum w_a v_a and G = t - p, and the constant blocks k w_a w_b I; with
+// lam_c = sum_a w_a lam_a it adds
+//   Phi_c = dt lam_c . F - dt^2 k lam_c . v'_c  (v'_c = sum w_a v'_a),
+//   dPhi_c/dx_a = dt k w_a (dG/dp)^T lam_c,
+//   dPhi_c/dv_a = -(dt c + dt^2 k) w_a lam_c.
+// A frame connection has a fixed target: dG/dp = -I. A sphere contact
+// (CollisionSphereComponent.cpp:112-193) targets t = p + n (r - |d|),
+// d = p - o, n = d / |d| (or (0, 0, 1) within 1e-5 of the centre o):
+//   G = d (r / |d| - 1),  dG/dp = (r / |d| - 1) I - r d d^T / |d|^3,
+// or G = (0, 0, r - |d|), dG/dp = -e_z d^T / |d| near the centre. Which
+// tets touch a sphere is piecewise constant in x and adds no derivative.
 // ---------------------------------------------------------------------------
 namespace oracle
 {
@@ -1530,6 +1548,47 @@ int adjoint_run(World<T>& w, T dt, int steps, std::vector<V3<T>> xb, std::vector
                 vb[t.n[a]] = add(vb[t.n[a]], o.vb[a]);
             }
         }
+        // point connections: the frame's, then the substep's sphere contacts,
+        // each as push_connection keeps it
+        using Conn = typename World<T>::Conn;
+        using Sphere = typename World<T>::Sphere;
+        auto connection_terms = [&](const Conn& c, const Sphere* sp) {
+            if (!w.conn_nodes_valid(c) || !finite(c.k) || !finite(c.c) || c.k < T(0) || c.c < T(0))
+                return;
+            V3<T> p{T(0), T(0), T(0)}, vel{T(0), T(0), T(0)}, lc{T(0), T(0), T(0)};
+            for (int a = 0; a < 4; ++a)
+            {
+                p = add(p, scl(w.x[c.n[a]], c.w[a]));
+                vel = add(vel, scl(w.v[c.n[a]], c.w[a]));
+                lc = add(lc, scl(lamNode[c.n[a]], c.w[a]));
+            }
+            if (!finite(p) || !finite(vel) || !finite(c.target))
+                return;
+            V3<T> gx = scl(lc, -dt * c.k);  // dG/dp = -I
+            if (sp)
+            {
+                const V3<T> d = sub(p, sp->c);
+                const T dist = std::sqrt(dot(d, d));
+                V3<T> jl{T(0), T(0), T(0)};  // (dG/dp)^T lam_c
+                if (dist > T(0.00001f))
+                    jl = sub(scl(lc, sp->r / dist - T(1)), scl(d, sp->r * dot(d, lc) / (dist * dist * dist)));
+                else if (dist > T(0))
+                    jl = scl(d, -lc.z / dist);
+                gx = scl(jl, dt * c.k);
+            }
+            const V3<T> gv = scl(lc, -(dt * c.c + dt * dt * c.k));
+            for (int a = 0; a < 4; ++a)
+            {
+                xb[c.n[a]] = add(xb[c.n[a]], scl(gx, c.w[a]));
+                vb[c.n[a]] = add(vb[c.n[a]], scl(gv, c.w[a]));
+            }
+        };
+        for (const Conn& c : w.conns)
+            connection_terms(c, nullptr);
+        std::vector<int> sphereOf;
+        const std::vector<Conn> contacts = w.sphere_contacts(&sphereOf);
+        for (size_t i = 0; i < contacts.size(); ++i)
+            connection_terms(contacts[i], &w.spheres[static_cast<size_t>(sphereOf[i])]);
     }
     w.x = xEnd;
     w.v = vEnd;
@@ -1938,8 +1997,8 @@ int orc_adjoint(void* p, double dt, int steps, const double* xbar, const double*
     if (h->precision != 64)
         return -1;
     auto& w = *h->w64;
-    if (!w.conns.empty() || !w.spheres.empty() || w.femModel != 1)
-        return -1;  // the reverse substep has no connection terms and covers the corotational model only
+    if (w.femModel != 1)
+        return -1;  // the reverse substep covers the corotational model only
     std::vector<oracle::V3<double>> xb(static_cast<size_t>(w.N)), vb(static_cast<size_t>(w.N));
     for (int i = 0; i < w.N; ++i)
     {

@@ -19,8 +19,24 @@
 //   k_adj_rhs       u on dynamic rows and the CG start for A lam = u
 //   k_adj_element   one lane per tet: dPhi/dlogE (accumulated in place) and
 //                   the tet's 4 corner terms of dPhi/dx and dPhi/dv
+//   k_adj_conn      one lane per point connection: its terms of dPhi/dx and
+//                   dPhi/dv before the corner weight
+//   k_adj_sphere    one lane per candidate tet: its sphere contacts' terms
 //   k_adj_nodes     one lane per node: xb, vb from the corner terms of its
-//                   tets in tet order (no atomics)
+//                   tets in tet order, then of its connections in connection
+//                   order and its candidate tets in candidate order (no
+//                   atomics)
+//
+// Contacts. A point connection (PointConnection.cpp:54-130) on nodes n_a with
+// weights w_a pushes w_a F, F = k G(p) - c u (p = sum w_a x_a, u = sum w_a
+// v_a, G = t - p), and the constant blocks k w_a w_b I; with lam_c = sum_a
+// w_a lam_a it adds dPhi/dx_a = dt k w_a (dG/dp)^T lam_c and dPhi/dv_a =
+// -(dt c + dt^2 k) w_a lam_c. A connection's target is fixed (dG/dp = -I);
+// a sphere contact (CollisionSphereComponent.cpp:112-193) targets
+// t = p + n (r - |d|), d = p - o, n = d / |d| (or (0, 0, 1) within 1e-5 of
+// the centre o), so G = d (r / |d| - 1) and dG/dp = (r / |d| - 1) I -
+// r d d^T / |d|^3 (near the centre G = (0, 0, r - |d|), dG/dp = -e_z d^T / |d|).
+// Which tets touch a sphere is piecewise constant in x: no derivative.
 #include "pk_device.h"
 #include "pk_fem.h"
 #include "pk_topology.h"
@@ -296,8 +312,14 @@ __global__ __launch_bounds__(NT) void k_adj_element(int T, const int4* __restric
 __global__ __launch_bounds__(NT) void k_adj_nodes(int N, int T, const int* __restrict__ astart,
                                                   const int* __restrict__ alist, const double* __restrict__ elem,
                                                   const int* __restrict__ nodeRow, const int* __restrict__ nodeGroup,
-                                                  const int* __restrict__ ok, double dt, double* __restrict__ xbar,
-                                                  double* __restrict__ vbar)
+                                                  const int* __restrict__ ok, const int* __restrict__ connStart,
+                                                  const int2* __restrict__ connList,
+                                                  const double* __restrict__ connData,
+                                                  const double* __restrict__ connAdj,
+                                                  const int* __restrict__ candStart,
+                                                  const int* __restrict__ candList,
+                                                  const double* __restrict__ candAdj, double dt,
+                                                  double* __restrict__ xbar, double* __restrict__ vbar)
 {
     const int n = xcd_block() * NT + static_cast<int>(threadIdx.x);
     if (n >= N || !ok[nodeGroup[n]])
@@ -329,12 +351,187 @@ __global__ __launch_bounds__(NT) void k_adj_nodes(int N, int T, const int* __res
             vb[i] += e[(EP_ADJ_V + i) * TT];
         }
     }
+    if (connStart)
+        for (int k = connStart[n]; k < connStart[n + 1]; ++k)
+        {
+            const int2 en = connList[k];  // (connection, corner)
+            const double wa = connData[9 * static_cast<size_t>(en.x) + en.y];
+            const double* e = connAdj + 6 * static_cast<size_t>(en.x);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                xb[i] += wa * e[i];
+                vb[i] += wa * e[3 + i];
+            }
+        }
+    if (candStart)
+        for (int k = candStart[n]; k < candStart[n + 1]; ++k)
+        {
+            const double* e = candAdj + 6 * static_cast<size_t>(candList[k]);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                xb[i] += e[i];
+                vb[i] += e[3 + i];
+            }
+        }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
         xbar[3 * n + i] = xb[i];
         vbar[3 * n + i] = vb[i];
     }
+}
+
+// lam at node n: the adjoint solution on its dynamic row, 0 elsewhere.
+__device__ inline double lam_at(const int* __restrict__ nodeRow, const double* __restrict__ lam, int n, int i)
+{
+    const int row = nodeRow[n];
+    return row >= 0 ? lam[3 * row + i] : 0.0;
+}
+
+// Per point connection: gx = -dt k lam_c and gv = -(dt c + dt^2 k) lam_c
+// (the node gather scales them by w_a); zeros when the forward pushed no
+// force (point or velocity not finite).
+__global__ __launch_bounds__(NT) void k_adj_conn(int n, const int4* __restrict__ nodes,
+                                                 const double* __restrict__ data, const double* __restrict__ x,
+                                                 const double* __restrict__ v, const int* __restrict__ nodeRow,
+                                                 const double* __restrict__ lam, double dt, double* __restrict__ out)
+{
+    const int c = blockIdx.x * NT + threadIdx.x;
+    if (c >= n)
+        return;
+    const int4 nd = nodes[c];
+    const int ids[4] = {nd.x, nd.y, nd.z, nd.w};
+    const double* d = data + 9 * static_cast<size_t>(c);
+    double lc[3] = {0.0, 0.0, 0.0};
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        // the point and velocity as k_conn_eval forms them
+        const double p = ((x[3 * nd.x + i] * d[0] + x[3 * nd.y + i] * d[1]) + x[3 * nd.z + i] * d[2]) +
+                         x[3 * nd.w + i] * d[3];
+        const double u = ((v[3 * nd.x + i] * d[0] + v[3 * nd.y + i] * d[1]) + v[3 * nd.z + i] * d[2]) +
+                         v[3 * nd.w + i] * d[3];
+        ok = ok && fin(p) && fin(u) && fin(d[4 + i]);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            lc[i] += d[a] * lam_at(nodeRow, lam, ids[a], i);
+    }
+    double* o = out + 6 * static_cast<size_t>(c);
+    const double k = d[7], cd = d[8];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        o[i] = ok ? -dt * k * lc[i] : 0.0;
+        o[3 + i] = ok ? -(dt * cd + dt * dt * k) * lc[i] : 0.0;
+    }
+}
+
+// Per candidate tet: the sum over the spheres it touches (the contacts
+// k_sphere_contacts makes, tested the same way) of dt k (dG/dp)^T lam_t and
+// -(dt c + dt^2 k) lam_t, lam_t = (1/4) sum_a lam_a, times the corner weight
+// 1/4: what each of its four nodes receives.
+__global__ __launch_bounds__(NT) void k_adj_sphere(int n, const int* __restrict__ cand,
+                                                   const unsigned char* __restrict__ tact,
+                                                   const int4* __restrict__ tets, const double* __restrict__ x,
+                                                   const double* __restrict__ v, int nsph,
+                                                   const double* __restrict__ sph, const int* __restrict__ nodeRow,
+                                                   const double* __restrict__ lam, double dt,
+                                                   double* __restrict__ out)
+{
+    const int i = xcd_block() * NT + static_cast<int>(threadIdx.x);
+    if (i >= n)
+        return;
+    double* o = out + 6 * static_cast<size_t>(i);
+    const int tt = cand[i];
+    if (!tact[tt])
+    {
+        for (int j = 0; j < 6; ++j)
+            o[j] = 0.0;
+        return;
+    }
+    const int4 nd = tets[tt];
+    const int ids[4] = {nd.x, nd.y, nd.z, nd.w};
+    const double w = 0.25;
+    double p[3], u[3], lt[3];
+    bool fin3 = true;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+    {
+        p[j] = ((x[3 * nd.x + j] * w + x[3 * nd.y + j] * w) + x[3 * nd.z + j] * w) + x[3 * nd.w + j] * w;
+        u[j] = ((v[3 * nd.x + j] * w + v[3 * nd.y + j] * w) + v[3 * nd.z + j] * w) + v[3 * nd.w + j] * w;
+        fin3 = fin3 && fin(p[j]) && fin(u[j]);
+        lt[j] = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            lt[j] += w * lam_at(nodeRow, lam, ids[a], j);
+    }
+    double gx[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
+    const double dt2 = dt * dt;
+    for (int s = 0; s < nsph; ++s)
+    {
+        const double* sp = sph + 6 * s;
+        const double r = sp[3];
+        if (!(r > 0.0))
+            continue;
+        const double d[3] = {p[0] - sp[0], p[1] - sp[1], p[2] - sp[2]};
+        const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (dist >= r)
+            continue;
+        const bool unit = dist > static_cast<double>(0.00001f);
+        const double depth = r - dist;
+        if (depth <= 0.0)
+            continue;
+        bool ok = fin3;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            const double nj = unit ? d[j] / dist : (j == 2 ? 1.0 : 0.0);
+            ok = ok && fin(p[j] + nj * depth);
+        }
+        if (!ok)
+            continue;
+        const double k = sp[4], c = sp[5];
+        double jl[3];
+        if (unit)
+        {
+            const double dl = d[0] * lt[0] + d[1] * lt[1] + d[2] * lt[2];
+            const double f = r / dist - 1.0, g = r * dl / (dist * dist * dist);
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                jl[j] = f * lt[j] - g * d[j];
+        }
+        else
+        {
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                jl[j] = dist > 0.0 ? -lt[2] * d[j] / dist : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            gx[j] += dt * k * jl[j];
+            gv[j] += -(dt * c + dt2 * k) * lt[j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+    {
+        o[j] = w * gx[j];
+        o[3 + j] = w * gv[j];
+    }
+}
+
+void launch_adj_contacts(const DeviceView& v, double dt)
+{
+    if (v.nconn > 0)
+        hipLaunchKernelGGL(k_adj_conn, dim3((v.nconn + NT - 1) / NT), dim3(NT), 0, v.stream, v.nconn, v.connNodes,
+                           v.connData, v.x, v.v, v.nodeRow, v.xs, dt, v.connAdj);
+    if (v.nsph > 0 && v.ncand > 0)
+        hipLaunchKernelGGL(k_adj_sphere, dim3(xcd_grid((v.ncand + NT - 1) / NT)), dim3(NT), 0, v.stream, v.ncand,
+                           v.cand, v.tact, v.tets, v.x, v.v, v.nsph, v.sph, v.nodeRow, v.xs, dt, v.candAdj);
 }
 
 void launch_record_ok(const DeviceView& v, int* ok)
@@ -391,8 +588,10 @@ void launch_adj_nodes(const DeviceView& v, const int* ok, double dt, double* xba
 {
     if (v.N > 0)
         hipLaunchKernelGGL(k_adj_nodes, dim3(xcd_grid((v.N + NT - 1) / NT)), dim3(NT), 0, v.stream, v.N, v.T,
-                           v.astart, v.alist,
-                           v.adj, v.nodeRow, v.nodeGroup, ok, dt, xbar, vbar);
+                           v.astart, v.alist, v.adj, v.nodeRow, v.nodeGroup, ok,
+                           v.nconn > 0 ? v.connNodeStart : nullptr, v.connNodeList, v.connData, v.connAdj,
+                           (v.nsph > 0 && v.ncand > 0) ? v.candNodeStart : nullptr, v.candNodeList, v.candAdj, dt,
+                           xbar, vbar);
 }
 
 template void launch_adj_rhs<double>(const DeviceView&, const int*, const double*, const double*, double);

@@ -387,6 +387,9 @@ struct DeviceView
     const int* connRowStart; // nconnRows + 1 offsets into connEntries
     const int2* connEntries; // (connection, corner a) in connection order
     double* crhs;            // nd x 3: the connections' right-hand side (or force) per row
+    const int* connNodeStart;  // N + 1 offsets into connNodeList
+    const int2* connNodeList;  // node -> (connection, corner a), w_a != 0, in connection order
+    double* connAdj;           // 6 per connection: reverse-mode terms before the corner weight
     const double* cslot;     // per upper slot: sum of k w_a w_b (null without connections)
     // collision spheres (none when nsph == 0): per substep, each candidate
     // tet's contact terms, then their sums per upper slot and per row on top
@@ -404,6 +407,9 @@ struct DeviceView
     const int* crList;       // dynamic row -> candidates, once per corner on that row
     double* sslot;           // per upper slot: cslot + the contacts' k/16
     double* srhs;            // nd x 3: crhs + the contacts' terms / 4
+    const int* candNodeStart;  // N + 1 offsets into candNodeList
+    const int* candNodeList;   // node -> candidate tets, once per corner, in candidate order
+    double* candAdj;           // 6 per candidate: reverse-mode terms each corner receives
     // what the assembly adds per upper slot and the right-hand side (or the
     // explicit force) per row: sslot/srhs with spheres, else cslot/crhs with
     // connections, else null
@@ -491,5 +497,6 @@ void launch_save_failed(const DeviceView& v, int* out);
 void launch_restore_failed(const DeviceView& v, const int* in);
 void launch_adj_element(const DeviceView& v, const int* ok, const double* vnext, double dt, double* gradLogE);
 void launch_adj_nodes(const DeviceView& v, const int* ok, double dt, double* xbar, double* vbar);
+void launch_adj_contacts(const DeviceView& v, double dt);
 
 }  // namespace pk

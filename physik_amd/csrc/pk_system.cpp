@@ -220,6 +220,11 @@ struct pk_system
     int nsph = 0, ncand = -1;
     pk::DevBuf<double> sph, ck, cg, sslot, srhs;
     pk::DevBuf<int> cand, csStart, csList, crStart, crList;
+    // reverse mode over contacts: per node its connections and candidate
+    // tets, and their per-substep terms
+    pk::DevBuf<int> connNodeStart, candNodeStart, candNodeList;
+    pk::DevBuf<int2> connNodeList;
+    pk::DevBuf<double> connAdj, candAdj;
 
     // CG chunks replayed from captured HIP graphs: the first chunk of a solve
     // (full tiles) and every later one (compaction first, for a batch). A
@@ -329,6 +334,12 @@ struct pk_system
         dv.nconnRows = nconnRows;
         dv.connRows = connRows.p;
         dv.connRowStart = connRowStart.p;
+        dv.connNodeStart = connNodeStart.p;
+        dv.connNodeList = connNodeList.p;
+        dv.connAdj = connAdj.p;
+        dv.candNodeStart = candNodeStart.p;
+        dv.candNodeList = candNodeList.p;
+        dv.candAdj = candAdj.p;
         dv.connEntries = connEntries.p;
         dv.crhs = crhs.p;
         dv.cslot = cslot.p;
@@ -1620,6 +1631,7 @@ static int reverse_substep(pk_system* s, double dt, int j, double adjTol, int ad
     if ((rc = run_cg<M>(s, adjMaxIt, s->lastItersAdj)) != PK_OK)
         return rc;
     launch_adj_element(dv, ok, s->segV.p + (j + 1) * n3, dt, s->gradE.p);
+    launch_adj_contacts(dv, dt);
     launch_adj_nodes(dv, ok, dt, s->xbar.p, s->vbar.p);
     PK_HIP(hipGetLastError());
     if ((rc = read_groups(s, s->lastItersAdj)) != PK_OK)
@@ -1761,11 +1773,6 @@ int pk_system_record(pk_system* s, double dt, int steps, int every)
     if (!s->implicit)
     {
         set_error("pk_system_record: the reverse mode covers the implicit integrator only");
-        return PK_ERR_ARG;
-    }
-    if (s->nconn > 0 || s->nsph > 0)
-    {
-        set_error("pk_system_record: the reverse mode does not cover point connections or collision spheres");
         return PK_ERR_ARG;
     }
     if (s->fem.p)
@@ -2023,6 +2030,32 @@ int pk_system_set_connections(pk_system* s, int count, const int* nodes, const d
     if ((rc = s->connRows.upload(rows, st)) != PK_OK) return rc;
     if ((rc = s->connRowStart.upload(rstart, st)) != PK_OK) return rc;
     if ((rc = s->connEntries.upload(entries, st)) != PK_OK) return rc;
+    {
+        // every node's (connection, corner) entries with w_a != 0, fixed nodes
+        // included, in connection order
+        std::vector<int> nstart(static_cast<size_t>(t.N) + 1, 0);
+        for (int ci = 0; ci < nc; ++ci)
+        {
+            const int n4[4] = {hn[ci].x, hn[ci].y, hn[ci].z, hn[ci].w};
+            for (int a = 0; a < 4; ++a)
+                if (hd[9 * static_cast<size_t>(ci) + a] != 0.0)
+                    ++nstart[n4[a] + 1];
+        }
+        for (int n = 0; n < t.N; ++n)
+            nstart[n + 1] += nstart[n];
+        std::vector<int2> nlistv(static_cast<size_t>(std::max(1, nstart[t.N])));
+        std::vector<int> fill(nstart.begin(), nstart.end() - 1);
+        for (int ci = 0; ci < nc; ++ci)
+        {
+            const int n4[4] = {hn[ci].x, hn[ci].y, hn[ci].z, hn[ci].w};
+            for (int a = 0; a < 4; ++a)
+                if (hd[9 * static_cast<size_t>(ci) + a] != 0.0)
+                    nlistv[fill[n4[a]]++] = make_int2(ci, a);
+        }
+        if ((rc = s->connNodeStart.upload(nstart, st)) != PK_OK) return rc;
+        if ((rc = s->connNodeList.upload(nlistv, st)) != PK_OK) return rc;
+        if ((rc = s->connAdj.alloc(6 * static_cast<size_t>(nc))) != PK_OK) return rc;
+    }
     if ((rc = s->slotIdx.upload(sidx, st)) != PK_OK) return rc;
     if ((rc = s->slotVal.upload(sval, st)) != PK_OK) return rc;
     launch_slot_scatter(st, static_cast<int>(sidx.size()), s->slotIdx.p, s->slotVal.p, s->cslot.p);
@@ -2127,6 +2160,23 @@ static int build_contact_lists(pk_system* s)
     if ((rc = s->csList.upload(slist, st)) != PK_OK) return rc;
     if ((rc = s->crStart.upload(rstart, st)) != PK_OK) return rc;
     if ((rc = s->crList.upload(rlist, st)) != PK_OK) return rc;
+    {
+        // every node's candidate tets (once per corner) in candidate order
+        std::vector<int> nstart(static_cast<size_t>(t.N) + 1, 0);
+        for (size_t i = 0; i < cand.size(); ++i)
+            for (int a = 0; a < 4; ++a)
+                ++nstart[t.tets[4 * static_cast<size_t>(cand[i]) + a] + 1];
+        for (int n = 0; n < t.N; ++n)
+            nstart[n + 1] += nstart[n];
+        std::vector<int> nlistv(static_cast<size_t>(std::max(1, nstart[t.N])));
+        std::vector<int> fill(nstart.begin(), nstart.end() - 1);
+        for (size_t i = 0; i < cand.size(); ++i)
+            for (int a = 0; a < 4; ++a)
+                nlistv[fill[t.tets[4 * static_cast<size_t>(cand[i]) + a]]++] = static_cast<int>(i);
+        if ((rc = s->candNodeStart.upload(nstart, st)) != PK_OK) return rc;
+        if ((rc = s->candNodeList.upload(nlistv, st)) != PK_OK) return rc;
+        if ((rc = s->candAdj.alloc(6 * std::max<size_t>(1, cand.size()))) != PK_OK) return rc;
+    }
     if ((rc = s->ck.alloc(std::max<size_t>(1, cand.size()))) != PK_OK) return rc;
     if ((rc = s->cg.alloc(3 * std::max<size_t>(1, cand.size()))) != PK_OK) return rc;
     if ((rc = s->sslot.alloc(static_cast<size_t>(std::max<long long>(1, nnzu)))) != PK_OK) return rc;

@@ -140,9 +140,6 @@ def test_clearing_connections_restores_the_plain_step():
     a.step(w.dt, 2, 1)
     b.step(w.dt, 2, 1)
     assert np.array_equal(a.state()[0], b.state()[0]) and np.array_equal(a.state()[1], b.state()[1])
-    a.set_connections(*[np.array([x[i] for x in c]) for i in range(5)])
-    with pytest.raises(RuntimeError, match="point connections"):
-        a.record(w.dt, 2)
 
 
 def test_physik_point_connection_pulls_the_point_toward_the_target():

@@ -112,7 +112,7 @@ def test_upper_and_fp32_storage_with_spheres():
         assert rel(xg, xo) <= 1e-4 and rel(vg, vo) <= 1e-4, (f, rel(xg, xo), rel(vg, vo))
 
 
-def test_removing_spheres_restores_the_plain_step_and_blocks_the_reverse_mode():
+def test_removing_spheres_restores_the_plain_step():
     w = beam()
     print("case remove spheres", flush=True)
     a = W.make_system(w)
@@ -122,9 +122,6 @@ def test_removing_spheres_restores_the_plain_step_and_blocks_the_reverse_mode():
     a.step(w.dt, 2, 1)
     b.step(w.dt, 2, 1)
     assert np.array_equal(a.state()[0], b.state()[0]) and np.array_equal(a.state()[1], b.state()[1])
-    a.set_spheres(*scene(0))
-    with pytest.raises(RuntimeError, match="collision spheres"):
-        a.record(w.dt, 2)
     with pytest.raises(RuntimeError, match="finite"):
         a.set_spheres([[0.0, 0.0, 0.0]], [-1.0])
 

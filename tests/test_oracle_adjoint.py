@@ -196,3 +196,98 @@ def test_one_step_velocity_seed_is_the_solve_transpose():
     free = p.fixed == 0
     assert np.allclose(gv[free], vbar[free], rtol=1e-12, atol=1e-14)
     assert not gx.any() and not gE.any()
+
+
+class ContactProblem(Problem):
+    """Problem with contact forces in every substep: two point connections
+    (the far face's barycentre to a fixed target, and one node to a second
+    target) and/or a collision sphere under the bottom far corner."""
+
+    def __init__(self, connections=True, sphere=True, steps=4):
+        super().__init__(steps=steps)
+        far = self.end
+        self.conns = []
+        if connections:
+            self.conns = [(far[:4], np.full(4, 0.25), self.x0[far[:4]].mean(0) + np.array([0.03, -0.02, 0.04]), 80.0,
+                           0.7),
+                          (np.array([far[-1]] * 4), np.array([1.0, 0.0, 0.0, 0.0]),
+                           self.x0[far[-1]] + np.array([0.0, 0.05, 0.0]), 40.0, 0.2)]
+        self.sphere = None
+        if sphere:
+            lo, hi = self.x0.min(0), self.x0.max(0)
+            self.sphere = (np.array([hi[0], 0.5 * (lo[1] + hi[1]), lo[2] - 0.02]), 0.13, 300.0, 1.5)
+
+    def world(self, **kw):
+        o = super().world(**kw)
+        if self.sphere is not None:
+            c, r, k, d = self.sphere
+            o.set_spheres(c[None], [r], [k], [d])
+        return o
+
+    def add_conns(self, o):
+        for n, wts, t, k, c in self.conns:
+            o.add_point_connection(np.asarray(n), wts, t, k, c)
+
+    def loss(self, **kw):
+        o = self.world(**kw)
+        for _ in range(self.steps):
+            self.add_conns(o)  # a step clears its frame's connections
+            assert o.step(self.dt) == 0
+        return self.loss_terms(*o.state())
+
+    def adjoint(self):
+        probe = self.world()
+        for _ in range(self.steps):
+            self.add_conns(probe)
+            probe.step(self.dt)
+            if self.sphere is not None:
+                assert probe.last_contact_count() > 0
+        xs, vs = probe.state()
+        o = self.world()
+        self.add_conns(o)  # the recorded substeps keep the connections
+        xbar = np.zeros((self.n, 3))
+        xbar[self.end] = 2 * (xs[self.end] - self.target)
+        vbar = 0.2 * vs
+        gE, gx, gv, its = o.adjoint(self.dt, self.steps, xbar, vbar, tol=1e-12, max_iter=1000)
+        assert np.array_equal(o.state()[0], xs)
+        return gE, gx, gv, its
+
+
+@pytest.fixture(scope="module", params=["connections", "sphere", "both"])
+def contact_problem(request):
+    p = ContactProblem(connections=request.param != "sphere", sphere=request.param != "connections")
+    return p, p.adjoint()
+
+
+def test_contact_gradients_match_finite_differences(contact_problem):
+    """dL/dlogE, dL/dv0 and dL/dx0 with point connections and sphere
+    contacts in the forward map (the sphere's target moves with x)."""
+    p, (gE, gx, gv, its) = contact_problem
+    assert (its > 0).all()
+    rng = np.random.default_rng(13)
+    free = np.flatnonzero(p.fixed == 0)
+    picks = [("E", e, 0) for e in rng.choice(p.t, 4, replace=False)]
+    # the nodes the contacts pull on, and a few others
+    near = list(p.end[:2]) + list(rng.choice(free, 2, replace=False))
+    picks += [(kind, i, k) for kind in ("v", "x") for i in near for k in range(3)]
+    for kind, i, k in picks:
+        def f(d, kind=kind, i=i, k=k):
+            if kind == "E":
+                le = p.logE.copy()
+                le[i] += d
+                return p.loss(logE=le)
+            a = (p.v0 if kind == "v" else p.x0).copy()
+            a[i, k] += d
+            return p.loss(**{("v0" if kind == "v" else "x0"): a})
+        fd = central(f)
+        g = {"E": gE, "v": gv, "x": gx}[kind]
+        got = g[i] if kind == "E" else g[i, k]
+        assert abs(got - fd) <= 1e-8 * np.abs(g).max() + 1e-6 * abs(fd), (kind, i, k, got, fd)
+
+
+def test_contact_terms_change_the_gradient():
+    """The contact terms are not negligible here: dropping the connections
+    and the sphere changes dL/dx0 far beyond the tolerance above."""
+    with_c = ContactProblem().adjoint()
+    without = Problem().adjoint()
+    assert np.abs(with_c[1] - without[1]).max() > 1e-3 * np.abs(without[1]).max()
