@@ -209,7 +209,7 @@ def prepare(name, prec_name, args):
         s.set_state(w.x0, w.v0)
         shape = s.get_tune() if args.no_autotune else s.autotune(w.dt)
     s.set_state(w.x0, w.v0)
-    s.set_graphs(args.cg_launch == "graphs")
+    s.set_graphs({"kernels": 0, "graphs": 1, "steps": 2}[args.cg_launch])
     return w, s, dict(zip(("lanes_per_row", "spmv_rows", "vector_rows", "unroll"), shape))
 
 
@@ -237,13 +237,16 @@ def frame_export(s, reps=5):
             "float_export_GBps": 24 * s.n / (wait * 1e6)}
 
 
-def run_plain(name, prec_name, steps, warmup, args, export=True):
-    """K plain implicit steps from the workload's initial state."""
+def run_plain(name, prec_name, steps, warmup, args, export=True, profile=True):
+    """K plain implicit steps from the workload's initial state (profiled
+    unless profile is False or the substeps are captured whole)."""
     w, s, shape = prepare(name, prec_name, args)
     for _ in range(warmup):
         s.step(w.dt)
     s.set_state(w.x0, w.v0)
-    s.profile(True)
+    # a profiled step launches its kernels: substeps captured whole run unprofiled
+    profiled = profile and args.cg_launch != "steps"
+    s.profile(profiled)
     t0 = time.perf_counter()
     s.step(w.dt, 1, steps)  # returns after the stream has drained
     el = time.perf_counter() - t0
@@ -260,9 +263,9 @@ def run_plain(name, prec_name, steps, warmup, args, export=True):
         "cg_iterations_last": int(its.max()),
         "cg_launch_shape": shape,
         "cg_launch": args.cg_launch,
-        "roofline": roofline(s, name, prec_name),
-        "assembly_roofline": assembly_roofline(s, w, name, prec_name),
-        "kernels_ms": kernels_ms(s),
+        "roofline": roofline(s, name, prec_name) if profiled else None,
+        "assembly_roofline": assembly_roofline(s, w, name, prec_name) if profiled else None,
+        "kernels_ms": kernels_ms(s) if profiled else None,
     }
     s.profile(False)
     if export:
@@ -565,8 +568,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default CG launch shape")
     ap.add_argument("--shape", default=None, help="CG launch shape lanes,spmv_rows,vector_rows,unroll (no autotune)")
-    ap.add_argument("--cg-launch", default="kernels", choices=["kernels", "graphs"],
-                    help="issue each CG launch from the host, or replay captured 4-iteration HIP graphs")
+    ap.add_argument("--cg-launch", default="kernels", choices=["kernels", "graphs", "steps"],
+                    help="issue each CG launch from the host, replay captured 4-iteration HIP graphs, or replay "
+                         "each implicit substep as one captured HIP graph (forward steps; unprofiled)")
     ap.add_argument("--no-oracle-error", action="store_true", help="skip the per-body FP64-oracle error")
     ap.add_argument("--oracle-budget", type=float, default=10.0, help="seconds of oracle work for the error sample")
     ap.add_argument("--no-batch-forward", action="store_true", help="skip the batch256 forward sub-record")
@@ -626,18 +630,24 @@ def main():
         line["vs_plain_step"] = head["value"] / plain["value"]
     if not args.no_batch_forward and name != "batch256":
         # the batched forward workload (256 bodies, FP64), launched kernel by
-        # kernel and replayed from captured HIP graphs
+        # kernel and with every substep replayed as one captured HIP graph
         progress("batched forward: batch256, kernels")
         bf, wbf = run_plain("batch256", DEFAULT_PRECISION["batch256"], DEFAULT_STEPS["batch256"], args.warmup, args,
                             export=False)
         rec = {k: v for k, v in bf.items() if k not in internal}
         if not args.no_oracle_error:
             rec["oracle_error"] = oracle_error(wbf, bf, args.oracle_budget / 2)
-        gargs = argparse.Namespace(**{**vars(args), "cg_launch": "graphs"})
-        progress("batched forward: batch256, graphs")
+        gargs = argparse.Namespace(**{**vars(args), "cg_launch": "steps"})
+        progress("batched forward: batch256, substeps captured as HIP graphs")
         bg, _ = run_plain("batch256", DEFAULT_PRECISION["batch256"], DEFAULT_STEPS["batch256"], args.warmup, gargs,
                           export=False)
-        rec["graphs"] = {k: bg[k] for k in ("value", "ms_per_step", "cg_launch", "roofline", "kernels_ms")}
+        rec["graphs"] = {k: bg[k] for k in ("value", "ms_per_step", "cg_launch", "cg_iterations_last")}
+        # the same steps launched kernel by kernel without profiling, beside it
+        kargs = argparse.Namespace(**{**vars(args), "cg_launch": "kernels"})
+        progress("batched forward: batch256, kernels unprofiled")
+        ku, _ = run_plain("batch256", DEFAULT_PRECISION["batch256"], DEFAULT_STEPS["batch256"], args.warmup, kargs,
+                          export=False, profile=False)
+        rec["graphs"]["kernels_unprofiled_ms_per_step"] = ku["ms_per_step"]
         line["batch_forward"] = rec
     if not args.no_cpu_baseline:
         progress("CPU baselines")

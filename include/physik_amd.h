@@ -256,12 +256,19 @@ int pk_system_set_spheres(pk_system* sys, int count, const double* centers, cons
  * either way. */
 int pk_system_set_matrix_store(pk_system* sys, long long max_bytes);
 
-/* CG launch mode (no reference counterpart). enable != 0: every solve replays
- * its CG four iterations at a time from HIP graphs captured on first use
- * (direction, SpMV and update launches, plus a batch's tile compaction),
- * instead of issuing each launch from the host; results are identical either
- * way. A capture is redone after pk_system_tune or a profiling change. */
-int pk_system_set_graphs(pk_system* sys, int enable);
+/* Launch mode (no reference counterpart); results are identical in every
+ * mode. 0: every kernel launched from the host. 1: every solve replays its CG
+ * four iterations at a time from HIP graphs captured on first use (direction,
+ * SpMV and update launches, plus a batch's tile compaction). 2: each implicit
+ * forward substep of pk_system_step is one captured graph -- reset,
+ * element, contacts, assembly, right-hand side, enough CG chunks for the
+ * previous solve's iteration count plus one, and the integration of every
+ * body whose solve ended within them; a body that needs more iterations is
+ * finished by launches from the host and integrated after (record and
+ * backward replay CG chunks as in mode 1, and a profiled step launches its
+ * kernels). A capture is redone when a setting, the launch shape, the step
+ * or that chunk count changes. PK_ERR_ARG for another mode. */
+int pk_system_set_graphs(pk_system* sys, int mode);
 
 /* Kernel timing on the system's stream: enable, reset, read. The CG kernels
  * (SpMV, update) are timed from the wall clock read inside the kernel (first

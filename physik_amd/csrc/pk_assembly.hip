@@ -1,21 +1,26 @@
 // Per-substep assembly kernels for gfx950: element, slot gather, right-hand
-// side with the CG start, and the integrators.
+// side with the CG start, the integrators, contact forces and frame export.
 //
-//   k_element   one lane per tet: F = Ds Dm^-1, Newton polar R, corotated
-//               forces, the 10 blocks R K_ab R^T and K v per corner
-//               (FEMModel.cpp:614-679, :377-424, :336-375, :115-163), each
-//               written where its consumer reads it: blocks into the slot
-//               gather lists (precomputed element->slot scatter map, oriented
-//               for the slot), forces and K v into the row lists
-//   k_assemble  one lane per upper BSR slot: contiguous sum of the slot's
-//               blocks in tet order (no atomics), A = M + dt^2 K written to
-//               both triangles, 3x3 block-Jacobi inverse on diagonal slots
-//               (SolverData.cpp:308-465, :260-306)
-//   k_rhs       b = dt (f + m g) - dt^2 K v per dynamic row, then the CG start
-//               x = 0, r = b, z = d = M^-1 r with the (r.z, r.r) partials
-//               (ConjugateGradientSolver.cpp:151-164)
-//   k_integrate all-or-nothing v += dv, x += dt v per group (World.cpp:644-695)
-//   k_explicit  explicit Euler (World.cpp:697-725)
+//   k_element        one lane per tet: F = Ds Dm^-1, Newton polar R (or R = I
+//                    for the linear model), the tet's basis record (R g_a,
+//                    R R^T) and per dynamic corner one entry dt f_a -
+//                    dt^2 (K v)_a (or f_a explicit), written in row-list order
+//                    (FEMModel.cpp:614-679, :377-424, :559-612, :115-163)
+//   k_stiff_records  one lane per tet: V lambda, V mu, V mu g_a.g_b
+//   k_assemble       16 lanes per dynamic row: the diagonal block over the
+//                    row's corner list, then its off-diagonal upper slots four
+//                    at a time, each block rebuilt from the basis and
+//                    stiffness records (lane-strided partial sums combined by
+//                    shuffles: a fixed order, no atomics); A = M + dt^2 K
+//                    into full or sliced upper storage and the block-Jacobi
+//                    inverse (SolverData.cpp:308-465, :260-306)
+//   k_rhs            b = sum of the row's corner entries + dt m g (+ contacts)
+//                    per dynamic row, then the CG start x = 0, r = b,
+//                    z = d = M^-1 r with the (r.z, r.r) partials
+//                    (ConjugateGradientSolver.cpp:151-164)
+//   k_integrate      all-or-nothing v += dv, x += dt v per group, for the
+//                    groups whose solve has ended (World.cpp:644-695)
+//   k_explicit       explicit Euler (World.cpp:697-725)
 #include "pk_device.h"
 #include "pk_fem.h"
 #include "pk_topology.h"
@@ -44,6 +49,7 @@ __global__ void k_reset(GroupState* gs, int G, int* active, int* kbase, double t
         s.converged = 0;
         s.failed = 0;
         s.nonfinite = 0;
+        s.integrated = 0;
         if (s.hasRows)
             atomicAdd(&cnt, 1);
     }
@@ -708,7 +714,9 @@ __global__ __launch_bounds__(NT) void k_integrate(TileView tv, const int* __rest
         return;
     const RowTile tl = tv.tiles[t];
     GroupState& s = gs[tl.group];
-    if (s.failed || (commit && s.nonfinite))
+    // a body still iterating (a substep graph integrates the bodies whose
+    // solve has ended, the host the rest) or already integrated is left alone
+    if (s.failed || (commit && s.nonfinite) || s.stopAt < 0 || s.integrated)
         return;
     bool ok = true;
     for (int row = tl.rowBegin + threadIdx.x; row < tl.rowEnd; row += NT)
@@ -815,6 +823,21 @@ void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt)
         hipLaunchKernelGGL((k_rhs<M>), dim3(grid_of(v.vec)), dim3(NT), 0, v.stream, v.vec, v.nstart, v.corner, v.mrow,
                            gx, gy, gz, dt, static_cast<const M*>(v.minv), v.nd, v.xs, v.r, static_cast<M*>(v.z),
                            static_cast<M*>(v.d[0]), v.pb[1], v.rowAdd);
+}
+
+// Marks the bodies whose solve has ended as integrated (after both
+// k_integrate passes), so a later integration skips them.
+__global__ void k_mark_integrated(GroupState* gs, int G)
+{
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x)
+        if (gs[g].stopAt >= 0)
+            gs[g].integrated = 1;
+}
+
+void launch_mark_integrated(const DeviceView& v)
+{
+    if (v.G > 0)
+        hipLaunchKernelGGL(k_mark_integrated, dim3((v.G + NT - 1) / NT), dim3(NT), 0, v.stream, v.gs, v.G);
 }
 
 void launch_integrate(const DeviceView& v, double dt, int commit)

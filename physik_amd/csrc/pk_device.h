@@ -37,7 +37,7 @@ struct GroupState
     int nonfinite;    // integration produced a non-finite value
     int hasRows;
     int jacobi;
-    int pad;
+    int integrated;   // the substep's state update of this body is done
 };
 
 // One workgroup of a row-tiled kernel: rows [rowBegin, rowEnd) of group `group`.
@@ -442,6 +442,7 @@ void launch_element(const DeviceView& v, double dt, int implicit);
 template <class M> void launch_assemble(const DeviceView& v, double dt);
 template <class M> void launch_rhs(const DeviceView& v, double gx, double gy, double gz, double dt);
 void launch_integrate(const DeviceView& v, double dt, int commit);
+void launch_mark_integrated(const DeviceView& v);
 void launch_explicit(const DeviceView& v, double gx, double gy, double gz, double dt);
 // Point connections: per-connection force and point velocity at the current
 // state, then each touched row's right-hand side terms (implicit) or force

@@ -231,7 +231,9 @@ struct pk_system
     // capture bakes in buffer addresses and tile shapes: `epoch` changes when
     // they do, and the stamp buffers in use are part of the key.
     bool useGraphs = false;
+    int graphMode = 0;     // pk_system_set_graphs: 0 off, 1 CG chunks, 2 whole forward substeps
     int epoch = 0;
+    long long cfgVer = 0;  // changes with every setting a captured substep bakes in
     struct ChunkGraphs
     {
         hipGraphExec_t first = nullptr, next = nullptr;
@@ -239,16 +241,29 @@ struct pk_system
         const void* stamps[3] = {nullptr, nullptr, nullptr};
         const void* matrix[2] = {nullptr, nullptr};
     } chunks;
+    // one implicit forward substep captured whole: reset, assembly,
+    // right-hand side, `chunks` CG chunks and the integration of the bodies
+    // whose solve ended within them
+    struct StepGraph
+    {
+        hipGraphExec_t exec = nullptr;
+        int epoch = -1;
+        long long cfgVer = -1;
+        double dt = 0.0;
+        int chunks = 0;
+        const void* matrix[2] = {nullptr, nullptr};
+    } stepGraph;
 
     void drop_graphs()
     {
-        for (hipGraphExec_t* g : {&chunks.first, &chunks.next})
+        for (hipGraphExec_t* g : {&chunks.first, &chunks.next, &stepGraph.exec})
             if (*g)
             {
                 (void)hipGraphExecDestroy(*g);
                 *g = nullptr;
             }
         chunks.epoch = -1;
+        stepGraph.epoch = -1;
     }
 
     ~pk_system()
@@ -760,9 +775,144 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
 
 // One substep; `failedOut` (reverse-mode replays) receives the assembly's
 // per-body rejection flags.
+// Captures one implicit forward substep: the assembly's launches, `chunks`
+// CG chunks (as capture_chunk records them, one after another, the device
+// iteration base advancing) and the integration of every body whose solve
+// ended within them.
+template <class M>
+static int capture_substep(pk_system* s, const DeviceView& dv, double dt, int chunks, hipGraphExec_t& exec)
+{
+    PK_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    launch_reset(dv, s->tol, s->maxIt);
+    launch_element(dv, dt, 1);
+    if (s->nconn > 0 || s->nsph > 0)
+    {
+        launch_connections(dv, dt, 1);
+        launch_spheres(dv, dt, 1);
+    }
+    launch_assemble<M>(dv, dt);
+    launch_rhs<M>(dv, s->gravity[0], s->gravity[1], s->gravity[2], dt);
+    DeviceView cv = dv;
+    const bool compact = s->spmvPerm.p != nullptr;
+    for (int c = 0; c < chunks; ++c)
+    {
+        if (compact && c > 0)
+        {
+            launch_compact(dv, s->vecPerm.p, s->spmvPerm.p);
+            cv.vec.perm = s->vecPerm.p;
+            cv.spmv.perm = s->spmvPerm.p;
+        }
+        for (int o = 0; o < kChunk; ++o)
+        {
+            launch_dir<M>(cv, dv.kbase, o, nullptr);
+            launch_spmv<M>(cv, dv.kbase, o, s->lanesPerRow, s->unroll, nullptr);
+            launch_update<M>(cv, dv.kbase, o, nullptr);
+        }
+        launch_advance(dv, dv.kbase, kChunk);
+    }
+    launch_integrate(dv, dt, 0);
+    launch_integrate(dv, dt, 1);
+    launch_mark_integrated(dv);
+    const hipError_t launchErr = hipGetLastError();
+    hipGraph_t g = nullptr;
+    const hipError_t endErr = hipStreamEndCapture(s->stream, &g);
+    if (launchErr != hipSuccess || endErr != hipSuccess)
+    {
+        if (g)
+            (void)hipGraphDestroy(g);
+        set_error(std::string("substep capture failed: ") +
+                  hipGetErrorString(launchErr != hipSuccess ? launchErr : endErr));
+        return PK_ERR_HIP;
+    }
+    const hipError_t instErr = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    PK_HIP(instErr);
+    return PK_OK;
+}
+
+// One implicit forward substep from a captured graph (pk_system_set_graphs
+// mode 2). The graph holds enough CG chunks for the previous solve's
+// iterations and one more; it is captured again when the settings, the
+// buffers or that count change. A body still iterating when the graph ends
+// is finished by launches from the host (the same kernels at the same
+// iterations), then integrated: the results are those of run_cg's launches.
+template <class M>
+static int run_substep_graph(pk_system* s, double dt)
+{
+    const DeviceView dv = s->view();
+    const int launches = std::max(0, s->maxIt) + 1;
+    const int most = (launches + kChunk - 1) / kChunk;
+    const int want = std::min(most, (std::min(std::max(s->lastIters, 0) + 1, launches) + kChunk - 1) / kChunk + 1);
+    auto& g = s->stepGraph;
+    const bool same = g.exec && g.epoch == s->epoch && g.cfgVer == s->cfgVer && g.dt == dt &&
+                      g.matrix[0] == dv.vals && g.matrix[1] == dv.minv;
+    if (!same || g.chunks < want || g.chunks > want + 4)
+    {
+        if (g.exec)
+        {
+            (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+        }
+        int rc;
+        if ((rc = capture_substep<M>(s, dv, dt, want, g.exec)) != PK_OK)
+            return rc;
+        g.epoch = s->epoch;
+        g.cfgVer = s->cfgVer;
+        g.dt = dt;
+        g.chunks = want;
+        g.matrix[0] = dv.vals;
+        g.matrix[1] = dv.minv;
+    }
+    PK_HIP(hipGraphLaunch(g.exec, s->stream));
+    // bodies still iterating: the remaining launches, checked every chunk
+    const bool compact = s->spmvPerm.p != nullptr;
+    DeviceView cv = dv;
+    if (compact && g.chunks > 1)
+    {
+        cv.vec.perm = s->vecPerm.p;
+        cv.spmv.perm = s->spmvPerm.p;
+    }
+    int k = g.chunks * kChunk;
+    bool tail = false;
+    while (true)
+    {
+        PK_HIP(hipMemcpyAsync(s->pinned, s->active.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        PK_HIP(hipEventRecord(s->pollEv[0], s->stream));
+        PK_HIP(hipEventSynchronize(s->pollEv[0]));
+        if (s->pinned[0] == 0 || k > s->maxIt)
+            break;
+        tail = true;
+        for (const int end = std::min(s->maxIt + 1, k + kChunk); k < end; ++k)
+        {
+            if (compact && k % kCompactEvery == 0)
+            {
+                launch_compact(dv, s->vecPerm.p, s->spmvPerm.p);
+                cv.vec.perm = s->vecPerm.p;
+                cv.spmv.perm = s->spmvPerm.p;
+            }
+            launch_dir<M>(cv, nullptr, k, nullptr);
+            if (k < s->maxIt)
+            {
+                launch_spmv<M>(cv, nullptr, k, s->lanesPerRow, s->unroll, nullptr);
+                launch_update<M>(cv, nullptr, k, nullptr);
+            }
+        }
+    }
+    if (tail)
+    {
+        launch_integrate(dv, dt, 0);
+        launch_integrate(dv, dt, 1);
+        launch_mark_integrated(dv);
+    }
+    PK_HIP(hipGetLastError());
+    return PK_OK;
+}
+
 template <class M>
 static int run_substep(pk_system* s, double dt, int* failedOut = nullptr)
 {
+    if (s->graphMode == 2 && s->implicit && !failedOut && !s->prof.on && s->topo.nd > 0)
+        return run_substep_graph<M>(s, dt);
     int rc = run_assembly<M>(s, dt, s->tol, s->maxIt, s->implicit ? kAsmImplicit : kAsmExplicit);
     if (rc != PK_OK || s->topo.nd == 0)
         return rc;
@@ -776,6 +926,7 @@ static int run_substep(pk_system* s, double dt, int* failedOut = nullptr)
         Bracket br(s, PK_K_INTEGRATE);
         launch_integrate(dv, dt, 0);
         launch_integrate(dv, dt, 1);
+        launch_mark_integrated(dv);
     }
     else
     {
@@ -1273,6 +1424,7 @@ int pk_system_tune(pk_system* s, int lanes_per_row, int spmv_rows, int vector_ro
         return PK_ERR_ARG;
     }
     s->tapeValid = false;  // a recorded run no longer replays bit-identically
+    ++s->cfgVer;  // a captured substep bakes the old setting in
     s->lanesPerRow = lanes_per_row;
     s->spmvRows = spmv_rows;
     s->vecRows = vector_rows;
@@ -1288,6 +1440,7 @@ int pk_system_set_spmv_staging(pk_system* s, int mode)
         return PK_ERR_ARG;
     }
     s->tapeValid = false;  // a recorded run no longer replays the same launches
+    ++s->cfgVer;  // a captured substep bakes the old setting in
     s->stagingMode = mode;
     ++s->epoch;
     const int rc = setup_windows(s);
@@ -1465,6 +1618,7 @@ int pk_system_set_gravity(pk_system* s, double gx, double gy, double gz)
     if (!s)
         return PK_ERR_ARG;
     s->tapeValid = false;  // a recorded run no longer replays the same map
+    ++s->cfgVer;  // a captured substep bakes the old setting in
     s->gravity[0] = gx;
     s->gravity[1] = gy;
     s->gravity[2] = gz;
@@ -1476,6 +1630,7 @@ int pk_system_set_solver(pk_system* s, double tolerance, int max_iterations, int
     if (!s)
         return PK_ERR_ARG;
     s->tapeValid = false;  // a recorded run no longer replays the same map
+    ++s->cfgVer;  // a captured substep bakes the old setting in
     s->tol = tolerance;
     s->maxIt = max_iterations;
     s->jacobi = use_jacobi ? 1 : 0;
@@ -1487,6 +1642,7 @@ int pk_system_set_mode(pk_system* s, int implicit)
     if (!s)
         return PK_ERR_ARG;
     s->tapeValid = false;  // a recorded run no longer replays the same map
+    ++s->cfgVer;  // a captured substep bakes the old setting in
     s->implicit = implicit ? 1 : 0;
     return PK_OK;
 }
@@ -1507,6 +1663,7 @@ int pk_system_set_fem_models(pk_system* s, const unsigned char* models)
         allCorot = allCorot && models[t] == FEM_COROTATIONAL;
     }
     s->tapeValid = false;  // a recorded run no longer replays the same map
+    ++s->cfgVer;  // a captured substep bakes the old setting in
     ++s->epoch;            // captured graphs hold the element launch's arguments
     if (allCorot)
     {
@@ -1526,6 +1683,7 @@ int pk_system_set_young(pk_system* s, const double* young)
     if (!s || !young)
         return PK_ERR_ARG;
     s->tapeValid = false;  // a recorded run no longer replays the same map
+    ++s->cfgVer;  // a captured substep bakes the old setting in
     Topology& t = s->topo;
     if (t.T == 0)
         return PK_OK;
@@ -2065,6 +2223,7 @@ int pk_system_set_connections(pk_system* s, int count, const int* nodes, const d
     s->nconnRows = static_cast<int>(rows.size());
     s->nslots = static_cast<int>(sidx.size());
     s->tapeValid = false;
+    ++s->cfgVer;  // a captured substep bakes the old setting in
     return PK_OK;
 }
 
@@ -2222,6 +2381,7 @@ int pk_system_set_spheres(pk_system* s, int count, const double* centers, const 
     PK_HIP(hipStreamSynchronize(s->stream));
     s->nsph = count;
     s->tapeValid = false;
+    ++s->cfgVer;  // a captured substep bakes the old setting in
     return PK_OK;
 }
 
@@ -2330,6 +2490,7 @@ int pk_system_set_tets_active(pk_system* s, int count, const int* tets, const un
     PK_HIP(hipGetLastError());
     PK_HIP(hipStreamSynchronize(st));
     s->tapeValid = false;
+    ++s->cfgVer;  // a captured substep bakes the old setting in
     return PK_OK;
 }
 
@@ -2348,13 +2509,16 @@ int pk_system_set_matrix_store(pk_system* s, long long max_bytes)
     return PK_OK;
 }
 
-int pk_system_set_graphs(pk_system* s, int enable)
+int pk_system_set_graphs(pk_system* s, int mode)
 {
-    if (!s)
+    if (!s || mode < 0 || mode > 2)
+    {
+        set_error("pk_system_set_graphs: mode 0 (launches), 1 (CG chunks) or 2 (whole substeps)");
         return PK_ERR_ARG;
-    s->useGraphs = enable != 0;
-    if (!s->useGraphs)
-        s->drop_graphs();
+    }
+    s->graphMode = mode;
+    s->useGraphs = mode != 0;
+    s->drop_graphs();
     return PK_OK;
 }
 

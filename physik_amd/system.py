@@ -256,10 +256,11 @@ class System:
         (default 8 GiB); 0 re-assembles every reverse substep instead."""
         N.check(self.lib.pk_system_set_matrix_store(self.h, int(max_bytes)), "pk_system_set_matrix_store")
 
-    def set_graphs(self, enable=True):
-        """Replay each solve's CG from captured HIP graphs, four iterations
-        per replay (identical results to launching each kernel)."""
-        N.check(self.lib.pk_system_set_graphs(self.h, int(enable)), "pk_system_set_graphs")
+    def set_graphs(self, mode=1):
+        """HIP graphs (identical results to launching each kernel): 0 off;
+        1 (or True) each solve's CG replayed four iterations per graph;
+        2 every implicit forward substep replayed as one captured graph."""
+        N.check(self.lib.pk_system_set_graphs(self.h, int(mode)), "pk_system_set_graphs")
 
     def spmv_work(self):
         """(full BSR blocks, upper-triangle blocks among them, dynamic rows,

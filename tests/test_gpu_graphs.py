@@ -115,3 +115,74 @@ def test_graph_adjoint_equals_launched_adjoint():
         s.close()
     for u, v in zip(*out):
         assert np.array_equal(u, v)
+
+
+# ---- whole substeps captured (pk_system_set_graphs mode 2) ----
+
+def substep_pair(w, precision=N.PK_FP64, shape=None):
+    a = W.make_system(w, precision)
+    b = W.make_system(w, precision)
+    if shape:
+        a.tune(*shape)
+        b.tune(*shape)
+    b.set_graphs(2)
+    return a, b
+
+
+@pytest.mark.parametrize("precision", [N.PK_FP64, N.PK_FP32_UPPER])
+def test_substep_graphs_equal_launched_steps(precision):
+    # the first step finds no previous iteration count: its graph holds two
+    # chunks and the host finishes the solve; later graphs hold the solve
+    w = twisted()
+    print(f"case substep graphs, single body, precision {precision}", flush=True)
+    a, b = substep_pair(w, precision)
+    same_step(a, b, w, w.steps)
+    # a maximum that is not a multiple of the chunk: the graph runs past it
+    a.set_solver(1e-12, 7)
+    b.set_solver(1e-12, 7)
+    same_step(a, b, w, 3)
+    assert (a.results()[0] == 7).all()
+    # settings the graph bakes in: captured again
+    a.set_solver(1e-7, 400)
+    b.set_solver(1e-7, 400)
+    a.set_gravity((0.0, -3.0, -9.81))
+    b.set_gravity((0.0, -3.0, -9.81))
+    same_step(a, b, w, 2)
+
+
+def test_substep_graphs_with_a_batch_and_contacts():
+    w = W.batch256(bodies=12, lo=3, hi=9, steps=6)
+    sph = (np.array([[0.2, 0.2, 0.2]]), np.array([0.25]), np.array([3000.0]), np.array([5.0]))
+    print("case substep graphs, batch of 12 bodies with a sphere", flush=True)
+    a, b = substep_pair(w, N.PK_FP64_UPPER)
+    a.set_spheres(*sph)
+    b.set_spheres(*sph)
+    for n in range(w.steps):
+        same_step(a, b, w, 1)
+    it = a.results()[0]
+    assert len(set(it.tolist())) > 1, "bodies should stop at different iterations"
+    # a shorter time step: fewer iterations, the graph captured again
+    w.dt = w.dt / 4
+    same_step(a, b, w, 2)
+
+
+def test_substep_graphs_leave_record_backward_and_profiling_unchanged():
+    w = W.batch128(bodies=3, lo=3, hi=5, h=0.1, steps=5)
+    w.young = w.young / 10.0
+    print("case substep graphs: record, backward, profiled step", flush=True)
+    out = []
+    for mode in (0, 2):
+        s = W.make_system(w)
+        s.set_graphs(mode)
+        s.step(w.dt)          # a graph-captured step in mode 2
+        s.profile(True)
+        s.step(w.dt)          # profiled: launched kernels
+        s.profile(False)
+        s.record(w.dt, w.steps, 2)
+        x, _ = s.state()
+        _, xbar = W.loss_and_seed(w, x)
+        out.append((x,) + tuple(s.backward(xbar, None, tol=w.adj_tol, max_iter=w.max_iter)))
+        assert s.lib.pk_system_set_graphs(s.h, 3) == N.PK_ERR_ARG
+        s.close()
+    for u, v in zip(*out):
+        assert np.array_equal(u, v)
