@@ -205,23 +205,29 @@ __global__ __launch_bounds__(NT) void k_adj_rhs(TileView tv, const int* __restri
         part[t] = Partial{pa, pb, flag, 0};
 }
 
-// Corner terms go to element planes EP_ADJ_X (dPhi/dx_a) and EP_ADJ_V
-// (dPhi/dv_a).
+// Corner terms (dPhi/dx_a, then dPhi/dv_a) go to the tet's four entries of
+// the node gather lists, at apos[a * T + t] (N_ADJ doubles each).
 __global__ __launch_bounds__(NT) void k_adj_element(int T, const int4* __restrict__ tets,
                                                     const unsigned char* __restrict__ live,
                                                     const double* __restrict__ prm,
                                                     const int* __restrict__ nodeGroup, const int* __restrict__ ok,
                                                     const int* __restrict__ nodeRow, const double* __restrict__ x,
                                                     const double* __restrict__ vnext, const double* __restrict__ lam,
-                                                    double dt, double* __restrict__ elem,
-                                                    double* __restrict__ gradLogE)
+                                                    double dt, const int* __restrict__ apos,
+                                                    double* __restrict__ elem, double* __restrict__ gradLogE)
 {
     const int t = xcd_block() * NT + static_cast<int>(threadIdx.x);
     if (t >= T || !live[t])
         return;
     const int4 n4 = tets[t];
     const size_t TT = static_cast<size_t>(T);
-    auto out = [&](int plane) -> double& { return elem[static_cast<size_t>(plane) * TT + t]; };
+    int pos[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        pos[a] = apos[a * TT + t];
+    // corner a's entry, value e (0..2 dPhi/dx_a, 3..5 dPhi/dv_a); every
+    // corner of a live tet has one
+    auto out = [&](int a, int e) -> double& { return elem[static_cast<size_t>(pos[a]) * N_ADJ + e]; };
     if (!ok[nodeGroup[n4.x]])
         return;  // the body's substep was identity: its nodes are not gathered
     const int ids[4] = {n4.x, n4.y, n4.z, n4.w};
@@ -257,8 +263,9 @@ __global__ __launch_bounds__(NT) void k_adj_element(int T, const int4* __restric
     if (!fin9(F))
     {
         // the forward skipped the tet: no terms
-        for (int k = 0; k < 24; ++k)
-            out(EP_ADJ_X + k) = 0.0;
+        for (int a = 0; a < 4; ++a)
+            for (int e = 0; e < N_ADJ; ++e)
+                out(a, e) = 0.0;
         return;
     }
     double R[9];
@@ -302,15 +309,15 @@ __global__ __launch_bounds__(NT) void k_adj_element(int T, const int4* __restric
 #pragma unroll
         for (int i = 0; i < 3; ++i)
         {
-            out(EP_ADJ_X + 3 * a + i) = Fb[3 * i] * g[a][0] + Fb[3 * i + 1] * g[a][1] + Fb[3 * i + 2] * g[a][2];
-            out(EP_ADJ_V + 3 * a + i) =
+            out(a, i) = Fb[3 * i] * g[a][0] + Fb[3 * i + 1] * g[a][1] + Fb[3 * i + 2] * g[a][2];
+            out(a, 3 + i) =
                 -dt2 * vol * (P[3 * i] * g[a][0] + P[3 * i + 1] * g[a][1] + P[3 * i + 2] * g[a][2]) -
                 dt * damp * ls[a][i];
         }
 }
 
-__global__ __launch_bounds__(NT) void k_adj_nodes(int N, int T, const int* __restrict__ astart,
-                                                  const int* __restrict__ alist, const double* __restrict__ elem,
+__global__ __launch_bounds__(NT) void k_adj_nodes(int N, const int* __restrict__ astart,
+                                                  const double* __restrict__ elem,
                                                   const int* __restrict__ nodeRow, const int* __restrict__ nodeGroup,
                                                   const int* __restrict__ ok, const int* __restrict__ connStart,
                                                   const int2* __restrict__ connList,
@@ -338,17 +345,16 @@ __global__ __launch_bounds__(NT) void k_adj_nodes(int N, int T, const int* __res
         for (int i = 0; i < 3; ++i)
             vb[i] = vb[i] + xb[i] * dt;
     }
+    // the node's corner entries: one contiguous run, in tet order
     const int k1 = astart[n + 1];
     for (int k = astart[n]; k < k1; ++k)
     {
-        const int c = alist[k];
-        const size_t TT = static_cast<size_t>(T);
-        const double* e = elem + (c >> 2) + 3 * static_cast<size_t>(c & 3) * TT;
+        const double* e = elem + static_cast<size_t>(k) * N_ADJ;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
         {
-            xb[i] += e[(EP_ADJ_X + i) * TT];
-            vb[i] += e[(EP_ADJ_V + i) * TT];
+            xb[i] += e[i];
+            vb[i] += e[3 + i];
         }
     }
     if (connStart)
@@ -581,14 +587,14 @@ void launch_adj_element(const DeviceView& v, const int* ok, const double* vnext,
     if (v.T > 0)
         hipLaunchKernelGGL(k_adj_element, dim3(xcd_grid((v.T + NT - 1) / NT)), dim3(NT), 0, v.stream, v.T, v.tets,
                            v.live,
-                           v.prm, v.nodeGroup, ok, v.nodeRow, v.x, vnext, v.xs, dt, v.adj, gradLogE);
+                           v.prm, v.nodeGroup, ok, v.nodeRow, v.x, vnext, v.xs, dt, v.apos, v.adj, gradLogE);
 }
 
 void launch_adj_nodes(const DeviceView& v, const int* ok, double dt, double* xbar, double* vbar)
 {
     if (v.N > 0)
-        hipLaunchKernelGGL(k_adj_nodes, dim3(xcd_grid((v.N + NT - 1) / NT)), dim3(NT), 0, v.stream, v.N, v.T,
-                           v.astart, v.alist, v.adj, v.nodeRow, v.nodeGroup, ok,
+        hipLaunchKernelGGL(k_adj_nodes, dim3(xcd_grid((v.N + NT - 1) / NT)), dim3(NT), 0, v.stream, v.N,
+                           v.astart, v.adj, v.nodeRow, v.nodeGroup, ok,
                            v.nconn > 0 ? v.connNodeStart : nullptr, v.connNodeList, v.connData, v.connAdj,
                            (v.nsph > 0 && v.ncand > 0) ? v.candNodeStart : nullptr, v.candNodeList, v.candAdj, dt,
                            xbar, vbar);

@@ -1135,8 +1135,9 @@ void launch_spheres(const DeviceView& v, double dt, int implicit)
 // still reach (basis planes, its corner entries, its reverse-mode planes)
 // become zeros, which stay since the element kernels skip tets not live.
 __global__ __launch_bounds__(NT) void k_zero_tets(int n, const int* __restrict__ ids, int T,
-                                                  const int* __restrict__ cpos, double* __restrict__ basis,
-                                                  double* __restrict__ corner, double* __restrict__ adj)
+                                                  const int* __restrict__ cpos, const int* __restrict__ apos,
+                                                  double* __restrict__ basis, double* __restrict__ corner,
+                                                  double* __restrict__ adj)
 {
     const int i = blockIdx.x * NT + threadIdx.x;
     if (i >= n)
@@ -1153,8 +1154,13 @@ __global__ __launch_bounds__(NT) void k_zero_tets(int n, const int* __restrict__
                 corner[static_cast<size_t>(pos) * N_CORNER + e] = 0.0;
     }
     if (adj)
-        for (int k = 0; k < EP_ADJ_COUNT; ++k)
-            adj[(EP_ADJ_X + k) * TT + t] = 0.0;
+        for (int a = 0; a < 4; ++a)
+        {
+            const int pos = apos[a * TT + t];
+            if (pos >= 0)
+                for (int e = 0; e < N_ADJ; ++e)
+                    adj[static_cast<size_t>(pos) * N_ADJ + e] = 0.0;
+        }
 }
 
 // live[ids[i]] = liveVal[i], tact[ids[i]] = actVal[i].
@@ -1174,8 +1180,8 @@ __global__ __launch_bounds__(NT) void k_tet_flags(int n, const int* __restrict__
 void launch_zero_tets(const DeviceView& v, int n, const int* ids)
 {
     if (n > 0)
-        hipLaunchKernelGGL(k_zero_tets, dim3((n + NT - 1) / NT), dim3(NT), 0, v.stream, n, ids, v.T, v.cpos, v.basis,
-                           v.corner, v.adj);
+        hipLaunchKernelGGL(k_zero_tets, dim3((n + NT - 1) / NT), dim3(NT), 0, v.stream, n, ids, v.T, v.cpos, v.apos,
+                           v.basis, v.corner, v.adj);
 }
 
 void launch_tet_flags(hipStream_t stream, int n, const int* ids, const unsigned char* liveVal,

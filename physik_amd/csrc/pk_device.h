@@ -366,7 +366,8 @@ struct DeviceView
     double* v;
     double* basis;       // per-tet R g_a and R R^T planes
     double* corner;      // forces and K v in row order
-    double* adj;         // reverse-mode corner terms, EP_ADJ_COUNT planes of T
+    double* adj;         // reverse-mode corner terms, N_ADJ per alist entry
+    const int* apos;     // 4 T: alist index of corner a of tet t, or -1
     void* minv;   // M: block-Jacobi inverse, 9 planes of nd
     double* xs;   // CG solution dv
     double* r;

@@ -132,7 +132,7 @@ struct pk_system
     pk::DevBuf<double> prm, kdat, x, v, basis, corner, adj, mrow, xs, r, pa;
     pk::DevBuf<char> minv;  // storage type, like the CG vectors
     pk::DevBuf<int> cstart, clist, uFull, uMirror, slotRow, rowGroup, fstart, fcol, nstart, nlist, rowNode, active;
-    pk::DevBuf<int> astart, alist, nodeRow, nodeGroup, cpos, diagSlot;
+    pk::DevBuf<int> astart, alist, apos, nodeRow, nodeGroup, cpos, diagSlot;
     // sliced upper-only storage: slice descriptors, lane masks, lower ELL
     // entries, the element of each upper slot, each row's (slice, lane) and
     // each SpMV tile's first slice
@@ -322,6 +322,7 @@ struct pk_system
         dv.rowNode = rowNode.p;
         dv.astart = astart.p;
         dv.alist = alist.p;
+        dv.apos = apos.p;
         dv.nodeRow = nodeRow.p;
         dv.nodeGroup = nodeGroup.p;
         dv.mrow = mrow.p;
@@ -1308,6 +1309,7 @@ static int setup_device(pk_system* s)
     if ((rc = s->rowNode.upload(t.rowNode, st)) != PK_OK) return rc;
     if ((rc = s->astart.upload(t.astart, st)) != PK_OK) return rc;
     if ((rc = s->alist.upload(t.alist, st)) != PK_OK) return rc;
+    if ((rc = s->apos.upload(t.apos, st)) != PK_OK) return rc;
     if ((rc = s->nodeRow.upload(t.dyn, st)) != PK_OK) return rc;
     if ((rc = s->nodeGroup.upload(t.nodeGroup, st)) != PK_OK) return rc;
     if ((rc = s->cpos.upload(t.cpos, st)) != PK_OK) return rc;
@@ -1979,11 +1981,11 @@ static int backward_entry(pk_system* s, const double* xbar, const double* vbar, 
     if ((rc = s->xbar.ensure(n3)) != PK_OK) return rc;
     if ((rc = s->vbar.ensure(n3)) != PK_OK) return rc;
     if ((rc = s->gradE.ensure(static_cast<size_t>(t.T))) != PK_OK) return rc;
-    if (s->adj.n < static_cast<size_t>(t.T) * EP_ADJ_COUNT)
+    if (s->adj.n < std::max<size_t>(1, t.alist.size()) * N_ADJ)
     {
-        if ((rc = s->adj.alloc(static_cast<size_t>(t.T) * EP_ADJ_COUNT)) != PK_OK) return rc;
+        if ((rc = s->adj.alloc(std::max<size_t>(1, t.alist.size()) * N_ADJ)) != PK_OK) return rc;
         // tets switched off in place are still in the node gather lists and
-        // the reverse element kernel skips them: their planes must read 0
+        // the reverse element kernel skips them: their entries must read 0
         PK_HIP(hipMemsetAsync(s->adj.p, 0, s->adj.n * sizeof(double), s->stream));
     }
     if (xbar)

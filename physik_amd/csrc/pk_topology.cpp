@@ -427,6 +427,9 @@ int build_topology(const pk_mesh_desc& d, Topology& topo, std::string& err, bool
                 for (int a = 0; a < 4; ++a)
                     topo.alist[afill[d.tets[4 * t + a]]++] = pack_corner(t, a);
     }
+    topo.apos.assign(static_cast<size_t>(T) * 4, -1);
+    for (size_t k = 0; k < topo.alist.size(); ++k)
+        topo.apos[static_cast<size_t>(topo.alist[k] & 3) * T + (topo.alist[k] >> 2)] = static_cast<int>(k);
 
     if (wantScatter)
     {

@@ -39,17 +39,16 @@ enum TetParam
 //           lists, written at cpos[a * T + t] so that the right-hand side
 //           reads a contiguous run: implicit, dt f_a - dt^2 sum_b (R K_ab R^T)
 //           v_b (the force dropped when not finite); explicit, f_a.
-// Reverse mode keeps its per-tet corner terms in planes adj[k * T + t]:
-// dPhi/dx_a (k = 3a + i), then dPhi/dv_a (k = 12 + 3a + i).
+// Reverse mode keeps its corner terms in the order of the node gather lists
+// (alist), N_ADJ doubles per (node, tet, corner) written at apos[a * T + t]:
+// dPhi/dx_a (3), then dPhi/dv_a (3), so that a node reads one contiguous run.
 enum ElemLayout
 {
     EB_RG = 0,
     EB_RRT = 12,
     EB_COUNT = 18,
-    N_CORNER = 3,     // doubles per corner entry
-    EP_ADJ_X = 0,
-    EP_ADJ_V = 12,
-    EP_ADJ_COUNT = 24
+    N_CORNER = 3,  // doubles per corner entry
+    N_ADJ = 6      // doubles per reverse-mode corner entry
 };
 
 // Constant stiffness factors of one tet, one 96-byte record per tet
@@ -109,6 +108,7 @@ struct Topology
     std::vector<int> nstart, nlist;      // dynamic row -> packed (tet, corner), increasing tet
     std::vector<int> cpos;               // 4 T: row-list index of corner a of tet t, or -1
     std::vector<int> astart, alist;      // every node -> packed (tet, corner) over live tets, increasing tet
+    std::vector<int> apos;               // 4 T: alist index of corner a of tet t, or -1
     std::vector<int> nodeGroup;          // N -> body
     std::vector<uint8_t> nodeActive;     // N
     std::vector<uint8_t> nodeFixed;      // N
