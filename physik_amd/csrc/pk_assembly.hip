@@ -40,12 +40,13 @@ __global__ void k_reset(GroupState* gs, int G, int* active, int* kbase, double t
     for (int g = threadIdx.x; g < G; g += blockDim.x)
     {
         GroupState& s = gs[g];
-        s.delta[0] = s.delta[1] = s.target = s.residual = 0.0;
+        s.delta[0] = s.delta[1] = s.alpha[0] = s.alpha[1] = s.target = s.residual = 0.0;
         s.tol = tol;
         s.maxIt = maxIt;
         s.iters = 0;
         s.stopAt = s.hasRows ? -1 : 0;
         s.stopInUpdate = 0;
+        s.dirStop = -1;
         s.converged = 0;
         s.failed = 0;
         s.nonfinite = 0;

@@ -1,17 +1,19 @@
 // Block-Jacobi PCG kernels for gfx950 (ConjugateGradientSolver.cpp:114-241).
 //
 // Iteration k is three launches:
-//   k_dir(k)     vector tiles. Every workgroup sums the (r.z, r.r) partials
-//                the previous launch left for its body, takes the reference's
-//                stop decisions for the end of iteration k-1 (k = 0: the CG
+//   k_dir(k)     vector tiles. Applies x += alpha_{k-1} d_{k-1} for its rows;
+//                every workgroup sums the (r.z, r.r) partials the previous
+//                launch left for its body, takes the reference's stop
+//                decisions for the end of iteration k-1 (k = 0: the CG
 //                start) and forms beta = delta_k / delta_{k-1}; it then writes
 //                d_k = z_k + beta d_{k-1} for its rows.
 //   k_spmv(k)    BSR row tiles, no reduction prologue: q = A d_k on the full
 //                BSR (both triangles) and a d.q partial per tile.
 //   k_update(k)  vector tiles. Every workgroup sums the body's d.q partials,
 //                alpha = delta_k / d.q (or stops on d.q <= 0), updates
-//                x += alpha d, r -= alpha q, z = M^-1 r and leaves (r.z, r.r)
-//                partials.
+//                r -= alpha q, z = M^-1 r and leaves (r.z, r.r) partials; the
+//                matching x += alpha d_k waits for k_dir(k + 1), which reads
+//                d_k anyway.
 // Totals are recomputed by every workgroup of the two vector kernels from the
 // same partials in the same order, so no workgroup waits on another and no
 // device-scope fence is needed inside a launch; the vector kernels have few
@@ -38,6 +40,8 @@ __device__ inline void record_stop(GroupState& s, int* active, int k, int inUpda
 {
     s.stopAt = k;
     s.stopInUpdate = inUpdate;
+    if (!inUpdate)
+        s.dirStop = k;
     s.iters = iters;
     s.converged = converged ? 1 : 0;
     s.residual = residual;
@@ -163,35 +167,94 @@ __device__ inline bool stop_rules(int k, GroupState& s, double rz, double rr, in
     return stop;
 }
 
-// d_k = z_k + beta d_{k-1} for the tile's rows, after the stop decisions for
-// the end of iteration k-1. At k = 0 the right-hand side already wrote d_0.
-template <class M>
+// The end of iteration k-1 and the start of iteration k for the tile's rows.
+// The solution's updates are applied two iterations at a time: at even k,
+// x += alpha_{k-2} d_{k-2}, then x += alpha_{k-1} d_{k-1} (two fused
+// multiply-adds in iteration order, so x is bit for bit what one update per
+// iteration gives, ConjugateGradientSolver.cpp:204); at odd k the update of
+// iteration k-1 waits, unless the loop ends here. Then the stop decisions for
+// the end of iteration k-1 and, when the loop goes on, d_k = z_k +
+// beta d_{k-1}. At k = 0 the right-hand side already wrote x = 0 and d_0. A
+// body whose loop an earlier launch ended is skipped (a stop that this
+// launch's designated workgroup records, stopAt == k, is reached by every
+// workgroup, after its share of x).
+template <class M, bool EVEN>
 __device__ inline void dir_body(int k, const TileView& vt, const M* __restrict__ z, const M* __restrict__ dprev,
-                                M* __restrict__ dcur, const Partial* __restrict__ pbPrev, GroupState* gs, int* active)
+                                const M* __restrict__ dprev2, M* __restrict__ dcur, double* __restrict__ xs,
+                                const Partial* __restrict__ pbPrev, GroupState* gs, int* active)
 {
     int t;
     RowTile tl;
     if (!tile_for(vt, t, tl))
         return;
+    // the thread's first kPre values of z, d_{k-1} (and at even k d_{k-2}
+    // and x; a 512-row tile whole) are in flight while the stop flag and the
+    // partials are read
+    constexpr int kPre = 6;
+    const int i0 = 3 * tl.rowBegin + static_cast<int>(threadIdx.x), iEnd = 3 * tl.rowEnd;
+    const bool two = EVEN && k > 0;
+    M zp[kPre], dp[kPre], ep[kPre];
+    double xp[kPre];
+    if (k > 0 && tl.rowEnd > tl.rowBegin)
+    {
+#pragma unroll
+        for (int j = 0; j < kPre; ++j)
+        {
+            const int i = i0 + j * NT < iEnd ? i0 + j * NT : i0 - static_cast<int>(threadIdx.x);
+            zp[j] = z[i];
+            dp[j] = dprev[i];
+            if (EVEN)
+            {
+                ep[j] = dprev2[i];
+                xp[j] = xs[i];
+            }
+        }
+    }
     const int g = tl.group;
     GroupState& s = gs[g];
-    if (read_uniform(&s.stopAt) >= 0)
+    const int stopAt = read_uniform(&s.stopAt);
+    if (stopAt >= 0 && stopAt < k)
         return;
+    const double a1 = k > 0 ? s.alpha[(k - 1) & 1] : 0.0;
+    const double a2 = two ? s.alpha[k & 1] : 0.0;  // alpha_{k-2}
     double rz, rr;
     int flags;
     sum_partials(pbPrev, vt.groupBegin[g], vt.groupCount[g], rz, rr, flags);
     double beta;
-    if (stop_rules(k, s, rz, rr, flags, t == vt.groupBegin[g] && threadIdx.x == 0, active, beta))
-        return;
+    const bool stop = stop_rules(k, s, rz, rr, flags, t == vt.groupBegin[g] && threadIdx.x == 0, active, beta);
     if (k == 0)
         return;
-    for (int i = 3 * tl.rowBegin + static_cast<int>(threadIdx.x); i < 3 * tl.rowEnd; i += NT)
-        dcur[i] = direction(z[i], dprev[i], beta);
+#pragma unroll
+    for (int j = 0; j < kPre; ++j)
+    {
+        const int i = i0 + j * NT;
+        if (i < iEnd)
+        {
+            if (EVEN)
+                xs[i] = __builtin_fma(static_cast<double>(dp[j]), a1,
+                                      __builtin_fma(static_cast<double>(ep[j]), a2, xp[j]));
+            else if (stop)
+                xs[i] = __builtin_fma(static_cast<double>(dp[j]), a1, xs[i]);
+            if (!stop)
+                dcur[i] = direction(zp[j], dp[j], beta);
+        }
+    }
+    for (int i = i0 + kPre * NT; i < iEnd; i += NT)
+    {
+        const M dv = dprev[i];
+        if (EVEN)
+            xs[i] = __builtin_fma(static_cast<double>(dv), a1, __builtin_fma(static_cast<double>(dprev2[i]), a2, xs[i]));
+        else if (stop)
+            xs[i] = __builtin_fma(static_cast<double>(dv), a1, xs[i]);
+        if (!stop)
+            dcur[i] = direction(z[i], dv, beta);
+    }
 }
 
-template <class M>
+template <class M, bool EVEN>
 __global__ __launch_bounds__(NT) void k_dir(const int* kb, int k0, TileView vt, const M* __restrict__ z,
-                                            const M* __restrict__ dprev, M* __restrict__ dcur,
+                                            const M* __restrict__ dprev, const M* __restrict__ dprev2,
+                                            M* __restrict__ dcur, double* __restrict__ xs,
                                             const Partial* __restrict__ pbPrev, GroupState* gs, int* active,
                                             unsigned long long* stamps)
 {
@@ -199,7 +262,7 @@ __global__ __launch_bounds__(NT) void k_dir(const int* kb, int k0, TileView vt, 
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    dir_body<M>(k, vt, z, dprev, dcur, pbPrev, gs, active);
+    dir_body<M, EVEN>(k, vt, z, dprev, dprev2, dcur, xs, pbPrev, gs, active);
     stamp_end(stamp);
 }
 
@@ -730,8 +793,8 @@ __global__ __launch_bounds__(NT) void k_spmv(const int* kb, int k0, TileView st,
 
 template <class M, bool LONG>
 __device__ inline void update_body(int k, const TileView& vt, const TileView& st, const M* __restrict__ minv,
-                                   int nd, const M* __restrict__ dcur, const M* __restrict__ q,
-                                   double* __restrict__ xs, double* __restrict__ rv, M* __restrict__ zv,
+                                   int nd, const M* __restrict__ q, double* __restrict__ rv, M* __restrict__ zv,
+                                   const M* __restrict__ dprev, double* __restrict__ xs,
                                    const double* __restrict__ pa, Partial* __restrict__ pbCur,
                                    const Partial* __restrict__ pbPrev, GroupState* gs, int* active)
 {
@@ -739,13 +802,13 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
     RowTile tl;
     if (!tile_for(vt, t, tl))
         return;
-    // the thread's first kPre rows (d, q, x, r, M^-1) are in flight while the
-    // stop flag and the d.q partials are read
+    // the thread's first kPre rows (q, r, M^-1) are in flight while the stop
+    // flag and the d.q partials are read
     constexpr int kPre = 2;
     const size_t N = static_cast<size_t>(nd);
     const int row0 = tl.rowBegin + static_cast<int>(threadIdx.x);
-    M dpre[kPre][3], qpre[kPre][3], mpre[kPre][9];
-    double xpre[kPre][3], rpre[kPre][3];
+    M qpre[kPre][3], mpre[kPre][9];
+    double rpre[kPre][3];
 #pragma unroll
     for (int j = 0; j < kPre; ++j)
     {
@@ -754,9 +817,7 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
 #pragma unroll
         for (int i = 0; i < 3; ++i)
         {
-            dpre[j][i] = dcur[3 * rr + i];
             qpre[j][i] = q[3 * rr + i];
-            xpre[j][i] = xs[3 * rr + i];
             rpre[j][i] = rv[3 * rr + i];
         }
 #pragma unroll
@@ -765,32 +826,45 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
     }
     const int g = tl.group;
     GroupState& s = gs[g];
+    // ended by an earlier launch (k_dir(k) included: dirStop is from a
+    // previous launch); a stop that this launch's designated workgroup
+    // records (stopAt == k) is reached by every workgroup below
     const int stopAt = read_uniform(&s.stopAt);
-    if (stopAt >= 0 && stopAt <= k)
+    if (stopAt >= 0 && (stopAt < k || s.dirStop == k))
         return;
     const double dq = LONG ? sum_doubles_long(pa, st.groupBegin[g], st.groupCount[g])
                            : sum_doubles(pa, st.groupBegin[g], st.groupCount[g]);
     const double delta = s.delta[k & 1];
     const double alpha = delta / dq;
+    const bool designated = t == vt.groupBegin[g] && threadIdx.x == 0;
     if (!fin(dq) || dq <= 0.0 || !fin(alpha))
     {
-        // break before any update (ConjugateGradientSolver.cpp:192-202)
+        // break before any update (ConjugateGradientSolver.cpp:192-202); at
+        // odd k the solution's update of iteration k-1 is still pending
+        if (k & 1)
+        {
+            const double a1 = s.alpha[(k - 1) & 1];
+            for (int i = 3 * tl.rowBegin + static_cast<int>(threadIdx.x); i < 3 * tl.rowEnd; i += NT)
+                xs[i] = __builtin_fma(static_cast<double>(dprev[i]), a1, xs[i]);
+        }
         double rz, rr;
         int fl;
         sum_partials(pbPrev, vt.groupBegin[g], vt.groupCount[g], rz, rr, fl);
-        if (t == vt.groupBegin[g] && threadIdx.x == 0)
+        if (designated)
             record_stop(s, active, k, 1, k, delta <= s.target, norm_of(rr), false);
         return;
     }
+    // x += alpha d_k is left to k_dir, which reads d_k anyway
+    if (designated)
+        s.alpha[k & 1] = alpha;
     double pz = 0.0, pr = 0.0;
     int flag = 0;
-    // one row: x += alpha d, r -= alpha q, z = M^-1 r, and the partials
-    auto one = [&](int row, const M* dd, const M* qq, const double* xx, const double* rq, const M* mm) {
+    // one row: r -= alpha q, z = M^-1 r, and the partials
+    auto one = [&](int row, const M* qq, const double* rq, const M* mm) {
         double r[3], z[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
         {
-            xs[3 * row + i] = xx[i] + static_cast<double>(dd[i]) * alpha;
             r[i] = rq[i] + static_cast<double>(qq[i]) * (-alpha);
             rv[3 * row + i] = r[i];
         }
@@ -813,23 +887,21 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
 #pragma unroll
     for (int j = 0; j < kPre; ++j)
         if (row0 + j * NT < tl.rowEnd)
-            one(row0 + j * NT, dpre[j], qpre[j], xpre[j], rpre[j], mpre[j]);
+            one(row0 + j * NT, qpre[j], rpre[j], mpre[j]);
     for (int row = row0 + kPre * NT; row < tl.rowEnd; row += NT)
     {
-        M dd[3], qq[3], mm[9];
-        double xx[3], rq[3];
+        M qq[3], mm[9];
+        double rq[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
         {
-            dd[i] = dcur[3 * row + i];
             qq[i] = q[3 * row + i];
-            xx[i] = xs[3 * row + i];
             rq[i] = rv[3 * row + i];
         }
 #pragma unroll
         for (int e = 0; e < 9; ++e)
             mm[e] = minv[e * N + row];
-        one(row, dd, qq, xx, rq, mm);
+        one(row, qq, rq, mm);
     }
     double u = 0.0;
     block_sum3(pz, pr, u);
@@ -840,9 +912,9 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
 
 template <class M, bool LONG>
 __global__ __launch_bounds__(NT) void k_update(const int* kb, int k0, TileView vt, TileView st,
-                                               const M* __restrict__ minv, int nd, const M* __restrict__ dcur,
-                                               const M* __restrict__ q, double* __restrict__ xs,
+                                               const M* __restrict__ minv, int nd, const M* __restrict__ q,
                                                double* __restrict__ rv, M* __restrict__ zv,
+                                               const M* __restrict__ dprev, double* __restrict__ xs,
                                                const double* __restrict__ pa, Partial* __restrict__ pbCur,
                                                const Partial* __restrict__ pbPrev, GroupState* gs, int* active,
                                                unsigned long long* stamps)
@@ -851,7 +923,7 @@ __global__ __launch_bounds__(NT) void k_update(const int* kb, int k0, TileView v
     const int k = li.k;
     unsigned long long* stamp = li.stamp;
     stamp_begin(stamp);
-    update_body<M, LONG>(k, vt, st, minv, nd, dcur, q, xs, rv, zv, pa, pbCur, pbPrev, gs, active);
+    update_body<M, LONG>(k, vt, st, minv, nd, q, rv, zv, dprev, xs, pa, pbCur, pbPrev, gs, active);
     stamp_end(stamp);
 }
 
@@ -961,7 +1033,7 @@ static void launch_spmv_upper(const DeviceView& v, const int* kb, int k0, unsign
     const size_t lds = 3 * sizeof(M) * static_cast<size_t>(v.winCap);
     auto kernel = v.stageSegs ? k_spmv_upper<M, U, true> : k_spmv_upper<M, U, false>;
     hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds, v.stream, kb, k0,
-                       v.spmv, v.slice, static_cast<const M*>(v.vals), static_cast<const M*>(v.d[k0 & 1]),
+                       v.spmv, v.slice, static_cast<const M*>(v.vals), static_cast<const M*>(v.d[k0 & 3]),
                        static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap, stamp);
 }
 
@@ -974,7 +1046,7 @@ static void launch_spmv_shape(const DeviceView& v, const int* kb, int k0, unsign
     auto kernel = v.stageSegs ? k_spmv<M, LPR, U, true> : k_spmv<M, LPR, U, false>;
     hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.spmv.per)), dim3(NT), lds, v.stream, kb, k0,
                        v.spmv, v.fstart, v.fcol, v.fdiag, static_cast<const M*>(v.vals), v.nnzf,
-                       static_cast<const M*>(v.d[k0 & 1]), static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap,
+                       static_cast<const M*>(v.d[k0 & 3]), static_cast<M*>(v.q), v.pa, v.gs, v.spmvWin, v.winCap,
                        stamp);
 }
 
@@ -983,9 +1055,13 @@ void launch_dir(const DeviceView& v, const int* kb, int k0, unsigned long long* 
 {
     if (v.vec.count == 0)
         return;
-    hipLaunchKernelGGL((k_dir<M>), dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, kb, k0,
-                       v.vec, static_cast<const M*>(v.z), static_cast<const M*>(v.d[(k0 + 1) & 1]),
-                       static_cast<M*>(v.d[k0 & 1]), v.pb[(k0 + 1) & 1], v.gs, v.active, stamp);
+    // d_k lives in d[k & 3]: a captured chunk starts at a multiple of 4, so
+    // the slot of its launch k0 is k0 & 3 as well
+    auto kernel = (k0 & 1) ? k_dir<M, false> : k_dir<M, true>;
+    hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, kb, k0, v.vec,
+                       static_cast<const M*>(v.z), static_cast<const M*>(v.d[(k0 + 3) & 3]),
+                       static_cast<const M*>(v.d[(k0 + 2) & 3]), static_cast<M*>(v.d[k0 & 3]), v.xs,
+                       v.pb[(k0 + 1) & 1], v.gs, v.active, stamp);
 }
 
 // Upper-only storage runs one lane per row (lanesPerRow 1) and one round
@@ -1024,8 +1100,8 @@ void launch_update(const DeviceView& v, const int* kb, int k0, unsigned long lon
         return;
     auto kernel = v.longSums ? k_update<M, true> : k_update<M, false>;
     hipLaunchKernelGGL(kernel, dim3(8u * static_cast<unsigned>(v.vec.per)), dim3(NT), 0, v.stream, kb, k0,
-                       v.vec, v.spmv, static_cast<const M*>(v.minv), v.nd, static_cast<const M*>(v.d[k0 & 1]),
-                       static_cast<const M*>(v.q), v.xs, v.r, static_cast<M*>(v.z), v.pa, v.pb[k0 & 1],
+                       v.vec, v.spmv, static_cast<const M*>(v.minv), v.nd, static_cast<const M*>(v.q), v.r,
+                       static_cast<M*>(v.z), static_cast<const M*>(v.d[(k0 + 3) & 3]), v.xs, v.pa, v.pb[k0 & 1],
                        v.pb[(k0 + 1) & 1], v.gs, v.active, stamp);
 }
 

@@ -25,6 +25,7 @@ enum FemModelId
 struct GroupState
 {
     double delta[2];  // r.z entering iteration k, at index k & 1
+    double alpha[2];  // step length of iteration k at index k & 1 (k_update(k) writes)
     double target;    // tol^2 * delta_0
     double residual;  // reported residual norm
     double tol;
@@ -32,6 +33,7 @@ struct GroupState
     int iters;
     int stopAt;       // launch index that ended the loop (-1 while running)
     int stopInUpdate; // 1 when the update launch of stopAt ended it, 0 for the SpMV launch
+    int dirStop;      // launch index whose k_dir ended the loop (-1 otherwise)
     int converged;
     int failed;       // the solve is unusable: do not integrate
     int nonfinite;    // integration produced a non-finite value
@@ -373,7 +375,7 @@ struct DeviceView
     double* r;
     void* vals;   // M
     void* z;      // M
-    void* d[2];   // M, by iteration parity
+    void* d[4];   // M, d_k at index k & 3
     void* q;      // M
     GroupState* gs;
     int* active;

@@ -144,7 +144,7 @@ struct pk_system
     std::vector<int> groupSlice;  // first slice of each body
     long long nE = 0;
     pk::DevBuf<int> ucol;
-    pk::DevBuf<char> vals, z, d0, d1, q;  // element type set by the precision
+    pk::DevBuf<char> vals, z, d0, d1, d2, d3, q;  // element type set by the precision
     pk::DevBuf<pk::Partial> pb0, pb1;
     // batched solves: the tile of each CG workgroup, over the bodies still
     // iterating (vector and SpMV tilings), rebuilt every few iterations
@@ -339,6 +339,8 @@ struct pk_system
         dv.z = z.p;
         dv.d[0] = d0.p;
         dv.d[1] = d1.p;
+        dv.d[2] = d2.p;
+        dv.d[3] = d3.p;
         dv.q = q.p;
         dv.gs = gs.p;
         dv.active = active.p;
@@ -1133,6 +1135,8 @@ static int alloc_work(pk_system* s)
     if ((rc = s->z.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->d0.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->d1.alloc(nd3 * es)) != PK_OK) return rc;
+    if ((rc = s->d2.alloc(nd3 * es)) != PK_OK) return rc;
+    if ((rc = s->d3.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->q.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->minv.alloc(static_cast<size_t>(t.nd) * 9 * es)) != PK_OK) return rc;
     if ((rc = s->xs.alloc(nd3)) != PK_OK) return rc;

@@ -222,13 +222,16 @@ def test_explicit_mode_matches_oracle():
     assert rel(x, xo) <= 1e-12 and rel(v, vo) <= 1e-12
 
 
-def test_iteration_cap_accepts_unconverged_solution():
-    # SolverData accepts a finite unconverged CG result (PointConnectionTests.cpp:2480-2510)
+@pytest.mark.parametrize("cap", [1, 2, 3, 4])
+def test_iteration_cap_accepts_unconverged_solution(cap):
+    # SolverData accepts a finite unconverged CG result (PointConnectionTests.cpp:2480-2510).
+    # Odd and even caps: the solution's updates are applied two iterations at
+    # a time, so a loop that ends at an odd count applies its last one alone.
     w = W.beam10(n=3)
-    w.max_iter = 1
+    w.max_iter = cap
     s, o = run_pair(w, N.PK_FP64, 2, 1e-9)
     it, res, conv, st = s.results()
-    assert it[0] == 1 and not conv[0] and st[0] == N.GROUP_INTEGRATED
+    assert it[0] == cap and not conv[0] and st[0] == N.GROUP_INTEGRATED
 
 
 def test_nonfinite_velocity_fails_the_solve_and_keeps_state():
