@@ -212,6 +212,7 @@ __device__ inline void dir_body(int k, const TileView& vt, const M* __restrict__
     }
     const int g = tl.group;
     GroupState& s = gs[g];
+    const Partial p0 = first_partial(pbPrev, tl.vBegin, tl.vCount);
     const int stopAt = read_uniform(&s.stopAt);
     if (stopAt >= 0 && stopAt < k)
         return;
@@ -219,9 +220,9 @@ __device__ inline void dir_body(int k, const TileView& vt, const M* __restrict__
     const double a2 = two ? s.alpha[k & 1] : 0.0;  // alpha_{k-2}
     double rz, rr;
     int flags;
-    sum_partials(pbPrev, vt.groupBegin[g], vt.groupCount[g], rz, rr, flags);
+    sum_partials_from(p0, pbPrev, tl.vBegin, tl.vCount, rz, rr, flags);
     double beta;
-    const bool stop = stop_rules(k, s, rz, rr, flags, t == vt.groupBegin[g] && threadIdx.x == 0, active, beta);
+    const bool stop = stop_rules(k, s, rz, rr, flags, t == tl.vBegin && threadIdx.x == 0, active, beta);
     if (k == 0)
         return;
 #pragma unroll
@@ -826,17 +827,17 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
     }
     const int g = tl.group;
     GroupState& s = gs[g];
+    const double pa0 = LONG ? 0.0 : first_double(pa, tl.sBegin, tl.sCount);
     // ended by an earlier launch (k_dir(k) included: dirStop is from a
     // previous launch); a stop that this launch's designated workgroup
     // records (stopAt == k) is reached by every workgroup below
     const int stopAt = read_uniform(&s.stopAt);
     if (stopAt >= 0 && (stopAt < k || s.dirStop == k))
         return;
-    const double dq = LONG ? sum_doubles_long(pa, st.groupBegin[g], st.groupCount[g])
-                           : sum_doubles(pa, st.groupBegin[g], st.groupCount[g]);
+    const double dq = LONG ? sum_doubles_long(pa, tl.sBegin, tl.sCount) : sum_doubles_from(pa0, pa, tl.sBegin, tl.sCount);
     const double delta = s.delta[k & 1];
     const double alpha = delta / dq;
-    const bool designated = t == vt.groupBegin[g] && threadIdx.x == 0;
+    const bool designated = t == tl.vBegin && threadIdx.x == 0;
     if (!fin(dq) || dq <= 0.0 || !fin(alpha))
     {
         // break before any update (ConjugateGradientSolver.cpp:192-202); at
@@ -849,7 +850,7 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
         }
         double rz, rr;
         int fl;
-        sum_partials(pbPrev, vt.groupBegin[g], vt.groupCount[g], rz, rr, fl);
+        sum_partials(pbPrev, tl.vBegin, tl.vCount, rz, rr, fl);
         if (designated)
             record_stop(s, active, k, 1, k, delta <= s.target, norm_of(rr), false);
         return;

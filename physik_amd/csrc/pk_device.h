@@ -42,10 +42,14 @@ struct GroupState
     int integrated;   // the substep's state update of this body is done
 };
 
-// One workgroup of a row-tiled kernel: rows [rowBegin, rowEnd) of group `group`.
+// One workgroup of a row-tiled kernel: rows [rowBegin, rowEnd) of group
+// `group`, and where the group's partials of each tiling lie (vector tiles
+// vBegin .. vBegin + vCount - 1, SpMV tiles sBegin .. sBegin + sCount - 1),
+// so a CG workgroup finds them in the same load as its rows.
 struct RowTile
 {
     int group, rowBegin, rowEnd, pad;
+    int vBegin, vCount, sBegin, sCount;
 };
 
 // Per-workgroup partial sums of a reduction.
@@ -250,6 +254,39 @@ __device__ inline void sum_partials(const Partial* p, int begin, int count, doub
     flag = __syncthreads_or(f);
 }
 
+// The first partial a thread adds in sum_partials (or sum_doubles), loaded
+// ahead of the workgroup's other work; the *_from forms continue from it in
+// the same order, so their totals are bit-identical to the plain forms'.
+__device__ inline Partial first_partial(const Partial* p, int begin, int count)
+{
+    const int i = static_cast<int>(threadIdx.x);
+    return i < count ? p[begin + i] : Partial{0.0, 0.0, 0, 0};
+}
+
+__device__ inline double first_double(const double* p, int begin, int count)
+{
+    const int i = static_cast<int>(threadIdx.x);
+    return i < count ? p[begin + i] : 0.0;
+}
+
+__device__ inline void sum_partials_from(const Partial& q0, const Partial* p, int begin, int count, double& a,
+                                         double& b, int& flag)
+{
+    double x = 0.0 + q0.a, y = 0.0 + q0.b, z = 0.0;
+    int f = q0.flag;
+    for (int i = static_cast<int>(threadIdx.x) + NT; i < count; i += NT)
+    {
+        const Partial q = p[begin + i];
+        x += q.a;
+        y += q.b;
+        f |= q.flag;
+    }
+    block_sum3(x, y, z);
+    a = x;
+    b = y;
+    flag = __syncthreads_or(f);
+}
+
 // sum_doubles for long lists (a large body's SpMV partials): eight loads in
 // flight per thread, the adds in the same order, so the total is
 // bit-identical to sum_doubles'.
@@ -277,6 +314,15 @@ __device__ inline double sum_doubles(const double* p, int begin, int count)
 {
     double x = 0.0, y = 0.0, z = 0.0;
     for (int i = threadIdx.x; i < count; i += NT)
+        x += p[begin + i];
+    block_sum3(x, y, z);
+    return x;
+}
+
+__device__ inline double sum_doubles_from(double v0, const double* p, int begin, int count)
+{
+    double x = 0.0 + v0, y = 0.0, z = 0.0;
+    for (int i = static_cast<int>(threadIdx.x) + NT; i < count; i += NT)
         x += p[begin + i];
     block_sum3(x, y, z);
     return x;

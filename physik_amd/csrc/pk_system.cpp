@@ -391,65 +391,72 @@ struct pk_system
 namespace pk
 {
 
-static int build_tiles(pk_system* s, Tiles& tt, int rowsPerTile)
+// One tiling of the dynamic rows: runs of `rows` consecutive rows of one
+// body, with the first tile and the tile count of each body.
+struct HostTiling
 {
-    const Topology& t = s->topo;
     std::vector<RowTile> tiles;
-    std::vector<int> gb(static_cast<size_t>(t.G), 0), gc(static_cast<size_t>(t.G), 0);
+    std::vector<int> gb, gc;
+};
+
+static HostTiling tile_rows(const Topology& t, int rows)
+{
+    HostTiling h;
+    h.gb.assign(static_cast<size_t>(t.G), 0);
+    h.gc.assign(static_cast<size_t>(t.G), 0);
     for (int g = 0; g < t.G; ++g)
     {
-        gb[g] = static_cast<int>(tiles.size());
-        for (int r = t.groupRowBegin[g]; r < t.groupRowEnd[g]; r += rowsPerTile)
-            tiles.push_back(RowTile{g, r, std::min(r + rowsPerTile, t.groupRowEnd[g]), 0});
-        gc[g] = static_cast<int>(tiles.size()) - gb[g];
+        h.gb[g] = static_cast<int>(h.tiles.size());
+        for (int r = t.groupRowBegin[g]; r < t.groupRowEnd[g]; r += rows)
+            h.tiles.push_back(RowTile{g, r, std::min(r + rows, t.groupRowEnd[g]), 0, 0, 0, 0, 0});
+        h.gc[g] = static_cast<int>(h.tiles.size()) - h.gb[g];
     }
-    tt.count = static_cast<int>(tiles.size());
+    return h;
+}
+
+static int upload_tiling(pk_system* s, Tiles& tt, const HostTiling& h)
+{
+    tt.count = static_cast<int>(h.tiles.size());
     tt.per = (tt.count + 7) / 8;
-    tt.longest = gc.empty() ? 0 : *std::max_element(gc.begin(), gc.end());
-    tt.hostBegin = gb;
+    tt.longest = h.gc.empty() ? 0 : *std::max_element(h.gc.begin(), h.gc.end());
+    tt.hostBegin = h.gb;
     int rc;
-    if ((rc = tt.tiles.upload(tiles, s->stream)) != PK_OK) return rc;
-    if ((rc = tt.groupBegin.upload(gb, s->stream)) != PK_OK) return rc;
-    if ((rc = tt.groupCount.upload(gc, s->stream)) != PK_OK) return rc;
-    return PK_OK;
+    if ((rc = tt.tiles.upload(h.tiles, s->stream)) != PK_OK) return rc;
+    if ((rc = tt.groupBegin.upload(h.gb, s->stream)) != PK_OK) return rc;
+    return tt.groupCount.upload(h.gc, s->stream);
 }
 
 static bool is_upper(const pk_system* s);
 
-// The SpMV tiling: runs of spmvRows consecutive rows of one body (with
-// sliced upper-only storage spmvRows is a multiple of kSliceRows, so every
-// tile is whole slices: tileSlice holds its first).
-static int build_spmv_tiles(pk_system* s)
+// The vector tiling (vecRows rows per tile) and the SpMV tiling (spmvRows;
+// with sliced upper-only storage a multiple of kSliceRows, so every tile is
+// whole slices: tileSlice holds its first). Every tile of either records
+// its body's partial ranges in both.
+static int build_tilings(pk_system* s)
 {
     const Topology& t = s->topo;
-    std::vector<RowTile>& tiles = s->spmvHostTiles;
-    tiles.clear();
-    const int R = s->spmvRows;
-    std::vector<int> gb(static_cast<size_t>(t.G), 0), gc(static_cast<size_t>(t.G), 0), first;
-    for (int g = 0; g < t.G; ++g)
-    {
-        gb[g] = static_cast<int>(tiles.size());
-        const int b = t.groupRowBegin[g], e = t.groupRowEnd[g];
-        for (int r = b; r < e; r += R)
+    HostTiling hv = tile_rows(t, s->vecRows), hs = tile_rows(t, s->spmvRows);
+    for (HostTiling* h : {&hv, &hs})
+        for (RowTile& tl : h->tiles)
         {
-            tiles.push_back(RowTile{g, r, std::min(r + R, e), 0});
-            if (is_upper(s))
-                first.push_back(s->groupSlice[g] + (r - b) / kSliceRows);
+            tl.vBegin = hv.gb[tl.group];
+            tl.vCount = hv.gc[tl.group];
+            tl.sBegin = hs.gb[tl.group];
+            tl.sCount = hs.gc[tl.group];
         }
-        gc[g] = static_cast<int>(tiles.size()) - gb[g];
-    }
-    int rc0;
-    if (is_upper(s) && (rc0 = s->tileSlice.upload(first, s->stream)) != PK_OK)
-        return rc0;
-    Tiles& tt = s->spmvTiles;
-    tt.count = static_cast<int>(tiles.size());
-    tt.per = (tt.count + 7) / 8;
-    tt.longest = gc.empty() ? 0 : *std::max_element(gc.begin(), gc.end());
-    tt.hostBegin = gb;
     int rc;
-    if ((rc = tt.tiles.upload(tiles, s->stream)) != PK_OK) return rc;
-    if ((rc = tt.groupBegin.upload(gb, s->stream)) != PK_OK) return rc;
-    return tt.groupCount.upload(gc, s->stream);
+    if (is_upper(s))
+    {
+        std::vector<int> first;
+        first.reserve(hs.tiles.size());
+        for (const RowTile& tl : hs.tiles)
+            first.push_back(s->groupSlice[tl.group] + (tl.rowBegin - t.groupRowBegin[tl.group]) / kSliceRows);
+        if ((rc = s->tileSlice.upload(first, s->stream)) != PK_OK)
+            return rc;
+    }
+    if ((rc = upload_tiling(s, s->vecTiles, hv)) != PK_OK) return rc;
+    s->spmvHostTiles = hs.tiles;
+    return upload_tiling(s, s->spmvTiles, hs);
 }
 
 static int upload_groups(pk_system* s)
@@ -1153,7 +1160,7 @@ static bool is_f32(const pk_system* s);
 // kernels' 7 waves per SIMD, 7 workgroups share a CU's 160 KB.
 constexpr size_t kStageBytes = 20 * 1024;
 
-// Staging of the SpMV tiles (same order as build_tiles). Each tile's
+// Staging of the SpMV tiles (same order as build_tilings). Each tile's
 // columns span a window [first, last]; when every window fits kStageBytes the
 // tiles stage their windows (one segment, the cheapest lookup). Otherwise
 // each tile stages its distinct columns cut at the three widest gaps between
@@ -1229,8 +1236,7 @@ static int setup_tiles(pk_system* s)
 {
     int rc;
     ++s->epoch;
-    if ((rc = build_tiles(s, s->vecTiles, s->vecRows)) != PK_OK) return rc;
-    if ((rc = build_spmv_tiles(s)) != PK_OK) return rc;
+    if ((rc = build_tilings(s)) != PK_OK) return rc;
     if ((rc = setup_windows(s)) != PK_OK) return rc;
     if ((rc = s->pa.alloc(std::max(1, s->spmvTiles.count))) != PK_OK) return rc;
     if ((rc = s->pb0.alloc(std::max(1, s->vecTiles.count))) != PK_OK) return rc;
