@@ -618,6 +618,234 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, int nd, const int* __
         gs[g].failed = 1;
 }
 
+// The assembly gather by (row, tet) pairs, atomic-free. A workgroup takes
+// one chunk of consecutive dynamic rows (at most kPairChunk pairs of the
+// rows' corner lists, kOffChunk off-diagonal contributions, kRowChunk rows
+// and kSlotChunk upper slots). Phase 1, one lane per pair (row r, tet t,
+// corner a of r): the lane reads t's basis and stiffness records once and
+// forms the blocks the pair contributes -- R K_aa R^T to r's diagonal slot,
+// R K_ab R^T to the upper slot (r, row of b) for every other corner b whose
+// row is greater -- with the arithmetic of gather_slot (the block of the
+// local pair (min, max), transposed when a > b), into LDS planes: diagonal
+// blocks at the pair's place, off-diagonal ones at their place in the
+// chunk's run of the off-diagonal lists (pairOff, three 10-bit fields,
+// 0x3ff for none). Meanwhile the other lookups go to LDS as well: each
+// slot's run descriptor (pack_run) and element, each row's mass, body and
+// preconditioner switch. Phase 2, one thread per (slot, component): the
+// slot's contributions summed in list order, which is tet order (the order
+// the reference adds them, SolverData.cpp:407-450), then the contacts, dt^2
+// and the mass as in k_assemble, written to full or sliced upper storage.
+// Phase 3, one thread per row: the block-Jacobi inverse (SolverData.cpp:
+// 260-306). A non-finite block or a refused inverse marks the body's solve
+// failed.
+// V mu g_a.g_b of local pair (a, b) from the stiffness record kd in global
+// memory (its line is in L1 after the record's load): kd[KD_VMUG +
+// kPairOf[a][b]], the index in closed form (lo, hi = min, max: 4 lo -
+// lo (lo - 1) / 2 + hi - lo). (Picking it from the record in registers by
+// selects makes the compiler index a scratch copy of the record.)
+__device__ inline double vmg_pick(const double* __restrict__ kd, int x, int y)
+{
+    const int lo = x < y ? x : y, hi = x < y ? y : x;
+    return kd[KD_VMUG + 4 * lo - (lo * (lo - 1)) / 2 + hi - lo];
+}
+
+// v[k] for a lane-varying k in 0..3, as selects (no indexed registers).
+__device__ inline double pick4(int k, double v0, double v1, double v2, double v3)
+{
+    return k == 0 ? v0 : (k == 1 ? v1 : (k == 2 ? v2 : v3));
+}
+
+template <class M>
+__global__ __launch_bounds__(NT) void k_assemble_pairs(
+    int nchunks, const int4* __restrict__ chunks, int nd, const int* __restrict__ nlist,
+    const int* __restrict__ pairOff, const int* __restrict__ slotRun, const int* __restrict__ uElem,
+    const int* __restrict__ uMirror,
+    const int* __restrict__ rowGroup, const double* __restrict__ basis, const double* __restrict__ kdat,
+    const double* __restrict__ mrow, double dt2, M* __restrict__ vals, long long nnzf, M* __restrict__ minv,
+    GroupState* gs, int upperOnly, const double* __restrict__ cslot)
+{
+    extern __shared__ __align__(16) double pk_pair_lds[];
+    double* P = pk_pair_lds;                 // 9 planes of kPairChunk: diagonal blocks
+    double* O = P + 9 * kPairChunk;          // 9 planes of kOffChunk: off-diagonal blocks
+    double* D = O + 9 * kOffChunk;           // kRowChunk diagonal blocks (row-major)
+    double* RM = D + 9 * kRowChunk;          // kRowChunk row masses
+    int* RG = reinterpret_cast<int*>(RM + kRowChunk);  // kRowChunk bodies (-1 - body: no preconditioner)
+    int* SR = RG + kRowChunk;                // kSlotChunk run descriptors
+    int* SE = SR + kSlotChunk;               // kSlotChunk elements
+    const int c = xcd_block();
+    if (c >= nchunks)
+        return;
+    const int tid = static_cast<int>(threadIdx.x);
+    const int4 ch = chunks[c];
+    const int r0 = ch.x, pBase = ch.y, sBase = ch.z;
+    const int nR = ch.w & 0xff, nS = (ch.w >> 8) & 0xff, pEnd = pBase + (ch.w >> 16);
+    const size_t F = static_cast<size_t>(nnzf);
+
+    // phase 1: the lookups of phases 2 and 3
+    for (int j = tid; j < nS; j += NT)
+    {
+        SR[j] = slotRun[sBase + j];
+        SE[j] = uElem[sBase + j];
+    }
+    if (tid < nR)
+    {
+        const int g = rowGroup[r0 + tid];
+        RM[tid] = mrow[r0 + tid];
+        RG[tid] = gs[g].jacobi ? g : -1 - g;
+    }
+    // phase 1: the pairs' blocks
+    const int p = pBase + tid;
+    if (p < pEnd)
+    {
+        const int cc = nlist[p];
+        const int t = cc >> 2, a = cc & 3;
+        const int offs = pairOff[p];
+        double w[12], rr[6];
+        const double2* rec2 = reinterpret_cast<const double2*>(basis + static_cast<size_t>(t) * EB_COUNT);
+        const double* kdg = kdat + static_cast<size_t>(t) * KD_COUNT;
+        const double2 lm = *reinterpret_cast<const double2*>(kdg + KD_VLAM);  // V lambda, V mu
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+        {
+            const double2 u = rec2[i];
+            w[2 * i] = u.x;
+            w[2 * i + 1] = u.y;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            const double2 u = rec2[EB_RRT / 2 + i];
+            rr[2 * i] = u.x;
+            rr[2 * i + 1] = u.y;
+        }
+        const double RRt[9] = {rr[0], rr[1], rr[2], rr[1], rr[3], rr[4], rr[2], rr[4], rr[5]};
+        double wa[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            wa[i] = pick4(a, w[i], w[3 + i], w[6 + i], w[9 + i]);
+        double K[9];
+        rotated_block_scaled(wa, wa, RRt, lm.x, lm.y, vmg_pick(kdg, a, a), K);
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            P[e * kPairChunk + tid] = K[e];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            const int o = (offs >> (10 * j)) & 0x3ff;
+            if (o == 0x3ff)
+                continue;
+            const int b = j < a ? j : j + 1;  // the other corners in increasing order
+            double wb[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                wb[i] = pick4(b, w[i], w[3 + i], w[6 + i], w[9 + i]);
+            // the stored block of the local pair (min, max), transposed when
+            // the row's corner is the larger one
+            const bool flip = a > b;
+            double lo[3], hi[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                lo[i] = flip ? wb[i] : wa[i];
+                hi[i] = flip ? wa[i] : wb[i];
+            }
+            rotated_block_scaled(lo, hi, RRt, lm.x, lm.y, vmg_pick(kdg, a, b), K);
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+#pragma unroll
+                for (int y = 0; y < 3; ++y)
+                    O[(3 * x + y) * kOffChunk + o] = flip ? K[3 * y + x] : K[3 * x + y];
+        }
+    }
+    __syncthreads();
+
+    // phase 2
+    for (int it = tid; it < 9 * nS; it += NT)
+    {
+        const int e = it / nS;
+        const int sl = it - e * nS;
+        const int run = SR[sl];
+        const int begin = run & 0x3ff, n = (run >> 10) & 0x3ff, row = (run >> 20) & 0x3f;
+        const bool diag = (run >> 26) & 1;
+        const double* src = (diag ? P + e * kPairChunk : O + e * kOffChunk) + begin;
+        // in list order; eight LDS loads in flight ahead of the adds
+        double v = 0.0;
+        int j = 0;
+        for (; j + 8 <= n; j += 8)
+        {
+            double u[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                u[k] = src[j + k];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                v += u[k];
+        }
+        if (j < n)
+        {
+            double u[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                u[k] = j + k < n ? src[j + k] : 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (j + k < n)
+                    v += u[k];
+        }
+        const bool onDiagonal = e == 0 || e == 4 || e == 8;
+        if (cslot && onDiagonal)
+            v += cslot[sBase + sl];  // point connections' and contacts' k w_a w_b I (PointConnection.cpp:110-128)
+        v *= dt2;
+        if (diag && onDiagonal)
+            v = RM[row] + v;
+        const int full = SE[sl];
+        vals[upperOnly ? SliceStore<M>::word(F, full, e) : BsrStore<M>::at(F, full, e)] = static_cast<M>(v);
+        if (!upperOnly && !diag)
+        {
+            const int x = e / 3, y = e - 3 * (e / 3);
+            vals[BsrStore<M>::at(F, uMirror[sBase + sl], 3 * y + x)] = static_cast<M>(v);
+        }
+        if (!fin(v))
+        {
+            const int g = RG[row];
+            gs[g >= 0 ? g : -1 - g].failed = 1;  // a non-finite block rejects the assembly
+        }
+        if (diag)
+            D[9 * row + e] = v;
+    }
+    __syncthreads();
+
+    // phase 3
+    if (tid < nR)
+    {
+        double A[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            A[e] = D[9 * tid + e];
+        const int gr = RG[tid];
+        double mi[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        if (gr >= 0)
+        {
+            bool pok = fin9(A);
+            if (pok)
+            {
+                const double d = det3(A);
+                pok = fin(d) && fabs(d) > static_cast<double>(1.0e-8f);
+            }
+            if (pok)
+            {
+                inv3(A, mi);
+                pok = fin9(mi);
+            }
+            if (!pok)
+                gs[gr].failed = 1;  // BuildInversePreconditioner fails the solve
+        }
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            minv[static_cast<size_t>(e) * nd + r0 + tid] = static_cast<M>(mi[e]);
+    }
+}
+
 // Lanes per dynamic row in the right-hand side gather (a node has ~20 corners).
 constexpr int kRowLanes = 4;
 
@@ -809,6 +1037,15 @@ void launch_assemble(const DeviceView& v, double dt)
     const int n = static_cast<int>(v.nnzu);
     if (n <= 0 || v.nd <= 0)
         return;
+    if (v.nchunks > 0)
+    {
+        hipLaunchKernelGGL((k_assemble_pairs<M>), dim3(xcd_grid(v.nchunks)), dim3(NT), kPairLds, v.stream,
+                           v.nchunks, v.chunks, v.nd, v.nlist, v.pairOff, v.slotRun,
+                           v.upper ? v.slotEll : v.uFull, v.uMirror, v.rowGroup, v.basis, v.kdat, v.mrow, dt * dt,
+                           static_cast<M*>(v.vals), v.upper ? v.nE : v.nnzf, static_cast<M*>(v.minv), v.gs, v.upper,
+                           v.slotAdd);
+        return;
+    }
     const long long groups = (static_cast<long long>(v.nd) * kRowGatherLanes + NT - 1) / NT;
     // upper-only storage: slots map to their elements (of nE)
     hipLaunchKernelGGL((k_assemble<M>), dim3(xcd_grid(groups)), dim3(NT), 0, v.stream, n, v.nd, v.cstart, v.clist,

@@ -91,6 +91,25 @@ struct TileView
     const RowTile* perm = nullptr;  // compacted: tile of each workgroup (8 * per entries)
 };
 
+// Limits of one chunk of the assembly gather by (row, tet) pairs: corner-
+// list pairs (one lane each), off-diagonal contributions, rows and upper
+// slots; its LDS (kPairLds bytes, 42.9 KB: three workgroups per CU) holds the blocks
+// (9 doubles each), the rows' diagonal blocks, masses and bodies, and the
+// slots' run descriptors and elements.
+constexpr int kPairChunk = NT;
+constexpr int kOffChunk = 256;
+constexpr int kRowChunk = 48;
+constexpr int kSlotChunk = 224;
+constexpr size_t kPairLds = 8 * 9 * static_cast<size_t>(kPairChunk + kOffChunk + kRowChunk) +
+                            8 * static_cast<size_t>(kSlotChunk) + 16 * static_cast<size_t>(kRowChunk);
+// Run descriptor of an upper slot in its chunk: where its contributions lie
+// in the chunk's block planes (the diagonal ones: the row's pairs), how many,
+// the row within the chunk and whether the slot is the row's diagonal.
+__host__ __device__ inline int pack_run(int begin, int len, int row, bool diag)
+{
+    return begin | (len << 10) | (row << 20) | (diag ? 1 << 26 : 0);
+}
+
 // Largest group count the compaction handles (its per-group offsets live in
 // LDS); larger batches run uncompacted.
 constexpr int kCompactMaxGroups = 4096;
@@ -402,6 +421,14 @@ struct DeviceView
     const int* ucol;
     const int* nstart;
     const int* nlist;
+    // the assembly gather by (row, tet) pairs: each chunk's rows, pairs and
+    // slots (nchunks 0: the gather by rows, k_assemble), each upper slot's run in
+    // its chunk, and per nlist pair the chunk-relative places of its
+    // off-diagonal contributions
+    const int4* chunks;  // nchunks: first row, first pair, first upper slot, rows | slots << 8 | pairs << 16
+    int nchunks;
+    const int* slotRun;  // nnzu run descriptors (pack_run)
+    const int* pairOff;
     const int* rowNode;
     const int* astart;     // node -> (tet, corner) over live tets
     const int* alist;

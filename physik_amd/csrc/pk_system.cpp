@@ -132,6 +132,9 @@ struct pk_system
     pk::DevBuf<double> prm, kdat, x, v, basis, corner, adj, mrow, xs, r, pa;
     pk::DevBuf<char> minv;  // storage type, like the CG vectors
     pk::DevBuf<int> cstart, clist, uFull, uMirror, slotRow, rowGroup, fstart, fcol, nstart, nlist, rowNode, active;
+    pk::DevBuf<int> slotRun, pairOff;  // the assembly gather by (row, tet) pairs
+    pk::DevBuf<int4> gatherChunks;
+    int nchunks = 0;
     pk::DevBuf<int> astart, alist, apos, nodeRow, nodeGroup, cpos, diagSlot;
     // sliced upper-only storage: slice descriptors, lane masks, lower ELL
     // entries, the element of each upper slot, each row's (slice, lane) and
@@ -319,6 +322,10 @@ struct pk_system
         dv.fcol = fcol.p;
         dv.nstart = nstart.p;
         dv.nlist = nlist.p;
+        dv.chunks = gatherChunks.p;
+        dv.nchunks = nchunks;
+        dv.slotRun = slotRun.p;
+        dv.pairOff = pairOff.p;
         dv.rowNode = rowNode.p;
         dv.astart = astart.p;
         dv.alist = alist.p;
@@ -1250,6 +1257,112 @@ static int setup_tiles(pk_system* s)
     return PK_OK;
 }
 
+// The assembly gather by (row, tet) pairs (k_assemble_pairs): consecutive
+// rows grouped into chunks of at most kPairChunk corner-list pairs,
+// kOffChunk off-diagonal contributions, kRowChunk rows and kSlotChunk upper
+// slots; each upper slot's run in its chunk (pack_run: the diagonal slot's
+// contributions are the row's pairs, an off-diagonal slot's lie in the
+// chunk's run of the contribution lists with the diagonal slots left out);
+// and per pair the chunk-relative places of its contributions to the row's
+// upper off-diagonal slots, one 10-bit field per other corner of the tet in
+// increasing order (0x3ff: none). A row too large for a chunk leaves
+// nchunks 0, and the assembly gathers by rows (k_assemble).
+static int build_pair_gather(pk_system* s, const std::vector<int>& diagSlot)
+{
+    const Topology& t = s->topo;
+    const int nd = t.nd;
+    const long long nnzu = t.nnzu();
+    s->nchunks = 0;
+    if (nd == 0 || nnzu == 0)
+        return PK_OK;
+    // off-diagonal contributions before each upper slot
+    std::vector<int> ostart(static_cast<size_t>(nnzu) + 1, 0);
+    for (int r = 0; r < nd; ++r)
+        for (int q = t.ustart[r]; q < t.ustart[r + 1]; ++q)
+            ostart[q + 1] = ostart[q] + (q == diagSlot[r] ? 0 : t.cstart[q + 1] - t.cstart[q]);
+    std::vector<int> chunkRow(1, 0);
+    int cp = 0, co = 0, cr = 0, cs = 0;
+    for (int r = 0; r < nd; ++r)
+    {
+        const int np = t.nstart[r + 1] - t.nstart[r];
+        const int no = ostart[t.ustart[r + 1]] - ostart[t.ustart[r]];
+        const int ns = t.ustart[r + 1] - t.ustart[r];
+        if (np > kPairChunk || no > kOffChunk || ns > kSlotChunk)
+            return PK_OK;  // a row too large: gather by rows
+        if (cp + np > kPairChunk || co + no > kOffChunk || cr + 1 > kRowChunk || cs + ns > kSlotChunk)
+        {
+            chunkRow.push_back(r);
+            cp = co = cr = cs = 0;
+        }
+        cp += np;
+        co += no;
+        ++cr;
+        cs += ns;
+    }
+    chunkRow.push_back(nd);
+    std::vector<int> pairOff(std::max<size_t>(1, t.nlist.size()), 0x3fffffff);
+    std::vector<int> slotRun(static_cast<size_t>(nnzu), 0);
+    for (size_t c = 0; c + 1 < chunkRow.size(); ++c)
+    {
+        const int r0 = chunkRow[c], r1 = chunkRow[c + 1];
+        const int oBase = ostart[t.ustart[r0]], pBase = t.nstart[r0];
+        for (int r = r0; r < r1; ++r)
+        {
+            const int* nb = t.nlist.data() + t.nstart[r];
+            const int* ne = t.nlist.data() + t.nstart[r + 1];
+            for (int q = t.ustart[r]; q < t.ustart[r + 1]; ++q)
+            {
+                if (q == diagSlot[r])
+                {
+                    slotRun[q] = pack_run(t.nstart[r] - pBase, t.nstart[r + 1] - t.nstart[r], r - r0, true);
+                    continue;
+                }
+                slotRun[q] = pack_run(ostart[q] - oBase, ostart[q + 1] - ostart[q], r - r0, false);
+                const int cc = t.ucol[q];
+                for (int k = t.cstart[q]; k < t.cstart[q + 1]; ++k)
+                {
+                    const int tet = t.clist[k] >> 5;
+                    // the row's pair of this tet (the row list is in tet order)
+                    const int* it = std::lower_bound(nb, ne, tet << 2);
+                    if (it == ne || (*it >> 2) != tet)
+                    {
+                        set_error("pk_system_create: the contribution lists disagree with the row lists");
+                        return PK_ERR_ARG;
+                    }
+                    const int a = *it & 3;
+                    int b = -1;
+                    for (int x = 0; x < 4; ++x)
+                        if (t.dyn[t.tets[4 * static_cast<size_t>(tet) + x]] == cc)
+                            b = x;
+                    if (b < 0 || b == a)
+                    {
+                        set_error("pk_system_create: a contribution without its corner");
+                        return PK_ERR_ARG;
+                    }
+                    const int j = b < a ? b : b - 1;
+                    const int o = ostart[q] + (k - t.cstart[q]) - oBase;
+                    int& w = pairOff[static_cast<size_t>(it - t.nlist.data())];
+                    w = (w & ~(0x3ff << (10 * j))) | (o << (10 * j));
+                }
+            }
+        }
+    }
+    // each chunk: first row, first pair, first upper slot, rows | slots << 8 | pairs << 16
+    std::vector<int4> chunks(chunkRow.size() - 1);
+    for (size_t c = 0; c + 1 < chunkRow.size(); ++c)
+    {
+        const int r0 = chunkRow[c], r1 = chunkRow[c + 1];
+        const int np = t.nstart[r1] - t.nstart[r0], ns = t.ustart[r1] - t.ustart[r0];
+        chunks[c] = make_int4(r0, t.nstart[r0], t.ustart[r0], (r1 - r0) | (ns << 8) | (np << 16));
+    }
+    int rc;
+    if ((rc = s->slotRun.upload(slotRun, s->stream)) != PK_OK) return rc;
+    if ((rc = s->pairOff.upload(pairOff, s->stream)) != PK_OK) return rc;
+    if ((rc = s->gatherChunks.upload(chunks, s->stream)) != PK_OK) return rc;
+    s->nchunks = static_cast<int>(chunks.size());
+    return PK_OK;
+}
+
 static int setup_device(pk_system* s)
 {
     const Topology& t = s->topo;
@@ -1304,6 +1417,7 @@ static int setup_device(pk_system* s)
         diagSlot[r] = t.ustart[r];
     }
     if ((rc = s->diagSlot.upload(diagSlot, st)) != PK_OK) return rc;
+    if ((rc = build_pair_gather(s, diagSlot)) != PK_OK) return rc;
     if (!is_upper(s))
     {
         std::vector<int> fdiag(static_cast<size_t>(t.nd));
