@@ -937,8 +937,14 @@ __global__ __launch_bounds__(NT) void k_stamp_reduce(const unsigned long long* _
     unsigned long long lo = ~0ull, hi = 0ull;
     for (int i = threadIdx.x; i < grid; i += NT)
     {
-        lo = min(lo, p[2 * i]);
-        hi = max(hi, p[2 * i + 1]);
+        // a zero slot belongs to a workgroup the launch did not have (its
+        // grid was cut to the tiles still iterating)
+        const unsigned long long b = p[2 * i];
+        if (b != 0ull)
+        {
+            lo = min(lo, b);
+            hi = max(hi, p[2 * i + 1]);
+        }
     }
     __shared__ unsigned long long sl[NT], sh[NT];
     sl[threadIdx.x] = lo;
@@ -1129,7 +1135,7 @@ void launch_advance(const DeviceView& v, int* kb, int by)
 // for the SpMV tiling.
 __global__ __launch_bounds__(NT) void k_compact(int G, const GroupState* __restrict__ gs, TileView vt, TileView st,
                                                 int vecBlocks, RowTile* __restrict__ vecPerm,
-                                                RowTile* __restrict__ spmvPerm)
+                                                RowTile* __restrict__ spmvPerm, int* __restrict__ counts)
 {
     __shared__ int offV[kCompactMaxGroups], offS[kCompactMaxGroups];
     __shared__ int scanV[NT], scanS[NT];
@@ -1169,6 +1175,13 @@ __global__ __launch_bounds__(NT) void k_compact(int G, const GroupState* __restr
         }
         __syncthreads();
     }
+    if (blockIdx.x == 0 && tid == 0)
+    {
+        // tiles still iterating: later launches need no more than 8 *
+        // ceil(count / 8) workgroups
+        counts[0] = carryV;
+        counts[1] = carryS;
+    }
     const bool vec = static_cast<int>(blockIdx.x) < vecBlocks;
     const TileView& tv = vec ? vt : st;
     RowTile* perm = vec ? vecPerm : spmvPerm;
@@ -1198,7 +1211,7 @@ void launch_compact(const DeviceView& v, RowTile* vecPerm, RowTile* spmvPerm)
     const int sb = (8 * v.spmv.per + NT - 1) / NT;
     if (v.G > 0 && v.G <= kCompactMaxGroups && vb + sb > 0)
         hipLaunchKernelGGL(k_compact, dim3(static_cast<unsigned>(vb + sb)), dim3(NT), 0, v.stream, v.G, v.gs, v.vec,
-                           v.spmv, vb, vecPerm, spmvPerm);
+                           v.spmv, vb, vecPerm, spmvPerm, v.active + 1);
 }
 
 int spmv_grid(const DeviceView& v) { return 8 * v.spmv.per; }
@@ -1210,6 +1223,20 @@ void launch_stamp_reduce(hipStream_t stream, const unsigned long long* raw, int 
     if (launches > 0 && grid > 0)
         hipLaunchKernelGGL(k_stamp_reduce, dim3(static_cast<unsigned>(launches)), dim3(NT), 0, stream, raw, grid,
                            span);
+}
+
+__global__ __launch_bounds__(NT) void k_zero_u64(unsigned long long* __restrict__ p, long long n)
+{
+    for (long long i = static_cast<long long>(blockIdx.x) * NT + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * NT)
+        p[i] = 0ull;
+}
+
+void launch_zero_stamps(hipStream_t stream, unsigned long long* raw, long long n)
+{
+    if (raw && n > 0)
+        hipLaunchKernelGGL(k_zero_u64, dim3(static_cast<unsigned>(std::min<long long>((n + NT - 1) / NT, 4096))),
+                           dim3(NT), 0, stream, raw, n);
 }
 
 template <class M>

@@ -451,7 +451,7 @@ struct DeviceView
     void* d[4];   // M, d_k at index k & 3
     void* q;      // M
     GroupState* gs;
-    int* active;
+    int* active;  // bodies still iterating; active[1], active[2]: vector and SpMV tiles of the last compaction
     int* kbase;        // base iteration of a replayed CG chunk (zeroed by the reset)
     // point connections (none when nconn == 0)
     int nconn;
@@ -558,6 +558,8 @@ int update_grid(const DeviceView& v);
 void launch_compact(const DeviceView& v, RowTile* vecPerm, RowTile* spmvPerm);
 void launch_stamp_reduce(hipStream_t stream, const unsigned long long* raw, int grid, int launches,
                          unsigned long long* span);
+// Zeroes n stamp words (a workgroup that does not run leaves its slots zero).
+void launch_zero_stamps(hipStream_t stream, unsigned long long* raw, long long n);
 template <class M> void launch_spmv_plain(const DeviceView& v, const double* x, double* y);
 // Record storage (FP32, or upper-only in either precision): writes each
 // block's column into word 9 of its record (one record per full block, or

@@ -706,6 +706,11 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
             if ((rc = s->stampSpmv.alloc(2 * n)) != PK_OK) return rc;
             if ((rc = s->stampUpd.alloc(2 * n)) != PK_OK) return rc;
             if ((rc = s->stampDir.alloc(2 * n)) != PK_OK) return rc;
+            // slots a launch does not write stay zero (launches cut to the
+            // tiles still iterating have fewer workgroups)
+            launch_zero_stamps(s->stream, s->rawSpmv.p, static_cast<long long>(s->rawSpmv.n));
+            launch_zero_stamps(s->stream, s->rawUpd.p, static_cast<long long>(s->rawUpd.n));
+            launch_zero_stamps(s->stream, s->rawDir.p, static_cast<long long>(s->rawDir.n));
         }
         stS = s->rawSpmv.p;
         stU = s->rawUpd.p;
@@ -737,8 +742,12 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
     // that decision, and launches past it find every body stopped; a batch
     // switches to the compacted tile maps at the first compaction (every
     // kChunk iterations)
+    // after a compaction, a poll also brings the count of tiles still
+    // iterating, and later launches cover only those (the count only falls,
+    // so a count one chunk old is enough)
     DeviceView cv = dv;
     int k = 0, poll = 0;
+    bool compacted = false, pollCompacted[2] = {false, false};
     int chunk = std::max(kChunk, lastIters - 4);
     while (k <= maxIt)
     {
@@ -757,6 +766,7 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
                     launch_compact(dv, s->vecPerm.p, s->spmvPerm.p);
                     cv.vec.perm = s->vecPerm.p;
                     cv.spmv.perm = s->spmvPerm.p;
+                    compacted = true;
                 }
                 launch_dir<M>(cv, nullptr, k, stD ? stD + 2 * static_cast<size_t>(gU) * k : nullptr);
                 if (k < maxIt)
@@ -768,13 +778,21 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
         }
         chunk = kChunk;
         const int slot = poll & 1;
-        PK_HIP(hipMemcpyAsync(s->pinned + slot, s->active.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        PK_HIP(hipMemcpyAsync(s->pinned + 4 * slot, s->active.p, 3 * sizeof(int), hipMemcpyDeviceToHost,
+                              s->stream));
+        pollCompacted[slot] = compacted;
         PK_HIP(hipEventRecord(s->pollEv[slot], s->stream));
         if (poll > 0)
         {
             PK_HIP(hipEventSynchronize(s->pollEv[slot ^ 1]));
-            if (s->pinned[slot ^ 1] == 0)
+            const int* pv = s->pinned + 4 * (slot ^ 1);
+            if (pv[0] == 0)
                 break;
+            if (pollCompacted[slot ^ 1] && !s->useGraphs)
+            {
+                cv.vec.per = std::max(1, std::min(cv.vec.per, (pv[1] + 7) / 8));
+                cv.spmv.per = std::max(1, std::min(cv.spmv.per, (pv[2] + 7) / 8));
+            }
         }
         ++poll;
     }
@@ -785,6 +803,10 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
         launch_stamp_reduce(s->stream, stD, gU, k, s->stampDir.p);
         launch_stamp_reduce(s->stream, stS, gS, nS, s->stampSpmv.p);
         launch_stamp_reduce(s->stream, stU, gU, nS, s->stampUpd.p);
+        const long long used = 2ll * k;
+        launch_zero_stamps(s->stream, stS, used * gS);
+        launch_zero_stamps(s->stream, stU, used * gU);
+        launch_zero_stamps(s->stream, stD, used * gU);
     }
     PK_HIP(hipGetLastError());
     return PK_OK;
@@ -1155,7 +1177,7 @@ static int alloc_work(pk_system* s)
     if ((rc = s->minv.alloc(static_cast<size_t>(t.nd) * 9 * es)) != PK_OK) return rc;
     if ((rc = s->xs.alloc(nd3)) != PK_OK) return rc;
     if ((rc = s->r.alloc(nd3)) != PK_OK) return rc;
-    if ((rc = s->active.alloc(1)) != PK_OK) return rc;
+    if ((rc = s->active.alloc(4)) != PK_OK) return rc;
     if ((rc = s->kbase.alloc(1)) != PK_OK) return rc;
     ++s->epoch;
     return PK_OK;
@@ -1503,7 +1525,7 @@ int pk_system_create(const pk_mesh_desc* desc, int precision, pk_system** out)
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess)
             s->wallClockKHz = khz;
     }
-    PK_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->pinned), 2 * sizeof(int), hipHostMallocDefault));
+    PK_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->pinned), 8 * sizeof(int), hipHostMallocDefault));
     PK_HIP(hipEventCreateWithFlags(&s->pollEv[0], hipEventDisableTiming));
     PK_HIP(hipEventCreateWithFlags(&s->pollEv[1], hipEventDisableTiming));
     if ((rc = setup_device(s.get())) != PK_OK)
