@@ -9,7 +9,8 @@ rather than guessed:
 
   stage 1  lanes per row, with (128 rows, unroll 2) (upper-only storage:
            one lane per row, no search);
-  stage 2  rows per workgroup x unroll, with the best lane count;
+  stage 2  rows per workgroup x unroll, with the best lane count (of shapes
+           within 1 % of the fastest, the one with the fewest rows);
   stage 3  rows per vector-kernel workgroup: the grid of the direction and
            update kernels, whose every workgroup first re-sums the other
            kernel's partials; this stage minimises SpMV + direction + update.
@@ -93,7 +94,13 @@ def autotune(system, dt, cache=None, verbose=False):
     best_l = 1 if upper else min(LANES, key=lambda lanes: trial((lanes, 128, vec0, 2))[0])
     # upper-only storage does not use unroll
     cands = [(best_l, r, vec0, u) for r in ROWS for u in ((1,) if upper else UNROLL) if _valid(best_l, u)]
-    best = min(cands, key=lambda s: trial(s)[0])
+    times = {s: trial(s)[0] for s in cands}
+    fastest = min(times.values())
+    # near-ties (within 1 %) go to the smaller tile: each SpMV tile re-reads the
+    # band of the direction vector around its rows, so smaller tiles that run
+    # as fast move no more bytes (batch128 FP32 upper: 256-row tiles read
+    # 1.03x the algorithmic bytes, 512-row tiles 1.29x, in the same time)
+    best = min((s for s, t in times.items() if t <= 1.01 * fastest), key=lambda s: (s[1], times[s]))
     best = min(((best[0], best[1], vr, best[3]) for vr in VEC_ROWS), key=lambda s: sum(trial(s)))
     system.tune(*best)
     system.set_state(x, v)

@@ -28,7 +28,9 @@ import os
 
 from .roofline import assembly_bytes, spmv_bytes
 
-ASSEMBLY_KERNELS = {"element": "k_element", "assemble": "k_assemble", "rhs": "k_rhs"}
+# one assembly's kernels: the slot gather is k_assemble_pairs (by row, tet
+# pairs) or, for a system with a row larger than a chunk, k_assemble
+ASSEMBLY_KERNELS = {"element": ("k_element",), "assemble": ("k_assemble_pairs", "k_assemble"), "rhs": ("k_rhs",)}
 
 
 def _csv_rows(directory, suffix):
@@ -106,8 +108,8 @@ def traffic(args):
         if "spmv" in _short(name):
             alg = spmv_alg
             spmv = (fm, wm)
-        for k, stem in ASSEMBLY_KERNELS.items():
-            if stem == _short(name).split("::")[-1].split("<")[0]:
+        for k, stems in ASSEMBLY_KERNELS.items():
+            if _short(name).split("::")[-1].split("<")[0] in stems:
                 asm[k] = fm + wm
         table.append((_short(name), len(fv), note, fm, wm, alg))
     lines = [f"# HBM traffic per dispatch: {args.key}", "",
