@@ -199,6 +199,7 @@ struct pk_system
     bool tapeValid = false;
     long long tapeId = 0;  // counts the recorded runs of this system
     int lastItersAdj = 0;
+    int firstStopAdj = 0;  // the previous adjoint solve's earliest stop among its bodies
 
     double gravity[3] = {0, 0, 0};
     double tol = 1e-4;
@@ -206,6 +207,7 @@ struct pk_system
     int jacobi = 1;
     int implicit = 1;
     int lastIters = 0;
+    int firstStop = 0;  // the previous forward solve's earliest stop among its bodies
     std::vector<pk::GroupState> hostGs;
     std::vector<long long> groupBlocks, groupUpper, groupRows;  // full and upper BSR blocks, dynamic rows per group
     pk::Profile prof;
@@ -678,12 +680,14 @@ static int capture_chunk(pk_system* s, const DeviceView& dv, bool compacted, uns
 }
 
 // The CG launches up to maxIt; lastIters (the previous solve's count of the
-// same kind) sizes the first run of launches before the host starts polling.
+// same kind) sizes the first run of launches before the host starts polling,
+// and firstStop (its earliest stop among the bodies) tells when compactions
+// can start to pay.
 // Launched one by one, or (useGraphs) replayed kChunk iterations at a time
 // from captured graphs: the same kernels in the same order either way, so the
 // results are identical.
 template <class M>
-static int run_cg(pk_system* s, int maxItArg, int lastIters)
+static int run_cg(pk_system* s, int maxItArg, int lastIters, int firstStop)
 {
     const DeviceView dv = s->view();
     const int maxIt = std::max(0, maxItArg);
@@ -745,9 +749,15 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
     // after a compaction, a poll also brings the count of tiles still
     // iterating, and later launches cover only those (the count only falls,
     // so a count one chunk old is enough)
+    // A compaction (every kCompactEvery launches) only pays once a body has
+    // stopped: it runs when a poll has shown fewer bodies iterating than the
+    // last map holds or, before the first poll, from a few launches before
+    // the previous solve's earliest stop. (It only changes which workgroups
+    // hold which tiles, never a result.)
     DeviceView cv = dv;
     int k = 0, poll = 0;
     bool compacted = false, pollCompacted[2] = {false, false};
+    int knownActive = -1, mapActive = INT_MAX;  // last polled count (-1: none yet); bodies in the last map
     int chunk = std::max(kChunk, lastIters - 4);
     while (k <= maxIt)
     {
@@ -761,12 +771,15 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
         {
             for (; k < end; ++k)
             {
-                if (compact && k > 0 && k % kCompactEvery == 0)
+                const bool due = knownActive >= 0 ? knownActive < mapActive : k + kCompactEvery >= firstStop;
+                if (compact && k > 0 && k % kCompactEvery == 0 && due)
                 {
                     launch_compact(dv, s->vecPerm.p, s->spmvPerm.p);
                     cv.vec.perm = s->vecPerm.p;
                     cv.spmv.perm = s->spmvPerm.p;
                     compacted = true;
+                    if (knownActive >= 0)
+                        mapActive = knownActive;
                 }
                 launch_dir<M>(cv, nullptr, k, stD ? stD + 2 * static_cast<size_t>(gU) * k : nullptr);
                 if (k < maxIt)
@@ -788,6 +801,7 @@ static int run_cg(pk_system* s, int maxItArg, int lastIters)
             const int* pv = s->pinned + 4 * (slot ^ 1);
             if (pv[0] == 0)
                 break;
+            knownActive = pv[0];
             if (pollCompacted[slot ^ 1] && !s->useGraphs)
             {
                 cv.vec.per = std::max(1, std::min(cv.vec.per, (pv[1] + 7) / 8));
@@ -960,7 +974,7 @@ static int run_substep(pk_system* s, double dt, int* failedOut = nullptr)
         launch_save_failed(dv, failedOut);
     if (s->implicit)
     {
-        if ((rc = run_cg<M>(s, s->maxIt, s->lastIters)) != PK_OK)
+        if ((rc = run_cg<M>(s, s->maxIt, s->lastIters, s->firstStop)) != PK_OK)
             return rc;
         Bracket br(s, PK_K_INTEGRATE);
         launch_integrate(dv, dt, 0);
@@ -977,18 +991,25 @@ static int run_substep(pk_system* s, double dt, int* failedOut = nullptr)
 }
 
 // Copies the per-body CG state to the host; `lastIters` receives the largest
-// iteration count (it sizes the next solve of the same kind).
-static int read_groups(pk_system* s, int& lastIters)
+// iteration count (it sizes the next solve of the same kind) and `firstStop`
+// the smallest among the bodies with rows (when the next solve's compactions
+// start to pay).
+static int read_groups(pk_system* s, int& lastIters, int& firstStop)
 {
     if (s->hostGs.empty())
         return PK_OK;
     PK_HIP(hipMemcpyAsync(s->hostGs.data(), s->gs.p, s->hostGs.size() * sizeof(GroupState), hipMemcpyDeviceToHost,
                           s->stream));
     PK_HIP(hipStreamSynchronize(s->stream));
-    int mx = 0;
+    int mx = 0, mn = INT_MAX;
     for (const auto& g : s->hostGs)
+    {
         mx = std::max(mx, g.iters);
+        if (g.hasRows)
+            mn = std::min(mn, g.iters);
+    }
     lastIters = mx;
+    firstStop = mn == INT_MAX ? 0 : mn;
     return PK_OK;
 }
 
@@ -1873,7 +1894,7 @@ int pk_system_step(pk_system* s, double dt, int substeps, int frames)
             if (s->implicit && s->topo.nd > 0)
             {
                 // the outcome sizes the next solve's first chunk
-                const int rc2 = read_groups(s, s->lastIters);
+                const int rc2 = read_groups(s, s->lastIters, s->firstStop);
                 if (rc2 != PK_OK)
                     return rc2;
                 harvest_profile(s);
@@ -1908,7 +1929,7 @@ static int replay_substep(pk_system* s, double dt, int* ok, int* failedOut = nul
         return rc;
     launch_record_ok(s->view(), ok);
     PK_HIP(hipGetLastError());
-    if ((rc = read_groups(s, s->lastIters)) != PK_OK)
+    if ((rc = read_groups(s, s->lastIters, s->firstStop)) != PK_OK)
         return rc;
     harvest_profile(s);
     return PK_OK;
@@ -1934,13 +1955,13 @@ static int reverse_substep(pk_system* s, double dt, int j, double adjTol, int ad
         return rc;
     const DeviceView dv = s->view();
     launch_adj_rhs<M>(dv, ok, s->xbar.p, s->vbar.p, dt);
-    if ((rc = run_cg<M>(s, adjMaxIt, s->lastItersAdj)) != PK_OK)
+    if ((rc = run_cg<M>(s, adjMaxIt, s->lastItersAdj, s->firstStopAdj)) != PK_OK)
         return rc;
     launch_adj_element(dv, ok, s->segV.p + (j + 1) * n3, dt, s->gradE.p);
     launch_adj_contacts(dv, dt);
     launch_adj_nodes(dv, ok, dt, s->xbar.p, s->vbar.p);
     PK_HIP(hipGetLastError());
-    if ((rc = read_groups(s, s->lastItersAdj)) != PK_OK)
+    if ((rc = read_groups(s, s->lastItersAdj, s->firstStopAdj)) != PK_OK)
         return rc;
     harvest_profile(s);
     if (iters)
@@ -2700,8 +2721,8 @@ int pk_system_assemble(pk_system* s, double dt)
     if (rc != PK_OK)
         return rc;
     PK_HIP(hipStreamSynchronize(s->stream));
-    int unused = 0;
-    const int rc2 = read_groups(s, unused);
+    int unused = 0, unusedFirst = 0;
+    const int rc2 = read_groups(s, unused, unusedFirst);
     s->prof.recs.clear();
     s->prof.used = 0;
     return rc2;
