@@ -93,11 +93,12 @@ struct TileView
 
 // Limits of one chunk of the assembly gather by (row, tet) pairs: corner-
 // list pairs (one lane each), off-diagonal contributions, rows and upper
-// slots; its LDS (kPairLds bytes, 42.9 KB: three workgroups per CU) holds the blocks
-// (9 doubles each), the rows' diagonal blocks, masses and bodies, and the
-// slots' run descriptors and elements.
+// slots; its LDS (kPairLds bytes, 52.1 KB: three workgroups per CU; 54 KB
+// admits two) holds the blocks (9 doubles each), the rows' diagonal blocks,
+// masses and bodies, and the slots' run descriptors and elements. A tet
+// gives 4 pairs and 6 off-diagonal contributions, so 256 pairs go with 384.
 constexpr int kPairChunk = NT;
-constexpr int kOffChunk = 256;
+constexpr int kOffChunk = 384;
 constexpr int kRowChunk = 48;
 constexpr int kSlotChunk = 224;
 constexpr size_t kPairLds = 8 * 9 * static_cast<size_t>(kPairChunk + kOffChunk + kRowChunk) +

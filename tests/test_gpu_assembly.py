@@ -2,7 +2,7 @@
 per node, against the FP64 oracle.
 
 The gather runs by (row, tet) pairs in chunks of consecutive rows (at most
-256 pairs, 384 off-diagonal contributions, 64 rows and 288 upper slots per
+256 pairs, 384 off-diagonal contributions, 48 rows and 224 upper slots per
 chunk); a system with a row larger than a chunk gathers by rows instead.
 A fan -- a ring of n nodes around a centre node, with an apex above and one
 below -- puts 2n tets and up to 6n off-diagonal contributions on the centre
