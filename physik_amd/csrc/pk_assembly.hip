@@ -7,13 +7,17 @@
 //                    dt^2 (K v)_a (or f_a explicit), written in row-list order
 //                    (FEMModel.cpp:614-679, :377-424, :559-612, :115-163)
 //   k_stiff_records  one lane per tet: V lambda, V mu, V mu g_a.g_b
-//   k_assemble       16 lanes per dynamic row: the diagonal block over the
-//                    row's corner list, then its off-diagonal upper slots four
-//                    at a time, each block rebuilt from the basis and
-//                    stiffness records (lane-strided partial sums combined by
-//                    shuffles: a fixed order, no atomics); A = M + dt^2 K
-//                    into full or sliced upper storage and the block-Jacobi
-//                    inverse (SolverData.cpp:308-465, :260-306)
+//   k_assemble_pairs the slot gather: a workgroup per chunk of consecutive
+//                    rows, one lane per (row, tet) pair forming the pair's
+//                    blocks from the basis and stiffness records into LDS,
+//                    each slot's contributions then added in tet order (the
+//                    reference's order, no atomics); A = M + dt^2 K into full
+//                    or sliced upper storage and the block-Jacobi inverse
+//                    (SolverData.cpp:308-465, :260-306)
+//   k_assemble       the same by 16 lanes per dynamic row (the diagonal block
+//                    over the row's corner list, then its off-diagonal upper
+//                    slots four at a time, blocks added in tet order by
+//                    shuffles), for a system with a row too large for a chunk
 //   k_rhs            b = sum of the row's corner entries + dt m g (+ contacts)
 //                    per dynamic row, then the CG start x = 0, r = b,
 //                    z = d = M^-1 r with the (r.z, r.r) partials
@@ -443,10 +447,22 @@ __device__ constexpr int kPairOf[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 
 // tets, each of the row's ~7 off-diagonal upper slots ~5.
 constexpr int kRowGatherLanes = 16;
 
+// Adds the blocks b of a group of `lanes` lanes to a, in lane order, for
+// the first n lanes: every lane of the group gets the same sum, added in
+// list order.
+template <int lanes>
+__device__ inline void add_in_lane_order(double* a, const double* b, int n)
+{
+    for (int l = 0; l < n; ++l)
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            a[e] += __shfl(b[e], l, lanes);
+}
+
 // A = dt^2 K (+ contacts) of off-diagonal upper slot s from its contribution
-// list, summed by the 4 lanes of a sub-group (lane-strided, then shuffles: a
-// fixed order) and written by the sub-group's first lane to both triangles
-// (SolverData.cpp:422-450).
+// list: the 4 lanes of a sub-group form 4 blocks at a time, added in list
+// order (tet order) by shuffles; the sub-group's first lane writes the block
+// to both triangles (SolverData.cpp:422-450).
 template <class M>
 __device__ inline void gather_slot(int s, int lane4, const int* __restrict__ cstart, const int* __restrict__ clist,
                                    const int* __restrict__ uFull, const int* __restrict__ uMirror,
@@ -459,31 +475,22 @@ __device__ inline void gather_slot(int s, int lane4, const int* __restrict__ cst
     for (int e = 0; e < 9; ++e)
         a[e] = 0.0;
     const int k1 = cstart[s + 1];
-    for (int k = cstart[s] + lane4; k < k1; k += kSlotLanes)
+    for (int k0 = cstart[s]; k0 < k1; k0 += kSlotLanes)
     {
-        const int c = clist[k];
-        double b[9];
-        element_block(basis, kdat, c >> 5, (c >> 1) & 15, b);
-        if (c & 1)
+        double b[9] = {};
+        if (k0 + lane4 < k1)
         {
+            const int c = clist[k0 + lane4];
+            double blk[9];
+            element_block(basis, kdat, c >> 5, (c >> 1) & 15, blk);
 #pragma unroll
             for (int r = 0; r < 3; ++r)
 #pragma unroll
                 for (int q = 0; q < 3; ++q)
-                    a[3 * r + q] += b[3 * q + r];
+                    b[3 * r + q] = (c & 1) ? blk[3 * q + r] : blk[3 * r + q];
         }
-        else
-        {
-#pragma unroll
-            for (int e = 0; e < 9; ++e)
-                a[e] += b[e];
-        }
+        add_in_lane_order<kSlotLanes>(a, b, k1 - k0 < kSlotLanes ? k1 - k0 : kSlotLanes);
     }
-#pragma unroll
-    for (int e = 0; e < 9; ++e)
-#pragma unroll
-        for (int o = kSlotLanes / 2; o > 0; o >>= 1)
-            a[e] += __shfl_xor(a[e], o, kSlotLanes);
     if (lane4 != 0)
         return;
     if (cslot)
@@ -545,20 +552,16 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, int nd, const int* __
     for (int e = 0; e < 9; ++e)
         a[e] = 0.0;
     const int k1 = nstart[row + 1];
-    for (int k = nstart[row] + lane; k < k1; k += kRowGatherLanes)
+    for (int k0 = nstart[row]; k0 < k1; k0 += kRowGatherLanes)
     {
-        const int c = nlist[k];
-        double b[9];
-        element_block(basis, kdat, c >> 2, kPairOf[c & 3][c & 3], b);
-#pragma unroll
-        for (int e = 0; e < 9; ++e)
-            a[e] += b[e];
+        double b[9] = {};
+        if (k0 + lane < k1)
+        {
+            const int c = nlist[k0 + lane];
+            element_block(basis, kdat, c >> 2, kPairOf[c & 3][c & 3], b);
+        }
+        add_in_lane_order<kRowGatherLanes>(a, b, k1 - k0 < kRowGatherLanes ? k1 - k0 : kRowGatherLanes);
     }
-#pragma unroll
-    for (int e = 0; e < 9; ++e)
-#pragma unroll
-        for (int o = kRowGatherLanes / 2; o > 0; o >>= 1)
-            a[e] += __shfl_xor(a[e], o, kRowGatherLanes);
     const int sd = diagSlot[row];
     const int se = row + 1 < nd ? diagSlot[row + 1] : nnzu;
     const int g = rowGroup[row];
@@ -618,35 +621,106 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, int nd, const int* __
         gs[g].failed = 1;
 }
 
-// The assembly gather by (row, tet) pairs, atomic-free. A workgroup takes
-// one chunk of consecutive dynamic rows (at most kPairChunk pairs of the
-// rows' corner lists, kOffChunk off-diagonal contributions, kRowChunk rows
-// and kSlotChunk upper slots). Phase 1, one lane per pair (row r, tet t,
-// corner a of r): the lane reads t's basis and stiffness records once and
-// forms the blocks the pair contributes -- R K_aa R^T to r's diagonal slot,
-// R K_ab R^T to the upper slot (r, row of b) for every other corner b whose
-// row is greater -- with the arithmetic of gather_slot (the block of the
-// local pair (min, max), transposed when a > b), into LDS planes: diagonal
-// blocks at the pair's place, off-diagonal ones at their place in the
-// chunk's run of the off-diagonal lists (pairOff, three 10-bit fields,
-// 0x3ff for none). Meanwhile the other lookups go to LDS as well: each
-// slot's run descriptor (pack_run) and element, each row's mass, body and
-// preconditioner switch. Phase 2, one thread per (slot, component): the
+// The assembly gather by (row, tet) pairs, atomic-free. The host cuts the
+// dynamic rows into chunks of consecutive rows (at most kPairChunk pairs of
+// the rows' corner lists, kOffChunk off-diagonal contributions, kRowChunk
+// rows and kSlotChunk upper slots); workgroup w takes chunks w, w + G,
+// w + 2G, ... (G workgroups, as many as are resident at once). Per chunk:
+// Phase 1, one lane per pair (row r, tet t, corner a of r): the lane forms
+// from t's basis and stiffness records the blocks the pair contributes --
+// R K_aa R^T to r's diagonal slot, R K_ab R^T to the upper slot (r, row of b)
+// for every other corner b whose row is greater -- with the arithmetic of
+// gather_slot (the block of the local pair (min, max), transposed when
+// a > b), into LDS planes: diagonal blocks at the pair's place, off-diagonal
+// ones at their place in the chunk's run of the off-diagonal lists (pairOff,
+// three 10-bit fields, 0x3ff for none); the other lookups go to LDS as well:
+// each slot's run descriptor (pack_run) and element, each row's mass, body
+// and preconditioner switch. Phase 2, one thread per (slot, component): the
 // slot's contributions summed in list order, which is tet order (the order
 // the reference adds them, SolverData.cpp:407-450), then the contacts, dt^2
 // and the mass as in k_assemble, written to full or sliced upper storage.
 // Phase 3, one thread per row: the block-Jacobi inverse (SolverData.cpp:
 // 260-306). A non-finite block or a refused inverse marks the body's solve
 // failed.
-// V mu g_a.g_b of local pair (a, b) from the stiffness record kd in global
-// memory (its line is in L1 after the record's load): kd[KD_VMUG +
-// kPairOf[a][b]], the index in closed form (lo, hi = min, max: 4 lo -
-// lo (lo - 1) / 2 + hi - lo). (Picking it from the record in registers by
-// selects makes the compiler index a scratch copy of the record.)
-__device__ inline double vmg_pick(const double* __restrict__ kd, int x, int y)
+// The loads run ahead of the arithmetic, so that a chunk's three phases
+// overlap the dependent loads (descriptor -> pair -> records) of the next
+// ones: chunk descriptors three chunks ahead (scalar loads), each lane's pair
+// and lookups two ahead, the pair's records and the rows' preconditioner
+// switches one ahead. None of these loads is conditional (a lane past a
+// chunk's pairs, slots or rows reads the chunk's first one; past the last
+// chunk the workgroup reads its last one again), and nothing reads what they
+// return before the next chunk: the compiler can then count the loads in
+// flight, and a chunk waits only for loads issued during the chunk before.
+
+// One lane's pair of a chunk: its corner-list entry (tet << 2 | corner) and
+// its three off-diagonal offsets.
+struct PairLane
 {
-    const int lo = x < y ? x : y, hi = x < y ? y : x;
-    return kd[KD_VMUG + 4 * lo - (lo * (lo - 1)) / 2 + hi - lo];
+    int cc, offs;
+};
+
+// One lane's lookups of a chunk: the run descriptor, element and mirror
+// element (full storage) of slot lane, the body and mass of row lane.
+struct PairLookups
+{
+    int run, elem, mirror, body;
+    double mass;
+};
+
+// The records of a pair (t, a): t's basis record (R g_0 .. R g_3, R R^T),
+// V lambda, V mu and V mu g_a.g_b for b = 0 .. 3.
+struct PairRecords
+{
+    double w[12], rr[6], vl, vm, vg[4];
+};
+
+__device__ inline PairLane load_pair_lane(const int4& ch, int tid, const int* __restrict__ nlist,
+                                          const int* __restrict__ pairOff)
+{
+    const int k = ch.y + (tid < (ch.w >> 16) ? tid : 0);
+    return PairLane{nlist[k], pairOff[k]};
+}
+
+__device__ inline PairLookups load_pair_lookups(const int4& ch, int tid, const int* __restrict__ slotRun,
+                                                const int* __restrict__ uElem, const int* __restrict__ mirror,
+                                                const int* __restrict__ rowGroup, const double* __restrict__ mrow)
+{
+    const int s = ch.z + (tid < ((ch.w >> 8) & 0xff) ? tid : 0);
+    const int r = ch.x + (tid < (ch.w & 0xff) ? tid : 0);
+    return PairLookups{slotRun[s], uElem[s], mirror[s], rowGroup[r], mrow[r]};
+}
+
+// V mu g_a.g_b of local pair (a, b) sits at KD_VMUG + 4 lo - lo (lo - 1) / 2
+// + hi - lo (lo, hi = min, max), the stored pair order.
+__device__ inline void load_pair_records(const PairLane& pl, const double* __restrict__ basis,
+                                         const double* __restrict__ kdat, PairRecords& R)
+{
+    const int t = pl.cc >> 2, a = pl.cc & 3;
+    const double2* rec2 = reinterpret_cast<const double2*>(basis + static_cast<size_t>(t) * EB_COUNT);
+    const double* kd = kdat + static_cast<size_t>(t) * KD_COUNT;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+    {
+        const double2 u = rec2[i];
+        R.w[2 * i] = u.x;
+        R.w[2 * i + 1] = u.y;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        const double2 u = rec2[EB_RRT / 2 + i];
+        R.rr[2 * i] = u.x;
+        R.rr[2 * i + 1] = u.y;
+    }
+    const double2 lm = *reinterpret_cast<const double2*>(kd + KD_VLAM);
+    R.vl = lm.x;
+    R.vm = lm.y;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+    {
+        const int lo = a < b ? a : b, hi = a < b ? b : a;
+        R.vg[b] = kd[KD_VMUG + 4 * lo - (lo * (lo - 1)) / 2 + hi - lo];
+    }
 }
 
 // v[k] for a lane-varying k in 0..3, as selects (no indexed registers).
@@ -654,6 +728,53 @@ __device__ inline double pick4(int k, double v0, double v1, double v2, double v3
 {
     return k == 0 ? v0 : (k == 1 ? v1 : (k == 2 ? v2 : v3));
 }
+
+// Phase 1's blocks of one pair into the LDS planes P (diagonal, at the pair's
+// place tid) and O (off-diagonal, at the offsets).
+__device__ inline void form_pair_blocks(const PairLane& pl, const PairRecords& R, int tid, double* P, double* O)
+{
+    const int a = pl.cc & 3;
+    const double RRt[9] = {R.rr[0], R.rr[1], R.rr[2], R.rr[1], R.rr[3], R.rr[4], R.rr[2], R.rr[4], R.rr[5]};
+    double wa[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        wa[i] = pick4(a, R.w[i], R.w[3 + i], R.w[6 + i], R.w[9 + i]);
+    double K[9];
+    rotated_block_scaled(wa, wa, RRt, R.vl, R.vm, pick4(a, R.vg[0], R.vg[1], R.vg[2], R.vg[3]), K);
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+        P[e * kPairChunk + tid] = K[e];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+    {
+        const int o = (pl.offs >> (10 * j)) & 0x3ff;
+        if (o == 0x3ff)
+            continue;
+        const int b = j < a ? j : j + 1;  // the other corners in increasing order
+        double wb[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            wb[i] = pick4(b, R.w[i], R.w[3 + i], R.w[6 + i], R.w[9 + i]);
+        // the stored block of the local pair (min, max), transposed when the
+        // row's corner is the larger one
+        const bool flip = a > b;
+        double lo[3], hi[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            lo[i] = flip ? wb[i] : wa[i];
+            hi[i] = flip ? wa[i] : wb[i];
+        }
+        rotated_block_scaled(lo, hi, RRt, R.vl, R.vm, pick4(b, R.vg[0], R.vg[1], R.vg[2], R.vg[3]), K);
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int y = 0; y < 3; ++y)
+                O[(3 * x + y) * kOffChunk + o] = flip ? K[3 * y + x] : K[3 * x + y];
+    }
+}
+
+static_assert(kSlotChunk <= NT && kRowChunk <= NT, "one lane per slot and per row of a chunk");
 
 template <class M>
 __global__ __launch_bounds__(NT) void k_assemble_pairs(
@@ -672,177 +793,157 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
     int* RG = reinterpret_cast<int*>(RM + kRowChunk);  // kRowChunk bodies (-1 - body: no preconditioner)
     int* SR = RG + kRowChunk;                // kSlotChunk run descriptors
     int* SE = SR + kSlotChunk;               // kSlotChunk elements
-    const int c = xcd_block();
-    if (c >= nchunks)
+    int* SM = SE + kSlotChunk;               // kSlotChunk mirror elements (full storage)
+    const int G = static_cast<int>(gridDim.x);
+    const int first = xcd_block();
+    if (first >= nchunks)
         return;
+    const int niter = (nchunks - first + G - 1) / G;  // chunks first + i G, i < niter
     const int tid = static_cast<int>(threadIdx.x);
-    const int4 ch = chunks[c];
-    const int r0 = ch.x, pBase = ch.y, sBase = ch.z;
-    const int nR = ch.w & 0xff, nS = (ch.w >> 8) & 0xff, pEnd = pBase + (ch.w >> 16);
     const size_t F = static_cast<size_t>(nnzf);
+    // upper-only storage has no mirrors: read the elements again instead
+    const int* mirrorOf = upperOnly ? uElem : uMirror;
 
-    // phase 1: the lookups of phases 2 and 3
-    for (int j = tid; j < nS; j += NT)
+    // the descriptor of the workgroup's chunk i (its last chunk past the end)
+    auto desc = [&](int i) { return chunks[first + (i < niter ? i : niter - 1) * G]; };
+
+    // chunk 0's pair, lookups, records and switches; chunk 1's pair and
+    // lookups; the descriptors of chunks 0 .. 2
+    int4 c0 = desc(0), c1 = desc(1), c2 = desc(2);
+    PairLane p0 = load_pair_lane(c0, tid, nlist, pairOff);
+    PairLookups l0 = load_pair_lookups(c0, tid, slotRun, uElem, mirrorOf, rowGroup, mrow);
+    PairLane p1 = load_pair_lane(c1, tid, nlist, pairOff);
+    PairLookups l1 = load_pair_lookups(c1, tid, slotRun, uElem, mirrorOf, rowGroup, mrow);
+    PairRecords r0;
+    load_pair_records(p0, basis, kdat, r0);
+    int j0 = gs[l0.body].jacobi;
+
+    for (int i = 0; i < niter; ++i)
     {
-        SR[j] = slotRun[sBase + j];
-        SE[j] = uElem[sBase + j];
-    }
-    if (tid < nR)
-    {
-        const int g = rowGroup[r0 + tid];
-        RM[tid] = mrow[r0 + tid];
-        RG[tid] = gs[g].jacobi ? g : -1 - g;
-    }
-    // phase 1: the pairs' blocks
-    const int p = pBase + tid;
-    if (p < pEnd)
-    {
-        const int cc = nlist[p];
-        const int t = cc >> 2, a = cc & 3;
-        const int offs = pairOff[p];
-        double w[12], rr[6];
-        const double2* rec2 = reinterpret_cast<const double2*>(basis + static_cast<size_t>(t) * EB_COUNT);
-        const double* kdg = kdat + static_cast<size_t>(t) * KD_COUNT;
-        const double2 lm = *reinterpret_cast<const double2*>(kdg + KD_VLAM);  // V lambda, V mu
-#pragma unroll
-        for (int i = 0; i < 6; ++i)
+        const int4 ch = c0;
+        const int r0w = ch.x, sBase = ch.z;
+        const int nR = ch.w & 0xff, nS = (ch.w >> 8) & 0xff, nP = ch.w >> 16;
+
+        // ahead: chunk i + 3's descriptor, chunk i + 1's records and
+        // switches, chunk i + 2's pair and lookups
+        const int4 c3 = desc(i + 3);
+        PairRecords r1;
+        load_pair_records(p1, basis, kdat, r1);
+        const int j1 = gs[l1.body].jacobi;
+        const PairLane p2 = load_pair_lane(c2, tid, nlist, pairOff);
+        const PairLookups l2 = load_pair_lookups(c2, tid, slotRun, uElem, mirrorOf, rowGroup, mrow);
+
+        // phase 1
+        if (tid < nS)
         {
-            const double2 u = rec2[i];
-            w[2 * i] = u.x;
-            w[2 * i + 1] = u.y;
+            SR[tid] = l0.run;
+            SE[tid] = l0.elem;
+            SM[tid] = l0.mirror;
         }
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
+        if (tid < nR)
         {
-            const double2 u = rec2[EB_RRT / 2 + i];
-            rr[2 * i] = u.x;
-            rr[2 * i + 1] = u.y;
+            RM[tid] = l0.mass;
+            RG[tid] = j0 ? l0.body : -1 - l0.body;
         }
-        const double RRt[9] = {rr[0], rr[1], rr[2], rr[1], rr[3], rr[4], rr[2], rr[4], rr[5]};
-        double wa[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            wa[i] = pick4(a, w[i], w[3 + i], w[6 + i], w[9 + i]);
-        double K[9];
-        rotated_block_scaled(wa, wa, RRt, lm.x, lm.y, vmg_pick(kdg, a, a), K);
-#pragma unroll
-        for (int e = 0; e < 9; ++e)
-            P[e * kPairChunk + tid] = K[e];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
+        if (tid < nP)
+            form_pair_blocks(p0, r0, tid, P, O);
+        __syncthreads();
+
+        // phase 2
+        for (int it = tid; it < 9 * nS; it += NT)
         {
-            const int o = (offs >> (10 * j)) & 0x3ff;
-            if (o == 0x3ff)
-                continue;
-            const int b = j < a ? j : j + 1;  // the other corners in increasing order
-            double wb[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                wb[i] = pick4(b, w[i], w[3 + i], w[6 + i], w[9 + i]);
-            // the stored block of the local pair (min, max), transposed when
-            // the row's corner is the larger one
-            const bool flip = a > b;
-            double lo[3], hi[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
+            const int e = it / nS;
+            const int sl = it - e * nS;
+            const int run = SR[sl];
+            const int begin = run & 0x3ff, n = (run >> 10) & 0x3ff, row = (run >> 20) & 0x3f;
+            const bool diag = (run >> 26) & 1;
+            const double* src = (diag ? P + e * kPairChunk : O + e * kOffChunk) + begin;
+            // in list order; eight LDS loads in flight ahead of the adds
+            double v = 0.0;
+            int j = 0;
+            for (; j + 8 <= n; j += 8)
             {
-                lo[i] = flip ? wb[i] : wa[i];
-                hi[i] = flip ? wa[i] : wb[i];
-            }
-            rotated_block_scaled(lo, hi, RRt, lm.x, lm.y, vmg_pick(kdg, a, b), K);
+                double u[8];
 #pragma unroll
-            for (int x = 0; x < 3; ++x)
+                for (int k = 0; k < 8; ++k)
+                    u[k] = src[j + k];
 #pragma unroll
-                for (int y = 0; y < 3; ++y)
-                    O[(3 * x + y) * kOffChunk + o] = flip ? K[3 * y + x] : K[3 * x + y];
-        }
-    }
-    __syncthreads();
-
-    // phase 2
-    for (int it = tid; it < 9 * nS; it += NT)
-    {
-        const int e = it / nS;
-        const int sl = it - e * nS;
-        const int run = SR[sl];
-        const int begin = run & 0x3ff, n = (run >> 10) & 0x3ff, row = (run >> 20) & 0x3f;
-        const bool diag = (run >> 26) & 1;
-        const double* src = (diag ? P + e * kPairChunk : O + e * kOffChunk) + begin;
-        // in list order; eight LDS loads in flight ahead of the adds
-        double v = 0.0;
-        int j = 0;
-        for (; j + 8 <= n; j += 8)
-        {
-            double u[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                u[k] = src[j + k];
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                v += u[k];
-        }
-        if (j < n)
-        {
-            double u[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                u[k] = j + k < n ? src[j + k] : 0.0;
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if (j + k < n)
+                for (int k = 0; k < 8; ++k)
                     v += u[k];
+            }
+            if (j < n)
+            {
+                double u[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    u[k] = j + k < n ? src[j + k] : 0.0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (j + k < n)
+                        v += u[k];
+            }
+            const bool onDiagonal = e == 0 || e == 4 || e == 8;
+            if (cslot && onDiagonal)
+                v += cslot[sBase + sl];  // point connections' and contacts' k w_a w_b I (PointConnection.cpp:110-128)
+            v *= dt2;
+            if (diag && onDiagonal)
+                v = RM[row] + v;
+            const int full = SE[sl];
+            vals[upperOnly ? SliceStore<M>::word(F, full, e) : BsrStore<M>::at(F, full, e)] = static_cast<M>(v);
+            if (!upperOnly && !diag)
+            {
+                const int x = e / 3, y = e - 3 * (e / 3);
+                vals[BsrStore<M>::at(F, SM[sl], 3 * y + x)] = static_cast<M>(v);
+            }
+            if (!fin(v))
+            {
+                const int g = RG[row];
+                gs[g >= 0 ? g : -1 - g].failed = 1;  // a non-finite block rejects the assembly
+            }
+            if (diag)
+                D[9 * row + e] = v;
         }
-        const bool onDiagonal = e == 0 || e == 4 || e == 8;
-        if (cslot && onDiagonal)
-            v += cslot[sBase + sl];  // point connections' and contacts' k w_a w_b I (PointConnection.cpp:110-128)
-        v *= dt2;
-        if (diag && onDiagonal)
-            v = RM[row] + v;
-        const int full = SE[sl];
-        vals[upperOnly ? SliceStore<M>::word(F, full, e) : BsrStore<M>::at(F, full, e)] = static_cast<M>(v);
-        if (!upperOnly && !diag)
-        {
-            const int x = e / 3, y = e - 3 * (e / 3);
-            vals[BsrStore<M>::at(F, uMirror[sBase + sl], 3 * y + x)] = static_cast<M>(v);
-        }
-        if (!fin(v))
-        {
-            const int g = RG[row];
-            gs[g >= 0 ? g : -1 - g].failed = 1;  // a non-finite block rejects the assembly
-        }
-        if (diag)
-            D[9 * row + e] = v;
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // phase 3
-    if (tid < nR)
-    {
-        double A[9];
-#pragma unroll
-        for (int e = 0; e < 9; ++e)
-            A[e] = D[9 * tid + e];
-        const int gr = RG[tid];
-        double mi[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-        if (gr >= 0)
+        // phase 3 (thread tid reads D, written before the barrier above, and
+        // RG[tid], its own: the next chunk's phase 1 may start meanwhile)
+        if (tid < nR)
         {
-            bool pok = fin9(A);
-            if (pok)
-            {
-                const double d = det3(A);
-                pok = fin(d) && fabs(d) > static_cast<double>(1.0e-8f);
-            }
-            if (pok)
-            {
-                inv3(A, mi);
-                pok = fin9(mi);
-            }
-            if (!pok)
-                gs[gr].failed = 1;  // BuildInversePreconditioner fails the solve
-        }
+            double A[9];
 #pragma unroll
-        for (int e = 0; e < 9; ++e)
-            minv[static_cast<size_t>(e) * nd + r0 + tid] = static_cast<M>(mi[e]);
+            for (int e = 0; e < 9; ++e)
+                A[e] = D[9 * tid + e];
+            const int gr = RG[tid];
+            double mi[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+            if (gr >= 0)
+            {
+                bool pok = fin9(A);
+                if (pok)
+                {
+                    const double d = det3(A);
+                    pok = fin(d) && fabs(d) > static_cast<double>(1.0e-8f);
+                }
+                if (pok)
+                {
+                    inv3(A, mi);
+                    pok = fin9(mi);
+                }
+                if (!pok)
+                    gs[gr].failed = 1;  // BuildInversePreconditioner fails the solve
+            }
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                minv[static_cast<size_t>(e) * nd + r0w + tid] = static_cast<M>(mi[e]);
+        }
+        c0 = c1;
+        c1 = c2;
+        c2 = c3;
+        p0 = p1;
+        l0 = l1;
+        r0 = r1;
+        j0 = j1;
+        p1 = p2;
+        l1 = l2;
     }
 }
 
@@ -1031,6 +1132,28 @@ void launch_element(const DeviceView& v, double dt, int implicit)
                            v.fem, v.prm, v.x, v.v, v.cpos, v.basis, v.corner, dt, implicit);
 }
 
+// Workgroups of the pair gather: as many as the current device holds at once
+// (a multiple of 8, found once per device), at most one per chunk.
+template <class M>
+static unsigned pair_gather_grid(int nchunks)
+{
+    static int resident[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        dev = 0;
+    int& n = resident[dev & 63];
+    if (n == 0)
+    {
+        int per = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_assemble_pairs<M>, NT, kPairLds) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            per = cus = 0;
+        n = per * cus >= 8 ? per * cus / 8 * 8 : 8;
+    }
+    const unsigned all = xcd_grid(nchunks);
+    return all < static_cast<unsigned>(n) ? all : static_cast<unsigned>(n);
+}
+
 template <class M>
 void launch_assemble(const DeviceView& v, double dt)
 {
@@ -1039,7 +1162,7 @@ void launch_assemble(const DeviceView& v, double dt)
         return;
     if (v.nchunks > 0)
     {
-        hipLaunchKernelGGL((k_assemble_pairs<M>), dim3(xcd_grid(v.nchunks)), dim3(NT), kPairLds, v.stream,
+        hipLaunchKernelGGL((k_assemble_pairs<M>), dim3(pair_gather_grid<M>(v.nchunks)), dim3(NT), kPairLds, v.stream,
                            v.nchunks, v.chunks, v.nd, v.nlist, v.pairOff, v.slotRun,
                            v.upper ? v.slotEll : v.uFull, v.uMirror, v.rowGroup, v.basis, v.kdat, v.mrow, dt * dt,
                            static_cast<M*>(v.vals), v.upper ? v.nE : v.nnzf, static_cast<M*>(v.minv), v.gs, v.upper,

@@ -93,16 +93,17 @@ struct TileView
 
 // Limits of one chunk of the assembly gather by (row, tet) pairs: corner-
 // list pairs (one lane each), off-diagonal contributions, rows and upper
-// slots; its LDS (kPairLds bytes, 52.1 KB: three workgroups per CU; 54 KB
+// slots; its LDS (kPairLds bytes, 53.0 KB: three workgroups per CU; 54 KB
 // admits two) holds the blocks (9 doubles each), the rows' diagonal blocks,
-// masses and bodies, and the slots' run descriptors and elements. A tet
-// gives 4 pairs and 6 off-diagonal contributions, so 256 pairs go with 384.
+// masses and bodies, and the slots' run descriptors, elements and mirror
+// elements. A tet gives 4 pairs and 6 off-diagonal contributions, so 256
+// pairs go with 384.
 constexpr int kPairChunk = NT;
 constexpr int kOffChunk = 384;
 constexpr int kRowChunk = 48;
 constexpr int kSlotChunk = 224;
 constexpr size_t kPairLds = 8 * 9 * static_cast<size_t>(kPairChunk + kOffChunk + kRowChunk) +
-                            8 * static_cast<size_t>(kSlotChunk) + 16 * static_cast<size_t>(kRowChunk);
+                            12 * static_cast<size_t>(kSlotChunk) + 16 * static_cast<size_t>(kRowChunk);
 // Run descriptor of an upper slot in its chunk: where its contributions lie
 // in the chunk's block planes (the diagonal ones: the row's pairs), how many,
 // the row within the chunk and whether the slot is the row's diagonal.

@@ -107,6 +107,10 @@ __device__ inline double gdot(const double* ga, const double* gb) { return ga[0]
 
 // R K_ab R^T from the premultiplied factors vl = V lam, vm = V mu and
 // vmg = V mu (g_a . g_b) (the tet's stiffness record); RRt is full 3x3.
+// The sum vl rga_i rgb_j + vm rgb_i rga_j + vmg RRt_ij is written as explicit
+// fused multiply-adds: left to the compiler, which products it fuses depends
+// on the code around each inlined copy, and every kernel that forms these
+// blocks (the gathers, the inspection records) must round them alike.
 __device__ inline void rotated_block_scaled(const double* rga, const double* rgb, const double* RRt, double vl,
                                             double vm, double vmg, double* K)
 {
@@ -114,7 +118,10 @@ __device__ inline void rotated_block_scaled(const double* rga, const double* rgb
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            K[3 * i + j] = vl * rga[i] * rgb[j] + vm * rgb[i] * rga[j] + vmg * RRt[3 * i + j];
+        {
+            const double sym = (vm * rgb[i]) * rga[j];
+            K[3 * i + j] = __builtin_fma(vmg, RRt[3 * i + j], __builtin_fma(vl * rga[i], rgb[j], sym));
+        }
 }
 
 

@@ -94,3 +94,66 @@ def test_fan_assembly_and_steps_match_the_oracle(n, precision):
         assert rel(xg, xo) <= 1e-9 and rel(vg, vo) <= 1e-9, f"step {k}: {rel(xg, xo):.2e} {rel(vg, vo):.2e}"
         assert abs(int(it[0]) - io) <= 1
     s.close()
+
+
+PAIRS = [(a, b) for a in range(4) for b in range(a, 4)]  # the 10 stored local pairs
+
+
+def tet_order_sums(p, tets, blocks):
+    """Each upper slot's sum of its tets' element blocks (the element kernel's
+    inspection records), added in tet order in float64 from zero: the
+    block of the local pair (min, max), transposed where the row's corner
+    is the larger one (SolverData.cpp:407-450 adds them in this order)."""
+    index = {}
+    for r in range(p.nd):
+        for k in range(p.full_start[r], p.full_start[r + 1]):
+            index[(r, int(p.full_col[k]))] = k
+    stored = {ab: i for i, ab in enumerate(PAIRS)}
+    sums = {}
+    for t, tet in enumerate(tets):
+        rows = p.dyn[tet]
+        for a in range(4):
+            for b in range(4):
+                ra, rb = int(rows[a]), int(rows[b])
+                if ra < 0 or rb < 0 or ra > rb or (a != b and ra == rb):
+                    continue
+                blk = blocks[t, stored[(min(a, b), max(a, b))]]
+                if a > b:
+                    blk = blk.T
+                k = index[(ra, rb)]
+                sums[k] = sums[k] + blk if k in sums else 0.0 + blk
+    return index, sums
+
+
+@pytest.mark.parametrize("mesh", ["fan40", "fan150", "beam10", "twist8"])
+@pytest.mark.parametrize("precision", [N.PK_FP64, N.PK_FP64_UPPER])
+def test_matrix_is_the_tet_order_sum_of_the_element_blocks(mesh, precision):
+    """Bitwise: A's entries are dt^2 times the tet-order sums of the blocks
+    the inspection kernel reports (all kernels form a block with the same
+    arithmetic; the pair gather and, for fan150, the row gather add them in
+    tet order). The three entries of a diagonal block that carry the mass
+    are left out (the lumped masses are not read back); mirrors are the
+    transposes."""
+    # fans (pair gather; fan150 takes the row gather), a 5-tet box at rest,
+    # a jittered Kuhn box twisted by pi/2
+    w = fan(int(mesh[3:])) if mesh.startswith("fan") else W.beam10() if mesh == "beam10" else W.twist64(n=8)
+    print(f"case {mesh}, precision {precision}", flush=True)
+    s = W.make_system(w, precision)
+    s.assemble(w.dt)
+    vals, rhs, minv = s.matrix()
+    blocks, forces, kv = s.elements()
+    p = Pattern(w.rest, w.tets, node_fixed=w.fixed)
+    index, sums = tet_order_sums(p, w.tets, blocks)
+    dt2 = w.dt * w.dt
+    off = np.ones((3, 3), bool)
+    off[[0, 1, 2], [0, 1, 2]] = False
+    for (r, c), k in index.items():
+        if r > c:
+            continue
+        expect = sums[k] * dt2
+        if r == c:
+            assert np.array_equal(vals[k][off], expect[off]), f"diagonal block of row {r}"
+        else:
+            assert np.array_equal(vals[k], expect), f"block ({r}, {c})"
+            assert np.array_equal(vals[index[(c, r)]], expect.T), f"mirror of ({r}, {c})"
+    s.close()
