@@ -776,6 +776,36 @@ __device__ inline void form_pair_blocks(const PairLane& pl, const PairRecords& R
 
 static_assert(kSlotChunk <= NT && kRowChunk <= NT, "one lane per slot and per row of a chunk");
 
+// The 9 sums of a slot's run in the LDS block planes (component e of term j
+// at src[e * stride + j]), each added in list order (tet order) from zero;
+// two terms of all 9 components in flight at a time.
+__device__ inline void sum_run(const double* src, int stride, int n, double* v)
+{
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+        v[e] = 0.0;
+    int j = 0;
+    for (; j + 2 <= n; j += 2)
+    {
+        double a[9], b[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+        {
+            a[e] = src[e * stride + j];
+            b[e] = src[e * stride + j + 1];
+        }
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            v[e] = (v[e] + a[e]) + b[e];
+    }
+    if (j < n)
+    {
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            v[e] += src[e * stride + j];
+    }
+}
+
 template <class M>
 __global__ __launch_bounds__(NT) void k_assemble_pairs(
     int nchunks, const int4* __restrict__ chunks, int nd, const int* __restrict__ nlist,
@@ -849,59 +879,60 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
             form_pair_blocks(p0, r0, tid, P, O);
         __syncthreads();
 
-        // phase 2
-        for (int it = tid; it < 9 * nS; it += NT)
+        // phase 2: one thread per slot, its 9 components at once
+        if (tid < nS)
         {
-            const int e = it / nS;
-            const int sl = it - e * nS;
-            const int run = SR[sl];
+            const int run = SR[tid];
             const int begin = run & 0x3ff, n = (run >> 10) & 0x3ff, row = (run >> 20) & 0x3f;
             const bool diag = (run >> 26) & 1;
-            const double* src = (diag ? P + e * kPairChunk : O + e * kOffChunk) + begin;
-            // in list order; eight LDS loads in flight ahead of the adds
-            double v = 0.0;
-            int j = 0;
-            for (; j + 8 <= n; j += 8)
+            double v[9];
+            sum_run(diag ? P + begin : O + begin, diag ? kPairChunk : kOffChunk, n, v);
+            if (cslot)
             {
-                double u[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    u[k] = src[j + k];
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    v += u[k];
+                // point connections' and contacts' k w_a w_b I (PointConnection.cpp:110-128)
+                const double cs = cslot[sBase + tid];
+                v[0] += cs;
+                v[4] += cs;
+                v[8] += cs;
             }
-            if (j < n)
+            bool bad = false;
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
             {
-                double u[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    u[k] = j + k < n ? src[j + k] : 0.0;
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (j + k < n)
-                        v += u[k];
+                v[e] *= dt2;
+                if (diag && (e == 0 || e == 4 || e == 8))
+                    v[e] = RM[row] + v[e];
+                bad = bad || !fin(v[e]);
             }
-            const bool onDiagonal = e == 0 || e == 4 || e == 8;
-            if (cslot && onDiagonal)
-                v += cslot[sBase + sl];  // point connections' and contacts' k w_a w_b I (PointConnection.cpp:110-128)
-            v *= dt2;
-            if (diag && onDiagonal)
-                v = RM[row] + v;
-            const int full = SE[sl];
-            vals[upperOnly ? SliceStore<M>::word(F, full, e) : BsrStore<M>::at(F, full, e)] = static_cast<M>(v);
-            if (!upperOnly && !diag)
+            const int full = SE[tid];
+            if (upperOnly)
+                SliceStore<M>::put(vals, F, full, v);
+            else
             {
-                const int x = e / 3, y = e - 3 * (e / 3);
-                vals[BsrStore<M>::at(F, SM[sl], 3 * y + x)] = static_cast<M>(v);
+#pragma unroll
+                for (int e = 0; e < 9; ++e)
+                    vals[BsrStore<M>::at(F, full, e)] = static_cast<M>(v[e]);
+                if (!diag)
+                {
+                    const int mirror = SM[tid];
+#pragma unroll
+                    for (int x = 0; x < 3; ++x)
+#pragma unroll
+                        for (int y = 0; y < 3; ++y)
+                            vals[BsrStore<M>::at(F, mirror, 3 * y + x)] = static_cast<M>(v[3 * x + y]);
+                }
             }
-            if (!fin(v))
+            if (bad)
             {
                 const int g = RG[row];
                 gs[g >= 0 ? g : -1 - g].failed = 1;  // a non-finite block rejects the assembly
             }
             if (diag)
-                D[9 * row + e] = v;
+            {
+#pragma unroll
+                for (int e = 0; e < 9; ++e)
+                    D[9 * row + e] = v[e];
+            }
         }
         __syncthreads();
 

@@ -104,6 +104,8 @@ constexpr int kRowChunk = 48;
 constexpr int kSlotChunk = 224;
 constexpr size_t kPairLds = 8 * 9 * static_cast<size_t>(kPairChunk + kOffChunk + kRowChunk) +
                             12 * static_cast<size_t>(kSlotChunk) + 16 * static_cast<size_t>(kRowChunk);
+// three workgroups per CU (160 KB of LDS, allocated in blocks of up to 4 KB)
+static_assert(kPairLds <= 53248, "pair gather LDS beyond a third of a CU");
 // Run descriptor of an upper slot in its chunk: where its contributions lie
 // in the chunk's block planes (the diagonal ones: the row's pairs), how many,
 // the row within the chunk and whether the slot is the row's diagonal.
@@ -184,6 +186,24 @@ struct SliceStore
         if (w < 8)
             return kGroup * ((static_cast<size_t>(w) / kGroup) * nE + e) + static_cast<size_t>(w) % kGroup;
         return 8 * nE + e;
+    }
+    // the 9 values of element e, each 16-byte plane entry in one store
+    __device__ static void put(M* vals, size_t nE, size_t e, const double* v)
+    {
+        if constexpr (sizeof(M) == 4)
+        {
+            *reinterpret_cast<float4*>(vals + 4 * e) = make_float4(static_cast<float>(v[0]), static_cast<float>(v[1]),
+                                                                   static_cast<float>(v[2]), static_cast<float>(v[3]));
+            *reinterpret_cast<float4*>(vals + 4 * (nE + e)) = make_float4(
+                static_cast<float>(v[4]), static_cast<float>(v[5]), static_cast<float>(v[6]), static_cast<float>(v[7]));
+        }
+        else
+        {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                *reinterpret_cast<double2*>(vals + 2 * (q * nE + e)) = make_double2(v[2 * q], v[2 * q + 1]);
+        }
+        vals[8 * nE + e] = static_cast<M>(v[8]);
     }
 };
 
