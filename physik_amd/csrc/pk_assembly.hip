@@ -993,7 +993,8 @@ __device__ inline void apply_minv(const M* __restrict__ minv, size_t nd, int row
 }
 
 // b = dt (f + m g) - dt^2 K v per dynamic row from its corner entries (each
-// dt f - dt^2 (K v) of one tet, k_element), then the CG start x = 0, r = b,
+// dt f - dt^2 (K v) of one tet, k_element, added in tet order), then the CG
+// start x = 0, r = b,
 // z = d = M^-1 r with the (r.z, r.r) partials (ConjugateGradientSolver.cpp:
 // 151-164).
 template <class M>
@@ -1019,20 +1020,27 @@ __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__
     {
         const int row = row0 + static_cast<int>(threadIdx.x) / RL;
         const bool live = row < tl.rowEnd;
+        // the row's entries in list order (tet order, as the reference adds
+        // them): the group's lanes load RL at a time, then every lane adds
+        // them one by one (shuffles within the group)
         double e3[3] = {0, 0, 0};
-        const int k1 = live ? nstart[row + 1] : 0;
-        for (int k = live ? nstart[row] + lane : 0; k < k1; k += RL)
+        const int k0 = live ? nstart[row] : 0, k1 = live ? nstart[row + 1] : 0;
+        for (int k = k0; k < k1; k += RL)
         {
-            const double* e = corner + static_cast<size_t>(k) * N_CORNER;
+            double u[3] = {0, 0, 0};
+            if (k + lane < k1)
+            {
+                const double* e = corner + static_cast<size_t>(k + lane) * N_CORNER;
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
-                e3[i] += e[i];
+                for (int i = 0; i < 3; ++i)
+                    u[i] = e[i];
+            }
+            const int n = k1 - k < RL ? k1 - k : RL;
+            for (int l = 0; l < n; ++l)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    e3[i] += __shfl(u[i], l, RL);
         }
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int o = RL / 2; o > 0; o >>= 1)
-                e3[i] += __shfl_xor(e3[i], o, RL);
         if (!live || lane != 0)
             continue;
         const double m = mrow[row];
