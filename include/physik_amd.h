@@ -137,15 +137,19 @@ int pk_system_get_tune(const pk_system* sys, int* lanes_per_row, int* spmv_rows,
 
 /* SpMV staging (no reference counterpart). mode 1 (the default) lets each
  * SpMV workgroup copy the rows of the direction vector its tile references
- * into LDS (up to 20 KB) and read the direction blocks from there: when every
- * tile's column window [first, last] fits, tiles stage their windows;
- * otherwise they stage up to four segments of rows (the columns cut at their
- * three widest gaps), and tiles that still need more than 20 KB gather from
- * global memory. mode 0 gathers from global memory. Results are identical in
- * every mode. get reports the mode, the rows a workgroup stages (0: none)
- * and whether segments are in use. */
+ * into LDS and read the direction blocks from there, up to a capacity: the
+ * LDS share that still leaves a CU every workgroup of the selected SpMV
+ * kernel its registers allow (so it depends on the storage, precision and
+ * launch shape). When every tile's column window [first, last] fits, tiles
+ * stage their windows; otherwise they stage up to four segments of rows (the
+ * columns cut at their three widest gaps), and tiles that still need more
+ * than the capacity gather from global memory. mode 0 gathers from global
+ * memory. Results are identical in every mode. get reports the mode, the
+ * rows a workgroup stages (0: none) and whether segments are in use;
+ * capacity the rows the capacity holds (0 with staging off). */
 int pk_system_set_spmv_staging(pk_system* sys, int mode);
 int pk_system_get_spmv_staging(const pk_system* sys, int* mode, int* rows, int* segmented);
+int pk_system_get_spmv_stage_capacity(const pk_system* sys, int* rows);
 
 /* Node state, node_count x 3 each; either pointer may be NULL. */
 int pk_system_set_state(pk_system* sys, const double* x, const double* v);

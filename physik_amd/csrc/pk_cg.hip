@@ -1100,6 +1100,53 @@ void launch_spmv(const DeviceView& v, const int* kb, int k0, int lanesPerRow, in
     launch_spmv_shape<M, 16, 1>(v, kb, k0, stamp);
 }
 
+// LDS one SpMV workgroup can stage d in, for the kernel a launch shape
+// selects (both staging forms), without fewer workgroups fitting on a CU
+// than its registers allow: the CU's LDS shared among that many in 2 KB
+// units, less the kernel's own LDS. 20 KB when the device cannot be asked.
+template <class M, int LPR, int U>
+static size_t stage_share(bool upper, int ldsPerCU)
+{
+    const void* kernels[2] = {upper ? reinterpret_cast<const void*>(k_spmv_upper<M, 1, true>)
+                                    : reinterpret_cast<const void*>(k_spmv<M, LPR, U, true>),
+                              upper ? reinterpret_cast<const void*>(k_spmv_upper<M, 1, false>)
+                                    : reinterpret_cast<const void*>(k_spmv<M, LPR, U, false>)};
+    size_t share = SIZE_MAX;
+    for (const void* k : kernels)
+    {
+        int blocks = 0;
+        hipFuncAttributes attr{};
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, NT, 0) != hipSuccess || blocks <= 0 ||
+            hipFuncGetAttributes(&attr, k) != hipSuccess)
+            return 20 * 1024;
+        const size_t each = (static_cast<size_t>(ldsPerCU) / static_cast<size_t>(blocks)) / 2048 * 2048;
+        const size_t own = attr.sharedSizeBytes;
+        share = std::min(share, each > own ? each - own : 0);
+    }
+    return share;
+}
+
+template <class M>
+size_t spmv_stage_bytes(bool upper, int lanesPerRow, int unroll)
+{
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || lds <= 0)
+        return 20 * 1024;
+    if (upper)
+        return stage_share<M, 1, 1>(true, lds);
+#define PK_SHAPE(L, U)                   \
+    if (lanesPerRow == L && unroll == U) \
+        return stage_share<M, L, U>(false, lds);
+    PK_SHAPE(1, 1) PK_SHAPE(1, 2) PK_SHAPE(1, 4)
+    PK_SHAPE(4, 1) PK_SHAPE(4, 2) PK_SHAPE(4, 4)
+    PK_SHAPE(8, 1) PK_SHAPE(8, 2) PK_SHAPE(8, 4)
+    PK_SHAPE(16, 1) PK_SHAPE(16, 2) PK_SHAPE(16, 4)
+    PK_SHAPE(32, 1) PK_SHAPE(32, 2)
+#undef PK_SHAPE
+    return stage_share<M, 16, 1>(false, lds);
+}
+
 template <class M>
 void launch_update(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp)
 {
@@ -1257,6 +1304,8 @@ void launch_spmv_plain(const DeviceView& v, const double* x, double* y)
 template void launch_dir<double>(const DeviceView&, const int*, int, unsigned long long*);
 template void launch_dir<float>(const DeviceView&, const int*, int, unsigned long long*);
 template void launch_spmv<double>(const DeviceView&, const int*, int, int, int, unsigned long long*);
+template size_t spmv_stage_bytes<double>(bool, int, int);
+template size_t spmv_stage_bytes<float>(bool, int, int);
 template void launch_spmv<float>(const DeviceView&, const int*, int, int, int, unsigned long long*);
 template void launch_update<double>(const DeviceView&, const int*, int, unsigned long long*);
 template void launch_update<float>(const DeviceView&, const int*, int, unsigned long long*);

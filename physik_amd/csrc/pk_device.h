@@ -562,6 +562,9 @@ void launch_zero_rows(hipStream_t stream, int n, const int* rows, double* crhs);
 // source is skipped).
 void launch_export_frame(hipStream_t stream, long long n3, const double* x, const double* v, float* out);
 template <class M> void launch_dir(const DeviceView& v, const int* kb, int k0, unsigned long long* stamp);
+// LDS bytes an SpMV workgroup of this shape may stage d in (pk_cg.hip).
+template <class M>
+size_t spmv_stage_bytes(bool upper, int lanesPerRow, int unroll);
 template <class M>
 void launch_spmv(const DeviceView& v, const int* kb, int k0, int lanesPerRow, int unroll,
                  unsigned long long* stamp);

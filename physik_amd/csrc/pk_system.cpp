@@ -165,6 +165,7 @@ struct pk_system
     // host copy of the SpMV tiling for the staging set-up
     std::vector<pk::RowTile> spmvHostTiles;
     int winCap = 0;
+    int stageCap = 0;  // rows the staging capacity holds (0: staging off)
     int stageSegs = 0;
     int stagingMode = 1;  // 0 off, 1 staging
     int* pinned = nullptr;
@@ -1206,12 +1207,10 @@ static int alloc_work(pk_system* s)
 
 static bool is_f32(const pk_system* s);
 
-// LDS an SpMV workgroup may take for its staged window of d: at the
-// kernels' 7 waves per SIMD, 7 workgroups share a CU's 160 KB.
-constexpr size_t kStageBytes = 20 * 1024;
-
 // Staging of the SpMV tiles (same order as build_tilings). Each tile's
-// columns span a window [first, last]; when every window fits kStageBytes the
+// columns span a window [first, last]; when every window fits the LDS share of
+// the SpMV kernel the system's shape selects (spmv_stage_bytes: as much as
+// leaves every workgroup its registers allow on a CU) the
 // tiles stage their windows (one segment, the cheapest lookup). Otherwise
 // each tile stages its distinct columns cut at the three widest gaps between
 // them into up to four segments (on a grid: the layers below, at and above
@@ -1222,10 +1221,14 @@ static int setup_windows(pk_system* s)
     const Topology& t = s->topo;
     s->winCap = 0;
     s->stageSegs = 0;
+    s->stageCap = 0;
     if (s->stagingMode == 0 || t.nd == 0)
         return PK_OK;
     const size_t bytesPerRow = 3 * (is_f32(s) ? sizeof(float) : sizeof(double));
-    const int cap = static_cast<int>(kStageBytes / bytesPerRow);
+    const size_t share = is_f32(s) ? pk::spmv_stage_bytes<float>(is_upper(s), s->lanesPerRow, s->unroll)
+                                   : pk::spmv_stage_bytes<double>(is_upper(s), s->lanesPerRow, s->unroll);
+    const int cap = static_cast<int>(share / bytesPerRow);
+    s->stageCap = cap;
     std::vector<pk::StageSegs> wins, segs;
     std::vector<int> cols;
     std::vector<std::pair<int, int>> gaps;  // (gap rows, index of the column before it)
@@ -1626,6 +1629,14 @@ int pk_system_get_spmv_staging(const pk_system* s, int* mode, int* rows, int* se
     if (mode) *mode = s->stagingMode;
     if (rows) *rows = s->winCap;
     if (segmented) *segmented = s->stageSegs;
+    return PK_OK;
+}
+
+int pk_system_get_spmv_stage_capacity(const pk_system* s, int* rows)
+{
+    if (!s || !rows)
+        return PK_ERR_ARG;
+    *rows = s->stageCap;
     return PK_OK;
 }
 

@@ -112,6 +112,12 @@ class System:
                 "pk_system_get_spmv_staging")
         return m.value, r.value, bool(g.value)
 
+    def stage_capacity(self):
+        """Rows of d an SpMV workgroup may stage (0 with staging off)."""
+        r = C.c_int()
+        N.check(self.lib.pk_system_get_spmv_stage_capacity(self.h, C.byref(r)), "pk_system_get_spmv_stage_capacity")
+        return r.value
+
     def get_tune(self):
         v = [C.c_int() for _ in range(4)]
         N.check(self.lib.pk_system_get_tune(self.h, *[C.byref(x) for x in v]), "pk_system_get_tune")

@@ -21,18 +21,16 @@ from launch_shapes import shape_for
 
 pytestmark = pytest.mark.gpu
 
-STAGE_BYTES = 20 * 1024  # the library's per-workgroup staging budget
-
-
 def mixed_batch(seed=21, bar=True):
-    """Two small boxes, whose tiles' windows fit, and (bar) a 700 x 2 x 2 cell
-    bar, whose tiles reference columns in clusters about 700 rows apart, so
-    that even their four segments need 2,200 rows or more (beyond the FP32
-    capacity of 1,706 rows and the FP64 one of 853)."""
+    """Two small boxes, whose tiles' windows fit, and (bar) a 1,400 x 3 x 3
+    cell bar, whose tiles reference columns in clusters 1,401 (y) and 5,604
+    (z) rows apart, so that even their four segments need about 7,000 rows
+    (beyond any capacity: at most a third of a CU's 160 KB, 4,437 FP32 rows;
+    the corner lines' tiles need fewer)."""
     rng = mg.Pcg32(seed)
     rests, tetss, fixeds, v0s, groups, body_nodes, body_tets = [], [], [], [], [], [], []
     n_off = t_off = 0
-    for b, (nx, ny, nz) in enumerate([(4, 3, 3), (8, 6, 5)] + ([(700, 2, 2)] if bar else [])):
+    for b, (nx, ny, nz) in enumerate([(4, 3, 3), (8, 6, 5)] + ([(1400, 3, 3)] if bar else [])):
         h = 0.05
         rest, tets = mg.box_mesh(nx, ny, nz, h, "kuhn")
         rest = mg.jitter(rest, h, 0.1, rng)
@@ -101,8 +99,8 @@ def test_staged_tiles_step_like_global_gathers(precision, shape, bar):
     a.set_staging(1)
     b.set_staging(0)
     got_mode, rows, segmented = a.get_staging()
-    f32 = precision in (N.PK_FP32_UPPER, N.PK_FP32_STORAGE)
-    cap = STAGE_BYTES // (3 * (4 if f32 else 8))
+    cap = a.stage_capacity()
+    assert cap > 0
     wins, segs = windows(w, shape[1])
     print(f"precision {precision} shape {shape} bar {bar}: staged rows {rows}, segmented {segmented}, "
           f"windows {wins.min()}..{wins.max()}, segments {segs.min()}..{segs.max()}", flush=True)
@@ -134,9 +132,9 @@ def test_staging_is_switchable():
     for precision in (N.PK_FP32_STORAGE, N.PK_FP32_UPPER):
         s = W.make_system(w, precision)
         mode, rows, segmented = s.get_staging()
-        assert mode == 1 and segmented and rows == 1706  # the bar's tiles need more than the capacity
+        assert mode == 1 and segmented and rows == s.stage_capacity()  # the bar's tiles need more than it
         s.set_staging(0)
-        assert s.get_staging() == (0, 0, False)
+        assert s.get_staging() == (0, 0, False) and s.stage_capacity() == 0
         s.set_staging(1)
         assert s.get_staging() == (1, rows, True)
         assert s.lib.pk_system_set_spmv_staging(s.h, 2) == N.PK_ERR_ARG
