@@ -178,11 +178,7 @@ __global__ __launch_bounds__(NT) void k_adj_rhs(TileView tv, const int* __restri
             b[i] = live ? vbar[3 * n + i] + xbar[3 * n + i] * dt : 0.0;
         if (!(fin(b[0]) && fin(b[1]) && fin(b[2])))
             flag |= 1;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            z[i] = static_cast<double>(minv[(3 * i) * N + row]) * b[0] +
-                   static_cast<double>(minv[(3 * i + 1) * N + row]) * b[1] +
-                   static_cast<double>(minv[(3 * i + 2) * N + row]) * b[2];
+        minv_apply(minv, N, row, b, z);
         if (!(fin(z[0]) && fin(z[1]) && fin(z[2])))
             flag |= 2;
         M zs[3];

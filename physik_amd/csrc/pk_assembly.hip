@@ -361,7 +361,10 @@ __device__ inline void element_block_from(const double* __restrict__ rec, const 
     const double r00 = rr0.x, r01 = rr0.y, r02 = rr1.x, r11 = rr1.y, r12 = rr2.x, r22 = rr2.y;
     const double RRt[9] = {r00, r01, r02, r01, r11, r12, r02, r12, r22};
     const double2 f = *reinterpret_cast<const double2*>(kd + KD_VLAM);
-    rotated_block_scaled(rga, rgb, RRt, f.x, f.y, kd[KD_VMUG + p], K);
+    if (a == b)
+        rotated_diag_block_scaled(rga, RRt, f.x, f.y, kd[KD_VMUG + p], K);
+    else
+        rotated_block_scaled(rga, rgb, RRt, f.x, f.y, kd[KD_VMUG + p], K);
 }
 
 // Block R K_ab R^T of local pair p of tet t from the basis and stiffness records.
@@ -601,15 +604,13 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, int nd, const int* __
             }
             if (pok)
             {
-                inv3(a, mi);
+                inv3_sym(a, mi);  // the block is symmetric bit for bit
                 pok = fin9(mi);
             }
             if (!pok)
                 failed = 1;  // BuildInversePreconditioner fails the solve
         }
-#pragma unroll
-        for (int e = 0; e < 9; ++e)
-            minv[static_cast<size_t>(e) * nd + row] = static_cast<M>(mi[e]);
+        minv_store(minv, static_cast<size_t>(nd), row, mi);
     }
     // the row's off-diagonal slots (sd + 1 .. se - 1), sub-group lane / 4 on
     // slots sd + 1 + sub, sd + 5 + sub, ...
@@ -740,7 +741,7 @@ __device__ inline void form_pair_blocks(const PairLane& pl, const PairRecords& R
     for (int i = 0; i < 3; ++i)
         wa[i] = pick4(a, R.w[i], R.w[3 + i], R.w[6 + i], R.w[9 + i]);
     double K[9];
-    rotated_block_scaled(wa, wa, RRt, R.vl, R.vm, pick4(a, R.vg[0], R.vg[1], R.vg[2], R.vg[3]), K);
+    rotated_diag_block_scaled(wa, RRt, R.vl, R.vm, pick4(a, R.vg[0], R.vg[1], R.vg[2], R.vg[3]), K);
 #pragma unroll
     for (int e = 0; e < 9; ++e)
         P[e * kPairChunk + tid] = K[e];
@@ -956,15 +957,13 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
                 }
                 if (pok)
                 {
-                    inv3(A, mi);
+                    inv3_sym(A, mi);  // the block is symmetric bit for bit
                     pok = fin9(mi);
                 }
                 if (!pok)
                     gs[gr].failed = 1;  // BuildInversePreconditioner fails the solve
             }
-#pragma unroll
-            for (int e = 0; e < 9; ++e)
-                minv[static_cast<size_t>(e) * nd + r0w + tid] = static_cast<M>(mi[e]);
+            minv_store(minv, static_cast<size_t>(nd), r0w + tid, mi);
         }
         c0 = c1;
         c1 = c2;
@@ -980,17 +979,6 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
 
 // Lanes per dynamic row in the right-hand side gather (a node has ~20 corners).
 constexpr int kRowLanes = 4;
-
-// z = M^-1 r with the block-Jacobi inverse stored as 9 planes of nd values.
-template <class M>
-__device__ inline void apply_minv(const M* __restrict__ minv, size_t nd, int row, const double* r, double* z)
-{
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-        z[i] = static_cast<double>(minv[(3 * i) * nd + row]) * r[0] +
-               static_cast<double>(minv[(3 * i + 1) * nd + row]) * r[1] +
-               static_cast<double>(minv[(3 * i + 2) * nd + row]) * r[2];
-}
 
 // b = dt (f + m g) - dt^2 K v per dynamic row from its corner entries (each
 // dt f - dt^2 (K v) of one tet, k_element, added in tet order), then the CG
@@ -1051,7 +1039,7 @@ __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__
             b[i] = e3[i] + (gv[i] * m) * dt + (crhs ? crhs[3 * row + i] : 0.0);
         if (!(fin(b[0]) && fin(b[1]) && fin(b[2])))
             flag |= 1;
-        apply_minv(minv, static_cast<size_t>(nd), row, b, z);
+        minv_apply(minv, static_cast<size_t>(nd), row, b, z);
         if (!(fin(z[0]) && fin(z[1]) && fin(z[2])))
             flag |= 2;
         M zs[3];

@@ -808,7 +808,7 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
     constexpr int kPre = 2;
     const size_t N = static_cast<size_t>(nd);
     const int row0 = tl.rowBegin + static_cast<int>(threadIdx.x);
-    M qpre[kPre][3], mpre[kPre][9];
+    M qpre[kPre][3], mpre[kPre][kMinvPlanes];
     double rpre[kPre][3];
 #pragma unroll
     for (int j = 0; j < kPre; ++j)
@@ -822,7 +822,7 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
             rpre[j][i] = rv[3 * rr + i];
         }
 #pragma unroll
-        for (int e = 0; e < 9; ++e)
+        for (int e = 0; e < kMinvPlanes; ++e)
             mpre[j][e] = minv[e * N + rr];
     }
     const int g = tl.group;
@@ -860,8 +860,10 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
         s.alpha[k & 1] = alpha;
     double pz = 0.0, pr = 0.0;
     int flag = 0;
-    // one row: r -= alpha q, z = M^-1 r, and the partials
-    auto one = [&](int row, const M* qq, const double* rq, const M* mm) {
+    // one row: r -= alpha q, z = M^-1 r (upper triangle m of the inverse),
+    // and the partials
+    auto one = [&](int row, const M* qq, const double* rq, const M* m) {
+        const M mm[9] = {m[0], m[1], m[2], m[1], m[3], m[4], m[2], m[4], m[5]};
         double r[3], z[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -891,7 +893,7 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
             one(row0 + j * NT, qpre[j], rpre[j], mpre[j]);
     for (int row = row0 + kPre * NT; row < tl.rowEnd; row += NT)
     {
-        M qq[3], mm[9];
+        M qq[3], mm[kMinvPlanes];
         double rq[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -900,7 +902,7 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
             rq[i] = rv[3 * row + i];
         }
 #pragma unroll
-        for (int e = 0; e < 9; ++e)
+        for (int e = 0; e < kMinvPlanes; ++e)
             mm[e] = minv[e * N + row];
         one(row, qq, rq, mm);
     }

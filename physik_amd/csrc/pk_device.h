@@ -403,6 +403,65 @@ __device__ inline void inv3(const double* m, double* o)
     o[6] = r2x * id; o[7] = r2y * id; o[8] = r2z * id;
 }
 
+// Inverse of a symmetric 3x3 block from its upper triangle, symmetric bit for
+// bit (each mirrored entry is a copy): the cofactors as explicit fused
+// multiply-adds, so every kernel it is inlined into rounds alike, and the
+// Identity below |det| <= 1e-6 as inv3 (Mat3.h:57-70).
+__device__ inline void inv3_sym(const double* m, double* o)
+{
+    const double a00 = m[0], a01 = m[1], a02 = m[2], a11 = m[4], a12 = m[5], a22 = m[8];
+    const double c00 = __builtin_fma(a11, a22, -(a12 * a12));
+    const double c01 = __builtin_fma(a12, a02, -(a01 * a22));
+    const double c02 = __builtin_fma(a01, a12, -(a11 * a02));
+    const double c11 = __builtin_fma(a22, a00, -(a02 * a02));
+    const double c12 = __builtin_fma(a02, a01, -(a12 * a00));
+    const double c22 = __builtin_fma(a00, a11, -(a01 * a01));
+    const double d = __builtin_fma(a00, c00, __builtin_fma(a01, c01, a02 * c02));
+    if (fabs(d) <= static_cast<double>(0.000001f))
+    {
+        o[0] = 1.0; o[1] = 0.0; o[2] = 0.0;
+        o[3] = 0.0; o[4] = 1.0; o[5] = 0.0;
+        o[6] = 0.0; o[7] = 0.0; o[8] = 1.0;
+        return;
+    }
+    const double id = 1.0 / d;
+    o[0] = c00 * id;
+    o[1] = o[3] = c01 * id;
+    o[2] = o[6] = c02 * id;
+    o[4] = c11 * id;
+    o[5] = o[7] = c12 * id;
+    o[8] = c22 * id;
+}
+
+// The block-Jacobi inverse of the CG kept as its upper triangle: 6 planes of
+// nd values, entries (0,0), (0,1), (0,2), (1,1), (1,2), (2,2).
+constexpr int kMinvPlanes = 6;
+
+// z = M^-1 r for row `row` of the 6-plane inverse.
+template <class M>
+__device__ inline void minv_apply(const M* __restrict__ minv, size_t nd, int row, const double* r, double* z)
+{
+    double m[kMinvPlanes];
+#pragma unroll
+    for (int p = 0; p < kMinvPlanes; ++p)
+        m[p] = static_cast<double>(minv[p * nd + row]);
+    z[0] = m[0] * r[0] + m[1] * r[1] + m[2] * r[2];
+    z[1] = m[1] * r[0] + m[3] * r[1] + m[4] * r[2];
+    z[2] = m[2] * r[0] + m[4] * r[1] + m[5] * r[2];
+}
+
+// Stores the block-Jacobi inverse mi (row-major, symmetric) of row `row`.
+template <class M>
+__device__ inline void minv_store(M* __restrict__ minv, size_t nd, int row, const double* mi)
+{
+    minv[0 * nd + row] = static_cast<M>(mi[0]);
+    minv[1 * nd + row] = static_cast<M>(mi[1]);
+    minv[2 * nd + row] = static_cast<M>(mi[2]);
+    minv[3 * nd + row] = static_cast<M>(mi[4]);
+    minv[4 * nd + row] = static_cast<M>(mi[5]);
+    minv[5 * nd + row] = static_cast<M>(mi[8]);
+}
+
 __device__ inline bool fin9(const double* m)
 {
     bool ok = true;
@@ -465,7 +524,7 @@ struct DeviceView
     double* corner;      // forces and K v in row order
     double* adj;         // reverse-mode corner terms, N_ADJ per alist entry
     const int* apos;     // 4 T: alist index of corner a of tet t, or -1
-    void* minv;   // M: block-Jacobi inverse, 9 planes of nd
+    void* minv;   // M: block-Jacobi inverse, kMinvPlanes planes of nd (upper triangle)
     double* xs;   // CG solution dv
     double* r;
     void* vals;   // M

@@ -124,5 +124,23 @@ __device__ inline void rotated_block_scaled(const double* rga, const double* rgb
         }
 }
 
+// R K_aa R^T of one corner, symmetric bit for bit: with w = R g_a the block is
+// V lambda w w^T + V mu w w^T + V mu (g_a . g_a) R R^T, and each product
+// w_i w_j is formed once for (i, j) and (j, i) (RRt is symmetric too), so the
+// diagonal blocks of A, their sums and their block-Jacobi inverses are
+// symmetric and the inverse can be kept as its upper triangle.
+__device__ inline void rotated_diag_block_scaled(const double* w, const double* RRt, double vl, double vm, double vmg,
+                                                 double* K)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            const double t = w[i < j ? i : j] * w[i < j ? j : i];
+            K[3 * i + j] = __builtin_fma(vmg, RRt[3 * i + j], __builtin_fma(vl, t, vm * t));
+        }
+}
+
 
 }  // namespace pk

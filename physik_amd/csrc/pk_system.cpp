@@ -1196,7 +1196,7 @@ static int alloc_work(pk_system* s)
     if ((rc = s->d2.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->d3.alloc(nd3 * es)) != PK_OK) return rc;
     if ((rc = s->q.alloc(nd3 * es)) != PK_OK) return rc;
-    if ((rc = s->minv.alloc(static_cast<size_t>(t.nd) * 9 * es)) != PK_OK) return rc;
+    if ((rc = s->minv.alloc(static_cast<size_t>(t.nd) * pk::kMinvPlanes * es)) != PK_OK) return rc;
     if ((rc = s->xs.alloc(nd3)) != PK_OK) return rc;
     if ((rc = s->r.alloc(nd3)) != PK_OK) return rc;
     if ((rc = s->active.alloc(4)) != PK_OK) return rc;
@@ -2790,12 +2790,14 @@ int pk_system_get_matrix(pk_system* s, double* full_vals, double* rhs, double* m
     if (minv && t.nd)
     {
         const bool f32 = is_f32(s);
-        std::vector<char> raw(static_cast<size_t>(t.nd) * 9 * (f32 ? 4 : 8));
+        std::vector<char> raw(static_cast<size_t>(t.nd) * pk::kMinvPlanes * (f32 ? 4 : 8));
         PK_HIP(hipMemcpy(raw.data(), s->minv.p, raw.size(), hipMemcpyDeviceToHost));
+        // the device keeps the upper triangle (the blocks are symmetric)
+        constexpr int plane[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
         for (int r = 0; r < t.nd; ++r)
             for (int e = 0; e < 9; ++e)
             {
-                const size_t i = static_cast<size_t>(e) * t.nd + r;
+                const size_t i = static_cast<size_t>(plane[e]) * t.nd + r;
                 minv[9 * r + e] = f32 ? reinterpret_cast<const float*>(raw.data())[i]
                                       : reinterpret_cast<const double*>(raw.data())[i];
             }
