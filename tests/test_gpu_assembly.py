@@ -133,7 +133,7 @@ def test_matrix_is_the_tet_order_sum_of_the_element_blocks(mesh, precision):
     arithmetic; the pair gather and, for fan150, the row gather add them in
     tet order). The three entries of a diagonal block that carry the mass
     are left out (the lumped masses are not read back); mirrors are the
-    transposes."""
+    transposes, and diagonal blocks their own."""
     # fans (pair gather; fan150 takes the row gather), a 5-tet box at rest,
     # a jittered Kuhn box twisted by pi/2
     w = fan(int(mesh[3:])) if mesh.startswith("fan") else W.beam10() if mesh == "beam10" else W.twist64(n=8)
@@ -153,6 +153,8 @@ def test_matrix_is_the_tet_order_sum_of_the_element_blocks(mesh, precision):
         expect = sums[k] * dt2
         if r == c:
             assert np.array_equal(vals[k][off], expect[off]), f"diagonal block of row {r}"
+            # symmetric bit for bit: the inverse is kept as its upper triangle
+            assert np.array_equal(vals[k], vals[k].T), f"diagonal block of row {r} not symmetric"
         else:
             assert np.array_equal(vals[k], expect), f"block ({r}, {c})"
             assert np.array_equal(vals[index[(c, r)]], expect.T), f"mirror of ({r}, {c})"
