@@ -7,13 +7,14 @@
 //                    dt^2 (K v)_a (or f_a explicit), written in row-list order
 //                    (FEMModel.cpp:614-679, :377-424, :559-612, :115-163)
 //   k_stiff_records  one lane per tet: V lambda, V mu, V mu g_a.g_b
-//   k_assemble_pairs the slot gather: a workgroup per chunk of consecutive
-//                    rows, one lane per (row, tet) pair forming the pair's
-//                    blocks from the basis and stiffness records into LDS,
-//                    each slot's contributions then added in tet order (the
-//                    reference's order, no atomics); A = M + dt^2 K into full
-//                    or sliced upper storage and the block-Jacobi inverse
-//                    (SolverData.cpp:308-465, :260-306)
+//   k_assemble_pairs the slot gather: resident workgroups walking chunks of
+//                    consecutive rows, one lane per (row, tet) pair forming
+//                    the pair's blocks from the basis and stiffness records
+//                    into LDS, then one thread per slot adding its
+//                    contributions in tet order (the reference's order, no
+//                    atomics); A = M + dt^2 K into full or sliced upper
+//                    storage and the block-Jacobi inverse (SolverData.cpp:
+//                    308-465, :260-306)
 //   k_assemble       the same by 16 lanes per dynamic row (the diagonal block
 //                    over the row's corner list, then its off-diagonal upper
 //                    slots four at a time, blocks added in tet order by
@@ -636,13 +637,13 @@ __global__ __launch_bounds__(NT) void k_assemble(int nnzu, int nd, const int* __
 // ones at their place in the chunk's run of the off-diagonal lists (pairOff,
 // three 10-bit fields, 0x3ff for none); the other lookups go to LDS as well:
 // each slot's run descriptor (pack_run) and element, each row's mass, body
-// and preconditioner switch. Phase 2, one thread per (slot, component): the
-// slot's contributions summed in list order, which is tet order (the order
-// the reference adds them, SolverData.cpp:407-450), then the contacts, dt^2
-// and the mass as in k_assemble, written to full or sliced upper storage.
-// Phase 3, one thread per row: the block-Jacobi inverse (SolverData.cpp:
-// 260-306). A non-finite block or a refused inverse marks the body's solve
-// failed.
+// and preconditioner switch. Phase 2, one thread per slot: its 9 components'
+// contributions summed in list order, which is tet order (the order the
+// reference adds them, SolverData.cpp:407-450), then the contacts, dt^2 and
+// the mass as in k_assemble, written to full or sliced upper storage. Phase
+// 3, one thread per row: the block-Jacobi inverse (SolverData.cpp:260-306),
+// kept as its upper triangle. A non-finite block or a refused inverse marks
+// the body's solve failed.
 // The loads run ahead of the arithmetic, so that a chunk's three phases
 // overlap the dependent loads (descriptor -> pair -> records) of the next
 // ones: chunk descriptors three chunks ahead (scalar loads), each lane's pair
