@@ -189,7 +189,11 @@ __device__ inline void dir_body(int k, const TileView& vt, const M* __restrict__
         return;
     // the thread's first kPre values of z, d_{k-1} (and at even k d_{k-2}
     // and x; a 512-row tile whole) are in flight while the stop flag and the
-    // partials are read
+    // partials are read. x (like r in k_update) is stored non-temporally: it
+    // is read again two iterations later, after the SpMV has streamed the
+    // matrix through the L2s, so it need not stay dirty in an L2 that the
+    // end of the launch has to write back (q, z and d are read by the very
+    // next launch on the same rows and keep the L2)
     constexpr int kPre = 6;
     const int i0 = 3 * tl.rowBegin + static_cast<int>(threadIdx.x), iEnd = 3 * tl.rowEnd;
     const bool two = EVEN && k > 0;
@@ -232,10 +236,11 @@ __device__ inline void dir_body(int k, const TileView& vt, const M* __restrict__
         if (i < iEnd)
         {
             if (EVEN)
-                xs[i] = __builtin_fma(static_cast<double>(dp[j]), a1,
-                                      __builtin_fma(static_cast<double>(ep[j]), a2, xp[j]));
+                __builtin_nontemporal_store(
+                    __builtin_fma(static_cast<double>(dp[j]), a1, __builtin_fma(static_cast<double>(ep[j]), a2, xp[j])),
+                    &xs[i]);
             else if (stop)
-                xs[i] = __builtin_fma(static_cast<double>(dp[j]), a1, xs[i]);
+                __builtin_nontemporal_store(__builtin_fma(static_cast<double>(dp[j]), a1, xs[i]), &xs[i]);
             if (!stop)
                 dcur[i] = direction(zp[j], dp[j], beta);
         }
@@ -244,9 +249,11 @@ __device__ inline void dir_body(int k, const TileView& vt, const M* __restrict__
     {
         const M dv = dprev[i];
         if (EVEN)
-            xs[i] = __builtin_fma(static_cast<double>(dv), a1, __builtin_fma(static_cast<double>(dprev2[i]), a2, xs[i]));
+            __builtin_nontemporal_store(
+                __builtin_fma(static_cast<double>(dv), a1, __builtin_fma(static_cast<double>(dprev2[i]), a2, xs[i])),
+                &xs[i]);
         else if (stop)
-            xs[i] = __builtin_fma(static_cast<double>(dv), a1, xs[i]);
+            __builtin_nontemporal_store(__builtin_fma(static_cast<double>(dv), a1, xs[i]), &xs[i]);
         if (!stop)
             dcur[i] = direction(z[i], dv, beta);
     }
@@ -846,7 +853,7 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
         {
             const double a1 = s.alpha[(k - 1) & 1];
             for (int i = 3 * tl.rowBegin + static_cast<int>(threadIdx.x); i < 3 * tl.rowEnd; i += NT)
-                xs[i] = __builtin_fma(static_cast<double>(dprev[i]), a1, xs[i]);
+                __builtin_nontemporal_store(__builtin_fma(static_cast<double>(dprev[i]), a1, xs[i]), &xs[i]);
         }
         double rz, rr;
         int fl;
@@ -869,7 +876,7 @@ __device__ inline void update_body(int k, const TileView& vt, const TileView& st
         for (int i = 0; i < 3; ++i)
         {
             r[i] = rq[i] + static_cast<double>(qq[i]) * (-alpha);
-            rv[3 * row + i] = r[i];
+            __builtin_nontemporal_store(r[i], &rv[3 * row + i]);
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i)
