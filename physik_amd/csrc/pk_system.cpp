@@ -182,10 +182,12 @@ struct pk_system
     unsigned long long stateVer = 0, frameVer = ~0ull;
 
     // reverse mode: checkpoints (x, v) every `tapeEvery` substeps of a
-    // recorded run, the states of the segment being reversed, per-body
-    // integrate flags of that segment, the adjoints and the gradient
+    // recorded run and after its last substep, the per-body integrate flags
+    // of each segment's last substep, the states of the segment being
+    // reversed, per-body integrate flags of that segment, the adjoints and
+    // the gradient
     pk::DevBuf<double> tapeX, tapeV, segX, segV, xbar, vbar, gradE, xEnd, vEnd;
-    pk::DevBuf<int> segOk;
+    pk::DevBuf<int> tapeOk, segOk;
     // the matrices of the segment being reversed (A and M^-1 of every
     // substep, and the assembly's rejection flags), kept from its replay when
     // they fit storeCap bytes, so the reverse step need not assemble again
@@ -1930,6 +1932,13 @@ static int dcopy(pk_system* s, double* dst, const double* src, size_t n)
     return PK_OK;
 }
 
+static int icopy(pk_system* s, int* dst, const int* src, size_t n)
+{
+    if (n)
+        PK_HIP(hipMemcpyAsync(dst, src, n * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
+    return PK_OK;
+}
+
 // One forward substep of a replay: the plain substep, then the per-body
 // integrate flags into `ok`.
 template <class M>
@@ -1984,23 +1993,30 @@ template <class M>
 static int record_run(pk_system* s, double dt, int steps, int every)
 {
     const size_t n3 = static_cast<size_t>(s->topo.N) * 3;
-    const int nck = steps / every + 1;
+    const size_t G = static_cast<size_t>(std::max(1, s->topo.G));
+    const int nseg = (steps + every - 1) / every;
     int rc;
     s->tapeValid = false;
-    if ((rc = s->tapeX.ensure(nck * n3)) != PK_OK) return rc;
-    if ((rc = s->tapeV.ensure(nck * n3)) != PK_OK) return rc;
-    if ((rc = s->segOk.ensure(static_cast<size_t>(every) * std::max(1, s->topo.G))) != PK_OK) return rc;
+    // checkpoint c: the state before substep c * every (c < nseg), and the
+    // end state (c = nseg)
+    if ((rc = s->tapeX.ensure((nseg + 1) * n3)) != PK_OK) return rc;
+    if ((rc = s->tapeV.ensure((nseg + 1) * n3)) != PK_OK) return rc;
+    if ((rc = s->tapeOk.ensure(nseg * G)) != PK_OK) return rc;
+    if ((rc = s->segOk.ensure(static_cast<size_t>(every) * G)) != PK_OK) return rc;
     if ((rc = dcopy(s, s->tapeX.p, s->x.p, n3)) != PK_OK) return rc;
     if ((rc = dcopy(s, s->tapeV.p, s->v.p, n3)) != PK_OK) return rc;
     for (int n = 0; n < steps; ++n)
     {
         if ((rc = replay_substep<M>(s, dt, s->segOk.p)) != PK_OK)
             return rc;
-        if ((n + 1) % every == 0)
+        if ((n + 1) % every == 0 || n + 1 == steps)
         {
-            const size_t c = static_cast<size_t>((n + 1) / every);
-            if ((rc = dcopy(s, s->tapeX.p + c * n3, s->x.p, n3)) != PK_OK) return rc;
-            if ((rc = dcopy(s, s->tapeV.p + c * n3, s->v.p, n3)) != PK_OK) return rc;
+            // a segment's last substep: its integrate flags, and the state
+            // after it (the next segment's start or the end state)
+            const size_t c = static_cast<size_t>(n / every);
+            if ((rc = icopy(s, s->tapeOk.p + c * G, s->segOk.p, G)) != PK_OK) return rc;
+            if ((rc = dcopy(s, s->tapeX.p + (c + 1) * n3, s->x.p, n3)) != PK_OK) return rc;
+            if ((rc = dcopy(s, s->tapeV.p + (c + 1) * n3, s->v.p, n3)) != PK_OK) return rc;
         }
     }
     PK_HIP(hipStreamSynchronize(s->stream));
@@ -2071,7 +2087,7 @@ static int backward_run(pk_system* s, double adjTol, int adjMaxIt, int* adjIters
         if ((rc = dcopy(s, s->v.p, s->tapeV.p + seg * n3, n3)) != PK_OK) return rc;
         if ((rc = dcopy(s, s->segX.p, s->x.p, n3)) != PK_OK) return rc;
         if ((rc = dcopy(s, s->segV.p, s->v.p, n3)) != PK_OK) return rc;
-        for (int j = 0; j < len; ++j)
+        for (int j = 0; j + 1 < len; ++j)
         {
             slot(j);
             rc = replay_substep<M>(s, dt, s->segOk.p + static_cast<size_t>(j) * t.G,
@@ -2081,6 +2097,31 @@ static int backward_run(pk_system* s, double adjTol, int adjMaxIt, int* adjIters
                 return rc;
             if ((rc = dcopy(s, s->segX.p + (j + 1) * n3, s->x.p, n3)) != PK_OK) return rc;
             if ((rc = dcopy(s, s->segV.p + (j + 1) * n3, s->v.p, n3)) != PK_OK) return rc;
+        }
+        {
+            // the segment's last substep is not solved again: the record run
+            // kept the state after it (the next checkpoint) and its integrate
+            // flags; only its matrix is needed, assembled here at its start
+            // state when the segment's matrices are kept (otherwise the
+            // reverse step assembles it)
+            const int j = len - 1;
+            const size_t G = static_cast<size_t>(std::max(1, t.G));
+            if ((rc = icopy(s, s->segOk.p + static_cast<size_t>(j) * G, s->tapeOk.p + seg * G, G)) != PK_OK)
+                return rc;
+            if ((rc = dcopy(s, s->segX.p + (j + 1) * n3, s->tapeX.p + (seg + 1) * n3, n3)) != PK_OK) return rc;
+            if ((rc = dcopy(s, s->segV.p + (j + 1) * n3, s->tapeV.p + (seg + 1) * n3, n3)) != PK_OK) return rc;
+            if (store)
+            {
+                if ((rc = dcopy(s, s->x.p, s->segX.p + j * n3, n3)) != PK_OK) return rc;
+                if ((rc = dcopy(s, s->v.p, s->segV.p + j * n3, n3)) != PK_OK) return rc;
+                slot(j);
+                rc = run_assembly<M>(s, dt, s->tol, s->maxIt, kAsmMatrix);
+                if (rc == PK_OK && t.nd > 0)
+                    launch_save_failed(s->view(), s->segFailed.p + static_cast<size_t>(j) * t.G);
+                s->valsView = s->minvView = nullptr;
+                if (rc != PK_OK)
+                    return rc;
+            }
         }
         for (int j = len - 1; j >= 0; --j)
         {
