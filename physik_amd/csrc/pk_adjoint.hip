@@ -186,8 +186,9 @@ __global__ __launch_bounds__(NT) void k_adj_rhs(TileView tv, const int* __restri
         for (int i = 0; i < 3; ++i)
         {
             zs[i] = static_cast<M>(z[i]);
-            xs[3 * row + i] = 0.0;
-            rv[3 * row + i] = b[i];
+            // x and r are read again after a whole SpMV: past the L2 (as in the CG)
+            __builtin_nontemporal_store(0.0, &xs[3 * row + i]);
+            __builtin_nontemporal_store(b[i], &rv[3 * row + i]);
             zv[3 * row + i] = zs[i];
             d0[3 * row + i] = zs[i];
         }

@@ -297,8 +297,14 @@ __global__ __launch_bounds__(NT) void k_element(int T, const int4* __restrict__ 
     __syncthreads();
     const int nrec = min(NT, T - t0);
     double2* dst = reinterpret_cast<double2*>(basis + static_cast<size_t>(t0) * EB_COUNT);
+    // whole lines, read only by the next kernel: stored past the L2, which
+    // the end of the launch would otherwise have to write back
     for (int i = threadIdx.x; i < nrec * (EB_COUNT / 2); i += NT)
-        dst[i] = stage[i];
+    {
+        const double2 u = stage[i];
+        __builtin_nontemporal_store(u.x, &dst[i].x);
+        __builtin_nontemporal_store(u.y, &dst[i].y);
+    }
 }
 
 // g_a . g_b of the 10 local pairs into the parameter planes (rest data, once
@@ -1048,8 +1054,9 @@ __global__ __launch_bounds__(NT) void k_rhs(TileView tv, const int* __restrict__
         for (int i = 0; i < 3; ++i)
         {
             zs[i] = static_cast<M>(z[i]);
-            xs[3 * row + i] = 0.0;
-            rv[3 * row + i] = b[i];
+            // x and r are read again after a whole SpMV: past the L2 (as in the CG)
+            __builtin_nontemporal_store(0.0, &xs[3 * row + i]);
+            __builtin_nontemporal_store(b[i], &rv[3 * row + i]);
             zv[3 * row + i] = zs[i];
             d0[3 * row + i] = zs[i];
         }
