@@ -151,6 +151,13 @@ int pk_system_set_spmv_staging(pk_system* sys, int mode);
 int pk_system_get_spmv_staging(const pk_system* sys, int* mode, int* rows, int* segmented);
 int pk_system_get_spmv_stage_capacity(const pk_system* sys, int* rows);
 
+/* Sliced upper-only storage layout (no reference counterpart): the slices
+ * of 64 rows, how many of them are stored compactly (a position's blocks
+ * back to back in lane order, for slices whose ELL positions are less than
+ * 97 % filled; the others keep one entry per lane and position), and the
+ * elements held, padding included. All zero under full storage. */
+int pk_system_get_slice_layout(const pk_system* sys, int* slices, int* compact_slices, long long* elements);
+
 /* Node state, node_count x 3 each; either pointer may be NULL. */
 int pk_system_set_state(pk_system* sys, const double* x, const double* v);
 int pk_system_get_state(pk_system* sys, double* x, double* v);

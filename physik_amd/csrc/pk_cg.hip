@@ -595,6 +595,13 @@ __device__ inline void load_element(const double* __restrict__ rec, size_t nE, i
 
 __device__ inline bool lane_bit(unsigned long long m, int lane) { return ((m >> lane) & 1ull) != 0; }
 
+// Number of lanes of the wave below this one whose bit is set in m.
+__device__ inline int lanes_below(unsigned long long m)
+{
+    return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(m >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(m), 0u)));
+}
+
 // One slice (64 rows, one lane per row) of sliced upper-only storage. Round
 // i multiplies the row's upper block at position i (i = 0: the diagonal)
 // and its lower block at position i - 1. The lower ELL runs down from the
@@ -612,57 +619,69 @@ __device__ inline bool lane_bit(unsigned long long m, int lane) { return ((m >> 
 // same way, so the two storages agree bit for bit. Every load of a wave
 // reads consecutive entries of one plane, or in aligned slices the mirror
 // records of consecutive rows. Returns the lane's d.q contribution.
-template <class M, bool ALIGNED, int U, class Dir>
+template <class M, bool ALIGNED, bool COMPACT, int U, class Dir>
 __device__ __attribute__((always_inline)) inline double slice_body(const SliceView& sv, const int4& ds, int row,
                                                                    int lane, bool live, const M* __restrict__ rec,
                                                                    size_t nE, const Dir& d, M* __restrict__ q)
 {
-    constexpr int W = kSliceRows;
-    const int wu = ds.z & 0xffff, wl = (ds.z >> 16) & 0x7fff;
+    const int wu = ds.z & 0xffff, wl = (ds.z >> 16) & 0x3fff;
     const int rounds = wu > wl + 1 ? wu : wl + 1;
     const int* __restrict__ off = sv.off + ds.w;
     const unsigned long long* __restrict__ mask = sv.mask + ds.w;
     double l0 = 0.0, l1 = 0.0, l2 = 0.0, u0 = 0.0, u1 = 0.0, u2 = 0.0;
-    // what rounds i0 .. i0 + U - 1 need before their values: lane masks,
-    // mirror elements and (packed slices) columns
+    // compact slices: first entries of the next rounds' upper and lower
+    // positions (a position's entries follow those of the positions before
+    // it); otherwise position k of lane l is entry base + 64 k + l
+    int ub = ds.x, lb = ds.y;
+    // what round i needs before its values: whether the lane has an upper
+    // and a lower block, its upper element, the mirror element of its lower
+    // block and (packed slices) their columns
+    auto fetch = [&](int i, bool& h_u, bool& h_l, int& e_u, int& e_m, int& c_u, int& c_l) {
+        const unsigned long long mu = i < wu ? mask[i] : 0ull;
+        const unsigned long long ml = (i >= 1 && i <= wl) ? mask[wu + i - 1] : 0ull;
+        h_u = lane_bit(mu, lane);
+        h_l = lane_bit(ml, lane);
+        int li;
+        if constexpr (COMPACT)
+        {
+            e_u = ub + lanes_below(mu);
+            li = lb + lanes_below(ml);
+            ub += __popcll(mu);
+            lb += __popcll(ml);
+        }
+        else
+        {
+            e_u = ds.x + kSliceRows * i + lane;
+            li = ds.y + kSliceRows * (i - 1) + lane;
+        }
+        e_m = 0;
+        c_u = 0;
+        c_l = 0;
+        if (h_l)
+            e_m = sv.mirE[li];
+        if (!ALIGNED && h_u)
+            c_u = sv.ecol[e_u];
+        if (!ALIGNED && h_l)
+            c_l = sv.mirC[li];
+    };
     bool hu[U], hl[U];
-    int em[U], cu[U], cl[U];
+    int eu[U], em[U], cu[U], cl[U];
 #pragma unroll
     for (int u = 0; u < U; ++u)
-    {
-        const int i = u;
-        hu[u] = i < wu && lane_bit(mask[i], lane);
-        hl[u] = i >= 1 && i <= wl && lane_bit(mask[wu + i - 1], lane);
-        em[u] = hl[u] ? sv.mirE[ds.y + W * (i - 1) + lane] : 0;
-        cu[u] = (!ALIGNED && hu[u]) ? sv.ecol[ds.x + W * i + lane] : 0;
-        cl[u] = (!ALIGNED && hl[u]) ? sv.mirC[ds.y + W * (i - 1) + lane] : 0;
-    }
+        fetch(u, hu[u], hl[u], eu[u], em[u], cu[u], cl[u]);
     for (int i0 = 0; i0 < rounds; i0 += U)
     {
         bool hu1[U], hl1[U];
-        int em1[U], cu1[U], cl1[U];
+        int eu1[U], em1[U], cu1[U], cl1[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
-        {
-            const int i = i0 + U + u;
-            hu1[u] = i < wu && lane_bit(mask[i], lane);
-            hl1[u] = i >= 1 && i <= wl && lane_bit(mask[wu + i - 1], lane);
-            em1[u] = 0;
-            cu1[u] = 0;
-            cl1[u] = 0;
-            if (hl1[u])
-                em1[u] = sv.mirE[ds.y + W * (i - 1) + lane];
-            if (!ALIGNED && hu1[u])
-                cu1[u] = sv.ecol[ds.x + W * i + lane];
-            if (!ALIGNED && hl1[u])
-                cl1[u] = sv.mirC[ds.y + W * (i - 1) + lane];
-        }
+            fetch(i0 + U + u, hu1[u], hl1[u], eu1[u], em1[u], cu1[u], cl1[u]);
         M a[U][9], b[U][9];
 #pragma unroll
         for (int u = 0; u < U; ++u)
         {
             if (hu[u])
-                load_element(rec, nE, ds.x + W * (i0 + u) + lane, a[u]);
+                load_element(rec, nE, eu[u], a[u]);
             if (hl[u])
                 load_element(rec, nE, em[u], b[u]);
         }
@@ -698,6 +717,7 @@ __device__ __attribute__((always_inline)) inline double slice_body(const SliceVi
         {
             hu[u] = hu1[u];
             hl[u] = hl1[u];
+            eu[u] = eu1[u];
             em[u] = em1[u];
             cu[u] = cu1[u];
             cl[u] = cl1[u];
@@ -732,10 +752,18 @@ __device__ __attribute__((always_inline)) inline double slice_rows(const RowTile
         const int4 ds = sv.slices[sl];
         const int row = tl.rowBegin + W * j + lane;
         const bool live = row < tl.rowEnd;
-        if (ds.z < 0)  // aligned (uniform)
-            pd += slice_body<M, true, U>(sv, ds, row, lane, live, rec, nE, d, q);
+        const bool compact = ((ds.z >> 30) & 1) != 0;  // uniform, like aligned
+        if (ds.z < 0)
+        {
+            if (compact)
+                pd += slice_body<M, true, true, U>(sv, ds, row, lane, live, rec, nE, d, q);
+            else
+                pd += slice_body<M, true, false, U>(sv, ds, row, lane, live, rec, nE, d, q);
+        }
+        else if (compact)
+            pd += slice_body<M, false, true, U>(sv, ds, row, lane, live, rec, nE, d, q);
         else
-            pd += slice_body<M, false, U>(sv, ds, row, lane, live, rec, nE, d, q);
+            pd += slice_body<M, false, false, U>(sv, ds, row, lane, live, rec, nE, d, q);
     }
     return pd;
 }
@@ -1012,13 +1040,21 @@ __global__ __launch_bounds__(NT) void k_spmv_plain_upper(int nd, SliceView sv, c
         return;
     const int2 rs = sv.rowSlice[row];  // (slice, lane)
     const int4 ds = sv.slices[rs.x];
-    const int wu = ds.z & 0xffff, wl = (ds.z >> 16) & 0x7fff, l = rs.y;
+    const int wu = ds.z & 0xffff, wl = (ds.z >> 16) & 0x3fff, l = rs.y;
+    const bool compact = ((ds.z >> 30) & 1) != 0;
+    const unsigned long long below = (1ull << l) - 1ull;
     const size_t n = static_cast<size_t>(sv.nE);
     double acc[3] = {0, 0, 0};
+    int lb = ds.y;  // first entry of lower position k
+    for (int k = 0; k < wl; ++k)
+        lb += compact ? __popcll(sv.mask[ds.w + wu + k]) : kSliceRows;
     for (int k = wl - 1; k >= 0; --k)  // the lower ELL runs from the diagonal down
-        if (lane_bit(sv.mask[ds.w + wu + k], l))
+    {
+        const unsigned long long m = sv.mask[ds.w + wu + k];
+        lb -= compact ? __popcll(m) : kSliceRows;
+        if (lane_bit(m, l))
         {
-            const int idx = ds.y + kSliceRows * k + l;
+            const int idx = lb + (compact ? __popcll(m & below) : l);
             M a[9], t[9];
             load_element(rec, n, sv.mirE[idx], a);
             for (int i = 0; i < 3; ++i)
@@ -1028,16 +1064,22 @@ __global__ __launch_bounds__(NT) void k_spmv_plain_upper(int nd, SliceView sv, c
             for (int i = 0; i < 3; ++i)
                 acc[i] += brow(t, i, xin[3 * c], xin[3 * c + 1], xin[3 * c + 2]);
         }
+    }
+    int ub = ds.x;  // first entry of upper position k
     for (int k = 0; k < wu; ++k)
-        if (lane_bit(sv.mask[ds.w + k], l))
+    {
+        const unsigned long long m = sv.mask[ds.w + k];
+        if (lane_bit(m, l))
         {
-            const int e = ds.x + kSliceRows * k + l;
+            const int e = ub + (compact ? __popcll(m & below) : l);
             M a[9];
             load_element(rec, n, e, a);
             const int c = sv.ecol[e];
             for (int i = 0; i < 3; ++i)
                 acc[i] += brow(a, i, xin[3 * c], xin[3 * c + 1], xin[3 * c + 2]);
         }
+        ub += compact ? __popcll(m) : kSliceRows;
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         y[3 * row + i] = acc[i];

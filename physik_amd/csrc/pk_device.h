@@ -140,27 +140,34 @@ template <> struct BsrStore<float>
 // Upper-only storage, both precisions, sliced for one lane per row. The
 // dynamic rows of each body are cut into slices of kSliceRows consecutive
 // rows (one wavefront). A slice keeps its rows' upper blocks as an ELL block
-// of width wu: the k-th upper block of the row at lane l (the diagonal, then
-// the columns above it ascending) is element ubase + kSliceRows * k + l. An
-// element's 9 values (row-major) are kept in planes of nE elements, so that
-// the lanes of a wave read one plane of consecutive elements as one
-// contiguous run:
+// of width wu: position k holds the k-th upper block of every row that has
+// one (the diagonal, then the columns above it ascending). Full slices keep
+// the block of the row at lane l at element ubase + kSliceRows * k + l; a
+// slice whose positions are far from full is stored compactly instead: the
+// entries of position k follow those of positions 0 .. k - 1 from element
+// ubase on, in lane order, so lane l's block is element ubase + (blocks at
+// positions before k) + (lanes below l with a block at k), and lanes that
+// skip a position leave no gap in the run their wave reads. An element's 9
+// values (row-major) are kept in planes of nE elements, so that the lanes
+// of a wave read one plane of consecutive elements as one contiguous run:
 //   FP32: values 0-3 (16 bytes) | values 4-7 (16 bytes) | value 8
 //   FP64: values 0-1 | 2-3 | 4-5 | 6-7 (16 bytes each) | value 8
 // and its column is entry e of a separate int32 plane (the pattern's, set
 // once). A row's lower blocks (columns c < r, descending) are a second ELL
-// block of width wl, each entry the element of its upper mirror (c, r) (one
-// int32 plane) and c (another).
-// Slice descriptor: (ubase, lbase, wu | wl << 16 | aligned << 31, mbase);
-// bit l of mask[mbase + k] says lane l has an upper position k, bit l of
-// mask[mbase + wu + k] a lower position k. In an aligned slice position k
-// means one stencil offset for every row -- the upper block at column
-// r + off[mbase + k], the lower block at column r - off[mbase + wu + k]
-// (offsets ascending; a row without that neighbour leaves a hole) -- so
-// columns need no load, and on a regular grid the transposed mirror of a
-// lower position sits at the same position of the mirror row's slice. Slices
-// whose offsets do not line up keep each row's blocks packed from position 0
-// and read their columns from the planes. Padding elements are never read.
+// block of width wl, laid out the same way from entry lbase on, each entry
+// the element of its upper mirror (c, r) (one int32 plane) and c (another).
+// Slice descriptor: (ubase, lbase, wu | wl << 16 | compact << 30 |
+// aligned << 31, mbase); bit l of mask[mbase + k] says lane l has an upper
+// position k, bit l of mask[mbase + wu + k] a lower position k. In an
+// aligned slice position k means one stencil offset for every row -- the
+// upper block at column r + off[mbase + k], the lower block at column
+// r - off[mbase + wu + k] (offsets ascending; a row without that neighbour
+// has its bit clear) -- so columns need no load, and on a regular grid the
+// transposed mirror of a lower position sits at the same position of the
+// mirror row's slice. Slices whose offsets do not line up keep each row's
+// blocks packed from position 0 and read their columns from the planes.
+// Entries of no block (in full slices, and to start every slice on a
+// multiple of 8 entries) are never read.
 constexpr int kSliceRows = 64;
 struct SliceView
 {

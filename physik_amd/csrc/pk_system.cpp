@@ -146,6 +146,7 @@ struct pk_system
     pk::DevBuf<int> slotEll, tileSlice, ecol, soff, mirE, mirC;
     std::vector<int> groupSlice;  // first slice of each body
     long long nE = 0;
+    int nSlices = 0, nCompact = 0;  // slices of upper-only storage, compact ones
     pk::DevBuf<int> ucol;
     pk::DevBuf<char> vals, z, d0, d1, d2, d3, q;  // element type set by the precision
     pk::DevBuf<pk::Partial> pb0, pb1;
@@ -1045,6 +1046,11 @@ static int build_slices(pk_system* s, const std::vector<int>& slotRow)
     // a slice aligns when its distinct offsets need at most this many more
     // positions than its longest row
     constexpr int kAlignSlack = 2;
+    // a slice is stored compactly when its blocks fill less than this share
+    // of its 64 (wu + wl) positions; every slice starts on a multiple of
+    // kSliceAlign entries
+    constexpr double kCompactBelow = 0.97;
+    constexpr long long kSliceAlign = 8;
     std::vector<int4> desc;
     std::vector<unsigned long long> masks;
     std::vector<int> offs;
@@ -1053,6 +1059,7 @@ static int build_slices(pk_system* s, const std::vector<int>& slotRow)
     // lower ELL position of each lower block, by its full index
     std::vector<int> lowPos(static_cast<size_t>(std::max<long long>(t.nnzf(), 1)), -1);
     s->groupSlice.assign(static_cast<size_t>(t.G), 0);
+    s->nCompact = 0;
     long long nE = 0, nL = 0;
     std::vector<int> up, lo;
     for (int g = 0; g < t.G; ++g)
@@ -1082,7 +1089,9 @@ static int build_slices(pk_system* s, const std::vector<int>& slotRow)
                                  static_cast<int>(lo.size()) <= ml + kAlignSlack;
             const int wu = aligned ? static_cast<int>(up.size()) : mu;
             const int wl = aligned ? static_cast<int>(lo.size()) : ml;
-            if (wu > 0xffff || wl > 0x7fff || nE + static_cast<long long>(W) * wu > INT_MAX ||
+            nE = (nE + kSliceAlign - 1) / kSliceAlign * kSliceAlign;
+            nL = (nL + kSliceAlign - 1) / kSliceAlign * kSliceAlign;
+            if (wu > 0xffff || wl > 0x3fff || nE + static_cast<long long>(W) * wu > INT_MAX ||
                 nL + static_cast<long long>(W) * wl > INT_MAX || masks.size() + wu + wl > static_cast<size_t>(INT_MAX))
             {
                 set_error("upper-only storage: the pattern exceeds the sliced layout's limits");
@@ -1099,6 +1108,11 @@ static int build_slices(pk_system* s, const std::vector<int>& slotRow)
                 std::copy(up.begin(), up.end(), offs.begin() + mbase);
                 std::copy(lo.begin(), lo.end(), offs.begin() + mbase + wu);
             }
+            // positions first, then entries: in a compact slice the entries
+            // of one position follow those of the positions before it, lane
+            // l's at its position's base plus the number of lanes below l
+            // that hold a block there; otherwise position k of lane l is
+            // entry base + 64 k + l
             for (int l = 0; l < n; ++l)
             {
                 const int r = r0 + l;
@@ -1113,7 +1127,7 @@ static int build_slices(pk_system* s, const std::vector<int>& slotRow)
                                                                up.begin())
                                             : k;
                     masks[mbase + pos] |= 1ull << l;
-                    slotEll[u] = static_cast<int>(nE + static_cast<long long>(W) * pos + l);
+                    slotEll[u] = pos;
                 }
                 // lower blocks at full indices f0 .. f0 + nlr - 1 (columns ascending);
                 // packed: the k-th below the diagonal at position k
@@ -1123,11 +1137,44 @@ static int build_slices(pk_system* s, const std::vector<int>& slotRow)
                                                                lo.begin())
                                             : (f0 + nlr - 1 - m);
                     masks[mbase + wu + pos] |= 1ull << l;
-                    lowPos[m] = static_cast<int>(nL + static_cast<long long>(W) * pos + l);
+                    lowPos[m] = pos;
                 }
             }
-            nE += static_cast<long long>(W) * wu;
-            nL += static_cast<long long>(W) * wl;
+            long long used = 0;
+            for (int k = 0; k < wu + wl; ++k)
+                used += __builtin_popcountll(masks[mbase + k]);
+            const bool compact = static_cast<double>(used) < kCompactBelow * W * (wu + wl);
+            if (compact)
+            {
+                desc.back().z |= 1 << 30;
+                ++s->nCompact;
+            }
+            std::vector<long long> ubase(static_cast<size_t>(wu) + 1, nE), lbase(static_cast<size_t>(wl) + 1, nL);
+            for (int k = 0; k < wu; ++k)
+                ubase[k + 1] = ubase[k] + (compact ? __builtin_popcountll(masks[mbase + k]) : W);
+            for (int k = 0; k < wl; ++k)
+                lbase[k + 1] = lbase[k] + (compact ? __builtin_popcountll(masks[mbase + wu + k]) : W);
+            for (int l = 0; l < n; ++l)
+            {
+                const int r = r0 + l;
+                const unsigned long long below = (1ull << l) - 1ull;
+                for (int u = t.ustart[r]; u < t.ustart[r + 1]; ++u)
+                {
+                    const int pos = slotEll[u];
+                    slotEll[u] = static_cast<int>(ubase[pos] +
+                                                  (compact ? __builtin_popcountll(masks[mbase + pos] & below) : l));
+                }
+                const int f0 = t.fstart[r];
+                const int nlr = (t.fstart[r + 1] - f0) - (t.ustart[r + 1] - t.ustart[r]);
+                for (int m = f0; m < f0 + nlr; ++m)
+                {
+                    const int pos = lowPos[m];
+                    lowPos[m] = static_cast<int>(
+                        lbase[pos] + (compact ? __builtin_popcountll(masks[mbase + wu + pos] & below) : l));
+                }
+            }
+            nE = ubase[wu];
+            nL = lbase[wl];
         }
     }
     // the lower block at full index m = uMirror[u] is the transposed mirror of
@@ -1162,6 +1209,7 @@ static int build_slices(pk_system* s, const std::vector<int>& slotRow)
         return PK_ERR_ARG;
     }
     s->nE = nE;
+    s->nSlices = static_cast<int>(desc.size());
     std::vector<int> ecol(static_cast<size_t>(std::max<long long>(nE, 1)), 0);
     for (long long u = 0; u < t.nnzu(); ++u)
         ecol[slotEll[u]] = t.ucol[u];
@@ -1639,6 +1687,17 @@ int pk_system_get_spmv_stage_capacity(const pk_system* s, int* rows)
     if (!s || !rows)
         return PK_ERR_ARG;
     *rows = s->stageCap;
+    return PK_OK;
+}
+
+int pk_system_get_slice_layout(const pk_system* s, int* slices, int* compact_slices, long long* elements)
+{
+    if (!s)
+        return PK_ERR_ARG;
+    const bool up = is_upper(s);
+    if (slices) *slices = up ? s->nSlices : 0;
+    if (compact_slices) *compact_slices = up ? s->nCompact : 0;
+    if (elements) *elements = up ? s->nE : 0;
     return PK_OK;
 }
 

@@ -84,6 +84,7 @@ PK_SIGNATURES = {
     "pk_system_set_spmv_staging": (C.c_int, [_vp, C.c_int]),
     "pk_system_get_spmv_staging": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "pk_system_get_spmv_stage_capacity": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "pk_system_get_slice_layout": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     "pk_system_set_state": (C.c_int, [_vp, _vp, _vp]),
     "pk_system_get_state": (C.c_int, [_vp, _vp, _vp]),
     "pk_system_set_state_device": (C.c_int, [_vp, _vp, _vp]),

@@ -118,6 +118,14 @@ class System:
         N.check(self.lib.pk_system_get_spmv_stage_capacity(self.h, C.byref(r)), "pk_system_get_spmv_stage_capacity")
         return r.value
 
+    def slice_layout(self):
+        """(slices, compact slices, elements) of sliced upper-only storage;
+        zeros under full storage."""
+        n, c, e = C.c_int(), C.c_int(), C.c_longlong()
+        N.check(self.lib.pk_system_get_slice_layout(self.h, C.byref(n), C.byref(c), C.byref(e)),
+                "pk_system_get_slice_layout")
+        return n.value, c.value, e.value
+
     def get_tune(self):
         v = [C.c_int() for _ in range(4)]
         N.check(self.lib.pk_system_get_tune(self.h, *[C.byref(x) for x in v]), "pk_system_get_tune")

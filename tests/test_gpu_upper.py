@@ -24,10 +24,10 @@ SHAPES = [(64, 1), (64, 2), (64, 4), (128, 2), (256, 1), (256, 2), (256, 4), (51
 DEFAULT = (1, 256, 1024, 2)
 
 
-def twisted(n=(8, 5, 5), seed=12):
+def twisted(n=(8, 5, 5), seed=12, split="kuhn"):
     nx, ny, nz = n
     h = 0.1
-    rest, tets = mg.box_mesh(nx, ny, nz, h, "kuhn")
+    rest, tets = mg.box_mesh(nx, ny, nz, h, split)
     rest = mg.jitter(rest, h, 0.1, mg.Pcg32(seed))
     fixed = np.zeros(len(rest), np.uint8)
     fixed[mg.face_nodes(nx, ny, nz, 0)] = 1
@@ -70,6 +70,31 @@ def test_upper_matrix_and_spmv_equal_full_storage(precisions):
         assert np.array_equal(u, v)
     x = np.random.default_rng(5).normal(size=(a.nd, 3))
     assert np.array_equal(a.spmv(x), b.spmv(x))
+
+
+@pytest.mark.parametrize("precisions", PAIRS)
+@pytest.mark.parametrize("n,split", [((63, 4, 4), "kuhn"), ((6, 5, 4), "five")])
+def test_compact_and_padded_slices_equal_full_storage(n, split, precisions):
+    """A slice whose ELL positions are less than 97 % filled keeps each
+    position's blocks back to back; a fuller one keeps an entry per lane and
+    position. The long Kuhn body has slices of both forms (rows along the
+    63-node lines fill their positions, the turns between lines do not), the
+    5-tet body only compact ones (its stencil alternates between rows)."""
+    w = twisted(n=n, seed=21, split=split)
+    print(f"case slices {n} {split} {precisions}", flush=True)
+    a, b = pair(w, precisions=precisions)
+    slices, compact, elements = b.slice_layout()
+    assert a.slice_layout() == (0, 0, 0)
+    assert 0 < compact <= slices and elements >= b.nnzb_upper
+    if split == "kuhn":
+        assert compact < slices
+    a.assemble(w.dt)
+    b.assemble(w.dt)
+    for u, v in zip(a.matrix(), b.matrix()):
+        assert np.array_equal(u, v)
+    x = np.random.default_rng(7).normal(size=(a.nd, 3))
+    assert np.array_equal(a.spmv(x), b.spmv(x))
+    same_steps(a, b, w, 3)
 
 
 @pytest.mark.parametrize("precisions", PAIRS)
