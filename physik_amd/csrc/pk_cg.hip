@@ -637,13 +637,13 @@ __device__ __attribute__((always_inline)) inline double slice_body(const SliceVi
     // and a lower block, its upper element, the mirror element of its lower
     // block and (packed slices) their columns
     auto fetch = [&](int i, bool& h_u, bool& h_l, int& e_u, int& e_m, int& c_u, int& c_l) {
-        const unsigned long long mu = i < wu ? mask[i] : 0ull;
-        const unsigned long long ml = (i >= 1 && i <= wl) ? mask[wu + i - 1] : 0ull;
-        h_u = lane_bit(mu, lane);
-        h_l = lane_bit(ml, lane);
         int li;
         if constexpr (COMPACT)
         {
+            const unsigned long long mu = i < wu ? mask[i] : 0ull;
+            const unsigned long long ml = (i >= 1 && i <= wl) ? mask[wu + i - 1] : 0ull;
+            h_u = lane_bit(mu, lane);
+            h_l = lane_bit(ml, lane);
             e_u = ub + lanes_below(mu);
             li = lb + lanes_below(ml);
             ub += __popcll(mu);
@@ -651,6 +651,19 @@ __device__ __attribute__((always_inline)) inline double slice_body(const SliceVi
         }
         else
         {
+            // the two ways of testing a lane's bit give the same flags but
+            // schedule differently: measured, the short-circuit form is
+            // faster with FP64 values and the select form with FP32
+            if constexpr (sizeof(M) == 8)
+            {
+                h_u = i < wu && lane_bit(mask[i], lane);
+                h_l = i >= 1 && i <= wl && lane_bit(mask[wu + i - 1], lane);
+            }
+            else
+            {
+                h_u = lane_bit(i < wu ? mask[i] : 0ull, lane);
+                h_l = lane_bit((i >= 1 && i <= wl) ? mask[wu + i - 1] : 0ull, lane);
+            }
             e_u = ds.x + kSliceRows * i + lane;
             li = ds.y + kSliceRows * (i - 1) + lane;
         }
@@ -681,7 +694,7 @@ __device__ __attribute__((always_inline)) inline double slice_body(const SliceVi
         for (int u = 0; u < U; ++u)
         {
             if (hu[u])
-                load_element(rec, nE, eu[u], a[u]);
+                load_element(rec, nE, COMPACT ? eu[u] : ds.x + kSliceRows * (i0 + u) + lane, a[u]);
             if (hl[u])
                 load_element(rec, nE, em[u], b[u]);
         }
