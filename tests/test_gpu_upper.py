@@ -24,7 +24,9 @@ SHAPES = [(64, 1), (64, 2), (64, 4), (128, 2), (256, 1), (256, 2), (256, 4), (51
 DEFAULT = (1, 256, 1024, 2)
 
 
-def twisted(n=(8, 5, 5), seed=12, split="kuhn"):
+def twisted(n=(8, 5, 5), seed=12, split="kuhn", shuffle=False):
+    """A jittered box twisted about x; with shuffle its nodes are numbered in
+    a random order, so neighbouring rows share no stencil offsets."""
     nx, ny, nz = n
     h = 0.1
     rest, tets = mg.box_mesh(nx, ny, nz, h, split)
@@ -32,6 +34,11 @@ def twisted(n=(8, 5, 5), seed=12, split="kuhn"):
     fixed = np.zeros(len(rest), np.uint8)
     fixed[mg.face_nodes(nx, ny, nz, 0)] = 1
     x0 = mg.twist_about_x(rest, nx, ny, nz, h, 0.8)
+    if shuffle:
+        order = np.random.default_rng(seed).permutation(len(rest))  # new node j is old node order[j]
+        new_of = np.empty_like(order)
+        new_of[order] = np.arange(len(order))
+        rest, fixed, x0, tets = rest[order], fixed[order], x0[order], new_of[tets]
     return W._single("upper", rest, tets, 1e5, 0.4, 1000.0, fixed, x0, np.zeros_like(rest), 1.0 / 60.0, 6, 1e-7, 500)
 
 
@@ -73,20 +80,23 @@ def test_upper_matrix_and_spmv_equal_full_storage(precisions):
 
 
 @pytest.mark.parametrize("precisions", PAIRS)
-@pytest.mark.parametrize("n,split", [((63, 4, 4), "kuhn"), ((6, 5, 4), "five")])
-def test_compact_and_padded_slices_equal_full_storage(n, split, precisions):
+@pytest.mark.parametrize("n,split,shuffle", [((63, 4, 4), "kuhn", False), ((6, 5, 4), "five", False),
+                                             ((10, 6, 6), "kuhn", True)])
+def test_compact_and_padded_slices_equal_full_storage(n, split, shuffle, precisions):
     """A slice whose ELL positions are less than 97 % filled keeps each
     position's blocks back to back; a fuller one keeps an entry per lane and
     position. The long Kuhn body has slices of both forms (rows along the
     63-node lines fill their positions, the turns between lines do not), the
-    5-tet body only compact ones (its stencil alternates between rows)."""
-    w = twisted(n=n, seed=21, split=split)
-    print(f"case slices {n} {split} {precisions}", flush=True)
+    5-tet body only compact ones (its stencil alternates between rows); the
+    shuffled body's slices keep each row's blocks packed (no common stencil
+    offsets), compactly."""
+    w = twisted(n=n, seed=21, split=split, shuffle=shuffle)
+    print(f"case slices {n} {split} shuffle {shuffle} {precisions}", flush=True)
     a, b = pair(w, precisions=precisions)
     slices, compact, elements = b.slice_layout()
     assert a.slice_layout() == (0, 0, 0)
     assert 0 < compact <= slices and elements >= b.nnzb_upper
-    if split == "kuhn":
+    if split == "kuhn" and not shuffle:
         assert compact < slices
     a.assemble(w.dt)
     b.assemble(w.dt)
